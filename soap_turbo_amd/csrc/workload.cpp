@@ -1,0 +1,224 @@
+// Synthetic atomic configurations and neighbour lists in the layout that
+// get_soap consumes (src/soap_turbo.f90:139-169): per site, the site itself
+// first (r = 0), then every other atom within the cutoff; per pair the
+// distance, polar angle and azimuth of the neighbour relative to the site.
+//
+// Positions come from random sequential addition in a cubic periodic box,
+// with slabs in z of independent densities (one slab for a uniform box).
+// The random stream is SplitMix64, so the same seed gives the same atoms on
+// every host.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct SplitMix64 {
+  uint64_t s;
+  explicit SplitMix64(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  // uniform in [0, 1) with 53 random bits
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+inline double wrap(double d, double L) { return d - L * std::nearbyint(d / L); }
+
+struct CellGrid {
+  int nc;
+  double cs, L;
+  std::vector<int> head, next;
+  CellGrid(double L_, double min_cell, int n_atoms) : L(L_) {
+    nc = (int)std::floor(L / min_cell);
+    if (nc < 1) nc = 1;
+    if (nc > 256) nc = 256;
+    cs = L / nc;
+    head.assign((size_t)nc * nc * nc, -1);
+    next.assign(n_atoms, -1);
+  }
+  int coord(double x) const {
+    int c = (int)std::floor(x / cs);
+    c %= nc;
+    if (c < 0) c += nc;
+    return c;
+  }
+  size_t cell(int cx, int cy, int cz) const { return ((size_t)cz * nc + cy) * nc + cx; }
+  void insert(int a, const double* p) {
+    size_t c = cell(coord(p[0]), coord(p[1]), coord(p[2]));
+    next[a] = head[c];
+    head[c] = (int)a;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// Random sequential addition. n_slabs slabs [zlo, zhi) each receive count[s]
+// atoms; a trial point closer than dmin (minimum image) to an accepted atom is
+// rejected. Returns the number of atoms placed (< total if max_trials ran out).
+int64_t stw_place_atoms(uint64_t seed, double L, double dmin, int n_slabs, const double* zlo,
+                        const double* zhi, const int64_t* count, int64_t max_trials_per_atom,
+                        double* pos) {
+  int64_t total = 0;
+  for (int s = 0; s < n_slabs; ++s) total += count[s];
+  SplitMix64 rng(seed);
+  CellGrid grid(L, dmin > 0 ? dmin : L, (int)total);
+  const double d2min = dmin * dmin;
+  int64_t placed = 0;
+  for (int s = 0; s < n_slabs; ++s) {
+    for (int64_t a = 0; a < count[s]; ++a) {
+      bool ok = false;
+      double p[3];
+      for (int64_t t = 0; t < max_trials_per_atom && !ok; ++t) {
+        p[0] = rng.uniform() * L;
+        p[1] = rng.uniform() * L;
+        p[2] = zlo[s] + rng.uniform() * (zhi[s] - zlo[s]);
+        ok = true;
+        if (dmin > 0) {
+          int cx = grid.coord(p[0]), cy = grid.coord(p[1]), cz = grid.coord(p[2]);
+          if (grid.nc < 3) {
+            // too few cells for a 27-cell stencil: test every placed atom
+            for (int64_t b = 0; b < placed && ok; ++b) {
+              double rx = wrap(pos[3 * b] - p[0], L), ry = wrap(pos[3 * b + 1] - p[1], L),
+                     rz = wrap(pos[3 * b + 2] - p[2], L);
+              if (rx * rx + ry * ry + rz * rz < d2min) ok = false;
+            }
+          } else
+          for (int dz = -1; dz <= 1 && ok; ++dz)
+            for (int dy = -1; dy <= 1 && ok; ++dy)
+              for (int dx = -1; dx <= 1 && ok; ++dx) {
+                int ex = ((cx + dx) % grid.nc + grid.nc) % grid.nc;
+                int ey = ((cy + dy) % grid.nc + grid.nc) % grid.nc;
+                int ez = ((cz + dz) % grid.nc + grid.nc) % grid.nc;
+                for (int b = grid.head[grid.cell(ex, ey, ez)]; b >= 0; b = grid.next[b]) {
+                  double rx = wrap(pos[3 * b] - p[0], L), ry = wrap(pos[3 * b + 1] - p[1], L),
+                         rz = wrap(pos[3 * b + 2] - p[2], L);
+                  if (rx * rx + ry * ry + rz * rz < d2min) { ok = false; break; }
+                }
+              }
+        }
+      }
+      if (!ok) return placed;
+      std::memcpy(pos + 3 * placed, p, sizeof p);
+      grid.insert((int)placed, p);
+      ++placed;
+    }
+  }
+  return placed;
+}
+
+// Species drawn independently per atom with cumulative probabilities cum[0..ns-1].
+void stw_draw_species(uint64_t seed, int64_t n_atoms, int ns, const double* cum, int32_t* species) {
+  SplitMix64 rng(seed);
+  for (int64_t a = 0; a < n_atoms; ++a) {
+    double u = rng.uniform();
+    int s = 0;
+    while (s < ns - 1 && u >= cum[s]) ++s;
+    species[a] = s;
+  }
+}
+
+// Standard-normal deviates (Box-Muller on the same stream).
+void stw_normal(uint64_t seed, int64_t n, double* out) {
+  SplitMix64 rng(seed);
+  const double two_pi = 6.283185307179586;
+  for (int64_t i = 0; i < n; i += 2) {
+    double u1 = rng.uniform(), u2 = rng.uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    double rad = std::sqrt(-2.0 * std::log(u1));
+    out[i] = rad * std::cos(two_pi * u2);
+    if (i + 1 < n) out[i + 1] = rad * std::sin(two_pi * u2);
+  }
+}
+
+// Neighbour list of the sites (site_atom[0..n_sites-1] are atom indices):
+// for each site its own atom first, then every other atom j with
+// |r_j - r_i| < rcut under the minimum image, in increasing j. Requires
+// L > 2 rcut. Two passes: with nbr == nullptr only n_neigh is filled and the
+// total pair count returned; then nbr (pair -> atom) and vec (3 per pair,
+// r_j - r_i) are filled.
+int64_t stw_neighbours(int64_t n_atoms, const double* pos, double L, double rcut, int64_t n_sites,
+                       const int64_t* site_atom, int32_t* n_neigh, int32_t* nbr, double* vec) {
+  CellGrid grid(L, rcut, (int)n_atoms);
+  for (int64_t a = n_atoms - 1; a >= 0; --a) grid.insert((int)a, pos + 3 * a);
+  const double rc2 = rcut * rcut;
+  const bool stencil = grid.nc >= 3;
+  int64_t k = 0;
+  std::vector<int> found;
+  for (int64_t s = 0; s < n_sites; ++s) {
+    const int64_t i = site_atom[s];
+    const double* pi = pos + 3 * i;
+    found.clear();
+    if (stencil) {
+      int cx = grid.coord(pi[0]), cy = grid.coord(pi[1]), cz = grid.coord(pi[2]);
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            int ex = ((cx + dx) % grid.nc + grid.nc) % grid.nc;
+            int ey = ((cy + dy) % grid.nc + grid.nc) % grid.nc;
+            int ez = ((cz + dz) % grid.nc + grid.nc) % grid.nc;
+            for (int b = grid.head[grid.cell(ex, ey, ez)]; b >= 0; b = grid.next[b]) found.push_back(b);
+          }
+    } else {
+      for (int b = 0; b < (int)n_atoms; ++b) found.push_back(b);
+    }
+    // increasing j, no duplicates (small grids can revisit a cell)
+    std::vector<int> uniq;
+    uniq.reserve(found.size());
+    for (int b : found) uniq.push_back(b);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    int32_t cnt = 1;
+    if (nbr) {
+      nbr[k] = (int32_t)i;
+      vec[3 * k] = vec[3 * k + 1] = vec[3 * k + 2] = 0.0;
+    }
+    ++k;
+    for (int b : uniq) {
+      if (b == i) continue;
+      double rx = wrap(pos[3 * b] - pi[0], L), ry = wrap(pos[3 * b + 1] - pi[1], L),
+             rz = wrap(pos[3 * b + 2] - pi[2], L);
+      double r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 >= rc2) continue;
+      if (nbr) {
+        nbr[k] = b;
+        vec[3 * k] = rx;
+        vec[3 * k + 1] = ry;
+        vec[3 * k + 2] = rz;
+      }
+      ++k;
+      ++cnt;
+    }
+    n_neigh[s] = cnt;
+  }
+  return k;
+}
+
+// Spherical coordinates of the pair vectors: r, theta = acos(z/r),
+// phi = atan2(y, x); the site's own pair (r = 0) gets zeros.
+void stw_spherical(int64_t n_pairs, const double* vec, double* rjs, double* thetas, double* phis) {
+  for (int64_t k = 0; k < n_pairs; ++k) {
+    double x = vec[3 * k], y = vec[3 * k + 1], z = vec[3 * k + 2];
+    double r = std::sqrt(x * x + y * y + z * z);
+    rjs[k] = r;
+    if (r > 0) {
+      double c = z / r;
+      if (c > 1) c = 1;
+      if (c < -1) c = -1;
+      thetas[k] = std::acos(c);
+      phis[k] = std::atan2(y, x);
+    } else {
+      thetas[k] = 0;
+      phis[k] = 0;
+    }
+  }
+}
+
+}  // extern "C"
