@@ -1,0 +1,155 @@
+/* C-ABI of the MI355X (gfx950) soap_turbo get_soap library, libsoapturbo_hip.so.
+ *
+ * Plain pointers and sizes; arrays are the reference's Fortran column-major
+ * arrays (first index fastest). Every entry point returns 0 on success or one
+ * of the ST_ERR_* codes; st_error() then gives the message, which for the
+ * conditions the reference stops on is the reference's own message.
+ *
+ * Entry points and the reference interfaces they replace:
+ *   st_get_soap             get_soap, src/soap_turbo.f90:139-169 (argument list, output layout and
+ *                           caller contract: soap and soap_cart_der zero on entry)
+ *   st_soap_contract        soap_contract, src/soap_turbo.f90:1723-1730
+ *   st_get_compress_indices get_compress_indices, src/soap_turbo_compress.f90:44-54 ("trivial" mode)
+ *   st_set_basis            the save'd W/S cache of get_soap, src/soap_turbo.f90:294-388
+ *                           (lets a caller inject the W the reference would have built)
+ *   st_get_basis            get_orthonormalization_matrix_poly3[gauss], src/soap_turbo_radial.f90:1023, :1113
+ *   st_upload_neighbours, st_compute, st_download, st_upload_dedq, st_pairs_full_jacobian,
+ *   st_adjoint_f32, st_download_pairs, st_download_forces
+ *                           the same get_soap / soap_contract split into upload, device-resident
+ *                           compute and download, for callers that keep their data on the GPU
+ *                           (get_soap's persistent work buffers, src/soap_turbo.f90:43-61, are the
+ *                           reference's counterpart of the resident state)
+ *   st_adjoint_f32          no reference counterpart: dE/dq -> pair gradients, forces and virial in FP32
+ *                           without forming soap_cart_der (the FP32 form of soap_contract, :1703-1721)
+ *
+ * Threading: one context per host thread; calls on a context are ordered on
+ * its HIP stream. Like the reference, a context is not reentrant.
+ */
+#ifndef SOAP_TURBO_HIP_H
+#define SOAP_TURBO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct st_ctx st_ctx;
+
+enum {
+  ST_OK = 0,
+  ST_ERR_ARG = 1,          /* invalid argument or size */
+  ST_ERR_HIP = 2,          /* HIP runtime failure */
+  ST_ERR_UNSUPPORTED = 3,  /* a case this build does not implement (l_max > 12, poly3operator, ...) */
+  ST_ERR_STATE = 4,        /* call out of order, e.g. soap_contract before get_soap(contract=.true.) */
+  ST_ERR_STOP = 5          /* a condition the reference stops on (alpha_max limits, ...) */
+};
+
+enum { ST_BASIS_POLY3 = 0, ST_BASIS_POLY3GAUSS = 1 };
+enum { ST_SCALING_POLYNOMIAL = 0, ST_SCALING_OTHER = 1 };
+
+/* Descriptor hyper-parameters (get_soap arguments; one entry per species for the arrays). */
+typedef struct {
+  int32_t n_species;
+  int32_t l_max;
+  int32_t radial_enhancement;
+  int32_t basis;               /* ST_BASIS_* */
+  int32_t scaling_mode;        /* ST_SCALING_* */
+  int32_t legacy_filter_seed;  /* soap_turbo_radial's legacy_filter_seed, src/soap_turbo_radial.f90:67 */
+  const int32_t* alpha_max;
+  const double* rcut_hard;
+  const double* rcut_soft;
+  const double* nf;
+  const double* global_scaling;
+  const double* atom_sigma_r;
+  const double* atom_sigma_r_scaling;
+  const double* atom_sigma_t;
+  const double* atom_sigma_t_scaling;
+  const double* amplitude_scaling;
+  const double* central_weight;
+} st_params;
+
+/* The neighbour list (get_soap arguments n_sites .. phis). Host pointers. */
+typedef struct {
+  int32_t n_sites;
+  int32_t n_pairs;
+  int32_t n_species;                   /* columns of mask */
+  int32_t max_mult;                    /* leading dimension of species */
+  const int32_t* n_neigh;              /* n_neigh(n_sites), the site itself first */
+  const int32_t* species;              /* species(max_mult, n_sites), 1-based */
+  const int32_t* species_multiplicity; /* species_multiplicity(n_sites) */
+  const int32_t* mask;                 /* mask(n_pairs, n_species), nonzero = .true. */
+  const double* rjs;
+  const double* thetas;
+  const double* phis;
+} st_neighbours;
+
+/* Compression (compress_P_nonzero, compress_P_i, compress_P_j, compress_P_el); P_nonzero = 0: none. */
+typedef struct {
+  int32_t P_nonzero;
+  const int32_t* P_i;
+  const int32_t* P_j;
+  const double* P_el;
+} st_compression;
+
+int st_create(st_ctx** ctx, int device);
+void st_destroy(st_ctx* ctx);
+const char* st_error(const st_ctx* ctx);
+
+/* n_soap that get_soap will produce for these parameters (maxval(P_i) when compressed). */
+int st_n_soap(const st_params* p, const st_compression* c);
+
+/* W and S (n_max x n_max, block-diagonal, column-major) the library uses for p. */
+int st_get_basis(st_ctx* ctx, const st_params* p, double* W, double* S);
+/* Use these W and S instead of building them, while n_max matches; n_max = 0 clears. */
+int st_set_basis(st_ctx* ctx, int32_t n_max, const double* W, const double* S);
+
+/* get_soap: soap(n_soap, n_sites) and, with do_derivatives and not contract,
+ * soap_cart_der(3, n_soap, n_pairs). soap_cart_der may be NULL otherwise. */
+int st_get_soap(st_ctx* ctx, const st_params* p, const st_neighbours* nb, const st_compression* c,
+                int32_t do_derivatives, int32_t contract, int32_t n_soap, double* soap, double* soap_cart_der);
+
+/* soap_contract: pair(3, n_pairs) = sgn * sum_s G(i,s) soap_cart_der(:,s,k), G(n_sites, n_soap). */
+int st_soap_contract(st_ctx* ctx, const double* soap, const double* G, double sgn, double* pair);
+
+/* get_compress_indices (mode "trivial"/"Trivial"): set_indices = 0 only fills dim and P_nonzero. */
+int st_get_compress_indices(const char* mode, int32_t n_species, const int32_t* alpha_max, int32_t l_max,
+                            int32_t* dim, int32_t* P_nonzero, int32_t* P_i, int32_t* P_j, double* P_el,
+                            int32_t set_indices);
+
+/* ---- device-resident use (data uploaded once, compute repeated, results read back on demand) ---- */
+int st_upload_neighbours(st_ctx* ctx, const st_neighbours* nb);
+/* pair_atom(n_pairs): 0-based atom index of each pair's neighbour (the site's own atom for its first
+ * pair), n_atoms atoms; needed only for forces. vec(3, n_pairs): r_j - r_i, needed only for the virial
+ * (may be NULL: it is rebuilt from rjs, thetas, phis). */
+int st_upload_atoms(st_ctx* ctx, int32_t n_atoms, const int32_t* pair_atom, const double* vec);
+/* dE/dq as G(n_sites, n_soap) */
+int st_upload_dedq(st_ctx* ctx, int32_t n_soap, const double* G);
+/* FP64 get_soap on the resident neighbour list. */
+int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t do_derivatives, int32_t contract);
+/* FP64 pair gradients from the resident soap_cart_der and dE/dq (what a full-Jacobian caller does). */
+int st_pairs_full_jacobian(st_ctx* ctx);
+/* FP64 soap_contract on the resident state and dE/dq (normalised soap from st_compute). */
+int st_contract_resident(st_ctx* ctx, double sgn);
+/* FP32 adjoint: descriptors, pair gradients, forces and virial from the resident data and dE/dq. */
+int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c);
+int st_download(st_ctx* ctx, double* soap, double* soap_cart_der);
+/* pair(3, n_pairs) of the last st_pairs_full_jacobian / st_contract_resident (FP64) or st_adjoint_f32. */
+int st_download_pairs(st_ctx* ctx, int32_t which, double* pair);   /* which: 0 FP64, 1 FP32 */
+/* forces(3, n_atoms) = -sum over pairs with that neighbour atom; virial(3,3) = -sum_k vec_k (x) pair_k. */
+int st_download_forces(st_ctx* ctx, int32_t which, double* forces, double* virial);
+/* soap(n_soap, n_sites) of the last st_adjoint_f32, widened to FP64. */
+int st_download_soap_f32(st_ctx* ctx, double* soap);
+
+int st_synchronize(st_ctx* ctx);
+/* Stream-ordered timing: record event slot (0..15) on the context's stream; elapsed ms between two. */
+int st_event_record(st_ctx* ctx, int32_t slot);
+int st_event_elapsed(st_ctx* ctx, int32_t a, int32_t b, float* ms);
+/* Per-kernel timing with events around each launch (off by default). */
+int st_kernel_timing(st_ctx* ctx, int32_t on);
+int st_kernel_time(st_ctx* ctx, const char* name, double* total_ms, int32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

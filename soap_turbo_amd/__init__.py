@@ -1,0 +1,266 @@
+"""MI355X-native get_soap: the soap_turbo descriptor hot path on gfx950 HIP kernels.
+
+Python mirror of the reference's interface (module ``soap_turbo_desc``,
+src/soap_turbo.f90:139-169 and :1723-1730; ``get_compress_indices``,
+src/soap_turbo_compress.f90:44-54), over the C-ABI of ``libsoapturbo_hip.so``:
+
+* ``get_soap(...)`` takes the reference's arguments, with arrays in the
+  reference's Fortran shapes (``mask`` is ``(n_atom_pairs, n_species)``,
+  ``soap`` is ``(n_soap, n_sites)``, ``soap_cart_der`` is
+  ``(3, n_soap, n_atom_pairs)``) and fills ``soap`` and ``soap_cart_der`` in
+  place. As for the reference, both must be zero on entry.
+* ``soap_contract(soap, G, sgn, pair)`` follows a ``get_soap(...,
+  contract=True)``; ``G`` is ``(n_sites, n_soap)``.
+* ``legacy_filter_seed`` mirrors soap_turbo_radial's flag
+  (src/soap_turbo_radial.f90:67), default True.
+
+Errors the reference stops on raise ``SoapTurboStop`` with its message; the
+rest raise ``SoapTurboError``. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Keep, StCompression, StNeighbours, StParams, lib, ptr
+
+legacy_filter_seed = True
+
+__all__ = ["get_soap", "soap_contract", "get_compress_indices", "Context", "SoapTurboError", "SoapTurboStop",
+           "legacy_filter_seed"]
+
+
+class SoapTurboError(RuntimeError):
+    pass
+
+
+class SoapTurboStop(SoapTurboError):
+    """A condition on which the reference writes its message and stops."""
+
+
+def _check(ctx_handle, rc):
+    if rc == _lib.ST_OK:
+        return
+    msg = lib().st_error(ctx_handle)
+    msg = msg.decode() if msg else ""
+    if rc == _lib.ST_ERR_STOP or rc == _lib.ST_ERR_STATE and msg.startswith("ERROR"):
+        raise SoapTurboStop(msg)
+    raise SoapTurboError(f"soap_turbo_hip error {rc}: {msg}")
+
+
+def make_params(keep: Keep, *, alpha_max, l_max, rcut_hard, rcut_soft, nf, global_scaling, atom_sigma_r,
+                atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, amplitude_scaling, radial_enhancement,
+                central_weight, basis, scaling_mode, legacy_seed=None):
+    basis = basis.strip()
+    codes = {"poly3": 0, "poly3gauss": 1}
+    if basis not in codes:
+        raise SoapTurboError(f'basis "{basis}" is not supported (poly3, poly3gauss)')
+    ns = len(alpha_max)
+    seed = legacy_filter_seed if legacy_seed is None else legacy_seed
+    return StParams(ns, int(l_max), int(radial_enhancement), codes[basis],
+                    0 if scaling_mode.strip() == "polynomial" else 1, 1 if seed else 0,
+                    keep(alpha_max, np.int32), keep(rcut_hard, np.float64), keep(rcut_soft, np.float64),
+                    keep(nf, np.float64), keep(global_scaling, np.float64), keep(atom_sigma_r, np.float64),
+                    keep(atom_sigma_r_scaling, np.float64), keep(atom_sigma_t, np.float64),
+                    keep(atom_sigma_t_scaling, np.float64), keep(amplitude_scaling, np.float64),
+                    keep(central_weight, np.float64))
+
+
+def make_compression(keep: Keep, compress_soap, P_nonzero=0, P_i=None, P_j=None, P_el=None):
+    if not compress_soap or not P_nonzero:
+        return StCompression(0, None, None, None)
+    return StCompression(int(P_nonzero), keep(np.asarray(P_i)[:P_nonzero], np.int32),
+                         keep(np.asarray(P_j)[:P_nonzero], np.int32), keep(np.asarray(P_el)[:P_nonzero], np.float64))
+
+
+class Context:
+    """One device context (stream, resident buffers, the W/S cache)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib().st_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise SoapTurboError(f"st_create failed ({rc}): no usable GPU {device}")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib().st_destroy(h)
+            self.h = None
+
+    def check(self, rc):
+        _check(self.h, rc)
+
+    # ---- basis
+    def set_basis(self, W, S):
+        W = np.asfortranarray(W, dtype=np.float64); S = np.asfortranarray(S, dtype=np.float64)
+        self.check(lib().st_set_basis(self.h, W.shape[0], ptr(W), ptr(S)))
+
+    def clear_basis(self):
+        self.check(lib().st_set_basis(self.h, 0, None, None))
+
+    def get_basis(self, params: StParams, n_max: int):
+        W = np.zeros((n_max, n_max), order="F"); S = np.zeros((n_max, n_max), order="F")
+        self.check(lib().st_get_basis(self.h, ctypes.byref(params), ptr(W), ptr(S)))
+        return W, S
+
+    # ---- resident use
+    def upload_neighbours(self, n_neigh, species, species_multiplicity, mask, rjs, thetas, phis):
+        keep = Keep()
+        mask = np.asfortranarray(mask)
+        species = np.asfortranarray(species, dtype=np.int32)
+        if species.ndim == 1:
+            species = species.reshape(1, -1, order="F")
+        nb = StNeighbours(len(n_neigh), len(rjs), mask.shape[1], species.shape[0], keep(n_neigh, np.int32),
+                          keep(species, np.int32), keep(species_multiplicity, np.int32), keep(mask, np.int32),
+                          keep(rjs, np.float64), keep(thetas, np.float64), keep(phis, np.float64))
+        self.check(lib().st_upload_neighbours(self.h, ctypes.byref(nb)))
+        self.n_sites, self.n_pairs = len(n_neigh), len(rjs)
+
+    def upload_atoms(self, n_atoms, pair_atom, vec=None):
+        keep = Keep()
+        v = keep(vec, np.float64, "C") if vec is not None else None
+        self.check(lib().st_upload_atoms(self.h, int(n_atoms), keep(pair_atom, np.int32), v))
+        self.n_atoms = int(n_atoms)
+
+    def upload_dedq(self, G):
+        """G(n_sites, n_soap), Fortran order (a C-ordered (n_soap, n_sites) array is the same memory)."""
+        G = np.asfortranarray(G, dtype=np.float64)
+        self.check(lib().st_upload_dedq(self.h, G.shape[1], ptr(G)))
+
+    def compute(self, params, compression, do_derivatives=True, contract=False):
+        self.check(lib().st_compute(self.h, ctypes.byref(params), ctypes.byref(compression), int(do_derivatives),
+                                    int(contract)))
+
+    def pairs_full_jacobian(self):
+        self.check(lib().st_pairs_full_jacobian(self.h))
+
+    def contract_resident(self, sgn=1.0):
+        self.check(lib().st_contract_resident(self.h, sgn))
+
+    def adjoint_f32(self, params, compression):
+        self.check(lib().st_adjoint_f32(self.h, ctypes.byref(params), ctypes.byref(compression)))
+
+    def download(self, n_soap, cart=False):
+        soap = np.zeros((n_soap, self.n_sites), order="F")
+        c = np.zeros((3, n_soap, self.n_pairs), order="F") if cart else None
+        self.check(lib().st_download(self.h, ptr(soap), ptr(c)))
+        return soap, c
+
+    def download_pairs(self, which=0):
+        pair = np.zeros((3, self.n_pairs), order="F")
+        self.check(lib().st_download_pairs(self.h, int(which), ptr(pair)))
+        return pair
+
+    def download_forces(self, which=0):
+        f = np.zeros((3, self.n_atoms), order="F")
+        v = np.zeros((3, 3), order="F")
+        self.check(lib().st_download_forces(self.h, int(which), ptr(f), ptr(v)))
+        return f, v
+
+    def download_soap_f32(self, n_soap):
+        soap = np.zeros((n_soap, self.n_sites), order="F")
+        self.check(lib().st_download_soap_f32(self.h, ptr(soap)))
+        return soap
+
+    def synchronize(self):
+        self.check(lib().st_synchronize(self.h))
+
+    def event_record(self, slot):
+        self.check(lib().st_event_record(self.h, slot))
+
+    def event_elapsed(self, a, b):
+        ms = ctypes.c_float()
+        self.check(lib().st_event_elapsed(self.h, a, b, ctypes.byref(ms)))
+        return ms.value
+
+    def kernel_timing(self, on=True):
+        self.check(lib().st_kernel_timing(self.h, int(on)))
+
+    def kernel_time(self, name):
+        t = ctypes.c_double(); n = ctypes.c_int32()
+        self.check(lib().st_kernel_time(self.h, name.encode(), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+
+_DEFAULT = None
+
+
+def default_context() -> Context:
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+def get_soap(n_sites, n_neigh, n_species, species, species_multiplicity, n_atom_pairs, mask, rjs, thetas, phis,
+             alpha_max, l_max, rcut_hard, rcut_soft, nf, global_scaling, atom_sigma_r, atom_sigma_r_scaling,
+             atom_sigma_t, atom_sigma_t_scaling, amplitude_scaling, radial_enhancement, central_weight, basis,
+             scaling_mode, do_timing, do_derivatives, compress_soap, compress_P_nonzero, compress_P_i, compress_P_j,
+             compress_P_el, soap, soap_cart_der, contract=False, ctx: Context | None = None):
+    """get_soap (src/soap_turbo.f90:139-169): fills soap(n_soap, n_sites) and, with derivatives and not
+    contract, soap_cart_der(3, n_soap, n_atom_pairs), both Fortran-ordered numpy arrays, in place."""
+    ctx = ctx or default_context()
+    keep = Keep()
+    prm = make_params(keep, alpha_max=alpha_max, l_max=l_max, rcut_hard=rcut_hard, rcut_soft=rcut_soft, nf=nf,
+                      global_scaling=global_scaling, atom_sigma_r=atom_sigma_r,
+                      atom_sigma_r_scaling=atom_sigma_r_scaling, atom_sigma_t=atom_sigma_t,
+                      atom_sigma_t_scaling=atom_sigma_t_scaling, amplitude_scaling=amplitude_scaling,
+                      radial_enhancement=radial_enhancement, central_weight=central_weight, basis=basis,
+                      scaling_mode=scaling_mode)
+    cmp_ = make_compression(keep, compress_soap, compress_P_nonzero, compress_P_i, compress_P_j, compress_P_el)
+    mask = np.asfortranarray(mask)
+    species = np.asfortranarray(species, dtype=np.int32)
+    if species.ndim == 1:
+        species = species.reshape(1, -1, order="F")
+    if mask.shape != (n_atom_pairs, n_species):
+        raise SoapTurboError("mask must be (n_atom_pairs, n_species)")
+    nb = StNeighbours(int(n_sites), int(n_atom_pairs), int(n_species), species.shape[0], keep(n_neigh, np.int32),
+                      keep(species, np.int32), keep(species_multiplicity, np.int32), keep(mask, np.int32),
+                      keep(rjs, np.float64), keep(thetas, np.float64), keep(phis, np.float64))
+    if not (isinstance(soap, np.ndarray) and soap.dtype == np.float64 and soap.flags.f_contiguous):
+        raise SoapTurboError("soap must be a Fortran-ordered float64 array (n_soap, n_sites)")
+    want_cart = do_derivatives and not contract
+    if want_cart and not (isinstance(soap_cart_der, np.ndarray) and soap_cart_der.dtype == np.float64
+                          and soap_cart_der.flags.f_contiguous and soap_cart_der.shape == (3, soap.shape[0],
+                                                                                            n_atom_pairs)):
+        raise SoapTurboError("soap_cart_der must be a Fortran-ordered float64 array (3, n_soap, n_atom_pairs)")
+    rc = lib().st_get_soap(ctx.h, ctypes.byref(prm), ctypes.byref(nb), ctypes.byref(cmp_), int(do_derivatives),
+                           int(bool(contract)), soap.shape[0], ptr(soap), ptr(soap_cart_der) if want_cart else None)
+    ctx.check(rc)
+    ctx.n_sites, ctx.n_pairs = int(n_sites), int(n_atom_pairs)
+
+
+def soap_contract(soap, G, sgn, pair, ctx: Context | None = None):
+    """soap_contract (src/soap_turbo.f90:1723-1730): pair(3, n_pairs) in place."""
+    ctx = ctx or default_context()
+    soap = np.asfortranarray(soap, dtype=np.float64)
+    G = np.asfortranarray(G, dtype=np.float64)
+    if not (isinstance(pair, np.ndarray) and pair.dtype == np.float64 and pair.flags.f_contiguous):
+        raise SoapTurboError("pair must be a Fortran-ordered float64 array (3, n_pairs)")
+    ctx.check(lib().st_soap_contract(ctx.h, ptr(soap), ptr(G), float(sgn), ptr(pair)))
+
+
+def get_compress_indices(compress_mode, alpha_max, l_max, what_to_do, P_i=None, P_j=None, P_el=None):
+    """get_compress_indices (src/soap_turbo_compress.f90:44-54). Returns (dim, P_nonzero); with
+    what_to_do == "set_indices" also fills P_i, P_j, P_el in place."""
+    if what_to_do not in ("get_dim", "set_indices"):
+        raise SoapTurboStop("ERROR: incorrect what_to_do in get_compress_indices() subroutine!")
+    am = np.asarray(alpha_max, dtype=np.int32)
+    dim = ctypes.c_int32(); nz = ctypes.c_int32()
+    setting = what_to_do == "set_indices"
+    if setting:
+        for a, dt in ((P_i, np.int32), (P_j, np.int32), (P_el, np.float64)):
+            if not (isinstance(a, np.ndarray) and a.dtype == dt and a.flags.c_contiguous):
+                raise SoapTurboError("P_i, P_j (int32) and P_el (float64) must be contiguous arrays")
+    rc = lib().st_get_compress_indices(compress_mode.encode(), len(am), ptr(am), int(l_max), ctypes.byref(dim),
+                                       ctypes.byref(nz), ptr(P_i) if setting else None,
+                                       ptr(P_j) if setting else None, ptr(P_el) if setting else None, int(setting))
+    if rc == _lib.ST_ERR_UNSUPPORTED:
+        raise SoapTurboError(f'compress_mode "{compress_mode}" is not supported by this build')
+    if rc != 0:
+        raise SoapTurboError(f"get_compress_indices failed ({rc})")
+    return dim.value, nz.value

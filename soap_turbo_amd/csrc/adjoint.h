@@ -1,0 +1,76 @@
+// FP32 adjoint path (descriptors + dE/dq -> pair gradients, forces, virial)
+// and the force/virial reduction shared with the FP64 path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/soap_turbo_hip.h"
+#include "st_internal.h"
+
+namespace st {
+
+struct AdjBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct AdjointState {
+  // tables of the radial and angular radial factors (rebuilt when the key changes)
+  std::string table_key;
+  AdjBuf rtab, atab, tabinfo;
+  int n_seg = 0;
+  // per-call work and outputs
+  AdjBuf cnk, soap, sdp, H, pair, forces, virial_part, virial, G32, order, atom_ptr, atom_pairs, vecbuf;
+  AdjBuf bins;
+  std::string atoms_key;
+  int n_pairs = 0, n_atoms = 0, n_sites = 0, n_soap = 0;
+  bool have_out = false, have_forces = false;
+};
+
+struct AdjointInputs {
+  const st_params* p;
+  const std::vector<double>* W;
+  const std::vector<double>* central;  // [species][n_max] FP64 central-atom term (poly3gauss)
+  int n_sites, n_pairs, n_max, k_max, l_max, n_soap, n_soap_unc;
+  const std::vector<int>* site_ptr_h;
+  const int* site_ptr;
+  const int32_t* species;
+  int max_mult;
+  const int32_t* mult;
+  const uint32_t* spmask;
+  const uint8_t* first;
+  const double *rjs, *thetas, *phis, *G;
+  const int4* comp;
+  const int* out_ptr;
+  const int* out_c;
+  const double* out_el;
+  const int32_t* pair_atom;
+  const double* vec;
+  int n_atoms;
+};
+
+using Timer = std::function<void(const char*, const std::function<void()>&)>;
+
+int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const Timer& timer, std::string& msg);
+int adjoint_download_pairs(AdjointState& s, hipStream_t st, int n_pairs, double* pair, std::string& msg);
+int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial,
+                            std::string& msg);
+int adjoint_download_soap(AdjointState& s, hipStream_t st, int n_sites, int n_soap, double* soap, std::string& msg);
+// forces(:, a) = -sum of pair(:, k) over pairs with neighbour atom a; virial = -sum_k vec_k (x) pair_k,
+// reduced in per-block partials (vpart holds SCATTER_BLOCKS * 9 doubles).
+constexpr int SCATTER_BLOCKS = 1024;
+void launch_scatter_f64(hipStream_t st, int n_pairs, int n_atoms, const double* pair, const int32_t* pair_atom,
+                        const double* vec, const double* rjs, const double* thetas, const double* phis,
+                        double* forces, double* vpart, double* virial);
+void launch_scatter_f32(hipStream_t st, int n_pairs, int n_atoms, const float* pair, const int32_t* pair_atom,
+                        const double* vec, const double* rjs, const double* thetas, const double* phis,
+                        double* forces, double* vpart, double* virial);
+void adjoint_invalidate(AdjointState& s);
+void adjoint_release(AdjointState& s);
+
+}  // namespace st

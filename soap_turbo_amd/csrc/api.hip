@@ -1,0 +1,918 @@
+// C-ABI of libsoapturbo_hip.so: device context, host-side set-up of the
+// per-species constants and index tables, and the orchestration of the
+// kernels on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/soap_turbo_hip.h"
+#include "adjoint.h"
+#include "kernels.h"
+#include "st_internal.h"
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct st_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // basis: injected or built (cache key = parameters that change W and S)
+  int inj_n_max = 0;
+  std::vector<double> injW, injS;
+  std::string basis_key;
+  std::vector<double> W, S;
+
+  // resident neighbour list
+  bool have_nb = false;
+  int n_sites = 0, n_pairs = 0, n_species_mask = 0, max_mult = 1;
+  std::vector<int> site_ptr_h;
+  DevBuf site_ptr, species, mult, mask, spmask, first, rjs, thetas, phis;
+  // atoms (forces / virial)
+  int n_atoms = 0;
+  bool have_atoms = false, have_vec = false;
+  DevBuf pair_atom, vec;
+  // dE/dq
+  int G_n_soap = 0;
+  bool have_G = false;
+  DevBuf G;
+
+  // FP64 work and outputs
+  DevBuf rad, rad_der, ang, ang_r, ang_a, ang_p, cnk, soap, sdp, cart, pair64, preflm, central;
+  DevBuf forces64, vpart64, virial64;
+  bool have_forces64 = false;
+  DevBuf comp, out_ptr, out_c, out_el, skip, unc_ptr, unc_e, P_i, P_el;
+  std::string tables_key;
+  int n_max = 0, k_max = 0, l_max = -1, n_soap = 0, n_soap_unc = 0;
+  bool compressed = false, have_soap = false, have_cart = false, ct_ready = false, have_pair64 = false;
+
+  // FP32 path
+  st::AdjointState adj;
+
+  // events
+  hipEvent_t ev[16] = {};
+  bool timing = false;
+  struct Pending { std::string name; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+  std::map<std::string, std::pair<double, int>> ktime;
+};
+
+namespace {
+
+int fail(st_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(ctx, ST_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(st_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.bytes >= bytes) return ST_OK;
+  if (b.p) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(ctx, ST_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return ST_OK;
+}
+
+int upload(st_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
+  int rc = ensure(ctx, b, bytes);
+  if (rc) return rc;
+  if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return ST_OK;
+}
+
+hipEvent_t get_event(st_ctx* ctx) {
+  if (!ctx->free_events.empty()) {
+    hipEvent_t e = ctx->free_events.back();
+    ctx->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Runs f() on the stream, bracketed by events when per-kernel timing is on.
+template <class F>
+void timed(st_ctx* ctx, const char* name, F&& f) {
+  if (!ctx->timing) {
+    f();
+    return;
+  }
+  hipEvent_t a = get_event(ctx), b = get_event(ctx);
+  (void)hipEventRecord(a, ctx->stream);
+  f();
+  (void)hipEventRecord(b, ctx->stream);
+  ctx->pending.push_back({name, a, b});
+}
+
+void resolve_timing(st_ctx* ctx) {
+  if (ctx->pending.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& pd : ctx->pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, pd.a, pd.b);
+    auto& t = ctx->ktime[pd.name];
+    t.first += ms;
+    t.second += 1;
+    ctx->free_events.push_back(pd.a);
+    ctx->free_events.push_back(pd.b);
+  }
+  ctx->pending.clear();
+}
+
+int n_max_of(const st_params* p) {
+  int n = 0;
+  for (int s = 0; s < p->n_species; ++s) n += p->alpha_max[s];
+  return n;
+}
+
+int k_max_of(int l) { return 1 + l * (l + 1) / 2 + l; }
+
+// Argument checks, with the reference's stop conditions and warnings
+// (src/soap_turbo_radial.f90:139-164, :205-230).
+int check_params(st_ctx* ctx, const st_params* p) {
+  if (!p || p->n_species < 1 || p->n_species > ST_MAX_SPECIES)
+    return fail(ctx, ST_ERR_ARG, "n_species must be in 1..32");
+  if (p->l_max < 0) return fail(ctx, ST_ERR_ARG, "l_max must be >= 0");
+  if (p->l_max > ST_MAX_LMAX) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max > 12 is not supported by this build");
+  if (p->basis != ST_BASIS_POLY3 && p->basis != ST_BASIS_POLY3GAUSS)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "only the poly3 and poly3gauss bases are supported");
+  static bool warned_p3 = false, warned_p3g = false;
+  for (int s = 0; s < p->n_species; ++s) {
+    const int a = p->alpha_max[s];
+    if (a < 1) return fail(ctx, ST_ERR_ARG, "alpha_max must be >= 1");
+    if (p->basis == ST_BASIS_POLY3) {
+      if (a > 10)
+        return fail(ctx, ST_ERR_STOP,
+                    "ERROR: Due to numerical instabilities in the basis construction for the poly3 basis, it is "
+                    "strongly recommended not to exceed alpha_max = 7. For alpha_max > 10 the instabilities are too "
+                    "large to proceed.");
+      if (a > 7 && !warned_p3) {
+        warned_p3 = true;
+        std::printf(" WARNING: Due to numerical instabilities in the basis construction for the poly3 basis, it is\n"
+                    " strongly recommended not to exceed alpha_max = 7.\n");
+      }
+    } else {
+      if (a > 11)
+        return fail(ctx, ST_ERR_STOP,
+                    "ERROR: Due to numerical instabilities in the basis construction for the poly3gauss basis, it is "
+                    "strongly recommended not to exceed alpha_max = 8. For alpha_max > 11 the instabilities are too "
+                    "large to proceed.");
+      if (a > 8 && !warned_p3g) {
+        warned_p3g = true;
+        std::printf(" WARNING: Due to numerical instabilities in the basis construction for the poly3gauss basis, it "
+                    "is\n strongly recommended not to exceed alpha_max = 8.\n");
+      }
+    }
+  }
+  return ST_OK;
+}
+
+// W and S for p: injected, cached, or built (block-diagonal, column-major).
+int basis_for(st_ctx* ctx, const st_params* p) {
+  const int n_max = n_max_of(p);
+  if (ctx->inj_n_max) {
+    if (ctx->inj_n_max != n_max) return fail(ctx, ST_ERR_ARG, "injected basis has a different n_max");
+    ctx->W = ctx->injW;
+    ctx->S = ctx->injS;
+    ctx->basis_key = "injected";
+    return ST_OK;
+  }
+  // the reference rebuilds W when the basis name or alpha_max change, and for
+  // poly3gauss also atom_sigma_r and rcut_hard (src/soap_turbo.f90:294-354)
+  std::string key(reinterpret_cast<const char*>(&p->basis), sizeof(int32_t));
+  key.append(reinterpret_cast<const char*>(p->alpha_max), sizeof(int32_t) * p->n_species);
+  if (p->basis == ST_BASIS_POLY3GAUSS) {
+    key.append(reinterpret_cast<const char*>(p->atom_sigma_r), sizeof(double) * p->n_species);
+    key.append(reinterpret_cast<const char*>(p->rcut_hard), sizeof(double) * p->n_species);
+  }
+  if (key == ctx->basis_key && (int)ctx->W.size() == n_max * n_max) return ST_OK;
+  ctx->W.assign((size_t)n_max * n_max, 0.0);
+  ctx->S.assign((size_t)n_max * n_max, 0.0);
+  int off = 0;
+  for (int s = 0; s < p->n_species; ++s) {
+    const int a = p->alpha_max[s];
+    std::vector<double> Sb((size_t)a * a), Wb((size_t)a * a);
+    st::overlap_matrix(p->basis, a, p->atom_sigma_r[s], p->rcut_hard[s], Sb.data());
+    if (!st::orthonormalization_matrix(a, Sb.data(), Wb.data()))
+      return fail(ctx, ST_ERR_STOP, "the overlap matrix of the radial basis is not positive definite");
+    for (int j = 0; j < a; ++j)
+      for (int i = 0; i < a; ++i) {
+        ctx->W[(off + i) + (size_t)(off + j) * n_max] = Wb[i + j * a];
+        ctx->S[(off + i) + (size_t)(off + j) * n_max] = Sb[i + j * a];
+      }
+    off += a;
+  }
+  ctx->basis_key = key;
+  return ST_OK;
+}
+
+// Per-species radial constants: radial_expansion_range set-up,
+// src/soap_turbo_radial.f90:436-540, computed with the same FP64 operations.
+st::RadialConsts radial_consts(const st_params* p, int s, const std::vector<double>& W, int n_max, int off,
+                               bool do_der) {
+  st::RadialConsts c;
+  std::memset(&c, 0, sizeof c);
+  const int a = p->alpha_max[s];
+  c.alpha_max = a;
+  c.n_max = n_max;
+  c.off = off;
+  c.species_bit = 1u << s;
+  c.with_gauss = p->basis == ST_BASIS_POLY3GAUSS;
+  c.do_central = c.with_gauss ? 0 : (p->central_weight[s] != 0.0);
+  c.unified = !p->legacy_filter_seed;
+  c.radial_enh = p->radial_enhancement;
+  c.do_der = do_der;
+  c.scaling = p->atom_sigma_r_scaling[s];
+  c.amp_scaling = p->amplitude_scaling[s];
+  c.amp_mode = p->scaling_mode != ST_SCALING_POLYNOMIAL ? 0
+               : c.amp_scaling == 0.0                    ? 1
+               : c.amp_scaling == 1.0                    ? 2
+                                                         : 3;
+  c.central_weight = p->central_weight[s];
+  c.global_scaling = p->global_scaling[s];
+  c.rcut_hard_in = p->rcut_hard[s];
+  c.inv_rcut = 1.0 / p->rcut_hard[s];
+  c.rcut_soft = p->rcut_soft[s] * c.inv_rcut;
+  c.atom_sigma = p->atom_sigma_r[s] * c.inv_rcut;
+  c.dr = 1.0 - c.rcut_soft;
+  c.sqrt_rcut = std::sqrt(p->rcut_hard[s]);
+  c.inv_sqrt_rcut = 1.0 / c.sqrt_rcut;
+  c.sigma_sq0 = c.atom_sigma * c.atom_sigma;
+  c.N_gauss = std::sqrt(2.0 / c.atom_sigma) / std::pow(std::acos(-1.0), 0.25);
+  c.has_filter = c.dr > 0.0;
+  if (c.has_filter) {
+    c.drnf = c.dr / p->nf[s];
+    c.q = c.drnf * c.drnf;
+  }
+  c.n_poly = c.with_gauss ? a - 1 : a;
+  c.n_hi = c.n_poly + (do_der ? 2 : 0);
+  double Nv[ST_REC + 8];  // Nv(n) at n + 3
+  Nv[0] = 1.0;
+  for (int n = -2; n <= a + 4; ++n) Nv[n + 3] = std::sqrt(1.0 / (double)(2 * n + 5));
+  for (int n = -1; n <= c.n_hi; ++n) {
+    c.ra[n + 1] = (double)(n + 1) * Nv[n - 2 + 3] / Nv[n + 3];
+    c.rb[n + 1] = Nv[n - 1 + 3] / Nv[n + 3];
+    c.rc[n + 1] = 1.0 / Nv[n + 3];
+  }
+  c.ra[0] = 0.0;
+  c.inv_Nm2 = 1.0 / Nv[1];
+  for (int n = 1; n <= c.n_poly; ++n) {
+    c.R1[n] = Nv[n + 1 + 3] / Nv[n + 3];
+    c.R2[n] = Nv[n + 2 + 3] / Nv[n + 3];
+  }
+  for (int j = 0; j < a; ++j)
+    for (int i = 0; i < a; ++i) c.W[i + j * a] = W[(off + i) + (size_t)(off + j) * n_max];
+  return c;
+}
+
+// get_preflm, src/soap_turbo_functions.f90:215-241
+void preflm_host(int l_max, std::vector<double>& out) {
+  out.clear();
+  for (int l = 0; l <= l_max; ++l) {
+    double ppi = std::sqrt((double)(2 * l + 1) / 4.0 / std::acos(-1.0));
+    double fact2 = 1.0;
+    for (int i = 1; i <= l; ++i) fact2 = fact2 * (double)i;
+    double fact1 = fact2;
+    for (int m = 0; m <= l; ++m) {
+      if (m > 0) {
+        fact1 = fact1 / (double)(l + 1 - m);
+        fact2 = fact2 * (double)(l + m);
+      }
+      out.push_back(ppi * std::sqrt(fact1 / fact2));
+    }
+  }
+}
+
+int n_soap_of(const st_params* p, const st_compression* c) {
+  const int n_max = n_max_of(p);
+  if (c && c->P_nonzero > 0) {
+    int m = 0;
+    for (int i = 0; i < c->P_nonzero; ++i) m = std::max(m, c->P_i[i]);
+    return m;
+  }
+  return n_max * (n_max + 1) / 2 * (p->l_max + 1);
+}
+
+// Index tables of the spectrum and the compression (see power_k and
+// contract_k), and the poly3gauss central-atom term.
+int build_tables(st_ctx* ctx, const st_params* p, const st_compression* c) {
+  const int n_max = n_max_of(p), L = p->l_max, L1 = L + 1;
+  const int n_unc = n_max * (n_max + 1) / 2 * L1;
+  const bool cmp = c && c->P_nonzero > 0;
+  std::string key(reinterpret_cast<const char*>(&n_max), sizeof n_max);
+  key.append(reinterpret_cast<const char*>(&L), sizeof L);
+  if (cmp) {
+    key.append(reinterpret_cast<const char*>(c->P_i), sizeof(int32_t) * c->P_nonzero);
+    key.append(reinterpret_cast<const char*>(c->P_j), sizeof(int32_t) * c->P_nonzero);
+    key.append(reinterpret_cast<const char*>(c->P_el), sizeof(double) * c->P_nonzero);
+  }
+  ctx->n_max = n_max;
+  ctx->k_max = k_max_of(L);
+  ctx->l_max = L;
+  ctx->n_soap_unc = n_unc;
+  ctx->n_soap = n_soap_of(p, c);
+  ctx->compressed = cmp;
+  int rc;
+  if (key != ctx->tables_key) {
+    std::vector<int32_t> comp((size_t)4 * n_unc);
+    int cnt = 0;
+    for (int n = 0; n < n_max; ++n)
+      for (int np = n; np < n_max; ++np)
+        for (int l = 0; l <= L; ++l, ++cnt) {
+          comp[4 * cnt] = n; comp[4 * cnt + 1] = np; comp[4 * cnt + 2] = l; comp[4 * cnt + 3] = 0;
+        }
+    std::vector<uint8_t> skip(n_unc, cmp ? 1 : 0);
+    if ((rc = upload(ctx, ctx->comp, comp.data(), comp.size() * 4))) return rc;
+    if (cmp) {
+      const int nz = c->P_nonzero, ns = ctx->n_soap;
+      for (int k = 0; k < nz; ++k) {
+        if (c->P_j[k] < 1 || c->P_j[k] > n_unc || c->P_i[k] < 1)
+          return fail(ctx, ST_ERR_ARG, "compression index out of range");
+        skip[c->P_j[k] - 1] = 0;
+      }
+      std::vector<int> order(nz);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return c->P_i[a] < c->P_i[b]; });
+      std::vector<int32_t> optr(ns + 1, 0), oc(nz);
+      std::vector<double> oel(nz);
+      for (int k = 0; k < nz; ++k) optr[c->P_i[k]]++;
+      for (int s = 0; s < ns; ++s) optr[s + 1] += optr[s];
+      for (int e = 0; e < nz; ++e) { oc[e] = c->P_j[order[e]] - 1; oel[e] = c->P_el[order[e]]; }
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return c->P_j[a] < c->P_j[b]; });
+      std::vector<int32_t> uptr(n_unc + 1, 0), ue(nz);
+      for (int k = 0; k < nz; ++k) uptr[c->P_j[k]]++;
+      for (int u = 0; u < n_unc; ++u) uptr[u + 1] += uptr[u];
+      for (int e = 0; e < nz; ++e) ue[e] = order[e];
+      if ((rc = upload(ctx, ctx->out_ptr, optr.data(), optr.size() * 4))) return rc;
+      if ((rc = upload(ctx, ctx->out_c, oc.data(), oc.size() * 4))) return rc;
+      if ((rc = upload(ctx, ctx->out_el, oel.data(), oel.size() * 8))) return rc;
+      if ((rc = upload(ctx, ctx->unc_ptr, uptr.data(), uptr.size() * 4))) return rc;
+      if ((rc = upload(ctx, ctx->unc_e, ue.data(), ue.size() * 4))) return rc;
+      if ((rc = upload(ctx, ctx->P_i, c->P_i, (size_t)nz * 4))) return rc;
+      if ((rc = upload(ctx, ctx->P_el, c->P_el, (size_t)nz * 8))) return rc;
+    }
+    if ((rc = upload(ctx, ctx->skip, skip.data(), skip.size()))) return rc;
+    std::vector<double> pf;
+    preflm_host(L, pf);
+    if ((rc = upload(ctx, ctx->preflm, pf.data(), pf.size() * 8))) return rc;
+    ctx->tables_key = key;
+  }
+  return ST_OK;
+}
+
+// poly3gauss central-atom term per species (src/soap_turbo.f90:646-661):
+// fac * (W S(:, last)) on the species block; zero for other bases.
+int build_central(st_ctx* ctx, const st_params* p, std::vector<double>& cen) {
+  const int n_max = n_max_of(p);
+  cen.assign((size_t)p->n_species * n_max, 0.0);
+  if (p->basis != ST_BASIS_POLY3GAUSS) return ST_OK;
+  const double pi = std::acos(-1.0);
+  int off = 0;
+  for (int j = 0; j < p->n_species; ++j) {
+    const int a = p->alpha_max[j];
+    if (p->central_weight[j] != 0.0) {
+      double amplitude;
+      if (p->radial_enhancement == 1) amplitude = std::sqrt(2.0 / pi) * p->atom_sigma_r[j] / p->rcut_hard[j];
+      else if (p->radial_enhancement == 2)
+        amplitude = (p->atom_sigma_r[j] * p->atom_sigma_r[j]) / (p->rcut_hard[j] * p->rcut_hard[j]);
+      else amplitude = 1.0;
+      const double rc = p->rcut_hard[j];
+      const double fac = amplitude * p->central_weight[j] * std::sqrt(4.0 * pi) * std::pow(pi, 0.25) *
+                         std::sqrt(p->atom_sigma_r[j] / 2.0) * (rc * rc * rc) /
+                         (p->atom_sigma_t[j] * p->atom_sigma_t[j]) / p->atom_sigma_r[j];
+      const int col = off + a - 1;
+      for (int n = 0; n < a; ++n) {
+        double v = 0.0;
+        for (int m = 0; m < a; ++m) v = v + ctx->W[(off + n) + (size_t)(off + m) * n_max] * ctx->S[(off + m) + (size_t)col * n_max];
+        cen[(size_t)j * n_max + off + n] = fac * v;
+      }
+    }
+    off += a;
+  }
+  return ST_OK;
+}
+
+int pack_mask(st_ctx* ctx, int n_species) {
+  if (n_species > ctx->n_species_mask) return fail(ctx, ST_ERR_ARG, "mask has fewer columns than n_species");
+  int rc;
+  if ((rc = ensure(ctx, ctx->spmask, (size_t)ctx->n_pairs * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->first, (size_t)ctx->n_pairs))) return rc;
+  st::launch_pack_pairs(ctx->stream, ctx->n_pairs, n_species, ctx->mask.as<int32_t>(), ctx->site_ptr.as<int>(),
+                        ctx->n_sites, ctx->spmask.as<uint32_t>(), ctx->first.as<uint8_t>());
+  return ST_OK;
+}
+
+// Forces and virial of the FP64 pair gradients, when the pair atoms are known.
+int scatter64(st_ctx* ctx) {
+  ctx->have_forces64 = false;
+  if (!ctx->have_atoms) return ST_OK;
+  int rc;
+  if ((rc = ensure(ctx, ctx->forces64, (size_t)ctx->n_atoms * 24))) return rc;
+  if ((rc = ensure(ctx, ctx->vpart64, (size_t)st::SCATTER_BLOCKS * 72))) return rc;
+  if ((rc = ensure(ctx, ctx->virial64, 72))) return rc;
+  timed(ctx, "forces_f64", [&] {
+    st::launch_scatter_f64(ctx->stream, ctx->n_pairs, ctx->n_atoms, ctx->pair64.as<double>(),
+                           ctx->pair_atom.as<int32_t>(), ctx->have_vec ? ctx->vec.as<double>() : nullptr,
+                           ctx->rjs.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
+                           ctx->forces64.as<double>(), ctx->vpart64.as<double>(), ctx->virial64.as<double>());
+  });
+  ctx->have_forces64 = true;
+  return ST_OK;
+}
+
+}  // namespace
+
+// =====================================================================================================
+extern "C" {
+
+int st_create(st_ctx** out, int device) {
+  if (!out) return ST_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return ST_ERR_HIP;
+  if (device < 0 || device >= n) return ST_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return ST_ERR_HIP;
+  st_ctx* c = new st_ctx;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return ST_ERR_HIP;
+  }
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return ST_OK;
+}
+
+void st_destroy(st_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev) (void)hipEventDestroy(e);
+  for (auto& pd : c->pending) { (void)hipEventDestroy(pd.a); (void)hipEventDestroy(pd.b); }
+  for (auto e : c->free_events) (void)hipEventDestroy(e);
+  st::adjoint_release(c->adj);
+  hipStream_t s = c->stream;
+  delete c;
+  (void)hipStreamDestroy(s);
+}
+
+const char* st_error(const st_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int st_n_soap(const st_params* p, const st_compression* c) { return p ? n_soap_of(p, c) : -1; }
+
+int st_get_basis(st_ctx* ctx, const st_params* p, double* W, double* S) {
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if ((rc = basis_for(ctx, p))) return rc;
+  if (W) std::memcpy(W, ctx->W.data(), ctx->W.size() * 8);
+  if (S) std::memcpy(S, ctx->S.data(), ctx->S.size() * 8);
+  return ST_OK;
+}
+
+int st_set_basis(st_ctx* ctx, int32_t n_max, const double* W, const double* S) {
+  if (!ctx) return ST_ERR_ARG;
+  if (n_max <= 0) {
+    ctx->inj_n_max = 0;
+    ctx->basis_key.clear();
+    return ST_OK;
+  }
+  if (!W || !S) return fail(ctx, ST_ERR_ARG, "W and S are required");
+  ctx->inj_n_max = n_max;
+  ctx->injW.assign(W, W + (size_t)n_max * n_max);
+  ctx->injS.assign(S, S + (size_t)n_max * n_max);
+  ctx->basis_key.clear();
+  return ST_OK;
+}
+
+int st_upload_neighbours(st_ctx* ctx, const st_neighbours* nb) {
+  if (!ctx || !nb) return ST_ERR_ARG;
+  if (nb->n_sites < 0 || nb->n_pairs < 0 || nb->n_species < 1 || nb->n_species > ST_MAX_SPECIES || nb->max_mult < 1)
+    return fail(ctx, ST_ERR_ARG, "bad neighbour-list sizes");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<int> ptr(nb->n_sites + 1, 0);
+  for (int i = 0; i < nb->n_sites; ++i) {
+    if (nb->n_neigh[i] < 1) return fail(ctx, ST_ERR_ARG, "every site needs at least its own pair (n_neigh >= 1)");
+    ptr[i + 1] = ptr[i] + nb->n_neigh[i];
+  }
+  if (ptr[nb->n_sites] != nb->n_pairs) return fail(ctx, ST_ERR_ARG, "sum(n_neigh) /= n_atom_pairs");
+  int rc;
+  ctx->n_sites = nb->n_sites;
+  ctx->n_pairs = nb->n_pairs;
+  ctx->n_species_mask = nb->n_species;
+  ctx->max_mult = nb->max_mult;
+  ctx->site_ptr_h = ptr;
+  const size_t P = nb->n_pairs;
+  if ((rc = upload(ctx, ctx->site_ptr, ptr.data(), ptr.size() * 4))) return rc;
+  if ((rc = upload(ctx, ctx->species, nb->species, (size_t)nb->max_mult * nb->n_sites * 4))) return rc;
+  if ((rc = upload(ctx, ctx->mult, nb->species_multiplicity, (size_t)nb->n_sites * 4))) return rc;
+  if ((rc = upload(ctx, ctx->mask, nb->mask, P * nb->n_species * 4))) return rc;
+  if ((rc = upload(ctx, ctx->rjs, nb->rjs, P * 8))) return rc;
+  if ((rc = upload(ctx, ctx->thetas, nb->thetas, P * 8))) return rc;
+  if ((rc = upload(ctx, ctx->phis, nb->phis, P * 8))) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // the host arrays may go away after return
+  ctx->have_nb = true;
+  ctx->have_soap = ctx->have_cart = ctx->ct_ready = ctx->have_pair64 = false;
+  ctx->have_vec = false;
+  st::adjoint_invalidate(ctx->adj);
+  return ST_OK;
+}
+
+int st_upload_atoms(st_ctx* ctx, int32_t n_atoms, const int32_t* pair_atom, const double* vec) {
+  if (!ctx || !ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
+  if (n_atoms < 1 || !pair_atom) return fail(ctx, ST_ERR_ARG, "pair_atom and n_atoms are required");
+  for (int p = 0; p < ctx->n_pairs; ++p)
+    if (pair_atom[p] < 0 || pair_atom[p] >= n_atoms) return fail(ctx, ST_ERR_ARG, "pair_atom out of range");
+  int rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  if ((rc = upload(ctx, ctx->pair_atom, pair_atom, (size_t)ctx->n_pairs * 4))) return rc;
+  ctx->have_vec = vec != nullptr;
+  if (vec && (rc = upload(ctx, ctx->vec, vec, (size_t)ctx->n_pairs * 24))) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->n_atoms = n_atoms;
+  ctx->have_atoms = true;
+  return ST_OK;
+}
+
+int st_upload_dedq(st_ctx* ctx, int32_t n_soap, const double* G) {
+  if (!ctx || !ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
+  if (n_soap < 1 || !G) return fail(ctx, ST_ERR_ARG, "bad dE/dq");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = upload(ctx, ctx->G, G, (size_t)n_soap * ctx->n_sites * 8);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->G_n_soap = n_soap;
+  ctx->have_G = true;
+  return ST_OK;
+}
+
+int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t do_derivatives, int32_t contract) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  if ((rc = basis_for(ctx, p))) return rc;
+  if ((rc = build_tables(ctx, p, c))) return rc;
+  const bool der = do_derivatives != 0, ctr = der && contract != 0;
+  const int n_max = ctx->n_max, k_max = ctx->k_max, L = ctx->l_max, n_soap = ctx->n_soap;
+  const int NS = ctx->n_sites;
+  const size_t P = ctx->n_pairs;
+  if ((rc = pack_mask(ctx, p->n_species))) return rc;
+  std::vector<double> cen;
+  build_central(ctx, p, cen);
+  if ((rc = upload(ctx, ctx->central, cen.data(), cen.size() * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->rad, P * n_max * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->rad_der, P * n_max * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->ang, P * k_max * 16))) return rc;
+  if (der) {
+    if ((rc = ensure(ctx, ctx->ang_r, P * k_max * 16))) return rc;
+    if ((rc = ensure(ctx, ctx->ang_a, P * k_max * 16))) return rc;
+    if ((rc = ensure(ctx, ctx->ang_p, P * k_max * 16))) return rc;
+  }
+  if ((rc = ensure(ctx, ctx->cnk, (size_t)NS * n_max * k_max * 16))) return rc;
+  if ((rc = ensure(ctx, ctx->soap, (size_t)NS * n_soap * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->sdp, (size_t)NS * 8))) return rc;
+  if (der && !ctr && (rc = ensure(ctx, ctx->cart, P * n_soap * 24))) return rc;
+  hipStream_t st = ctx->stream;
+
+  timed(ctx, "fill_rad", [&] {
+    st::launch_fill(st, ctx->rad.as<double>(), P * n_max, 0.0);
+    if (der) st::launch_fill(st, ctx->rad_der.as<double>(), P * n_max, 0.0);
+  });
+  int off = 0;
+  for (int s = 0; s < p->n_species; ++s) {
+    st::RadialConsts rcst = radial_consts(p, s, ctx->W, n_max, off, der);
+    bool ok = true;
+    timed(ctx, "radial_f64", [&] {
+      ok = st::launch_radial(st, rcst, (int)P, ctx->rjs.as<double>(), ctx->spmask.as<uint32_t>(),
+                             ctx->first.as<uint8_t>(), ctx->rad.as<double>(), ctx->rad_der.as<double>());
+    });
+    if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "alpha_max out of the compiled range");
+    off += p->alpha_max[s];
+  }
+  st::AngularConsts ac;
+  std::memset(&ac, 0, sizeof ac);
+  ac.n_species = p->n_species;
+  ac.l_max = L;
+  ac.k_max = k_max;
+  ac.do_der = der;
+  for (int s = 0; s < p->n_species; ++s) {
+    ac.rcut_max = std::max(ac.rcut_max, p->rcut_hard[s]);
+    ac.sigma_t[s] = p->atom_sigma_t[s];
+    ac.sigma_t_scaling[s] = p->atom_sigma_t_scaling[s];
+  }
+  ac.rcut2 = ac.rcut_max * ac.rcut_max;
+  bool ok = true;
+  timed(ctx, "angular_f64", [&] {
+    ok = st::launch_angular(st, ac, (int)P, ctx->rjs.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
+                            ctx->spmask.as<uint32_t>(), ctx->preflm.as<double>(), ctx->ang.as<double2>(),
+                            der ? ctx->ang_r.as<double2>() : nullptr, der ? ctx->ang_a.as<double2>() : nullptr,
+                            der ? ctx->ang_p.as<double2>() : nullptr);
+  });
+  if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max out of the compiled range");
+  timed(ctx, "cnk_f64", [&] {
+    st::launch_cnk(st, NS, n_max, k_max, ctx->site_ptr.as<int>(), ctx->rad.as<double>(), ctx->ang.as<double2>(),
+                   ctx->species.as<int32_t>(), ctx->max_mult, ctx->mult.as<int32_t>(),
+                   p->basis == ST_BASIS_POLY3GAUSS ? ctx->central.as<double>() : nullptr, ctx->cnk.as<double2>());
+  });
+  const int* optr = ctx->compressed ? ctx->out_ptr.as<int>() : nullptr;
+  timed(ctx, "power_f64", [&] {
+    st::launch_power(st, NS, n_max, k_max, n_soap, ctx->comp.as<int4>(), optr, ctx->out_c.as<int>(),
+                     ctx->out_el.as<double>(), ctx->cnk.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>());
+  });
+  if (der && !ctr) {
+    timed(ctx, "derivatives_f64", [&] {
+      ok = st::launch_derivatives(st, NS, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(), ctx->comp.as<int4>(), optr,
+                                  ctx->out_c.as<int>(), ctx->out_el.as<double>(), ctx->cnk.as<double2>(),
+                                  ctx->rad.as<double>(), ctx->rad_der.as<double>(), ctx->ang.as<double2>(),
+                                  ctx->ang_r.as<double2>(), ctx->ang_a.as<double2>(), ctx->ang_p.as<double2>(),
+                                  ctx->soap.as<double>(), ctx->sdp.as<double>(), ctx->rjs.as<double>(),
+                                  ctx->thetas.as<double>(), ctx->phis.as<double>(), ctx->cart.as<double>());
+    });
+    if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "n_soap > 3072 is not supported with derivatives");
+  }
+  timed(ctx, "normalise_f64", [&] { st::launch_normalise(st, NS, n_soap, ctx->sdp.as<double>(), ctx->soap.as<double>()); });
+  HIPCHK(hipGetLastError());
+  ctx->have_soap = true;
+  ctx->have_cart = der && !ctr;
+  ctx->ct_ready = ctr;
+  ctx->have_pair64 = false;
+  return ST_OK;
+}
+
+int st_pairs_full_jacobian(st_ctx* ctx) {
+  if (!ctx || !ctx->have_cart) return fail(ctx, ST_ERR_STATE, "soap_cart_der has not been computed");
+  if (!ctx->have_G || ctx->G_n_soap != ctx->n_soap) return fail(ctx, ST_ERR_STATE, "dE/dq missing or of another size");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = ensure(ctx, ctx->pair64, (size_t)ctx->n_pairs * 24);
+  if (rc) return rc;
+  timed(ctx, "pairs_full_jacobian_f64", [&] {
+    st::launch_contract_cart(ctx->stream, ctx->n_sites, ctx->n_pairs, ctx->n_soap, ctx->site_ptr.as<int>(),
+                             ctx->G.as<double>(), ctx->cart.as<double>(), ctx->pair64.as<double>());
+  });
+  HIPCHK(hipGetLastError());
+  ctx->have_pair64 = true;
+  return scatter64(ctx);
+}
+
+int st_contract_resident(st_ctx* ctx, double sgn) {
+  if (!ctx || !ctx->ct_ready)
+    return fail(ctx, ST_ERR_STATE, "ERROR: soap_contract needs get_soap called with contract = .true. first");
+  if (!ctx->have_G || ctx->G_n_soap != ctx->n_soap) return fail(ctx, ST_ERR_STATE, "dE/dq missing or of another size");
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t sh = st::contract_lds(ctx->n_max, ctx->k_max, ctx->n_soap, ctx->n_soap_unc);
+  if (sh > 160 * 1024) return fail(ctx, ST_ERR_UNSUPPORTED, "soap_contract working set exceeds the LDS");
+  int rc = ensure(ctx, ctx->pair64, (size_t)ctx->n_pairs * 24);
+  if (rc) return rc;
+  timed(ctx, "contract_f64", [&] {
+    st::launch_contract(ctx->stream, ctx->n_sites, ctx->n_max, ctx->l_max, ctx->k_max, ctx->n_soap, ctx->n_soap_unc,
+                        ctx->site_ptr.as<int>(), ctx->skip.as<uint8_t>(),
+                        ctx->compressed ? ctx->unc_ptr.as<int>() : nullptr, ctx->unc_e.as<int>(),
+                        ctx->P_i.as<int32_t>(), ctx->P_el.as<double>(), ctx->cnk.as<double2>(), ctx->rad.as<double>(),
+                        ctx->rad_der.as<double>(), ctx->ang.as<double2>(), ctx->ang_r.as<double2>(),
+                        ctx->ang_a.as<double2>(), ctx->ang_p.as<double2>(), ctx->soap.as<double>(), ctx->G.as<double>(),
+                        sgn, ctx->sdp.as<double>(), ctx->rjs.as<double>(), ctx->thetas.as<double>(),
+                        ctx->phis.as<double>(), ctx->pair64.as<double>());
+  });
+  HIPCHK(hipGetLastError());
+  ctx->have_pair64 = true;
+  return scatter64(ctx);
+}
+
+int st_download(st_ctx* ctx, double* soap, double* cart) {
+  if (!ctx || !ctx->have_soap) return fail(ctx, ST_ERR_STATE, "nothing computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (soap)
+    HIPCHK(hipMemcpyAsync(soap, ctx->soap.p, (size_t)ctx->n_sites * ctx->n_soap * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (cart) {
+    if (!ctx->have_cart) return fail(ctx, ST_ERR_STATE, "soap_cart_der was not computed");
+    HIPCHK(hipMemcpyAsync(cart, ctx->cart.p, (size_t)ctx->n_pairs * ctx->n_soap * 24, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return ST_OK;
+}
+
+int st_download_pairs(st_ctx* ctx, int32_t which, double* pair) {
+  if (!ctx || !pair) return ST_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (which == 0) {
+    if (!ctx->have_pair64) return fail(ctx, ST_ERR_STATE, "no FP64 pair gradients computed");
+    HIPCHK(hipMemcpyAsync(pair, ctx->pair64.p, (size_t)ctx->n_pairs * 24, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ST_OK;
+  }
+  std::string msg;
+  int rc = st::adjoint_download_pairs(ctx->adj, ctx->stream, ctx->n_pairs, pair, msg);
+  return rc ? fail(ctx, rc, msg) : ST_OK;
+}
+
+int st_download_forces(st_ctx* ctx, int32_t which, double* forces, double* virial) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!ctx->have_atoms) return fail(ctx, ST_ERR_STATE, "upload the pair atoms first (st_upload_atoms)");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::string msg;
+  int rc;
+  if (which == 0) {
+    if (!ctx->have_forces64) return fail(ctx, ST_ERR_STATE, "no FP64 forces computed");
+    if (forces) HIPCHK(hipMemcpyAsync(forces, ctx->forces64.p, (size_t)ctx->n_atoms * 24, hipMemcpyDeviceToHost, ctx->stream));
+    if (virial) HIPCHK(hipMemcpyAsync(virial, ctx->virial64.p, 72, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ST_OK;
+  } else {
+    rc = st::adjoint_download_forces(ctx->adj, ctx->stream, ctx->n_atoms, forces, virial, msg);
+  }
+  return rc ? fail(ctx, rc, msg) : ST_OK;
+}
+
+int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  if ((rc = basis_for(ctx, p))) return rc;
+  if ((rc = build_tables(ctx, p, c))) return rc;
+  if (!ctx->have_G || ctx->G_n_soap != ctx->n_soap) return fail(ctx, ST_ERR_STATE, "dE/dq missing or of another size");
+  if ((rc = pack_mask(ctx, p->n_species))) return rc;
+  std::vector<double> cen;
+  build_central(ctx, p, cen);
+  st::AdjointInputs in;
+  in.p = p;
+  in.W = &ctx->W;
+  in.central = &cen;
+  in.n_sites = ctx->n_sites;
+  in.n_pairs = ctx->n_pairs;
+  in.n_max = ctx->n_max;
+  in.k_max = ctx->k_max;
+  in.l_max = ctx->l_max;
+  in.n_soap = ctx->n_soap;
+  in.n_soap_unc = ctx->n_soap_unc;
+  in.site_ptr_h = &ctx->site_ptr_h;
+  in.site_ptr = ctx->site_ptr.as<int>();
+  in.species = ctx->species.as<int32_t>();
+  in.max_mult = ctx->max_mult;
+  in.mult = ctx->mult.as<int32_t>();
+  in.spmask = ctx->spmask.as<uint32_t>();
+  in.first = ctx->first.as<uint8_t>();
+  in.rjs = ctx->rjs.as<double>();
+  in.thetas = ctx->thetas.as<double>();
+  in.phis = ctx->phis.as<double>();
+  in.G = ctx->G.as<double>();
+  in.comp = ctx->comp.as<int4>();
+  in.out_ptr = ctx->compressed ? ctx->out_ptr.as<int>() : nullptr;
+  in.out_c = ctx->out_c.as<int>();
+  in.out_el = ctx->out_el.as<double>();
+  in.pair_atom = ctx->have_atoms ? ctx->pair_atom.as<int32_t>() : nullptr;
+  in.vec = ctx->have_vec ? ctx->vec.as<double>() : nullptr;
+  in.n_atoms = ctx->have_atoms ? ctx->n_atoms : 0;
+  std::string msg;
+  auto timer = [ctx](const char* name, const std::function<void()>& f) { timed(ctx, name, f); };
+  rc = st::adjoint_run(ctx->adj, ctx->stream, in, timer, msg);
+  if (rc) return fail(ctx, rc, msg);
+  HIPCHK(hipGetLastError());
+  return ST_OK;
+}
+
+int st_download_soap_f32(st_ctx* ctx, double* soap) {
+  if (!ctx || !soap) return ST_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::string msg;
+  int rc = st::adjoint_download_soap(ctx->adj, ctx->stream, ctx->n_sites, ctx->n_soap, soap, msg);
+  return rc ? fail(ctx, rc, msg) : ST_OK;
+}
+
+int st_get_soap(st_ctx* ctx, const st_params* p, const st_neighbours* nb, const st_compression* c,
+                int32_t do_derivatives, int32_t contract, int32_t n_soap, double* soap, double* cart) {
+  if (!ctx || !p || !nb || !soap) return fail(ctx, ST_ERR_ARG, "missing argument");
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (n_soap != n_soap_of(p, c)) return fail(ctx, ST_ERR_ARG, "soap has the wrong leading dimension");
+  if ((rc = st_upload_neighbours(ctx, nb))) return rc;
+  if ((rc = st_compute(ctx, p, c, do_derivatives, contract))) return rc;
+  const bool want_cart = do_derivatives && !contract;
+  if (want_cart && !cart) return fail(ctx, ST_ERR_ARG, "soap_cart_der is required with derivatives");
+  return st_download(ctx, soap, want_cart ? cart : nullptr);
+}
+
+int st_soap_contract(st_ctx* ctx, const double* soap, const double* G, double sgn, double* pair) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!ctx->ct_ready)
+    return fail(ctx, ST_ERR_STATE, "ERROR: soap_contract needs get_soap called with contract = .true. first");
+  if (!soap || !G || !pair) return fail(ctx, ST_ERR_ARG, "missing argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc;
+  // the reference contracts against the soap it is handed, not its own copy
+  if ((rc = upload(ctx, ctx->soap, soap, (size_t)ctx->n_sites * ctx->n_soap * 8))) return rc;
+  if ((rc = upload(ctx, ctx->G, G, (size_t)ctx->n_sites * ctx->n_soap * 8))) return rc;
+  ctx->G_n_soap = ctx->n_soap;
+  ctx->have_G = true;
+  if ((rc = st_contract_resident(ctx, sgn))) return rc;
+  return st_download_pairs(ctx, 0, pair);
+}
+
+int st_get_compress_indices(const char* mode, int32_t n_species, const int32_t* alpha_max, int32_t l_max,
+                            int32_t* dim, int32_t* P_nonzero, int32_t* P_i, int32_t* P_j, double* P_el,
+                            int32_t set_indices) {
+  // "trivial": keep (n, m) when either is the first radial index of a species
+  // (src/soap_turbo_compress.f90:86-119)
+  if (!mode || !alpha_max || !dim || !P_nonzero || n_species < 1 || l_max < 0) return ST_ERR_ARG;
+  if (std::strcmp(mode, "trivial") != 0 && std::strcmp(mode, "Trivial") != 0) return ST_ERR_UNSUPPORTED;
+  std::vector<int> pivot(n_species);
+  pivot[0] = 1;
+  for (int i = 0; i + 1 < n_species; ++i) pivot[i + 1] = pivot[i] + alpha_max[i];
+  int n_max = 0;
+  for (int i = 0; i < n_species; ++i) n_max += alpha_max[i];
+  int counter = 0, k = 1;
+  for (int n = 1; n <= n_max; ++n)
+    for (int m = n; m <= n_max; ++m)
+      for (int l = 0; l <= l_max; ++l) {
+        bool keep = false;
+        for (int q : pivot) keep = keep || n == q || m == q;
+        if (keep) {
+          ++counter;
+          if (set_indices) {
+            P_i[counter - 1] = counter;
+            P_j[counter - 1] = k;
+            P_el[counter - 1] = 1.0;
+          }
+        }
+        ++k;
+      }
+  *dim = counter;
+  *P_nonzero = counter;
+  return ST_OK;
+}
+
+int st_synchronize(st_ctx* ctx) {
+  if (!ctx) return ST_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return ST_OK;
+}
+
+int st_event_record(st_ctx* ctx, int32_t slot) {
+  if (!ctx || slot < 0 || slot >= 16) return ST_ERR_ARG;
+  HIPCHK(hipEventRecord(ctx->ev[slot], ctx->stream));
+  return ST_OK;
+}
+
+int st_event_elapsed(st_ctx* ctx, int32_t a, int32_t b, float* ms) {
+  if (!ctx || a < 0 || a >= 16 || b < 0 || b >= 16 || !ms) return ST_ERR_ARG;
+  HIPCHK(hipEventSynchronize(ctx->ev[b]));
+  HIPCHK(hipEventElapsedTime(ms, ctx->ev[a], ctx->ev[b]));
+  return ST_OK;
+}
+
+int st_kernel_timing(st_ctx* ctx, int32_t on) {
+  if (!ctx) return ST_ERR_ARG;
+  resolve_timing(ctx);
+  ctx->timing = on != 0;
+  ctx->ktime.clear();
+  return ST_OK;
+}
+
+int st_kernel_time(st_ctx* ctx, const char* name, double* total_ms, int32_t* launches) {
+  if (!ctx || !name) return ST_ERR_ARG;
+  resolve_timing(ctx);
+  auto it = ctx->ktime.find(name);
+  if (total_ms) *total_ms = it == ctx->ktime.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == ctx->ktime.end() ? 0 : it->second.second;
+  return ST_OK;
+}
+
+}  // extern "C"

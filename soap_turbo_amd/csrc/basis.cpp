@@ -1,0 +1,145 @@
+// Host FP64 construction of the radial basis transformation W = S^{-1/2}.
+//
+// The reference (get_orthonormalization_matrix_poly3 / _poly3gauss,
+// src/soap_turbo_radial.f90:1023-1219) builds S in closed form (poly3) or by
+// the Gaussian-polynomial overlap recursion (poly3gauss), then takes an SVD of
+// S, forms S^{1/2} = U sqrt(Sigma) V^T, and inverts it through a Cholesky
+// factorisation (dpotrf/dpotri). The same steps are taken here with a cyclic
+// Jacobi eigensolver (S is symmetric positive definite, so its SVD is its
+// eigendecomposition) and a hand-written Cholesky inverse, in FP64, on the
+// host: the matrices are at most 12 x 12 and are built once per basis.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "st_internal.h"
+
+namespace st {
+
+// Overlap matrix of the primitive functions, column-major a x a.
+void overlap_matrix(int basis, int a, double atom_sigma_r, double rcut_hard, double* S) {
+  std::memset(S, 0, sizeof(double) * a * a);
+  const int np = basis == 1 ? a - 1 : a;
+  for (int i = 1; i <= np; ++i) {
+    S[(i - 1) + (i - 1) * a] = 1.0;
+    for (int j = i + 1; j <= np; ++j) {
+      double v = std::sqrt((double)(5 + 2 * i) * (double)(5 + 2 * j)) / (double)(5 + i + j);
+      S[(i - 1) + (j - 1) * a] = v;
+      S[(j - 1) + (i - 1) * a] = v;
+    }
+  }
+  if (basis != 1) return;
+  // Gaussian at the origin against the polynomials: the recursion of
+  // src/soap_turbo_radial.f90:1133-1172, in units of rcut_hard.
+  const double sg = atom_sigma_r / rcut_hard, rc = 1.0;
+  const double s2 = sg * sg, sq2 = std::sqrt(2.0), pi = std::acos(-1.0);
+  double I_n = 0.0, N_n = 1.0;
+  double N_np1 = std::sqrt(rc / 1.0);  // N_a(rc, -2) = sqrt(rc / (2*(-2)+5))
+  double I_np1 = std::sqrt(pi / 2.0) * sg * std::erf(rc / sq2 / sg) / N_np1;
+  double C2 = s2 / rc;
+  for (int n = -1; n <= a - 1; ++n) {
+    C2 = C2 * rc;
+    double N_np2 = std::sqrt(rc / (double)(2 * n + 5));
+    double I_np2 = s2 * (double)(n + 1) * N_n / N_np2 * I_n + N_np1 * rc / N_np2 * I_np1 - C2 / N_np2;
+    if (n > 0) {
+      double v = I_np2 * sq2 / std::sqrt(sg) / std::pow(pi, 0.25);
+      S[(a - 1) + (n - 1) * a] = v;
+      S[(n - 1) + (a - 1) * a] = v;
+    }
+    N_n = N_np1;
+    N_np1 = N_np2;
+    I_n = I_np1;
+    I_np1 = I_np2;
+  }
+  S[(a - 1) + (a - 1) * a] = 1.0;
+}
+
+// Cyclic Jacobi: A (column-major, symmetric) -> eigenvalues w, eigenvectors V.
+static void jacobi_eigen(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
+  V.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[i + i * n] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) (i == j ? diag : off) += A[i + j * n] * A[i + j * n];
+    if (off <= 1e-34 * diag) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[p + q * n];
+        if (apq == 0.0) continue;
+        double app = A[p + p * n], aqq = A[q + q * n];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // A <- A J (columns p, q)
+          double akp = A[k + p * n], akq = A[k + q * n];
+          A[k + p * n] = c * akp - s * akq;
+          A[k + q * n] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // A <- J^T A (rows p, q)
+          double apk = A[p + k * n], aqk = A[q + k * n];
+          A[p + k * n] = c * apk - s * aqk;
+          A[q + k * n] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vkp = V[k + p * n], vkq = V[k + q * n];
+          V[k + p * n] = c * vkp - s * vkq;
+          V[k + q * n] = s * vkp + c * vkq;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; ++i) w[i] = A[i + i * n];
+}
+
+// W = (S^{1/2})^{-1}, S^{1/2} from the eigendecomposition, inverse through
+// S^{1/2} = U^T U (Cholesky, upper) as dpotrf/dpotri do. Returns false if S
+// is not positive definite.
+bool orthonormalization_matrix(int a, const double* S, double* W) {
+  std::vector<double> A(S, S + (size_t)a * a), w, V;
+  jacobi_eigen(a, A, w, V);
+  std::vector<double> H((size_t)a * a, 0.0);
+  for (int i = 0; i < a; ++i) {
+    if (!(w[i] > 0.0)) return false;
+  }
+  for (int j = 0; j < a; ++j)
+    for (int i = 0; i < a; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < a; ++k) s += V[i + k * a] * std::sqrt(w[k]) * V[j + k * a];
+      H[i + j * a] = s;
+    }
+  // Cholesky H = U^T U, U upper
+  std::vector<double> U((size_t)a * a, 0.0);
+  for (int j = 0; j < a; ++j) {
+    double d = H[j + j * a];
+    for (int k = 0; k < j; ++k) d -= U[k + j * a] * U[k + j * a];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    U[j + j * a] = d;
+    for (int i = j + 1; i < a; ++i) {
+      double s = H[j + i * a];
+      for (int k = 0; k < j; ++k) s -= U[k + j * a] * U[k + i * a];
+      U[j + i * a] = s / d;
+    }
+  }
+  // Ui = U^{-1} (upper triangular)
+  std::vector<double> Ui((size_t)a * a, 0.0);
+  for (int j = 0; j < a; ++j) {
+    Ui[j + j * a] = 1.0 / U[j + j * a];
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += U[i + k * a] * Ui[k + j * a];
+      Ui[i + j * a] = -s / U[i + i * a];
+    }
+  }
+  // H^{-1} = Ui Ui^T
+  for (int j = 0; j < a; ++j)
+    for (int i = 0; i < a; ++i) {
+      double s = 0.0;
+      for (int k = (i > j ? i : j); k < a; ++k) s += Ui[i + k * a] * Ui[j + k * a];
+      W[i + j * a] = s;
+    }
+  return true;
+}
+
+}  // namespace st

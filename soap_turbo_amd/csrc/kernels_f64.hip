@@ -1,0 +1,863 @@
+// FP64 kernels of the get_soap hot path ("plain" path): radial and angular
+// expansions per pair, the expansion coefficients cnk and the power spectrum
+// per site, the full Jacobian soap_cart_der, and soap_contract.
+//
+// Every formula keeps the reference's operation order and this file is built
+// without fused multiply-adds (the pragma below), so that the results round
+// as the reference's x86-64 build does; what remains are the last-bit
+// differences of the device's exp/erf/sin/cos and of parallel reductions.
+//
+// Device layouts are the reference's (column-major, pair- or site-major):
+//   rad[p*n_max + n], ang[p*k_max + k] (double2), cnk[(i*n_max + n)*k_max + k],
+//   soap[i*n_soap + s], cart[(p*n_soap + s)*3 + c], G[s*n_sites + i], pair[p*3 + c].
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace st {
+
+__device__ __forceinline__ int site_of(const int* site_ptr, int n_sites, int p) {
+  int lo = 0, hi = n_sites;  // largest i with site_ptr[i] <= p
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (site_ptr[mid] <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------- set-up
+// Packs mask(n_pairs, n_species) into one bit per species and marks the
+// first pair of every site (the site itself, src/soap_turbo.f90:849-852).
+__global__ void pack_pairs_k(int n_pairs, int n_species, const int32_t* __restrict__ mask, const int* __restrict__ site_ptr,
+                             int n_sites, uint32_t* __restrict__ spmask, uint8_t* __restrict__ first) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  uint32_t b = 0;
+  for (int s = 0; s < n_species; ++s)
+    if (mask[(size_t)s * n_pairs + p] != 0) b |= 1u << s;
+  spmask[p] = b;
+  int i = site_of(site_ptr, n_sites, p);
+  first[p] = site_ptr[i] == p;
+}
+
+__global__ void fill_k(double* __restrict__ x, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) x[i] = v;
+}
+
+// ---------------------------------------------------------------- radial
+// One thread per pair: process_block of src/soap_turbo_radial.f90:550-1002
+// for one species block, then global_scaling (src/soap_turbo.f90:510-511).
+template <int A>
+__global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, const double* __restrict__ rjs,
+                                                const uint32_t* __restrict__ spmask, const uint8_t* __restrict__ first,
+                                                double* __restrict__ rad, double* __restrict__ rad_der) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const bool is_central = first[p] != 0;
+  if (is_central && !c.do_central) return;
+  const double rjs_p = rjs[p];
+  if (rjs_p >= c.rcut_hard_in) return;
+  if (!(spmask[p] & c.species_bit)) return;
+
+  const double pi = 3.141592653589793, sq2 = 1.4142135623730951;
+  const double isq2 = 1.0 / sq2, sqpi2 = sqrt(pi / 2.0), sq2pi = sqrt(2.0 / pi), sq8pi = sqrt(8.0 / pi);
+  const double sc = c.scaling, rs = c.rcut_soft;
+  double T1[A + 6], T2[A + 6], pol[A + 3], DT[A + 1];  // T(n) at n+3, pol(n)/DT(n) at n
+#pragma unroll
+  for (int n = 0; n < A + 6; ++n) T2[n] = 0.0;
+#pragma unroll
+  for (int n = 0; n <= A; ++n) DT[n] = 0.0;
+
+  const double rj = rjs_p * c.inv_rcut;
+  const double sig = c.atom_sigma + sc * rj, s2 = sig * sig, isig = 1.0 / sig;
+  const double is2 = isig * isig, is3 = is2 * isig;
+  double amp, ampd;
+  if (c.amp_mode == 0) {
+    amp = 1.0; ampd = 0.0;
+  } else if (c.amp_mode == 1) {
+    amp = isig; ampd = -sc * is2;
+  } else if (c.amp_mode == 2) {
+    double h = 1.0 + 2.0 * (rj * rj * rj) - 3.0 * (rj * rj);
+    double hp = 6.0 * (rj * rj) - 6.0 * rj;
+    double g = isig * h;
+    amp = h <= 1e-10 ? 0.0 : g;
+    ampd = h <= 1e-10 ? 0.0 : hp * isig - sc * isig * g;
+  } else {
+    double h = 1.0 + 2.0 * (rj * rj * rj) - 3.0 * (rj * rj);
+    if (h <= 1e-10) {
+      amp = 0.0; ampd = 0.0;
+    } else {
+      double hp = 6.0 * (rj * rj) - 6.0 * rj, aa = c.amp_scaling;
+      amp = isig * pow(h, aa);
+      ampd = aa * pow(h, aa - 1.0) * hp * isig - sc * isig * amp;
+    }
+  }
+  if (c.do_central && c.central_weight != 1.0 && is_central) {
+    amp = c.central_weight * amp;
+    ampd = c.central_weight * ampd;
+  }
+  if (c.radial_enh == 1) {
+    double g = rj + sq2pi * sig, f = 1.0 + sq2pi * sc;
+    ampd = amp * f + ampd * g;
+    amp = amp * g;
+  } else if (c.radial_enh == 2) {
+    double g = (rj * rj) + s2 + sq8pi * sig * rj;
+    double f = 2.0 * rj + 2.0 * sig * sc + sq8pi * sig + sq8pi * rj * sc;
+    ampd = amp * f + ampd * g;
+    amp = amp * g;
+  }
+  // overlap on [0, rcut_soft]
+  const double es = exp(-0.5 * ((rs - rj) * (rs - rj)) * is2);
+  const double e0 = exp(-0.5 * (rj * rj) * is2);
+  double C1 = s2 * es;
+  const double C2 = s2 * e0;
+  T1[0] = 0.0;
+  T1[1] = sqpi2 * sig * (erf((rs - rj) * isq2 * isig) + erf(rj * isq2 * isig)) * c.inv_Nm2;
+#pragma unroll
+  for (int n = -1; n <= A + 2; ++n) {
+    if (n > c.n_hi) break;
+    if (n >= 0) C1 = C1 * c.dr;
+    T1[n + 3] = s2 * c.ra[n + 1] * T1[n + 1] - c.rb[n + 1] * (rj - 1.0) * T1[n + 2] + (C1 - C2) * c.rc[n + 1];
+  }
+  const bool do_poly = c.n_poly >= 1;
+  if (c.do_der && do_poly && !c.unified) {
+    double f = rj - 1.0, g = sc * f * isig;
+    double ca = f * is2 * (g - 1.0), cb = is2 * (2.0 * g - 1.0), cc = sc * is3;
+#pragma unroll
+    for (int n = 1; n <= A; ++n)
+      if (n <= c.n_poly) DT[n] = ca * T1[n + 3] + c.R1[n] * cb * T1[n + 4] + c.R2[n] * cc * T1[n + 5];
+  }
+  double pref = 0.0;
+  if (c.has_filter) {
+    double den = s2 + c.q, iD = 1.0 / den, sqiD = sqrt(iD);
+    double sgf = sig * c.drnf * sqiD;
+    double rjf = (s2 * rs + c.q * rj) * iD;
+    double sf2 = sgf * sgf;
+    bool keep = (rs - rj) < 4.0 * sig;
+    double isgf = 1.0 / sgf, isf2 = isgf * isgf;
+    pref = keep ? exp(-0.5 * ((rs - rj) * (rs - rj)) * iD) : 0.0;
+    double C2f = sf2 * exp(-0.5 * ((rs - rjf) * (rs - rjf)) * isf2);
+    T2[0] = 0.0;
+    T2[1] = sqpi2 * sgf * (erf((1.0 - rjf) * isq2 * isgf) - erf((rs - rjf) * isq2 * isgf)) * c.inv_Nm2;
+#pragma unroll
+    for (int n = -1; n <= A + 2; ++n) {
+      if (n > c.n_hi) break;
+      if (n >= 0) C2f = C2f * c.dr;
+      T2[n + 3] = sf2 * c.ra[n + 1] * T2[n + 1] - c.rb[n + 1] * (rjf - 1.0) * T2[n + 2] - C2f * c.rc[n + 1];
+      if (n == -1 && c.unified) T2[2] = T2[2] + sf2 * exp(-0.5 * ((1.0 - rjf) * (1.0 - rjf)) * isf2) * c.rc[0];
+    }
+    if (c.do_der && do_poly && !c.unified) {
+      double f = rs - rj;
+      double dpref = pref * (f * iD + f * f * iD * iD * sig * sc);
+      double drjf = (2.0 * sig * rs * sc + c.q) * iD - (s2 * rs + c.q * rj) * 2.0 * sig * sc * iD * iD;
+      double dsgf = sc * c.drnf * sqiD * (1.0 - s2 * iD);
+      double ff = rjf - 1.0, g = dsgf * ff * isgf;
+      double ca = pref * ff * isf2 * (g - drjf);
+      double cb = pref * isf2 * (2.0 * g - drjf);
+      double cc = pref * dsgf * isgf * isf2;
+#pragma unroll
+      for (int n = 1; n <= A; ++n)
+        if (n <= c.n_poly)
+          DT[n] = DT[n] + ca * T2[n + 3] + c.R1[n] * cb * T2[n + 4] + c.R2[n] * cc * T2[n + 5] + dpref * T2[n + 3];
+    }
+  }
+#pragma unroll
+  for (int n = 1; n <= A + 2; ++n)
+    if (n <= c.n_hi) pol[n] = T1[n + 3] + pref * T2[n + 3];
+  if (c.do_der && do_poly && c.unified) {
+    double f = 1.0 - rj;
+    double g0 = f * is2 + sc * f * f * is3;
+    double g1 = -(is2 + 2.0 * sc * f * is3);
+    double g2 = sc * is3;
+#pragma unroll
+    for (int n = 1; n <= A; ++n)
+      if (n <= c.n_poly) DT[n] = g0 * pol[n] + c.R1[n] * g1 * pol[n + 1] + c.R2[n] * g2 * pol[n + 2];
+  }
+  if (c.with_gauss) {
+    double ss2 = c.sigma_sq0 + s2, iss = 1.0 / ss2, siss = sqrt(iss);
+    bool gm = rj < 4.0 * (c.atom_sigma + sig);
+    double ge = gm ? c.N_gauss * e0 : 0.0;
+    double M0 = gm ? exp(-0.5 * (rj * rj) * iss) * sqpi2 * sig * c.atom_sigma * siss *
+                         (1.0 + erf(c.atom_sigma * isig * rj * isq2 * siss)) * c.N_gauss
+                   : 0.0;
+    pol[A] = M0;
+    if (c.do_der) {
+      double sc0 = c.sigma_sq0 * s2 * iss, pp = s2 * rj * iss;
+      double M1 = sc0 * ge - pp * M0;
+      double M2 = sc0 * ge * (-rj) - pp * M1 + sc0 * M0;
+      DT[A] = M1 * is2 + sc * M2 * is3;
+    }
+  }
+  const double gs = c.global_scaling;
+  double* out = rad + (size_t)p * c.n_max + c.off;
+  if (c.do_der) {
+    double dtot[A + 1];
+#pragma unroll
+    for (int n = 1; n <= A; ++n) dtot[n] = amp * DT[n] + ampd * pol[n];
+    double* outd = rad_der + (size_t)p * c.n_max + c.off;
+#pragma unroll
+    for (int n = 1; n <= A; ++n) {
+      double acc = c.W[n - 1] * dtot[1];
+#pragma unroll
+      for (int m = 2; m <= A; ++m) acc = acc + c.W[(n - 1) + (m - 1) * A] * dtot[m];
+      outd[n - 1] = acc * c.inv_sqrt_rcut * gs;
+    }
+  }
+#pragma unroll
+  for (int n = 1; n <= A; ++n) {
+    double acc = c.W[n - 1] * pol[1];
+#pragma unroll
+    for (int m = 2; m <= A; ++m) acc = acc + c.W[(n - 1) + (m - 1) * A] * pol[m];
+    out[n - 1] = amp * acc * c.sqrt_rcut * gs;
+  }
+}
+
+// ---------------------------------------------------------------- angular
+// One thread per pair: get_angular_expansion_coefficients,
+// src/soap_turbo_angular.f90:548-605, with get_eimphi_conjg (:267-329),
+// get_ilexp (:361-422), get_ilexp_der (:436-463), get_plm_array (:108-160)
+// and get_plm_array_der (:181-248).
+template <int L>
+__global__ __launch_bounds__(128) void angular_k(AngularConsts c, int n_pairs, const double* __restrict__ rjs,
+                                                 const double* __restrict__ thetas, const double* __restrict__ phis,
+                                                 const uint32_t* __restrict__ spmask, const double* __restrict__ preflm,
+                                                 double2* __restrict__ ang, double2* __restrict__ ang_r,
+                                                 double2* __restrict__ ang_a, double2* __restrict__ ang_p) {
+  constexpr int KMAX = 1 + L * (L + 1) / 2 + L;
+  constexpr int KMAX1 = 1 + (L + 1) * (L + 2) / 2 + (L + 1);
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const double rj = rjs[p];
+  const uint32_t msk = spmask[p];
+  double2* A = ang + (size_t)p * KMAX;
+  if (!(rj < c.rcut_max) || msk == 0) {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) A[k] = make_double2(0.0, 0.0);
+    if (c.do_der) {
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) {
+        ang_r[(size_t)p * KMAX + k] = make_double2(0.0, 0.0);
+        ang_a[(size_t)p * KMAX + k] = make_double2(0.0, 0.0);
+        ang_p[(size_t)p * KMAX + k] = make_double2(0.0, 0.0);
+      }
+    }
+    return;
+  }
+  const int i_sp = __ffs(msk) - 1;
+  const double phi = phis[p], x = cos(thetas[p]);
+  const double sigma = c.sigma_t[i_sp] + c.sigma_t_scaling[i_sp] * rj;
+  const double amplitude = c.rcut2 / (sigma * sigma);
+  // ilexp(rj / sigma)
+  double il[L + 2];
+  {
+    const double xx = rj / sigma, xcut = 1e-7;
+    double x2 = xx * xx, x4 = x2 * x2;
+    double inv_x2 = x2 > 0.0 ? 1.0 / x2 : 0.0;
+    double x2pow = x2 * x2;
+    double flm2, flm1;
+    if (xx > 0.0) {
+      double e = exp(-2.0 * x2);
+      flm2 = fabs((1.0 - e) / 2.0 / x2);
+      flm1 = fabs((x2 - 1.0 + e * (x2 + 1.0)) / 2.0 / x4);
+    } else {
+      flm2 = 1.0;
+      flm1 = 0.0;
+    }
+    double fact = 1.0;
+    il[0] = xx < xcut ? 1.0 - x2 : flm2;
+    if (L >= 1) {
+      fact = fact * 3.0;
+      il[1] = x2 / 1000.0 < xcut ? (x2 - x4) / fact : flm1;
+    }
+#pragma unroll
+    for (int l = 2; l <= L; ++l) {
+      fact = fact * (2.0 * (double)l + 1.0);
+      double ratio = x2pow / fact, fl;
+      if (ratio * (double)l < xcut)
+        fl = ratio;
+      else
+        fl = fabs(flm2 - (2.0 * (double)l - 1.0) * inv_x2 * flm1);
+      x2pow = x2pow * x2;
+      flm2 = flm1;
+      flm1 = fl;
+      il[l] = fl;
+    }
+  }
+  // e^{-i m phi} by the Chebyshev recursion
+  double pm_re[L + 1], pm_im[L + 1];
+  {
+    double cosm2 = cos(phi), cosphi2 = 2.0 * cosm2, sinm2 = -sin(phi), cosm1 = 1.0, sinm1 = 0.0;
+    pm_re[0] = 1.0; pm_im[0] = 0.0;
+#pragma unroll
+    for (int l = 1; l <= L; ++l) {
+      double cos0 = cosphi2 * cosm1 - cosm2, sin0 = cosphi2 * sinm1 - sinm2;
+      cosm2 = cosm1; sinm2 = sinm1; cosm1 = cos0; sinm1 = sin0;
+      pm_re[l] = cos0; pm_im[l] = -sin0;
+    }
+  }
+  // P_l^m(x) to l = L (+1 with derivatives)
+  double plm[KMAX1];
+  {
+    const int LL = c.do_der ? L + 1 : L;
+    double x2 = x * x, sq = sqrt(1.0 - x2);
+    plm[0] = 1.0;
+    if (LL >= 1) { plm[1] = x; plm[2] = -sq; }
+    if (L >= 1 && LL >= 2) { plm[3] = 1.5 * x2 - 0.5; plm[4] = -3.0 * x * sq; plm[5] = 3.0 - 3.0 * x2; }
+#pragma unroll
+    for (int l = 3; l <= L + 1; ++l) {
+      if (l > LL) break;
+#pragma unroll
+      for (int m = 0; m <= l - 2; ++m) {
+        int k = 1 + l * (l + 1) / 2 + m;
+        double a = (double)(2 * l - 1) / (double)(l - m), b = (double)(l - 1 + m) / (double)(l - m);
+        plm[k - 1] = a * x * plm[k - l - 1] - b * plm[k - 2 * l];
+      }
+      int k = 1 + l * (l + 1) / 2 + l - 1;
+      plm[k - 1] = x * (double)(2 * l - 1) * plm[k - l - 1];
+      k = k + 1;
+      plm[k - 1] = -((double)(2 * l - 1) * sq * plm[k - l - 2]);
+    }
+  }
+  if (c.do_der) {
+    const double scal = c.sigma_t_scaling[i_sp];
+    double ild[L + 1];
+    {
+      double coeff1 = 2.0 * rj / (sigma * sigma);
+      double coeff2 = 1.0 - scal * rj / sigma;
+#pragma unroll
+      for (int l = 0; l <= L; ++l) ild[l] = 0.0;
+      if (!(rj < 1e-5)) {
+        // il[L+1] is read only for L = 0, where the reference reads past its array
+        ild[0] = coeff1 * ((L >= 1 ? il[1] : 0.0) - il[0]);
+#pragma unroll
+        for (int l = 1; l <= L; ++l) ild[l] = (-coeff1 - (double)(2 * l + 2) / rj) * il[l] + coeff1 * il[l - 1];
+#pragma unroll
+        for (int l = 0; l <= L; ++l) ild[l] = ild[l] * coeff2;
+      }
+    }
+    const double sig_fac = 2.0 * scal / sigma;
+    double2* Ar = ang_r + (size_t)p * KMAX;
+    double2* Aa = ang_a + (size_t)p * KMAX;
+    double2* Ap = ang_p + (size_t)p * KMAX;
+#pragma unroll
+    for (int l = 0; l <= L; ++l) {
+#pragma unroll
+      for (int m = 0; m <= l; ++m) {
+        const int kk = l * (l + 1) / 2 + m;
+        const int k = kk + 1;
+        // sin(t) dP/dcos(t) and -m P/sin(t)
+        double part1, part2;
+        if (m == 0) part1 = l == 0 ? 0.0 : -0.5 * plm[l * (l + 1) / 2 + 1];
+        else part1 = (0.5 * (double)(l + m) * (double)(l - m + 1)) * plm[k - 2];
+        part2 = m == l ? 0.0 : -0.5 * plm[k];
+        const double dms = part1 + part2;
+        double dsin = 0.0;
+        if (m >= 1) {
+          const int kp = 1 + (l + 1) * (l + 2) / 2 + m + 1, km = 1 + (l + 1) * (l + 2) / 2 + m - 1;
+          dsin = (0.5 * (double)(l - m + 1) * (double)(l - m + 2)) * plm[km - 1] + 0.5 * plm[kp - 1];
+        }
+        const double e_re = il[l] * pm_re[m], e_im = il[l] * pm_im[m];
+        const double er_re = ild[l] * pm_re[m], er_im = ild[l] * pm_im[m];
+        const double u = amplitude * preflm[kk], v = u * plm[kk];
+        const double ec_re = v * e_re, ec_im = v * e_im;
+        A[kk] = make_double2(ec_re, ec_im);
+        Ar[kk] = make_double2(v * er_re - sig_fac * ec_re, v * er_im - sig_fac * ec_im);
+        const double t = u * dsin;
+        Aa[kk] = make_double2(-(t * e_im), t * e_re);
+        const double t2 = u * dms;
+        Ap[kk] = make_double2(t2 * e_re, t2 * e_im);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l <= L; ++l) {
+#pragma unroll
+      for (int m = 0; m <= l; ++m) {
+        const int kk = l * (l + 1) / 2 + m;
+        const double e_re = il[l] * pm_re[m], e_im = il[l] * pm_im[m];
+        const double t = amplitude * preflm[kk] * plm[kk];
+        A[kk] = make_double2(t * e_re, t * e_im);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- cnk
+// One workgroup per site: cnk(k,n,i) = sum_j 4 pi R(n,j) A(k,j) in
+// neighbour order, then the poly3gauss central-atom term
+// (src/soap_turbo.f90:626-664). central[s*n_max + n] holds that term per
+// species (zero where the species has none).
+__global__ __launch_bounds__(256) void cnk_k(int n_sites, int n_max, int k_max, const int* __restrict__ site_ptr,
+                                             const double* __restrict__ rad, const double2* __restrict__ ang,
+                                             const int32_t* __restrict__ site_species, int max_mult,
+                                             const int32_t* __restrict__ species_mult,
+                                             const double* __restrict__ central, double2* __restrict__ cnk) {
+  const int i = blockIdx.x;
+  const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  const double fourpi = 4.0 * 3.141592653589793;
+  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
+    const int n = t / k_max, k = t - n * k_max;
+    double re = 0.0, im = 0.0;
+    for (int p = p0; p < p1; ++p) {
+      const double fr = fourpi * rad[(size_t)p * n_max + n];
+      const double2 a = ang[(size_t)p * k_max + k];
+      re = re + fr * a.x;
+      im = im + fr * a.y;
+    }
+    if (k == 0 && central) {
+      for (int m = 0; m < species_mult[i]; ++m) {
+        const int s = site_species[(size_t)i * max_mult + m] - 1;
+        re = re + central[(size_t)s * n_max + n];
+      }
+    }
+    cnk[((size_t)i * n_max + n) * k_max + k] = make_double2(re, im);
+  }
+}
+
+// ---------------------------------------------------------------- power spectrum
+// Component table of the uncompressed spectrum: comp[c] = (n, np, l, 0) for
+// c in the reference's (n, np >= n, l) order. The compressed output s is
+// sum over e in [out_ptr[s], out_ptr[s+1]) of P_el[e] * value(P_j[e]) in the
+// reference's order; without compression out_ptr is null and s = c.
+__device__ __forceinline__ double spectrum_value(const double2* cn, int k_max, int4 cmp) {
+  const int n = cmp.x, np = cmp.y, l = cmp.z;
+  const int kb = l * (l + 1) / 2;
+  const double base = n != np ? 1.4142135623730951 : 1.0;
+  double acc = 0.0;
+  for (int m = 0; m <= l; ++m) {
+    const double mult = m > 0 ? base * 2.0 : base;
+    const double2 u = cn[n * k_max + kb + m], v = cn[np * k_max + kb + m];
+    acc = acc + mult * (u.x * v.x + u.y * v.y);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int j = 0; j < nw; ++j) s += red[j];
+  return s;
+}
+
+// Unnormalised soap and sqrt_dot_p (src/soap_turbo.f90:750-813).
+__global__ __launch_bounds__(256) void power_k(int n_sites, int n_max, int k_max, int n_soap,
+                                               const int4* __restrict__ comp, const int* __restrict__ out_ptr,
+                                               const int* __restrict__ out_c, const double* __restrict__ out_el,
+                                               const double2* __restrict__ cnk, double* __restrict__ soap,
+                                               double* __restrict__ sqrt_dot_p) {
+  extern __shared__ double2 smem2[];
+  __shared__ double red[16];
+  const int i = blockIdx.x;
+  double2* cn = smem2;
+  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) cn[t] = cnk[(size_t)i * n_max * k_max + t];
+  __syncthreads();
+  double ss = 0.0;
+  for (int s = threadIdx.x; s < n_soap; s += blockDim.x) {
+    double v;
+    if (out_ptr) {
+      v = 0.0;
+      for (int e = out_ptr[s]; e < out_ptr[s + 1]; ++e) v = v + out_el[e] * spectrum_value(cn, k_max, comp[out_c[e]]);
+    } else {
+      v = spectrum_value(cn, k_max, comp[s]);
+    }
+    soap[(size_t)i * n_soap + s] = v;
+    ss += v * v;
+  }
+  double tot = block_sum(ss, red);
+  if (threadIdx.x == 0) {
+    double d = sqrt(tot);
+    if (d < 1e-5) d = 1.0;
+    sqrt_dot_p[i] = d;
+  }
+}
+
+__global__ void normalise_k(int n_sites, int n_soap, const double* __restrict__ sqrt_dot_p, double* __restrict__ soap) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n_sites * n_soap) return;
+  soap[t] = soap[t] / sqrt_dot_p[t / n_soap];
+}
+
+// ---------------------------------------------------------------- full Jacobian
+// One workgroup per site, neighbours in order: the four contractions over m
+// (soap_derivatives_from_expansion, src/soap_turbo.f90:1996-2025), the
+// assembly of the surviving components (:2027-2066), then normalisation and
+// rotation to Cartesian with the self column accumulated as minus the sum of
+// the others (:837-895). soap holds the unnormalised spectrum here.
+template <int DER_NSPT>
+__global__ __launch_bounds__(256) void derivatives_k(
+    int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* __restrict__ site_ptr,
+    const int4* __restrict__ comp, const int* __restrict__ out_ptr, const int* __restrict__ out_c,
+    const double* __restrict__ out_el, const double2* __restrict__ cnk, const double* __restrict__ rad,
+    const double* __restrict__ rad_der, const double2* __restrict__ ang, const double2* __restrict__ ang_r,
+    const double2* __restrict__ ang_a, const double2* __restrict__ ang_p, const double* __restrict__ soap,
+    const double* __restrict__ sqrt_dot_p, const double* __restrict__ rjs, const double* __restrict__ thetas,
+    const double* __restrict__ phis, double* __restrict__ cart) {
+  extern __shared__ double2 smem2[];
+  __shared__ double red[16];
+  const int i = blockIdx.x;
+  const int nl = n_max * (l_max + 1);
+  double2* cn = smem2;                          // n_max * k_max
+  double2* sA = cn + n_max * k_max;             // 4 * k_max
+  double* sR = (double*)(sA + 4 * k_max);       // 2 * n_max
+  double* PQ = sR + 2 * n_max;                  // 4 * n_max * (l_max + 1), [kind][l][n]
+  const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) cn[t] = cnk[(size_t)i * n_max * k_max + t];
+  const double sdp = sqrt_dot_p[i];
+  const double inv_p = 1.0 / sdp, inv_p3 = 1.0 / (sdp * sdp * sdp);
+  const double fourpi = 4.0 * 3.141592653589793, sq2 = 1.4142135623730951;
+  double selfacc[DER_NSPT][3];
+  double qv[DER_NSPT];
+#pragma unroll
+  for (int u = 0; u < DER_NSPT; ++u) {
+    selfacc[u][0] = selfacc[u][1] = selfacc[u][2] = 0.0;
+    const int s = threadIdx.x + u * blockDim.x;
+    qv[u] = s < n_soap ? soap[(size_t)i * n_soap + s] : 0.0;
+  }
+  for (int p = p0 + 1; p < p1; ++p) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < 4 * k_max; t += blockDim.x) {
+      const int kind = t / k_max, k = t - kind * k_max;
+      const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
+      sA[t] = src[(size_t)p * k_max + k];
+    }
+    for (int t = threadIdx.x; t < 2 * n_max; t += blockDim.x)
+      sR[t] = t < n_max ? rad[(size_t)p * n_max + t] : rad_der[(size_t)p * n_max + t - n_max];
+    __syncthreads();
+    for (int t = threadIdx.x; t < 4 * nl; t += blockDim.x) {
+      const int kind = t / nl, r = t - kind * nl, l = r / n_max, n = r - l * n_max;
+      const int kb = l * (l + 1) / 2;
+      const double2* a = sA + kind * k_max;
+      double acc = 0.0;
+      for (int m = 0; m <= l; ++m) {
+        const double w = m == 0 ? 1.0 : 2.0;
+        const double2 cv = cn[n * k_max + kb + m], av = a[kb + m];
+        acc = acc + w * (av.x * cv.x + av.y * cv.y);
+      }
+      PQ[t] = acc;
+    }
+    __syncthreads();
+    const double* P = PQ;
+    const double* Q = PQ + nl;
+    const double* Qa = PQ + 2 * nl;
+    const double* Qo = PQ + 3 * nl;
+    const double* r = sR;
+    const double* rd = sR + n_max;
+    double dv[DER_NSPT][3];
+    double dr_ = 0.0, da_ = 0.0, do_ = 0.0;
+#pragma unroll
+    for (int u = 0; u < DER_NSPT; ++u) {
+      const int s = threadIdx.x + u * blockDim.x;
+      double vr = 0.0, va = 0.0, vo = 0.0;
+      if (s < n_soap) {
+        const int e0 = out_ptr ? out_ptr[s] : s, e1 = out_ptr ? out_ptr[s + 1] : s + 1;
+        for (int e = e0; e < e1; ++e) {
+          const int4 cm = comp[out_ptr ? out_c[e] : e];
+          const int n = cm.x, np = cm.y, l = cm.z;
+          const double pre = fourpi * (n == np ? 1.0 : sq2);
+          const double tr = pre * (rd[n] * P[l * n_max + np] + r[n] * Q[l * n_max + np] + rd[np] * P[l * n_max + n] +
+                                   r[np] * Q[l * n_max + n]);
+          const double ta = pre * (r[n] * Qa[l * n_max + np] + r[np] * Qa[l * n_max + n]);
+          const double to = pre * (r[n] * Qo[l * n_max + np] + r[np] * Qo[l * n_max + n]);
+          if (out_ptr) {
+            const double el = out_el[e];
+            vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
+          } else {
+            vr = tr; va = ta; vo = to;
+          }
+        }
+        dr_ += qv[u] * vr; da_ += qv[u] * va; do_ += qv[u] * vo;
+      }
+      dv[u][0] = vr; dv[u][1] = va; dv[u][2] = vo;
+    }
+    const double dot_r = block_sum(dr_, red);
+    const double dot_a = block_sum(da_, red + 4);
+    const double dot_o = block_sum(do_, red + 8);
+    const double sin_t = sin(thetas[p]), cos_t = cos(thetas[p]);
+    const double sin_f = sin(phis[p]), cos_f = cos(phis[p]);
+    const double inv_r = 1.0 / rjs[p];
+    const double cx = sin_t * cos_f, cy = sin_t * sin_f, cz = cos_t;
+    const double px = -(cos_t * cos_f * inv_r), py = -(cos_t * sin_f * inv_r), pz = sin_t * inv_r;
+    const double ax = -(sin_f * inv_r), ay = cos_f * inv_r;
+#pragma unroll
+    for (int u = 0; u < DER_NSPT; ++u) {
+      const int s = threadIdx.x + u * blockDim.x;
+      if (s < n_soap) {
+        const double sv = qv[u];
+        const double vr = dv[u][0] * inv_p - sv * inv_p3 * dot_r;
+        const double va = dv[u][1] * inv_p - sv * inv_p3 * dot_a;
+        const double vo = dv[u][2] * inv_p - sv * inv_p3 * dot_o;
+        const double c0 = cx * vr + px * vo + ax * va;
+        const double c1 = cy * vr + py * vo + ay * va;
+        const double c2 = cz * vr + pz * vo;
+        double* C = cart + ((size_t)p * n_soap + s) * 3;
+        C[0] = c0; C[1] = c1; C[2] = c2;
+        selfacc[u][0] = selfacc[u][0] - c0;
+        selfacc[u][1] = selfacc[u][1] - c1;
+        selfacc[u][2] = selfacc[u][2] - c2;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < DER_NSPT; ++u) {
+    const int s = threadIdx.x + u * blockDim.x;
+    if (s < n_soap) {
+      double* C = cart + ((size_t)p0 * n_soap + s) * 3;
+      C[0] = selfacc[u][0]; C[1] = selfacc[u][1]; C[2] = selfacc[u][2];
+    }
+  }
+}
+
+// pair(:,k) = sum_s G(i,s) soap_cart_der(:,s,k): what a caller of the
+// full-Jacobian get_soap does to turn dE/dq into pair gradients. One
+// wavefront per pair.
+__global__ __launch_bounds__(256) void contract_cart_k(int n_sites, int n_pairs, int n_soap,
+                                                       const int* __restrict__ site_ptr,
+                                                       const double* __restrict__ G, const double* __restrict__ cart,
+                                                       double* __restrict__ pair) {
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= n_pairs) return;
+  const int i = site_of(site_ptr, n_sites, p);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  const double* C = cart + (size_t)p * n_soap * 3;
+  for (int s = lane; s < n_soap; s += 64) {
+    const double g = G[(size_t)s * n_sites + i];
+    a0 += g * C[3 * s]; a1 += g * C[3 * s + 1]; a2 += g * C[3 * s + 2];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, 64); a1 += __shfl_xor(a1, o, 64); a2 += __shfl_xor(a2, o, 64);
+  }
+  if (lane == 0) { pair[3 * (size_t)p] = a0; pair[3 * (size_t)p + 1] = a1; pair[3 * (size_t)p + 2] = a2; }
+}
+
+// ---------------------------------------------------------------- soap_contract
+// One workgroup per site (src/soap_turbo.f90:1780-1877): back through the
+// normalisation and compression, H(k,n) = w(m) sum_np Y(n,np,l) cnk(k,np),
+// then one thread per neighbour pair against H. unc_ptr/unc_e list, per
+// uncompressed component, the P entries that feed it, in P order.
+__global__ __launch_bounds__(256) void contract_k(
+    int n_sites, int n_max, int l_max, int k_max, int n_soap, int n_soap_unc, const int* __restrict__ site_ptr,
+    const uint8_t* __restrict__ skip, const int* __restrict__ unc_ptr, const int* __restrict__ unc_e,
+    const int32_t* __restrict__ P_i, const double* __restrict__ P_el, const double2* __restrict__ cnk,
+    const double* __restrict__ rad, const double* __restrict__ rad_der, const double2* __restrict__ ang,
+    const double2* __restrict__ ang_r, const double2* __restrict__ ang_a, const double2* __restrict__ ang_p,
+    const double* __restrict__ soap, const double* __restrict__ G, double sgn, const double* __restrict__ sqrt_dot_p,
+    const double* __restrict__ rjs, const double* __restrict__ thetas, const double* __restrict__ phis,
+    double* __restrict__ pair) {
+  extern __shared__ double2 smem2[];
+  __shared__ double red[16];
+  const int i = blockIdx.x;
+  double2* cn = smem2;                       // n_max * k_max
+  double2* H = cn + n_max * k_max;           // n_max * k_max
+  double* Gp = (double*)(H + n_max * k_max); // n_soap
+  double* Gu = Gp + n_soap;                  // n_soap_unc
+  const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) cn[t] = cnk[(size_t)i * n_max * k_max + t];
+  double qg_part = 0.0;
+  for (int s = threadIdx.x; s < n_soap; s += blockDim.x) qg_part += soap[(size_t)i * n_soap + s] * G[(size_t)s * n_sites + i];
+  const double qg = block_sum(qg_part, red);
+  const double inv = 1.0 / sqrt_dot_p[i];
+  for (int s = threadIdx.x; s < n_soap; s += blockDim.x)
+    Gp[s] = (G[(size_t)s * n_sites + i] - soap[(size_t)i * n_soap + s] * qg) * inv;
+  __syncthreads();
+  for (int c = threadIdx.x; c < n_soap_unc; c += blockDim.x) {
+    double v;
+    if (unc_ptr) {
+      v = 0.0;
+      for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v = v + P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
+    } else {
+      v = Gp[c];
+    }
+    Gu[c] = v;
+  }
+  __syncthreads();
+  const int L1 = l_max + 1;
+  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
+    const int n = t / k_max, k = t - n * k_max;
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= k) ++l;
+    const int m = k - l * (l + 1) / 2;
+    double hr = 0.0, hi = 0.0;
+    for (int np = 0; np < n_max; ++np) {
+      const int a = n < np ? n : np, b = n < np ? np : n;
+      const int c = (a * n_max - a * (a - 1) / 2 + (b - a)) * L1 + l;
+      double x = 0.0;
+      if (!skip[c]) x = Gu[c] * (a == b ? 1.0 : 1.4142135623730951);
+      const double y = a == b ? 2.0 * x : x;
+      const double2 cv = cn[np * k_max + k];
+      hr = hr + y * cv.x;
+      hi = hi + y * cv.y;
+    }
+    const double w = m == 0 ? 1.0 : 2.0;
+    H[n * k_max + k] = make_double2(w * hr, w * hi);
+  }
+  __syncthreads();
+  const double fourpi = 4.0 * 3.141592653589793;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int p = p0 + 1 + threadIdx.x; p < p1; p += blockDim.x) {
+    const double2* A = ang + (size_t)p * k_max;
+    const double2* Ar = ang_r + (size_t)p * k_max;
+    const double2* Aa = ang_a + (size_t)p * k_max;
+    const double2* Ap = ang_p + (size_t)p * k_max;
+    double f_rad = 0.0, f_azi = 0.0, f_pol = 0.0;
+    for (int n = 0; n < n_max; ++n) {
+      double sd = 0.0, sr = 0.0, sa = 0.0, so = 0.0;
+      for (int k = 0; k < k_max; ++k) {
+        const double2 h = H[n * k_max + k];
+        const double2 a = A[k], ar = Ar[k], aa = Aa[k], ap = Ap[k];
+        sd = sd + a.x * h.x + a.y * h.y;
+        sr = sr + ar.x * h.x + ar.y * h.y;
+        sa = sa + aa.x * h.x + aa.y * h.y;
+        so = so + ap.x * h.x + ap.y * h.y;
+      }
+      f_rad = f_rad + rad_der[(size_t)p * n_max + n] * sd + rad[(size_t)p * n_max + n] * sr;
+      f_azi = f_azi + rad[(size_t)p * n_max + n] * sa;
+      f_pol = f_pol + rad[(size_t)p * n_max + n] * so;
+    }
+    f_rad = fourpi * f_rad; f_azi = fourpi * f_azi; f_pol = fourpi * f_pol;
+    const double sin_t = sin(thetas[p]), cos_t = cos(thetas[p]);
+    const double sin_f = sin(phis[p]), cos_f = cos(phis[p]);
+    const double inv_r = 1.0 / rjs[p];
+    const double x0 = sgn * (sin_t * cos_f * f_rad - cos_t * cos_f * inv_r * f_pol - sin_f * inv_r * f_azi);
+    const double x1 = sgn * (sin_t * sin_f * f_rad - cos_t * sin_f * inv_r * f_pol + cos_f * inv_r * f_azi);
+    const double x2 = sgn * (cos_t * f_rad + sin_t * inv_r * f_pol);
+    pair[3 * (size_t)p] = x0; pair[3 * (size_t)p + 1] = x1; pair[3 * (size_t)p + 2] = x2;
+    s0 -= x0; s1 -= x1; s2 -= x2;
+  }
+  s0 = block_sum(s0, red);
+  s1 = block_sum(s1, red + 4);
+  s2 = block_sum(s2, red + 8);
+  if (threadIdx.x == 0) {
+    pair[3 * (size_t)p0] = s0; pair[3 * (size_t)p0 + 1] = s1; pair[3 * (size_t)p0 + 2] = s2;
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+void launch_pack_pairs(hipStream_t st, int n_pairs, int n_species, const int32_t* mask, const int* site_ptr,
+                       int n_sites, uint32_t* spmask, uint8_t* first) {
+  if (n_pairs == 0) return;
+  pack_pairs_k<<<(n_pairs + 255) / 256, 256, 0, st>>>(n_pairs, n_species, mask, site_ptr, n_sites, spmask, first);
+}
+
+void launch_fill(hipStream_t st, double* x, size_t n, double v) {
+  if (n == 0) return;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  fill_k<<<(unsigned)blocks, 256, 0, st>>>(x, n, v);
+}
+
+template <int A>
+static void radial_one(hipStream_t st, const RadialConsts& c, int n_pairs, const double* rjs, const uint32_t* spmask,
+                       const uint8_t* first, double* rad, double* rad_der) {
+  radial_k<A><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, spmask, first, rad, rad_der);
+}
+
+bool launch_radial(hipStream_t st, const RadialConsts& c, int n_pairs, const double* rjs, const uint32_t* spmask,
+                   const uint8_t* first, double* rad, double* rad_der) {
+  if (n_pairs == 0) return true;
+  switch (c.alpha_max) {
+#define RC(a) case a: radial_one<a>(st, c, n_pairs, rjs, spmask, first, rad, rad_der); return true;
+    RC(1) RC(2) RC(3) RC(4) RC(5) RC(6) RC(7) RC(8) RC(9) RC(10) RC(11)
+#undef RC
+    default: return false;
+  }
+}
+
+template <int L>
+static void angular_one(hipStream_t st, const AngularConsts& c, int n_pairs, const double* rjs, const double* th,
+                        const double* ph, const uint32_t* spmask, const double* preflm, double2* a, double2* ar,
+                        double2* aa, double2* ap) {
+  angular_k<L><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, th, ph, spmask, preflm, a, ar, aa, ap);
+}
+
+bool launch_angular(hipStream_t st, const AngularConsts& c, int n_pairs, const double* rjs, const double* th,
+                    const double* ph, const uint32_t* spmask, const double* preflm, double2* a, double2* ar,
+                    double2* aa, double2* ap) {
+  if (n_pairs == 0) return true;
+  switch (c.l_max) {
+#define AC(l) case l: angular_one<l>(st, c, n_pairs, rjs, th, ph, spmask, preflm, a, ar, aa, ap); return true;
+    AC(0) AC(1) AC(2) AC(3) AC(4) AC(5) AC(6) AC(7) AC(8) AC(9) AC(10) AC(11) AC(12)
+#undef AC
+    default: return false;
+  }
+}
+
+void launch_cnk(hipStream_t st, int n_sites, int n_max, int k_max, const int* site_ptr, const double* rad,
+                const double2* ang, const int32_t* site_species, int max_mult, const int32_t* species_mult,
+                const double* central, double2* cnk) {
+  if (n_sites == 0) return;
+  cnk_k<<<n_sites, 256, 0, st>>>(n_sites, n_max, k_max, site_ptr, rad, ang, site_species, max_mult, species_mult,
+                                 central, cnk);
+}
+
+void launch_power(hipStream_t st, int n_sites, int n_max, int k_max, int n_soap, const int4* comp, const int* out_ptr,
+                  const int* out_c, const double* out_el, const double2* cnk, double* soap, double* sqrt_dot_p) {
+  if (n_sites == 0) return;
+  size_t sh = sizeof(double2) * n_max * k_max;
+  power_k<<<n_sites, 256, sh, st>>>(n_sites, n_max, k_max, n_soap, comp, out_ptr, out_c, out_el, cnk, soap, sqrt_dot_p);
+}
+
+void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqrt_dot_p, double* soap) {
+  size_t n = (size_t)n_sites * n_soap;
+  if (n == 0) return;
+  normalise_k<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n_sites, n_soap, sqrt_dot_p, soap);
+}
+
+size_t derivatives_lds(int n_max, int l_max, int k_max) {
+  return sizeof(double2) * (n_max * k_max + 4 * k_max) + sizeof(double) * (2 * n_max + 4 * n_max * (l_max + 1));
+}
+
+bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* site_ptr,
+                        const int4* comp, const int* out_ptr, const int* out_c, const double* out_el,
+                        const double2* cnk, const double* rad, const double* rad_der, const double2* ang,
+                        const double2* ang_r, const double2* ang_a, const double2* ang_p, const double* soap,
+                        const double* sqrt_dot_p, const double* rjs, const double* thetas, const double* phis,
+                        double* cart) {
+  if (n_sites == 0) return true;
+  const int need = (n_soap + 255) / 256;
+  size_t sh = derivatives_lds(n_max, l_max, k_max);
+#define DL(N)                                                                                                     \
+  if (need <= N) {                                                                                                \
+    derivatives_k<N><<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp, out_ptr,     \
+                                               out_c, out_el, cnk, rad, rad_der, ang, ang_r, ang_a, ang_p, soap,  \
+                                               sqrt_dot_p, rjs, thetas, phis, cart);                              \
+    return true;                                                                                                  \
+  }
+  DL(1) DL(2) DL(3) DL(4) DL(6) DL(8) DL(12)
+#undef DL
+  return false;
+}
+
+void launch_contract_cart(hipStream_t st, int n_sites, int n_pairs, int n_soap, const int* site_ptr, const double* G,
+                          const double* cart, double* pair) {
+  if (n_pairs == 0) return;
+  contract_cart_k<<<(n_pairs + 3) / 4, 256, 0, st>>>(n_sites, n_pairs, n_soap, site_ptr, G, cart, pair);
+}
+
+size_t contract_lds(int n_max, int k_max, int n_soap, int n_soap_unc) {
+  return sizeof(double2) * 2 * n_max * k_max + sizeof(double) * (n_soap + n_soap_unc);
+}
+
+void launch_contract(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, int n_soap_unc,
+                     const int* site_ptr, const uint8_t* skip, const int* unc_ptr, const int* unc_e,
+                     const int32_t* P_i, const double* P_el, const double2* cnk, const double* rad,
+                     const double* rad_der, const double2* ang, const double2* ang_r, const double2* ang_a,
+                     const double2* ang_p, const double* soap, const double* G, double sgn, const double* sqrt_dot_p,
+                     const double* rjs, const double* thetas, const double* phis, double* pair) {
+  if (n_sites == 0) return;
+  size_t sh = contract_lds(n_max, k_max, n_soap, n_soap_unc);
+  contract_k<<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, n_soap_unc, site_ptr, skip, unc_ptr, unc_e,
+                                       P_i, P_el, cnk, rad, rad_der, ang, ang_r, ang_a, ang_p, soap, G, sgn,
+                                       sqrt_dot_p, rjs, thetas, phis, pair);
+}
+
+}  // namespace st

@@ -1,0 +1,75 @@
+"""The C-ABI library: it loads, exports every symbol include/*.h declares, and its host-only logic."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import soap_turbo_amd as st
+from soap_turbo_amd import _lib
+from tests.golden_cases import CASES, load
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(st_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.SIGNATURES)
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if "cfg2" in c or "gauss_amp0" in c])
+def test_compress_indices_match_reference(name):
+    _, comp, d = load(name)
+    dim, nz = st.get_compress_indices("trivial", d["alpha_max"], int(d["l_max"]), "get_dim")
+    assert dim == nz == len(d["P_i"])
+    P_i = np.zeros(nz, dtype=np.int32); P_j = np.zeros(nz, dtype=np.int32); P_el = np.zeros(nz)
+    st.get_compress_indices("Trivial", d["alpha_max"], int(d["l_max"]), "set_indices", P_i, P_j, P_el)
+    np.testing.assert_array_equal(P_i, d["P_i"])
+    np.testing.assert_array_equal(P_j, d["P_j"])
+    np.testing.assert_array_equal(P_el, d["P_el"])
+
+
+def test_compress_indices_brute_force_small():
+    # trivial compression keeps (n, m >= n) when n or m is the first radial index of a species
+    for am, L in (([2, 3], 1), ([1, 1, 2], 2), ([4], 0)):
+        dim, nz = st.get_compress_indices("trivial", am, L, "get_dim")
+        P_i = np.zeros(nz, dtype=np.int32); P_j = np.zeros(nz, dtype=np.int32); P_el = np.zeros(nz)
+        st.get_compress_indices("trivial", am, L, "set_indices", P_i, P_j, P_el)
+        piv = set(np.cumsum([0] + am[:-1]) + 1)
+        n_max = sum(am)
+        expect = [k for k, (n, m, l) in enumerate(
+            (n, m, l) for n in range(1, n_max + 1) for m in range(n, n_max + 1) for l in range(L + 1))
+            if n in piv or m in piv]
+        np.testing.assert_array_equal(P_j - 1, expect)
+        np.testing.assert_array_equal(P_i, np.arange(1, nz + 1))
+
+
+def test_unknown_compress_mode_and_what_to_do_fail():
+    with pytest.raises(st.SoapTurboError):
+        st.get_compress_indices("0_0_darby", [2], 1, "get_dim")
+    with pytest.raises(st.SoapTurboStop):
+        st.get_compress_indices("trivial", [2], 1, "bogus")
+
+
+def test_n_soap():
+    keep = _lib.Keep()
+    p = st.make_params(keep, alpha_max=[8, 8, 8], l_max=8, rcut_hard=[5.0] * 3, rcut_soft=[4.5] * 3, nf=[4.0] * 3,
+                       global_scaling=[1.0] * 3, atom_sigma_r=[0.5] * 3, atom_sigma_r_scaling=[0.0] * 3,
+                       atom_sigma_t=[0.5] * 3, atom_sigma_t_scaling=[0.0] * 3, amplitude_scaling=[1.0] * 3,
+                       radial_enhancement=1, central_weight=[1.0] * 3, basis="poly3gauss", scaling_mode="polynomial")
+    assert _lib.lib().st_n_soap(ctypes.byref(p), None) == 24 * 25 // 2 * 9
