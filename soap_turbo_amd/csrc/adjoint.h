@@ -26,8 +26,9 @@ struct AdjointState {
   int n_seg = 0;
   // per-call work and outputs
   AdjBuf cnk, soap, sdp, H, pair, forces, virial_part, virial, G32, order, atom_ptr, atom_pairs, vecbuf;
-  AdjBuf bins;
-  std::string atoms_key;
+  AdjBuf bins, el32;
+  std::string atoms_key, el_key;
+  int n_pf = 0;
   int n_pairs = 0, n_atoms = 0, n_sites = 0, n_soap = 0;
   bool have_out = false, have_forces = false;
 };
@@ -46,9 +47,15 @@ struct AdjointInputs {
   const uint8_t* first;
   const double *rjs, *thetas, *phis, *G;
   const int4* comp;
-  const int* out_ptr;
+  const int* out_ptr;   // null without compression
   const int* out_c;
-  const double* out_el;
+  const int* unc_ptr;
+  const int* unc_e;
+  const int32_t* P_i;
+  const std::vector<double>* out_el_h;  // compression elements by output (host)
+  const std::vector<double>* P_el_h;    // compression elements in P order (host)
+  const std::string* tables_key;
+  std::vector<RadialConsts> radial_tab;  // per species, block offset 0, n_max = alpha_max, with derivatives
   const int32_t* pair_atom;
   const double* vec;
   int n_atoms;

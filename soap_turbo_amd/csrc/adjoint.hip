@@ -125,21 +125,6 @@ void launch_scatter_f32(hipStream_t st, int n_pairs, int n_atoms, const float* p
   scatter<float>(st, n_pairs, n_atoms, pair, pair_atom, vec, rjs, thetas, phis, forces, vpart, virial);
 }
 
-int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const Timer& timer, std::string& msg) {
-  (void)s; (void)st; (void)in; (void)timer;
-  msg = "the FP32 adjoint path is not available in this build";
-  return ST_ERR_UNSUPPORTED;
-}
-
-int adjoint_download_pairs(AdjointState& s, hipStream_t st, int n_pairs, double* pair, std::string& msg) {
-  (void)st; (void)n_pairs; (void)pair;
-  if (!s.have_out) {
-    msg = "no FP32 pair gradients computed";
-    return ST_ERR_STATE;
-  }
-  return ST_OK;
-}
-
 int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial,
                             std::string& msg) {
   if (!s.have_forces) {
@@ -149,15 +134,6 @@ int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double
   return download_forces(s, st, n_atoms, forces, virial, msg);
 }
 
-int adjoint_download_soap(AdjointState& s, hipStream_t st, int n_sites, int n_soap, double* soap, std::string& msg) {
-  (void)st; (void)n_sites; (void)n_soap; (void)soap;
-  if (!s.have_out) {
-    msg = "no FP32 descriptors computed";
-    return ST_ERR_STATE;
-  }
-  return ST_OK;
-}
-
 void adjoint_invalidate(AdjointState& s) {
   s.have_out = false;
   s.have_forces = false;
@@ -165,7 +141,7 @@ void adjoint_invalidate(AdjointState& s) {
 
 void adjoint_release(AdjointState& s) {
   AdjBuf* bufs[] = {&s.rtab, &s.atab, &s.tabinfo, &s.cnk, &s.soap, &s.sdp, &s.H, &s.pair, &s.forces,
-                    &s.virial_part, &s.virial, &s.G32, &s.order, &s.atom_ptr, &s.atom_pairs, &s.vecbuf, &s.bins};
+                    &s.virial_part, &s.virial, &s.G32, &s.order, &s.atom_ptr, &s.atom_pairs, &s.vecbuf, &s.bins, &s.el32};
   for (AdjBuf* b : bufs)
     if (b->p) {
       (void)hipFree(b->p);

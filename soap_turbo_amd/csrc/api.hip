@@ -62,6 +62,7 @@ struct st_ctx {
   bool have_forces64 = false;
   DevBuf comp, out_ptr, out_c, out_el, skip, unc_ptr, unc_e, P_i, P_el;
   std::string tables_key;
+  std::vector<double> out_el_h, P_el_h;
   int n_max = 0, k_max = 0, l_max = -1, n_soap = 0, n_soap_unc = 0;
   bool compressed = false, have_soap = false, have_cart = false, ct_ready = false, have_pair64 = false;
 
@@ -382,6 +383,8 @@ int build_tables(st_ctx* ctx, const st_params* p, const st_compression* c) {
       if ((rc = upload(ctx, ctx->out_ptr, optr.data(), optr.size() * 4))) return rc;
       if ((rc = upload(ctx, ctx->out_c, oc.data(), oc.size() * 4))) return rc;
       if ((rc = upload(ctx, ctx->out_el, oel.data(), oel.size() * 8))) return rc;
+      ctx->out_el_h = oel;
+      ctx->P_el_h.assign(c->P_el, c->P_el + nz);
       if ((rc = upload(ctx, ctx->unc_ptr, uptr.data(), uptr.size() * 4))) return rc;
       if ((rc = upload(ctx, ctx->unc_e, ue.data(), ue.size() * 4))) return rc;
       if ((rc = upload(ctx, ctx->P_i, c->P_i, (size_t)nz * 4))) return rc;
@@ -652,11 +655,15 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
                    p->basis == ST_BASIS_POLY3GAUSS ? ctx->central.as<double>() : nullptr, ctx->cnk.as<double2>());
   });
   const int* optr = ctx->compressed ? ctx->out_ptr.as<int>() : nullptr;
+  if (sizeof(double2) * n_max * k_max > 64 * 1024)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "n_max * k_max too large for the spectrum kernel");
   timed(ctx, "power_f64", [&] {
     st::launch_power(st, NS, n_max, k_max, n_soap, ctx->comp.as<int4>(), optr, ctx->out_c.as<int>(),
                      ctx->out_el.as<double>(), ctx->cnk.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>());
   });
   if (der && !ctr) {
+    if (st::derivatives_lds(n_max, L, k_max) > 64 * 1024)
+      return fail(ctx, ST_ERR_UNSUPPORTED, "the derivative working set exceeds the LDS");
     timed(ctx, "derivatives_f64", [&] {
       ok = st::launch_derivatives(st, NS, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(), ctx->comp.as<int4>(), optr,
                                   ctx->out_c.as<int>(), ctx->out_el.as<double>(), ctx->cnk.as<double2>(),
@@ -697,7 +704,7 @@ int st_contract_resident(st_ctx* ctx, double sgn) {
   if (!ctx->have_G || ctx->G_n_soap != ctx->n_soap) return fail(ctx, ST_ERR_STATE, "dE/dq missing or of another size");
   HIPCHK(hipSetDevice(ctx->device));
   size_t sh = st::contract_lds(ctx->n_max, ctx->k_max, ctx->n_soap, ctx->n_soap_unc);
-  if (sh > 160 * 1024) return fail(ctx, ST_ERR_UNSUPPORTED, "soap_contract working set exceeds the LDS");
+  if (sh > 64 * 1024) return fail(ctx, ST_ERR_UNSUPPORTED, "soap_contract working set exceeds the LDS");
   int rc = ensure(ctx, ctx->pair64, (size_t)ctx->n_pairs * 24);
   if (rc) return rc;
   timed(ctx, "contract_f64", [&] {
@@ -797,7 +804,24 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   in.comp = ctx->comp.as<int4>();
   in.out_ptr = ctx->compressed ? ctx->out_ptr.as<int>() : nullptr;
   in.out_c = ctx->out_c.as<int>();
-  in.out_el = ctx->out_el.as<double>();
+  in.unc_ptr = ctx->unc_ptr.as<int>();
+  in.unc_e = ctx->unc_e.as<int>();
+  in.P_i = ctx->P_i.as<int32_t>();
+  in.out_el_h = &ctx->out_el_h;
+  in.P_el_h = &ctx->P_el_h;
+  in.tables_key = &ctx->tables_key;
+  {
+    int off = 0;
+    for (int s = 0; s < p->n_species; ++s) {
+      st::RadialConsts c = radial_consts(p, s, ctx->W, n_max_of(p), off, true);
+      // the table is one species block on its own: offset 0, leading dimension alpha_max
+      c.off = 0;
+      c.n_max = p->alpha_max[s];
+      c.species_bit = 1u;
+      in.radial_tab.push_back(c);
+      off += p->alpha_max[s];
+    }
+  }
   in.pair_atom = ctx->have_atoms ? ctx->pair_atom.as<int32_t>() : nullptr;
   in.vec = ctx->have_vec ? ctx->vec.as<double>() : nullptr;
   in.n_atoms = ctx->have_atoms ? ctx->n_atoms : 0;

@@ -1,0 +1,91 @@
+"""The Fortran drop-in modules (soap_turbo_desc, soap_turbo_compress_module,
+soap_turbo_radial) driven through their flat C-callable wrappers."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from tests.golden_cases import CASES, load, rel
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FLIB = os.path.join(os.path.dirname(HERE), "soap_turbo_amd", "libsoap_turbo_f.so")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(FLIB), reason="Fortran drop-in not built (no flang)")
+
+
+def flib():
+    L = ctypes.CDLL(FLIB)
+    return L
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+@pytest.mark.parametrize("name", ["cfg2_legacy", "var_poly3gauss_amp0"])
+def test_fortran_compress_indices(name):
+    _, comp, d = load(name)
+    L = flib()
+    am = np.asarray(d["alpha_max"], dtype=np.int32)
+    dim = ctypes.c_int32(); nz = ctypes.c_int32()
+    z = np.zeros(1, dtype=np.int32); zf = np.zeros(1)
+    L.st_f90_compress_indices(ctypes.c_int32(len(am)), _p(am), ctypes.c_int32(int(d["l_max"])), ctypes.byref(dim),
+                              ctypes.byref(nz), _p(z), _p(z), _p(zf), ctypes.c_int32(0))
+    assert nz.value == len(d["P_i"])
+    P_i = np.zeros(nz.value, np.int32); P_j = np.zeros(nz.value, np.int32); P_el = np.zeros(nz.value)
+    L.st_f90_compress_indices(ctypes.c_int32(len(am)), _p(am), ctypes.c_int32(int(d["l_max"])), ctypes.byref(dim),
+                              ctypes.byref(nz), _p(P_i), _p(P_j), _p(P_el), ctypes.c_int32(1))
+    np.testing.assert_array_equal(P_i, d["P_i"])
+    np.testing.assert_array_equal(P_j, d["P_j"])
+    np.testing.assert_array_equal(P_el, d["P_el"])
+
+
+def run_fortran_get_soap(L, w, comp, d, do_der, contract, legacy):
+    p = w.params
+    ps = p.per_species()
+    n_max = sum(p.alpha_max)
+    n_soap = int(np.max(comp[0])) if comp is not None else n_max * (n_max + 1) // 2 * (p.l_max + 1)
+    soap = np.zeros((n_soap, w.n_sites), order="F")
+    ncart = w.n_pairs if do_der and not contract else 1
+    cart = np.zeros((3, n_soap, ncart), order="F")
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    keep = [i32(w.n_neigh), i32(w.species[:, 0]), i32(w.species_multiplicity), i32(w.mask), f64(w.rjs),
+            f64(w.thetas), f64(w.phis), i32(p.alpha_max)] + [f64(ps[k]) for k in (
+                "rcut_hard", "rcut_soft", "nf", "global_scaling", "atom_sigma_r", "atom_sigma_r_scaling",
+                "atom_sigma_t", "atom_sigma_t_scaling", "amplitude_scaling")] + [f64(ps["central_weight"])]
+    if comp is not None:
+        cp = [i32(comp[0]), i32(comp[1]), f64(comp[2])]
+    else:
+        cp = [np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(1)]
+    I = ctypes.c_int32
+    args = [I(w.n_sites), I(p.n_species), I(w.n_pairs), I(1)] + [_p(a) for a in keep[:8]] + [I(p.l_max)] + \
+        [_p(a) for a in keep[8:17]] + [I(p.radial_enhancement), _p(keep[17]),
+                                      I(1 if p.basis == "poly3gauss" else 0),
+                                      I(0 if p.scaling_mode == "polynomial" else 1), I(int(do_der)),
+                                      I(int(comp is not None)), I(len(cp[0]) if comp is not None else 0)] + \
+        [_p(a) for a in cp] + [I(n_soap), _p(soap), _p(cart), I(int(contract)), I(int(legacy))]
+    L.st_f90_get_soap(*args)
+    return soap, cart
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cfg1_legacy", "cfg2_unified", "var_poly3_scalings"])
+def test_fortran_get_soap_matches_reference(name):
+    # the drop-in builds its own W (FP64 eigen-route), so parity with the reference's W is
+    # limited by the conditioning of S: 1e-4 at poly3 alpha_max = 8, 1e-6 otherwise
+    w, comp, d = load(name)
+    legacy = bool(d["legacy_seed"])
+    L = flib()
+    tol = 1e-3 if name.startswith("cfg1") else 1e-5
+    soap, cart = run_fortran_get_soap(L, w, comp, d, True, False, legacy)
+    assert rel(soap.T, d["soap"]) <= tol
+    assert rel(cart.transpose(2, 1, 0)[:len(d["cart_head"])], d["cart_head"]) <= tol
+    q, _ = run_fortran_get_soap(L, w, comp, d, True, True, legacy)
+    assert rel(q.T, d["soap_contract_q"]) <= tol
+    pair = np.zeros((3, w.n_pairs), order="F")
+    G = np.asfortranarray(d["G"].T)
+    L.st_f90_soap_contract(ctypes.c_int32(w.n_sites), ctypes.c_int32(q.shape[0]), ctypes.c_int32(w.n_pairs), _p(q),
+                           _p(G), ctypes.c_double(1.0), _p(pair))
+    assert rel(pair.T, d["pair"]) <= tol

@@ -1,0 +1,108 @@
+"""The FP32 adjoint path against the FP64 oracle: descriptors, pair gradients, forces, virial.
+
+Tolerance: 1e-5 relative (max |a-b| / max |b| over each array), the bound the
+FP32 path is held to for forces; the oracle runs in FP64 with the same W.
+"""
+import numpy as np
+import pytest
+
+import soap_turbo_amd as st
+from oracle import oracle as orc
+from soap_turbo_amd.workload import BoxSpec, SoapParams, make_workload, normal
+from tests.golden_cases import rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return st.Context(0)
+
+
+def setup(ctx, w, legacy=True, seed=5):
+    p = w.params
+    keep = st._lib.Keep()
+    ps = p.per_species()
+    prm = st.make_params(keep, alpha_max=p.alpha_max, l_max=p.l_max, radial_enhancement=p.radial_enhancement,
+                         basis=p.basis, scaling_mode=p.scaling_mode, legacy_seed=legacy, **ps)
+    comp = None
+    if p.compress_mode == "trivial":
+        dim, nz = st.get_compress_indices("trivial", p.alpha_max, p.l_max, "get_dim")
+        comp = (np.zeros(nz, np.int32), np.zeros(nz, np.int32), np.zeros(nz))
+        st.get_compress_indices("trivial", p.alpha_max, p.l_max, "set_indices", *comp)
+    cmp_ = st.make_compression(keep, comp is not None, len(comp[0]) if comp else 0, *(comp or (None, None, None)))
+    n_max = sum(p.alpha_max)
+    n_soap = int(comp[0].max()) if comp else n_max * (n_max + 1) // 2 * (p.l_max + 1)
+    W, S = ctx.get_basis(prm, n_max)
+    G = normal(seed, (n_soap, w.n_sites))
+    ctx.upload_neighbours(w.n_neigh, w.species.T, w.species_multiplicity, w.mask.T, w.rjs, w.thetas, w.phis)
+    ctx.upload_atoms(len(w.pos), w.nbr, w.vec)
+    ctx.upload_dedq(np.asfortranarray(G.T))
+    return keep, prm, cmp_, comp, n_soap, W, S, G
+
+
+@pytest.mark.parametrize("cfg,n_sites,legacy", [(1, None, True), (2, None, True), (2, 300, False), (4, 400, True)])
+def test_adjoint_f32_matches_fp64_oracle(ctx, cfg, n_sites, legacy):
+    w = make_workload(cfg, n_sites=n_sites)
+    keep, prm, cmp_, comp, n_soap, W, S, G = setup(ctx, w, legacy)
+    ctx.adjoint_f32(prm, cmp_)
+    q32 = ctx.download_soap_f32(n_soap).T
+    pair32 = ctx.download_pairs(1).T
+    f32, v32 = ctx.download_forces(1)
+    qo, pair_o = orc.contract(w, W, S, G, 1.0, comp, legacy)
+    fo = np.zeros_like(w.pos)
+    np.add.at(fo, w.nbr, -pair_o)
+    vo = -np.einsum("ka,kb->ab", w.vec, pair_o)
+    eq, ep, ef, ev = rel(q32, qo), rel(pair32, pair_o), rel(f32.T, fo), rel(v32, vo)
+    print(f"cfg {cfg}: rel err q {eq:.2e} pair {ep:.2e} forces {ef:.2e} virial {ev:.2e}")
+    assert eq <= TOL
+    assert ef <= TOL
+    assert ev <= TOL
+    assert ep <= 10 * TOL
+
+
+def test_adjoint_f32_forces_sum_to_zero_and_repeat(ctx):
+    w = make_workload(1)
+    keep, prm, cmp_, comp, n_soap, W, S, G = setup(ctx, w)
+    ctx.adjoint_f32(prm, cmp_)
+    f1, _ = ctx.download_forces(1)
+    ctx.adjoint_f32(prm, cmp_)
+    f2, _ = ctx.download_forces(1)
+    scale = np.abs(f1).max()
+    assert np.abs(f1.sum(axis=1)).max() <= 1e-5 * scale * np.sqrt(f1.shape[1])
+    assert rel(f2, f1) <= 1e-6
+
+
+def test_adjoint_f32_central_differences(ctx):
+    # forces = -dE/dx with E = sum_i G_i . q_i (FP64 oracle energies), step 1e-4 A, 64 seeded atoms
+    spec = BoxSpec(64, 0.06, 1.2, [1.0], [6], seed=21)
+    par = SoapParams(alpha_max=[6], l_max=6, rcut_hard=4.5, rcut_soft=4.0, basis="poly3")
+    w = make_workload((spec, par))
+    keep, prm, cmp_, comp, n_soap, W, S, G = setup(ctx, w, legacy=False)
+    ctx.adjoint_f32(prm, cmp_)
+    f32, _ = ctx.download_forces(1)
+    from soap_turbo_amd.workload import spherical
+
+    def energy(pos):
+        site = np.repeat(w.site_atom, w.n_neigh)
+        d = pos[w.nbr] - pos[site]
+        d -= w.L * np.round(d / w.L)
+        d[w.nbr == site] = 0.0
+        wq = make_workload((spec, par))
+        wq.rjs, wq.thetas, wq.phis = spherical(d)
+        q, _, _ = orc.get_soap(wq, W, S, do_der=False, legacy_seed=False)
+        return float(np.sum(q * G.T))
+
+    h = 1e-4
+    scale = np.abs(f32).max()
+    worst = 0.0
+    for a in range(0, 64, 9):
+        for c in range(3):
+            pp = w.pos.copy(); pp[a, c] += h
+            pm = w.pos.copy(); pm[a, c] -= h
+            fd = -(energy(pp) - energy(pm)) / (2 * h)
+            worst = max(worst, abs(fd - f32[c, a]) / scale)
+    print(f"central differences: max rel err {worst:.2e}")
+    assert worst <= 1e-5
