@@ -142,7 +142,7 @@ def main():
     except st.SoapTurboError as e:
         log(f"FP32 adjoint path unavailable: {e}")
         have32 = False
-    k32 = ["adj_tables", "adj_expand_f32", "adj_power_f32", "adj_vjp_f32", "adj_forces"]
+    k32 = ["adj_tables", "adj_transpose_g", "adj_site_f32", "adj_forces"]
     if have32:
         log("timing the FP32 adjoint path")
         ms32, wall32, kt32 = time_path(ctx, step32, args.steps, args.warmup, k32)
@@ -155,10 +155,12 @@ def main():
     n_max, k_max = sum(p.alpha_max), 1 + p.l_max * (p.l_max + 1) // 2 + p.l_max
     if have32:
         step_ms = ms32 / args.steps
-        name, (tot, nl) = max(kt32.items(), key=lambda kv: kv[1][0])
+        name = "adj_site_f32"
+        tot, nl = kt32[name]
         avg_ms = tot / max(nl, 1)
-        # algorithmic FP32 flops per site: cnk 4 + VJP 16 per neighbour and (n, k); spectrum 4 + H 4 per (n, n', k)
-        flops_site = n_nb * 20 * n_max * k_max + 8 * n_max * n_max * k_max
+        # algorithmic FP32 flops per site: per neighbour and (n, k), cnk 4 (real x complex FMA) and the two
+        # contractions u = R H, v = R' H 8; per site and (n, n', k), spectrum 4 and H = Y c 4
+        flops_site = n_nb * 12 * n_max * k_max + 8 * n_max * n_max * k_max
         achieved = flops_site * w.n_sites / (avg_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,

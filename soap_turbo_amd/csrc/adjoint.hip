@@ -75,12 +75,14 @@ __global__ __launch_bounds__(256) void scatter_k(int n_pairs, const T* __restric
   }
 }
 
-__global__ void virial_sum_k(int nb, const double* __restrict__ vpart, double* __restrict__ virial) {
-  if (threadIdx.x < 9) {
-    double t = 0.0;
-    for (int b = 0; b < nb; ++b) t += vpart[(size_t)b * 9 + threadIdx.x];
-    virial[threadIdx.x] = t;
-  }
+// one wavefront per virial component, lanes striding over the block partials
+__global__ __launch_bounds__(576) void virial_sum_k(int nb, const double* __restrict__ vpart,
+                                                    double* __restrict__ virial) {
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double t = 0.0;
+  for (int b = lane; b < nb; b += 64) t += vpart[(size_t)b * 9 + q];
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if (lane == 0) virial[q] = t;
 }
 
 template <class T>
@@ -92,7 +94,7 @@ void scatter(hipStream_t st, int n_pairs, int n_atoms, const T* pair, const int3
   if (nb > SCATTER_BLOCKS) nb = SCATTER_BLOCKS;
   if (nb < 1) nb = 1;
   scatter_k<T><<<nb, 256, 0, st>>>(n_pairs, pair, pair_atom, vec, rjs, thetas, phis, forces, vpart);
-  virial_sum_k<<<1, 64, 0, st>>>(nb, vpart, virial);
+  virial_sum_k<<<1, 576, 0, st>>>(nb, vpart, virial);
 }
 
 int download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial, std::string& msg) {

@@ -41,16 +41,19 @@ constexpr int NTHREADS = 256;
 
 struct SpeciesTab {
   int a, off, do_central, nseg;
-  float rcut, cw;
-  float seg_lo[MAXSEG], seg_hi[MAXSEG], seg_inv_h[MAXSEG], seg_h[MAXSEG];
-  int seg_base[MAXSEG], seg_n[MAXSEG];
-  int rtab_off;   // floats into rtab: node-major, 2*a floats per node (values then derivatives)
-  int atab_off;   // floats into atab: node-major, 2*(L+1) floats per node
+  double rcut;
+  float cw;
+  double seg_lo[MAXSEG], seg_inv_h[MAXSEG];
+  float seg_hi[MAXSEG], seg_ih[MAXSEG];
+  int seg_base[MAXSEG], seg_n[MAXSEG];   // first cell and node count of each segment
+  int rtab_off;   // floats into rtab: cell-major, 4*a floats per cell (cubic coefficients c0..c3)
+  int atab_off;   // floats into atab: cell-major, 4*(L+1) floats per cell
 };
 
 struct AdjParams {
   int n_species, n_max, k_max, l_max, n_soap, n_soap_unc, n_sites, compressed, gauss;
-  float rcut_max, ang_inv_h, ang_h;
+  double rcut_max, ang_inv_h;
+  float ang_ih;
   int ang_n;
   SpeciesTab sp[ADJ_MAX_SPECIES];
 };
@@ -124,30 +127,19 @@ __global__ void ang_table_k(int n_nodes, double h, int L, double rcut2, double s
 }
 
 // ------------------------------------------------------------ device helpers
-__device__ __forceinline__ void hermite(float t, float h, float& h00, float& h10, float& h01, float& h11, float& d00,
-                                        float& d10, float& d01, float& d11) {
-  const float t2 = t * t, t3 = t2 * t;
-  h00 = 2.f * t3 - 3.f * t2 + 1.f;
-  h10 = (t3 - 2.f * t2 + t) * h;
-  h01 = -2.f * t3 + 3.f * t2;
-  h11 = (t3 - t2) * h;
-  const float ih = 1.f / h;
-  d00 = (6.f * t2 - 6.f * t) * ih;
-  d10 = 3.f * t2 - 4.f * t + 1.f;
-  d01 = (-6.f * t2 + 6.f * t) * ih;
-  d11 = 3.f * t2 - 2.f * t;
-}
-
-// Locates r in the species' radial segments: returns the node row index and t.
-__device__ __forceinline__ int radial_cell(const SpeciesTab& s, float r, float& t, float& h) {
+// Tables hold, per grid cell and function, the cubic c0 + c1 t + c2 t^2 + c3 t^3
+// (t in [0, 1) across the cell) of the Hermite interpolant of the FP64 values
+// and derivatives at the cell's ends; the coefficients are formed in FP64, so
+// the FP32 derivative (c1 + 2 c2 t + 3 c3 t^2)/h has no cancellation.
+__device__ __forceinline__ int radial_cell(const SpeciesTab& s, double r, float& t, float& ih) {
   int j = 0;
   while (j + 1 < s.nseg && r >= s.seg_lo[j + 1]) ++j;
-  float x = (r - s.seg_lo[j]) * s.seg_inv_h[j];
+  const double x = (r - s.seg_lo[j]) * s.seg_inv_h[j];
   int i = (int)x;
   if (i < 0) i = 0;
   if (i > s.seg_n[j] - 2) i = s.seg_n[j] - 2;
-  t = x - (float)i;
-  h = s.seg_h[j];
+  t = (float)(x - (double)i);
+  ih = s.seg_ih[j];
   return s.seg_base[j] + i;
 }
 
@@ -204,9 +196,85 @@ __device__ __forceinline__ void angular_rows(float x, float sq, const float* __r
   }
 }
 
-// One site per workgroup.
+// P_l^m(x) for l <= L + 1 into out[l (l + 1) / 2 + m] (Condon-Shortley phase,
+// the recursion of get_plm_array, src/soap_turbo_angular.f90:108-160).
 template <int L>
-__global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
+__device__ __forceinline__ void plm_rows(float x, float sq, float* out) {
+  float pm[L + 3], pc[L + 3];
+#pragma unroll
+  for (int m = 0; m < L + 3; ++m) pm[m] = pc[m] = 0.f;
+  pm[0] = 1.f;             // l = 0
+  pc[0] = x; pc[1] = -sq;  // l = 1
+  out[0] = 1.f;
+  out[1] = x;
+  out[2] = -sq;
+#pragma unroll
+  for (int l = 2; l <= L + 1; ++l) {
+    float pn[L + 3];
+#pragma unroll
+    for (int m = 0; m < L + 3; ++m) pn[m] = 0.f;
+#pragma unroll
+    for (int m = 0; m <= l - 2; ++m)
+      pn[m] = ((float)(2 * l - 1) * x * pc[m] - (float)(l - 1 + m) * pm[m]) / (float)(l - m);
+    pn[l - 1] = x * (float)(2 * l - 1) * pc[l - 1];
+    pn[l] = -(float)(2 * l - 1) * sq * pc[l - 1];
+#pragma unroll
+    for (int m = 0; m <= l; ++m) out[l * (l + 1) / 2 + m] = pn[m];
+#pragma unroll
+    for (int m = 0; m < L + 3; ++m) { pm[m] = pc[m]; pc[m] = pn[m]; }
+  }
+}
+
+// Per-pair radial values R_n(r) for the species blocks the pair belongs to
+// (n in [nlo, nhi) only), and optionally dR_n/dr; zero elsewhere.
+template <int NR>
+__device__ __forceinline__ void radial_eval(const AdjParams& P, const float* __restrict__ rtab, double rd,
+                                            uint32_t msk, bool self, int nlo, int nhi, float* Rn, float* Rd) {
+#pragma unroll
+  for (int j = 0; j < NR; ++j) { Rn[j] = 0.f; if (Rd) Rd[j] = 0.f; }
+  for (int s = 0; s < P.n_species; ++s) {
+    const SpeciesTab& sp = P.sp[s];
+    if (!(msk & (1u << s)) || !(rd < sp.rcut)) continue;
+    if (self && !sp.do_central) continue;
+    const int lo = max(sp.off, nlo), hi = min(sp.off + sp.a, nhi);
+    if (lo >= hi) continue;
+    float t, ih;
+    const int cell = radial_cell(sp, rd, t, ih);
+    const float* c = rtab + sp.rtab_off + (size_t)cell * 4 * sp.a;
+    const float sc = self ? sp.cw : 1.f;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const int n = nlo + j;
+      if (n >= lo && n < hi) {
+        const int q = n - sp.off;
+        const float c0 = c[q], c1 = c[sp.a + q], c2 = c[2 * sp.a + q], c3 = c[3 * sp.a + q];
+        Rn[j] = sc * (c0 + t * (c1 + t * (c2 + t * c3)));
+        if (Rd) Rd[j] = (c1 + t * (2.f * c2 + 3.f * t * c3)) * ih;
+      }
+    }
+  }
+}
+
+// Angular radial factors g_l(r) and dg_l/dr of the pair's first species.
+template <int L>
+__device__ __forceinline__ void angular_factors(const AdjParams& P, const float* __restrict__ atab, double rd, int s0,
+                                                float* g, float* gd) {
+  const double xa = rd * P.ang_inv_h;
+  int ia = (int)xa;
+  if (ia > P.ang_n - 2) ia = P.ang_n - 2;
+  const float ta = (float)(xa - (double)ia);
+  const float* gc = atab + P.sp[s0].atab_off + (size_t)ia * 4 * (L + 1);
+#pragma unroll
+  for (int l = 0; l <= L; ++l) {
+    const float c0 = gc[l], c1 = gc[L + 1 + l], c2 = gc[2 * (L + 1) + l], c3 = gc[3 * (L + 1) + l];
+    g[l] = c0 + ta * (c1 + ta * (c2 + ta * c3));
+    gd[l] = (c1 + ta * (2.f * c2 + 3.f * ta * c3)) * P.ang_ih;
+  }
+}
+
+// One site per workgroup of 256 threads (4 wavefronts).
+template <int L>
+__global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
     AdjParams P, const int* __restrict__ order, const int* __restrict__ site_ptr, const float* __restrict__ rtab,
     const float* __restrict__ atab, const float* __restrict__ preflm_g, const float* __restrict__ central,
     const int32_t* __restrict__ site_species, int max_mult, const int32_t* __restrict__ species_mult,
@@ -217,6 +285,10 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
     const float* __restrict__ P_el, float* __restrict__ q_out, float* __restrict__ sdp_out,
     float* __restrict__ pair_out) {
   constexpr int KM = (L + 1) * (L + 2) / 2;
+  constexpr int KM1 = (L + 2) * (L + 3) / 2;
+  constexpr int KS = KM1 | 1;       // odd row stride of the per-pair P_l^m rows
+  constexpr int L1 = L + 1;
+  constexpr int L1S = L1 | 1;
   extern __shared__ float smem[];
   __shared__ float red[16];
   __shared__ float preflm[KM];
@@ -224,13 +296,20 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
   const int n_max = P.n_max, k_max = KM;
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const float fourpi = 12.566370614359172f;
-  // LDS: C hi/lo, H (float2 [n][k]); tile region
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float2* C = reinterpret_cast<float2*>(smem);                 // n_max * k_max
-  float2* Clo = C + n_max * k_max;                             // compensation
+  float2* Clo = C + n_max * k_max;                             // Kahan compensation
   float2* H = Clo + n_max * k_max;                             // n_max * k_max
   float* region = reinterpret_cast<float*>(H + n_max * k_max);
+  // forward tile
   float* Rt = region;                                          // PT * n_max
   float2* At = reinterpret_cast<float2*>(Rt + PT * n_max);     // PT * k_max
+  // backward tile
+  float* Fp = region;                                          // PT * KS: P_l^m, l <= L + 1
+  float* Gl = Fp + PT * KS;                                    // PT * L1S: g_l
+  float* Gd = Gl + PT * L1S;                                   // PT * L1S: dg_l/dr
+  float2* Em = reinterpret_cast<float2*>(Gd + PT * L1S);       // PT * L1S: e^{-i m phi}
+  float* fred = reinterpret_cast<float*>(Em + PT * L1S);       // 4 * PT * 3
 
   for (int t = threadIdx.x; t < KM; t += blockDim.x) preflm[t] = preflm_g[t];
   for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
@@ -241,50 +320,27 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
   for (int tb = p0; tb < p1; tb += PT) {
     const int np_ = min(PT, p1 - tb);
     __syncthreads();
-    if (threadIdx.x < PT) {
-      const int lp = threadIdx.x;
-      float* R = Rt + lp * n_max;
-      float2* A = At + lp * k_max;
-      for (int n = 0; n < n_max; ++n) R[n] = 0.f;
-      for (int k = 0; k < k_max; ++k) A[k] = make_float2(0.f, 0.f);
-      if (lp < np_) {
-        const int p = tb + lp;
-        const float r = (float)rjs[p];
+    if (wv == 0) {
+      float* R = Rt + lane * n_max;
+      float2* A = At + lane * k_max;
+      bool done = false;
+      if (lane < np_) {
+        const int p = tb + lane;
+        const double rd = rjs[p];
         const uint32_t msk = spmask[p];
-        const bool self = p == p0;
-        if (msk != 0 && r < P.rcut_max) {
-          for (int s = 0; s < P.n_species; ++s) {
-            const SpeciesTab& sp = P.sp[s];
-            if (!(msk & (1u << s)) || !(r < sp.rcut)) continue;
-            if (self && !sp.do_central) continue;
-            float t, h;
-            const int row = radial_cell(sp, r, t, h);
-            float h00, h10, h01, h11, d00, d10, d01, d11;
-            hermite(t, h, h00, h10, h01, h11, d00, d10, d01, d11);
-            const float* n0 = rtab + sp.rtab_off + (size_t)row * 2 * sp.a;
-            const float* n1 = n0 + 2 * sp.a;
-            const float sc = self ? sp.cw : 1.f;
-            for (int n = 0; n < sp.a; ++n)
-              R[sp.off + n] = sc * (h00 * n0[n] + h10 * n0[sp.a + n] + h01 * n1[n] + h11 * n1[sp.a + n]);
+        if (msk != 0 && rd < P.rcut_max) {
+          done = true;
+          for (int n0 = 0; n0 < n_max; n0 += 8) {
+            float v[8];
+            radial_eval<8>(P, rtab, rd, msk, p == p0, n0, min(n_max, n0 + 8), v, nullptr);
+            for (int j = 0; j < 8 && n0 + j < n_max; ++j) R[n0 + j] = v[j];
           }
-          const int s0 = __ffs(msk) - 1;
-          // angular radial factors g_l(r)
-          float x = r * P.ang_inv_h;
-          int ia = (int)x;
-          if (ia > P.ang_n - 2) ia = P.ang_n - 2;
-          const float ta = x - (float)ia;
-          float h00, h10, h01, h11, d00, d10, d01, d11;
-          hermite(ta, P.ang_h, h00, h10, h01, h11, d00, d10, d01, d11);
-          const float* g0 = atab + P.sp[s0].atab_off + (size_t)ia * 2 * (L + 1);
-          const float* g1 = g0 + 2 * (L + 1);
-          float g[L + 1];
-#pragma unroll
-          for (int l = 0; l <= L; ++l) g[l] = h00 * g0[l] + h10 * g0[L + 1 + l] + h01 * g1[l] + h11 * g1[L + 1 + l];
+          float g[L1], gd[L1];
+          angular_factors<L>(P, atab, rd, __ffs(msk) - 1, g, gd);
           float st_, ct_, sf_, cf_;
           sincosf((float)thetas[p], &st_, &ct_);
           sincosf((float)phis[p], &sf_, &cf_);
-          // e^{-i m phi}
-          float er[L + 1], ei[L + 1];
+          float er[L1], ei[L1];
           er[0] = 1.f; ei[0] = 0.f;
 #pragma unroll
           for (int m = 1; m <= L; ++m) {
@@ -296,6 +352,10 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
             A[k] = make_float2(v * er[m], v * ei[m]);
           });
         }
+      }
+      if (!done) {
+        for (int n = 0; n < n_max; ++n) R[n] = 0.f;
+        for (int k = 0; k < k_max; ++k) A[k] = make_float2(0.f, 0.f);
       }
     }
     __syncthreads();
@@ -322,18 +382,14 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
     const float2 c = Clo[t];
     s.x -= c.x;
     s.y -= c.y;
-    C[t] = s;
-  }
-  __syncthreads();
-  if (P.gauss) {
-    for (int n = threadIdx.x; n < n_max; n += blockDim.x) {
-      float add = 0.f;
+    if (P.gauss && t % k_max == 0) {
+      const int n = t / k_max;
       for (int m = 0; m < species_mult[i]; ++m) {
-        const int s = site_species[(size_t)i * max_mult + m] - 1;
-        add += central[(size_t)s * n_max + n];
+        const int sp = site_species[(size_t)i * max_mult + m] - 1;
+        s.x += central[(size_t)sp * n_max + n];
       }
-      C[n * k_max].x += add;
     }
+    C[t] = s;
   }
   __syncthreads();
   // --------------------------------------------------------------- spectrum
@@ -387,14 +443,14 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
     Gu[c] = v;
   }
   __syncthreads();
-  // H(k,n) = w(m) sum_np Y(n,np,l) c(k,np)
-  const int L1 = L + 1;
+  // H(k,n) = 4 pi w(m) sum_np Y(n,np,l) c(k,np), Y doubled on the diagonal, sqrt 2 off it
   for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
     const int n = t / k_max, k = t - n * k_max;
     int l = 0;
     while ((l + 1) * (l + 2) / 2 <= k) ++l;
     const int m = k - l * (l + 1) / 2;
     float hr = 0.f, hi = 0.f;
+    // c(a, b, l) = (a n_max - a(a-1)/2 + b - a) L1 + l for a <= b
     for (int np = 0; np < n_max; ++np) {
       const int a = n < np ? n : np, b = n < np ? np : n;
       const int c = (a * n_max - a * (a - 1) / 2 + (b - a)) * L1 + l;
@@ -407,69 +463,43 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
     H[n * k_max + k] = make_float2(w * hr, w * hi);
   }
   // --------------------------------------------------------------- backward
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ng = (n_max + 3) / 4;
   const int nlo = wv * ng, nhi = min(n_max, nlo + ng);
-  float* fred = region;  // [4][PT][3]
   float self0 = 0.f, self1 = 0.f, self2 = 0.f;
   for (int tb = p0; tb < p1; tb += PT) {
     __syncthreads();
     const int p = tb + lane;
+    const bool live = p < p1 && p != p0 && spmask[p] != 0 && rjs[p] < P.rcut_max;
+    // per-pair angular factors, one pair per lane of wavefront 0
+    if (wv == 0 && live) {
+      const double rd = rjs[p];
+      float g[L1], gd[L1];
+      angular_factors<L>(P, atab, rd, __ffs(spmask[p]) - 1, g, gd);
+#pragma unroll
+      for (int l = 0; l <= L; ++l) { Gl[lane * L1S + l] = g[l]; Gd[lane * L1S + l] = gd[l]; }
+      float st_, ct_, sf_, cf_;
+      sincosf((float)thetas[p], &st_, &ct_);
+      sincosf((float)phis[p], &sf_, &cf_);
+      float er = 1.f, ei = 0.f;
+      Em[lane * L1S] = make_float2(1.f, 0.f);
+      for (int m = 1; m <= L; ++m) {
+        const float nr = er * cf_ + ei * sf_, ni = ei * cf_ - er * sf_;
+        er = nr; ei = ni;
+        Em[lane * L1S + m] = make_float2(er, ei);
+      }
+      plm_rows<L>(ct_, st_, Fp + lane * KS);
+    }
+    __syncthreads();
     float fr = 0.f, fa = 0.f, fo = 0.f;
-    if (p < p1 && p != p0) {
-      const float r = (float)rjs[p];
-      const uint32_t msk = spmask[p];
-      if (msk != 0 && r < P.rcut_max) {
-        float Rn[12], Rd[12];
-#pragma unroll
-        for (int j = 0; j < 12; ++j) Rn[j] = Rd[j] = 0.f;
-        for (int s = 0; s < P.n_species; ++s) {
-          const SpeciesTab& sp = P.sp[s];
-          if (!(msk & (1u << s)) || !(r < sp.rcut)) continue;
-          const int lo = max(sp.off, nlo), hi = min(sp.off + sp.a, nhi);
-          if (lo >= hi) continue;
-          float t, h;
-          const int row = radial_cell(sp, r, t, h);
-          float h00, h10, h01, h11, d00, d10, d01, d11;
-          hermite(t, h, h00, h10, h01, h11, d00, d10, d01, d11);
-          const float* n0 = rtab + sp.rtab_off + (size_t)row * 2 * sp.a;
-          const float* n1 = n0 + 2 * sp.a;
-#pragma unroll
-          for (int j = 0; j < 12; ++j) {
-            const int n = nlo + j;
-            if (n >= lo && n < hi) {
-              const int q = n - sp.off;
-              Rn[j] = h00 * n0[q] + h10 * n0[sp.a + q] + h01 * n1[q] + h11 * n1[sp.a + q];
-              Rd[j] = d00 * n0[q] + d10 * n0[sp.a + q] + d01 * n1[q] + d11 * n1[sp.a + q];
-            }
-          }
-        }
-        const int s0 = __ffs(msk) - 1;
-        float x = r * P.ang_inv_h;
-        int ia = (int)x;
-        if (ia > P.ang_n - 2) ia = P.ang_n - 2;
-        const float ta = x - (float)ia;
-        float h00, h10, h01, h11, d00, d10, d01, d11;
-        hermite(ta, P.ang_h, h00, h10, h01, h11, d00, d10, d01, d11);
-        const float* g0 = atab + P.sp[s0].atab_off + (size_t)ia * 2 * (L + 1);
-        const float* g1 = g0 + 2 * (L + 1);
-        float g[L + 1], gd[L + 1];
-#pragma unroll
-        for (int l = 0; l <= L; ++l) {
-          g[l] = h00 * g0[l] + h10 * g0[L + 1 + l] + h01 * g1[l] + h11 * g1[L + 1 + l];
-          gd[l] = d00 * g0[l] + d10 * g0[L + 1 + l] + d01 * g1[l] + d11 * g1[L + 1 + l];
-        }
-        float st_, ct_, sf_, cf_;
-        sincosf((float)thetas[p], &st_, &ct_);
-        sincosf((float)phis[p], &sf_, &cf_);
-        float er[L + 1], ei[L + 1];
-        er[0] = 1.f; ei[0] = 0.f;
-#pragma unroll
-        for (int m = 1; m <= L; ++m) {
-          er[m] = er[m - 1] * cf_ + ei[m - 1] * sf_;
-          ei[m] = ei[m - 1] * cf_ - er[m - 1] * sf_;
-        }
-        angular_rows<L>(ct_, st_, preflm, [&](int k, int l, int m, float pl, float dms, float dsin) {
+    if (live) {
+      float Rn[12], Rd[12];
+      radial_eval<12>(P, rtab, rjs[p], spmask[p], false, nlo, nhi, Rn, Rd);
+      const float* pr = Fp + lane * KS;
+      int k = 0;
+      for (int l = 0; l <= L; ++l) {
+        const float gl = Gl[lane * L1S + l], gdl = Gd[lane * L1S + l];
+        const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
+        for (int m = 0; m <= l; ++m, ++k) {
           float ur = 0.f, ui = 0.f, vr = 0.f, vi = 0.f;
 #pragma unroll
           for (int j = 0; j < 12; ++j) {
@@ -479,15 +509,26 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
               vr += Rd[j] * hh.x; vi += Rd[j] * hh.y;
             }
           }
-          const float e_r = er[m], e_i = ei[m];
+          const float2 e = Em[lane * L1S + m];
+          const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
+          const float ev = e.x * vr + e.y * vi;     // Re(E conj v)
+          const float ex = e.x * ui - e.y * ur;     // Re(i E conj u)
+          const float pf = preflm[k];
+          const float pl = pf * pr[kb + m];
+          // sin(t) dP/dcos(t) and -m P/sin(t) (src/soap_turbo_angular.f90:194-247)
+          const float pp1 = m < l ? pr[kb + m + 1] : 0.f;
+          float dms, dsin = 0.f;
+          if (m == 0) {
+            dms = -pp1;
+          } else {
+            dms = 0.5f * (float)((l + m) * (l - m + 1)) * pr[kb + m - 1] - 0.5f * pp1;
+            dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * pr[kb1 + m - 1] + 0.5f * pr[kb1 + m + 1];
+          }
           // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E
-          const float a = pl * g[l], ar = pl * gd[l], ap = dms * g[l], az = dsin * g[l];
-          const float Ar = a * e_r, Ai = a * e_i;
-          fr += Ar * vr + Ai * vi + ar * (e_r * ur + e_i * ui);
-          fo += ap * (e_r * ur + e_i * ui);
-          // A_azi = i * az * E = (-az e_i, az e_r)
-          fa += -az * e_i * ur + az * e_r * ui;
-        });
+          fr += pl * (gl * ev + gdl * eu);
+          fo += pf * dms * gl * eu;
+          fa += pf * dsin * gl * ex;
+        }
       }
     }
     fred[(wv * PT + lane) * 3] = fr;
@@ -513,8 +554,6 @@ __global__ __launch_bounds__(NTHREADS) void adjoint_site_k(
       pair_out[3 * (size_t)p + 1] = x1;
       pair_out[3 * (size_t)p + 2] = x2;
       self0 -= x0; self1 -= x1; self2 -= x2;
-    } else if (wv == 0 && p < p1) {
-      // beyond the cutoff or of no species: zero gradient, as the reference's zero columns give
     }
   }
   self0 = block_sum_f(wv == 0 ? self0 : 0.f, red);
@@ -548,12 +587,13 @@ __global__ void widen_k(size_t n, const float* __restrict__ a, double* __restric
   if (i < n) b[i] = (double)a[i];
 }
 
-size_t site_lds(int n_max, int k_max, int n_soap, int n_soap_unc) {
+size_t site_lds(int n_max, int l_max, int n_soap, int n_soap_unc) {
+  const int k_max = (l_max + 1) * (l_max + 2) / 2, KS = ((l_max + 2) * (l_max + 3) / 2) | 1, L1S = (l_max + 1) | 1;
   size_t c = sizeof(float2) * 3 * n_max * k_max;
   size_t fwd = sizeof(float) * PT * n_max + sizeof(float2) * PT * k_max;
-  size_t bwd = sizeof(float) * (2 * n_soap + n_soap_unc);
-  size_t red = sizeof(float) * 4 * PT * 3;
-  return c + std::max(std::max(fwd, bwd), red);
+  size_t spec = sizeof(float) * (2 * n_soap + n_soap_unc);
+  size_t bwd = sizeof(float) * PT * (KS + 2 * L1S) + sizeof(float2) * PT * L1S + sizeof(float) * 4 * PT * 3;
+  return c + std::max(std::max(fwd, spec), bwd);
 }
 
 template <int L>
@@ -589,6 +629,17 @@ void preflm_f(int L, std::vector<float>& out) {
 
 }  // namespace
 
+// Cubic coefficients of the Hermite interpolant on [x0, x0 + h] from the FP64
+// values and derivatives at both ends: f(t) = c0 + c1 t + c2 t^2 + c3 t^3.
+static void cell_cubic(double v0, double d0, double v1, double d1, double h, float* c0, float* c1, float* c2,
+                       float* c3) {
+  const double a = d0 * h, b = d1 * h;
+  *c0 = (float)v0;
+  *c1 = (float)a;
+  *c2 = (float)(3.0 * (v1 - v0) - 2.0 * a - b);
+  *c3 = (float)(2.0 * (v0 - v1) + a + b);
+}
+
 // Builds (or reuses) the FP32 tables; fills P.
 static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in, AdjParams& P, std::string& msg) {
   const st_params* p = in.p;
@@ -609,7 +660,7 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
   P.gauss = p->basis == ST_BASIS_POLY3GAUSS;
   double rcut_max = 0.0;
   for (int q = 0; q < ns; ++q) rcut_max = std::max(rcut_max, p->rcut_hard[q]);
-  P.rcut_max = (float)rcut_max;
+  P.rcut_max = rcut_max;
   // key: everything the tables depend on
   std::string key;
   auto add = [&](const void* x, size_t n) { key.append(static_cast<const char*>(x), n); };
@@ -620,9 +671,10 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
                             p->central_weight})
     add(arr, 8 * ns);
   add(in.W->data(), in.W->size() * 8);
-  // radial segments
-  int node_total = 0, rfloats = 0;
-  double seg_edge_lo[ADJ_MAX_SPECIES][MAXSEG], seg_edge_hi[ADJ_MAX_SPECIES][MAXSEG];
+  // radial segments: split where c_n(r) jumps (in A: the filter's 4-sigma cut at
+  // rcut_soft - r = 4 sigma(r), the origin Gaussian's at r = 4 (sigma_0 + sigma(r)))
+  std::vector<std::vector<double>> edges(ns);
+  int rfloats = 0;
   for (int q = 0; q < ns; ++q) {
     SpeciesTab& sp = P.sp[q];
     const double rc = p->rcut_hard[q];
@@ -637,89 +689,80 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
       if (b > 0.0 && b < rc) br.push_back(b);
     }
     std::sort(br.begin(), br.end());
-    std::vector<double> edges{0.0};
+    std::vector<double>& e = edges[q];
+    e.push_back(0.0);
     for (double b : br)
-      if (b - edges.back() > 1e-6 * rc) edges.push_back(b);
-    edges.push_back(rc);
+      if (b - e.back() > 1e-6 * rc) e.push_back(b);
+    e.push_back(rc);
     sp.a = p->alpha_max[q];
-    sp.rcut = (float)rc;
+    sp.rcut = rc;
     sp.cw = (float)p->central_weight[q];
     sp.do_central = P.gauss ? 0 : (p->central_weight[q] != 0.0);
-    sp.nseg = (int)edges.size() - 1;
+    sp.nseg = (int)e.size() - 1;
     sp.rtab_off = rfloats;
-    int base = 0;
+    int cells = 0;
     for (int j = 0; j < sp.nseg; ++j) {
-      // segment starts rounded up to float so that a float r >= seg_lo lies beyond the jump
-      float lo_f = (float)edges[j];
-      if ((double)lo_f < edges[j]) lo_f = std::nextafter(lo_f, 1e30f);
-      float hi_f = (float)edges[j + 1];
-      if (j + 1 < sp.nseg && (double)hi_f < edges[j + 1]) hi_f = std::nextafter(hi_f, 1e30f);
-      const double lo = j == 0 ? 0.0 : (double)lo_f, hi = (double)hi_f;
-      int nn = std::max(8, (int)std::ceil((hi - lo) / rc * NTAB) + 1);
-      sp.seg_lo[j] = (float)lo;
-      sp.seg_hi[j] = hi_f;
-      sp.seg_h[j] = (float)((hi - lo) / (nn - 1));
-      sp.seg_inv_h[j] = (float)((nn - 1) / (hi - lo));
-      sp.seg_base[j] = base;
+      const double lo = e[j], hi = e[j + 1];
+      const int nn = std::max(8, (int)std::ceil((hi - lo) / rc * NTAB) + 1);
+      sp.seg_lo[j] = lo;
+      sp.seg_inv_h[j] = (nn - 1) / (hi - lo);
+      sp.seg_ih[j] = (float)((nn - 1) / (hi - lo));
+      sp.seg_hi[j] = (float)hi;
+      sp.seg_base[j] = cells;
       sp.seg_n[j] = nn;
-      seg_edge_lo[q][j] = edges[j];
-      seg_edge_hi[q][j] = edges[j + 1];
-      base += nn;
+      cells += nn - 1;
     }
-    node_total += base;
-    rfloats += base * 2 * sp.a;
+    rfloats += cells * 4 * sp.a;
   }
   int off = 0;
   for (int q = 0; q < ns; ++q) { P.sp[q].off = off; off += p->alpha_max[q]; }
   P.ang_n = NTAB + 1;
-  P.ang_h = (float)(rcut_max / NTAB);
-  P.ang_inv_h = (float)(NTAB / rcut_max);
-  for (int q = 0; q < ns; ++q) P.sp[q].atab_off = q * P.ang_n * 2 * (L + 1);
+  P.ang_inv_h = NTAB / rcut_max;
+  P.ang_ih = (float)(NTAB / rcut_max);
+  for (int q = 0; q < ns; ++q) P.sp[q].atab_off = q * (P.ang_n - 1) * 4 * (L + 1);
   if (key == s.table_key) return ST_OK;
 
   int rc_;
-  // radial: FP64 radial kernel on the nodes, one species at a time
   std::vector<float> rt(rfloats);
   for (int q = 0; q < ns; ++q) {
     SpeciesTab& sp = P.sp[q];
     const double rc = p->rcut_hard[q];
+    const double eps = 1e-9 * rc;
+    const int a = sp.a;
     int nn = 0;
     for (int j = 0; j < sp.nseg; ++j) nn += sp.seg_n[j];
-    std::vector<double> r(nn);
-    const double eps = 1e-9 * rc;
+    // nodes, evaluated strictly inside their segment (beyond the jump below, before the one above)
+    std::vector<double> r(nn), hseg(sp.nseg);
+    int k0 = 0;
     for (int j = 0; j < sp.nseg; ++j) {
-      const double lo = (double)sp.seg_lo[j];
+      const double lo = edges[q][j], hi = edges[q][j + 1];
+      hseg[j] = (hi - lo) / (sp.seg_n[j] - 1);
       for (int k = 0; k < sp.seg_n[j]; ++k) {
-        double x = lo + (double)sp.seg_h[j] * 0.0 + ((double)sp.seg_hi[j] - lo) * k / (sp.seg_n[j] - 1);
-        // evaluate strictly inside the segment's true bounds: beyond the jump below, before the one above
-        if (j > 0) x = std::max(x, seg_edge_lo[q][j] + eps);
-        x = std::min(x, seg_edge_hi[q][j] - eps);
-        r[sp.seg_base[j] + k] = x;
+        double x = lo + (hi - lo) * k / (sp.seg_n[j] - 1);
+        if (j > 0) x = std::max(x, lo + eps);
+        x = std::min(x, hi - eps);
+        r[k0 + k] = x;
       }
+      k0 += sp.seg_n[j];
     }
     AdjBuf dr, dsp, dfirst, drad, dder;
     std::vector<uint32_t> m(nn, 1u);
     std::vector<uint8_t> f(nn, 0);
     bool ok = ensure(dr, nn * 8, msg) == 0 && ensure(dsp, nn * 4, msg) == 0 && ensure(dfirst, nn, msg) == 0 &&
-              ensure(drad, (size_t)nn * sp.a * 8, msg) == 0 && ensure(dder, (size_t)nn * sp.a * 8, msg) == 0;
+              ensure(drad, (size_t)nn * a * 8, msg) == 0 && ensure(dder, (size_t)nn * a * 8, msg) == 0;
+    std::vector<double> v((size_t)nn * a), d((size_t)nn * a);
     if (ok) {
       (void)hipMemcpyAsync(dr.p, r.data(), nn * 8, hipMemcpyHostToDevice, st);
       (void)hipMemcpyAsync(dsp.p, m.data(), nn * 4, hipMemcpyHostToDevice, st);
       (void)hipMemcpyAsync(dfirst.p, f.data(), nn, hipMemcpyHostToDevice, st);
-      launch_fill(st, drad.as<double>(), (size_t)nn * sp.a, 0.0);
-      launch_fill(st, dder.as<double>(), (size_t)nn * sp.a, 0.0);
+      launch_fill(st, drad.as<double>(), (size_t)nn * a, 0.0);
+      launch_fill(st, dder.as<double>(), (size_t)nn * a, 0.0);
       RadialConsts c = in.radial_tab[q];
       ok = launch_radial(st, c, nn, dr.as<double>(), dsp.as<uint32_t>(), dfirst.as<uint8_t>(), drad.as<double>(),
                          dder.as<double>());
-      std::vector<double> v((size_t)nn * sp.a), d((size_t)nn * sp.a);
       (void)hipMemcpyAsync(v.data(), drad.p, v.size() * 8, hipMemcpyDeviceToHost, st);
       (void)hipMemcpyAsync(d.data(), dder.p, d.size() * 8, hipMemcpyDeviceToHost, st);
       if (hipStreamSynchronize(st) != hipSuccess) { msg = "radial table build failed"; ok = false; }
-      for (int k = 0; k < nn && ok; ++k)
-        for (int n = 0; n < sp.a; ++n) {
-          rt[sp.rtab_off + (size_t)k * 2 * sp.a + n] = (float)v[(size_t)k * sp.a + n];
-          rt[sp.rtab_off + (size_t)k * 2 * sp.a + sp.a + n] = (float)d[(size_t)k * sp.a + n];
-        }
     }
     for (AdjBuf* b : {&dr, &dsp, &dfirst, &drad, &dder})
       if (b->p) (void)hipFree(b->p);
@@ -727,27 +770,44 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
       if (msg.empty()) msg = "radial table build failed";
       return ST_ERR_HIP;
     }
+    k0 = 0;
+    for (int j = 0; j < sp.nseg; ++j) {
+      for (int k = 0; k + 1 < sp.seg_n[j]; ++k) {
+        float* cc = rt.data() + sp.rtab_off + (size_t)(sp.seg_base[j] + k) * 4 * a;
+        const size_t n0 = (size_t)(k0 + k) * a, n1 = n0 + a;
+        for (int n = 0; n < a; ++n)
+          cell_cubic(v[n0 + n], d[n0 + n], v[n1 + n], d[n1 + n], hseg[j], cc + n, cc + a + n, cc + 2 * a + n,
+                     cc + 3 * a + n);
+      }
+      k0 += sp.seg_n[j];
+    }
   }
   if ((rc_ = ensure(s.rtab, rt.size() * 4, msg))) return rc_;
   (void)hipMemcpyAsync(s.rtab.p, rt.data(), rt.size() * 4, hipMemcpyHostToDevice, st);
-  // angular radial factors
-  const size_t af = (size_t)ns * P.ang_n * 2 * (L + 1);
-  std::vector<float> at(af);
+  // angular radial factors, uniform grid on [0, rcut_max]
+  const int L1 = L + 1, ncell = P.ang_n - 1;
+  std::vector<float> at((size_t)ns * ncell * 4 * L1);
   {
     AdjBuf dout;
-    if ((rc_ = ensure(dout, (size_t)P.ang_n * 2 * (L + 1) * 8, msg))) return rc_;
-    std::vector<double> h((size_t)P.ang_n * 2 * (L + 1));
+    if ((rc_ = ensure(dout, (size_t)P.ang_n * 2 * L1 * 8, msg))) return rc_;
+    std::vector<double> h((size_t)P.ang_n * 2 * L1);
+    const double hh = rcut_max / NTAB;
     for (int q = 0; q < ns; ++q) {
-      ang_table_k<<<(P.ang_n + 127) / 128, 128, 0, st>>>(P.ang_n, rcut_max / NTAB, L, rcut_max * rcut_max,
-                                                          p->atom_sigma_t[q], p->atom_sigma_t_scaling[q],
-                                                          dout.as<double>());
+      ang_table_k<<<(P.ang_n + 127) / 128, 128, 0, st>>>(P.ang_n, hh, L, rcut_max * rcut_max, p->atom_sigma_t[q],
+                                                          p->atom_sigma_t_scaling[q], dout.as<double>());
       (void)hipMemcpyAsync(h.data(), dout.p, h.size() * 8, hipMemcpyDeviceToHost, st);
       if (hipStreamSynchronize(st) != hipSuccess) {
         (void)hipFree(dout.p);
         msg = "angular table build failed";
         return ST_ERR_HIP;
       }
-      for (size_t k = 0; k < h.size(); ++k) at[P.sp[q].atab_off + k] = (float)h[k];
+      for (int k = 0; k < ncell; ++k) {
+        float* cc = at.data() + P.sp[q].atab_off + (size_t)k * 4 * L1;
+        const double* a0 = h.data() + (size_t)k * 2 * L1;
+        const double* a1 = a0 + 2 * L1;
+        for (int l = 0; l < L1; ++l)
+          cell_cubic(a0[l], a0[L1 + l], a1[l], a1[L1 + l], hh, cc + l, cc + L1 + l, cc + 2 * L1 + l, cc + 3 * L1 + l);
+      }
     }
     (void)hipFree(dout.p);
   }
@@ -774,7 +834,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   int rc;
   timer("adj_tables", [&] { rc = build_tables(s, st, in, P, msg); });
   if (rc) return rc;
-  const size_t sh = site_lds(in.n_max, in.k_max, in.n_soap, in.n_soap_unc);
+  const size_t sh = site_lds(in.n_max, in.l_max, in.n_soap, in.n_soap_unc);
   if (sh > 64 * 1024) {
     msg = "the FP32 site working set exceeds the LDS";
     return ST_ERR_UNSUPPORTED;

@@ -500,6 +500,16 @@ const char* st_error(const st_ctx* c) { return c ? c->err.c_str() : "no context"
 int st_n_soap(const st_params* p, const st_compression* c) { return p ? n_soap_of(p, c) : -1; }
 
 int st_get_basis(st_ctx* ctx, const st_params* p, double* W, double* S) {
+  if (!ctx) {
+    // host-only: build the basis without a device context
+    st_ctx tmp;
+    int rc = check_params(&tmp, p);
+    if (!rc) rc = basis_for(&tmp, p);
+    if (rc) return rc;
+    if (W) std::memcpy(W, tmp.W.data(), tmp.W.size() * 8);
+    if (S) std::memcpy(S, tmp.S.data(), tmp.S.size() * 8);
+    return ST_OK;
+  }
   int rc = check_params(ctx, p);
   if (rc) return rc;
   if ((rc = basis_for(ctx, p))) return rc;

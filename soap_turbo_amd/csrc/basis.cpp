@@ -7,7 +7,11 @@
 // factorisation (dpotrf/dpotri). The same steps are taken here with a cyclic
 // Jacobi eigensolver (S is symmetric positive definite, so its SVD is its
 // eigendecomposition) and a hand-written Cholesky inverse, in FP64, on the
-// host: the matrices are at most 12 x 12 and are built once per basis.
+// host: the matrices are at most 12 x 12 and are built once per basis. The
+// decomposition runs in x87 extended precision (64-bit mantissa): at poly3
+// alpha_max = 8, cond(S) = 1.3e13 and a double-precision route loses ~1e-4 of
+// W's accuracy; in extended precision W matches S^{-1/2} to ~1e-9, and only
+// the final rounding to double remains.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -54,35 +58,37 @@ void overlap_matrix(int basis, int a, double atom_sigma_r, double rcut_hard, dou
   S[(a - 1) + (a - 1) * a] = 1.0;
 }
 
+using real_x = long double;
+
 // Cyclic Jacobi: A (column-major, symmetric) -> eigenvalues w, eigenvectors V.
-static void jacobi_eigen(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
-  V.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) V[i + i * n] = 1.0;
+static void jacobi_eigen(int n, std::vector<real_x> A, std::vector<real_x>& w, std::vector<real_x>& V) {
+  V.assign((size_t)n * n, 0.0L);
+  for (int i = 0; i < n; ++i) V[i + i * n] = 1.0L;
   for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, diag = 0.0;
+    real_x off = 0.0L, diag = 0.0L;
     for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) (i == j ? diag : off) += A[i + j * n] * A[i + j * n];
-    if (off <= 1e-34 * diag) break;
+    if (off <= 1e-40L * diag) break;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
-        double apq = A[p + q * n];
-        if (apq == 0.0) continue;
-        double app = A[p + p * n], aqq = A[q + q * n];
-        double theta = (aqq - app) / (2.0 * apq);
-        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        real_x apq = A[p + q * n];
+        if (apq == 0.0L) continue;
+        real_x app = A[p + p * n], aqq = A[q + q * n];
+        real_x theta = (aqq - app) / (2.0L * apq);
+        real_x t = (theta >= 0 ? 1.0L : -1.0L) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0L));
+        real_x c = 1.0L / std::sqrt(t * t + 1.0L), s = t * c;
         for (int k = 0; k < n; ++k) {  // A <- A J (columns p, q)
-          double akp = A[k + p * n], akq = A[k + q * n];
+          real_x akp = A[k + p * n], akq = A[k + q * n];
           A[k + p * n] = c * akp - s * akq;
           A[k + q * n] = s * akp + c * akq;
         }
         for (int k = 0; k < n; ++k) {  // A <- J^T A (rows p, q)
-          double apk = A[p + k * n], aqk = A[q + k * n];
+          real_x apk = A[p + k * n], aqk = A[q + k * n];
           A[p + k * n] = c * apk - s * aqk;
           A[q + k * n] = s * apk + c * aqk;
         }
         for (int k = 0; k < n; ++k) {
-          double vkp = V[k + p * n], vkq = V[k + q * n];
+          real_x vkp = V[k + p * n], vkq = V[k + q * n];
           V[k + p * n] = c * vkp - s * vkq;
           V[k + q * n] = s * vkp + c * vkq;
         }
@@ -96,48 +102,44 @@ static void jacobi_eigen(int n, std::vector<double> A, std::vector<double>& w, s
 // S^{1/2} = U^T U (Cholesky, upper) as dpotrf/dpotri do. Returns false if S
 // is not positive definite.
 bool orthonormalization_matrix(int a, const double* S, double* W) {
-  std::vector<double> A(S, S + (size_t)a * a), w, V;
+  std::vector<real_x> A(S, S + (size_t)a * a), w, V;
   jacobi_eigen(a, A, w, V);
-  std::vector<double> H((size_t)a * a, 0.0);
-  for (int i = 0; i < a; ++i) {
-    if (!(w[i] > 0.0)) return false;
-  }
+  std::vector<real_x> H((size_t)a * a, 0.0L);
+  for (int i = 0; i < a; ++i)
+    if (!(w[i] > 0.0L)) return false;
   for (int j = 0; j < a; ++j)
     for (int i = 0; i < a; ++i) {
-      double s = 0.0;
+      real_x s = 0.0L;
       for (int k = 0; k < a; ++k) s += V[i + k * a] * std::sqrt(w[k]) * V[j + k * a];
       H[i + j * a] = s;
     }
-  // Cholesky H = U^T U, U upper
-  std::vector<double> U((size_t)a * a, 0.0);
+  std::vector<real_x> U((size_t)a * a, 0.0L);
   for (int j = 0; j < a; ++j) {
-    double d = H[j + j * a];
+    real_x d = H[j + j * a];
     for (int k = 0; k < j; ++k) d -= U[k + j * a] * U[k + j * a];
-    if (!(d > 0.0)) return false;
+    if (!(d > 0.0L)) return false;
     d = std::sqrt(d);
     U[j + j * a] = d;
     for (int i = j + 1; i < a; ++i) {
-      double s = H[j + i * a];
+      real_x s = H[j + i * a];
       for (int k = 0; k < j; ++k) s -= U[k + j * a] * U[k + i * a];
       U[j + i * a] = s / d;
     }
   }
-  // Ui = U^{-1} (upper triangular)
-  std::vector<double> Ui((size_t)a * a, 0.0);
+  std::vector<real_x> Ui((size_t)a * a, 0.0L);
   for (int j = 0; j < a; ++j) {
-    Ui[j + j * a] = 1.0 / U[j + j * a];
+    Ui[j + j * a] = 1.0L / U[j + j * a];
     for (int i = j - 1; i >= 0; --i) {
-      double s = 0.0;
+      real_x s = 0.0L;
       for (int k = i + 1; k <= j; ++k) s += U[i + k * a] * Ui[k + j * a];
       Ui[i + j * a] = -s / U[i + i * a];
     }
   }
-  // H^{-1} = Ui Ui^T
   for (int j = 0; j < a; ++j)
     for (int i = 0; i < a; ++i) {
-      double s = 0.0;
+      real_x s = 0.0L;
       for (int k = (i > j ? i : j); k < a; ++k) s += Ui[i + k * a] * Ui[j + k * a];
-      W[i + j * a] = s;
+      W[i + j * a] = (double)s;
     }
   return true;
 }

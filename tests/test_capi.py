@@ -73,3 +73,31 @@ def test_n_soap():
                        atom_sigma_t=[0.5] * 3, atom_sigma_t_scaling=[0.0] * 3, amplitude_scaling=[1.0] * 3,
                        radial_enhancement=1, central_weight=[1.0] * 3, basis="poly3gauss", scaling_mode="polynomial")
     assert _lib.lib().st_n_soap(ctypes.byref(p), None) == 24 * 25 // 2 * 9
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_own_basis_is_the_exact_inverse_square_root(name):
+    # W = S^{-1/2} from the library's extended-precision eigen-route (host only, no GPU) against a
+    # 50-digit mpmath S^{-1/2}, and against the reference's LAPACK W, which is itself only as
+    # accurate as cond(S) allows (2.8e-6 at poly3 alpha_max = 8, cond(S) = 1.3e13)
+    import mpmath as mp
+    w, comp, d = load(name)
+    p = w.params
+    keep = _lib.Keep()
+    prm = st.make_params(keep, alpha_max=p.alpha_max, l_max=p.l_max, radial_enhancement=p.radial_enhancement,
+                         basis=p.basis, scaling_mode=p.scaling_mode, **p.per_species())
+    n = sum(p.alpha_max)
+    W = np.zeros((n, n), order="F"); S = np.zeros((n, n), order="F")
+    assert _lib.lib().st_get_basis(None, ctypes.byref(prm), _lib.ptr(W), _lib.ptr(S)) == 0
+    np.testing.assert_array_equal(S, d["S"])
+    mp.mp.dps = 50
+    Wex = np.zeros((n, n))
+    o = 0
+    for a in p.alpha_max:
+        E, Q = mp.eigsy(mp.matrix(S[o:o + a, o:o + a].tolist()))
+        Wex[o:o + a, o:o + a] = np.array((Q * mp.diag([1 / mp.sqrt(E[i]) for i in range(a)]) * Q.T).tolist(),
+                                         dtype=float)
+        o += a
+    from tests.golden_cases import rel
+    assert rel(W, Wex) <= 1e-7
+    assert rel(W, d["W"]) <= 1e-5

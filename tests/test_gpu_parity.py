@@ -77,21 +77,6 @@ def test_hip_matches_reference_fixtures(ctx, name):
         ctx.clear_basis()
 
 
-@pytest.mark.parametrize("name", ["cfg1_legacy", "cfg2_unified", "var_poly3gauss_amp0", "var_poly3_scalings"])
-def test_own_basis_close_to_reference_basis(ctx, name):
-    # W = S^{-1/2} from the library's own FP64 eigen-route vs the reference's LAPACK route:
-    # they agree as far as the conditioning of S allows
-    w, comp, d = load(name)
-    keep = st.Keep() if hasattr(st, "Keep") else st._lib.Keep()
-    prm = _params(keep, w, bool(d["legacy_seed"]))
-    n_max = sum(w.params.alpha_max)
-    W, S = ctx.get_basis(prm, n_max)
-    assert rel(S, d["S"]) <= 1e-15
-    tol = 1e-4 if max(w.params.alpha_max) >= 8 and w.params.basis == "poly3" else 1e-6
-    assert rel(W, d["W"]) <= tol
-    assert np.abs(W @ S @ W - np.eye(n_max)).max() <= 1e-5
-
-
 @pytest.mark.parametrize("cfg", [1, 2])
 def test_hip_matches_oracle_full_config(ctx, cfg):
     w = make_workload(cfg)
