@@ -487,11 +487,27 @@ __global__ void normalise_k(int n_sites, int n_soap, const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------- full Jacobian
-// One workgroup per site, neighbours in order: the four contractions over m
-// (soap_derivatives_from_expansion, src/soap_turbo.f90:1996-2025), the
+// One workgroup per site, neighbours in batches of PB: the four contractions
+// over m (soap_derivatives_from_expansion, src/soap_turbo.f90:1996-2025), the
 // assembly of the surviving components (:2027-2066), then normalisation and
 // rotation to Cartesian with the self column accumulated as minus the sum of
-// the others (:837-895). soap holds the unnormalised spectrum here.
+// the others in neighbour order (:837-895). soap holds the unnormalised
+// spectrum here. Each thread owns the outputs s = tid + u * 256 for the whole
+// site; a batch costs four barriers.
+constexpr int DER_PB = 4;
+
+__device__ __forceinline__ void der_component(const int4 cm, const double* __restrict__ P, const double* __restrict__ Q,
+                                              const double* __restrict__ Qa, const double* __restrict__ Qo,
+                                              const double* __restrict__ r, const double* __restrict__ rd, int n_max,
+                                              double& tr, double& ta, double& to) {
+  const int n = cm.x, np = cm.y, l = cm.z;
+  const double pre = (4.0 * 3.141592653589793) * (n == np ? 1.0 : 1.4142135623730951);
+  tr = pre * (rd[n] * P[l * n_max + np] + r[n] * Q[l * n_max + np] + rd[np] * P[l * n_max + n] +
+              r[np] * Q[l * n_max + n]);
+  ta = pre * (r[n] * Qa[l * n_max + np] + r[np] * Qa[l * n_max + n]);
+  to = pre * (r[n] * Qo[l * n_max + np] + r[np] * Qo[l * n_max + n]);
+}
+
 template <int DER_NSPT>
 __global__ __launch_bounds__(256) void derivatives_k(
     int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* __restrict__ site_ptr,
@@ -502,18 +518,19 @@ __global__ __launch_bounds__(256) void derivatives_k(
     const double* __restrict__ sqrt_dot_p, const double* __restrict__ rjs, const double* __restrict__ thetas,
     const double* __restrict__ phis, double* __restrict__ cart) {
   extern __shared__ double2 smem2[];
-  __shared__ double red[16];
+  __shared__ double red[4][3 * DER_PB];
+  __shared__ double rot[DER_PB][8];
   const int i = blockIdx.x;
   const int nl = n_max * (l_max + 1);
-  double2* cn = smem2;                          // n_max * k_max
-  double2* sA = cn + n_max * k_max;             // 4 * k_max
-  double* sR = (double*)(sA + 4 * k_max);       // 2 * n_max
-  double* PQ = sR + 2 * n_max;                  // 4 * n_max * (l_max + 1), [kind][l][n]
+  double2* cn = smem2;                                   // n_max * k_max
+  double2* sA = cn + n_max * k_max;                      // PB * 4 * k_max
+  double* sR = (double*)(sA + DER_PB * 4 * k_max);       // PB * 2 * n_max
+  double* PQ = sR + DER_PB * 2 * n_max;                  // PB * 4 * nl, [b][kind][l][n]
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) cn[t] = cnk[(size_t)i * n_max * k_max + t];
   const double sdp = sqrt_dot_p[i];
   const double inv_p = 1.0 / sdp, inv_p3 = 1.0 / (sdp * sdp * sdp);
-  const double fourpi = 4.0 * 3.141592653589793, sq2 = 1.4142135623730951;
   double selfacc[DER_NSPT][3];
   double qv[DER_NSPT];
 #pragma unroll
@@ -522,87 +539,123 @@ __global__ __launch_bounds__(256) void derivatives_k(
     const int s = threadIdx.x + u * blockDim.x;
     qv[u] = s < n_soap ? soap[(size_t)i * n_soap + s] : 0.0;
   }
-  for (int p = p0 + 1; p < p1; ++p) {
+  for (int pb = p0 + 1; pb < p1; pb += DER_PB) {
+    const int nb = min(DER_PB, p1 - pb);
     __syncthreads();
-    for (int t = threadIdx.x; t < 4 * k_max; t += blockDim.x) {
-      const int kind = t / k_max, k = t - kind * k_max;
+    for (int t = threadIdx.x; t < nb * 4 * k_max; t += blockDim.x) {
+      const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
       const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
-      sA[t] = src[(size_t)p * k_max + k];
+      sA[t] = src[(size_t)(pb + b) * k_max + k];
     }
-    for (int t = threadIdx.x; t < 2 * n_max; t += blockDim.x)
-      sR[t] = t < n_max ? rad[(size_t)p * n_max + t] : rad_der[(size_t)p * n_max + t - n_max];
+    for (int t = threadIdx.x; t < nb * 2 * n_max; t += blockDim.x) {
+      const int b = t / (2 * n_max), rem = t - b * 2 * n_max;
+      sR[t] = rem < n_max ? rad[(size_t)(pb + b) * n_max + rem] : rad_der[(size_t)(pb + b) * n_max + rem - n_max];
+    }
+    if (threadIdx.x < nb) {
+      const int p = pb + threadIdx.x;
+      const double sin_t = sin(thetas[p]), cos_t = cos(thetas[p]);
+      const double sin_f = sin(phis[p]), cos_f = cos(phis[p]);
+      const double inv_r = 1.0 / rjs[p];
+      double* rr = rot[threadIdx.x];
+      rr[0] = sin_t * cos_f; rr[1] = sin_t * sin_f; rr[2] = cos_t;
+      rr[3] = -(cos_t * cos_f * inv_r); rr[4] = -(cos_t * sin_f * inv_r); rr[5] = sin_t * inv_r;
+      rr[6] = -(sin_f * inv_r); rr[7] = cos_f * inv_r;
+    }
     __syncthreads();
-    for (int t = threadIdx.x; t < 4 * nl; t += blockDim.x) {
-      const int kind = t / nl, r = t - kind * nl, l = r / n_max, n = r - l * n_max;
+    for (int t = threadIdx.x; t < nb * 4 * nl; t += blockDim.x) {
+      const int b = t / (4 * nl), rem = t - b * 4 * nl;
+      const int kind = rem / nl, r = rem - kind * nl, l = r / n_max, n = r - l * n_max;
       const int kb = l * (l + 1) / 2;
-      const double2* a = sA + kind * k_max;
+      const double2* a = sA + (b * 4 + kind) * k_max;
       double acc = 0.0;
       for (int m = 0; m <= l; ++m) {
-        const double w = m == 0 ? 1.0 : 2.0;
+        const double wm = m == 0 ? 1.0 : 2.0;
         const double2 cv = cn[n * k_max + kb + m], av = a[kb + m];
-        acc = acc + w * (av.x * cv.x + av.y * cv.y);
+        acc = acc + wm * (av.x * cv.x + av.y * cv.y);
       }
       PQ[t] = acc;
     }
     __syncthreads();
-    const double* P = PQ;
-    const double* Q = PQ + nl;
-    const double* Qa = PQ + 2 * nl;
-    const double* Qo = PQ + 3 * nl;
-    const double* r = sR;
-    const double* rd = sR + n_max;
-    double dv[DER_NSPT][3];
-    double dr_ = 0.0, da_ = 0.0, do_ = 0.0;
+    // pass 1: q . d for every pair of the batch
+    double dots[DER_PB][3];
+#pragma unroll
+    for (int b = 0; b < DER_PB; ++b) dots[b][0] = dots[b][1] = dots[b][2] = 0.0;
 #pragma unroll
     for (int u = 0; u < DER_NSPT; ++u) {
       const int s = threadIdx.x + u * blockDim.x;
-      double vr = 0.0, va = 0.0, vo = 0.0;
       if (s < n_soap) {
         const int e0 = out_ptr ? out_ptr[s] : s, e1 = out_ptr ? out_ptr[s + 1] : s + 1;
-        for (int e = e0; e < e1; ++e) {
-          const int4 cm = comp[out_ptr ? out_c[e] : e];
-          const int n = cm.x, np = cm.y, l = cm.z;
-          const double pre = fourpi * (n == np ? 1.0 : sq2);
-          const double tr = pre * (rd[n] * P[l * n_max + np] + r[n] * Q[l * n_max + np] + rd[np] * P[l * n_max + n] +
-                                   r[np] * Q[l * n_max + n]);
-          const double ta = pre * (r[n] * Qa[l * n_max + np] + r[np] * Qa[l * n_max + n]);
-          const double to = pre * (r[n] * Qo[l * n_max + np] + r[np] * Qo[l * n_max + n]);
-          if (out_ptr) {
-            const double el = out_el[e];
-            vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
-          } else {
-            vr = tr; va = ta; vo = to;
+#pragma unroll
+        for (int b = 0; b < DER_PB; ++b) {
+          if (b < nb) {
+            const double* P = PQ + b * 4 * nl;
+            double vr = 0.0, va = 0.0, vo = 0.0;
+            for (int e = e0; e < e1; ++e) {
+              double tr, ta, to;
+              der_component(comp[out_ptr ? out_c[e] : e], P, P + nl, P + 2 * nl, P + 3 * nl, sR + b * 2 * n_max,
+                            sR + b * 2 * n_max + n_max, n_max, tr, ta, to);
+              if (out_ptr) {
+                const double el = out_el[e];
+                vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
+              } else {
+                vr = tr; va = ta; vo = to;
+              }
+            }
+            dots[b][0] += qv[u] * vr; dots[b][1] += qv[u] * va; dots[b][2] += qv[u] * vo;
           }
         }
-        dr_ += qv[u] * vr; da_ += qv[u] * va; do_ += qv[u] * vo;
       }
-      dv[u][0] = vr; dv[u][1] = va; dv[u][2] = vo;
     }
-    const double dot_r = block_sum(dr_, red);
-    const double dot_a = block_sum(da_, red + 4);
-    const double dot_o = block_sum(do_, red + 8);
-    const double sin_t = sin(thetas[p]), cos_t = cos(thetas[p]);
-    const double sin_f = sin(phis[p]), cos_f = cos(phis[p]);
-    const double inv_r = 1.0 / rjs[p];
-    const double cx = sin_t * cos_f, cy = sin_t * sin_f, cz = cos_t;
-    const double px = -(cos_t * cos_f * inv_r), py = -(cos_t * sin_f * inv_r), pz = sin_t * inv_r;
-    const double ax = -(sin_f * inv_r), ay = cos_f * inv_r;
 #pragma unroll
-    for (int u = 0; u < DER_NSPT; ++u) {
-      const int s = threadIdx.x + u * blockDim.x;
-      if (s < n_soap) {
-        const double sv = qv[u];
-        const double vr = dv[u][0] * inv_p - sv * inv_p3 * dot_r;
-        const double va = dv[u][1] * inv_p - sv * inv_p3 * dot_a;
-        const double vo = dv[u][2] * inv_p - sv * inv_p3 * dot_o;
-        const double c0 = cx * vr + px * vo + ax * va;
-        const double c1 = cy * vr + py * vo + ay * va;
-        const double c2 = cz * vr + pz * vo;
-        double* C = cart + ((size_t)p * n_soap + s) * 3;
-        C[0] = c0; C[1] = c1; C[2] = c2;
-        selfacc[u][0] = selfacc[u][0] - c0;
-        selfacc[u][1] = selfacc[u][1] - c1;
-        selfacc[u][2] = selfacc[u][2] - c2;
+    for (int b = 0; b < DER_PB; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double v = dots[b][c];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[w][3 * b + c] = v;
+      }
+    __syncthreads();
+    // pass 2: normalise, rotate, store, accumulate the self column
+#pragma unroll
+    for (int b = 0; b < DER_PB; ++b) {
+      if (b < nb) {
+        const double dot_r = red[0][3 * b] + red[1][3 * b] + red[2][3 * b] + red[3][3 * b];
+        const double dot_a = red[0][3 * b + 1] + red[1][3 * b + 1] + red[2][3 * b + 1] + red[3][3 * b + 1];
+        const double dot_o = red[0][3 * b + 2] + red[1][3 * b + 2] + red[2][3 * b + 2] + red[3][3 * b + 2];
+        const double* rr = rot[b];
+        const double* P = PQ + b * 4 * nl;
+        const size_t p = pb + b;
+#pragma unroll
+        for (int u = 0; u < DER_NSPT; ++u) {
+          const int s = threadIdx.x + u * blockDim.x;
+          if (s < n_soap) {
+            const int e0 = out_ptr ? out_ptr[s] : s, e1 = out_ptr ? out_ptr[s + 1] : s + 1;
+            double vr = 0.0, va = 0.0, vo = 0.0;
+            for (int e = e0; e < e1; ++e) {
+              double tr, ta, to;
+              der_component(comp[out_ptr ? out_c[e] : e], P, P + nl, P + 2 * nl, P + 3 * nl, sR + b * 2 * n_max,
+                            sR + b * 2 * n_max + n_max, n_max, tr, ta, to);
+              if (out_ptr) {
+                const double el = out_el[e];
+                vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
+              } else {
+                vr = tr; va = ta; vo = to;
+              }
+            }
+            const double sv = qv[u];
+            vr = vr * inv_p - sv * inv_p3 * dot_r;
+            va = va * inv_p - sv * inv_p3 * dot_a;
+            vo = vo * inv_p - sv * inv_p3 * dot_o;
+            const double c0 = rr[0] * vr + rr[3] * vo + rr[6] * va;
+            const double c1 = rr[1] * vr + rr[4] * vo + rr[7] * va;
+            const double c2 = rr[2] * vr + rr[5] * vo;
+            double* C = cart + (p * n_soap + s) * 3;
+            C[0] = c0; C[1] = c1; C[2] = c2;
+            selfacc[u][0] = selfacc[u][0] - c0;
+            selfacc[u][1] = selfacc[u][1] - c1;
+            selfacc[u][2] = selfacc[u][2] - c2;
+          }
+        }
       }
     }
   }
@@ -618,25 +671,29 @@ __global__ __launch_bounds__(256) void derivatives_k(
 
 // pair(:,k) = sum_s G(i,s) soap_cart_der(:,s,k): what a caller of the
 // full-Jacobian get_soap does to turn dE/dq into pair gradients. One
-// wavefront per pair.
+// workgroup per site holds G(i,:) in LDS; one wavefront per pair.
 __global__ __launch_bounds__(256) void contract_cart_k(int n_sites, int n_pairs, int n_soap,
                                                        const int* __restrict__ site_ptr,
                                                        const double* __restrict__ G, const double* __restrict__ cart,
                                                        double* __restrict__ pair) {
-  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (p >= n_pairs) return;
-  const int i = site_of(site_ptr, n_sites, p);
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  const double* C = cart + (size_t)p * n_soap * 3;
-  for (int s = lane; s < n_soap; s += 64) {
-    const double g = G[(size_t)s * n_sites + i];
-    a0 += g * C[3 * s]; a1 += g * C[3 * s + 1]; a2 += g * C[3 * s + 2];
+  extern __shared__ double gs[];
+  const int i = blockIdx.x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int s = threadIdx.x; s < n_soap; s += blockDim.x) gs[s] = G[(size_t)s * n_sites + i];
+  __syncthreads();
+  const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  for (int p = p0 + w; p < p1; p += 4) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const double* C = cart + (size_t)p * n_soap * 3;
+    for (int s = lane; s < n_soap; s += 64) {
+      const double g = gs[s];
+      a0 += g * C[3 * s]; a1 += g * C[3 * s + 1]; a2 += g * C[3 * s + 2];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      a0 += __shfl_xor(a0, o, 64); a1 += __shfl_xor(a1, o, 64); a2 += __shfl_xor(a2, o, 64);
+    }
+    if (lane == 0) { pair[3 * (size_t)p] = a0; pair[3 * (size_t)p + 1] = a1; pair[3 * (size_t)p + 2] = a2; }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    a0 += __shfl_xor(a0, o, 64); a1 += __shfl_xor(a1, o, 64); a2 += __shfl_xor(a2, o, 64);
-  }
-  if (lane == 0) { pair[3 * (size_t)p] = a0; pair[3 * (size_t)p + 1] = a1; pair[3 * (size_t)p + 2] = a2; }
 }
 
 // ---------------------------------------------------------------- soap_contract
@@ -813,7 +870,8 @@ void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqr
 }
 
 size_t derivatives_lds(int n_max, int l_max, int k_max) {
-  return sizeof(double2) * (n_max * k_max + 4 * k_max) + sizeof(double) * (2 * n_max + 4 * n_max * (l_max + 1));
+  return sizeof(double2) * (n_max * k_max + DER_PB * 4 * k_max) +
+         sizeof(double) * DER_PB * (2 * n_max + 4 * n_max * (l_max + 1));
 }
 
 bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* site_ptr,
@@ -839,8 +897,8 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k
 
 void launch_contract_cart(hipStream_t st, int n_sites, int n_pairs, int n_soap, const int* site_ptr, const double* G,
                           const double* cart, double* pair) {
-  if (n_pairs == 0) return;
-  contract_cart_k<<<(n_pairs + 3) / 4, 256, 0, st>>>(n_sites, n_pairs, n_soap, site_ptr, G, cart, pair);
+  if (n_pairs == 0 || n_sites == 0) return;
+  contract_cart_k<<<n_sites, 256, sizeof(double) * n_soap, st>>>(n_sites, n_pairs, n_soap, site_ptr, G, cart, pair);
 }
 
 size_t contract_lds(int n_max, int k_max, int n_soap, int n_soap_unc) {
