@@ -272,7 +272,27 @@ __device__ __forceinline__ void angular_factors(const AdjParams& P, const float*
   }
 }
 
-// One site per workgroup of 256 threads (4 wavefronts).
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x4f32: D(16x16) += A(16x4) B(4x16). Lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16]; it holds D[4 (l / 16) + v][l % 16] in register v.
+__device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// hi + lo += x without losing the rounding error of the add (TwoSum).
+__device__ __forceinline__ void two_sum_acc(floatx4& hi, floatx4& lo, floatx4 x) {
+  const floatx4 s = hi + x;
+  const floatx4 bp = s - hi;
+  lo += (hi - (s - bp)) + (x - bp);
+  hi = s;
+}
+
+constexpr int MAX_TILES_PER_WAVE = 9;  // forward cnk tiles per wavefront: n_max <= 48, l_max <= 12
+
+// One site per workgroup of 256 threads (4 wavefronts). The GEMM-shaped steps
+// run on FP32 MFMA: cnk = sum_pairs 4 pi R^T A, the per-l Hermitian products
+// S_l = C_l W C_l^T of the spectrum, H_l = Y_l C_l, and U = R H, V = R' H.
 template <int L>
 __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
     AdjParams P, const int* __restrict__ order, const int* __restrict__ site_ptr, const float* __restrict__ rtab,
@@ -286,57 +306,77 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
     float* __restrict__ pair_out) {
   constexpr int KM = (L + 1) * (L + 2) / 2;
   constexpr int KM1 = (L + 2) * (L + 3) / 2;
-  constexpr int KS = KM1 | 1;       // odd row stride of the per-pair P_l^m rows
+  constexpr int KS = KM1 | 1;                    // odd row stride of the per-pair P_l^m rows
   constexpr int L1 = L + 1;
   constexpr int L1S = L1 | 1;
+  constexpr int CSTR = ((2 * KM + 15) / 16) * 16; // columns of C and H: 2k + {re, im}, padded
+  constexpr int CT = CSTR / 16;
   extern __shared__ float smem[];
   __shared__ float red[16];
   __shared__ float preflm[KM];
+  __shared__ int klm[KM];
   const int i = order ? order[blockIdx.x] : blockIdx.x;
-  const int n_max = P.n_max, k_max = KM;
+  const int n_max = P.n_max;
+  const int NT = (n_max + 15) / 16;
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const float fourpi = 12.566370614359172f;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float2* C = reinterpret_cast<float2*>(smem);                 // n_max * k_max
-  float2* Clo = C + n_max * k_max;                             // Kahan compensation
-  float2* H = Clo + n_max * k_max;                             // n_max * k_max
-  float* region = reinterpret_cast<float*>(H + n_max * k_max);
-  // forward tile
-  float* Rt = region;                                          // PT * n_max
-  float2* At = reinterpret_cast<float2*>(Rt + PT * n_max);     // PT * k_max
-  // backward tile
-  float* Fp = region;                                          // PT * KS: P_l^m, l <= L + 1
-  float* Gl = Fp + PT * KS;                                    // PT * L1S: g_l
-  float* Gd = Gl + PT * L1S;                                   // PT * L1S: dg_l/dr
-  float2* Em = reinterpret_cast<float2*>(Gd + PT * L1S);       // PT * L1S: e^{-i m phi}
-  float* fred = reinterpret_cast<float*>(Em + PT * L1S);       // 4 * PT * 3
+  const int li = lane & 15, lk = lane >> 4;
+  float* C = smem;                                 // n_max * CSTR
+  float* H = C + n_max * CSTR;                     // n_max * CSTR
+  float* region = H + n_max * CSTR;
+  float* Rt = region;                              // forward: PT * n_max
+  float* At = Rt + PT * n_max;                     //          PT * CSTR
+  float* Sl = region;                              // spectrum: L1 * n_max * n_max (then Y_l)
+  float* qv = Sl + L1 * n_max * n_max;             //           n_soap
+  float* Gp = qv + P.n_soap;                       //           n_soap
+  float* Gu = Gp + P.n_soap;                       //           n_soap_unc
+  float* Rb = region;                              // backward: PT * n_max (R)
+  float* Rdb = Rb + PT * n_max;                    //           PT * n_max (dR/dr)
+  float* Fp = Rdb + PT * n_max;                    //           PT * KS (P_l^m, l <= L + 1)
+  float* Gl = Fp + PT * KS;                        //           PT * L1S (g_l)
+  float* Gd = Gl + PT * L1S;                       //           PT * L1S (dg_l/dr)
+  float2* Em = reinterpret_cast<float2*>(Gd + PT * L1S);  //    PT * L1S (e^{-i m phi})
+  float* fout = reinterpret_cast<float*>(Em + PT * L1S);  //    PT * 3
 
-  for (int t = threadIdx.x; t < KM; t += blockDim.x) preflm[t] = preflm_g[t];
-  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
-    C[t] = make_float2(0.f, 0.f);
-    Clo[t] = make_float2(0.f, 0.f);
+  for (int t = threadIdx.x; t < KM; t += blockDim.x) {
+    preflm[t] = preflm_g[t];
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= t) ++l;
+    klm[t] = l | ((t - l * (l + 1) / 2) << 8);
   }
   // --------------------------------------------------------------- forward
+  const int n_tiles = NT * CT;
+  floatx4 hi[MAX_TILES_PER_WAVE], lo[MAX_TILES_PER_WAVE];
+#pragma unroll
+  for (int j = 0; j < MAX_TILES_PER_WAVE; ++j) {
+    hi[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    lo[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  }
   for (int tb = p0; tb < p1; tb += PT) {
     const int np_ = min(PT, p1 - tb);
     __syncthreads();
-    if (wv == 0) {
-      float* R = Rt + lane * n_max;
-      float2* A = At + lane * k_max;
-      bool done = false;
-      if (lane < np_) {
-        const int p = tb + lane;
-        const double rd = rjs[p];
-        const uint32_t msk = spmask[p];
-        if (msk != 0 && rd < P.rcut_max) {
-          done = true;
+    {
+      const int p = tb + lane;
+      const bool live = lane < np_ && spmask[p] != 0 && rjs[p] < P.rcut_max;
+      if (wv == 0) {
+        // radial values of the pair, 4 pi folded in
+        float* R = Rt + lane * n_max;
+        if (live) {
           for (int n0 = 0; n0 < n_max; n0 += 8) {
             float v[8];
-            radial_eval<8>(P, rtab, rd, msk, p == p0, n0, min(n_max, n0 + 8), v, nullptr);
-            for (int j = 0; j < 8 && n0 + j < n_max; ++j) R[n0 + j] = v[j];
+            radial_eval<8>(P, rtab, rjs[p], spmask[p], p == p0, n0, min(n_max, n0 + 8), v, nullptr);
+            for (int j = 0; j < 8 && n0 + j < n_max; ++j) R[n0 + j] = fourpi * v[j];
           }
+        } else {
+          for (int n = 0; n < n_max; ++n) R[n] = 0.f;
+        }
+      } else if (wv == 1) {
+        // angular coefficients A_k = preflm P_l^m g_l e^{-i m phi}, as (re, im) columns
+        float* A = At + lane * CSTR;
+        if (live) {
           float g[L1], gd[L1];
-          angular_factors<L>(P, atab, rd, __ffs(msk) - 1, g, gd);
+          angular_factors<L>(P, atab, rjs[p], __ffs(spmask[p]) - 1, g, gd);
           float st_, ct_, sf_, cf_;
           sincosf((float)thetas[p], &st_, &ct_);
           sincosf((float)phis[p], &sf_, &cf_);
@@ -349,68 +389,91 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
           }
           angular_rows<L>(ct_, st_, preflm, [&](int k, int l, int m, float pl, float, float) {
             const float v = pl * g[l];
-            A[k] = make_float2(v * er[m], v * ei[m]);
+            A[2 * k] = v * er[m];
+            A[2 * k + 1] = v * ei[m];
           });
+          for (int c = 2 * KM; c < CSTR; ++c) A[c] = 0.f;
+        } else {
+          for (int c = 0; c < CSTR; ++c) A[c] = 0.f;
         }
-      }
-      if (!done) {
-        for (int n = 0; n < n_max; ++n) R[n] = 0.f;
-        for (int k = 0; k < k_max; ++k) A[k] = make_float2(0.f, 0.f);
       }
     }
     __syncthreads();
-    // cnk += sum over the tile of 4 pi R_n A_k (Kahan across pairs)
-    for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
-      const int n = t / k_max, k = t - n * k_max;
-      float2 s = C[t], c = Clo[t];
-      for (int lp = 0; lp < np_; ++lp) {
-        const float rr = fourpi * Rt[lp * n_max + n];
-        const float2 a = At[lp * k_max + k];
-        float y = rr * a.x - c.x, tt = s.x + y;
-        c.x = (tt - s.x) - y; s.x = tt;
-        y = rr * a.y - c.y; tt = s.y + y;
-        c.y = (tt - s.y) - y; s.y = tt;
+    // cnk: the tile's pairs in chunks of 16 neighbours, each chunk on MFMA, chunks
+    // combined with TwoSum compensation
+    const int npad = (np_ + 3) & ~3;
+#pragma unroll
+    for (int j = 0; j < MAX_TILES_PER_WAVE; ++j) {
+      const int t = wv + 4 * j;
+      if (t < n_tiles) {
+        const int a = t / CT, b = t - a * CT;
+        const int n = 16 * a + li;
+        const int col = 16 * b + li;
+        for (int c0 = 0; c0 < npad; c0 += 16) {
+          floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+          const int c1 = min(npad, c0 + 16);
+          for (int pk = c0; pk < c1; pk += 4) {
+            const int pp = pk + lk;
+            const float av = n < n_max ? Rt[pp * n_max + n] : 0.f;
+            const float bv = At[pp * CSTR + col];
+            acc = mfma4(av, bv, acc);
+          }
+          two_sum_acc(hi[j], lo[j], acc);
+        }
       }
-      C[t] = s;
-      Clo[t] = c;
     }
   }
-  __syncthreads();
-  // apply the Kahan compensation, then the poly3gauss central-atom term on k = 0
-  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
-    float2 s = C[t];
-    const float2 c = Clo[t];
-    s.x -= c.x;
-    s.y -= c.y;
-    if (P.gauss && t % k_max == 0) {
-      const int n = t / k_max;
-      for (int m = 0; m < species_mult[i]; ++m) {
-        const int sp = site_species[(size_t)i * max_mult + m] - 1;
-        s.x += central[(size_t)sp * n_max + n];
+  // C = hi + lo, plus the poly3gauss central-atom term on k = 0 (real part)
+#pragma unroll
+  for (int j = 0; j < MAX_TILES_PER_WAVE; ++j) {
+    const int t = wv + 4 * j;
+    if (t < n_tiles) {
+      const int a = t / CT, b = t - a * CT;
+      const int col = 16 * b + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = 16 * a + 4 * lk + v;
+        if (n < n_max) {
+          float val = hi[j][v] + lo[j][v];
+          if (P.gauss && col == 0)
+            for (int m = 0; m < species_mult[i]; ++m)
+              val += central[(size_t)(site_species[(size_t)i * max_mult + m] - 1) * n_max + n];
+          C[n * CSTR + col] = val;
+        }
       }
     }
-    C[t] = s;
   }
   __syncthreads();
   // --------------------------------------------------------------- spectrum
+  // S_l(n, n') = sum over the l block of w(m) C(n, c) C(n', c), w = 1 (m = 0), 2 (m > 0)
+  for (int job = wv; job < L1 * NT * NT; job += 4) {
+    const int l = job / (NT * NT), ab = job - l * NT * NT, a = ab / NT, b = ab - a * NT;
+    const int cb = l * (l + 1), cw = 2 * (l + 1);
+    const int n = 16 * a + li, np = 16 * b + li;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int c4 = 0; c4 < cw; c4 += 4) {
+      const int c = c4 + lk;
+      const bool inb = c < cw;
+      const float av = (inb && n < n_max) ? C[n * CSTR + cb + c] : 0.f;
+      const float bv = (inb && np < n_max) ? (c >= 2 ? 2.f : 1.f) * C[np * CSTR + cb + c] : 0.f;
+      acc = mfma4(av, bv, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * a + 4 * lk + v, cc = 16 * b + li;
+      if (r < n_max && cc < n_max) Sl[(l * n_max + r) * n_max + cc] = acc[v];
+    }
+  }
+  __syncthreads();
   const int n_soap = P.n_soap;
-  float* qv = region;                 // n_soap: spectrum, then q
-  float* Gp = qv + n_soap;            // n_soap
-  float* Gu = Gp + n_soap;            // n_soap_unc
   float ss = 0.f;
   for (int s = threadIdx.x; s < n_soap; s += blockDim.x) {
     float v = 0.f;
     const int e0 = P.compressed ? out_ptr[s] : s, e1 = P.compressed ? out_ptr[s + 1] : s + 1;
     for (int e = e0; e < e1; ++e) {
       const int4 cm = comp[P.compressed ? out_c[e] : e];
-      const int n = cm.x, np = cm.y, l = cm.z, kb = l * (l + 1) / 2;
-      const float base = n != np ? 1.41421356f : 1.f;
-      float acc = 0.f;
-      for (int m = 0; m <= l; ++m) {
-        const float2 u = C[n * k_max + kb + m], w = C[np * k_max + kb + m];
-        acc += (m > 0 ? 2.f * base : base) * (u.x * w.x + u.y * w.y);
-      }
-      v += (P.compressed ? out_el[e] : 1.f) * acc;
+      const float base = cm.x != cm.y ? 1.41421356f : 1.f;
+      v += (P.compressed ? out_el[e] : 1.f) * base * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
     }
     qv[s] = v;
     ss += v * v;
@@ -443,79 +506,130 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
     Gu[c] = v;
   }
   __syncthreads();
-  // H(k,n) = 4 pi w(m) sum_np Y(n,np,l) c(k,np), Y doubled on the diagonal, sqrt 2 off it
-  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
-    const int n = t / k_max, k = t - n * k_max;
-    int l = 0;
-    while ((l + 1) * (l + 2) / 2 <= k) ++l;
-    const int m = k - l * (l + 1) / 2;
-    float hr = 0.f, hi = 0.f;
-    // c(a, b, l) = (a n_max - a(a-1)/2 + b - a) L1 + l for a <= b
-    for (int np = 0; np < n_max; ++np) {
-      const int a = n < np ? n : np, b = n < np ? np : n;
-      const int c = (a * n_max - a * (a - 1) / 2 + (b - a)) * L1 + l;
-      const float y = a == b ? 2.f * Gu[c] : 1.41421356f * Gu[c];
-      const float2 cv = C[np * k_max + k];
-      hr += y * cv.x;
-      hi += y * cv.y;
+  // Y_l(n, n') = Gu(c) x 2 on the diagonal, sqrt 2 off it; c(a, b, l) = (a n_max - a(a-1)/2 + b - a) L1 + l
+  float* Y = Sl;
+  for (int t = threadIdx.x; t < L1 * n_max * n_max; t += blockDim.x) {
+    const int l = t / (n_max * n_max), r = t - l * n_max * n_max, n = r / n_max, np = r - n * n_max;
+    const int a = n < np ? n : np, b = n < np ? np : n;
+    const float g = Gu[(a * n_max - a * (a - 1) / 2 + (b - a)) * L1 + l];
+    Y[t] = a == b ? 2.f * g : 1.41421356f * g;
+  }
+  for (int t = threadIdx.x; t < n_max * (CSTR - 2 * KM); t += blockDim.x) {
+    const int n = t / (CSTR - 2 * KM);
+    H[n * CSTR + 2 * KM + (t - n * (CSTR - 2 * KM))] = 0.f;
+  }
+  __syncthreads();
+  // H_l = 4 pi w(m) Y_l C_l, on the columns of block l
+  {
+    int njobs = 0;
+    for (int l = 0; l <= L; ++l) njobs += NT * ((2 * (l + 1) + 15) / 16);
+    for (int job = wv; job < njobs; job += 4) {
+      int l = 0, rem = job;
+      while (rem >= NT * ((2 * (l + 1) + 15) / 16)) { rem -= NT * ((2 * (l + 1) + 15) / 16); ++l; }
+      const int nb_ = (2 * (l + 1) + 15) / 16;
+      const int a = rem / nb_, b = rem - a * nb_;
+      const int cb = l * (l + 1), cw = 2 * (l + 1);
+      const int n = 16 * a + li, c = 16 * b + li;
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int k4 = 0; k4 < n_max; k4 += 4) {
+        const int np = k4 + lk;
+        const float av = (n < n_max && np < n_max) ? Y[(l * n_max + n) * n_max + np] : 0.f;
+        const float bv = (np < n_max && c < cw) ? C[np * CSTR + cb + c] : 0.f;
+        acc = mfma4(av, bv, acc);
+      }
+      if (c < cw) {
+        const float w = (c >= 2 ? 2.f : 1.f) * fourpi;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = 16 * a + 4 * lk + v;
+          if (r < n_max) H[r * CSTR + cb + c] = w * acc[v];
+        }
+      }
     }
-    const float w = m == 0 ? fourpi : 2.f * fourpi;
-    H[n * k_max + k] = make_float2(w * hr, w * hi);
   }
   // --------------------------------------------------------------- backward
-  const int ng = (n_max + 3) / 4;
-  const int nlo = wv * ng, nhi = min(n_max, nlo + ng);
   float self0 = 0.f, self1 = 0.f, self2 = 0.f;
   for (int tb = p0; tb < p1; tb += PT) {
     __syncthreads();
-    const int p = tb + lane;
-    const bool live = p < p1 && p != p0 && spmask[p] != 0 && rjs[p] < P.rcut_max;
-    // per-pair angular factors, one pair per lane of wavefront 0
-    if (wv == 0 && live) {
-      const double rd = rjs[p];
-      float g[L1], gd[L1];
-      angular_factors<L>(P, atab, rd, __ffs(spmask[p]) - 1, g, gd);
+    {
+      const int p = tb + lane;
+      const bool live = p < p1 && p != p0 && spmask[p] != 0 && rjs[p] < P.rcut_max;
+      if (wv == 0) {
+        float* R = Rb + lane * n_max;
+        float* Rd = Rdb + lane * n_max;
+        if (live) {
+          for (int n0 = 0; n0 < n_max; n0 += 8) {
+            float v[8], d[8];
+            radial_eval<8>(P, rtab, rjs[p], spmask[p], false, n0, min(n_max, n0 + 8), v, d);
+            for (int j = 0; j < 8 && n0 + j < n_max; ++j) { R[n0 + j] = v[j]; Rd[n0 + j] = d[j]; }
+          }
+        } else {
+          for (int n = 0; n < n_max; ++n) { R[n] = 0.f; Rd[n] = 0.f; }
+        }
+      } else if (wv == 1) {
+        if (live) {
+          float st_, ct_;
+          sincosf((float)thetas[p], &st_, &ct_);
+          plm_rows<L>(ct_, st_, Fp + lane * KS);
+        } else {
+          for (int k = 0; k < KS; ++k) Fp[lane * KS + k] = 0.f;
+        }
+      } else if (wv == 2) {
+        if (live) {
+          float g[L1], gd[L1];
+          angular_factors<L>(P, atab, rjs[p], __ffs(spmask[p]) - 1, g, gd);
 #pragma unroll
-      for (int l = 0; l <= L; ++l) { Gl[lane * L1S + l] = g[l]; Gd[lane * L1S + l] = gd[l]; }
-      float st_, ct_, sf_, cf_;
-      sincosf((float)thetas[p], &st_, &ct_);
-      sincosf((float)phis[p], &sf_, &cf_);
-      float er = 1.f, ei = 0.f;
-      Em[lane * L1S] = make_float2(1.f, 0.f);
-      for (int m = 1; m <= L; ++m) {
-        const float nr = er * cf_ + ei * sf_, ni = ei * cf_ - er * sf_;
-        er = nr; ei = ni;
-        Em[lane * L1S + m] = make_float2(er, ei);
+          for (int l = 0; l <= L; ++l) { Gl[lane * L1S + l] = g[l]; Gd[lane * L1S + l] = gd[l]; }
+          float sf_, cf_;
+          sincosf((float)phis[p], &sf_, &cf_);
+          float er = 1.f, ei = 0.f;
+          Em[lane * L1S] = make_float2(1.f, 0.f);
+          for (int m = 1; m <= L; ++m) {
+            const float nr = er * cf_ + ei * sf_, ni = ei * cf_ - er * sf_;
+            er = nr; ei = ni;
+            Em[lane * L1S + m] = make_float2(er, ei);
+          }
+        } else {
+#pragma unroll
+          for (int l = 0; l <= L; ++l) { Gl[lane * L1S + l] = 0.f; Gd[lane * L1S + l] = 0.f; }
+          for (int m = 0; m <= L; ++m) Em[lane * L1S + m] = make_float2(0.f, 0.f);
+        }
       }
-      plm_rows<L>(ct_, st_, Fp + lane * KS);
     }
     __syncthreads();
-    float fr = 0.f, fa = 0.f, fo = 0.f;
-    if (live) {
-      float Rn[12], Rd[12];
-      radial_eval<12>(P, rtab, rjs[p], spmask[p], false, nlo, nhi, Rn, Rd);
-      const float* pr = Fp + lane * KS;
-      int k = 0;
-      for (int l = 0; l <= L; ++l) {
-        const float gl = Gl[lane * L1S + l], gdl = Gd[lane * L1S + l];
-        const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
-        for (int m = 0; m <= l; ++m, ++k) {
-          float ur = 0.f, ui = 0.f, vr = 0.f, vi = 0.f;
+    // wavefront wv: pairs 16 wv .. 16 wv + 15 of the tile
+    float fr[4] = {0.f, 0.f, 0.f, 0.f}, fa[4] = {0.f, 0.f, 0.f, 0.f}, fo[4] = {0.f, 0.f, 0.f, 0.f};
+    const int pa = 16 * wv + li;  // A-operand row (pair) of this lane
+    for (int ct = 0; ct < CT; ++ct) {
+      floatx4 U = {0.f, 0.f, 0.f, 0.f}, V = {0.f, 0.f, 0.f, 0.f};
+      const int col = 16 * ct + li;
+      for (int k4 = 0; k4 < n_max; k4 += 4) {
+        const int n = k4 + lk;
+        const bool inb = n < n_max;
+        const float hv = inb ? H[n * CSTR + col] : 0.f;
+        U = mfma4(inb ? Rb[pa * n_max + n] : 0.f, hv, U);
+        V = mfma4(inb ? Rdb[pa * n_max + n] : 0.f, hv, V);
+      }
+      // lane holds columns col (re if even, im if odd) of rows 16 wv + 4 lk + v
+      const int k = col >> 1;
+      const bool act = (col & 1) == 0 && k < KM;
+      const int lm = act ? klm[k] : 0;
+      const int l = lm & 255, m = lm >> 8;
+      const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
+      const float pf = act ? preflm[k] : 0.f;
 #pragma unroll
-          for (int j = 0; j < 12; ++j) {
-            if (nlo + j < nhi) {
-              const float2 hh = H[(nlo + j) * k_max + k];
-              ur += Rn[j] * hh.x; ui += Rn[j] * hh.y;
-              vr += Rd[j] * hh.x; vi += Rd[j] * hh.y;
-            }
-          }
-          const float2 e = Em[lane * L1S + m];
+      for (int v = 0; v < 4; ++v) {
+        const float ur = U[v], vr = V[v];
+        const float ui = __shfl_xor(ur, 1, 64), vi = __shfl_xor(vr, 1, 64);
+        if (act) {
+          const int lp = 16 * wv + 4 * lk + v;
+          const float* pr = Fp + lp * KS;
+          const float gl = Gl[lp * L1S + l], gdl = Gd[lp * L1S + l];
+          const float2 e = Em[lp * L1S + m];
           const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
           const float ev = e.x * vr + e.y * vi;     // Re(E conj v)
           const float ex = e.x * ui - e.y * ur;     // Re(i E conj u)
-          const float pf = preflm[k];
           const float pl = pf * pr[kb + m];
-          // sin(t) dP/dcos(t) and -m P/sin(t) (src/soap_turbo_angular.f90:194-247)
           const float pp1 = m < l ? pr[kb + m + 1] : 0.f;
           float dms, dsin = 0.f;
           if (m == 0) {
@@ -525,24 +639,31 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
             dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * pr[kb1 + m - 1] + 0.5f * pr[kb1 + m + 1];
           }
           // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E
-          fr += pl * (gl * ev + gdl * eu);
-          fo += pf * dms * gl * eu;
-          fa += pf * dsin * gl * ex;
+          fr[v] += pl * (gl * ev + gdl * eu);
+          fo[v] += pf * dms * gl * eu;
+          fa[v] += pf * dsin * gl * ex;
         }
       }
     }
-    fred[(wv * PT + lane) * 3] = fr;
-    fred[(wv * PT + lane) * 3 + 1] = fa;
-    fred[(wv * PT + lane) * 3 + 2] = fo;
-    __syncthreads();
-    if (wv == 0 && p < p1 && p != p0) {
-      float f_rad = 0.f, f_azi = 0.f, f_pol = 0.f;
 #pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        f_rad += fred[(w * PT + lane) * 3];
-        f_azi += fred[(w * PT + lane) * 3 + 1];
-        f_pol += fred[(w * PT + lane) * 3 + 2];
+    for (int v = 0; v < 4; ++v) {
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        fr[v] += __shfl_xor(fr[v], o, 64);
+        fa[v] += __shfl_xor(fa[v], o, 64);
+        fo[v] += __shfl_xor(fo[v], o, 64);
       }
+      if (li == 0) {
+        const int lp = 16 * wv + 4 * lk + v;
+        fout[lp * 3] = fr[v];
+        fout[lp * 3 + 1] = fa[v];
+        fout[lp * 3 + 2] = fo[v];
+      }
+    }
+    __syncthreads();
+    const int p = tb + lane;
+    if (wv == 0 && p < p1 && p != p0) {
+      const float f_rad = fout[lane * 3], f_azi = fout[lane * 3 + 1], f_pol = fout[lane * 3 + 2];
       float st_, ct_, sf_, cf_;
       sincosf((float)thetas[p], &st_, &ct_);
       sincosf((float)phis[p], &sf_, &cf_);
@@ -588,11 +709,12 @@ __global__ void widen_k(size_t n, const float* __restrict__ a, double* __restric
 }
 
 size_t site_lds(int n_max, int l_max, int n_soap, int n_soap_unc) {
-  const int k_max = (l_max + 1) * (l_max + 2) / 2, KS = ((l_max + 2) * (l_max + 3) / 2) | 1, L1S = (l_max + 1) | 1;
-  size_t c = sizeof(float2) * 3 * n_max * k_max;
-  size_t fwd = sizeof(float) * PT * n_max + sizeof(float2) * PT * k_max;
-  size_t spec = sizeof(float) * (2 * n_soap + n_soap_unc);
-  size_t bwd = sizeof(float) * PT * (KS + 2 * L1S) + sizeof(float2) * PT * L1S + sizeof(float) * 4 * PT * 3;
+  const int KM = (l_max + 1) * (l_max + 2) / 2, KS = ((l_max + 2) * (l_max + 3) / 2) | 1, L1S = (l_max + 1) | 1;
+  const int CSTR = ((2 * KM + 15) / 16) * 16;
+  size_t c = sizeof(float) * 2 * n_max * CSTR;
+  size_t fwd = sizeof(float) * PT * (n_max + CSTR);
+  size_t spec = sizeof(float) * ((l_max + 1) * n_max * n_max + 2 * n_soap + n_soap_unc);
+  size_t bwd = sizeof(float) * PT * (2 * n_max + KS + 2 * L1S + 2 * L1S + 3);
   return c + std::max(std::max(fwd, spec), bwd);
 }
 
