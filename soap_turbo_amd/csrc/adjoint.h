@@ -55,6 +55,10 @@ struct AdjointInputs {
   const std::vector<double>* out_el_h;  // compression elements by output (host)
   const std::vector<double>* P_el_h;    // compression elements in P order (host)
   const std::string* tables_key;
+  const int4* ocomp;                     // direct maps (null unless single-entry compression or none)
+  const int* umap;
+  const std::vector<double>* oel_h;
+  const std::vector<double>* uel_h;
   std::vector<RadialConsts> radial_tab;  // per species, block offset 0, n_max = alpha_max, with derivatives
   const int32_t* pair_atom;
   const double* vec;

@@ -255,6 +255,30 @@ __device__ __forceinline__ void radial_eval(const AdjParams& P, const float* __r
   }
 }
 
+// The pair's whole radial row R_n(r) (times scale) and optionally dR_n/dr,
+// written to R[0..n_max) / Rd; each species' table cell is located once.
+__device__ __forceinline__ void radial_row(const AdjParams& P, const float* __restrict__ rtab, double rd, uint32_t msk,
+                                           bool self, float scale, float* R, float* Rd) {
+  for (int n = 0; n < P.n_max; ++n) {
+    R[n] = 0.f;
+    if (Rd) Rd[n] = 0.f;
+  }
+  for (int s = 0; s < P.n_species; ++s) {
+    const SpeciesTab& sp = P.sp[s];
+    if (!(msk & (1u << s)) || !(rd < sp.rcut)) continue;
+    if (self && !sp.do_central) continue;
+    float t, ih;
+    const int cell = radial_cell(sp, rd, t, ih);
+    const float* c = rtab + sp.rtab_off + (size_t)cell * 4 * sp.a;
+    const float sc = (self ? sp.cw : 1.f) * scale;
+    for (int q = 0; q < sp.a; ++q) {
+      const float c0 = c[q], c1 = c[sp.a + q], c2 = c[2 * sp.a + q], c3 = c[3 * sp.a + q];
+      R[sp.off + q] = sc * (c0 + t * (c1 + t * (c2 + t * c3)));
+      if (Rd) Rd[sp.off + q] = (c1 + t * (2.f * c2 + 3.f * t * c3)) * ih;
+    }
+  }
+}
+
 // Angular radial factors g_l(r) and dg_l/dr of the pair's first species.
 template <int L>
 __device__ __forceinline__ void angular_factors(const AdjParams& P, const float* __restrict__ atab, double rd, int s0,
@@ -302,8 +326,9 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
     const double* __restrict__ phis, const float* __restrict__ G32, const int4* __restrict__ comp,
     const int* __restrict__ out_ptr, const int* __restrict__ out_c, const float* __restrict__ out_el,
     const int* __restrict__ unc_ptr, const int* __restrict__ unc_e, const int32_t* __restrict__ P_i,
-    const float* __restrict__ P_el, float* __restrict__ q_out, float* __restrict__ sdp_out,
-    float* __restrict__ pair_out) {
+    const float* __restrict__ P_el, const int4* __restrict__ ocomp, const float* __restrict__ oel,
+    const int* __restrict__ umap, const float* __restrict__ uel, float* __restrict__ q_out,
+    float* __restrict__ sdp_out, float* __restrict__ pair_out) {
   constexpr int KM = (L + 1) * (L + 2) / 2;
   constexpr int KM1 = (L + 2) * (L + 3) / 2;
   constexpr int KS = KM1 | 1;                    // odd row stride of the per-pair P_l^m rows
@@ -363,11 +388,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
         // radial values of the pair, 4 pi folded in
         float* R = Rt + lane * n_max;
         if (live) {
-          for (int n0 = 0; n0 < n_max; n0 += 8) {
-            float v[8];
-            radial_eval<8>(P, rtab, rjs[p], spmask[p], p == p0, n0, min(n_max, n0 + 8), v, nullptr);
-            for (int j = 0; j < 8 && n0 + j < n_max; ++j) R[n0 + j] = fourpi * v[j];
-          }
+          radial_row(P, rtab, rjs[p], spmask[p], p == p0, fourpi, R, nullptr);
         } else {
           for (int n = 0; n < n_max; ++n) R[n] = 0.f;
         }
@@ -469,11 +490,16 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
   float ss = 0.f;
   for (int s = threadIdx.x; s < n_soap; s += blockDim.x) {
     float v = 0.f;
-    const int e0 = P.compressed ? out_ptr[s] : s, e1 = P.compressed ? out_ptr[s + 1] : s + 1;
-    for (int e = e0; e < e1; ++e) {
-      const int4 cm = comp[P.compressed ? out_c[e] : e];
-      const float base = cm.x != cm.y ? 1.41421356f : 1.f;
-      v += (P.compressed ? out_el[e] : 1.f) * base * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
+    if (ocomp) {
+      const int4 cm = ocomp[s];
+      v = oel[s] * (cm.x != cm.y ? 1.41421356f : 1.f) * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
+    } else {
+      const int e0 = P.compressed ? out_ptr[s] : s, e1 = P.compressed ? out_ptr[s + 1] : s + 1;
+      for (int e = e0; e < e1; ++e) {
+        const int4 cm = comp[P.compressed ? out_c[e] : e];
+        const float base = cm.x != cm.y ? 1.41421356f : 1.f;
+        v += (P.compressed ? out_el[e] : 1.f) * base * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
+      }
     }
     qv[s] = v;
     ss += v * v;
@@ -497,7 +523,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
   __syncthreads();
   for (int c = threadIdx.x; c < P.n_soap_unc; c += blockDim.x) {
     float v;
-    if (P.compressed) {
+    if (umap) {
+      const int q = umap[c];
+      v = q >= 0 ? uel[c] * Gp[q] : 0.f;
+    } else if (P.compressed) {
       v = 0.f;
       for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v += P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
     } else {
@@ -558,11 +587,7 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
         float* R = Rb + lane * n_max;
         float* Rd = Rdb + lane * n_max;
         if (live) {
-          for (int n0 = 0; n0 < n_max; n0 += 8) {
-            float v[8], d[8];
-            radial_eval<8>(P, rtab, rjs[p], spmask[p], false, n0, min(n_max, n0 + 8), v, d);
-            for (int j = 0; j < 8 && n0 + j < n_max; ++j) { R[n0 + j] = v[j]; Rd[n0 + j] = d[j]; }
-          }
+          radial_row(P, rtab, rjs[p], spmask[p], false, 1.f, R, Rd);
         } else {
           for (int n = 0; n < n_max; ++n) { R[n] = 0.f; Rd[n] = 0.f; }
         }
@@ -724,10 +749,11 @@ void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order
                  const int32_t* species, int max_mult, const int32_t* mult, const uint32_t* spmask, const double* rjs,
                  const double* th, const double* ph, const float* G32, const int4* comp, const int* out_ptr,
                  const int* out_c, const float* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
-                 const float* P_el, float* q, float* sdp, float* pair) {
+                 const float* P_el, const int4* ocomp, const float* oel, const int* umap, const float* uel, float* q,
+                 float* sdp, float* pair) {
   adjoint_site_k<L><<<P.n_sites, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, central, species, max_mult,
                                                      mult, spmask, rjs, th, ph, G32, comp, out_ptr, out_c, out_el,
-                                                     unc_ptr, unc_e, P_i, P_el, q, sdp, pair);
+                                                     unc_ptr, unc_e, P_i, P_el, ocomp, oel, umap, uel, q, sdp, pair);
 }
 
 // host copy of the per-call constant tables that live next to the FP64 ones
@@ -966,15 +992,24 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     return ST_ERR_UNSUPPORTED;
   }
   // compression element tables in FP32
-  if (in.out_ptr) {
-    if (s.el_key != *in.tables_key) {
-      std::vector<float> oe(in.out_el_h->begin(), in.out_el_h->end()), pe(in.P_el_h->begin(), in.P_el_h->end());
-      if ((rc = ensure(s.el32, (oe.size() + pe.size()) * 4, msg))) return rc;
-      (void)hipMemcpyAsync(s.el32.p, oe.data(), oe.size() * 4, hipMemcpyHostToDevice, st);
-      (void)hipMemcpyAsync(s.el32.as<float>() + oe.size(), pe.data(), pe.size() * 4, hipMemcpyHostToDevice, st);
-      (void)hipStreamSynchronize(st);
-      s.el_key = *in.tables_key;
+  // FP32 copies: compression elements by output and in P order, then the direct-map elements
+  size_t n_oe = 0, n_pe = 0;
+  if (in.out_ptr) { n_oe = in.out_el_h->size(); n_pe = in.P_el_h->size(); }
+  const size_t n_o1 = in.ocomp ? in.oel_h->size() : 0, n_u1 = in.ocomp ? in.uel_h->size() : 0;
+  if (s.el_key != *in.tables_key) {
+    std::vector<float> all;
+    if (in.out_ptr) {
+      all.insert(all.end(), in.out_el_h->begin(), in.out_el_h->end());
+      all.insert(all.end(), in.P_el_h->begin(), in.P_el_h->end());
     }
+    if (in.ocomp) {
+      all.insert(all.end(), in.oel_h->begin(), in.oel_h->end());
+      all.insert(all.end(), in.uel_h->begin(), in.uel_h->end());
+    }
+    if ((rc = ensure(s.el32, (all.size() + 1) * 4, msg))) return rc;
+    if (!all.empty()) (void)hipMemcpyAsync(s.el32.p, all.data(), all.size() * 4, hipMemcpyHostToDevice, st);
+    (void)hipStreamSynchronize(st);
+    s.el_key = *in.tables_key;
   }
   if ((rc = ensure(s.G32, (size_t)in.n_sites * in.n_soap * 4, msg))) return rc;
   if ((rc = ensure(s.soap, (size_t)in.n_sites * in.n_soap * 4, msg))) return rc;
@@ -987,7 +1022,10 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   const float* pf = s.tabinfo.as<float>();
   const float* cen = pf + s.n_pf;
   const float* oel = in.out_ptr ? s.el32.as<float>() : nullptr;
-  const float* pel = in.out_ptr ? s.el32.as<float>() + in.out_el_h->size() : nullptr;
+  const float* pel = in.out_ptr ? s.el32.as<float>() + n_oe : nullptr;
+  const float* oel1 = in.ocomp ? s.el32.as<float>() + n_oe + n_pe : nullptr;
+  const float* uel1 = in.ocomp ? oel1 + n_o1 : nullptr;
+  (void)n_u1;
   bool ok = true;
   timer("adj_site_f32", [&] {
     switch (in.l_max) {
@@ -995,8 +1033,8 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   case l:                                                                                                          \
     launch_site<l>(st, P, sh, nullptr, in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cen, in.species,   \
                    in.max_mult, in.mult, in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(), in.comp,        \
-                   in.out_ptr, in.out_c, oel, in.unc_ptr, in.unc_e, in.P_i, pel, s.soap.as<float>(),               \
-                   s.sdp.as<float>(), s.pair.as<float>());                                                         \
+                   in.out_ptr, in.out_c, oel, in.unc_ptr, in.unc_e, in.P_i, pel, in.ocomp, oel1,                   \
+                   in.ocomp ? in.umap : nullptr, uel1, s.soap.as<float>(), s.sdp.as<float>(), s.pair.as<float>());  \
     break;
       LS(0) LS(1) LS(2) LS(3) LS(4) LS(5) LS(6) LS(7) LS(8) LS(9) LS(10) LS(11) LS(12)
 #undef LS

@@ -61,6 +61,10 @@ struct st_ctx {
   DevBuf forces64, vpart64, virial64;
   bool have_forces64 = false;
   DevBuf comp, out_ptr, out_c, out_el, skip, unc_ptr, unc_e, P_i, P_el;
+  // direct maps when every output has one P entry and every component at most one
+  DevBuf ocomp, oel, umap, uel;
+  bool single = false;
+  std::vector<double> oel_h, uel_h;
   std::string tables_key;
   std::vector<double> out_el_h, P_el_h;
   int n_max = 0, k_max = 0, l_max = -1, n_soap = 0, n_soap_unc = 0;
@@ -391,6 +395,43 @@ int build_tables(st_ctx* ctx, const st_params* p, const st_compression* c) {
       if ((rc = upload(ctx, ctx->P_el, c->P_el, (size_t)nz * 8))) return rc;
     }
     if ((rc = upload(ctx, ctx->skip, skip.data(), skip.size()))) return rc;
+    // direct maps: output s <- one component, component c -> one output (or none)
+    {
+      const int ns = ctx->n_soap;
+      std::vector<int32_t> oc4((size_t)4 * ns);
+      std::vector<double> oel(ns, 1.0), uel(n_unc, 0.0);
+      std::vector<int32_t> um(n_unc, -1);
+      bool single = true;
+      if (!cmp) {
+        for (int q = 0; q < ns; ++q) {
+          for (int j = 0; j < 4; ++j) oc4[4 * q + j] = comp[4 * q + j];
+          um[q] = q;
+          uel[q] = 1.0;
+        }
+      } else {
+        std::vector<int> nout(ns, 0), nunc(n_unc, 0);
+        for (int k = 0; k < c->P_nonzero; ++k) { nout[c->P_i[k] - 1]++; nunc[c->P_j[k] - 1]++; }
+        for (int q = 0; q < ns && single; ++q) single = nout[q] == 1;
+        for (int u = 0; u < n_unc && single; ++u) single = nunc[u] <= 1;
+        if (single)
+          for (int k = 0; k < c->P_nonzero; ++k) {
+            const int q = c->P_i[k] - 1, u = c->P_j[k] - 1;
+            for (int j = 0; j < 4; ++j) oc4[4 * q + j] = comp[4 * u + j];
+            oel[q] = c->P_el[k];
+            um[u] = q;
+            uel[u] = c->P_el[k];
+          }
+      }
+      ctx->single = single;
+      if (single) {
+        if ((rc = upload(ctx, ctx->ocomp, oc4.data(), oc4.size() * 4))) return rc;
+        if ((rc = upload(ctx, ctx->oel, oel.data(), oel.size() * 8))) return rc;
+        if ((rc = upload(ctx, ctx->umap, um.data(), um.size() * 4))) return rc;
+        if ((rc = upload(ctx, ctx->uel, uel.data(), uel.size() * 8))) return rc;
+        ctx->oel_h = oel;
+        ctx->uel_h = uel;
+      }
+    }
     std::vector<double> pf;
     preflm_host(L, pf);
     if ((rc = upload(ctx, ctx->preflm, pf.data(), pf.size() * 8))) return rc;
@@ -669,14 +710,17 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
     return fail(ctx, ST_ERR_UNSUPPORTED, "n_max * k_max too large for the spectrum kernel");
   timed(ctx, "power_f64", [&] {
     st::launch_power(st, NS, n_max, k_max, n_soap, ctx->comp.as<int4>(), optr, ctx->out_c.as<int>(),
-                     ctx->out_el.as<double>(), ctx->cnk.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>());
+                     ctx->out_el.as<double>(), ctx->single ? ctx->ocomp.as<int4>() : nullptr, ctx->oel.as<double>(),
+                     ctx->cnk.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>());
   });
   if (der && !ctr) {
     if (st::derivatives_lds(n_max, L, k_max) > 64 * 1024)
       return fail(ctx, ST_ERR_UNSUPPORTED, "the derivative working set exceeds the LDS");
     timed(ctx, "derivatives_f64", [&] {
       ok = st::launch_derivatives(st, NS, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(), ctx->comp.as<int4>(), optr,
-                                  ctx->out_c.as<int>(), ctx->out_el.as<double>(), ctx->cnk.as<double2>(),
+                                  ctx->out_c.as<int>(), ctx->out_el.as<double>(),
+                                  ctx->single ? ctx->ocomp.as<int4>() : nullptr, ctx->oel.as<double>(),
+                                  ctx->cnk.as<double2>(),
                                   ctx->rad.as<double>(), ctx->rad_der.as<double>(), ctx->ang.as<double2>(),
                                   ctx->ang_r.as<double2>(), ctx->ang_a.as<double2>(), ctx->ang_p.as<double2>(),
                                   ctx->soap.as<double>(), ctx->sdp.as<double>(), ctx->rjs.as<double>(),
@@ -721,6 +765,7 @@ int st_contract_resident(st_ctx* ctx, double sgn) {
     st::launch_contract(ctx->stream, ctx->n_sites, ctx->n_max, ctx->l_max, ctx->k_max, ctx->n_soap, ctx->n_soap_unc,
                         ctx->site_ptr.as<int>(), ctx->skip.as<uint8_t>(),
                         ctx->compressed ? ctx->unc_ptr.as<int>() : nullptr, ctx->unc_e.as<int>(),
+                        ctx->single ? ctx->umap.as<int>() : nullptr, ctx->uel.as<double>(),
                         ctx->P_i.as<int32_t>(), ctx->P_el.as<double>(), ctx->cnk.as<double2>(), ctx->rad.as<double>(),
                         ctx->rad_der.as<double>(), ctx->ang.as<double2>(), ctx->ang_r.as<double2>(),
                         ctx->ang_a.as<double2>(), ctx->ang_p.as<double2>(), ctx->soap.as<double>(), ctx->G.as<double>(),
@@ -819,6 +864,10 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   in.P_i = ctx->P_i.as<int32_t>();
   in.out_el_h = &ctx->out_el_h;
   in.P_el_h = &ctx->P_el_h;
+  in.ocomp = ctx->single ? ctx->ocomp.as<int4>() : nullptr;
+  in.umap = ctx->umap.as<int>();
+  in.oel_h = &ctx->oel_h;
+  in.uel_h = &ctx->uel_h;
   in.tables_key = &ctx->tables_key;
   {
     int off = 0;

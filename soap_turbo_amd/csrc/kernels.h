@@ -19,12 +19,13 @@ void launch_cnk(hipStream_t st, int n_sites, int n_max, int k_max, const int* si
                 const double2* ang, const int32_t* site_species, int max_mult, const int32_t* species_mult,
                 const double* central, double2* cnk);
 void launch_power(hipStream_t st, int n_sites, int n_max, int k_max, int n_soap, const int4* comp, const int* out_ptr,
-                  const int* out_c, const double* out_el, const double2* cnk, double* soap, double* sqrt_dot_p);
+                  const int* out_c, const double* out_el, const int4* ocomp, const double* oel, const double2* cnk,
+                  double* soap, double* sqrt_dot_p);
 void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqrt_dot_p, double* soap);
 size_t derivatives_lds(int n_max, int l_max, int k_max);
 bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* site_ptr,
                         const int4* comp, const int* out_ptr, const int* out_c, const double* out_el,
-                        const double2* cnk, const double* rad, const double* rad_der, const double2* ang,
+                        const int4* ocomp, const double* oel, const double2* cnk, const double* rad, const double* rad_der, const double2* ang,
                         const double2* ang_r, const double2* ang_a, const double2* ang_p, const double* soap,
                         const double* sqrt_dot_p, const double* rjs, const double* thetas, const double* phis,
                         double* cart);
@@ -33,7 +34,7 @@ void launch_contract_cart(hipStream_t st, int n_sites, int n_pairs, int n_soap, 
 size_t contract_lds(int n_max, int k_max, int n_soap, int n_soap_unc);
 void launch_contract(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, int n_soap_unc,
                      const int* site_ptr, const uint8_t* skip, const int* unc_ptr, const int* unc_e,
-                     const int32_t* P_i, const double* P_el, const double2* cnk, const double* rad,
+                     const int* umap, const double* uel, const int32_t* P_i, const double* P_el, const double2* cnk, const double* rad,
                      const double* rad_der, const double2* ang, const double2* ang_r, const double2* ang_a,
                      const double2* ang_p, const double* soap, const double* G, double sgn, const double* sqrt_dot_p,
                      const double* rjs, const double* thetas, const double* phis, double* pair);

@@ -452,6 +452,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 __global__ __launch_bounds__(256) void power_k(int n_sites, int n_max, int k_max, int n_soap,
                                                const int4* __restrict__ comp, const int* __restrict__ out_ptr,
                                                const int* __restrict__ out_c, const double* __restrict__ out_el,
+                                               const int4* __restrict__ ocomp, const double* __restrict__ oel,
                                                const double2* __restrict__ cnk, double* __restrict__ soap,
                                                double* __restrict__ sqrt_dot_p) {
   extern __shared__ double2 smem2[];
@@ -463,7 +464,10 @@ __global__ __launch_bounds__(256) void power_k(int n_sites, int n_max, int k_max
   double ss = 0.0;
   for (int s = threadIdx.x; s < n_soap; s += blockDim.x) {
     double v;
-    if (out_ptr) {
+    if (ocomp) {
+      // one entry per output: 0 + P_el value, as the reference's accumulation gives
+      v = out_ptr ? 0.0 + oel[s] * spectrum_value(cn, k_max, ocomp[s]) : spectrum_value(cn, k_max, ocomp[s]);
+    } else if (out_ptr) {
       v = 0.0;
       for (int e = out_ptr[s]; e < out_ptr[s + 1]; ++e) v = v + out_el[e] * spectrum_value(cn, k_max, comp[out_c[e]]);
     } else {
@@ -512,7 +516,8 @@ template <int DER_NSPT>
 __global__ __launch_bounds__(256) void derivatives_k(
     int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* __restrict__ site_ptr,
     const int4* __restrict__ comp, const int* __restrict__ out_ptr, const int* __restrict__ out_c,
-    const double* __restrict__ out_el, const double2* __restrict__ cnk, const double* __restrict__ rad,
+    const double* __restrict__ out_el, const int4* __restrict__ ocomp, const double* __restrict__ oel,
+    const double2* __restrict__ cnk, const double* __restrict__ rad,
     const double* __restrict__ rad_der, const double2* __restrict__ ang, const double2* __restrict__ ang_r,
     const double2* __restrict__ ang_a, const double2* __restrict__ ang_p, const double* __restrict__ soap,
     const double* __restrict__ sqrt_dot_p, const double* __restrict__ rjs, const double* __restrict__ thetas,
@@ -533,12 +538,44 @@ __global__ __launch_bounds__(256) void derivatives_k(
   const double inv_p = 1.0 / sdp, inv_p3 = 1.0 / (sdp * sdp * sdp);
   double selfacc[DER_NSPT][3];
   double qv[DER_NSPT];
+  int4 cmu[DER_NSPT];
+  double elu[DER_NSPT];
 #pragma unroll
   for (int u = 0; u < DER_NSPT; ++u) {
     selfacc[u][0] = selfacc[u][1] = selfacc[u][2] = 0.0;
     const int s = threadIdx.x + u * blockDim.x;
     qv[u] = s < n_soap ? soap[(size_t)i * n_soap + s] : 0.0;
+    cmu[u] = (ocomp && s < n_soap) ? ocomp[s] : make_int4(0, 0, 0, 0);
+    elu[u] = (ocomp && s < n_soap) ? oel[s] : 0.0;
   }
+  // one P entry per output (or no compression): the component of each output is kept in registers
+  auto component = [&](int u, int s, int b, double& vr, double& va, double& vo) {
+    const double* P = PQ + b * 4 * nl;
+    const double* rb = sR + b * 2 * n_max;
+    if (ocomp) {
+      double tr, ta, to;
+      der_component(cmu[u], P, P + nl, P + 2 * nl, P + 3 * nl, rb, rb + n_max, n_max, tr, ta, to);
+      if (out_ptr) {
+        vr = 0.0 + elu[u] * tr; va = 0.0 + elu[u] * ta; vo = 0.0 + elu[u] * to;
+      } else {
+        vr = tr; va = ta; vo = to;
+      }
+      return;
+    }
+    const int e0 = out_ptr ? out_ptr[s] : s, e1 = out_ptr ? out_ptr[s + 1] : s + 1;
+    vr = va = vo = 0.0;
+    for (int e = e0; e < e1; ++e) {
+      double tr, ta, to;
+      der_component(comp[out_ptr ? out_c[e] : e], P, P + nl, P + 2 * nl, P + 3 * nl, rb, rb + n_max, n_max, tr, ta,
+                    to);
+      if (out_ptr) {
+        const double el = out_el[e];
+        vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
+      } else {
+        vr = tr; va = ta; vo = to;
+      }
+    }
+  };
   for (int pb = p0 + 1; pb < p1; pb += DER_PB) {
     const int nb = min(DER_PB, p1 - pb);
     __syncthreads();
@@ -584,23 +621,11 @@ __global__ __launch_bounds__(256) void derivatives_k(
     for (int u = 0; u < DER_NSPT; ++u) {
       const int s = threadIdx.x + u * blockDim.x;
       if (s < n_soap) {
-        const int e0 = out_ptr ? out_ptr[s] : s, e1 = out_ptr ? out_ptr[s + 1] : s + 1;
 #pragma unroll
         for (int b = 0; b < DER_PB; ++b) {
           if (b < nb) {
-            const double* P = PQ + b * 4 * nl;
-            double vr = 0.0, va = 0.0, vo = 0.0;
-            for (int e = e0; e < e1; ++e) {
-              double tr, ta, to;
-              der_component(comp[out_ptr ? out_c[e] : e], P, P + nl, P + 2 * nl, P + 3 * nl, sR + b * 2 * n_max,
-                            sR + b * 2 * n_max + n_max, n_max, tr, ta, to);
-              if (out_ptr) {
-                const double el = out_el[e];
-                vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
-              } else {
-                vr = tr; va = ta; vo = to;
-              }
-            }
+            double vr, va, vo;
+            component(u, s, b, vr, va, vo);
             dots[b][0] += qv[u] * vr; dots[b][1] += qv[u] * va; dots[b][2] += qv[u] * vo;
           }
         }
@@ -623,25 +648,13 @@ __global__ __launch_bounds__(256) void derivatives_k(
         const double dot_a = red[0][3 * b + 1] + red[1][3 * b + 1] + red[2][3 * b + 1] + red[3][3 * b + 1];
         const double dot_o = red[0][3 * b + 2] + red[1][3 * b + 2] + red[2][3 * b + 2] + red[3][3 * b + 2];
         const double* rr = rot[b];
-        const double* P = PQ + b * 4 * nl;
         const size_t p = pb + b;
 #pragma unroll
         for (int u = 0; u < DER_NSPT; ++u) {
           const int s = threadIdx.x + u * blockDim.x;
           if (s < n_soap) {
-            const int e0 = out_ptr ? out_ptr[s] : s, e1 = out_ptr ? out_ptr[s + 1] : s + 1;
-            double vr = 0.0, va = 0.0, vo = 0.0;
-            for (int e = e0; e < e1; ++e) {
-              double tr, ta, to;
-              der_component(comp[out_ptr ? out_c[e] : e], P, P + nl, P + 2 * nl, P + 3 * nl, sR + b * 2 * n_max,
-                            sR + b * 2 * n_max + n_max, n_max, tr, ta, to);
-              if (out_ptr) {
-                const double el = out_el[e];
-                vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
-              } else {
-                vr = tr; va = ta; vo = to;
-              }
-            }
+            double vr, va, vo;
+            component(u, s, b, vr, va, vo);
             const double sv = qv[u];
             vr = vr * inv_p - sv * inv_p3 * dot_r;
             va = va * inv_p - sv * inv_p3 * dot_a;
@@ -704,6 +717,7 @@ __global__ __launch_bounds__(256) void contract_cart_k(int n_sites, int n_pairs,
 __global__ __launch_bounds__(256) void contract_k(
     int n_sites, int n_max, int l_max, int k_max, int n_soap, int n_soap_unc, const int* __restrict__ site_ptr,
     const uint8_t* __restrict__ skip, const int* __restrict__ unc_ptr, const int* __restrict__ unc_e,
+    const int* __restrict__ umap, const double* __restrict__ uel,
     const int32_t* __restrict__ P_i, const double* __restrict__ P_el, const double2* __restrict__ cnk,
     const double* __restrict__ rad, const double* __restrict__ rad_der, const double2* __restrict__ ang,
     const double2* __restrict__ ang_r, const double2* __restrict__ ang_a, const double2* __restrict__ ang_p,
@@ -728,7 +742,10 @@ __global__ __launch_bounds__(256) void contract_k(
   __syncthreads();
   for (int c = threadIdx.x; c < n_soap_unc; c += blockDim.x) {
     double v;
-    if (unc_ptr) {
+    if (umap) {
+      const int q = umap[c];
+      v = unc_ptr ? (q >= 0 ? 0.0 + uel[c] * Gp[q] : 0.0) : Gp[c];
+    } else if (unc_ptr) {
       v = 0.0;
       for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v = v + P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
     } else {
@@ -857,10 +874,12 @@ void launch_cnk(hipStream_t st, int n_sites, int n_max, int k_max, const int* si
 }
 
 void launch_power(hipStream_t st, int n_sites, int n_max, int k_max, int n_soap, const int4* comp, const int* out_ptr,
-                  const int* out_c, const double* out_el, const double2* cnk, double* soap, double* sqrt_dot_p) {
+                  const int* out_c, const double* out_el, const int4* ocomp, const double* oel, const double2* cnk,
+                  double* soap, double* sqrt_dot_p) {
   if (n_sites == 0) return;
   size_t sh = sizeof(double2) * n_max * k_max;
-  power_k<<<n_sites, 256, sh, st>>>(n_sites, n_max, k_max, n_soap, comp, out_ptr, out_c, out_el, cnk, soap, sqrt_dot_p);
+  power_k<<<n_sites, 256, sh, st>>>(n_sites, n_max, k_max, n_soap, comp, out_ptr, out_c, out_el, ocomp, oel, cnk, soap,
+                                    sqrt_dot_p);
 }
 
 void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqrt_dot_p, double* soap) {
@@ -876,6 +895,7 @@ size_t derivatives_lds(int n_max, int l_max, int k_max) {
 
 bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* site_ptr,
                         const int4* comp, const int* out_ptr, const int* out_c, const double* out_el,
+                        const int4* ocomp, const double* oel,
                         const double2* cnk, const double* rad, const double* rad_der, const double2* ang,
                         const double2* ang_r, const double2* ang_a, const double2* ang_p, const double* soap,
                         const double* sqrt_dot_p, const double* rjs, const double* thetas, const double* phis,
@@ -886,7 +906,8 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k
 #define DL(N)                                                                                                     \
   if (need <= N) {                                                                                                \
     derivatives_k<N><<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp, out_ptr,     \
-                                               out_c, out_el, cnk, rad, rad_der, ang, ang_r, ang_a, ang_p, soap,  \
+                                               out_c, out_el, ocomp, oel, cnk, rad, rad_der, ang, ang_r, ang_a,   \
+                                               ang_p, soap,                                                       \
                                                sqrt_dot_p, rjs, thetas, phis, cart);                              \
     return true;                                                                                                  \
   }
@@ -907,6 +928,7 @@ size_t contract_lds(int n_max, int k_max, int n_soap, int n_soap_unc) {
 
 void launch_contract(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, int n_soap_unc,
                      const int* site_ptr, const uint8_t* skip, const int* unc_ptr, const int* unc_e,
+                     const int* umap, const double* uel,
                      const int32_t* P_i, const double* P_el, const double2* cnk, const double* rad,
                      const double* rad_der, const double2* ang, const double2* ang_r, const double2* ang_a,
                      const double2* ang_p, const double* soap, const double* G, double sgn, const double* sqrt_dot_p,
@@ -914,7 +936,7 @@ void launch_contract(hipStream_t st, int n_sites, int n_max, int l_max, int k_ma
   if (n_sites == 0) return;
   size_t sh = contract_lds(n_max, k_max, n_soap, n_soap_unc);
   contract_k<<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, n_soap_unc, site_ptr, skip, unc_ptr, unc_e,
-                                       P_i, P_el, cnk, rad, rad_der, ang, ang_r, ang_a, ang_p, soap, G, sgn,
+                                       umap, uel, P_i, P_el, cnk, rad, rad_der, ang, ang_r, ang_a, ang_p, soap, G, sgn,
                                        sqrt_dot_p, rjs, thetas, phis, pair);
 }
 
