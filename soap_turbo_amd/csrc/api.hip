@@ -714,7 +714,7 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
                      ctx->cnk.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>());
   });
   if (der && !ctr) {
-    if (st::derivatives_lds(n_max, L, k_max) > 64 * 1024)
+    if (st::derivatives_lds(n_max, L, k_max, n_soap) > 64 * 1024)
       return fail(ctx, ST_ERR_UNSUPPORTED, "the derivative working set exceeds the LDS");
     timed(ctx, "derivatives_f64", [&] {
       ok = st::launch_derivatives(st, NS, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(), ctx->comp.as<int4>(), optr,

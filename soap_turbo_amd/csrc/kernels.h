@@ -22,7 +22,7 @@ void launch_power(hipStream_t st, int n_sites, int n_max, int k_max, int n_soap,
                   const int* out_c, const double* out_el, const int4* ocomp, const double* oel, const double2* cnk,
                   double* soap, double* sqrt_dot_p);
 void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqrt_dot_p, double* soap);
-size_t derivatives_lds(int n_max, int l_max, int k_max);
+size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap);
 bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* site_ptr,
                         const int4* comp, const int* out_ptr, const int* out_c, const double* out_el,
                         const int4* ocomp, const double* oel, const double2* cnk, const double* rad, const double* rad_der, const double2* ang,

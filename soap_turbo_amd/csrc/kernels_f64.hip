@@ -497,23 +497,84 @@ __global__ void normalise_k(int n_sites, int n_soap, const double* __restrict__ 
 // rotation to Cartesian with the self column accumulated as minus the sum of
 // the others in neighbour order (:837-895). soap holds the unnormalised
 // spectrum here. Each thread owns the outputs s = tid + u * 256 for the whole
-// site; a batch costs four barriers.
-constexpr int DER_PB = 4;
+// site. A batch computes each output's three components once, keeps them in
+// registers across the block reduction of q . d, then normalises, rotates and
+// stores them: three barriers per batch.
+//
+// LDS layout: the batch's P, Q, Qa, Qo tables are [n][l][pair][kind] and the
+// radial values [n][pair][value], so that one output's reads for every pair
+// and kind share one address register and differ by immediate offsets; rows
+// are padded by one double to spread adjacent (n, l) over the banks. The
+// angular rows are [k][pair][kind] (double2), padded likewise.
+template <int NSPT>
+struct DerShape {
+  // component registers per thread: PB * NSPT * 3 doubles, kept near 36
+  static constexpr int PB = NSPT <= 3 ? 4 : NSPT <= 4 ? 3 : NSPT <= 6 ? 2 : 1;
+  // waves per SIMD the register allocation is held to
+  static constexpr int WAVES = NSPT <= 1 ? 3 : 2;
+  static constexpr int PQS = 4 * PB + 1;   // doubles per (n, l) row
+  static constexpr int RS = 2 * PB + 1;    // doubles per n row of radial values
+  static constexpr int AS = 4 * PB + 1;    // double2 per k row of angular values
+  static constexpr int NRED = 3 * PB <= 4 ? 4 : 3 * PB <= 8 ? 8 : 16;
+};
 
-__device__ __forceinline__ void der_component(const int4 cm, const double* __restrict__ P, const double* __restrict__ Q,
-                                              const double* __restrict__ Qa, const double* __restrict__ Qo,
-                                              const double* __restrict__ r, const double* __restrict__ rd, int n_max,
-                                              double& tr, double& ta, double& to) {
-  const int n = cm.x, np = cm.y, l = cm.z;
-  const double pre = (4.0 * 3.141592653589793) * (n == np ? 1.0 : 1.4142135623730951);
-  tr = pre * (rd[n] * P[l * n_max + np] + r[n] * Q[l * n_max + np] + rd[np] * P[l * n_max + n] +
-              r[np] * Q[l * n_max + n]);
-  ta = pre * (r[n] * Qa[l * n_max + np] + r[np] * Qa[l * n_max + n]);
-  to = pre * (r[n] * Qo[l * n_max + np] + r[np] * Qo[l * n_max + n]);
+__device__ __forceinline__ int pack_component(const int4 c) { return c.x | (c.y << 10) | (c.z << 20); }
+
+// Sum each of V values over the 64 lanes of a wave by recursive halving:
+// every exchange moves half of the still-unreduced values, so the wave needs
+// V/2 + V/4 + ... shuffles instead of 6 V. Lane j ends with the total of value
+// idx(j) = bit5(j) V/2 + bit4(j) V/4 + ...; lanes with the low bits below that
+// hold the same totals.
+template <int V>
+__device__ __forceinline__ double wave_multi_sum(double (&x)[V], int lane) {
+  static_assert(V == 4 || V == 8 || V == 16, "V");
+#pragma unroll
+  for (int h = V / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+    const bool hi = (lane & bit) != 0;
+#pragma unroll
+    for (int j = 0; j < h; ++j) {
+      const double keep = hi ? x[j + h] : x[j];
+      const double send = hi ? x[j] : x[j + h];
+      x[j] = keep + __shfl_xor(send, bit, 64);
+    }
+  }
+  double t = x[0];
+  constexpr int LOG = V == 4 ? 2 : V == 8 ? 3 : 4;
+#pragma unroll
+  for (int bit = 32 >> LOG; bit >= 1; bit >>= 1) t += __shfl_xor(t, bit, 64);
+  return t;
 }
 
-template <int DER_NSPT>
-__global__ __launch_bounds__(256) void derivatives_k(
+template <int V>
+__device__ __forceinline__ int wave_multi_index(int lane) {
+  constexpr int LOG = V == 4 ? 2 : V == 8 ? 3 : 4;
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; k < LOG; ++k) idx |= ((lane >> (5 - k)) & 1) << (LOG - 1 - k);
+  return idx;
+}
+
+template <int PB>
+__device__ __forceinline__ void der_component(const int cm, int b, const double* __restrict__ PQ,
+                                              const double* __restrict__ R, int L1, double& tr, double& ta,
+                                              double& to) {
+  constexpr int PQS = 4 * PB + 1, RS = 2 * PB + 1;
+  const int n = cm & 1023, np = (cm >> 10) & 1023, l = cm >> 20;
+  const double* A = PQ + (np * L1 + l) * PQS + 4 * b;   // P, Q, Qa, Qo at (np, l)
+  const double* C = PQ + (n * L1 + l) * PQS + 4 * b;    // ... at (n, l)
+  const double* Rn = R + n * RS + 2 * b;                // r, r' at n
+  const double* Rp = R + np * RS + 2 * b;               // ... at np
+  const double pre = (4.0 * 3.141592653589793) * (n == np ? 1.0 : 1.4142135623730951);
+  tr = pre * (Rn[1] * A[0] + Rn[0] * A[1] + Rp[1] * C[0] + Rp[0] * C[1]);
+  ta = pre * (Rn[0] * A[2] + Rp[0] * C[2]);
+  to = pre * (Rn[0] * A[3] + Rp[0] * C[3]);
+}
+
+// DIRECT: every output has exactly one (component, P_el) entry (ocomp/oel);
+// otherwise outputs sum their CSR entries (out_ptr/out_c/out_el), or, with
+// no compression (out_ptr null), output s is component s.
+template <int NSPT, bool DIRECT>
+__global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
     int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* __restrict__ site_ptr,
     const int4* __restrict__ comp, const int* __restrict__ out_ptr, const int* __restrict__ out_c,
     const double* __restrict__ out_el, const int4* __restrict__ ocomp, const double* __restrict__ oel,
@@ -522,71 +583,53 @@ __global__ __launch_bounds__(256) void derivatives_k(
     const double2* __restrict__ ang_a, const double2* __restrict__ ang_p, const double* __restrict__ soap,
     const double* __restrict__ sqrt_dot_p, const double* __restrict__ rjs, const double* __restrict__ thetas,
     const double* __restrict__ phis, double* __restrict__ cart) {
+  using Sh = DerShape<NSPT>;
+  constexpr int PB = Sh::PB, PQS = Sh::PQS, RS = Sh::RS, AS = Sh::AS, NRED = Sh::NRED;
   extern __shared__ double2 smem2[];
-  __shared__ double red[4][3 * DER_PB];
-  __shared__ double rot[DER_PB][8];
+  __shared__ double red[4][NRED];
+  __shared__ double rot[PB][8];
   const int i = blockIdx.x;
-  const int nl = n_max * (l_max + 1);
+  const int L1 = l_max + 1, nl = n_max * L1;
   double2* cn = smem2;                                   // n_max * k_max
-  double2* sA = cn + n_max * k_max;                      // PB * 4 * k_max
-  double* sR = (double*)(sA + DER_PB * 4 * k_max);       // PB * 2 * n_max
-  double* PQ = sR + DER_PB * 2 * n_max;                  // PB * 4 * nl, [b][kind][l][n]
+  double2* sA = cn + n_max * k_max;                      // k_max * AS
+  double* sR = (double*)(sA + k_max * AS);               // n_max * RS
+  double* PQ = sR + n_max * RS;                          // nl * PQS
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) cn[t] = cnk[(size_t)i * n_max * k_max + t];
   const double sdp = sqrt_dot_p[i];
   const double inv_p = 1.0 / sdp, inv_p3 = 1.0 / (sdp * sdp * sdp);
-  double selfacc[DER_NSPT][3];
-  double qv[DER_NSPT];
-  int4 cmu[DER_NSPT];
-  double elu[DER_NSPT];
+  const bool scaled = out_ptr != nullptr;
+  double selfacc[NSPT][3];
+  double qv[NSPT];
+  int cmu[NSPT];
+  double elu[NSPT];
 #pragma unroll
-  for (int u = 0; u < DER_NSPT; ++u) {
+  for (int u = 0; u < NSPT; ++u) {
     selfacc[u][0] = selfacc[u][1] = selfacc[u][2] = 0.0;
     const int s = threadIdx.x + u * blockDim.x;
-    qv[u] = s < n_soap ? soap[(size_t)i * n_soap + s] : 0.0;
-    cmu[u] = (ocomp && s < n_soap) ? ocomp[s] : make_int4(0, 0, 0, 0);
-    elu[u] = (ocomp && s < n_soap) ? oel[s] : 0.0;
+    const bool live = s < n_soap;
+    qv[u] = live ? soap[(size_t)i * n_soap + s] : 0.0;
+    if (DIRECT) {
+      cmu[u] = live ? pack_component(ocomp[s]) : 0;
+      elu[u] = live ? oel[s] : 0.0;
+    } else {
+      cmu[u] = (live && !scaled) ? pack_component(comp[s]) : 0;
+      elu[u] = 0.0;
+    }
   }
-  // one P entry per output (or no compression): the component of each output is kept in registers
-  auto component = [&](int u, int s, int b, double& vr, double& va, double& vo) {
-    const double* P = PQ + b * 4 * nl;
-    const double* rb = sR + b * 2 * n_max;
-    if (ocomp) {
-      double tr, ta, to;
-      der_component(cmu[u], P, P + nl, P + 2 * nl, P + 3 * nl, rb, rb + n_max, n_max, tr, ta, to);
-      if (out_ptr) {
-        vr = 0.0 + elu[u] * tr; va = 0.0 + elu[u] * ta; vo = 0.0 + elu[u] * to;
-      } else {
-        vr = tr; va = ta; vo = to;
-      }
-      return;
-    }
-    const int e0 = out_ptr ? out_ptr[s] : s, e1 = out_ptr ? out_ptr[s + 1] : s + 1;
-    vr = va = vo = 0.0;
-    for (int e = e0; e < e1; ++e) {
-      double tr, ta, to;
-      der_component(comp[out_ptr ? out_c[e] : e], P, P + nl, P + 2 * nl, P + 3 * nl, rb, rb + n_max, n_max, tr, ta,
-                    to);
-      if (out_ptr) {
-        const double el = out_el[e];
-        vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
-      } else {
-        vr = tr; va = ta; vo = to;
-      }
-    }
-  };
-  for (int pb = p0 + 1; pb < p1; pb += DER_PB) {
-    const int nb = min(DER_PB, p1 - pb);
+  for (int pb = p0 + 1; pb < p1; pb += PB) {
+    const int nb = min(PB, p1 - pb);
     __syncthreads();
-    for (int t = threadIdx.x; t < nb * 4 * k_max; t += blockDim.x) {
+    for (int t = threadIdx.x; t < PB * 4 * k_max; t += blockDim.x) {
       const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
       const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
-      sA[t] = src[(size_t)(pb + b) * k_max + k];
+      sA[k * AS + 4 * b + kind] = b < nb ? src[(size_t)(pb + b) * k_max + k] : make_double2(0.0, 0.0);
     }
-    for (int t = threadIdx.x; t < nb * 2 * n_max; t += blockDim.x) {
-      const int b = t / (2 * n_max), rem = t - b * 2 * n_max;
-      sR[t] = rem < n_max ? rad[(size_t)(pb + b) * n_max + rem] : rad_der[(size_t)(pb + b) * n_max + rem - n_max];
+    for (int t = threadIdx.x; t < PB * 2 * n_max; t += blockDim.x) {
+      const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
+      const double* src = d == 0 ? rad : rad_der;
+      sR[n * RS + 2 * b + d] = b < nb ? src[(size_t)(pb + b) * n_max + n] : 0.0;
     }
     if (threadIdx.x < nb) {
       const int p = pb + threadIdx.x;
@@ -599,50 +642,68 @@ __global__ __launch_bounds__(256) void derivatives_k(
       rr[6] = -(sin_f * inv_r); rr[7] = cos_f * inv_r;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < nb * 4 * nl; t += blockDim.x) {
-      const int b = t / (4 * nl), rem = t - b * 4 * nl;
-      const int kind = rem / nl, r = rem - kind * nl, l = r / n_max, n = r - l * n_max;
+    // P, Q, Qa, Qo(n, l) for every pair of the batch: sum over m of w(m) Re(conj(A) c)
+    for (int x = threadIdx.x; x < nl; x += blockDim.x) {
+      const int n = x / L1, l = x - n * L1;
       const int kb = l * (l + 1) / 2;
-      const double2* a = sA + (b * 4 + kind) * k_max;
-      double acc = 0.0;
+      double acc[4 * PB];
+#pragma unroll
+      for (int j = 0; j < 4 * PB; ++j) acc[j] = 0.0;
       for (int m = 0; m <= l; ++m) {
         const double wm = m == 0 ? 1.0 : 2.0;
-        const double2 cv = cn[n * k_max + kb + m], av = a[kb + m];
-        acc = acc + wm * (av.x * cv.x + av.y * cv.y);
-      }
-      PQ[t] = acc;
-    }
-    __syncthreads();
-    // pass 1: q . d for every pair of the batch
-    double dots[DER_PB][3];
+        const double2 cv = cn[n * k_max + kb + m];
+        const double2* a = sA + (kb + m) * AS;
 #pragma unroll
-    for (int b = 0; b < DER_PB; ++b) dots[b][0] = dots[b][1] = dots[b][2] = 0.0;
-#pragma unroll
-    for (int u = 0; u < DER_NSPT; ++u) {
-      const int s = threadIdx.x + u * blockDim.x;
-      if (s < n_soap) {
-#pragma unroll
-        for (int b = 0; b < DER_PB; ++b) {
-          if (b < nb) {
-            double vr, va, vo;
-            component(u, s, b, vr, va, vo);
-            dots[b][0] += qv[u] * vr; dots[b][1] += qv[u] * va; dots[b][2] += qv[u] * vo;
-          }
+        for (int j = 0; j < 4 * PB; ++j) {
+          const double2 av = a[j];
+          acc[j] = acc[j] + wm * (av.x * cv.x + av.y * cv.y);
         }
       }
+#pragma unroll
+      for (int j = 0; j < 4 * PB; ++j) PQ[x * PQS + j] = acc[j];
     }
+    __syncthreads();
+    // pass 1: the components of every pair of the batch, and q . d
+    double v[PB][NSPT][3];
+    double dots[NRED];
 #pragma unroll
-    for (int b = 0; b < DER_PB; ++b)
+    for (int j = 0; j < NRED; ++j) dots[j] = 0.0;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        double v = dots[b][c];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (lane == 0) red[w][3 * b + c] = v;
+    for (int u = 0; u < NSPT; ++u) {
+      const int s = threadIdx.x + u * blockDim.x;
+#pragma unroll
+      for (int b = 0; b < PB; ++b) {
+        double vr = 0.0, va = 0.0, vo = 0.0;
+        if (s < n_soap) {
+          if (DIRECT || !scaled) {
+            double tr, ta, to;
+            der_component<PB>(cmu[u], b, PQ, sR, L1, tr, ta, to);
+            if (DIRECT && scaled) {
+              vr = 0.0 + elu[u] * tr; va = 0.0 + elu[u] * ta; vo = 0.0 + elu[u] * to;
+            } else {
+              vr = tr; va = ta; vo = to;
+            }
+          } else {
+            for (int e = out_ptr[s]; e < out_ptr[s + 1]; ++e) {
+              double tr, ta, to;
+              der_component<PB>(pack_component(comp[out_c[e]]), b, PQ, sR, L1, tr, ta, to);
+              const double el = out_el[e];
+              vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
+            }
+          }
+        }
+        v[b][u][0] = vr; v[b][u][1] = va; v[b][u][2] = vo;
+        dots[3 * b] += qv[u] * vr; dots[3 * b + 1] += qv[u] * va; dots[3 * b + 2] += qv[u] * vo;
       }
+    }
+    {
+      const double t = wave_multi_sum<NRED>(dots, lane);
+      if ((lane & (64 / NRED - 1)) == 0) red[w][wave_multi_index<NRED>(lane)] = t;
+    }
     __syncthreads();
     // pass 2: normalise, rotate, store, accumulate the self column
 #pragma unroll
-    for (int b = 0; b < DER_PB; ++b) {
+    for (int b = 0; b < PB; ++b) {
       if (b < nb) {
         const double dot_r = red[0][3 * b] + red[1][3 * b] + red[2][3 * b] + red[3][3 * b];
         const double dot_a = red[0][3 * b + 1] + red[1][3 * b + 1] + red[2][3 * b + 1] + red[3][3 * b + 1];
@@ -650,15 +711,13 @@ __global__ __launch_bounds__(256) void derivatives_k(
         const double* rr = rot[b];
         const size_t p = pb + b;
 #pragma unroll
-        for (int u = 0; u < DER_NSPT; ++u) {
+        for (int u = 0; u < NSPT; ++u) {
           const int s = threadIdx.x + u * blockDim.x;
           if (s < n_soap) {
-            double vr, va, vo;
-            component(u, s, b, vr, va, vo);
             const double sv = qv[u];
-            vr = vr * inv_p - sv * inv_p3 * dot_r;
-            va = va * inv_p - sv * inv_p3 * dot_a;
-            vo = vo * inv_p - sv * inv_p3 * dot_o;
+            const double vr = v[b][u][0] * inv_p - sv * inv_p3 * dot_r;
+            const double va = v[b][u][1] * inv_p - sv * inv_p3 * dot_a;
+            const double vo = v[b][u][2] * inv_p - sv * inv_p3 * dot_o;
             const double c0 = rr[0] * vr + rr[3] * vo + rr[6] * va;
             const double c1 = rr[1] * vr + rr[4] * vo + rr[7] * va;
             const double c2 = rr[2] * vr + rr[5] * vo;
@@ -673,7 +732,7 @@ __global__ __launch_bounds__(256) void derivatives_k(
     }
   }
 #pragma unroll
-  for (int u = 0; u < DER_NSPT; ++u) {
+  for (int u = 0; u < NSPT; ++u) {
     const int s = threadIdx.x + u * blockDim.x;
     if (s < n_soap) {
       double* C = cart + ((size_t)p0 * n_soap + s) * 3;
@@ -888,9 +947,15 @@ void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqr
   normalise_k<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n_sites, n_soap, sqrt_dot_p, soap);
 }
 
-size_t derivatives_lds(int n_max, int l_max, int k_max) {
-  return sizeof(double2) * (n_max * k_max + DER_PB * 4 * k_max) +
-         sizeof(double) * DER_PB * (2 * n_max + 4 * n_max * (l_max + 1));
+static int der_pb(int n_soap) {
+  const int need = (n_soap + 255) / 256;
+  return need <= 3 ? DerShape<3>::PB : need <= 4 ? DerShape<4>::PB : need <= 6 ? DerShape<6>::PB : DerShape<12>::PB;
+}
+
+size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap) {
+  const int pb = der_pb(n_soap);
+  return sizeof(double2) * (n_max * k_max + k_max * (4 * pb + 1)) +
+         sizeof(double) * (n_max * (2 * pb + 1) + n_max * (l_max + 1) * (4 * pb + 1));
 }
 
 bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* site_ptr,
@@ -902,13 +967,19 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k
                         double* cart) {
   if (n_sites == 0) return true;
   const int need = (n_soap + 255) / 256;
-  size_t sh = derivatives_lds(n_max, l_max, k_max);
+  size_t sh = derivatives_lds(n_max, l_max, k_max, n_soap);
 #define DL(N)                                                                                                     \
   if (need <= N) {                                                                                                \
-    derivatives_k<N><<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp, out_ptr,     \
-                                               out_c, out_el, ocomp, oel, cnk, rad, rad_der, ang, ang_r, ang_a,   \
-                                               ang_p, soap,                                                       \
-                                               sqrt_dot_p, rjs, thetas, phis, cart);                              \
+    if (ocomp)                                                                                                    \
+      derivatives_k<N, true><<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,      \
+                                                       out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der, ang,\
+                                                       ang_r, ang_a, ang_p, soap, sqrt_dot_p, rjs, thetas, phis,  \
+                                                       cart);                                                     \
+    else                                                                                                          \
+      derivatives_k<N, false><<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,     \
+                                                        out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der,    \
+                                                        ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rjs, thetas,  \
+                                                        phis, cart);                                              \
     return true;                                                                                                  \
   }
   DL(1) DL(2) DL(3) DL(4) DL(6) DL(8) DL(12)
