@@ -496,8 +496,8 @@ __global__ void normalise_k(int n_sites, int n_soap, const double* __restrict__ 
 // assembly of the surviving components (:2027-2066), then normalisation and
 // rotation to Cartesian with the self column accumulated as minus the sum of
 // the others in neighbour order (:837-895). soap holds the unnormalised
-// spectrum here. Each thread owns the outputs s = tid + u * 256 for the whole
-// site. A batch computes each output's three components once, keeps them in
+// spectrum here. Each thread owns the outputs s = tid + u * NT for the whole
+// site (256 or 512 threads). A batch computes each output's three components once, keeps them in
 // registers across the block reduction of q . d, then normalises, rotates and
 // stores them: three barriers per batch.
 //
@@ -510,8 +510,8 @@ template <int NSPT>
 struct DerShape {
   // component registers per thread: PB * NSPT * 3 doubles, kept near 36
   static constexpr int PB = NSPT <= 3 ? 4 : NSPT <= 4 ? 3 : NSPT <= 6 ? 2 : 1;
-  // waves per SIMD the register allocation is held to
-  static constexpr int WAVES = NSPT <= 1 ? 3 : 2;
+  // waves per SIMD the register allocation is held to, for NT threads per block
+  static constexpr int waves(int nt) { return nt == 256 && NSPT <= 2 ? 3 : 2; }
   static constexpr int PQS = 4 * PB + 1;   // doubles per (n, l) row
   static constexpr int RS = 2 * PB + 1;    // doubles per n row of radial values
   static constexpr int AS = 4 * PB + 1;    // double2 per k row of angular values
@@ -525,19 +525,22 @@ __device__ __forceinline__ int pack_component(const int4 c) { return c.x | (c.y 
 // V/2 + V/4 + ... shuffles instead of 6 V. Lane j ends with the total of value
 // idx(j) = bit5(j) V/2 + bit4(j) V/4 + ...; lanes with the low bits below that
 // hold the same totals.
+template <int H, int BIT, int V>
+__device__ __forceinline__ void halving_stage(double (&x)[V], int lane) {
+  const bool hi = (lane & BIT) != 0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const double keep = hi ? x[j + H] : x[j];
+    const double send = hi ? x[j] : x[j + H];
+    x[j] = keep + __shfl_xor(send, BIT, 64);
+  }
+  if constexpr (H > 1) halving_stage<H / 2, BIT / 2, V>(x, lane);
+}
+
 template <int V>
 __device__ __forceinline__ double wave_multi_sum(double (&x)[V], int lane) {
   static_assert(V == 4 || V == 8 || V == 16, "V");
-#pragma unroll
-  for (int h = V / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
-    const bool hi = (lane & bit) != 0;
-#pragma unroll
-    for (int j = 0; j < h; ++j) {
-      const double keep = hi ? x[j + h] : x[j];
-      const double send = hi ? x[j] : x[j + h];
-      x[j] = keep + __shfl_xor(send, bit, 64);
-    }
-  }
+  halving_stage<V / 2, 32, V>(x, lane);
   double t = x[0];
   constexpr int LOG = V == 4 ? 2 : V == 8 ? 3 : 4;
 #pragma unroll
@@ -573,8 +576,8 @@ __device__ __forceinline__ void der_component(const int cm, int b, const double*
 // DIRECT: every output has exactly one (component, P_el) entry (ocomp/oel);
 // otherwise outputs sum their CSR entries (out_ptr/out_c/out_el), or, with
 // no compression (out_ptr null), output s is component s.
-template <int NSPT, bool DIRECT>
-__global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
+template <int NSPT, int NT, bool DIRECT>
+__global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* __restrict__ site_ptr,
     const int4* __restrict__ comp, const int* __restrict__ out_ptr, const int* __restrict__ out_c,
     const double* __restrict__ out_el, const int4* __restrict__ ocomp, const double* __restrict__ oel,
@@ -586,7 +589,8 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
   using Sh = DerShape<NSPT>;
   constexpr int PB = Sh::PB, PQS = Sh::PQS, RS = Sh::RS, AS = Sh::AS, NRED = Sh::NRED;
   extern __shared__ double2 smem2[];
-  __shared__ double red[4][NRED];
+  constexpr int NW = NT / 64;
+  __shared__ double red[NW][NRED];
   __shared__ double rot[PB][8];
   const int i = blockIdx.x;
   const int L1 = l_max + 1, nl = n_max * L1;
@@ -596,7 +600,7 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
   double* PQ = sR + n_max * RS;                          // nl * PQS
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) cn[t] = cnk[(size_t)i * n_max * k_max + t];
+  for (int t = threadIdx.x; t < n_max * k_max; t += NT) cn[t] = cnk[(size_t)i * n_max * k_max + t];
   const double sdp = sqrt_dot_p[i];
   const double inv_p = 1.0 / sdp, inv_p3 = 1.0 / (sdp * sdp * sdp);
   const bool scaled = out_ptr != nullptr;
@@ -607,7 +611,7 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
 #pragma unroll
   for (int u = 0; u < NSPT; ++u) {
     selfacc[u][0] = selfacc[u][1] = selfacc[u][2] = 0.0;
-    const int s = threadIdx.x + u * blockDim.x;
+    const int s = threadIdx.x + u * NT;
     const bool live = s < n_soap;
     qv[u] = live ? soap[(size_t)i * n_soap + s] : 0.0;
     if (DIRECT) {
@@ -621,12 +625,12 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
   for (int pb = p0 + 1; pb < p1; pb += PB) {
     const int nb = min(PB, p1 - pb);
     __syncthreads();
-    for (int t = threadIdx.x; t < PB * 4 * k_max; t += blockDim.x) {
+    for (int t = threadIdx.x; t < PB * 4 * k_max; t += NT) {
       const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
       const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
       sA[k * AS + 4 * b + kind] = b < nb ? src[(size_t)(pb + b) * k_max + k] : make_double2(0.0, 0.0);
     }
-    for (int t = threadIdx.x; t < PB * 2 * n_max; t += blockDim.x) {
+    for (int t = threadIdx.x; t < PB * 2 * n_max; t += NT) {
       const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
       const double* src = d == 0 ? rad : rad_der;
       sR[n * RS + 2 * b + d] = b < nb ? src[(size_t)(pb + b) * n_max + n] : 0.0;
@@ -643,7 +647,7 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
     }
     __syncthreads();
     // P, Q, Qa, Qo(n, l) for every pair of the batch: sum over m of w(m) Re(conj(A) c)
-    for (int x = threadIdx.x; x < nl; x += blockDim.x) {
+    for (int x = threadIdx.x; x < nl; x += NT) {
       const int n = x / L1, l = x - n * L1;
       const int kb = l * (l + 1) / 2;
       double acc[4 * PB];
@@ -653,11 +657,12 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
         const double wm = m == 0 ? 1.0 : 2.0;
         const double2 cv = cn[n * k_max + kb + m];
         const double2* a = sA + (kb + m) * AS;
+        // all loads of the row first, so that they are in flight together
+        double2 av[4 * PB];
 #pragma unroll
-        for (int j = 0; j < 4 * PB; ++j) {
-          const double2 av = a[j];
-          acc[j] = acc[j] + wm * (av.x * cv.x + av.y * cv.y);
-        }
+        for (int j = 0; j < 4 * PB; ++j) av[j] = a[j];
+#pragma unroll
+        for (int j = 0; j < 4 * PB; ++j) acc[j] = acc[j] + wm * (av[j].x * cv.x + av[j].y * cv.y);
       }
 #pragma unroll
       for (int j = 0; j < 4 * PB; ++j) PQ[x * PQS + j] = acc[j];
@@ -670,7 +675,7 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
     for (int j = 0; j < NRED; ++j) dots[j] = 0.0;
 #pragma unroll
     for (int u = 0; u < NSPT; ++u) {
-      const int s = threadIdx.x + u * blockDim.x;
+      const int s = threadIdx.x + u * NT;
 #pragma unroll
       for (int b = 0; b < PB; ++b) {
         double vr = 0.0, va = 0.0, vo = 0.0;
@@ -705,14 +710,16 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
 #pragma unroll
     for (int b = 0; b < PB; ++b) {
       if (b < nb) {
-        const double dot_r = red[0][3 * b] + red[1][3 * b] + red[2][3 * b] + red[3][3 * b];
-        const double dot_a = red[0][3 * b + 1] + red[1][3 * b + 1] + red[2][3 * b + 1] + red[3][3 * b + 1];
-        const double dot_o = red[0][3 * b + 2] + red[1][3 * b + 2] + red[2][3 * b + 2] + red[3][3 * b + 2];
+        double dot_r = red[0][3 * b], dot_a = red[0][3 * b + 1], dot_o = red[0][3 * b + 2];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) {
+          dot_r += red[q][3 * b]; dot_a += red[q][3 * b + 1]; dot_o += red[q][3 * b + 2];
+        }
         const double* rr = rot[b];
         const size_t p = pb + b;
 #pragma unroll
         for (int u = 0; u < NSPT; ++u) {
-          const int s = threadIdx.x + u * blockDim.x;
+          const int s = threadIdx.x + u * NT;
           if (s < n_soap) {
             const double sv = qv[u];
             const double vr = v[b][u][0] * inv_p - sv * inv_p3 * dot_r;
@@ -733,7 +740,7 @@ __global__ __launch_bounds__(256, DerShape<NSPT>::WAVES) void derivatives_k(
   }
 #pragma unroll
   for (int u = 0; u < NSPT; ++u) {
-    const int s = threadIdx.x + u * blockDim.x;
+    const int s = threadIdx.x + u * NT;
     if (s < n_soap) {
       double* C = cart + ((size_t)p0 * n_soap + s) * 3;
       C[0] = selfacc[u][0]; C[1] = selfacc[u][1]; C[2] = selfacc[u][2];
@@ -947,9 +954,18 @@ void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqr
   normalise_k<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n_sites, n_soap, sqrt_dot_p, soap);
 }
 
+// (outputs per thread, threads per block) for n_soap outputs: 256 threads up
+// to 768 outputs, 512 threads up to 3072.
+static void der_config(int n_soap, int& nspt, int& nt) {
+  nt = n_soap <= 768 ? 256 : 512;
+  nspt = (n_soap + nt - 1) / nt;
+  if (nspt == 5) nspt = 6;
+}
+
 static int der_pb(int n_soap) {
-  const int need = (n_soap + 255) / 256;
-  return need <= 3 ? DerShape<3>::PB : need <= 4 ? DerShape<4>::PB : need <= 6 ? DerShape<6>::PB : DerShape<12>::PB;
+  int nspt, nt;
+  der_config(n_soap, nspt, nt);
+  return nspt <= 3 ? DerShape<3>::PB : nspt <= 4 ? DerShape<4>::PB : DerShape<6>::PB;
 }
 
 size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap) {
@@ -966,23 +982,25 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k
                         const double* sqrt_dot_p, const double* rjs, const double* thetas, const double* phis,
                         double* cart) {
   if (n_sites == 0) return true;
-  const int need = (n_soap + 255) / 256;
+  if (n_soap > 3072) return false;
+  int nspt, nt;
+  der_config(n_soap, nspt, nt);
   size_t sh = derivatives_lds(n_max, l_max, k_max, n_soap);
-#define DL(N)                                                                                                     \
-  if (need <= N) {                                                                                                \
+#define DL(N, T)                                                                                                  \
+  if (nspt == N && nt == T) {                                                                                     \
     if (ocomp)                                                                                                    \
-      derivatives_k<N, true><<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,      \
-                                                       out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der, ang,\
-                                                       ang_r, ang_a, ang_p, soap, sqrt_dot_p, rjs, thetas, phis,  \
-                                                       cart);                                                     \
-    else                                                                                                          \
-      derivatives_k<N, false><<<n_sites, 256, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,     \
+      derivatives_k<N, T, true><<<n_sites, T, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,    \
                                                         out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der,    \
                                                         ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rjs, thetas,  \
                                                         phis, cart);                                              \
+    else                                                                                                          \
+      derivatives_k<N, T, false><<<n_sites, T, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,   \
+                                                         out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der,   \
+                                                         ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rjs, thetas, \
+                                                         phis, cart);                                             \
     return true;                                                                                                  \
   }
-  DL(1) DL(2) DL(3) DL(4) DL(6) DL(8) DL(12)
+  DL(1, 256) DL(2, 256) DL(3, 256) DL(2, 512) DL(3, 512) DL(4, 512) DL(6, 512)
 #undef DL
   return false;
 }

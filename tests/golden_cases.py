@@ -1,4 +1,4 @@
-"""Loads the golden fixtures (tests/golden/*.npz) back into workloads."""
+"""Loads the golden fixtures (tests/golden/*.npz) back into workloads; shared test helpers."""
 from __future__ import annotations
 
 import glob
@@ -40,3 +40,25 @@ def rel(a, b):
 def checksums(cart):
     w = np.cos(np.arange(cart.size, dtype=np.float64) * 0.37).reshape(cart.shape)
     return np.array([cart.sum(), (cart * cart).sum(), (w * cart).sum(), np.abs(cart).max()])
+
+
+def compression_indices(kind, n_unc, seed):
+    """Compression indices (1-based, the reference's P_i/P_j/P_el) of three kinds: None, one entry per
+    output with a weight ("scaled"), and outputs summing one to three weighted components ("multi")."""
+    if kind is None:
+        return None
+    rng = np.random.default_rng(seed)
+    if kind == "scaled":
+        cols = np.sort(rng.choice(n_unc, size=(2 * n_unc) // 3, replace=False))
+        P_j = (cols + 1).astype(np.int32)
+        P_i = np.arange(1, len(cols) + 1, dtype=np.int32)
+        return P_i, P_j, rng.uniform(0.25, 2.0, len(cols))
+    P_i, P_j = [], []
+    out = 0
+    for c in rng.permutation(n_unc)[: (3 * n_unc) // 4]:
+        if not P_i or rng.random() < 0.5:
+            out += 1
+        P_i.append(out)
+        P_j.append(c + 1)
+    return np.array(P_i, np.int32), np.array(P_j, np.int32), rng.uniform(-1.5, 1.5, len(P_i))
+

@@ -11,7 +11,7 @@ import pytest
 import soap_turbo_amd as st
 from oracle import oracle as orc
 from soap_turbo_amd.workload import BoxSpec, SoapParams, Workload, make_workload, normal
-from tests.golden_cases import CASES, checksums, load, rel
+from tests.golden_cases import CASES, checksums, compression_indices, load, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -169,3 +169,34 @@ def test_reference_stop_conditions(ctx):
     w.params.l_max = 13
     with pytest.raises(st.SoapTurboError):
         run_get_soap(ctx, w, None, True)
+
+
+# every block shape of the Jacobian kernel (outputs per thread x threads per block), and both of its
+# compression paths (a direct output -> component map, and a list of weighted components per output)
+@pytest.mark.parametrize("alpha_max,l_max,kind", [
+    ([2, 2], 2, None),            # n_soap 30
+    ([5, 5], 6, "scaled"),        # 256 of 385 components, one per output: 256 x 1
+    ([5, 5], 6, None),            # 385: 256 x 2
+    ([6, 6], 6, "multi"),         # 409 of 546 components into 202 outputs
+    ([6, 6], 6, None),            # 546: 256 x 3
+    ([8, 8], 8, None),            # 1224: 512 x 3
+    ([8, 8, 4], 8, None),         # 1890: 512 x 4
+    ([8, 8, 8], 8, "multi"),      # 2025 of 2700 components into 1013 outputs: 512 x 2
+    ([8, 8, 8], 8, None),         # 2700: 512 x 6
+])
+def test_jacobian_kernel_shapes_match_oracle(ctx, alpha_max, l_max, kind):
+    ns = len(alpha_max)
+    probs = [1.0 / ns] * ns
+    spec = BoxSpec(343, 0.09, 1.0, probs, [1, 6, 8][:ns], seed=11 + ns)
+    par = SoapParams(alpha_max=alpha_max, l_max=l_max, rcut_hard=4.5, rcut_soft=4.0, basis="poly3gauss",
+                     radial_enhancement=1)
+    w = make_workload((spec, par), n_sites=6)
+    n_max = sum(alpha_max)
+    comp = compression_indices(kind, n_max * (n_max + 1) // 2 * (l_max + 1), 5 + n_max)
+    keep = st._lib.Keep()
+    prm = _params(keep, w, False)
+    W, S = ctx.get_basis(prm, n_max)
+    q, cart = run_get_soap(ctx, w, comp, False)
+    qo, carto, _ = orc.get_soap(w, W, S, comp, do_der=True, legacy_seed=False)
+    assert rel(q, qo) <= TOL
+    assert rel(cart, carto) <= TOL

@@ -15,7 +15,7 @@ from scipy import integrate, special
 
 from oracle import oracle as orc
 from soap_turbo_amd.workload import BoxSpec, SoapParams, make_workload, normal
-from tests.golden_cases import CASES, checksums, load, rel
+from tests.golden_cases import CASES, checksums, compression_indices, load, rel
 
 ORACLE_TOL = 1e-12
 
@@ -234,3 +234,29 @@ def test_forces_match_finite_differences_of_energy():
             pm = w.pos.copy(); pm[a, c] -= h
             fd = -(energy(pp) - energy(pm)) / (2 * h)
             assert abs(fd - forces[a, c]) <= 1e-6 * max(1.0, np.abs(forces).max())
+
+
+@pytest.mark.parametrize("kind", ["scaled", "multi"])
+def test_compression_is_the_weighted_sum_of_uncompressed_components(kind):
+    # soap(P_i) += P_el * component(P_j) before the normalisation: the compressed, normalised descriptor is
+    # P u / |P u| for the uncompressed normalised u, and its derivative follows by the quotient rule from du.
+    spec = BoxSpec(200, 0.09, 1.0, [0.5, 0.5], [6, 8], seed=21)
+    par = SoapParams(alpha_max=[3, 4], l_max=4, rcut_hard=4.5, rcut_soft=4.0, basis="poly3gauss",
+                     radial_enhancement=1)
+    w = make_workload((spec, par), n_sites=5)
+    W, S = orc.basis_matrices(w)
+    n_unc = 7 * 8 // 2 * 5
+    comp = compression_indices(kind, n_unc, 9)
+    u, du, _ = orc.get_soap(w, W, S, None, do_der=True)
+    q, dq, _ = orc.get_soap(w, W, S, comp, do_der=True)
+    P = np.zeros((int(comp[0].max()), n_unc))
+    np.add.at(P, (comp[0] - 1, comp[1] - 1), comp[2])
+    x = u @ P.T
+    nrm = np.linalg.norm(x, axis=1)
+    assert rel(q, x / nrm[:, None]) < 1e-13
+    site = np.repeat(np.arange(w.n_sites), w.n_neigh)
+    dx = np.einsum("os,ksc->koc", P, du)
+    xs, ns = x[site], nrm[site]
+    proj = np.einsum("ko,koc->kc", xs, dx)
+    want = dx / ns[:, None, None] - xs[:, :, None] * proj[:, None, :] / (ns ** 3)[:, None, None]
+    assert rel(dq, want) < 1e-11
