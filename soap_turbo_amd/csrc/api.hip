@@ -34,6 +34,7 @@ struct DevBuf {
 struct st_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  size_t lds_max = 64 * 1024;  // LDS bytes one workgroup may allocate on this device
   std::string err;
 
   // basis: injected or built (cache key = parameters that change W and S)
@@ -57,7 +58,7 @@ struct st_ctx {
   DevBuf G;
 
   // FP64 work and outputs
-  DevBuf rad, rad_der, ang, ang_r, ang_a, ang_p, cnk, soap, sdp, cart, pair64, preflm, central;
+  DevBuf rad, rad_der, ang, ang_r, ang_a, ang_p, cnk, soap, sdp, cart, pair64, preflm, central, rotp;
   DevBuf forces64, vpart64, virial64;
   bool have_forces64 = false;
   DevBuf comp, out_ptr, out_c, out_el, skip, unc_ptr, unc_e, P_i, P_el;
@@ -514,6 +515,9 @@ int st_create(st_ctx** out, int device) {
   if (hipSetDevice(device) != hipSuccess) return ST_ERR_HIP;
   st_ctx* c = new st_ctx;
   c->device = device;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->lds_max = (size_t)lds;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return ST_ERR_HIP;
@@ -663,6 +667,7 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
   if ((rc = ensure(ctx, ctx->soap, (size_t)NS * n_soap * 8))) return rc;
   if ((rc = ensure(ctx, ctx->sdp, (size_t)NS * 8))) return rc;
   if (der && !ctr && (rc = ensure(ctx, ctx->cart, P * n_soap * 24))) return rc;
+  if (der && !ctr && (rc = ensure(ctx, ctx->rotp, P * 64))) return rc;
   hipStream_t st = ctx->stream;
 
   timed(ctx, "fill_rad", [&] {
@@ -714,17 +719,18 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
                      ctx->cnk.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>());
   });
   if (der && !ctr) {
-    if (st::derivatives_lds(n_max, L, k_max, n_soap) > 64 * 1024)
+    if (st::derivatives_lds(n_max, L, k_max, n_soap) > ctx->lds_max)
       return fail(ctx, ST_ERR_UNSUPPORTED, "the derivative working set exceeds the LDS");
     timed(ctx, "derivatives_f64", [&] {
-      ok = st::launch_derivatives(st, NS, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(), ctx->comp.as<int4>(), optr,
+      ok = st::launch_derivatives(st, NS, (int)P, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(), ctx->comp.as<int4>(), optr,
                                   ctx->out_c.as<int>(), ctx->out_el.as<double>(),
                                   ctx->single ? ctx->ocomp.as<int4>() : nullptr, ctx->oel.as<double>(),
                                   ctx->cnk.as<double2>(),
                                   ctx->rad.as<double>(), ctx->rad_der.as<double>(), ctx->ang.as<double2>(),
                                   ctx->ang_r.as<double2>(), ctx->ang_a.as<double2>(), ctx->ang_p.as<double2>(),
                                   ctx->soap.as<double>(), ctx->sdp.as<double>(), ctx->rjs.as<double>(),
-                                  ctx->thetas.as<double>(), ctx->phis.as<double>(), ctx->cart.as<double>());
+                                  ctx->thetas.as<double>(), ctx->phis.as<double>(), ctx->rotp.as<double>(),
+                                  ctx->cart.as<double>());
     });
     if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "n_soap > 3072 is not supported with derivatives");
   }

@@ -506,12 +506,26 @@ __global__ void normalise_k(int n_sites, int n_soap, const double* __restrict__ 
 // and kind share one address register and differ by immediate offsets; rows
 // are padded by one double to spread adjacent (n, l) over the banks. The
 // angular rows are [k][pair][kind] (double2), padded likewise.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// tuning: pairs per batch and waves per SIMD for up to 3 outputs per thread,
+// and the largest n_soap run with 256-thread blocks
+#ifndef ST_DER_PB3
+#define ST_DER_PB3 4
+#endif
+#ifndef ST_DER_WAVES3
+#define ST_DER_WAVES3 2
+#endif
+#ifndef ST_DER_NT256_MAX
+#define ST_DER_NT256_MAX 768
+#endif
+
 template <int NSPT>
 struct DerShape {
   // component registers per thread: PB * NSPT * 3 doubles, kept near 36
-  static constexpr int PB = NSPT <= 3 ? 4 : NSPT <= 4 ? 3 : NSPT <= 6 ? 2 : 1;
+  static constexpr int PB = NSPT <= 2 ? 4 : NSPT <= 3 ? ST_DER_PB3 : NSPT <= 4 ? 3 : NSPT <= 6 ? 2 : 1;
   // waves per SIMD the register allocation is held to, for NT threads per block
-  static constexpr int waves(int nt) { return nt == 256 && NSPT <= 2 ? 3 : 2; }
+  static constexpr int waves(int nt) { return nt == 256 && NSPT <= 1 ? 3 : NSPT == 3 ? ST_DER_WAVES3 : 2; }
   static constexpr int PQS = 4 * PB + 1;   // doubles per (n, l) row
   static constexpr int RS = 2 * PB + 1;    // doubles per n row of radial values
   static constexpr int AS = 4 * PB + 1;    // double2 per k row of angular values
@@ -573,9 +587,33 @@ __device__ __forceinline__ void der_component(const int cm, int b, const double*
   to = pre * (Rn[0] * A[3] + Rp[0] * C[3]);
 }
 
+// Rotation of the spherical derivatives (d/dr, d/dtheta / r, d/dphi / (r sin theta))
+// to Cartesian components for every pair (src/soap_turbo.f90:864-876):
+// rot = {sin t cos f, sin t sin f, cos t, -cos t cos f / r, -cos t sin f / r,
+// sin t / r, -sin f / r, cos f / r}.
+__global__ void rotation_k(int n_pairs, const double* __restrict__ rjs, const double* __restrict__ thetas,
+                           const double* __restrict__ phis, double* __restrict__ rotp) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const double sin_t = sin(thetas[p]), cos_t = cos(thetas[p]);
+  const double sin_f = sin(phis[p]), cos_f = cos(phis[p]);
+  const double inv_r = 1.0 / rjs[p];
+  double* rr = rotp + (size_t)p * 8;
+  rr[0] = sin_t * cos_f; rr[1] = sin_t * sin_f; rr[2] = cos_t;
+  rr[3] = -(cos_t * cos_f * inv_r); rr[4] = -(cos_t * sin_f * inv_r); rr[5] = sin_t * inv_r;
+  rr[6] = -(sin_f * inv_r); rr[7] = cos_f * inv_r;
+}
+
 // DIRECT: every output has exactly one (component, P_el) entry (ocomp/oel);
 // otherwise outputs sum their CSR entries (out_ptr/out_c/out_el), or, with
 // no compression (out_ptr null), output s is component s.
+//
+// The batch inputs (angular rows, radial values, rotation) are double
+// buffered in LDS: the loads of batch k + 1 are issued to registers before the
+// P/Q products of batch k and written to the other buffer during its pass 1,
+// so that their latency is hidden behind a batch of work. The FA/FR registers
+// per thread cover k_max <= 48 (l_max <= 8) and n_max <= 32; larger shapes
+// load the remainder when the buffer is written.
 template <int NSPT, int NT, bool DIRECT>
 __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* __restrict__ site_ptr,
@@ -584,22 +622,83 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     const double2* __restrict__ cnk, const double* __restrict__ rad,
     const double* __restrict__ rad_der, const double2* __restrict__ ang, const double2* __restrict__ ang_r,
     const double2* __restrict__ ang_a, const double2* __restrict__ ang_p, const double* __restrict__ soap,
-    const double* __restrict__ sqrt_dot_p, const double* __restrict__ rjs, const double* __restrict__ thetas,
-    const double* __restrict__ phis, double* __restrict__ cart) {
+    const double* __restrict__ sqrt_dot_p, const double* __restrict__ rotp, double* __restrict__ cart) {
   using Sh = DerShape<NSPT>;
   constexpr int PB = Sh::PB, PQS = Sh::PQS, RS = Sh::RS, AS = Sh::AS, NRED = Sh::NRED;
+  constexpr int FA = (PB * 4 * 48 + NT - 1) / NT, FR = (PB * 2 * 32 + NT - 1) / NT;
   extern __shared__ double2 smem2[];
   constexpr int NW = NT / 64;
   __shared__ double red[NW][NRED];
-  __shared__ double rot[PB][8];
+  __shared__ double rot[2][PB][8];
   const int i = blockIdx.x;
   const int L1 = l_max + 1, nl = n_max * L1;
+  const int nA = PB * 4 * k_max, nR = PB * 2 * n_max;
   double2* cn = smem2;                                   // n_max * k_max
-  double2* sA = cn + n_max * k_max;                      // k_max * AS
-  double* sR = (double*)(sA + k_max * AS);               // n_max * RS
-  double* PQ = sR + n_max * RS;                          // nl * PQS
+  double2* sAbuf = cn + n_max * k_max;                   // 2 x k_max * AS
+  double* sRbuf = (double*)(sAbuf + 2 * k_max * AS);     // 2 x n_max * RS
+  double* PQ = sRbuf + 2 * n_max * RS;                   // nl * PQS
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+
+  // batch inputs: global -> registers (issue), registers -> LDS buffer (complete)
+  double2 fa[FA];
+  double fr[FR], fo = 0.0;
+  auto fetch = [&](int pb) {
+    const int nb = min(PB, p1 - pb);
+#pragma unroll
+    for (int it = 0; it < FA; ++it) {
+      const int t = threadIdx.x + it * NT;
+      fa[it] = make_double2(0.0, 0.0);
+      if (t < nA) {
+        const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
+        const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
+        if (b < nb) fa[it] = src[(size_t)(pb + b) * k_max + k];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < FR; ++it) {
+      const int t = threadIdx.x + it * NT;
+      fr[it] = 0.0;
+      if (t < nR) {
+        const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
+        if (b < nb) fr[it] = (d == 0 ? rad : rad_der)[(size_t)(pb + b) * n_max + n];
+      }
+    }
+    fo = 0.0;
+    if (threadIdx.x < PB * 8 && (int)threadIdx.x / 8 < nb) fo = rotp[(size_t)pb * 8 + threadIdx.x];
+  };
+  auto commit = [&](int pb, int buf) {
+    const int nb = min(PB, p1 - pb);
+    double2* sA = sAbuf + buf * k_max * AS;
+    double* sR = sRbuf + buf * n_max * RS;
+#pragma unroll
+    for (int it = 0; it < FA; ++it) {
+      const int t = threadIdx.x + it * NT;
+      if (t < nA) {
+        const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
+        sA[k * AS + 4 * b + kind] = fa[it];
+      }
+    }
+    for (int t = threadIdx.x + FA * NT; t < nA; t += NT) {
+      const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
+      const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
+      sA[k * AS + 4 * b + kind] = b < nb ? src[(size_t)(pb + b) * k_max + k] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int it = 0; it < FR; ++it) {
+      const int t = threadIdx.x + it * NT;
+      if (t < nR) {
+        const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
+        sR[n * RS + 2 * b + d] = fr[it];
+      }
+    }
+    for (int t = threadIdx.x + FR * NT; t < nR; t += NT) {
+      const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
+      sR[n * RS + 2 * b + d] = b < nb ? (d == 0 ? rad : rad_der)[(size_t)(pb + b) * n_max + n] : 0.0;
+    }
+    if (threadIdx.x < PB * 8) rot[buf][threadIdx.x / 8][threadIdx.x % 8] = fo;
+  };
+
   for (int t = threadIdx.x; t < n_max * k_max; t += NT) cn[t] = cnk[(size_t)i * n_max * k_max + t];
   const double sdp = sqrt_dot_p[i];
   const double inv_p = 1.0 / sdp, inv_p3 = 1.0 / (sdp * sdp * sdp);
@@ -622,52 +721,66 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
       elu[u] = 0.0;
     }
   }
-  for (int pb = p0 + 1; pb < p1; pb += PB) {
+  if (p0 + 1 < p1) {
+    fetch(p0 + 1);
+    commit(p0 + 1, 0);
+  }
+  int buf = 0;
+  for (int pb = p0 + 1; pb < p1; pb += PB, buf ^= 1) {
     const int nb = min(PB, p1 - pb);
+    const bool more = pb + PB < p1;
+    const double2* sA = sAbuf + buf * k_max * AS;
+    const double* sR = sRbuf + buf * n_max * RS;
     __syncthreads();
-    for (int t = threadIdx.x; t < PB * 4 * k_max; t += NT) {
-      const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
-      const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
-      sA[k * AS + 4 * b + kind] = b < nb ? src[(size_t)(pb + b) * k_max + k] : make_double2(0.0, 0.0);
-    }
-    for (int t = threadIdx.x; t < PB * 2 * n_max; t += NT) {
-      const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
-      const double* src = d == 0 ? rad : rad_der;
-      sR[n * RS + 2 * b + d] = b < nb ? src[(size_t)(pb + b) * n_max + n] : 0.0;
-    }
-    if (threadIdx.x < nb) {
-      const int p = pb + threadIdx.x;
-      const double sin_t = sin(thetas[p]), cos_t = cos(thetas[p]);
-      const double sin_f = sin(phis[p]), cos_f = cos(phis[p]);
-      const double inv_r = 1.0 / rjs[p];
-      double* rr = rot[threadIdx.x];
-      rr[0] = sin_t * cos_f; rr[1] = sin_t * sin_f; rr[2] = cos_t;
-      rr[3] = -(cos_t * cos_f * inv_r); rr[4] = -(cos_t * sin_f * inv_r); rr[5] = sin_t * inv_r;
-      rr[6] = -(sin_f * inv_r); rr[7] = cos_f * inv_r;
-    }
-    __syncthreads();
-    // P, Q, Qa, Qo(n, l) for every pair of the batch: sum over m of w(m) Re(conj(A) c)
-    for (int x = threadIdx.x; x < nl; x += NT) {
-      const int n = x / L1, l = x - n * L1;
-      const int kb = l * (l + 1) / 2;
-      double acc[4 * PB];
+    if (more) fetch(pb + PB);
+    // P, Q, Qa, Qo(n, l) for every pair of the batch: sum over m of w(m) Re(conj(A) c), as one
+    // block-diagonal product per l on the FP64 matrix cores: rows n (tiles of 16), columns
+    // j = 4 pair + kind (16), K = the 2 (l + 1) real terms (re, im of each m), padded to 4.
+    {
+      const int ntile = (n_max + 15) >> 4, njob = ntile * L1;
+      const int r16 = lane & 15, g = lane >> 4;
+      // jobs in order of falling l, dealt to the waves in a snake, so that their K steps even out
+      for (int q = 0; q < njob; ++q) {
+        const int round = q / NW, pos = q - round * NW;
+        if ((round & 1 ? NW - 1 - pos : pos) != w) continue;
+        const int l = l_max - q / ntile, tile = q - (q / ntile) * ntile;
+        const int kb = l * (l + 1) / 2, nK = 2 * (l + 1), nsteps = (nK + 3) >> 2;
+        const int n = tile * 16 + r16;
+        constexpr int MAXS = 7;   // l_max <= 12
+        double av[MAXS], bv[MAXS];
 #pragma unroll
-      for (int j = 0; j < 4 * PB; ++j) acc[j] = 0.0;
-      for (int m = 0; m <= l; ++m) {
-        const double wm = m == 0 ? 1.0 : 2.0;
-        const double2 cv = cn[n * k_max + kb + m];
-        const double2* a = sA + (kb + m) * AS;
-        // all loads of the row first, so that they are in flight together
-        double2 av[4 * PB];
+        for (int st = 0; st < MAXS; ++st) {
+          const int kappa = 4 * st + g, m = kappa >> 1;
+          av[st] = 0.0;
+          bv[st] = 0.0;
+          if (st < nsteps && kappa < nK) {
+            const double wm = m == 0 ? 1.0 : 2.0;
+            if (n < n_max) {
+              const double2 c = cn[n * k_max + kb + m];
+              av[st] = wm * (kappa & 1 ? c.y : c.x);
+            }
+            if (r16 < 4 * PB) {
+              const double2 t = sA[(kb + m) * AS + r16];
+              bv[st] = kappa & 1 ? t.y : t.x;
+            }
+          }
+        }
+        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int j = 0; j < 4 * PB; ++j) av[j] = a[j];
+        for (int st = 0; st < MAXS; ++st)
+          if (st < nsteps) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st], bv[st], acc, 0, 0, 0);
+        // D(row, col) sits in lane col + 16 (row % 4), element row / 4
+        if (r16 < 4 * PB) {
 #pragma unroll
-        for (int j = 0; j < 4 * PB; ++j) acc[j] = acc[j] + wm * (av[j].x * cv.x + av[j].y * cv.y);
+          for (int v = 0; v < 4; ++v) {
+            const int nn = tile * 16 + g + 4 * v;
+            if (nn < n_max) PQ[(nn * L1 + l) * PQS + r16] = acc[v];
+          }
+        }
       }
-#pragma unroll
-      for (int j = 0; j < 4 * PB; ++j) PQ[x * PQS + j] = acc[j];
     }
     __syncthreads();
+    if (more) commit(pb + PB, buf ^ 1);
     // pass 1: the components of every pair of the batch, and q . d
     double v[PB][NSPT][3];
     double dots[NRED];
@@ -715,7 +828,7 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
         for (int q = 1; q < NW; ++q) {
           dot_r += red[q][3 * b]; dot_a += red[q][3 * b + 1]; dot_o += red[q][3 * b + 2];
         }
-        const double* rr = rot[b];
+        const double* rr = rot[buf][b];
         const size_t p = pb + b;
 #pragma unroll
         for (int u = 0; u < NSPT; ++u) {
@@ -957,7 +1070,7 @@ void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqr
 // (outputs per thread, threads per block) for n_soap outputs: 256 threads up
 // to 768 outputs, 512 threads up to 3072.
 static void der_config(int n_soap, int& nspt, int& nt) {
-  nt = n_soap <= 768 ? 256 : 512;
+  nt = n_soap <= ST_DER_NT256_MAX ? 256 : 512;
   nspt = (n_soap + nt - 1) / nt;
   if (nspt == 5) nspt = 6;
 }
@@ -965,39 +1078,45 @@ static void der_config(int n_soap, int& nspt, int& nt) {
 static int der_pb(int n_soap) {
   int nspt, nt;
   der_config(n_soap, nspt, nt);
-  return nspt <= 3 ? DerShape<3>::PB : nspt <= 4 ? DerShape<4>::PB : DerShape<6>::PB;
+  return nspt == 1 ? DerShape<1>::PB : nspt == 2 ? DerShape<2>::PB : nspt == 3 ? DerShape<3>::PB
+       : nspt == 4 ? DerShape<4>::PB : DerShape<6>::PB;
 }
 
 size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap) {
   const int pb = der_pb(n_soap);
-  return sizeof(double2) * (n_max * k_max + k_max * (4 * pb + 1)) +
-         sizeof(double) * (n_max * (2 * pb + 1) + n_max * (l_max + 1) * (4 * pb + 1));
+  return sizeof(double2) * (n_max * k_max + 2 * k_max * (4 * pb + 1)) +
+         sizeof(double) * (2 * n_max * (2 * pb + 1) + n_max * (l_max + 1) * (4 * pb + 1));
 }
 
-bool launch_derivatives(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* site_ptr,
-                        const int4* comp, const int* out_ptr, const int* out_c, const double* out_el,
-                        const int4* ocomp, const double* oel,
+bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int k_max, int n_soap,
+                        const int* site_ptr, const int4* comp, const int* out_ptr, const int* out_c,
+                        const double* out_el, const int4* ocomp, const double* oel,
                         const double2* cnk, const double* rad, const double* rad_der, const double2* ang,
                         const double2* ang_r, const double2* ang_a, const double2* ang_p, const double* soap,
                         const double* sqrt_dot_p, const double* rjs, const double* thetas, const double* phis,
-                        double* cart) {
+                        double* rotp, double* cart) {
   if (n_sites == 0) return true;
   if (n_soap > 3072) return false;
+  if (n_pairs > 0) rotation_k<<<(n_pairs + 255) / 256, 256, 0, st>>>(n_pairs, rjs, thetas, phis, rotp);
   int nspt, nt;
   der_config(n_soap, nspt, nt);
   size_t sh = derivatives_lds(n_max, l_max, k_max, n_soap);
 #define DL(N, T)                                                                                                  \
   if (nspt == N && nt == T) {                                                                                     \
+    if (sh > 64 * 1024) {                                                                                         \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&derivatives_k<N, T, true>),                       \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                             \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&derivatives_k<N, T, false>),                      \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                             \
+    }                                                                                                             \
     if (ocomp)                                                                                                    \
       derivatives_k<N, T, true><<<n_sites, T, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,    \
                                                         out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der,    \
-                                                        ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rjs, thetas,  \
-                                                        phis, cart);                                              \
+                                                        ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rotp, cart);  \
     else                                                                                                          \
       derivatives_k<N, T, false><<<n_sites, T, sh, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,   \
                                                          out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der,   \
-                                                         ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rjs, thetas, \
-                                                         phis, cart);                                             \
+                                                         ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rotp, cart); \
     return true;                                                                                                  \
   }
   DL(1, 256) DL(2, 256) DL(3, 256) DL(2, 512) DL(3, 512) DL(4, 512) DL(6, 512)
