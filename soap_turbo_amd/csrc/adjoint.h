@@ -63,6 +63,7 @@ struct AdjointInputs {
   const int32_t* pair_atom;
   const double* vec;
   int n_atoms;
+  size_t lds_max;                        // LDS bytes one workgroup may allocate
 };
 
 using Timer = std::function<void(const char*, const std::function<void()>&)>;

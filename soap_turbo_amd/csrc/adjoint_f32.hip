@@ -143,157 +143,63 @@ __device__ __forceinline__ int radial_cell(const SpeciesTab& s, double r, float&
   return s.seg_base[j] + i;
 }
 
-__device__ __forceinline__ float block_sum_f(float v, float* red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  float s = 0.f;
-  for (int j = 0; j < (int)(blockDim.x >> 6); ++j) s += red[j];
-  return s;
-}
-
-// Angular quantities of one pair, row by row in l. For every (l, m) it calls
-// f(k, l, m, Plm, dms, dsin) with preflm folded in, where dms = sin(t) dP/dcos(t)
-// and dsin = -m P/sin(t) (the reference's two derivative recursions,
-// src/soap_turbo_angular.f90:181-248).
-template <int L, class F>
-__device__ __forceinline__ void angular_rows(float x, float sq, const float* __restrict__ preflm, F&& f) {
-  float pm1[L + 3], p0[L + 3], p1[L + 3];  // rows l-1, l, l+1 (index m)
-#pragma unroll
-  for (int m = 0; m < L + 3; ++m) pm1[m] = p0[m] = p1[m] = 0.f;
-  p0[0] = 1.f;           // l = 0
-  p1[0] = x; p1[1] = -sq; // l = 1
-#pragma unroll
-  for (int l = 0; l <= L; ++l) {
-    // rows: pm1 = P_{l-1}, p0 = P_l, p1 = P_{l+1}
-    const int kb = l * (l + 1) / 2;
-#pragma unroll
-    for (int m = 0; m <= l; ++m) {
-      float part1 = m == 0 ? (l == 0 ? 0.f : -0.5f * p0[1]) : 0.5f * (float)((l + m) * (l - m + 1)) * p0[m - 1];
-      float part2 = m == l ? 0.f : -0.5f * p0[m + 1];
-      float dms = part1 + part2;
-      float dsin = 0.f;
-      if (m >= 1) dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * p1[m - 1] + 0.5f * p1[m + 1];
-      const float pf = preflm[kb + m];
-      f(kb + m, l, m, pf * p0[m], pf * dms, pf * dsin);
-    }
-    // advance: P_{l+2} from P_{l+1}, P_l
-    const int ln = l + 2;
-    float pn[L + 3];
-#pragma unroll
-    for (int m = 0; m < L + 3; ++m) pn[m] = 0.f;
-    if (ln <= L + 1) {
-#pragma unroll
-      for (int m = 0; m <= ln - 2; ++m)
-        pn[m] = ((float)(2 * ln - 1) * x * p1[m] - (float)(ln - 1 + m) * p0[m]) / (float)(ln - m);
-      pn[ln - 1] = x * (float)(2 * ln - 1) * p1[ln - 1];
-      pn[ln] = -(float)(2 * ln - 1) * sq * p1[ln - 1];
-    }
-#pragma unroll
-    for (int m = 0; m < L + 3; ++m) { pm1[m] = p0[m]; p0[m] = p1[m]; p1[m] = pn[m]; }
+// Column m of P_l^m(x) (Condon-Shortley phase) for l = m .. LT, by the
+// recursions of get_plm_array (src/soap_turbo_angular.f90:108-160): P_m^m from
+// P_{m-1}^{m-1}, P_{m+1}^m = (2m+1) x P_m^m, then the three-term recursion in
+// l. Calls f(l, P_l^m) for each l.
+template <int LT, class F>
+__device__ __forceinline__ void plm_column(int m, float x, float sq, F&& f) {
+  float pmm = 1.f;
+  for (int j = 1; j <= m; ++j) pmm = -(float)(2 * j - 1) * sq * pmm;
+  f(m, pmm);
+  if (m + 1 > LT) return;
+  float p1 = x * (float)(2 * m + 1) * pmm, p0 = pmm;
+  f(m + 1, p1);
+  for (int l = m + 2; l <= LT; ++l) {
+    const float pn = ((float)(2 * l - 1) * x * p1 - (float)(l - 1 + m) * p0) / (float)(l - m);
+    p0 = p1;
+    p1 = pn;
+    f(l, pn);
   }
 }
 
-// P_l^m(x) for l <= L + 1 into out[l (l + 1) / 2 + m] (Condon-Shortley phase,
-// the recursion of get_plm_array, src/soap_turbo_angular.f90:108-160).
-template <int L>
-__device__ __forceinline__ void plm_rows(float x, float sq, float* out) {
-  float pm[L + 3], pc[L + 3];
-#pragma unroll
-  for (int m = 0; m < L + 3; ++m) pm[m] = pc[m] = 0.f;
-  pm[0] = 1.f;             // l = 0
-  pc[0] = x; pc[1] = -sq;  // l = 1
-  out[0] = 1.f;
-  out[1] = x;
-  out[2] = -sq;
-#pragma unroll
-  for (int l = 2; l <= L + 1; ++l) {
-    float pn[L + 3];
-#pragma unroll
-    for (int m = 0; m < L + 3; ++m) pn[m] = 0.f;
-#pragma unroll
-    for (int m = 0; m <= l - 2; ++m)
-      pn[m] = ((float)(2 * l - 1) * x * pc[m] - (float)(l - 1 + m) * pm[m]) / (float)(l - m);
-    pn[l - 1] = x * (float)(2 * l - 1) * pc[l - 1];
-    pn[l] = -(float)(2 * l - 1) * sq * pc[l - 1];
-#pragma unroll
-    for (int m = 0; m <= l; ++m) out[l * (l + 1) / 2 + m] = pn[m];
-#pragma unroll
-    for (int m = 0; m < L + 3; ++m) { pm[m] = pc[m]; pc[m] = pn[m]; }
-  }
+// R_n(r) of one pair for n in species block s (table sp), times scale (and the
+// central weight for the site's own pair), zero unless the pair is of that
+// species and inside its cutoff; optionally dR_n/dr.
+__device__ __forceinline__ float radial_one(const SpeciesTab& sp, int s, const float* __restrict__ rtab, double rd,
+                                            uint32_t msk, bool self, float scale, int n, float* dR) {
+  if (dR) *dR = 0.f;
+  if (!(msk & (1u << s)) || !(rd < sp.rcut) || (self && !sp.do_central)) return 0.f;
+  float t, ih;
+  const int cell = radial_cell(sp, rd, t, ih);
+  const float* c = rtab + sp.rtab_off + (size_t)cell * 4 * sp.a;
+  const int q = n - sp.off;
+  const float c0 = c[q], c1 = c[sp.a + q], c2 = c[2 * sp.a + q], c3 = c[3 * sp.a + q];
+  if (dR) *dR = (c1 + t * (2.f * c2 + 3.f * t * c3)) * ih;
+  const float sc = (self ? sp.cw : 1.f) * scale;
+  return sc * (c0 + t * (c1 + t * (c2 + t * c3)));
 }
 
-// Per-pair radial values R_n(r) for the species blocks the pair belongs to
-// (n in [nlo, nhi) only), and optionally dR_n/dr; zero elsewhere.
-template <int NR>
-__device__ __forceinline__ void radial_eval(const AdjParams& P, const float* __restrict__ rtab, double rd,
-                                            uint32_t msk, bool self, int nlo, int nhi, float* Rn, float* Rd) {
-#pragma unroll
-  for (int j = 0; j < NR; ++j) { Rn[j] = 0.f; if (Rd) Rd[j] = 0.f; }
-  for (int s = 0; s < P.n_species; ++s) {
-    const SpeciesTab& sp = P.sp[s];
-    if (!(msk & (1u << s)) || !(rd < sp.rcut)) continue;
-    if (self && !sp.do_central) continue;
-    const int lo = max(sp.off, nlo), hi = min(sp.off + sp.a, nhi);
-    if (lo >= hi) continue;
-    float t, ih;
-    const int cell = radial_cell(sp, rd, t, ih);
-    const float* c = rtab + sp.rtab_off + (size_t)cell * 4 * sp.a;
-    const float sc = self ? sp.cw : 1.f;
-#pragma unroll
-    for (int j = 0; j < NR; ++j) {
-      const int n = nlo + j;
-      if (n >= lo && n < hi) {
-        const int q = n - sp.off;
-        const float c0 = c[q], c1 = c[sp.a + q], c2 = c[2 * sp.a + q], c3 = c[3 * sp.a + q];
-        Rn[j] = sc * (c0 + t * (c1 + t * (c2 + t * c3)));
-        if (Rd) Rd[j] = (c1 + t * (2.f * c2 + 3.f * t * c3)) * ih;
-      }
-    }
-  }
-}
-
-// The pair's whole radial row R_n(r) (times scale) and optionally dR_n/dr,
-// written to R[0..n_max) / Rd; each species' table cell is located once.
-__device__ __forceinline__ void radial_row(const AdjParams& P, const float* __restrict__ rtab, double rd, uint32_t msk,
-                                           bool self, float scale, float* R, float* Rd) {
-  for (int n = 0; n < P.n_max; ++n) {
-    R[n] = 0.f;
-    if (Rd) Rd[n] = 0.f;
-  }
-  for (int s = 0; s < P.n_species; ++s) {
-    const SpeciesTab& sp = P.sp[s];
-    if (!(msk & (1u << s)) || !(rd < sp.rcut)) continue;
-    if (self && !sp.do_central) continue;
-    float t, ih;
-    const int cell = radial_cell(sp, rd, t, ih);
-    const float* c = rtab + sp.rtab_off + (size_t)cell * 4 * sp.a;
-    const float sc = (self ? sp.cw : 1.f) * scale;
-    for (int q = 0; q < sp.a; ++q) {
-      const float c0 = c[q], c1 = c[sp.a + q], c2 = c[2 * sp.a + q], c3 = c[3 * sp.a + q];
-      R[sp.off + q] = sc * (c0 + t * (c1 + t * (c2 + t * c3)));
-      if (Rd) Rd[sp.off + q] = (c1 + t * (2.f * c2 + 3.f * t * c3)) * ih;
-    }
-  }
-}
-
-// Angular radial factors g_l(r) and dg_l/dr of the pair's first species.
-template <int L>
-__device__ __forceinline__ void angular_factors(const AdjParams& P, const float* __restrict__ atab, double rd, int s0,
-                                                float* g, float* gd) {
+// Cubic-table cell of the angular radial factors g_l(r) of a species whose
+// table starts atab_off floats into atab.
+__device__ __forceinline__ const float* ang_cell(const AdjParams& P, const float* __restrict__ atab, int L1, double rd,
+                                                 int atab_off, float& ta) {
   const double xa = rd * P.ang_inv_h;
   int ia = (int)xa;
   if (ia > P.ang_n - 2) ia = P.ang_n - 2;
-  const float ta = (float)(xa - (double)ia);
-  const float* gc = atab + P.sp[s0].atab_off + (size_t)ia * 4 * (L + 1);
-#pragma unroll
-  for (int l = 0; l <= L; ++l) {
-    const float c0 = gc[l], c1 = gc[L + 1 + l], c2 = gc[2 * (L + 1) + l], c3 = gc[3 * (L + 1) + l];
-    g[l] = c0 + ta * (c1 + ta * (c2 + ta * c3));
-    gd[l] = (c1 + ta * (2.f * c2 + 3.f * ta * c3)) * P.ang_ih;
+  ta = (float)(xa - (double)ia);
+  return atab + atab_off + (size_t)ia * 4 * L1;
+}
+
+// e^{-i m phi} by m complex multiplications from (cos phi, sin phi).
+__device__ __forceinline__ float2 exp_mphi(int m, float cf, float sf) {
+  float er = 1.f, ei = 0.f;
+  for (int j = 1; j <= m; ++j) {
+    const float nr = er * cf + ei * sf, ni = ei * cf - er * sf;
+    er = nr;
+    ei = ni;
   }
+  return make_float2(er, ei);
 }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -312,13 +218,58 @@ __device__ __forceinline__ void two_sum_acc(floatx4& hi, floatx4& lo, floatx4 x)
   hi = s;
 }
 
-constexpr int MAX_TILES_PER_WAVE = 9;  // forward cnk tiles per wavefront: n_max <= 48, l_max <= 12
+// Sums of NV values over the workgroup (every thread gets them).
+template <int NV>
+__device__ __forceinline__ void block_sums_f(float (&v)[NV], float* red) {
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    for (int o = 32; o > 0; o >>= 1) v[j] += __shfl_xor(v[j], o, 64);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) red[4 * j + w] = v[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = red[4 * j] + red[4 * j + 1] + red[4 * j + 2] + red[4 * j + 3];
+}
 
-// One site per workgroup of 256 threads (4 wavefronts). The GEMM-shaped steps
-// run on FP32 MFMA: cnk = sum_pairs 4 pi R^T A, the per-l Hermitian products
-// S_l = C_l W C_l^T of the spectrum, H_l = Y_l C_l, and U = R H, V = R' H.
+// x of the neighbouring lane (lanes 2j and 2j + 1 exchange), by DPP quad_perm [1, 0, 3, 2].
+__device__ __forceinline__ float swap_adjacent(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+
+// Sum of x over the 16 lanes of a DPP row, in the row's last lane: prefix sums by
+// row_shr 1, 2, 4, 8 (lanes shifted in from outside the row read 0).
+__device__ __forceinline__ float row_total(float x) {
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xF, 0xF, false));
+  return x;
+}
+
+constexpr int PTS = PT + 1;  // LDS stride of pair-fastest rows (odd: adjacent rows on other banks)
+
 template <int L>
-__global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
+struct SiteShape {
+  static constexpr int KM = (L + 1) * (L + 2) / 2;    // (l, m) with m <= l <= L
+  static constexpr int KM1 = (L + 2) * (L + 3) / 2;   // ... l <= L + 1
+  static constexpr int L1 = L + 1;
+  static constexpr int CSTR = ((2 * KM + 15) / 16) * 16;  // columns of C and H: 2k + {re, im}, padded
+  static constexpr int CT = CSTR / 16;
+};
+
+// One site per workgroup of 256 threads (4 wavefronts); NTT = ceil(n_max / 16).
+// Per tile of up to 64 neighbours, the pairs' geometry is staged in LDS once;
+// then every thread takes part in the set-up: (n, pair) radial items and
+// (l, pair) angular-factor items, whose table loads are all issued before any
+// is used, then (m, pair) columns of P_l^m. The GEMM-shaped steps run on FP32
+// MFMA with operands loaded ahead: cnk = sum_pairs 4 pi R^T A, the per-l
+// Hermitian products S_l = C_l W C_l^T of the spectrum, H_l = Y_l C_l (Y_l
+// formed from dE/dp on the fly; H written over C), and U = R H, V = R' H.
+template <int L, int NTT>
+__global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     AdjParams P, const int* __restrict__ order, const int* __restrict__ site_ptr, const float* __restrict__ rtab,
     const float* __restrict__ atab, const float* __restrict__ preflm_g, const float* __restrict__ central,
     const int32_t* __restrict__ site_species, int max_mult, const int32_t* __restrict__ species_mult,
@@ -329,93 +280,158 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
     const float* __restrict__ P_el, const int4* __restrict__ ocomp, const float* __restrict__ oel,
     const int* __restrict__ umap, const float* __restrict__ uel, float* __restrict__ q_out,
     float* __restrict__ sdp_out, float* __restrict__ pair_out) {
-  constexpr int KM = (L + 1) * (L + 2) / 2;
-  constexpr int KM1 = (L + 2) * (L + 3) / 2;
-  constexpr int KS = KM1 | 1;                    // odd row stride of the per-pair P_l^m rows
-  constexpr int L1 = L + 1;
-  constexpr int L1S = L1 | 1;
-  constexpr int CSTR = ((2 * KM + 15) / 16) * 16; // columns of C and H: 2k + {re, im}, padded
-  constexpr int CT = CSTR / 16;
+  using Sh = SiteShape<L>;
+  constexpr int KM = Sh::KM, KM1 = Sh::KM1, L1 = Sh::L1, CSTR = Sh::CSTR, CT = Sh::CT;
+  constexpr int NC = 2 * KM;                        // live columns
+  constexpr int TPW = (NTT * CT + 3) / 4;           // cnk tiles per wavefront
+  constexpr int KN = 4 * NTT;                       // K steps over n (padded to 16 NTT)
+  constexpr int RIT = 4 * NTT;                      // (n, pair) radial items per thread
+  constexpr int GIT = (L1 + 3) / 4;                 // (l, pair) items per thread, l <= L
+  constexpr int PIT = (L + 5) / 4;                  // (m, pair) items per thread, m <= L + 1
+  constexpr uint32_t SELF = 1u << 31;               // pair flag: the site's own pair
   extern __shared__ float smem[];
   __shared__ float red[16];
   __shared__ float preflm[KM];
   __shared__ int klm[KM];
+  __shared__ double pr_rd[PT];
+  __shared__ uint32_t pr_msk[PT];                   // species bits (0: no contribution) | SELF
+  __shared__ SpeciesTab spl[ADJ_MAX_SPECIES];       // the species tables, indexed per lane from LDS
+  __shared__ unsigned char nsp[64];                 // species block of each n
+  __shared__ float pr_st[PT], pr_ct[PT], pr_sf[PT], pr_cf[PT];
   const int i = order ? order[blockIdx.x] : blockIdx.x;
   const int n_max = P.n_max;
-  const int NT = (n_max + 15) / 16;
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const float fourpi = 12.566370614359172f;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
-  float* C = smem;                                 // n_max * CSTR
-  float* H = C + n_max * CSTR;                     // n_max * CSTR
-  float* region = H + n_max * CSTR;
-  float* Rt = region;                              // forward: PT * n_max
-  float* At = Rt + PT * n_max;                     //          PT * CSTR
-  float* Sl = region;                              // spectrum: L1 * n_max * n_max (then Y_l)
+  float* C = smem;                                 // n_max * CSTR (C, then H)
+  float* region = C + n_max * CSTR;
+  float* Rt = region;                              // forward: n_max * PTS (4 pi R, pair fastest)
+  float* Gt = Rt + n_max * PTS;                    //          L1 * PTS (g_l)
+  float* At = Gt + L1 * PTS;                       //          NC * PTS (A columns, pair fastest)
+  float* Sl = region;                              // spectrum: L1 * n_max * n_max
   float* qv = Sl + L1 * n_max * n_max;             //           n_soap
   float* Gp = qv + P.n_soap;                       //           n_soap
   float* Gu = Gp + P.n_soap;                       //           n_soap_unc
-  float* Rb = region;                              // backward: PT * n_max (R)
-  float* Rdb = Rb + PT * n_max;                    //           PT * n_max (dR/dr)
-  float* Fp = Rdb + PT * n_max;                    //           PT * KS (P_l^m, l <= L + 1)
-  float* Gl = Fp + PT * KS;                        //           PT * L1S (g_l)
-  float* Gd = Gl + PT * L1S;                       //           PT * L1S (dg_l/dr)
-  float2* Em = reinterpret_cast<float2*>(Gd + PT * L1S);  //    PT * L1S (e^{-i m phi})
-  float* fout = reinterpret_cast<float*>(Em + PT * L1S);  //    PT * 3
+  float* Rb = region;                              // backward: n_max * PTS (R)
+  float* Rdb = Rb + n_max * PTS;                   //           n_max * PTS (dR/dr)
+  float* Fp = Rdb + n_max * PTS;                   //           KM1 * PTS (P_l^m, l <= L + 1)
+  float* Gl = Fp + ((KM1 * PTS + 1) & ~1);         //           L1 * PTS (g_l)
+  float* Gd = Gl + L1 * PTS;                       //           L1 * PTS (dg_l/dr)
+  float2* Em = reinterpret_cast<float2*>(Gd + L1 * PTS);  //    L1 * PTS (e^{-i m phi})
+  float* fout = reinterpret_cast<float*>(Em + L1 * PTS);  //    3 * PT
 
-  for (int t = threadIdx.x; t < KM; t += blockDim.x) {
+  // the tile's pairs: r, species bits of contributing pairs, sin/cos of theta and phi
+  auto stage = [&](int tb, bool with_self) {
+    if (threadIdx.x < PT) {
+      const int p = tb + threadIdx.x;
+      uint32_t msk = 0;
+      double rd = 1.0, th = 0.0, ph = 0.0;
+      if (p < p1) {
+        rd = rjs[p];
+        th = thetas[p];
+        ph = phis[p];
+        msk = spmask[p];
+        if (!(rd < P.rcut_max) || (p == p0 && !with_self)) msk = 0;
+        if (msk != 0 && p == p0) msk |= SELF;
+      }
+      float st_, ct_, sf_, cf_;
+      sincosf((float)th, &st_, &ct_);
+      sincosf((float)ph, &sf_, &cf_);
+      pr_rd[threadIdx.x] = rd;
+      pr_msk[threadIdx.x] = msk;
+      pr_st[threadIdx.x] = st_;
+      pr_ct[threadIdx.x] = ct_;
+      pr_sf[threadIdx.x] = sf_;
+      pr_cf[threadIdx.x] = cf_;
+    }
+  };
+  // (n, pair) radial items: R_n times scale, and dR_n/dr if Rd is given
+  auto radial_items = [&](float scale, float* R, float* Rd) {
+#pragma unroll
+    for (int u = 0; u < RIT; ++u) {
+      const int t = threadIdx.x + NTHREADS * u, n = t / PT, pl = t - n * PT;
+      if (n < n_max) {
+        const uint32_t msk = pr_msk[pl];
+        float v = 0.f, d = 0.f;
+        if (msk != 0) {
+          const int s = nsp[n];
+          v = radial_one(spl[s], s, rtab, pr_rd[pl], msk & ~SELF, (msk & SELF) != 0, scale, n, Rd ? &d : nullptr);
+        }
+        R[n * PTS + pl] = v;
+        if (Rd) Rd[n * PTS + pl] = d;
+      }
+    }
+  };
+  // (l, pair) items: g_l(r), and dg_l/dr if Gdr is given
+  auto gl_items = [&](float* Gr, float* Gdr) {
+#pragma unroll
+    for (int u = 0; u < GIT; ++u) {
+      const int t = threadIdx.x + NTHREADS * u, l = t / PT, pl = t - l * PT;
+      if (l < L1) {
+        const uint32_t msk = pr_msk[pl];
+        float g = 0.f, gd = 0.f;
+        if (msk != 0) {
+          float ta;
+          const float* gc = ang_cell(P, atab, L1, pr_rd[pl], spl[__ffs(msk & ~SELF) - 1].atab_off, ta);
+          const float c0 = gc[l], c1 = gc[L1 + l], c2 = gc[2 * L1 + l], c3 = gc[3 * L1 + l];
+          g = c0 + ta * (c1 + ta * (c2 + ta * c3));
+          gd = (c1 + ta * (2.f * c2 + 3.f * ta * c3)) * P.ang_ih;
+        }
+        Gr[l * PTS + pl] = g;
+        if (Gdr) Gdr[l * PTS + pl] = gd;
+      }
+    }
+  };
+
+  for (int t = threadIdx.x; t < KM; t += NTHREADS) {
     preflm[t] = preflm_g[t];
     int l = 0;
     while ((l + 1) * (l + 2) / 2 <= t) ++l;
     klm[t] = l | ((t - l * (l + 1) / 2) << 8);
   }
+  // uniform-index copies only: a per-lane index into the kernel arguments would
+  // move them to scratch memory
+  if (threadIdx.x == 0) {
+    for (int s = 0; s < P.n_species; ++s) {
+      spl[s] = P.sp[s];
+      for (int q = 0; q < P.sp[s].a; ++q) nsp[P.sp[s].off + q] = (unsigned char)s;
+    }
+  }
   // --------------------------------------------------------------- forward
-  const int n_tiles = NT * CT;
-  floatx4 hi[MAX_TILES_PER_WAVE], lo[MAX_TILES_PER_WAVE];
+  floatx4 hi[TPW], lo[TPW];
 #pragma unroll
-  for (int j = 0; j < MAX_TILES_PER_WAVE; ++j) {
+  for (int j = 0; j < TPW; ++j) {
     hi[j] = floatx4{0.f, 0.f, 0.f, 0.f};
     lo[j] = floatx4{0.f, 0.f, 0.f, 0.f};
   }
   for (int tb = p0; tb < p1; tb += PT) {
     const int np_ = min(PT, p1 - tb);
     __syncthreads();
-    {
-      const int p = tb + lane;
-      const bool live = lane < np_ && spmask[p] != 0 && rjs[p] < P.rcut_max;
-      if (wv == 0) {
-        // radial values of the pair, 4 pi folded in
-        float* R = Rt + lane * n_max;
-        if (live) {
-          radial_row(P, rtab, rjs[p], spmask[p], p == p0, fourpi, R, nullptr);
-        } else {
-          for (int n = 0; n < n_max; ++n) R[n] = 0.f;
-        }
-      } else if (wv == 1) {
-        // angular coefficients A_k = preflm P_l^m g_l e^{-i m phi}, as (re, im) columns
-        float* A = At + lane * CSTR;
-        if (live) {
-          float g[L1], gd[L1];
-          angular_factors<L>(P, atab, rjs[p], __ffs(spmask[p]) - 1, g, gd);
-          float st_, ct_, sf_, cf_;
-          sincosf((float)thetas[p], &st_, &ct_);
-          sincosf((float)phis[p], &sf_, &cf_);
-          float er[L1], ei[L1];
-          er[0] = 1.f; ei[0] = 0.f;
+    stage(tb, true);
+    __syncthreads();
+    radial_items(fourpi, Rt, nullptr);
+    gl_items(Gt, nullptr);
+    __syncthreads();
+    // (m, pair) items: A_k = preflm P_l^m g_l e^{-i m phi} for l = m .. L, as (re, im) rows
 #pragma unroll
-          for (int m = 1; m <= L; ++m) {
-            er[m] = er[m - 1] * cf_ + ei[m - 1] * sf_;
-            ei[m] = ei[m - 1] * cf_ - er[m - 1] * sf_;
-          }
-          angular_rows<L>(ct_, st_, preflm, [&](int k, int l, int m, float pl, float, float) {
-            const float v = pl * g[l];
-            A[2 * k] = v * er[m];
-            A[2 * k + 1] = v * ei[m];
+    for (int u = 0; u < GIT; ++u) {
+      const int t = threadIdx.x + NTHREADS * u, m = t / PT, pl = t - m * PT;
+      if (m < L1) {
+        if (pr_msk[pl] != 0) {
+          const float2 e = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
+          plm_column<L>(m, pr_ct[pl], pr_st[pl], [&](int l, float plm) {
+            const int k = l * (l + 1) / 2 + m;
+            const float v = preflm[k] * plm * Gt[l * PTS + pl];
+            At[(2 * k) * PTS + pl] = v * e.x;
+            At[(2 * k + 1) * PTS + pl] = v * e.y;
           });
-          for (int c = 2 * KM; c < CSTR; ++c) A[c] = 0.f;
         } else {
-          for (int c = 0; c < CSTR; ++c) A[c] = 0.f;
+          for (int l = m; l <= L; ++l) {
+            const int k = l * (l + 1) / 2 + m;
+            At[(2 * k) * PTS + pl] = 0.f;
+            At[(2 * k + 1) * PTS + pl] = 0.f;
+          }
         }
       }
     }
@@ -423,22 +439,24 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
     // cnk: the tile's pairs in chunks of 16 neighbours, each chunk on MFMA, chunks
     // combined with TwoSum compensation
     const int npad = (np_ + 3) & ~3;
+    for (int c0 = 0; c0 < npad; c0 += 16) {
+      const int ns = (min(npad, c0 + 16) - c0) >> 2;
 #pragma unroll
-    for (int j = 0; j < MAX_TILES_PER_WAVE; ++j) {
-      const int t = wv + 4 * j;
-      if (t < n_tiles) {
-        const int a = t / CT, b = t - a * CT;
-        const int n = 16 * a + li;
-        const int col = 16 * b + li;
-        for (int c0 = 0; c0 < npad; c0 += 16) {
-          floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-          const int c1 = min(npad, c0 + 16);
-          for (int pk = c0; pk < c1; pk += 4) {
-            const int pp = pk + lk;
-            const float av = n < n_max ? Rt[pp * n_max + n] : 0.f;
-            const float bv = At[pp * CSTR + col];
-            acc = mfma4(av, bv, acc);
+      for (int j = 0; j < TPW; ++j) {
+        const int t = wv + 4 * j;
+        if (t < NTT * CT) {
+          const int a = t / CT, b = t - a * CT;
+          const int n = 16 * a + li, col = 16 * b + li;
+          float av[4], bv[4];
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int pp = c0 + 4 * s + lk;
+            av[s] = (s < ns && n < n_max) ? Rt[n * PTS + pp] : 0.f;
+            bv[s] = (s < ns && col < NC) ? At[col * PTS + pp] : 0.f;
           }
+          floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int s = 0; s < 4; ++s) acc = mfma4(av[s], bv[s], acc);
           two_sum_acc(hi[j], lo[j], acc);
         }
       }
@@ -446,9 +464,9 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
   }
   // C = hi + lo, plus the poly3gauss central-atom term on k = 0 (real part)
 #pragma unroll
-  for (int j = 0; j < MAX_TILES_PER_WAVE; ++j) {
+  for (int j = 0; j < TPW; ++j) {
     const int t = wv + 4 * j;
-    if (t < n_tiles) {
+    if (t < NTT * CT) {
       const int a = t / CT, b = t - a * CT;
       const int col = 16 * b + li;
 #pragma unroll
@@ -466,249 +484,292 @@ __global__ __launch_bounds__(NTHREADS, 2) void adjoint_site_k(
   }
   __syncthreads();
   // --------------------------------------------------------------- spectrum
-  // S_l(n, n') = sum over the l block of w(m) C(n, c) C(n', c), w = 1 (m = 0), 2 (m > 0)
-  for (int job = wv; job < L1 * NT * NT; job += 4) {
-    const int l = job / (NT * NT), ab = job - l * NT * NT, a = ab / NT, b = ab - a * NT;
-    const int cb = l * (l + 1), cw = 2 * (l + 1);
-    const int n = 16 * a + li, np = 16 * b + li;
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int c4 = 0; c4 < cw; c4 += 4) {
-      const int c = c4 + lk;
-      const bool inb = c < cw;
-      const float av = (inb && n < n_max) ? C[n * CSTR + cb + c] : 0.f;
-      const float bv = (inb && np < n_max) ? (c >= 2 ? 2.f : 1.f) * C[np * CSTR + cb + c] : 0.f;
-      acc = mfma4(av, bv, acc);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = 16 * a + 4 * lk + v, cc = 16 * b + li;
-      if (r < n_max && cc < n_max) Sl[(l * n_max + r) * n_max + cc] = acc[v];
-    }
-  }
-  __syncthreads();
-  const int n_soap = P.n_soap;
-  float ss = 0.f;
-  for (int s = threadIdx.x; s < n_soap; s += blockDim.x) {
-    float v = 0.f;
-    if (ocomp) {
-      const int4 cm = ocomp[s];
-      v = oel[s] * (cm.x != cm.y ? 1.41421356f : 1.f) * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
-    } else {
-      const int e0 = P.compressed ? out_ptr[s] : s, e1 = P.compressed ? out_ptr[s + 1] : s + 1;
-      for (int e = e0; e < e1; ++e) {
-        const int4 cm = comp[P.compressed ? out_c[e] : e];
-        const float base = cm.x != cm.y ? 1.41421356f : 1.f;
-        v += (P.compressed ? out_el[e] : 1.f) * base * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
-      }
-    }
-    qv[s] = v;
-    ss += v * v;
-  }
-  const float tot = block_sum_f(ss, red);
-  float sdp = sqrtf(tot);
-  if (sdp < 1e-5f) sdp = 1.f;
-  const float inv = 1.f / sdp;
-  float qg_part = 0.f;
-  for (int s = threadIdx.x; s < n_soap; s += blockDim.x) {
-    const float q = qv[s] * inv;
-    qv[s] = q;
-    q_out[(size_t)i * n_soap + s] = q;
-    const float g = G32[(size_t)i * n_soap + s];
-    Gp[s] = g;
-    qg_part += q * g;
-  }
-  if (threadIdx.x == 0) sdp_out[i] = sdp;
-  const float qg = block_sum_f(qg_part, red + 4);
-  for (int s = threadIdx.x; s < n_soap; s += blockDim.x) Gp[s] = (Gp[s] - qv[s] * qg) * inv;
-  __syncthreads();
-  for (int c = threadIdx.x; c < P.n_soap_unc; c += blockDim.x) {
-    float v;
-    if (umap) {
-      const int q = umap[c];
-      v = q >= 0 ? uel[c] * Gp[q] : 0.f;
-    } else if (P.compressed) {
-      v = 0.f;
-      for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v += P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
-    } else {
-      v = Gp[c];
-    }
-    Gu[c] = v;
-  }
-  __syncthreads();
-  // Y_l(n, n') = Gu(c) x 2 on the diagonal, sqrt 2 off it; c(a, b, l) = (a n_max - a(a-1)/2 + b - a) L1 + l
-  float* Y = Sl;
-  for (int t = threadIdx.x; t < L1 * n_max * n_max; t += blockDim.x) {
-    const int l = t / (n_max * n_max), r = t - l * n_max * n_max, n = r / n_max, np = r - n * n_max;
-    const int a = n < np ? n : np, b = n < np ? np : n;
-    const float g = Gu[(a * n_max - a * (a - 1) / 2 + (b - a)) * L1 + l];
-    Y[t] = a == b ? 2.f * g : 1.41421356f * g;
-  }
-  for (int t = threadIdx.x; t < n_max * (CSTR - 2 * KM); t += blockDim.x) {
-    const int n = t / (CSTR - 2 * KM);
-    H[n * CSTR + 2 * KM + (t - n * (CSTR - 2 * KM))] = 0.f;
-  }
-  __syncthreads();
-  // H_l = 4 pi w(m) Y_l C_l, on the columns of block l
+  // S_l(n, n') = sum over the l block of w(m) C(n, c) C(n', c), w = 1 (m = 0), 2 (m > 0);
+  // tiles with a <= b (the outputs read n <= n'); a wavefront's jobs are unrolled so their
+  // loads are in flight together
   {
-    int njobs = 0;
-    for (int l = 0; l <= L; ++l) njobs += NT * ((2 * (l + 1) + 15) / 16);
-    for (int job = wv; job < njobs; job += 4) {
-      int l = 0, rem = job;
-      while (rem >= NT * ((2 * (l + 1) + 15) / 16)) { rem -= NT * ((2 * (l + 1) + 15) / 16); ++l; }
-      const int nb_ = (2 * (l + 1) + 15) / 16;
-      const int a = rem / nb_, b = rem - a * nb_;
-      const int cb = l * (l + 1), cw = 2 * (l + 1);
-      const int n = 16 * a + li, c = 16 * b + li;
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int k4 = 0; k4 < n_max; k4 += 4) {
-        const int np = k4 + lk;
-        const float av = (n < n_max && np < n_max) ? Y[(l * n_max + n) * n_max + np] : 0.f;
-        const float bv = (np < n_max && c < cw) ? C[np * CSTR + cb + c] : 0.f;
-        acc = mfma4(av, bv, acc);
-      }
-      if (c < cw) {
-        const float w = (c >= 2 ? 2.f : 1.f) * fourpi;
+    constexpr int NAB = NTT * (NTT + 1) / 2, NJ = L1 * NAB, JPW = (NJ + 3) / 4;
+    constexpr int MS = (2 * L1 + 3) / 4;
+#pragma unroll
+    for (int jj = 0; jj < JPW; ++jj) {
+      const int job = wv + 4 * jj;
+      if (job < NJ) {
+        const int l = job / NAB;
+        int ab = job - l * NAB, a = 0;
+        while (ab >= NTT - a) { ab -= NTT - a; ++a; }
+        const int b = a + ab;
+        const int cb = l * (l + 1), cw = 2 * (l + 1);
+        const int n = 16 * a + li, np = 16 * b + li;
+        float av[MS], bv[MS];
+#pragma unroll
+        for (int s = 0; s < MS; ++s) {
+          const int c = 4 * s + lk;
+          const bool inb = c < cw;
+          av[s] = (inb && n < n_max) ? C[n * CSTR + cb + c] : 0.f;
+          bv[s] = (inb && np < n_max) ? (c >= 2 ? 2.f : 1.f) * C[np * CSTR + cb + c] : 0.f;
+        }
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+          if (4 * s < cw) acc = mfma4(av[s], bv[s], acc);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int r = 16 * a + 4 * lk + v;
-          if (r < n_max) H[r * CSTR + cb + c] = w * acc[v];
+          const int r = 16 * a + 4 * lk + v, cc = 16 * b + li;
+          if (r < n_max && cc < n_max) Sl[(l * n_max + r) * n_max + cc] = acc[v];
         }
       }
     }
   }
+  __syncthreads();
+  // q, |p|, and Gp = (G - q (q.G)) / |p| from one pass: q.G = (p.G) / |p|
+  const int n_soap = P.n_soap;
+  float sums[2] = {0.f, 0.f};
+  for (int s0 = threadIdx.x; s0 < n_soap; s0 += 4 * NTHREADS) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int s = s0 + NTHREADS * u;
+      if (s < n_soap) {
+        float v = 0.f;
+        if (ocomp) {
+          const int4 cm = ocomp[s];
+          v = oel[s] * (cm.x != cm.y ? 1.41421356f : 1.f) * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
+        } else {
+          const int e0 = P.compressed ? out_ptr[s] : s, e1 = P.compressed ? out_ptr[s + 1] : s + 1;
+          for (int e = e0; e < e1; ++e) {
+            const int4 cm = comp[P.compressed ? out_c[e] : e];
+            const float base = cm.x != cm.y ? 1.41421356f : 1.f;
+            v += (P.compressed ? out_el[e] : 1.f) * base * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
+          }
+        }
+        const float g = G32[(size_t)i * n_soap + s];
+        qv[s] = v;
+        Gp[s] = g;
+        sums[0] += v * v;
+        sums[1] += v * g;
+      }
+    }
+  }
+  block_sums_f<2>(sums, red);
+  float sdp = sqrtf(sums[0]);
+  if (sdp < 1e-5f) sdp = 1.f;
+  const float inv = 1.f / sdp;
+  const float qg = sums[1] * inv;
+  for (int s = threadIdx.x; s < n_soap; s += NTHREADS) {
+    const float q = qv[s] * inv;
+    q_out[(size_t)i * n_soap + s] = q;
+    Gp[s] = (Gp[s] - q * qg) * inv;
+  }
+  if (threadIdx.x == 0) sdp_out[i] = sdp;
+  __syncthreads();
+  // dE/dp per uncompressed component
+  for (int c0 = threadIdx.x; c0 < P.n_soap_unc; c0 += 4 * NTHREADS) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + NTHREADS * u;
+      if (c < P.n_soap_unc) {
+        float v;
+        if (umap) {
+          const int q = umap[c];
+          v = q >= 0 ? uel[c] * Gp[q] : 0.f;
+        } else if (P.compressed) {
+          v = 0.f;
+          for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v += P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
+        } else {
+          v = Gp[c];
+        }
+        Gu[c] = v;
+      }
+    }
+  }
+  __syncthreads();
+  // H_l = 4 pi w(m) Y_l C_l over the columns of block l, written over C, with
+  // Y_l(n, n') = Gu(c(min, max, l)) x 2 on the diagonal, sqrt 2 off it,
+  // c(a, b, l) = (a n_max - a(a-1)/2 + b - a) L1 + l. A job owns one column tile of one
+  // block and every row tile, so it reads its columns before writing them.
+  {
+    constexpr int NJH = [] {
+      int n = 0;
+      for (int l = 0; l <= L; ++l) n += (2 * (l + 1) + 15) / 16;
+      return n;
+    }();
+    constexpr int JHW = (NJH + 3) / 4;
+#pragma unroll
+    for (int jj = 0; jj < JHW; ++jj) {
+      const int job = wv + 4 * jj;
+      if (job < NJH) {
+        int l = 0, rem = job;
+        while (rem >= (2 * (l + 1) + 15) / 16) { rem -= (2 * (l + 1) + 15) / 16; ++l; }
+        const int b = rem;
+        const int cb = l * (l + 1), cw = 2 * (l + 1);
+        const int c = 16 * b + li;
+        float bv[KN];
+#pragma unroll
+        for (int s = 0; s < KN; ++s) {
+          const int np = 4 * s + lk;
+          bv[s] = (np < n_max && c < cw) ? C[np * CSTR + cb + c] : 0.f;
+        }
+        floatx4 acc[NTT];
+#pragma unroll
+        for (int a = 0; a < NTT; ++a) {
+          const int n = 16 * a + li;
+          float av[KN];
+#pragma unroll
+          for (int s = 0; s < KN; ++s) {
+            const int np = 4 * s + lk;
+            float y = 0.f;
+            if (n < n_max && np < n_max) {
+              const int lo_ = n < np ? n : np, hi_ = n < np ? np : n;
+              const float g = Gu[(lo_ * n_max - lo_ * (lo_ - 1) / 2 + (hi_ - lo_)) * L1 + l];
+              y = lo_ == hi_ ? 2.f * g : 1.41421356f * g;
+            }
+            av[s] = y;
+          }
+          acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int s = 0; s < KN; ++s) acc[a] = mfma4(av[s], bv[s], acc[a]);
+        }
+        if (c < cw) {
+          const float w = (c >= 2 ? 2.f : 1.f) * fourpi;
+#pragma unroll
+          for (int a = 0; a < NTT; ++a)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int r = 16 * a + 4 * lk + v;
+              if (r < n_max) C[r * CSTR + cb + c] = w * acc[a][v];
+            }
+        }
+      }
+    }
+  }
+  const float* H = C;
   // --------------------------------------------------------------- backward
-  float self0 = 0.f, self1 = 0.f, self2 = 0.f;
+  float self[3] = {0.f, 0.f, 0.f};
   for (int tb = p0; tb < p1; tb += PT) {
     __syncthreads();
-    {
-      const int p = tb + lane;
-      const bool live = p < p1 && p != p0 && spmask[p] != 0 && rjs[p] < P.rcut_max;
-      if (wv == 0) {
-        float* R = Rb + lane * n_max;
-        float* Rd = Rdb + lane * n_max;
-        if (live) {
-          radial_row(P, rtab, rjs[p], spmask[p], false, 1.f, R, Rd);
-        } else {
-          for (int n = 0; n < n_max; ++n) { R[n] = 0.f; Rd[n] = 0.f; }
-        }
-      } else if (wv == 1) {
-        if (live) {
-          float st_, ct_;
-          sincosf((float)thetas[p], &st_, &ct_);
-          plm_rows<L>(ct_, st_, Fp + lane * KS);
-        } else {
-          for (int k = 0; k < KS; ++k) Fp[lane * KS + k] = 0.f;
-        }
-      } else if (wv == 2) {
-        if (live) {
-          float g[L1], gd[L1];
-          angular_factors<L>(P, atab, rjs[p], __ffs(spmask[p]) - 1, g, gd);
+    stage(tb, false);
+    __syncthreads();
+    radial_items(1.f, Rb, Rdb);
+    gl_items(Gl, Gd);
+    // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
 #pragma unroll
-          for (int l = 0; l <= L; ++l) { Gl[lane * L1S + l] = g[l]; Gd[lane * L1S + l] = gd[l]; }
-          float sf_, cf_;
-          sincosf((float)phis[p], &sf_, &cf_);
-          float er = 1.f, ei = 0.f;
-          Em[lane * L1S] = make_float2(1.f, 0.f);
-          for (int m = 1; m <= L; ++m) {
-            const float nr = er * cf_ + ei * sf_, ni = ei * cf_ - er * sf_;
-            er = nr; ei = ni;
-            Em[lane * L1S + m] = make_float2(er, ei);
-          }
+    for (int u = 0; u < PIT; ++u) {
+      const int t = threadIdx.x + NTHREADS * u, m = t / PT, pl = t - m * PT;
+      if (m <= L + 1) {
+        if (pr_msk[pl] != 0) {
+          plm_column<L + 1>(m, pr_ct[pl], pr_st[pl],
+                            [&](int l, float plm) { Fp[(l * (l + 1) / 2 + m) * PTS + pl] = plm; });
+          if (m <= L) Em[m * PTS + pl] = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
         } else {
-#pragma unroll
-          for (int l = 0; l <= L; ++l) { Gl[lane * L1S + l] = 0.f; Gd[lane * L1S + l] = 0.f; }
-          for (int m = 0; m <= L; ++m) Em[lane * L1S + m] = make_float2(0.f, 0.f);
+          for (int l = m; l <= L + 1; ++l) Fp[(l * (l + 1) / 2 + m) * PTS + pl] = 0.f;
+          if (m <= L) Em[m * PTS + pl] = make_float2(0.f, 0.f);
         }
       }
     }
     __syncthreads();
-    // wavefront wv: pairs 16 wv .. 16 wv + 15 of the tile
+    // wavefront wv: pairs 16 wv .. 16 wv + 15 of the tile; U = R H, V = R' H for a group of
+    // column tiles, then their contraction with the angular factors
     float fr[4] = {0.f, 0.f, 0.f, 0.f}, fa[4] = {0.f, 0.f, 0.f, 0.f}, fo[4] = {0.f, 0.f, 0.f, 0.f};
     const int pa = 16 * wv + li;  // A-operand row (pair) of this lane
-    for (int ct = 0; ct < CT; ++ct) {
-      floatx4 U = {0.f, 0.f, 0.f, 0.f}, V = {0.f, 0.f, 0.f, 0.f};
-      const int col = 16 * ct + li;
-      for (int k4 = 0; k4 < n_max; k4 += 4) {
-        const int n = k4 + lk;
-        const bool inb = n < n_max;
-        const float hv = inb ? H[n * CSTR + col] : 0.f;
-        U = mfma4(inb ? Rb[pa * n_max + n] : 0.f, hv, U);
-        V = mfma4(inb ? Rdb[pa * n_max + n] : 0.f, hv, V);
-      }
-      // lane holds columns col (re if even, im if odd) of rows 16 wv + 4 lk + v
-      const int k = col >> 1;
-      const bool act = (col & 1) == 0 && k < KM;
-      const int lm = act ? klm[k] : 0;
-      const int l = lm & 255, m = lm >> 8;
-      const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
-      const float pf = act ? preflm[k] : 0.f;
+    float ra[KN], rda[KN];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float ur = U[v], vr = V[v];
-        const float ui = __shfl_xor(ur, 1, 64), vi = __shfl_xor(vr, 1, 64);
-        if (act) {
-          const int lp = 16 * wv + 4 * lk + v;
-          const float* pr = Fp + lp * KS;
-          const float gl = Gl[lp * L1S + l], gdl = Gd[lp * L1S + l];
-          const float2 e = Em[lp * L1S + m];
-          const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
-          const float ev = e.x * vr + e.y * vi;     // Re(E conj v)
-          const float ex = e.x * ui - e.y * ur;     // Re(i E conj u)
-          const float pl = pf * pr[kb + m];
-          const float pp1 = m < l ? pr[kb + m + 1] : 0.f;
-          float dms, dsin = 0.f;
-          if (m == 0) {
-            dms = -pp1;
-          } else {
-            dms = 0.5f * (float)((l + m) * (l - m + 1)) * pr[kb + m - 1] - 0.5f * pp1;
-            dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * pr[kb1 + m - 1] + 0.5f * pr[kb1 + m + 1];
+    for (int s = 0; s < KN; ++s) {
+      const int n = 4 * s + lk;
+      ra[s] = n < n_max ? Rb[n * PTS + pa] : 0.f;
+      rda[s] = n < n_max ? Rdb[n * PTS + pa] : 0.f;
+    }
+    constexpr int CG = CT < 3 ? CT : 3;
+#pragma unroll
+    for (int ct0 = 0; ct0 < CT; ct0 += CG) {
+      floatx4 U[CG], V[CG];
+#pragma unroll
+      for (int g = 0; g < CG; ++g) {
+        const int col = 16 * (ct0 + g) + li;
+        U[g] = floatx4{0.f, 0.f, 0.f, 0.f};
+        V[g] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if (ct0 + g < CT) {
+          float hv[KN];
+#pragma unroll
+          for (int s = 0; s < KN; ++s) {
+            const int n = 4 * s + lk;
+            hv[s] = n < n_max ? H[n * CSTR + col] : 0.f;
           }
-          // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E
-          fr[v] += pl * (gl * ev + gdl * eu);
-          fo[v] += pf * dms * gl * eu;
-          fa[v] += pf * dsin * gl * ex;
+#pragma unroll
+          for (int s = 0; s < KN; ++s) {
+            U[g] = mfma4(ra[s], hv[s], U[g]);
+            V[g] = mfma4(rda[s], hv[s], V[g]);
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < CG; ++g) {
+        const int col = 16 * (ct0 + g) + li;
+        // lane holds columns col (re if even, im if odd) of rows 16 wv + 4 lk + v
+        const int k = col >> 1;
+        const bool act = ct0 + g < CT && (col & 1) == 0 && k < KM;
+        const int lm = act ? klm[k] : 0;
+        const int l = lm & 255, m = lm >> 8;
+        const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
+        const float pf = act ? preflm[k] : 0.f;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float ur = U[g][v], vr = V[g][v];
+          const float ui = swap_adjacent(ur), vi = swap_adjacent(vr);
+          if (act) {
+            const int lp = 16 * wv + 4 * lk + v;
+            const float gl = Gl[l * PTS + lp], gdl = Gd[l * PTS + lp];
+            const float2 e = Em[m * PTS + lp];
+            const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
+            const float ev = e.x * vr + e.y * vi;     // Re(E conj v)
+            const float ex = e.x * ui - e.y * ur;     // Re(i E conj u)
+            const float pl = pf * Fp[(kb + m) * PTS + lp];
+            const float pp1 = m < l ? Fp[(kb + m + 1) * PTS + lp] : 0.f;
+            float dms, dsin = 0.f;
+            if (m == 0) {
+              dms = -pp1;
+            } else {
+              dms = 0.5f * (float)((l + m) * (l - m + 1)) * Fp[(kb + m - 1) * PTS + lp] - 0.5f * pp1;
+              dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * Fp[(kb1 + m - 1) * PTS + lp] +
+                     0.5f * Fp[(kb1 + m + 1) * PTS + lp];
+            }
+            // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E
+            fr[v] += pl * (gl * ev + gdl * eu);
+            fo[v] += pf * dms * gl * eu;
+            fa[v] += pf * dsin * gl * ex;
+          }
         }
       }
     }
+    // sums over the 16 columns of a row: prefix sums along the DPP row, total in lane 15
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) {
-        fr[v] += __shfl_xor(fr[v], o, 64);
-        fa[v] += __shfl_xor(fa[v], o, 64);
-        fo[v] += __shfl_xor(fo[v], o, 64);
-      }
-      if (li == 0) {
+      const float tr = row_total(fr[v]), ta = row_total(fa[v]), to = row_total(fo[v]);
+      if (li == 15) {
         const int lp = 16 * wv + 4 * lk + v;
-        fout[lp * 3] = fr[v];
-        fout[lp * 3 + 1] = fa[v];
-        fout[lp * 3 + 2] = fo[v];
+        fout[lp] = tr;
+        fout[PT + lp] = ta;
+        fout[2 * PT + lp] = to;
       }
     }
     __syncthreads();
     const int p = tb + lane;
     if (wv == 0 && p < p1 && p != p0) {
-      const float f_rad = fout[lane * 3], f_azi = fout[lane * 3 + 1], f_pol = fout[lane * 3 + 2];
-      float st_, ct_, sf_, cf_;
-      sincosf((float)thetas[p], &st_, &ct_);
-      sincosf((float)phis[p], &sf_, &cf_);
-      const float inv_r = 1.f / (float)rjs[p];
+      const float f_rad = fout[lane], f_azi = fout[PT + lane], f_pol = fout[2 * PT + lane];
+      const float st_ = pr_st[lane], ct_ = pr_ct[lane], sf_ = pr_sf[lane], cf_ = pr_cf[lane];
+      const float inv_r = 1.f / (float)pr_rd[lane];
       const float x0 = st_ * cf_ * f_rad - ct_ * cf_ * inv_r * f_pol - sf_ * inv_r * f_azi;
       const float x1 = st_ * sf_ * f_rad - ct_ * sf_ * inv_r * f_pol + cf_ * inv_r * f_azi;
       const float x2 = ct_ * f_rad + st_ * inv_r * f_pol;
       pair_out[3 * (size_t)p] = x0;
       pair_out[3 * (size_t)p + 1] = x1;
       pair_out[3 * (size_t)p + 2] = x2;
-      self0 -= x0; self1 -= x1; self2 -= x2;
+      self[0] -= x0; self[1] -= x1; self[2] -= x2;
     }
   }
-  self0 = block_sum_f(wv == 0 ? self0 : 0.f, red);
-  self1 = block_sum_f(wv == 0 ? self1 : 0.f, red + 4);
-  self2 = block_sum_f(wv == 0 ? self2 : 0.f, red + 8);
+  block_sums_f<3>(self, red);
   if (threadIdx.x == 0) {
-    pair_out[3 * (size_t)p0] = self0;
-    pair_out[3 * (size_t)p0 + 1] = self1;
-    pair_out[3 * (size_t)p0 + 2] = self2;
+    pair_out[3 * (size_t)p0] = self[0];
+    pair_out[3 * (size_t)p0 + 1] = self[1];
+    pair_out[3 * (size_t)p0 + 2] = self[2];
   }
 }
 
@@ -734,16 +795,16 @@ __global__ void widen_k(size_t n, const float* __restrict__ a, double* __restric
 }
 
 size_t site_lds(int n_max, int l_max, int n_soap, int n_soap_unc) {
-  const int KM = (l_max + 1) * (l_max + 2) / 2, KS = ((l_max + 2) * (l_max + 3) / 2) | 1, L1S = (l_max + 1) | 1;
+  const int KM = (l_max + 1) * (l_max + 2) / 2, KM1 = (l_max + 2) * (l_max + 3) / 2, L1 = l_max + 1;
   const int CSTR = ((2 * KM + 15) / 16) * 16;
-  size_t c = sizeof(float) * 2 * n_max * CSTR;
-  size_t fwd = sizeof(float) * PT * (n_max + CSTR);
-  size_t spec = sizeof(float) * ((l_max + 1) * n_max * n_max + 2 * n_soap + n_soap_unc);
-  size_t bwd = sizeof(float) * PT * (2 * n_max + KS + 2 * L1S + 2 * L1S + 3);
-  return c + std::max(std::max(fwd, spec), bwd);
+  const size_t c = (size_t)n_max * CSTR;
+  const size_t fwd = (size_t)(n_max + L1 + 2 * KM) * PTS;
+  const size_t spec = (size_t)L1 * n_max * n_max + 2 * n_soap + n_soap_unc;
+  const size_t bwd = (size_t)2 * n_max * PTS + ((KM1 * PTS + 1) & ~1) + 4 * L1 * PTS + 3 * PT;
+  return sizeof(float) * (c + std::max(std::max(fwd, spec), bwd));
 }
 
-template <int L>
+template <int L, int NTT>
 void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order, const int* site_ptr,
                  const float* rtab, const float* atab, const float* preflm, const float* central,
                  const int32_t* species, int max_mult, const int32_t* mult, const uint32_t* spmask, const double* rjs,
@@ -751,9 +812,13 @@ void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order
                  const int* out_c, const float* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
                  const float* P_el, const int4* ocomp, const float* oel, const int* umap, const float* uel, float* q,
                  float* sdp, float* pair) {
-  adjoint_site_k<L><<<P.n_sites, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, central, species, max_mult,
-                                                     mult, spmask, rjs, th, ph, G32, comp, out_ptr, out_c, out_el,
-                                                     unc_ptr, unc_e, P_i, P_el, ocomp, oel, umap, uel, q, sdp, pair);
+  if (sh > 64 * 1024)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&adjoint_site_k<L, NTT>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  adjoint_site_k<L, NTT><<<P.n_sites, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, central, species,
+                                                          max_mult, mult, spmask, rjs, th, ph, G32, comp, out_ptr,
+                                                          out_c, out_el, unc_ptr, unc_e, P_i, P_el, ocomp, oel, umap,
+                                                          uel, q, sdp, pair);
 }
 
 // host copy of the per-call constant tables that live next to the FP64 ones
@@ -983,7 +1048,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   timer("adj_tables", [&] { rc = build_tables(s, st, in, P, msg); });
   if (rc) return rc;
   const size_t sh = site_lds(in.n_max, in.l_max, in.n_soap, in.n_soap_unc);
-  if (sh > 64 * 1024) {
+  if (sh > in.lds_max) {
     msg = "the FP32 site working set exceeds the LDS";
     return ST_ERR_UNSUPPORTED;
   }
@@ -1028,16 +1093,20 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   (void)n_u1;
   bool ok = true;
   timer("adj_site_f32", [&] {
-    switch (in.l_max) {
-#define LS(l)                                                                                                      \
-  case l:                                                                                                          \
-    launch_site<l>(st, P, sh, nullptr, in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cen, in.species,   \
-                   in.max_mult, in.mult, in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(), in.comp,        \
-                   in.out_ptr, in.out_c, oel, in.unc_ptr, in.unc_e, in.P_i, pel, in.ocomp, oel1,                   \
-                   in.ocomp ? in.umap : nullptr, uel1, s.soap.as<float>(), s.sdp.as<float>(), s.pair.as<float>());  \
+    const int ntt = (in.n_max + 15) / 16;
+    switch (in.l_max * 4 + ntt) {
+#define LS1(l, t)                                                                                                  \
+  case l * 4 + t:                                                                                                  \
+    launch_site<l, t>(st, P, sh, nullptr, in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cen,            \
+                      in.species, in.max_mult, in.mult, in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(),  \
+                      in.comp, in.out_ptr, in.out_c, oel, in.unc_ptr, in.unc_e, in.P_i, pel, in.ocomp, oel1,       \
+                      in.ocomp ? in.umap : nullptr, uel1, s.soap.as<float>(), s.sdp.as<float>(),                   \
+                      s.pair.as<float>());                                                                         \
     break;
+#define LS(l) LS1(l, 1) LS1(l, 2) LS1(l, 3)
       LS(0) LS(1) LS(2) LS(3) LS(4) LS(5) LS(6) LS(7) LS(8) LS(9) LS(10) LS(11) LS(12)
 #undef LS
+#undef LS1
       default: ok = false;
     }
   });

@@ -875,6 +875,7 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   in.oel_h = &ctx->oel_h;
   in.uel_h = &ctx->uel_h;
   in.tables_key = &ctx->tables_key;
+  in.lds_max = ctx->lds_max;
   {
     int off = 0;
     for (int s = 0; s < p->n_species; ++s) {
