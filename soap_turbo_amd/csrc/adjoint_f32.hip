@@ -250,24 +250,33 @@ __device__ __forceinline__ float row_total(float x) {
 }
 
 constexpr int PTS = PT + 1;  // LDS stride of pair-fastest rows (odd: adjacent rows on other banks)
+constexpr int PTF = PT + 4;  // ... of the forward MFMA operand rows (4 mod 64 words)
 
 template <int L>
 struct SiteShape {
   static constexpr int KM = (L + 1) * (L + 2) / 2;    // (l, m) with m <= l <= L
   static constexpr int KM1 = (L + 2) * (L + 3) / 2;   // ... l <= L + 1
   static constexpr int L1 = L + 1;
-  static constexpr int CSTR = ((2 * KM + 15) / 16) * 16;  // columns of C and H: 2k + {re, im}, padded
+  static constexpr int CSTR = ((2 * KM + 15) / 16) * 16;  // columns of C: 2k + {re, im}, padded
   static constexpr int CT = CSTR / 16;
+  static constexpr int CS = CSTR + 4;                     // C row stride
+  static constexpr int KMP = ((KM + 15) / 16) * 16;       // k padded to tiles of 16
+  static constexpr int HS = 2 * KMP + 16;                 // H row stride: re block, im block, pad
 };
 
 // One site per workgroup of 256 threads (4 wavefronts); NTT = ceil(n_max / 16).
-// Per tile of up to 64 neighbours, the pairs' geometry is staged in LDS once;
-// then every thread takes part in the set-up: (n, pair) radial items and
-// (l, pair) angular-factor items, whose table loads are all issued before any
-// is used, then (m, pair) columns of P_l^m. The GEMM-shaped steps run on FP32
-// MFMA with operands loaded ahead: cnk = sum_pairs 4 pi R^T A, the per-l
-// Hermitian products S_l = C_l W C_l^T of the spectrum, H_l = Y_l C_l (Y_l
-// formed from dE/dp on the fly; H written over C), and U = R H, V = R' H.
+// Per tile of up to 64 neighbours, the pairs' geometry and table cells are
+// staged in LDS once; then every thread takes part in the set-up: (n, pair)
+// radial items and (l, pair) angular-factor items, whose table loads are all
+// issued before any is used, then (m, pair) columns of P_l^m. The GEMM-shaped
+// steps run on FP32 MFMA with operands loaded ahead: cnk = sum_pairs 4 pi R^T A,
+// the per-l Hermitian products S_l = C_l W C_l^T of the spectrum, H_l = Y_l C_l
+// (H stored with the real and imaginary parts of k in separate column blocks),
+// and U = R H, V = R' H, so that in the last step each lane owns one k.
+//
+// LDS row strides: PTF (= 4 mod 64 words) for the forward R and A rows, so the
+// 16 x 4 MFMA operand reads hit 64 distinct banks; PTS (odd) elsewhere; C rows
+// CSTR + 4 and H rows 2 KMP + 16 for the same reason.
 template <int L, int NTT>
 __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     AdjParams P, const int* __restrict__ order, const int* __restrict__ site_ptr, const float* __restrict__ rtab,
@@ -282,7 +291,8 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     float* __restrict__ sdp_out, float* __restrict__ pair_out) {
   using Sh = SiteShape<L>;
   constexpr int KM = Sh::KM, KM1 = Sh::KM1, L1 = Sh::L1, CSTR = Sh::CSTR, CT = Sh::CT;
-  constexpr int NC = 2 * KM;                        // live columns
+  constexpr int CS = Sh::CS, KMP = Sh::KMP, HS = Sh::HS, KT = KMP / 16;
+  constexpr int NC = 2 * KM;                        // live columns of C
   constexpr int TPW = (NTT * CT + 3) / 4;           // cnk tiles per wavefront
   constexpr int KN = 4 * NTT;                       // K steps over n (padded to 16 NTT)
   constexpr int RIT = 4 * NTT;                      // (n, pair) radial items per thread
@@ -293,23 +303,28 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   __shared__ float red[16];
   __shared__ float preflm[KM];
   __shared__ int klm[KM];
-  __shared__ double pr_rd[PT];
   __shared__ uint32_t pr_msk[PT];                   // species bits (0: no contribution) | SELF
-  __shared__ SpeciesTab spl[ADJ_MAX_SPECIES];       // the species tables, indexed per lane from LDS
+  __shared__ float pr_r[PT], pr_st[PT], pr_ct[PT], pr_sf[PT], pr_cf[PT];
+  __shared__ int pr_acell[PT];                      // angular table cell (floats into atab)
+  __shared__ float pr_ta[PT];
+  __shared__ int sp_off[ADJ_MAX_SPECIES], sp_a[ADJ_MAX_SPECIES], sp_rt[ADJ_MAX_SPECIES];
+  __shared__ float sp_cw[ADJ_MAX_SPECIES];
   __shared__ unsigned char nsp[64];                 // species block of each n
-  __shared__ float pr_st[PT], pr_ct[PT], pr_sf[PT], pr_cf[PT];
   const int i = order ? order[blockIdx.x] : blockIdx.x;
-  const int n_max = P.n_max;
+  const int n_max = P.n_max, n_species = P.n_species;
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const float fourpi = 12.566370614359172f;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
-  float* C = smem;                                 // n_max * CSTR (C, then H)
-  float* region = C + n_max * CSTR;
-  float* Rt = region;                              // forward: n_max * PTS (4 pi R, pair fastest)
-  float* Gt = Rt + n_max * PTS;                    //          L1 * PTS (g_l)
-  float* At = Gt + L1 * PTS;                       //          NC * PTS (A columns, pair fastest)
-  float* Sl = region;                              // spectrum: L1 * n_max * n_max
+  int* cell = reinterpret_cast<int*>(smem);        // n_species * PT radial cells (-1: none)
+  float* cell_t = smem + n_species * PT;           // n_species * PT
+  float* cell_ih = cell_t + n_species * PT;        // n_species * PT
+  float* C = cell_ih + n_species * PT;             // n_max * max(CS, HS) (C, then H)
+  float* region = C + n_max * (CS > HS ? CS : HS);
+  float* Rt = region;                              // forward: n_max * PTF (4 pi R, pair fastest)
+  float* Gt = Rt + n_max * PTF;                    //          L1 * PTS (g_l)
+  float* At = Gt + L1 * PTS;                       //          NC * PTF (A columns, pair fastest)
+  float* Sl = region;                              // spectrum: L1 * n_max * n_max (S_l, then Y_l)
   float* qv = Sl + L1 * n_max * n_max;             //           n_soap
   float* Gp = qv + P.n_soap;                       //           n_soap
   float* Gu = Gp + P.n_soap;                       //           n_soap_unc
@@ -321,10 +336,11 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   float2* Em = reinterpret_cast<float2*>(Gd + L1 * PTS);  //    L1 * PTS (e^{-i m phi})
   float* fout = reinterpret_cast<float*>(Em + L1 * PTS);  //    3 * PT
 
-  // the tile's pairs: r, species bits of contributing pairs, sin/cos of theta and phi
+  // The tile's pairs: r, species bits of contributing pairs, sin/cos of theta and phi,
+  // and per species the radial table cell (FP64 cell coordinate) or -1.
   auto stage = [&](int tb, bool with_self) {
     if (threadIdx.x < PT) {
-      const int p = tb + threadIdx.x;
+      const int pl = threadIdx.x, p = tb + pl;
       uint32_t msk = 0;
       double rd = 1.0, th = 0.0, ph = 0.0;
       if (p < p1) {
@@ -333,47 +349,77 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
         ph = phis[p];
         msk = spmask[p];
         if (!(rd < P.rcut_max) || (p == p0 && !with_self)) msk = 0;
-        if (msk != 0 && p == p0) msk |= SELF;
+      }
+      const bool self = p == p0;
+      for (int s = 0; s < n_species; ++s) {
+        const SpeciesTab& sp = P.sp[s];
+        int c = -1;
+        float t = 0.f, ih = 0.f;
+        if ((msk & (1u << s)) && rd < sp.rcut && !(self && !sp.do_central)) c = radial_cell(sp, rd, t, ih);
+        cell[s * PT + pl] = c;
+        cell_t[s * PT + pl] = t;
+        cell_ih[s * PT + pl] = ih;
+      }
+      int acell = 0;
+      float ta = 0.f;
+      if (msk != 0) {
+        const double xa = rd * P.ang_inv_h;
+        int ia = (int)xa;
+        if (ia > P.ang_n - 2) ia = P.ang_n - 2;
+        ta = (float)(xa - (double)ia);
+        const int s0 = __ffs(msk) - 1;
+        int aoff = 0;
+        for (int s = 0; s < n_species; ++s)
+          if (s == s0) aoff = P.sp[s].atab_off;
+        acell = aoff + ia * 4 * L1;
       }
       float st_, ct_, sf_, cf_;
       sincosf((float)th, &st_, &ct_);
       sincosf((float)ph, &sf_, &cf_);
-      pr_rd[threadIdx.x] = rd;
-      pr_msk[threadIdx.x] = msk;
-      pr_st[threadIdx.x] = st_;
-      pr_ct[threadIdx.x] = ct_;
-      pr_sf[threadIdx.x] = sf_;
-      pr_cf[threadIdx.x] = cf_;
+      pr_msk[pl] = msk != 0 && self ? (msk | SELF) : msk;
+      pr_r[pl] = (float)rd;
+      pr_st[pl] = st_;
+      pr_ct[pl] = ct_;
+      pr_sf[pl] = sf_;
+      pr_cf[pl] = cf_;
+      pr_acell[pl] = acell;
+      pr_ta[pl] = ta;
     }
   };
-  // (n, pair) radial items: R_n times scale, and dR_n/dr if Rd is given
-  auto radial_items = [&](float scale, float* R, float* Rd) {
+  // (n, pair) radial items: R_n times scale (and the central weight on the site's own
+  // pair), and dR_n/dr if Rd is given
+  auto radial_items = [&](int tid, float scale, float* R, float* Rd, int stride) {
 #pragma unroll
     for (int u = 0; u < RIT; ++u) {
-      const int t = threadIdx.x + NTHREADS * u, n = t / PT, pl = t - n * PT;
+      const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
       if (n < n_max) {
-        const uint32_t msk = pr_msk[pl];
+        const int s = nsp[n];
+        const int c = cell[s * PT + pl];
         float v = 0.f, d = 0.f;
-        if (msk != 0) {
-          const int s = nsp[n];
-          v = radial_one(spl[s], s, rtab, pr_rd[pl], msk & ~SELF, (msk & SELF) != 0, scale, n, Rd ? &d : nullptr);
+        if (c >= 0) {
+          const float tt = cell_t[s * PT + pl];
+          const int a = sp_a[s], q = n - sp_off[s];
+          const float* cf = rtab + sp_rt[s] + (size_t)c * 4 * a;
+          const float c0 = cf[q], c1 = cf[a + q], c2 = cf[2 * a + q], c3 = cf[3 * a + q];
+          const float sc = ((pr_msk[pl] & SELF) ? sp_cw[s] : 1.f) * scale;
+          v = sc * (c0 + tt * (c1 + tt * (c2 + tt * c3)));
+          if (Rd) d = (c1 + tt * (2.f * c2 + 3.f * tt * c3)) * cell_ih[s * PT + pl];
         }
-        R[n * PTS + pl] = v;
-        if (Rd) Rd[n * PTS + pl] = d;
+        R[n * stride + pl] = v;
+        if (Rd) Rd[n * stride + pl] = d;
       }
     }
   };
   // (l, pair) items: g_l(r), and dg_l/dr if Gdr is given
-  auto gl_items = [&](float* Gr, float* Gdr) {
+  auto gl_items = [&](int tid, float* Gr, float* Gdr) {
 #pragma unroll
     for (int u = 0; u < GIT; ++u) {
-      const int t = threadIdx.x + NTHREADS * u, l = t / PT, pl = t - l * PT;
+      const int t = tid + NTHREADS * u, l = t / PT, pl = t - l * PT;
       if (l < L1) {
-        const uint32_t msk = pr_msk[pl];
         float g = 0.f, gd = 0.f;
-        if (msk != 0) {
-          float ta;
-          const float* gc = ang_cell(P, atab, L1, pr_rd[pl], spl[__ffs(msk & ~SELF) - 1].atab_off, ta);
+        if (pr_msk[pl] != 0) {
+          const float ta = pr_ta[pl];
+          const float* gc = atab + pr_acell[pl];
           const float c0 = gc[l], c1 = gc[L1 + l], c2 = gc[2 * L1 + l], c3 = gc[3 * L1 + l];
           g = c0 + ta * (c1 + ta * (c2 + ta * c3));
           gd = (c1 + ta * (2.f * c2 + 3.f * ta * c3)) * P.ang_ih;
@@ -390,11 +436,14 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     while ((l + 1) * (l + 2) / 2 <= t) ++l;
     klm[t] = l | ((t - l * (l + 1) / 2) << 8);
   }
-  // uniform-index copies only: a per-lane index into the kernel arguments would
+  // uniform-index reads of the kernel arguments only: a per-lane index into them would
   // move them to scratch memory
   if (threadIdx.x == 0) {
-    for (int s = 0; s < P.n_species; ++s) {
-      spl[s] = P.sp[s];
+    for (int s = 0; s < n_species; ++s) {
+      sp_off[s] = P.sp[s].off;
+      sp_a[s] = P.sp[s].a;
+      sp_rt[s] = P.sp[s].rtab_off;
+      sp_cw[s] = P.sp[s].cw;
       for (int q = 0; q < P.sp[s].a; ++q) nsp[P.sp[s].off + q] = (unsigned char)s;
     }
   }
@@ -410,27 +459,30 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     __syncthreads();
     stage(tb, true);
     __syncthreads();
-    radial_items(fourpi, Rt, nullptr);
-    gl_items(Gt, nullptr);
+    // the thread index made opaque per tile: item addresses are formed here, not hoisted
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    radial_items(tid, fourpi, Rt, nullptr, PTF);
+    gl_items(tid, Gt, nullptr);
     __syncthreads();
     // (m, pair) items: A_k = preflm P_l^m g_l e^{-i m phi} for l = m .. L, as (re, im) rows
 #pragma unroll
     for (int u = 0; u < GIT; ++u) {
-      const int t = threadIdx.x + NTHREADS * u, m = t / PT, pl = t - m * PT;
+      const int t = tid + NTHREADS * u, m = t / PT, pl = t - m * PT;
       if (m < L1) {
         if (pr_msk[pl] != 0) {
           const float2 e = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
           plm_column<L>(m, pr_ct[pl], pr_st[pl], [&](int l, float plm) {
             const int k = l * (l + 1) / 2 + m;
             const float v = preflm[k] * plm * Gt[l * PTS + pl];
-            At[(2 * k) * PTS + pl] = v * e.x;
-            At[(2 * k + 1) * PTS + pl] = v * e.y;
+            At[(2 * k) * PTF + pl] = v * e.x;
+            At[(2 * k + 1) * PTF + pl] = v * e.y;
           });
         } else {
           for (int l = m; l <= L; ++l) {
             const int k = l * (l + 1) / 2 + m;
-            At[(2 * k) * PTS + pl] = 0.f;
-            At[(2 * k + 1) * PTS + pl] = 0.f;
+            At[(2 * k) * PTF + pl] = 0.f;
+            At[(2 * k + 1) * PTF + pl] = 0.f;
           }
         }
       }
@@ -451,8 +503,8 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
             const int pp = c0 + 4 * s + lk;
-            av[s] = (s < ns && n < n_max) ? Rt[n * PTS + pp] : 0.f;
-            bv[s] = (s < ns && col < NC) ? At[col * PTS + pp] : 0.f;
+            av[s] = (s < ns && n < n_max) ? Rt[n * PTF + pp] : 0.f;
+            bv[s] = (s < ns && col < NC) ? At[col * PTF + pp] : 0.f;
           }
           floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -477,7 +529,7 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
           if (P.gauss && col == 0)
             for (int m = 0; m < species_mult[i]; ++m)
               val += central[(size_t)(site_species[(size_t)i * max_mult + m] - 1) * n_max + n];
-          C[n * CSTR + col] = val;
+          C[n * CS + col] = val;
         }
       }
     }
@@ -505,8 +557,8 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
         for (int s = 0; s < MS; ++s) {
           const int c = 4 * s + lk;
           const bool inb = c < cw;
-          av[s] = (inb && n < n_max) ? C[n * CSTR + cb + c] : 0.f;
-          bv[s] = (inb && np < n_max) ? (c >= 2 ? 2.f : 1.f) * C[np * CSTR + cb + c] : 0.f;
+          av[s] = (inb && n < n_max) ? C[n * CS + cb + c] : 0.f;
+          bv[s] = (inb && np < n_max) ? (c >= 2 ? 2.f : 1.f) * C[np * CS + cb + c] : 0.f;
         }
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -582,10 +634,25 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     }
   }
   __syncthreads();
-  // H_l = 4 pi w(m) Y_l C_l over the columns of block l, written over C, with
-  // Y_l(n, n') = Gu(c(min, max, l)) x 2 on the diagonal, sqrt 2 off it,
-  // c(a, b, l) = (a n_max - a(a-1)/2 + b - a) L1 + l. A job owns one column tile of one
-  // block and every row tile, so it reads its columns before writing them.
+  // Y_l(n, n') = Gu(c) x 2 on the diagonal, sqrt 2 off it, over S_l; one (l, n) row per thread,
+  // c(a, b, l) = (a n_max - a(a-1)/2 + b - a) L1 + l stepped along the row
+  float* Y = Sl;
+  for (int row = threadIdx.x; row < L1 * n_max; row += NTHREADS) {
+    const int l = row / n_max, n = row - l * n_max;
+    float* yr = Y + (size_t)row * n_max;
+    int pid = 0;  // a n_max - a(a-1)/2 - a for a = np
+    for (int np = 0; np < n; ++np) {
+      yr[np] = 1.41421356f * Gu[(pid + n) * L1 + l];
+      pid += n_max - np - 1;
+    }
+    yr[n] = 2.f * Gu[(pid + n) * L1 + l];
+    for (int np = n + 1; np < n_max; ++np) yr[np] = 1.41421356f * Gu[(pid + np) * L1 + l];
+  }
+  __syncthreads();
+  // H_l = 4 pi w(m) Y_l C_l over the columns of block l. A job is one column tile of one
+  // block and every row tile; each wavefront holds its jobs' C operands in registers across a
+  // barrier, after which H is written over C: real parts of k in columns k, imaginary
+  // parts in columns KMP + k.
   {
     constexpr int NJH = [] {
       int n = 0;
@@ -593,21 +660,27 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
       return n;
     }();
     constexpr int JHW = (NJH + 3) / 4;
+    float bv[JHW][KN];
+#pragma unroll
+    for (int jj = 0; jj < JHW; ++jj) {
+      const int job = wv + 4 * jj;
+      int l = 0, rem = job < NJH ? job : 0;
+      while (rem >= (2 * (l + 1) + 15) / 16) { rem -= (2 * (l + 1) + 15) / 16; ++l; }
+      const int cb = l * (l + 1), cw = 2 * (l + 1), c = 16 * rem + li;
+#pragma unroll
+      for (int s = 0; s < KN; ++s) {
+        const int np = 4 * s + lk;
+        bv[jj][s] = (job < NJH && np < n_max && c < cw) ? C[np * CS + cb + c] : 0.f;
+      }
+    }
+    __syncthreads();
 #pragma unroll
     for (int jj = 0; jj < JHW; ++jj) {
       const int job = wv + 4 * jj;
       if (job < NJH) {
         int l = 0, rem = job;
         while (rem >= (2 * (l + 1) + 15) / 16) { rem -= (2 * (l + 1) + 15) / 16; ++l; }
-        const int b = rem;
-        const int cb = l * (l + 1), cw = 2 * (l + 1);
-        const int c = 16 * b + li;
-        float bv[KN];
-#pragma unroll
-        for (int s = 0; s < KN; ++s) {
-          const int np = 4 * s + lk;
-          bv[s] = (np < n_max && c < cw) ? C[np * CSTR + cb + c] : 0.f;
-        }
+        const int cw = 2 * (l + 1), c = 16 * rem + li;
         floatx4 acc[NTT];
 #pragma unroll
         for (int a = 0; a < NTT; ++a) {
@@ -616,29 +689,31 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
 #pragma unroll
           for (int s = 0; s < KN; ++s) {
             const int np = 4 * s + lk;
-            float y = 0.f;
-            if (n < n_max && np < n_max) {
-              const int lo_ = n < np ? n : np, hi_ = n < np ? np : n;
-              const float g = Gu[(lo_ * n_max - lo_ * (lo_ - 1) / 2 + (hi_ - lo_)) * L1 + l];
-              y = lo_ == hi_ ? 2.f * g : 1.41421356f * g;
-            }
-            av[s] = y;
+            av[s] = (n < n_max && np < n_max) ? Y[(l * n_max + n) * n_max + np] : 0.f;
           }
           acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int s = 0; s < KN; ++s) acc[a] = mfma4(av[s], bv[s], acc[a]);
+          for (int s = 0; s < KN; ++s) acc[a] = mfma4(av[s], bv[jj][s], acc[a]);
         }
         if (c < cw) {
           const float w = (c >= 2 ? 2.f : 1.f) * fourpi;
+          const int k = l * (l + 1) / 2 + (c >> 1), hc = (c & 1) * KMP + k;
 #pragma unroll
           for (int a = 0; a < NTT; ++a)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               const int r = 16 * a + 4 * lk + v;
-              if (r < n_max) C[r * CSTR + cb + c] = w * acc[a][v];
+              if (r < n_max) C[r * HS + hc] = w * acc[a][v];
             }
         }
       }
+      __builtin_amdgcn_sched_barrier(0);  // one job's operands live at a time
+    }
+    // H columns k in [KM, KMP) and [KMP + KM, 2 KMP): zero
+    for (int t = threadIdx.x; t < n_max * 2 * (KMP - KM); t += NTHREADS) {
+      const int r = t / (2 * (KMP - KM)), j = t - r * 2 * (KMP - KM);
+      const int hc = j < KMP - KM ? KM + j : KMP + KM + (j - (KMP - KM));
+      C[r * HS + hc] = 0.f;
     }
   }
   const float* H = C;
@@ -648,12 +723,14 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     __syncthreads();
     stage(tb, false);
     __syncthreads();
-    radial_items(1.f, Rb, Rdb);
-    gl_items(Gl, Gd);
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    radial_items(tid, 1.f, Rb, Rdb, PTS);
+    gl_items(tid, Gl, Gd);
     // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
 #pragma unroll
     for (int u = 0; u < PIT; ++u) {
-      const int t = threadIdx.x + NTHREADS * u, m = t / PT, pl = t - m * PT;
+      const int t = tid + NTHREADS * u, m = t / PT, pl = t - m * PT;
       if (m <= L + 1) {
         if (pr_msk[pl] != 0) {
           plm_column<L + 1>(m, pr_ct[pl], pr_st[pl],
@@ -666,10 +743,14 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
       }
     }
     __syncthreads();
-    // wavefront wv: pairs 16 wv .. 16 wv + 15 of the tile; U = R H, V = R' H for a group of
-    // column tiles, then their contraction with the angular factors
+    // wavefront wv: pairs 16 wv .. 16 wv + 15 of the tile. Per tile of 16 k: U = R H and
+    // V = R' H for the real and the imaginary columns; lane li then owns k = 16 kt + li
     float fr[4] = {0.f, 0.f, 0.f, 0.f}, fa[4] = {0.f, 0.f, 0.f, 0.f}, fo[4] = {0.f, 0.f, 0.f, 0.f};
-    const int pa = 16 * wv + li;  // A-operand row (pair) of this lane
+    // lane coordinates made opaque here, so that the LDS addresses derived from them are
+    // formed in the loop rather than hoisted out of it into long-lived registers
+    int lrow = 16 * wv + 4 * lk, lcol = li;
+    asm volatile("" : "+v"(lrow), "+v"(lcol));
+    const int pa = 16 * wv + lcol;  // A-operand row (pair) of this lane
     float ra[KN], rda[KN];
 #pragma unroll
     for (int s = 0; s < KN; ++s) {
@@ -677,45 +758,44 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
       ra[s] = n < n_max ? Rb[n * PTS + pa] : 0.f;
       rda[s] = n < n_max ? Rdb[n * PTS + pa] : 0.f;
     }
-    constexpr int CG = CT < 3 ? CT : 3;
+    constexpr int KG = KT < 3 ? KT : 3;
 #pragma unroll
-    for (int ct0 = 0; ct0 < CT; ct0 += CG) {
-      floatx4 U[CG], V[CG];
+    for (int kt0 = 0; kt0 < KT; kt0 += KG) {
+      floatx4 Ur[KG], Ui[KG], Vr[KG], Vi[KG];
 #pragma unroll
-      for (int g = 0; g < CG; ++g) {
-        const int col = 16 * (ct0 + g) + li;
-        U[g] = floatx4{0.f, 0.f, 0.f, 0.f};
-        V[g] = floatx4{0.f, 0.f, 0.f, 0.f};
-        if (ct0 + g < CT) {
-          float hv[KN];
+      for (int g = 0; g < KG; ++g) {
+        Ur[g] = Ui[g] = Vr[g] = Vi[g] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if (kt0 + g < KT) {
+          const int k = 16 * (kt0 + g) + lcol;
+          float hr[KN], hm[KN];
 #pragma unroll
           for (int s = 0; s < KN; ++s) {
             const int n = 4 * s + lk;
-            hv[s] = n < n_max ? H[n * CSTR + col] : 0.f;
+            hr[s] = n < n_max ? H[n * HS + k] : 0.f;
+            hm[s] = n < n_max ? H[n * HS + KMP + k] : 0.f;
           }
 #pragma unroll
           for (int s = 0; s < KN; ++s) {
-            U[g] = mfma4(ra[s], hv[s], U[g]);
-            V[g] = mfma4(rda[s], hv[s], V[g]);
+            Ur[g] = mfma4(ra[s], hr[s], Ur[g]);
+            Ui[g] = mfma4(ra[s], hm[s], Ui[g]);
+            Vr[g] = mfma4(rda[s], hr[s], Vr[g]);
+            Vi[g] = mfma4(rda[s], hm[s], Vi[g]);
           }
         }
       }
 #pragma unroll
-      for (int g = 0; g < CG; ++g) {
-        const int col = 16 * (ct0 + g) + li;
-        // lane holds columns col (re if even, im if odd) of rows 16 wv + 4 lk + v
-        const int k = col >> 1;
-        const bool act = ct0 + g < CT && (col & 1) == 0 && k < KM;
+      for (int g = 0; g < KG; ++g) {
+        const int k = 16 * (kt0 + g) + lcol;
+        const bool act = kt0 + g < KT && k < KM;
         const int lm = act ? klm[k] : 0;
         const int l = lm & 255, m = lm >> 8;
         const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
         const float pf = act ? preflm[k] : 0.f;
+        if (act) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const float ur = U[g][v], vr = V[g][v];
-          const float ui = swap_adjacent(ur), vi = swap_adjacent(vr);
-          if (act) {
-            const int lp = 16 * wv + 4 * lk + v;
+          for (int v = 0; v < 4; ++v) {
+            const float ur = Ur[g][v], ui = Ui[g][v], vr = Vr[g][v], vi = Vi[g][v];
+            const int lp = lrow + v;
             const float gl = Gl[l * PTS + lp], gdl = Gd[l * PTS + lp];
             const float2 e = Em[m * PTS + lp];
             const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
@@ -739,7 +819,7 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
         }
       }
     }
-    // sums over the 16 columns of a row: prefix sums along the DPP row, total in lane 15
+    // sums over the 16 k of a row: prefix sums along the DPP row, total in lane 15
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const float tr = row_total(fr[v]), ta = row_total(fa[v]), to = row_total(fo[v]);
@@ -755,7 +835,7 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     if (wv == 0 && p < p1 && p != p0) {
       const float f_rad = fout[lane], f_azi = fout[PT + lane], f_pol = fout[2 * PT + lane];
       const float st_ = pr_st[lane], ct_ = pr_ct[lane], sf_ = pr_sf[lane], cf_ = pr_cf[lane];
-      const float inv_r = 1.f / (float)pr_rd[lane];
+      const float inv_r = 1.f / pr_r[lane];
       const float x0 = st_ * cf_ * f_rad - ct_ * cf_ * inv_r * f_pol - sf_ * inv_r * f_azi;
       const float x1 = st_ * sf_ * f_rad - ct_ * sf_ * inv_r * f_pol + cf_ * inv_r * f_azi;
       const float x2 = ct_ * f_rad + st_ * inv_r * f_pol;
@@ -794,14 +874,15 @@ __global__ void widen_k(size_t n, const float* __restrict__ a, double* __restric
   if (i < n) b[i] = (double)a[i];
 }
 
-size_t site_lds(int n_max, int l_max, int n_soap, int n_soap_unc) {
+size_t site_lds(int n_species, int n_max, int l_max, int n_soap, int n_soap_unc) {
   const int KM = (l_max + 1) * (l_max + 2) / 2, KM1 = (l_max + 2) * (l_max + 3) / 2, L1 = l_max + 1;
-  const int CSTR = ((2 * KM + 15) / 16) * 16;
-  const size_t c = (size_t)n_max * CSTR;
-  const size_t fwd = (size_t)(n_max + L1 + 2 * KM) * PTS;
+  const int CSTR = ((2 * KM + 15) / 16) * 16, CS = CSTR + 4, KMP = ((KM + 15) / 16) * 16, HS = 2 * KMP + 16;
+  const size_t cells = (size_t)3 * n_species * PT;
+  const size_t c = (size_t)n_max * std::max(CS, HS);
+  const size_t fwd = (size_t)n_max * PTF + (size_t)L1 * PTS + (size_t)2 * KM * PTF;
   const size_t spec = (size_t)L1 * n_max * n_max + 2 * n_soap + n_soap_unc;
   const size_t bwd = (size_t)2 * n_max * PTS + ((KM1 * PTS + 1) & ~1) + 4 * L1 * PTS + 3 * PT;
-  return sizeof(float) * (c + std::max(std::max(fwd, spec), bwd));
+  return sizeof(float) * (cells + c + std::max(std::max(fwd, spec), bwd));
 }
 
 template <int L, int NTT>
@@ -1047,7 +1128,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   int rc;
   timer("adj_tables", [&] { rc = build_tables(s, st, in, P, msg); });
   if (rc) return rc;
-  const size_t sh = site_lds(in.n_max, in.l_max, in.n_soap, in.n_soap_unc);
+  const size_t sh = site_lds(in.p->n_species, in.n_max, in.l_max, in.n_soap, in.n_soap_unc);
   if (sh > in.lds_max) {
     msg = "the FP32 site working set exceeds the LDS";
     return ST_ERR_UNSUPPORTED;
