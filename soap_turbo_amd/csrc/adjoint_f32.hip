@@ -389,24 +389,35 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   // (n, pair) radial items: R_n times scale (and the central weight on the site's own
   // pair), and dR_n/dr if Rd is given
   auto radial_items = [&](int tid, float scale, float* R, float* Rd, int stride) {
+    // pass 1: every table load of the thread's items; pass 2: evaluate and store (an LDS store
+    // between them would hold back the next item's loads, as the compiler cannot tell
+    // the LDS arrays apart)
+    float c0[RIT], c1[RIT], c2[RIT], c3[RIT], tt[RIT], sc[RIT], ih[RIT];
+#pragma unroll
+    for (int u = 0; u < RIT; ++u) {
+      const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
+      c0[u] = c1[u] = c2[u] = c3[u] = tt[u] = ih[u] = 0.f;
+      sc[u] = 0.f;
+      if (n < n_max) {
+        const int s = nsp[n];
+        const int c = cell[s * PT + pl];
+        if (c >= 0) {
+          tt[u] = cell_t[s * PT + pl];
+          ih[u] = cell_ih[s * PT + pl];
+          const int a = sp_a[s], q = n - sp_off[s];
+          const float* cf = rtab + sp_rt[s] + (size_t)c * 4 * a;
+          c0[u] = cf[q]; c1[u] = cf[a + q]; c2[u] = cf[2 * a + q]; c3[u] = cf[3 * a + q];
+          sc[u] = ((pr_msk[pl] & SELF) ? sp_cw[s] : 1.f) * scale;
+        }
+      }
+    }
 #pragma unroll
     for (int u = 0; u < RIT; ++u) {
       const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
       if (n < n_max) {
-        const int s = nsp[n];
-        const int c = cell[s * PT + pl];
-        float v = 0.f, d = 0.f;
-        if (c >= 0) {
-          const float tt = cell_t[s * PT + pl];
-          const int a = sp_a[s], q = n - sp_off[s];
-          const float* cf = rtab + sp_rt[s] + (size_t)c * 4 * a;
-          const float c0 = cf[q], c1 = cf[a + q], c2 = cf[2 * a + q], c3 = cf[3 * a + q];
-          const float sc = ((pr_msk[pl] & SELF) ? sp_cw[s] : 1.f) * scale;
-          v = sc * (c0 + tt * (c1 + tt * (c2 + tt * c3)));
-          if (Rd) d = (c1 + tt * (2.f * c2 + 3.f * tt * c3)) * cell_ih[s * PT + pl];
-        }
-        R[n * stride + pl] = v;
-        if (Rd) Rd[n * stride + pl] = d;
+        const float x = tt[u];
+        R[n * stride + pl] = sc[u] * (c0[u] + x * (c1[u] + x * (c2[u] + x * c3[u])));
+        if (Rd) Rd[n * stride + pl] = (c1[u] + x * (2.f * c2[u] + 3.f * x * c3[u])) * ih[u];
       }
     }
   };
@@ -465,24 +476,37 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     radial_items(tid, fourpi, Rt, nullptr, PTF);
     gl_items(tid, Gt, nullptr);
     __syncthreads();
-    // (m, pair) items: A_k = preflm P_l^m g_l e^{-i m phi} for l = m .. L, as (re, im) rows
+    // (m, pair) items: A_k = preflm P_l^m g_l e^{-i m phi} for l = m .. L, as (re, im) rows.
+    // m is uniform over a wavefront; l is unrolled, P_l^m by the recursions of get_plm_array
+    // (src/soap_turbo_angular.f90:108-160): P_m^m, P_{m+1}^m = (2m+1) x P_m^m, three-term in l.
 #pragma unroll
     for (int u = 0; u < GIT; ++u) {
       const int t = tid + NTHREADS * u, m = t / PT, pl = t - m * PT;
       if (m < L1) {
-        if (pr_msk[pl] != 0) {
-          const float2 e = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
-          plm_column<L>(m, pr_ct[pl], pr_st[pl], [&](int l, float plm) {
+        const bool live = pr_msk[pl] != 0;
+        float gv[L1];
+#pragma unroll
+        for (int l = 0; l < L1; ++l) gv[l] = l >= m ? Gt[l * PTS + pl] : 0.f;
+        const float x = pr_ct[pl], sq = pr_st[pl];
+        const float2 e = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
+        float pmm = 1.f;
+#pragma unroll
+        for (int j = 1; j <= L; ++j)
+          if (j <= m) pmm = -(float)(2 * j - 1) * sq * pmm;
+        float p0 = 0.f, p1 = 0.f;
+#pragma unroll
+        for (int l = 0; l <= L; ++l) {
+          if (l >= m) {
+            float pc;
+            if (l == m) pc = pmm;
+            else if (l == m + 1) pc = x * (float)(2 * m + 1) * pmm;
+            else pc = ((float)(2 * l - 1) * x * p1 - (float)(l - 1 + m) * p0) / (float)(l - m);
+            p0 = p1;
+            p1 = pc;
             const int k = l * (l + 1) / 2 + m;
-            const float v = preflm[k] * plm * Gt[l * PTS + pl];
+            const float v = live ? preflm[k] * pc * gv[l] : 0.f;
             At[(2 * k) * PTF + pl] = v * e.x;
             At[(2 * k + 1) * PTF + pl] = v * e.y;
-          });
-        } else {
-          for (int l = m; l <= L; ++l) {
-            const int k = l * (l + 1) / 2 + m;
-            At[(2 * k) * PTF + pl] = 0.f;
-            At[(2 * k + 1) * PTF + pl] = 0.f;
           }
         }
       }
@@ -542,6 +566,24 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   {
     constexpr int NAB = NTT * (NTT + 1) / 2, NJ = L1 * NAB, JPW = (NJ + 3) / 4;
     constexpr int MS = (2 * L1 + 3) / 4;
+    float av[JPW][MS], bv[JPW][MS];
+#pragma unroll
+    for (int jj = 0; jj < JPW; ++jj) {
+      const int job = wv + 4 * jj;
+      const int l = job / NAB;
+      int ab = job - l * NAB, a = 0;
+      while (ab >= NTT - a) { ab -= NTT - a; ++a; }
+      const int b = a + ab;
+      const int cb = l * (l + 1), cw = 2 * (l + 1);
+      const int n = 16 * a + li, np = 16 * b + li;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        const int c = 4 * s + lk;
+        const bool inb = job < NJ && c < cw;
+        av[jj][s] = (inb && n < n_max) ? C[n * CS + cb + c] : 0.f;
+        bv[jj][s] = (inb && np < n_max) ? (c >= 2 ? 2.f : 1.f) * C[np * CS + cb + c] : 0.f;
+      }
+    }
 #pragma unroll
     for (int jj = 0; jj < JPW; ++jj) {
       const int job = wv + 4 * jj;
@@ -550,20 +592,11 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
         int ab = job - l * NAB, a = 0;
         while (ab >= NTT - a) { ab -= NTT - a; ++a; }
         const int b = a + ab;
-        const int cb = l * (l + 1), cw = 2 * (l + 1);
-        const int n = 16 * a + li, np = 16 * b + li;
-        float av[MS], bv[MS];
-#pragma unroll
-        for (int s = 0; s < MS; ++s) {
-          const int c = 4 * s + lk;
-          const bool inb = c < cw;
-          av[s] = (inb && n < n_max) ? C[n * CS + cb + c] : 0.f;
-          bv[s] = (inb && np < n_max) ? (c >= 2 ? 2.f : 1.f) * C[np * CS + cb + c] : 0.f;
-        }
+        const int cw = 2 * (l + 1);
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < MS; ++s)
-          if (4 * s < cw) acc = mfma4(av[s], bv[s], acc);
+          if (4 * s < cw) acc = mfma4(av[jj][s], bv[jj][s], acc);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = 16 * a + 4 * lk + v, cc = 16 * b + li;
@@ -577,9 +610,11 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   const int n_soap = P.n_soap;
   float sums[2] = {0.f, 0.f};
   for (int s0 = threadIdx.x; s0 < n_soap; s0 += 4 * NTHREADS) {
+    float vq[4], gq[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int s = s0 + NTHREADS * u;
+      vq[u] = gq[u] = 0.f;
       if (s < n_soap) {
         float v = 0.f;
         if (ocomp) {
@@ -594,10 +629,18 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
           }
         }
         const float g = G32[(size_t)i * n_soap + s];
-        qv[s] = v;
-        Gp[s] = g;
+        vq[u] = v;
+        gq[u] = g;
         sums[0] += v * v;
         sums[1] += v * g;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int s = s0 + NTHREADS * u;
+      if (s < n_soap) {
+        qv[s] = vq[u];
+        Gp[s] = gq[u];
       }
     }
   }
@@ -615,9 +658,11 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   __syncthreads();
   // dE/dp per uncompressed component
   for (int c0 = threadIdx.x; c0 < P.n_soap_unc; c0 += 4 * NTHREADS) {
+    float gu[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int c = c0 + NTHREADS * u;
+      gu[u] = 0.f;
       if (c < P.n_soap_unc) {
         float v;
         if (umap) {
@@ -629,24 +674,36 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
         } else {
           v = Gp[c];
         }
-        Gu[c] = v;
+        gu[u] = v;
       }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + NTHREADS * u;
+      if (c < P.n_soap_unc) Gu[c] = gu[u];
     }
   }
   __syncthreads();
   // Y_l(n, n') = Gu(c) x 2 on the diagonal, sqrt 2 off it, over S_l; one (l, n) row per thread,
-  // c(a, b, l) = (a n_max - a(a-1)/2 + b - a) L1 + l stepped along the row
+  // eight columns at a time (their loads ahead of their stores); c(a, b, l) for a <= b is the
+  // uncompressed component (a n_max - a(a-1)/2 + b - a) L1 + l
   float* Y = Sl;
   for (int row = threadIdx.x; row < L1 * n_max; row += NTHREADS) {
     const int l = row / n_max, n = row - l * n_max;
     float* yr = Y + (size_t)row * n_max;
-    int pid = 0;  // a n_max - a(a-1)/2 - a for a = np
-    for (int np = 0; np < n; ++np) {
-      yr[np] = 1.41421356f * Gu[(pid + n) * L1 + l];
-      pid += n_max - np - 1;
+    for (int np0 = 0; np0 < n_max; np0 += 8) {
+      float y[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int np = np0 + j;
+        const int lo_ = n < np ? n : np, hi_ = n < np ? np : n;
+        const int c = (((lo_ * (2 * n_max - lo_ - 1)) >> 1) + hi_) * L1 + l;  // (a n_max - a(a+1)/2 + b) L1 + l
+        y[j] = np < n_max ? (lo_ == hi_ ? 2.f : 1.41421356f) * Gu[c] : 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (np0 + j < n_max) yr[np0 + j] = y[j];
     }
-    yr[n] = 2.f * Gu[(pid + n) * L1 + l];
-    for (int np = n + 1; np < n_max; ++np) yr[np] = 1.41421356f * Gu[(pid + np) * L1 + l];
   }
   __syncthreads();
   // H_l = 4 pi w(m) Y_l C_l over the columns of block l. A job is one column tile of one
