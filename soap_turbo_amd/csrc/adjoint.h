@@ -61,6 +61,8 @@ struct AdjointInputs {
   const std::vector<double>* uel_h;
   std::vector<RadialConsts> radial_tab;  // per species, block offset 0, n_max = alpha_max, with derivatives
   const int32_t* pair_atom;
+  const int* atom_ptr;                   // pairs of each atom (CSR over pair_atom)
+  const int* atom_pairs;
   const double* vec;
   int n_atoms;
   size_t lds_max;                        // LDS bytes one workgroup may allocate
@@ -73,15 +75,16 @@ int adjoint_download_pairs(AdjointState& s, hipStream_t st, int n_pairs, double*
 int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial,
                             std::string& msg);
 int adjoint_download_soap(AdjointState& s, hipStream_t st, int n_sites, int n_soap, double* soap, std::string& msg);
-// forces(:, a) = -sum of pair(:, k) over pairs with neighbour atom a; virial = -sum_k vec_k (x) pair_k,
-// reduced in per-block partials (vpart holds SCATTER_BLOCKS * 9 doubles).
-constexpr int SCATTER_BLOCKS = 1024;
-void launch_scatter_f64(hipStream_t st, int n_pairs, int n_atoms, const double* pair, const int32_t* pair_atom,
-                        const double* vec, const double* rjs, const double* thetas, const double* phis,
-                        double* forces, double* vpart, double* virial);
-void launch_scatter_f32(hipStream_t st, int n_pairs, int n_atoms, const float* pair, const int32_t* pair_atom,
-                        const double* vec, const double* rjs, const double* thetas, const double* phis,
-                        double* forces, double* vpart, double* virial);
+// forces(:, a) = -sum of pair(:, k) over pairs with neighbour atom a (atom_ptr/atom_pairs: the
+// pairs of each atom); virial = -sum_k vec_k (x) pair_k, reduced through per-workgroup partials
+// (vpart holds gather_blocks(n_atoms) * 9 doubles).
+inline int gather_blocks(int n_atoms) { return n_atoms > 0 ? (n_atoms + 15) / 16 : 1; }
+void launch_forces_f64(hipStream_t st, int n_atoms, const double* pair, const int* atom_ptr, const int* atom_pairs,
+                       const double* vec, const double* rjs, const double* thetas, const double* phis,
+                       double* forces, double* vpart, double* virial);
+void launch_forces_f32(hipStream_t st, int n_atoms, const float* pair, const int* atom_ptr, const int* atom_pairs,
+                       const double* vec, const double* rjs, const double* thetas, const double* phis,
+                       double* forces, double* vpart, double* virial);
 void adjoint_invalidate(AdjointState& s);
 void adjoint_release(AdjointState& s);
 

@@ -26,56 +26,65 @@ int ensure(AdjBuf& b, size_t bytes, std::string& msg) {
   return ST_OK;
 }
 
-__global__ void zero_k(double* x, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = 0.0;
-}
-
-// forces(:, a) -= pair(:, k) for every pair k whose neighbour is atom a;
-// per-block partial virials -sum_k vec_k (x) pair_k.
+// forces(:, a) = -sum of pair(:, k) over the pairs k whose neighbour is atom a
+// (atom_ptr/atom_pairs: those pairs per atom, in increasing k), gathered by 16
+// lanes per atom, so no atomics and no clearing pass; each workgroup also
+// leaves its partial virial -sum_k vec_k (x) pair_k over the pairs of its atoms.
 template <class T>
-__global__ __launch_bounds__(256) void scatter_k(int n_pairs, const T* __restrict__ pair,
-                                                 const int32_t* __restrict__ pair_atom, const double* __restrict__ vec,
-                                                 const double* __restrict__ rjs, const double* __restrict__ thetas,
-                                                 const double* __restrict__ phis, double* __restrict__ forces,
-                                                 double* __restrict__ vpart) {
-  __shared__ double red[9][4];
-  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_pairs; k += gridDim.x * blockDim.x) {
-    const double g0 = pair[3 * (size_t)k], g1 = pair[3 * (size_t)k + 1], g2 = pair[3 * (size_t)k + 2];
-    const int a = pair_atom[k];
-    atomicAdd(forces + 3 * (size_t)a, -g0);
-    atomicAdd(forces + 3 * (size_t)a + 1, -g1);
-    atomicAdd(forces + 3 * (size_t)a + 2, -g2);
-    double x, y, z;
-    if (vec) {
-      x = vec[3 * (size_t)k]; y = vec[3 * (size_t)k + 1]; z = vec[3 * (size_t)k + 2];
-    } else {
-      const double r = rjs[k], st_ = sin(thetas[k]);
-      x = r * st_ * cos(phis[k]); y = r * st_ * sin(phis[k]); z = r * cos(thetas[k]);
+__global__ __launch_bounds__(256) void gather_k(int n_atoms, const T* __restrict__ pair,
+                                                const int* __restrict__ atom_ptr, const int* __restrict__ atom_pairs,
+                                                const double* __restrict__ vec, const double* __restrict__ rjs,
+                                                const double* __restrict__ thetas, const double* __restrict__ phis,
+                                                double* __restrict__ forces, double* __restrict__ vpart) {
+  __shared__ double red[9][16];
+  const int g = threadIdx.x >> 4, li = threadIdx.x & 15;   // 16 atom groups per workgroup
+  const int a = blockIdx.x * 16 + g;
+  double f[3] = {0.0, 0.0, 0.0}, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (a < n_atoms) {
+    for (int e = atom_ptr[a] + li; e < atom_ptr[a + 1]; e += 16) {
+      const int k = atom_pairs[e];
+      const double g0 = pair[3 * (size_t)k], g1 = pair[3 * (size_t)k + 1], g2 = pair[3 * (size_t)k + 2];
+      f[0] -= g0; f[1] -= g1; f[2] -= g2;
+      double x, y, z;
+      if (vec) {
+        x = vec[3 * (size_t)k]; y = vec[3 * (size_t)k + 1]; z = vec[3 * (size_t)k + 2];
+      } else {
+        const double r = rjs[k], st_ = sin(thetas[k]);
+        x = r * st_ * cos(phis[k]); y = r * st_ * sin(phis[k]); z = r * cos(thetas[k]);
+      }
+      const double r3[3] = {x, y, z}, gg[3] = {g0, g1, g2};
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v[3 * i + j] -= r3[i] * gg[j];
     }
-    const double r3[3] = {x, y, z}, g[3] = {g0, g1, g2};
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) v[3 * i + j] -= r3[i] * g[j];
   }
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // sums over the 16 lanes of each atom group
 #pragma unroll
-  for (int q = 0; q < 9; ++q) {
-    double t = v[q];
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    if (lane == 0) red[q][w] = t;
+  for (int o = 8; o > 0; o >>= 1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] += __shfl_xor(f[c], o, 16);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) v[q] += __shfl_xor(v[q], o, 16);
+  }
+  if (li == 0) {
+    if (a < n_atoms) {
+      forces[3 * (size_t)a] = f[0];
+      forces[3 * (size_t)a + 1] = f[1];
+      forces[3 * (size_t)a + 2] = f[2];
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) red[q][g] = v[q];
   }
   __syncthreads();
   if (threadIdx.x < 9) {
     double t = 0.0;
-    for (int j = 0; j < 4; ++j) t += red[threadIdx.x][j];
+    for (int j = 0; j < 16; ++j) t += red[threadIdx.x][j];
     vpart[(size_t)blockIdx.x * 9 + threadIdx.x] = t;
   }
 }
 
-// one wavefront per virial component, lanes striding over the block partials
+// one wavefront per virial component, lanes striding over the workgroup partials
 __global__ __launch_bounds__(576) void virial_sum_k(int nb, const double* __restrict__ vpart,
                                                     double* __restrict__ virial) {
   const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -86,14 +95,11 @@ __global__ __launch_bounds__(576) void virial_sum_k(int nb, const double* __rest
 }
 
 template <class T>
-void scatter(hipStream_t st, int n_pairs, int n_atoms, const T* pair, const int32_t* pair_atom, const double* vec,
-             const double* rjs, const double* thetas, const double* phis, double* forces, double* vpart,
-             double* virial) {
-  zero_k<<<256, 256, 0, st>>>(forces, (size_t)n_atoms * 3);
-  int nb = (n_pairs + 255) / 256;
-  if (nb > SCATTER_BLOCKS) nb = SCATTER_BLOCKS;
-  if (nb < 1) nb = 1;
-  scatter_k<T><<<nb, 256, 0, st>>>(n_pairs, pair, pair_atom, vec, rjs, thetas, phis, forces, vpart);
+void gather(hipStream_t st, int n_atoms, const T* pair, const int* atom_ptr, const int* atom_pairs,
+            const double* vec, const double* rjs, const double* thetas, const double* phis, double* forces,
+            double* vpart, double* virial) {
+  const int nb = gather_blocks(n_atoms);
+  gather_k<T><<<nb, 256, 0, st>>>(n_atoms, pair, atom_ptr, atom_pairs, vec, rjs, thetas, phis, forces, vpart);
   virial_sum_k<<<1, 576, 0, st>>>(nb, vpart, virial);
 }
 
@@ -115,16 +121,16 @@ int download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces
 
 }  // namespace
 
-void launch_scatter_f64(hipStream_t st, int n_pairs, int n_atoms, const double* pair, const int32_t* pair_atom,
-                        const double* vec, const double* rjs, const double* thetas, const double* phis,
-                        double* forces, double* vpart, double* virial) {
-  scatter<double>(st, n_pairs, n_atoms, pair, pair_atom, vec, rjs, thetas, phis, forces, vpart, virial);
+void launch_forces_f64(hipStream_t st, int n_atoms, const double* pair, const int* atom_ptr, const int* atom_pairs,
+                       const double* vec, const double* rjs, const double* thetas, const double* phis,
+                       double* forces, double* vpart, double* virial) {
+  gather<double>(st, n_atoms, pair, atom_ptr, atom_pairs, vec, rjs, thetas, phis, forces, vpart, virial);
 }
 
-void launch_scatter_f32(hipStream_t st, int n_pairs, int n_atoms, const float* pair, const int32_t* pair_atom,
-                        const double* vec, const double* rjs, const double* thetas, const double* phis,
-                        double* forces, double* vpart, double* virial) {
-  scatter<float>(st, n_pairs, n_atoms, pair, pair_atom, vec, rjs, thetas, phis, forces, vpart, virial);
+void launch_forces_f32(hipStream_t st, int n_atoms, const float* pair, const int* atom_ptr, const int* atom_pairs,
+                       const double* vec, const double* rjs, const double* thetas, const double* phis,
+                       double* forces, double* vpart, double* virial) {
+  gather<float>(st, n_atoms, pair, atom_ptr, atom_pairs, vec, rjs, thetas, phis, forces, vpart, virial);
 }
 
 int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial,

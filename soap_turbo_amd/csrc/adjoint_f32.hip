@@ -1259,11 +1259,11 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   s.have_forces = false;
   if (in.pair_atom) {
     if ((rc = ensure(s.forces, (size_t)in.n_atoms * 24, msg))) return rc;
-    if ((rc = ensure(s.virial_part, (size_t)SCATTER_BLOCKS * 72, msg))) return rc;
+    if ((rc = ensure(s.virial_part, (size_t)gather_blocks(in.n_atoms) * 72, msg))) return rc;
     if ((rc = ensure(s.virial, 72, msg))) return rc;
     timer("adj_forces", [&] {
-      launch_scatter_f32(st, in.n_pairs, in.n_atoms, s.pair.as<float>(), in.pair_atom, in.vec, in.rjs, in.thetas,
-                         in.phis, s.forces.as<double>(), s.virial_part.as<double>(), s.virial.as<double>());
+      launch_forces_f32(st, in.n_atoms, s.pair.as<float>(), in.atom_ptr, in.atom_pairs, in.vec, in.rjs, in.thetas,
+                        in.phis, s.forces.as<double>(), s.virial_part.as<double>(), s.virial.as<double>());
     });
     s.have_forces = true;
     s.n_atoms = in.n_atoms;

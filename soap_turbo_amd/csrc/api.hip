@@ -51,7 +51,7 @@ struct st_ctx {
   // atoms (forces / virial)
   int n_atoms = 0;
   bool have_atoms = false, have_vec = false;
-  DevBuf pair_atom, vec;
+  DevBuf pair_atom, vec, atom_ptr, atom_pairs;
   // dE/dq
   int G_n_soap = 0;
   bool have_G = false;
@@ -484,18 +484,18 @@ int pack_mask(st_ctx* ctx, int n_species) {
 }
 
 // Forces and virial of the FP64 pair gradients, when the pair atoms are known.
-int scatter64(st_ctx* ctx) {
+int forces64(st_ctx* ctx) {
   ctx->have_forces64 = false;
   if (!ctx->have_atoms) return ST_OK;
   int rc;
   if ((rc = ensure(ctx, ctx->forces64, (size_t)ctx->n_atoms * 24))) return rc;
-  if ((rc = ensure(ctx, ctx->vpart64, (size_t)st::SCATTER_BLOCKS * 72))) return rc;
+  if ((rc = ensure(ctx, ctx->vpart64, (size_t)st::gather_blocks(ctx->n_atoms) * 72))) return rc;
   if ((rc = ensure(ctx, ctx->virial64, 72))) return rc;
   timed(ctx, "forces_f64", [&] {
-    st::launch_scatter_f64(ctx->stream, ctx->n_pairs, ctx->n_atoms, ctx->pair64.as<double>(),
-                           ctx->pair_atom.as<int32_t>(), ctx->have_vec ? ctx->vec.as<double>() : nullptr,
-                           ctx->rjs.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
-                           ctx->forces64.as<double>(), ctx->vpart64.as<double>(), ctx->virial64.as<double>());
+    st::launch_forces_f64(ctx->stream, ctx->n_atoms, ctx->pair64.as<double>(), ctx->atom_ptr.as<int>(),
+                          ctx->atom_pairs.as<int>(), ctx->have_vec ? ctx->vec.as<double>() : nullptr,
+                          ctx->rjs.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
+                          ctx->forces64.as<double>(), ctx->vpart64.as<double>(), ctx->virial64.as<double>());
   });
   ctx->have_forces64 = true;
   return ST_OK;
@@ -619,6 +619,16 @@ int st_upload_atoms(st_ctx* ctx, int32_t n_atoms, const int32_t* pair_atom, cons
   int rc;
   HIPCHK(hipSetDevice(ctx->device));
   if ((rc = upload(ctx, ctx->pair_atom, pair_atom, (size_t)ctx->n_pairs * 4))) return rc;
+  // the pairs of each atom, in increasing pair index (counting sort)
+  std::vector<int> aptr((size_t)n_atoms + 1, 0), apairs(ctx->n_pairs);
+  for (int p = 0; p < ctx->n_pairs; ++p) aptr[pair_atom[p] + 1]++;
+  for (int a = 0; a < n_atoms; ++a) aptr[a + 1] += aptr[a];
+  {
+    std::vector<int> fill(aptr.begin(), aptr.end() - 1);
+    for (int p = 0; p < ctx->n_pairs; ++p) apairs[fill[pair_atom[p]]++] = p;
+  }
+  if ((rc = upload(ctx, ctx->atom_ptr, aptr.data(), aptr.size() * 4))) return rc;
+  if ((rc = upload(ctx, ctx->atom_pairs, apairs.data(), apairs.size() * 4))) return rc;
   ctx->have_vec = vec != nullptr;
   if (vec && (rc = upload(ctx, ctx->vec, vec, (size_t)ctx->n_pairs * 24))) return rc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -755,7 +765,7 @@ int st_pairs_full_jacobian(st_ctx* ctx) {
   });
   HIPCHK(hipGetLastError());
   ctx->have_pair64 = true;
-  return scatter64(ctx);
+  return forces64(ctx);
 }
 
 int st_contract_resident(st_ctx* ctx, double sgn) {
@@ -780,7 +790,7 @@ int st_contract_resident(st_ctx* ctx, double sgn) {
   });
   HIPCHK(hipGetLastError());
   ctx->have_pair64 = true;
-  return scatter64(ctx);
+  return forces64(ctx);
 }
 
 int st_download(st_ctx* ctx, double* soap, double* cart) {
@@ -889,6 +899,8 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
     }
   }
   in.pair_atom = ctx->have_atoms ? ctx->pair_atom.as<int32_t>() : nullptr;
+  in.atom_ptr = ctx->have_atoms ? ctx->atom_ptr.as<int>() : nullptr;
+  in.atom_pairs = ctx->have_atoms ? ctx->atom_pairs.as<int>() : nullptr;
   in.vec = ctx->have_vec ? ctx->vec.as<double>() : nullptr;
   in.n_atoms = ctx->have_atoms ? ctx->n_atoms : 0;
   std::string msg;
