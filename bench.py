@@ -38,6 +38,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel, path=os.path.join(ROOT, "profiles", "config2_hbm_traffic.md")):
+    """HBM bytes per launch of `kernel` on config 2, from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a copy kernel of known size); None if absent."""
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def build_problem(cfg):
     import soap_turbo_amd as st
     from soap_turbo_amd.workload import make_workload, normal
@@ -60,10 +72,11 @@ def build_problem(cfg):
 
 
 def time_path(ctx, step, steps, warmup, kernels):
+    """Times `steps` steps with HIP events on the library's stream: once as is (the step time), then
+    again with per-kernel events around every launch (the per-kernel times)."""
     for _ in range(warmup):
         step()
     ctx.synchronize()
-    ctx.kernel_timing(True)
     ctx.event_record(0)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -72,6 +85,10 @@ def time_path(ctx, step, steps, warmup, kernels):
     ctx.synchronize()
     wall = time.perf_counter() - t0
     ms = ctx.event_elapsed(0, 1)
+    ctx.kernel_timing(True)
+    for _ in range(steps):
+        step()
+    ctx.synchronize()
     ktimes = {k: ctx.kernel_time(k) for k in kernels}
     ctx.kernel_timing(False)
     return ms, wall, ktimes
@@ -108,6 +125,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2, help="workload 1..5 (default 2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-size kernel against which "
+                         "HBM traffic counters are calibrated)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not part of this build",
@@ -115,6 +135,15 @@ def main():
         sys.exit(2)
 
     import soap_turbo_amd as st
+    if args.copy_mib > 0:
+        import torch
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_mib} MiB device to device")
+        del src, dst
     log(f"building workload {args.config}")
     w, keep, prm, cmp_, comp, n_soap, G = build_problem(args.config)
     log(f"sites {w.n_sites} pairs {w.n_pairs} n_soap {n_soap}")
@@ -163,8 +192,8 @@ def main():
         flops_site = n_nb * 12 * n_max * k_max + 8 * n_max * n_max * k_max
         achieved = flops_site * w.n_sites / (avg_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                    "avg_launch_ms": avg_ms}
+                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                    "traffic": measured_traffic(name) if args.config == 2 else None, "avg_launch_ms": avg_ms}
         metric_path = "FP32 adjoint"
     else:
         step_ms = ms64 / args.steps
@@ -175,7 +204,9 @@ def main():
         bytes_launch = w.n_pairs * (24 + 24 * n_soap) + 8 * n_soap * w.n_sites
         achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": "derivatives_f64", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms}
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": measured_traffic("derivatives_f64") if args.config == 2 else None,
+                    "avg_launch_ms": avg_ms}
         metric_path = "FP64 full-Jacobian (FP32 adjoint unavailable)"
     value = w.n_sites / (step_ms * 1e-3)
     out = {

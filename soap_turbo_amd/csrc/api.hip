@@ -46,6 +46,7 @@ struct st_ctx {
   // resident neighbour list
   bool have_nb = false;
   int n_sites = 0, n_pairs = 0, n_species_mask = 0, max_mult = 1;
+  int packed_species = 0;  // n_species the packed pair masks were made for (0: none since the upload)
   std::vector<int> site_ptr_h;
   DevBuf site_ptr, species, mult, mask, spmask, first, rjs, thetas, phis;
   // atoms (forces / virial)
@@ -478,8 +479,10 @@ int pack_mask(st_ctx* ctx, int n_species) {
   int rc;
   if ((rc = ensure(ctx, ctx->spmask, (size_t)ctx->n_pairs * 4))) return rc;
   if ((rc = ensure(ctx, ctx->first, (size_t)ctx->n_pairs))) return rc;
+  if (ctx->packed_species == n_species) return ST_OK;
   st::launch_pack_pairs(ctx->stream, ctx->n_pairs, n_species, ctx->mask.as<int32_t>(), ctx->site_ptr.as<int>(),
                         ctx->n_sites, ctx->spmask.as<uint32_t>(), ctx->first.as<uint8_t>());
+  ctx->packed_species = n_species;
   return ST_OK;
 }
 
@@ -593,6 +596,7 @@ int st_upload_neighbours(st_ctx* ctx, const st_neighbours* nb) {
   ctx->n_sites = nb->n_sites;
   ctx->n_pairs = nb->n_pairs;
   ctx->n_species_mask = nb->n_species;
+  ctx->packed_species = 0;
   ctx->max_mult = nb->max_mult;
   ctx->site_ptr_h = ptr;
   const size_t P = nb->n_pairs;
