@@ -171,7 +171,7 @@ def main():
     except st.SoapTurboError as e:
         log(f"FP32 adjoint path unavailable: {e}")
         have32 = False
-    k32 = ["adj_tables", "adj_transpose_g", "adj_site_f32", "adj_forces"]
+    k32 = ["adj_tables", "adj_site_f32", "adj_forces"]
     if have32:
         log("timing the FP32 adjoint path")
         ms32, wall32, kt32 = time_path(ctx, step32, args.steps, args.warmup, k32)

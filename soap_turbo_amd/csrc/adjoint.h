@@ -25,8 +25,9 @@ struct AdjointState {
   AdjBuf rtab, atab, tabinfo;
   int n_seg = 0;
   // per-call work and outputs
-  AdjBuf cnk, soap, sdp, H, pair, forces, virial_part, virial, G32, order, atom_ptr, atom_pairs, vecbuf;
-  AdjBuf bins, el32;
+  AdjBuf soap, sdp, pair, forces, virial_part, virial, G32;
+  int g32_sites = 0, g32_soap = 0;  // shape of the site-major FP32 dE/dq in G32 (0: none)
+  AdjBuf el32;
   std::string atoms_key, el_key;
   int n_pf = 0;
   int n_pairs = 0, n_atoms = 0, n_sites = 0, n_soap = 0;
@@ -71,6 +72,9 @@ struct AdjointInputs {
 using Timer = std::function<void(const char*, const std::function<void()>&)>;
 
 int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const Timer& timer, std::string& msg);
+// dE/dq G(n_sites, n_soap) (FP64, site fastest) -> the site-major FP32 copy the site kernel reads;
+// run when dE/dq is uploaded.
+int adjoint_set_dedq(AdjointState& s, hipStream_t st, int n_sites, int n_soap, const double* G, std::string& msg);
 int adjoint_download_pairs(AdjointState& s, hipStream_t st, int n_pairs, double* pair, std::string& msg);
 int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial,
                             std::string& msg);

@@ -30,6 +30,8 @@ int ensure(AdjBuf& b, size_t bytes, std::string& msg) {
 // (atom_ptr/atom_pairs: those pairs per atom, in increasing k), gathered by 16
 // lanes per atom, so no atomics and no clearing pass; each workgroup also
 // leaves its partial virial -sum_k vec_k (x) pair_k over the pairs of its atoms.
+// (Summing the partials in the last workgroup to finish needs a device-scope
+// release per workgroup, which on this GPU costs more than a second launch.)
 template <class T>
 __global__ __launch_bounds__(256) void gather_k(int n_atoms, const T* __restrict__ pair,
                                                 const int* __restrict__ atom_ptr, const int* __restrict__ atom_pairs,
@@ -145,11 +147,12 @@ int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double
 void adjoint_invalidate(AdjointState& s) {
   s.have_out = false;
   s.have_forces = false;
+  s.g32_sites = s.g32_soap = 0;
 }
 
 void adjoint_release(AdjointState& s) {
-  AdjBuf* bufs[] = {&s.rtab, &s.atab, &s.tabinfo, &s.cnk, &s.soap, &s.sdp, &s.H, &s.pair, &s.forces,
-                    &s.virial_part, &s.virial, &s.G32, &s.order, &s.atom_ptr, &s.atom_pairs, &s.vecbuf, &s.bins, &s.el32};
+  AdjBuf* bufs[] = {&s.rtab, &s.atab, &s.tabinfo, &s.soap, &s.sdp, &s.pair, &s.forces,
+                    &s.virial_part, &s.virial, &s.G32, &s.el32};
   for (AdjBuf* b : bufs)
     if (b->p) {
       (void)hipFree(b->p);

@@ -1214,14 +1214,13 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     (void)hipStreamSynchronize(st);
     s.el_key = *in.tables_key;
   }
-  if ((rc = ensure(s.G32, (size_t)in.n_sites * in.n_soap * 4, msg))) return rc;
   if ((rc = ensure(s.soap, (size_t)in.n_sites * in.n_soap * 4, msg))) return rc;
   if ((rc = ensure(s.sdp, (size_t)in.n_sites * 4, msg))) return rc;
   if ((rc = ensure(s.pair, (size_t)in.n_pairs * 12, msg))) return rc;
-  timer("adj_transpose_g", [&] {
-    dim3 grid((in.n_soap + 31) / 32, (in.n_sites + 31) / 32);
-    transpose_g_k<<<grid, 256, 0, st>>>(in.n_sites, in.n_soap, in.G, s.G32.as<float>());
-  });
+  if (s.g32_sites != in.n_sites || s.g32_soap != in.n_soap) {
+    msg = "dE/dq of this shape has not been uploaded";
+    return ST_ERR_STATE;
+  }
   const float* pf = s.tabinfo.as<float>();
   const float* cen = pf + s.n_pf;
   const float* oel = in.out_ptr ? s.el32.as<float>() : nullptr;
@@ -1268,6 +1267,22 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     s.have_forces = true;
     s.n_atoms = in.n_atoms;
   }
+  return ST_OK;
+}
+
+int adjoint_set_dedq(AdjointState& s, hipStream_t st, int n_sites, int n_soap, const double* G, std::string& msg) {
+  int rc;
+  if ((rc = ensure(s.G32, (size_t)n_sites * n_soap * 4 + 4, msg))) return rc;
+  if (n_sites > 0) {
+    dim3 grid((n_soap + 31) / 32, (n_sites + 31) / 32);
+    transpose_g_k<<<grid, 256, 0, st>>>(n_sites, n_soap, G, s.G32.as<float>());
+    if (hipGetLastError() != hipSuccess) {
+      msg = "dE/dq conversion failed";
+      return ST_ERR_HIP;
+    }
+  }
+  s.g32_sites = n_sites;
+  s.g32_soap = n_soap;
   return ST_OK;
 }
 

@@ -611,6 +611,7 @@ int st_upload_neighbours(st_ctx* ctx, const st_neighbours* nb) {
   ctx->have_nb = true;
   ctx->have_soap = ctx->have_cart = ctx->ct_ready = ctx->have_pair64 = false;
   ctx->have_vec = false;
+  ctx->have_G = false;  // dE/dq is per site: upload it again for the new sites
   st::adjoint_invalidate(ctx->adj);
   return ST_OK;
 }
@@ -647,6 +648,10 @@ int st_upload_dedq(st_ctx* ctx, int32_t n_soap, const double* G) {
   HIPCHK(hipSetDevice(ctx->device));
   int rc = upload(ctx, ctx->G, G, (size_t)n_soap * ctx->n_sites * 8);
   if (rc) return rc;
+  // the FP32 adjoint path reads dE/dq site-major in FP32: converted here, once per upload
+  std::string msg;
+  if ((rc = st::adjoint_set_dedq(ctx->adj, ctx->stream, ctx->n_sites, n_soap, ctx->G.as<double>(), msg)))
+    return fail(ctx, rc, msg);
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->G_n_soap = n_soap;
   ctx->have_G = true;
@@ -946,6 +951,9 @@ int st_soap_contract(st_ctx* ctx, const double* soap, const double* G, double sg
   // the reference contracts against the soap it is handed, not its own copy
   if ((rc = upload(ctx, ctx->soap, soap, (size_t)ctx->n_sites * ctx->n_soap * 8))) return rc;
   if ((rc = upload(ctx, ctx->G, G, (size_t)ctx->n_sites * ctx->n_soap * 8))) return rc;
+  std::string msg;
+  if ((rc = st::adjoint_set_dedq(ctx->adj, ctx->stream, ctx->n_sites, ctx->n_soap, ctx->G.as<double>(), msg)))
+    return fail(ctx, rc, msg);
   ctx->G_n_soap = ctx->n_soap;
   ctx->have_G = true;
   if ((rc = st_contract_resident(ctx, sgn))) return rc;

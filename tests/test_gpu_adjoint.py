@@ -73,6 +73,22 @@ def test_adjoint_f32_forces_sum_to_zero_and_repeat(ctx):
     scale = np.abs(f1).max()
     assert np.abs(f1.sum(axis=1)).max() <= 1e-5 * scale * np.sqrt(f1.shape[1])
     assert rel(f2, f1) <= 1e-6
+    # the forces are gathered per atom in a fixed order: repeat calls agree exactly, virial included
+    _, v1 = ctx.download_forces(1)
+    ctx.adjoint_f32(prm, cmp_)
+    f3, v3 = ctx.download_forces(1)
+    assert np.array_equal(f3, f2) and np.array_equal(v3, v1)
+
+
+def test_adjoint_f32_needs_dedq_for_the_current_sites(ctx):
+    # a new neighbour list (other sites) invalidates dE/dq: the FP32 path refuses until it is uploaded
+    w = make_workload(1)
+    keep, prm, cmp_, comp, n_soap, W, S, G = setup(ctx, w)
+    ctx.adjoint_f32(prm, cmp_)
+    w2 = make_workload(1, n_sites=50)
+    ctx.upload_neighbours(w2.n_neigh, w2.species.T, w2.species_multiplicity, w2.mask.T, w2.rjs, w2.thetas, w2.phis)
+    with pytest.raises(st.SoapTurboError):
+        ctx.adjoint_f32(prm, cmp_)
 
 
 def test_adjoint_f32_central_differences(ctx):
