@@ -313,6 +313,9 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   const int i = order ? order[blockIdx.x] : blockIdx.x;
   const int n_max = P.n_max, n_species = P.n_species;
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  // one tile of neighbours: the backward pass reuses the forward pass's set-up (kept in registers,
+  // affordable up to n_max = 32)
+  const bool one_tile = NTT <= 2 && p1 - p0 <= PT;
   const float fourpi = 12.566370614359172f;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
@@ -387,8 +390,10 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     }
   };
   // (n, pair) radial items: R_n times scale (and the central weight on the site's own
-  // pair), and dR_n/dr if Rd is given
-  auto radial_items = [&](int tid, float scale, float* R, float* Rd, int stride) {
+  // pair), and dR_n/dr if Rd is given. With kv/kd, R_n and dR_n/dr of every pair but the
+  // site's own are also kept in registers, for the backward pass of a one-tile site.
+  float kv[RIT], kd[RIT], kg[GIT], kgd[GIT];
+  auto radial_items = [&](int tid, float scale, float* R, float* Rd, int stride, bool keep) {
     // pass 1: every table load of the thread's items; pass 2: evaluate and store (an LDS store
     // between them would hold back the next item's loads, as the compiler cannot tell
     // the LDS arrays apart)
@@ -416,13 +421,20 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
       const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
       if (n < n_max) {
         const float x = tt[u];
-        R[n * stride + pl] = sc[u] * (c0[u] + x * (c1[u] + x * (c2[u] + x * c3[u])));
-        if (Rd) Rd[n * stride + pl] = (c1[u] + x * (2.f * c2[u] + 3.f * x * c3[u])) * ih[u];
+        const float v = c0[u] + x * (c1[u] + x * (c2[u] + x * c3[u]));
+        const float d = (c1[u] + x * (2.f * c2[u] + 3.f * x * c3[u])) * ih[u];
+        R[n * stride + pl] = sc[u] * v;
+        if (Rd) Rd[n * stride + pl] = d;
+        if (keep) {
+          const bool self = (pr_msk[pl] & SELF) != 0;
+          kv[u] = self ? 0.f : v;
+          kd[u] = self ? 0.f : d;
+        }
       }
     }
   };
-  // (l, pair) items: g_l(r), and dg_l/dr if Gdr is given
-  auto gl_items = [&](int tid, float* Gr, float* Gdr) {
+  // (l, pair) items: g_l(r), and dg_l/dr if Gdr is given (both kept in kg/kgd with keep)
+  auto gl_items = [&](int tid, float* Gr, float* Gdr, bool keep) {
 #pragma unroll
     for (int u = 0; u < GIT; ++u) {
       const int t = tid + NTHREADS * u, l = t / PT, pl = t - l * PT;
@@ -437,6 +449,10 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
         }
         Gr[l * PTS + pl] = g;
         if (Gdr) Gdr[l * PTS + pl] = gd;
+        if (keep) {
+          kg[u] = g;
+          kgd[u] = gd;
+        }
       }
     }
   };
@@ -473,8 +489,8 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     // the thread index made opaque per tile: item addresses are formed here, not hoisted
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    radial_items(tid, fourpi, Rt, nullptr, PTF);
-    gl_items(tid, Gt, nullptr);
+    radial_items(tid, fourpi, Rt, nullptr, PTF, one_tile);
+    gl_items(tid, Gt, nullptr, one_tile);
     __syncthreads();
     // (m, pair) items: A_k = preflm P_l^m g_l e^{-i m phi} for l = m .. L, as (re, im) rows.
     // m is uniform over a wavefront; l is unrolled, P_l^m by the recursions of get_plm_array
@@ -778,18 +794,38 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   float self[3] = {0.f, 0.f, 0.f};
   for (int tb = p0; tb < p1; tb += PT) {
     __syncthreads();
-    stage(tb, false);
-    __syncthreads();
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    radial_items(tid, 1.f, Rb, Rdb, PTS);
-    gl_items(tid, Gl, Gd);
+    if (one_tile) {
+      // the forward pass staged this tile and kept R, R', g, g' of its items: no table loads
+#pragma unroll
+      for (int u = 0; u < RIT; ++u) {
+        const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
+        if (n < n_max) {
+          Rb[n * PTS + pl] = kv[u];
+          Rdb[n * PTS + pl] = kd[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < GIT; ++u) {
+        const int t = tid + NTHREADS * u, l = t / PT, pl = t - l * PT;
+        if (l < L1) {
+          Gl[l * PTS + pl] = kg[u];
+          Gd[l * PTS + pl] = kgd[u];
+        }
+      }
+    } else {
+      stage(tb, false);
+      __syncthreads();
+      radial_items(tid, 1.f, Rb, Rdb, PTS, false);
+      gl_items(tid, Gl, Gd, false);
+    }
     // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
 #pragma unroll
     for (int u = 0; u < PIT; ++u) {
       const int t = tid + NTHREADS * u, m = t / PT, pl = t - m * PT;
       if (m <= L + 1) {
-        if (pr_msk[pl] != 0) {
+        if (pr_msk[pl] != 0 && tb + pl != p0) {
           plm_column<L + 1>(m, pr_ct[pl], pr_st[pl],
                             [&](int l, float plm) { Fp[(l * (l + 1) / 2 + m) * PTS + pl] = plm; });
           if (m <= L) Em[m * PTS + pl] = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
