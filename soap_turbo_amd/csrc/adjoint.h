@@ -25,7 +25,9 @@ struct AdjointState {
   AdjBuf rtab, atab, tabinfo;
   int n_seg = 0;
   // per-call work and outputs
-  AdjBuf soap, sdp, pair, forces, virial_part, virial, G32;
+  AdjBuf soap, sdp, pair, forces, virial_part, virial, G32, order, tickets;
+  bool order_ok = false;                 // order holds the sites of the current neighbour list
+  unsigned long long ticket_base = 0;    // the site queue's counter at the start of the next launch
   int g32_sites = 0, g32_soap = 0;  // shape of the site-major FP32 dE/dq in G32 (0: none)
   AdjBuf el32;
   std::string atoms_key, el_key;
@@ -67,6 +69,7 @@ struct AdjointInputs {
   const double* vec;
   int n_atoms;
   size_t lds_max;                        // LDS bytes one workgroup may allocate
+  int n_cu;                              // compute units of the device
 };
 
 using Timer = std::function<void(const char*, const std::function<void()>&)>;

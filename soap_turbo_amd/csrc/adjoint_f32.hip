@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -262,6 +263,8 @@ struct SiteShape {
   static constexpr int CS = CSTR + 4;                     // C row stride
   static constexpr int KMP = ((KM + 15) / 16) * 16;       // k padded to tiles of 16
   static constexpr int HS = 2 * KMP + 16;                 // H row stride: re block, im block, pad
+  // waves per SIMD the registers are held to: 3 (168 VGPRs) up to n_max 32 and l_max 9
+  static constexpr int waves(int ntt) { return ntt >= 3 || L >= 10 ? 2 : 3; }
 };
 
 // One site per workgroup of 256 threads (4 wavefronts); NTT = ceil(n_max / 16).
@@ -278,7 +281,7 @@ struct SiteShape {
 // 16 x 4 MFMA operand reads hit 64 distinct banks; PTS (odd) elsewhere; C rows
 // CSTR + 4 and H rows 2 KMP + 16 for the same reason.
 template <int L, int NTT>
-__global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
+__global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_site_k(
     AdjParams P, const int* __restrict__ order, const int* __restrict__ site_ptr, const float* __restrict__ rtab,
     const float* __restrict__ atab, const float* __restrict__ preflm_g, const float* __restrict__ central,
     const int32_t* __restrict__ site_species, int max_mult, const int32_t* __restrict__ species_mult,
@@ -288,7 +291,8 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     const int* __restrict__ unc_ptr, const int* __restrict__ unc_e, const int32_t* __restrict__ P_i,
     const float* __restrict__ P_el, const int4* __restrict__ ocomp, const float* __restrict__ oel,
     const int* __restrict__ umap, const float* __restrict__ uel, float* __restrict__ q_out,
-    float* __restrict__ sdp_out, float* __restrict__ pair_out) {
+    float* __restrict__ sdp_out, float* __restrict__ pair_out, unsigned long long* __restrict__ tickets,
+    unsigned long long ticket_base) {
   using Sh = SiteShape<L>;
   constexpr int KM = Sh::KM, KM1 = Sh::KM1, L1 = Sh::L1, CSTR = Sh::CSTR, CT = Sh::CT;
   constexpr int CS = Sh::CS, KMP = Sh::KMP, HS = Sh::HS, KT = KMP / 16;
@@ -310,15 +314,16 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   __shared__ int sp_off[ADJ_MAX_SPECIES], sp_a[ADJ_MAX_SPECIES], sp_rt[ADJ_MAX_SPECIES];
   __shared__ float sp_cw[ADJ_MAX_SPECIES];
   __shared__ unsigned char nsp[64];                 // species block of each n
-  const int i = order ? order[blockIdx.x] : blockIdx.x;
+  __shared__ int site_next;
   const int n_max = P.n_max, n_species = P.n_species;
-  const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  int i = 0, p0 = 0, p1 = 0;  // the current site and its pairs
   // one tile of neighbours: the backward pass reuses the forward pass's set-up (kept in registers,
   // affordable up to n_max = 32)
-  const bool one_tile = NTT <= 2 && p1 - p0 <= PT;
+  bool one_tile = false;
   const float fourpi = 12.566370614359172f;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int li = lane & 15, lk = lane >> 4;
+  int tx = threadIdx.x;
+  int wv = tx >> 6, lane = tx & 63;
+  int li = lane & 15, lk = lane >> 4;
   int* cell = reinterpret_cast<int*>(smem);        // n_species * PT radial cells (-1: none)
   float* cell_t = smem + n_species * PT;           // n_species * PT
   float* cell_ih = cell_t + n_species * PT;        // n_species * PT
@@ -342,8 +347,8 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   // The tile's pairs: r, species bits of contributing pairs, sin/cos of theta and phi,
   // and per species the radial table cell (FP64 cell coordinate) or -1.
   auto stage = [&](int tb, bool with_self) {
-    if (threadIdx.x < PT) {
-      const int pl = threadIdx.x, p = tb + pl;
+    if (tx < PT) {
+      const int pl = tx, p = tb + pl;
       uint32_t msk = 0;
       double rd = 1.0, th = 0.0, ph = 0.0;
       if (p < p1) {
@@ -474,6 +479,23 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
       for (int q = 0; q < P.sp[s].a; ++q) nsp[P.sp[s].off + q] = (unsigned char)s;
     }
   }
+  // Sites are taken from a queue (tickets counted from ticket_base) in the order given, heaviest
+  // first, until it runs dry: the grid is one resident wave of workgroups.
+  for (;;) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = atomicAdd(tickets, 1ull) - ticket_base;
+    site_next = t < (unsigned long long)P.n_sites ? (order ? order[t] : (int)t) : -1;
+  }
+  __syncthreads();
+  i = site_next;
+  if (i < 0) break;
+  // lane coordinates made opaque per site: what derives from them is formed per site, not
+  // hoisted out of the queue loop into long-lived registers
+  asm volatile("" : "+v"(tx), "+v"(wv), "+v"(lane), "+v"(li), "+v"(lk));
+  p0 = site_ptr[i];
+  p1 = site_ptr[i + 1];
+  one_tile = NTT <= 2 && p1 - p0 <= PT;
   // --------------------------------------------------------------- forward
   floatx4 hi[TPW], lo[TPW];
 #pragma unroll
@@ -487,7 +509,7 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     stage(tb, true);
     __syncthreads();
     // the thread index made opaque per tile: item addresses are formed here, not hoisted
-    int tid = threadIdx.x;
+    int tid = tx;
     asm volatile("" : "+v"(tid));
     radial_items(tid, fourpi, Rt, nullptr, PTF, one_tile);
     gl_items(tid, Gt, nullptr, one_tile);
@@ -625,7 +647,7 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   // q, |p|, and Gp = (G - q (q.G)) / |p| from one pass: q.G = (p.G) / |p|
   const int n_soap = P.n_soap;
   float sums[2] = {0.f, 0.f};
-  for (int s0 = threadIdx.x; s0 < n_soap; s0 += 4 * NTHREADS) {
+  for (int s0 = tx; s0 < n_soap; s0 += 4 * NTHREADS) {
     float vq[4], gq[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -665,15 +687,15 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   if (sdp < 1e-5f) sdp = 1.f;
   const float inv = 1.f / sdp;
   const float qg = sums[1] * inv;
-  for (int s = threadIdx.x; s < n_soap; s += NTHREADS) {
+  for (int s = tx; s < n_soap; s += NTHREADS) {
     const float q = qv[s] * inv;
     q_out[(size_t)i * n_soap + s] = q;
     Gp[s] = (Gp[s] - q * qg) * inv;
   }
-  if (threadIdx.x == 0) sdp_out[i] = sdp;
+  if (tx == 0) sdp_out[i] = sdp;
   __syncthreads();
   // dE/dp per uncompressed component
-  for (int c0 = threadIdx.x; c0 < P.n_soap_unc; c0 += 4 * NTHREADS) {
+  for (int c0 = tx; c0 < P.n_soap_unc; c0 += 4 * NTHREADS) {
     float gu[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -704,7 +726,7 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   // eight columns at a time (their loads ahead of their stores); c(a, b, l) for a <= b is the
   // uncompressed component (a n_max - a(a-1)/2 + b - a) L1 + l
   float* Y = Sl;
-  for (int row = threadIdx.x; row < L1 * n_max; row += NTHREADS) {
+  for (int row = tx; row < L1 * n_max; row += NTHREADS) {
     const int l = row / n_max, n = row - l * n_max;
     float* yr = Y + (size_t)row * n_max;
     for (int np0 = 0; np0 < n_max; np0 += 8) {
@@ -783,7 +805,7 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
       __builtin_amdgcn_sched_barrier(0);  // one job's operands live at a time
     }
     // H columns k in [KM, KMP) and [KMP + KM, 2 KMP): zero
-    for (int t = threadIdx.x; t < n_max * 2 * (KMP - KM); t += NTHREADS) {
+    for (int t = tx; t < n_max * 2 * (KMP - KM); t += NTHREADS) {
       const int r = t / (2 * (KMP - KM)), j = t - r * 2 * (KMP - KM);
       const int hc = j < KMP - KM ? KM + j : KMP + KM + (j - (KMP - KM));
       C[r * HS + hc] = 0.f;
@@ -794,7 +816,7 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
   float self[3] = {0.f, 0.f, 0.f};
   for (int tb = p0; tb < p1; tb += PT) {
     __syncthreads();
-    int tid = threadIdx.x;
+    int tid = tx;
     asm volatile("" : "+v"(tid));
     if (one_tile) {
       // the forward pass staged this tile and kept R, R', g, g' of its items: no table loads
@@ -939,11 +961,12 @@ __global__ __launch_bounds__(NTHREADS, 3) void adjoint_site_k(
     }
   }
   block_sums_f<3>(self, red);
-  if (threadIdx.x == 0) {
+  if (tx == 0) {
     pair_out[3 * (size_t)p0] = self[0];
     pair_out[3 * (size_t)p0 + 1] = self[1];
     pair_out[3 * (size_t)p0 + 2] = self[2];
   }
+  }  // site queue
 }
 
 // G(n_sites, n_soap) FP64 (site fastest) -> site-major FP32
@@ -985,14 +1008,18 @@ void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order
                  const double* th, const double* ph, const float* G32, const int4* comp, const int* out_ptr,
                  const int* out_c, const float* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
                  const float* P_el, const int4* ocomp, const float* oel, const int* umap, const float* uel, float* q,
-                 float* sdp, float* pair) {
-  if (sh > 64 * 1024)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&adjoint_site_k<L, NTT>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-  adjoint_site_k<L, NTT><<<P.n_sites, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, central, species,
-                                                          max_mult, mult, spmask, rjs, th, ph, G32, comp, out_ptr,
-                                                          out_c, out_el, unc_ptr, unc_e, P_i, P_el, ocomp, oel, umap,
-                                                          uel, q, sdp, pair);
+                 float* sdp, float* pair, unsigned long long* tickets, unsigned long long& ticket_base, int n_cu) {
+  const void* fn = reinterpret_cast<const void*>(&adjoint_site_k<L, NTT>);
+  if (sh > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTHREADS, sh) != hipSuccess || per_cu < 1) per_cu = 1;
+  const int grid = std::max(1, std::min(P.n_sites, per_cu * n_cu));
+  adjoint_site_k<L, NTT><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, central, species,
+                                                     max_mult, mult, spmask, rjs, th, ph, G32, comp, out_ptr, out_c,
+                                                     out_el, unc_ptr, unc_e, P_i, P_el, ocomp, oel, umap, uel, q, sdp,
+                                                     pair, tickets, ticket_base);
+  // every workgroup takes one ticket past the last site
+  ticket_base += (unsigned long long)P.n_sites + (unsigned long long)grid;
 }
 
 // host copy of the per-call constant tables that live next to the FP64 ones
@@ -1264,17 +1291,43 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   const float* oel1 = in.ocomp ? s.el32.as<float>() + n_oe + n_pe : nullptr;
   const float* uel1 = in.ocomp ? oel1 + n_o1 : nullptr;
   (void)n_u1;
+  // sites by falling neighbour count (longest first from the queue), once per neighbour list
+  if (!s.order_ok) {
+    std::vector<int> ord(in.n_sites);
+    std::iota(ord.begin(), ord.end(), 0);
+    const std::vector<int>& sp = *in.site_ptr_h;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return sp[a + 1] - sp[a] > sp[b + 1] - sp[b]; });
+    if ((rc = ensure(s.order, (size_t)in.n_sites * 4, msg))) return rc;
+    if (in.n_sites > 0 &&
+        hipMemcpyAsync(s.order.p, ord.data(), ord.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+      msg = "upload of the site order failed";
+      return ST_ERR_HIP;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+      msg = "upload of the site order failed";
+      return ST_ERR_HIP;
+    }
+    s.order_ok = true;
+  }
+  if (!s.tickets.p) {
+    if ((rc = ensure(s.tickets, 8, msg))) return rc;
+    if (hipMemsetAsync(s.tickets.p, 0, 8, st) != hipSuccess) {
+      msg = "clearing the site queue failed";
+      return ST_ERR_HIP;
+    }
+    s.ticket_base = 0;
+  }
   bool ok = true;
   timer("adj_site_f32", [&] {
     const int ntt = (in.n_max + 15) / 16;
     switch (in.l_max * 4 + ntt) {
 #define LS1(l, t)                                                                                                  \
   case l * 4 + t:                                                                                                  \
-    launch_site<l, t>(st, P, sh, nullptr, in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cen,            \
+    launch_site<l, t>(st, P, sh, s.order.as<int>(), in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cen,  \
                       in.species, in.max_mult, in.mult, in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(),  \
                       in.comp, in.out_ptr, in.out_c, oel, in.unc_ptr, in.unc_e, in.P_i, pel, in.ocomp, oel1,       \
                       in.ocomp ? in.umap : nullptr, uel1, s.soap.as<float>(), s.sdp.as<float>(),                   \
-                      s.pair.as<float>());                                                                         \
+                      s.pair.as<float>(), s.tickets.as<unsigned long long>(), s.ticket_base, in.n_cu);            \
     break;
 #define LS(l) LS1(l, 1) LS1(l, 2) LS1(l, 3)
       LS(0) LS(1) LS(2) LS(3) LS(4) LS(5) LS(6) LS(7) LS(8) LS(9) LS(10) LS(11) LS(12)

@@ -35,6 +35,7 @@ struct st_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   size_t lds_max = 64 * 1024;  // LDS bytes one workgroup may allocate on this device
+  int n_cu = 1;                // compute units
   std::string err;
 
   // basis: injected or built (cache key = parameters that change W and S)
@@ -521,6 +522,9 @@ int st_create(st_ctx** out, int device) {
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
     c->lds_max = (size_t)lds;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    c->n_cu = ncu;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return ST_ERR_HIP;
@@ -895,6 +899,7 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   in.uel_h = &ctx->uel_h;
   in.tables_key = &ctx->tables_key;
   in.lds_max = ctx->lds_max;
+  in.n_cu = ctx->n_cu;
   {
     int off = 0;
     for (int s = 0; s < p->n_species; ++s) {
