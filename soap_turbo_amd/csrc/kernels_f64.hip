@@ -387,10 +387,14 @@ __global__ __launch_bounds__(128) void angular_k(AngularConsts c, int n_pairs, c
 }
 
 // ---------------------------------------------------------------- cnk
-// One workgroup per site: cnk(k,n,i) = sum_j 4 pi R(n,j) A(k,j) in
-// neighbour order, then the poly3gauss central-atom term
-// (src/soap_turbo.f90:626-664). central[s*n_max + n] holds that term per
-// species (zero where the species has none).
+// One workgroup per site: cnk(k,n,i) = sum_j 4 pi R(n,j) A(k,j), then the
+// poly3gauss central-atom term (src/soap_turbo.f90:626-664); central[s*n_max + n]
+// holds that term per species (zero where the species has none). The sum over
+// neighbours is a product on the FP64 matrix cores: rows n (tiles of 16),
+// columns the real and imaginary parts of A_k (tiles of 16), K the neighbours
+// four at a time; each wavefront takes every fourth tile.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void cnk_k(int n_sites, int n_max, int k_max, const int* __restrict__ site_ptr,
                                              const double* __restrict__ rad, const double2* __restrict__ ang,
                                              const int32_t* __restrict__ site_species, int max_mult,
@@ -399,22 +403,40 @@ __global__ __launch_bounds__(256) void cnk_k(int n_sites, int n_max, int k_max, 
   const int i = blockIdx.x;
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const double fourpi = 4.0 * 3.141592653589793;
-  for (int t = threadIdx.x; t < n_max * k_max; t += blockDim.x) {
-    const int n = t / k_max, k = t - n * k_max;
-    double re = 0.0, im = 0.0;
-    for (int p = p0; p < p1; ++p) {
-      const double fr = fourpi * rad[(size_t)p * n_max + n];
-      const double2 a = ang[(size_t)p * k_max + k];
-      re = re + fr * a.x;
-      im = im + fr * a.y;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
+  const int nt = (n_max + 15) >> 4, ct = (2 * k_max + 15) >> 4;
+  const double* angd = reinterpret_cast<const double*>(ang);
+  double* cnkd = reinterpret_cast<double*>(cnk);
+  for (int t = w; t < nt * ct; t += 4) {
+    const int a = t / ct, b = t - a * ct;
+    const int n = 16 * a + li, col = 16 * b + li;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int q0 = p0; q0 < p1; q0 += 16) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int p = q0 + 4 * s + g;
+        const bool live = p < p1;
+        av[s] = (live && n < n_max) ? fourpi * rad[(size_t)p * n_max + n] : 0.0;
+        bv[s] = (live && col < 2 * k_max) ? angd[(size_t)p * 2 * k_max + col] : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv[s], acc, 0, 0, 0);
     }
-    if (k == 0 && central) {
-      for (int m = 0; m < species_mult[i]; ++m) {
-        const int s = site_species[(size_t)i * max_mult + m] - 1;
-        re = re + central[(size_t)s * n_max + n];
+    // D(row, col) sits in lane col + 16 (row % 4), element row / 4
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int nn = 16 * a + g + 4 * v;
+      if (nn < n_max && col < 2 * k_max) {
+        double val = acc[v];
+        if (col == 0 && central)
+          for (int m = 0; m < species_mult[i]; ++m) {
+            const int sp = site_species[(size_t)i * max_mult + m] - 1;
+            val = val + central[(size_t)sp * n_max + nn];
+          }
+        cnkd[((size_t)i * n_max + nn) * 2 * k_max + col] = val;
       }
     }
-    cnk[((size_t)i * n_max + n) * k_max + k] = make_double2(re, im);
   }
 }
 
@@ -506,7 +528,6 @@ __global__ void normalise_k(int n_sites, int n_soap, const double* __restrict__ 
 // and kind share one address register and differ by immediate offsets; rows
 // are padded by one double to spread adjacent (n, l) over the banks. The
 // angular rows are [k][pair][kind] (double2), padded likewise.
-typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // tuning: pairs per batch and waves per SIMD for up to 3 outputs per thread,
 // and the largest n_soap run with 256-thread blocks
@@ -874,17 +895,33 @@ __global__ __launch_bounds__(256) void contract_cart_k(int n_sites, int n_pairs,
   for (int s = threadIdx.x; s < n_soap; s += blockDim.x) gs[s] = G[(size_t)s * n_sites + i];
   __syncthreads();
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  const int len = 3 * n_soap;  // the pair's row of soap_cart_der, read as one contiguous stream
+  // lane l reads elements e = l + 64 j; as 64 = 1 (mod 3), element e is component (l + j) % 3 of
+  // output e / 3, so with j taken three at a time acc[u] collects component (l + u) % 3
+  int s_u[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) s_u[u] = (lane + 64 * u) / 3;
   for (int p = p0 + w; p < p1; p += 4) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    const double* C = cart + (size_t)p * n_soap * 3;
-    for (int s = lane; s < n_soap; s += 64) {
-      const double g = gs[s];
-      a0 += g * C[3 * s]; a1 += g * C[3 * s + 1]; a2 += g * C[3 * s + 2];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double* C = cart + (size_t)p * len;
+    // 30 loads per lane in flight before they are used (a 621-output row is one chunk)
+    for (int e0 = lane, sb = 0; e0 < len; e0 += 1920, sb += 640) {
+      double x[30];
+#pragma unroll
+      for (int j = 0; j < 30; ++j) x[j] = e0 + 64 * j < len ? C[e0 + 64 * j] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 30; ++j)
+        if (e0 + 64 * j < len) acc[j % 3] += gs[sb + 64 * (j / 3) + s_u[j % 3]] * x[j];
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      a0 += __shfl_xor(a0, o, 64); a1 += __shfl_xor(a1, o, 64); a2 += __shfl_xor(a2, o, 64);
-    }
-    if (lane == 0) { pair[3 * (size_t)p] = a0; pair[3 * (size_t)p + 1] = a1; pair[3 * (size_t)p + 2] = a2; }
+    // back to component order: component c is acc[(c - l) mod 3]
+    const int r = lane % 3;
+    const double c0 = r == 0 ? acc[0] : r == 1 ? acc[2] : acc[1];
+    const double c1 = r == 0 ? acc[1] : r == 1 ? acc[0] : acc[2];
+    const double c2 = r == 0 ? acc[2] : r == 1 ? acc[1] : acc[0];
+    acc[0] = c0; acc[1] = c1; acc[2] = c2;
+    const double t = wave_multi_sum<4>(acc, lane);
+    const int q = wave_multi_index<4>(lane);
+    if ((lane & 15) == 0 && q < 3) pair[3 * (size_t)p + q] = t;
   }
 }
 
