@@ -122,3 +122,45 @@ def test_adjoint_f32_central_differences(ctx):
             worst = max(worst, abs(fd - f32[c, a]) / scale)
     print(f"central differences: max rel err {worst:.2e}")
     assert worst <= 1e-5
+
+
+# every site-kernel shape class: n_max tiles 1-3, l_max 4-12, 1-5 species, the compression paths
+# (none, one weighted component per output, several per output), and sites with more than one tile
+# of 64 neighbours
+@pytest.mark.parametrize("alpha_max,l_max,kind,rcut", [
+    ([3, 3], 4, None, 4.5),
+    ([5, 5], 6, "scaled", 4.5),
+    ([8, 8, 8], 8, "multi", 4.5),
+    ([11, 11, 11], 4, None, 4.5),       # n_max 33: three row tiles
+    ([4], 10, None, 4.5),
+    ([3, 2], 12, None, 4.5),
+    ([2, 2, 2, 2, 2], 5, None, 4.5),
+    ([4], 4, None, 6.5),                # about 100 neighbours per site: two tiles
+])
+def test_adjoint_f32_shapes_match_fp64_oracle(ctx, alpha_max, l_max, kind, rcut):
+    from tests.golden_cases import compression_indices
+    ns = len(alpha_max)
+    spec = BoxSpec(1000, 0.09, 1.0, [1.0 / ns] * ns, [1, 6, 7, 8, 14][:ns], seed=31 + ns)
+    par = SoapParams(alpha_max=alpha_max, l_max=l_max, rcut_hard=rcut, rcut_soft=rcut - 0.5, basis="poly3gauss",
+                     radial_enhancement=1)
+    w = make_workload((spec, par), n_sites=24)
+    keep, prm, cmp_, _, n_soap, W, S, G = setup(ctx, w)
+    n_max = sum(alpha_max)
+    comp = compression_indices(kind, n_max * (n_max + 1) // 2 * (l_max + 1), 7 + n_max)
+    if comp is not None:
+        cmp_ = st.make_compression(keep, True, len(comp[0]), *comp)
+        n_soap = int(comp[0].max())
+        G = normal(5, (n_soap, w.n_sites))
+        ctx.upload_dedq(np.asfortranarray(G.T))
+    ctx.adjoint_f32(prm, cmp_)
+    q32 = ctx.download_soap_f32(n_soap).T
+    pair32 = ctx.download_pairs(1).T
+    f32, v32 = ctx.download_forces(1)
+    qo, pair_o = orc.contract(w, W, S, G, 1.0, comp, True)
+    fo = np.zeros_like(w.pos)
+    np.add.at(fo, w.nbr, -pair_o)
+    vo = -np.einsum("ka,kb->ab", w.vec, pair_o)
+    eq, ep, ef, ev = rel(q32, qo), rel(pair32, pair_o), rel(f32.T, fo), rel(v32, vo)
+    print(f"{alpha_max} l {l_max} {kind} rcut {rcut} (max {w.n_neigh.max()} neighbours): "
+          f"rel err q {eq:.2e} pair {ep:.2e} forces {ef:.2e} virial {ev:.2e}")
+    assert eq <= TOL and ef <= TOL and ev <= TOL and ep <= 10 * TOL
