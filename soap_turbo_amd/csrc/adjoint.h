@@ -29,7 +29,7 @@ struct AdjointState {
   bool order_ok = false;                 // order holds the sites of the current neighbour list
   unsigned long long ticket_base = 0;    // the site queue's counter at the start of the next launch
   int g32_sites = 0, g32_soap = 0;  // shape of the site-major FP32 dE/dq in G32 (0: none)
-  AdjBuf el32;
+  AdjBuf el32, ytab;
   std::string atoms_key, el_key;
   int n_pf = 0;
   int n_pairs = 0, n_atoms = 0, n_sites = 0, n_soap = 0;
@@ -62,6 +62,7 @@ struct AdjointInputs {
   const int* umap;
   const std::vector<double>* oel_h;
   const std::vector<double>* uel_h;
+  const std::vector<int32_t>* umap_h;
   std::vector<RadialConsts> radial_tab;  // per species, block offset 0, n_max = alpha_max, with derivatives
   const int32_t* pair_atom;
   const int* atom_ptr;                   // pairs of each atom (CSR over pair_atom)

@@ -290,7 +290,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     const int* __restrict__ out_ptr, const int* __restrict__ out_c, const float* __restrict__ out_el,
     const int* __restrict__ unc_ptr, const int* __restrict__ unc_e, const int32_t* __restrict__ P_i,
     const float* __restrict__ P_el, const int4* __restrict__ ocomp, const float* __restrict__ oel,
-    const int* __restrict__ umap, const float* __restrict__ uel, float* __restrict__ q_out,
+    const int* __restrict__ ymap, const float* __restrict__ yw, float* __restrict__ q_out,
     float* __restrict__ sdp_out, float* __restrict__ pair_out, unsigned long long* __restrict__ tickets,
     unsigned long long ticket_base) {
   using Sh = SiteShape<L>;
@@ -694,56 +694,74 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   }
   if (tx == 0) sdp_out[i] = sdp;
   __syncthreads();
-  // dE/dp per uncompressed component
-  for (int c0 = tx; c0 < P.n_soap_unc; c0 += 4 * NTHREADS) {
-    float gu[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int c = c0 + NTHREADS * u;
-      gu[u] = 0.f;
-      if (c < P.n_soap_unc) {
-        float v;
-        if (umap) {
-          const int q = umap[c];
-          v = q >= 0 ? uel[c] * Gp[q] : 0.f;
-        } else if (P.compressed) {
-          v = 0.f;
-          for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v += P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
-        } else {
-          v = Gp[c];
-        }
-        gu[u] = v;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int c = c0 + NTHREADS * u;
-      if (c < P.n_soap_unc) Gu[c] = gu[u];
-    }
-  }
-  __syncthreads();
-  // Y_l(n, n') = Gu(c) x 2 on the diagonal, sqrt 2 off it, over S_l; one (l, n) row per thread,
-  // eight columns at a time (their loads ahead of their stores); c(a, b, l) for a <= b is the
-  // uncompressed component (a n_max - a(a-1)/2 + b - a) L1 + l
   float* Y = Sl;
-  for (int row = tx; row < L1 * n_max; row += NTHREADS) {
-    const int l = row / n_max, n = row - l * n_max;
-    float* yr = Y + (size_t)row * n_max;
-    for (int np0 = 0; np0 < n_max; np0 += 8) {
-      float y[8];
+  if (ymap) {
+    // Y_l(n, n') straight from dE/dq through the direct maps: one output q and weight per entry
+    const int ny = L1 * n_max * n_max;
+    for (int t0 = tx; t0 < ny; t0 += 4 * NTHREADS) {
+      int q[4];
+      float wq[4], y[4];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int np = np0 + j;
-        const int lo_ = n < np ? n : np, hi_ = n < np ? np : n;
-        const int c = (((lo_ * (2 * n_max - lo_ - 1)) >> 1) + hi_) * L1 + l;  // (a n_max - a(a+1)/2 + b) L1 + l
-        y[j] = np < n_max ? (lo_ == hi_ ? 2.f : 1.41421356f) * Gu[c] : 0.f;
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + NTHREADS * u;
+        q[u] = t < ny ? ymap[t] : -1;
+        wq[u] = t < ny ? yw[t] : 0.f;
       }
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (np0 + j < n_max) yr[np0 + j] = y[j];
+      for (int u = 0; u < 4; ++u) y[u] = q[u] >= 0 ? wq[u] * Gp[q[u]] : 0.f;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (t0 + NTHREADS * u < ny) Y[t0 + NTHREADS * u] = y[u];
     }
+    __syncthreads();
+  } else {
+    // general compression: dE/dp per uncompressed component, then Y_l from it
+    for (int c0 = tx; c0 < P.n_soap_unc; c0 += 4 * NTHREADS) {
+      float gu[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = c0 + NTHREADS * u;
+        gu[u] = 0.f;
+        if (c < P.n_soap_unc) {
+          float v;
+          if (P.compressed) {
+            v = 0.f;
+            for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v += P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
+          } else {
+            v = Gp[c];
+          }
+          gu[u] = v;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = c0 + NTHREADS * u;
+        if (c < P.n_soap_unc) Gu[c] = gu[u];
+      }
+    }
+    __syncthreads();
+    // Y_l(n, n') = Gu(c) x 2 on the diagonal, sqrt 2 off it, over S_l; one (l, n) row per thread,
+    // eight columns at a time (their loads ahead of their stores); c(a, b, l) for a <= b is the
+    // uncompressed component (a n_max - a(a-1)/2 + b - a) L1 + l
+    for (int row = tx; row < L1 * n_max; row += NTHREADS) {
+      const int l = row / n_max, n = row - l * n_max;
+      float* yr = Y + (size_t)row * n_max;
+      for (int np0 = 0; np0 < n_max; np0 += 8) {
+        float y[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int np = np0 + j;
+          const int lo_ = n < np ? n : np, hi_ = n < np ? np : n;
+          const int c = (((lo_ * (2 * n_max - lo_ - 1)) >> 1) + hi_) * L1 + l;  // (a n_max - a(a+1)/2 + b) L1 + l
+          y[j] = np < n_max ? (lo_ == hi_ ? 2.f : 1.41421356f) * Gu[c] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (np0 + j < n_max) yr[np0 + j] = y[j];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // H_l = 4 pi w(m) Y_l C_l over the columns of block l. A job is one column tile of one
   // block and every row tile; each wavefront holds its jobs' C operands in registers across a
   // barrier, after which H is written over C: real parts of k in columns k, imaginary
@@ -1007,8 +1025,9 @@ void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order
                  const int32_t* species, int max_mult, const int32_t* mult, const uint32_t* spmask, const double* rjs,
                  const double* th, const double* ph, const float* G32, const int4* comp, const int* out_ptr,
                  const int* out_c, const float* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
-                 const float* P_el, const int4* ocomp, const float* oel, const int* umap, const float* uel, float* q,
-                 float* sdp, float* pair, unsigned long long* tickets, unsigned long long& ticket_base, int n_cu) {
+                 const float* P_el, const int4* ocomp, const float* oel, const int* ymap, const float* yw, float* q,
+                 float* sdp, float* pair, unsigned long long* tickets,
+                 unsigned long long& ticket_base, int n_cu) {
   const void* fn = reinterpret_cast<const void*>(&adjoint_site_k<L, NTT>);
   if (sh > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   int per_cu = 0;
@@ -1016,7 +1035,7 @@ void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order
   const int grid = std::max(1, std::min(P.n_sites, per_cu * n_cu));
   adjoint_site_k<L, NTT><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, central, species,
                                                      max_mult, mult, spmask, rjs, th, ph, G32, comp, out_ptr, out_c,
-                                                     out_el, unc_ptr, unc_e, P_i, P_el, ocomp, oel, umap, uel, q, sdp,
+                                                     out_el, unc_ptr, unc_e, P_i, P_el, ocomp, oel, ymap, yw, q, sdp,
                                                      pair, tickets, ticket_base);
   // every workgroup takes one ticket past the last site
   ticket_base += (unsigned long long)P.n_sites + (unsigned long long)grid;
@@ -1274,6 +1293,27 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     }
     if ((rc = ensure(s.el32, (all.size() + 1) * 4, msg))) return rc;
     if (!all.empty()) (void)hipMemcpyAsync(s.el32.p, all.data(), all.size() * 4, hipMemcpyHostToDevice, st);
+    // with direct maps, Y_l(n, n') = w * dE/dq(q) for one output q (or none) per (l, n, n'):
+    // q and w = (2 on the diagonal, sqrt 2 off it) x P_el, in Y's own layout
+    std::vector<int32_t> ymap;
+    std::vector<float> yw;
+    if (in.ocomp) {
+      const int nm = in.n_max, L1 = in.l_max + 1;
+      ymap.resize((size_t)L1 * nm * nm);
+      yw.resize(ymap.size());
+      for (int l = 0; l < L1; ++l)
+        for (int n = 0; n < nm; ++n)
+          for (int np = 0; np < nm; ++np) {
+            const int a = std::min(n, np), b = std::max(n, np);
+            const int c = (a * nm - a * (a - 1) / 2 + (b - a)) * L1 + l;
+            const size_t t = ((size_t)l * nm + n) * nm + np;
+            ymap[t] = (*in.umap_h)[c];
+            yw[t] = (float)((a == b ? 2.0 : 1.4142135623730951) * (*in.uel_h)[c]);
+          }
+      if ((rc = ensure(s.ytab, ymap.size() * 8, msg))) return rc;
+      (void)hipMemcpyAsync(s.ytab.p, ymap.data(), ymap.size() * 4, hipMemcpyHostToDevice, st);
+      (void)hipMemcpyAsync(s.ytab.as<int32_t>() + ymap.size(), yw.data(), yw.size() * 4, hipMemcpyHostToDevice, st);
+    }
     (void)hipStreamSynchronize(st);
     s.el_key = *in.tables_key;
   }
@@ -1289,8 +1329,11 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   const float* oel = in.out_ptr ? s.el32.as<float>() : nullptr;
   const float* pel = in.out_ptr ? s.el32.as<float>() + n_oe : nullptr;
   const float* oel1 = in.ocomp ? s.el32.as<float>() + n_oe + n_pe : nullptr;
-  const float* uel1 = in.ocomp ? oel1 + n_o1 : nullptr;
+  (void)n_o1;
   (void)n_u1;
+  const size_t ny = (size_t)(in.l_max + 1) * in.n_max * in.n_max;
+  const int* ymap = in.ocomp ? s.ytab.as<int>() : nullptr;
+  const float* yw = in.ocomp ? reinterpret_cast<const float*>(s.ytab.as<int>() + ny) : nullptr;
   // sites by falling neighbour count (longest first from the queue), once per neighbour list
   if (!s.order_ok) {
     std::vector<int> ord(in.n_sites);
@@ -1326,7 +1369,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     launch_site<l, t>(st, P, sh, s.order.as<int>(), in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cen,  \
                       in.species, in.max_mult, in.mult, in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(),  \
                       in.comp, in.out_ptr, in.out_c, oel, in.unc_ptr, in.unc_e, in.P_i, pel, in.ocomp, oel1,       \
-                      in.ocomp ? in.umap : nullptr, uel1, s.soap.as<float>(), s.sdp.as<float>(),                   \
+                      ymap, yw, s.soap.as<float>(), s.sdp.as<float>(),                                             \
                       s.pair.as<float>(), s.tickets.as<unsigned long long>(), s.ticket_base, in.n_cu);            \
     break;
 #define LS(l) LS1(l, 1) LS1(l, 2) LS1(l, 3)

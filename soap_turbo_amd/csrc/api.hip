@@ -68,6 +68,7 @@ struct st_ctx {
   DevBuf ocomp, oel, umap, uel;
   bool single = false;
   std::vector<double> oel_h, uel_h;
+  std::vector<int32_t> umap_h;
   std::string tables_key;
   std::vector<double> out_el_h, P_el_h;
   int n_max = 0, k_max = 0, l_max = -1, n_soap = 0, n_soap_unc = 0;
@@ -433,6 +434,7 @@ int build_tables(st_ctx* ctx, const st_params* p, const st_compression* c) {
         if ((rc = upload(ctx, ctx->uel, uel.data(), uel.size() * 8))) return rc;
         ctx->oel_h = oel;
         ctx->uel_h = uel;
+        ctx->umap_h = um;
       }
     }
     std::vector<double> pf;
@@ -897,6 +899,7 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   in.umap = ctx->umap.as<int>();
   in.oel_h = &ctx->oel_h;
   in.uel_h = &ctx->uel_h;
+  in.umap_h = &ctx->umap_h;
   in.tables_key = &ctx->tables_key;
   in.lds_max = ctx->lds_max;
   in.n_cu = ctx->n_cu;
