@@ -598,10 +598,12 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   }
   __syncthreads();
   // --------------------------------------------------------------- spectrum
-  // S_l(n, n') = sum over the l block of w(m) C(n, c) C(n', c), w = 1 (m = 0), 2 (m > 0);
-  // tiles with a <= b (the outputs read n <= n'); a wavefront's jobs are unrolled so their
-  // loads are in flight together
-  {
+  // S_l(n, n') = sum over the l block of w(m) C(n, c) C(n', c), w = 1 (m = 0), 2 (m > 0).
+  // With a direct output map and at most 3 outputs per thread, each output's entry is summed
+  // where it is used, below. Otherwise S_l is formed on MFMA: tiles with a <= b (the outputs
+  // read n <= n'), a wavefront's jobs unrolled so that their loads are in flight together.
+  const bool spec_direct = ocomp != nullptr && P.n_soap <= 3 * NTHREADS;
+  if (!spec_direct) {
     constexpr int NAB = NTT * (NTT + 1) / 2, NJ = L1 * NAB, JPW = (NJ + 3) / 4;
     constexpr int MS = (2 * L1 + 3) / 4;
     float av[JPW][MS], bv[JPW][MS];
@@ -655,7 +657,15 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       vq[u] = gq[u] = 0.f;
       if (s < n_soap) {
         float v = 0.f;
-        if (ocomp) {
+        if (spec_direct) {
+          const int4 cm = ocomp[s];
+          const int cb = cm.z * (cm.z + 1), cw = 2 * (cm.z + 1);
+          const float* ca = C + cm.x * CS + cb;
+          const float* cc = C + cm.y * CS + cb;
+          float acc = ca[0] * cc[0] + ca[1] * cc[1];
+          for (int c = 2; c < cw; ++c) acc += ca[c] * (2.f * cc[c]);
+          v = oel[s] * (cm.x != cm.y ? 1.41421356f : 1.f) * acc;
+        } else if (ocomp) {
           const int4 cm = ocomp[s];
           v = oel[s] * (cm.x != cm.y ? 1.41421356f : 1.f) * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
         } else {
