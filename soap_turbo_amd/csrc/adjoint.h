@@ -29,7 +29,10 @@ struct AdjointState {
   bool order_ok = false;                 // order holds the sites of the current neighbour list
   unsigned long long ticket_base = 0;    // the site queue's counter at the start of the next launch
   int g32_sites = 0, g32_soap = 0;  // shape of the site-major FP32 dE/dq in G32 (0: none)
-  AdjBuf el32, ytab;
+  AdjBuf el32, odir;
+  AdjBuf cenv;                  // per-site central-atom term (poly3gauss)
+  bool cen_ok = false;          // cenv holds the sites of the current neighbour list ...
+  std::string cen_key;          // ... for the tables of this key
   std::string atoms_key, el_key;
   int n_pf = 0;
   int n_pairs = 0, n_atoms = 0, n_sites = 0, n_soap = 0;

@@ -149,11 +149,12 @@ void adjoint_invalidate(AdjointState& s) {
   s.have_forces = false;
   s.g32_sites = s.g32_soap = 0;
   s.order_ok = false;
+  s.cen_ok = false;
 }
 
 void adjoint_release(AdjointState& s) {
   AdjBuf* bufs[] = {&s.rtab, &s.atab, &s.tabinfo, &s.soap, &s.sdp, &s.pair, &s.forces,
-                    &s.virial_part, &s.virial, &s.G32, &s.el32, &s.order, &s.tickets, &s.ytab};
+                    &s.virial_part, &s.virial, &s.G32, &s.el32, &s.order, &s.tickets, &s.odir, &s.cenv};
   for (AdjBuf* b : bufs)
     if (b->p) {
       (void)hipFree(b->p);

@@ -283,14 +283,12 @@ struct SiteShape {
 template <int L, int NTT>
 __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_site_k(
     AdjParams P, const int* __restrict__ order, const int* __restrict__ site_ptr, const float* __restrict__ rtab,
-    const float* __restrict__ atab, const float* __restrict__ preflm_g, const float* __restrict__ central,
-    const int32_t* __restrict__ site_species, int max_mult, const int32_t* __restrict__ species_mult,
+    const float* __restrict__ atab, const float* __restrict__ preflm_g, const float* __restrict__ cenv,
     const uint32_t* __restrict__ spmask, const double* __restrict__ rjs, const double* __restrict__ thetas,
     const double* __restrict__ phis, const float* __restrict__ G32, const int4* __restrict__ comp,
     const int* __restrict__ out_ptr, const int* __restrict__ out_c, const float* __restrict__ out_el,
     const int* __restrict__ unc_ptr, const int* __restrict__ unc_e, const int32_t* __restrict__ P_i,
-    const float* __restrict__ P_el, const int4* __restrict__ ocomp, const float* __restrict__ oel,
-    const int* __restrict__ ymap, const float* __restrict__ yw, float* __restrict__ q_out,
+    const float* __restrict__ P_el, const int2* __restrict__ odir, float* __restrict__ q_out,
     float* __restrict__ sdp_out, float* __restrict__ pair_out, unsigned long long* __restrict__ tickets,
     unsigned long long ticket_base) {
   using Sh = SiteShape<L>;
@@ -314,6 +312,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   __shared__ int sp_off[ADJ_MAX_SPECIES], sp_a[ADJ_MAX_SPECIES], sp_rt[ADJ_MAX_SPECIES];
   __shared__ float sp_cw[ADJ_MAX_SPECIES];
   __shared__ unsigned char nsp[64];                 // species block of each n
+  __shared__ float cen_s[64];                       // the site's central-atom term per n (poly3gauss)
   __shared__ int site_next;
   const int n_max = P.n_max, n_species = P.n_species;
   int i = 0, p0 = 0, p1 = 0;  // the current site and its pairs
@@ -479,14 +478,15 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       for (int q = 0; q < P.sp[s].a; ++q) nsp[P.sp[s].off + q] = (unsigned char)s;
     }
   }
-  // Sites are taken from a queue (tickets counted from ticket_base) in the order given, heaviest
-  // first, until it runs dry: the grid is one resident wave of workgroups.
+  // Sites in the order given (heaviest first): workgroup b starts on site b, and each workgroup
+  // takes its next site from a queue (tickets counted from ticket_base, after the grid's first
+  // sites) while it works on the current one, until the queue runs dry. The grid is one resident
+  // wave of workgroups.
+  if (threadIdx.x == 0)
+    site_next = (int)blockIdx.x < P.n_sites ? (order ? order[blockIdx.x] : (int)blockIdx.x) : -1;
+  unsigned long long tk = 0;  // thread 0: the ticket for the next site
+  int nxt = -1;               // thread 0: the next site (-1: none)
   for (;;) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned long long t = atomicAdd(tickets, 1ull) - ticket_base;
-    site_next = t < (unsigned long long)P.n_sites ? (order ? order[t] : (int)t) : -1;
-  }
   __syncthreads();
   i = site_next;
   if (i < 0) break;
@@ -497,6 +497,10 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   p1 = site_ptr[i + 1];
   one_tile = NTT <= 2 && p1 - p0 <= PT;
   // --------------------------------------------------------------- forward
+  // With a direct output map (every output one uncompressed component, every component at
+  // most one output) and at most 3 outputs per thread, each thread's outputs are computed from
+  // C where they are used, without S_l.
+  const bool spec_direct = odir != nullptr && P.n_soap <= 3 * NTHREADS;
   floatx4 hi[TPW], lo[TPW];
 #pragma unroll
   for (int j = 0; j < TPW; ++j) {
@@ -507,6 +511,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     const int np_ = min(PT, p1 - tb);
     __syncthreads();
     stage(tb, true);
+    // the site's central-atom term, loaded by wavefront 1 while wavefront 0 stages the pairs
+    if (cenv && tb == p0 && tx >= 64 && tx < 64 + n_max) cen_s[tx - 64] = cenv[(size_t)i * n_max + tx - 64];
     __syncthreads();
     // the thread index made opaque per tile: item addresses are formed here, not hoisted
     int tid = tx;
@@ -588,9 +594,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         const int n = 16 * a + 4 * lk + v;
         if (n < n_max) {
           float val = hi[j][v] + lo[j][v];
-          if (P.gauss && col == 0)
-            for (int m = 0; m < species_mult[i]; ++m)
-              val += central[(size_t)(site_species[(size_t)i * max_mult + m] - 1) * n_max + n];
+          if (cenv && col == 0) val += cen_s[n];
           C[n * CS + col] = val;
         }
       }
@@ -602,7 +606,6 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   // With a direct output map and at most 3 outputs per thread, each output's entry is summed
   // where it is used, below. Otherwise S_l is formed on MFMA: tiles with a <= b (the outputs
   // read n <= n'), a wavefront's jobs unrolled so that their loads are in flight together.
-  const bool spec_direct = ocomp != nullptr && P.n_soap <= 3 * NTHREADS;
   if (!spec_direct) {
     constexpr int NAB = NTT * (NTT + 1) / 2, NJ = L1 * NAB, JPW = (NJ + 3) / 4;
     constexpr int MS = (2 * L1 + 3) / 4;
@@ -646,132 +649,173 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     }
   }
   __syncthreads();
-  // q, |p|, and Gp = (G - q (q.G)) / |p| from one pass: q.G = (p.G) / |p|
+  // q, |p|, and Gp = (G - q (q.G)) / |p| from one pass: q.G = (p.G) / |p|. With a direct
+  // output map, Y_l is then filled from the outputs: zero, and for output s of component
+  // (n, n', l) with weight w, Y_l(n, n') = Y_l(n', n) = w Gp(s) (2 w on the diagonal).
   const int n_soap = P.n_soap;
-  float sums[2] = {0.f, 0.f};
-  for (int s0 = tx; s0 < n_soap; s0 += 4 * NTHREADS) {
-    float vq[4], gq[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int s = s0 + NTHREADS * u;
-      vq[u] = gq[u] = 0.f;
-      if (s < n_soap) {
-        float v = 0.f;
-        if (spec_direct) {
-          const int4 cm = ocomp[s];
-          const int cb = cm.z * (cm.z + 1), cw = 2 * (cm.z + 1);
-          const float* ca = C + cm.x * CS + cb;
-          const float* cc = C + cm.y * CS + cb;
-          float acc = ca[0] * cc[0] + ca[1] * cc[1];
-          for (int c = 2; c < cw; ++c) acc += ca[c] * (2.f * cc[c]);
-          v = oel[s] * (cm.x != cm.y ? 1.41421356f : 1.f) * acc;
-        } else if (ocomp) {
-          const int4 cm = ocomp[s];
-          v = oel[s] * (cm.x != cm.y ? 1.41421356f : 1.f) * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
-        } else {
-          const int e0 = P.compressed ? out_ptr[s] : s, e1 = P.compressed ? out_ptr[s + 1] : s + 1;
-          for (int e = e0; e < e1; ++e) {
-            const int4 cm = comp[P.compressed ? out_c[e] : e];
-            const float base = cm.x != cm.y ? 1.41421356f : 1.f;
-            v += (P.compressed ? out_el[e] : 1.f) * base * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
-          }
-        }
-        const float g = G32[(size_t)i * n_soap + s];
-        vq[u] = v;
-        gq[u] = g;
-        sums[0] += v * v;
-        sums[1] += v * g;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int s = s0 + NTHREADS * u;
-      if (s < n_soap) {
-        qv[s] = vq[u];
-        Gp[s] = gq[u];
-      }
-    }
-  }
-  block_sums_f<2>(sums, red);
-  float sdp = sqrtf(sums[0]);
-  if (sdp < 1e-5f) sdp = 1.f;
-  const float inv = 1.f / sdp;
-  const float qg = sums[1] * inv;
-  for (int s = tx; s < n_soap; s += NTHREADS) {
-    const float q = qv[s] * inv;
-    q_out[(size_t)i * n_soap + s] = q;
-    Gp[s] = (Gp[s] - q * qg) * inv;
-  }
-  if (tx == 0) sdp_out[i] = sdp;
-  __syncthreads();
+  const int ny = L1 * n_max * n_max;
   float* Y = Sl;
-  if (ymap) {
-    // Y_l(n, n') straight from dE/dq through the direct maps: one output q and weight per entry
-    const int ny = L1 * n_max * n_max;
-    for (int t0 = tx; t0 < ny; t0 += 4 * NTHREADS) {
-      int q[4];
-      float wq[4], y[4];
+  float sums[2] = {0.f, 0.f};
+  if (spec_direct) {
+    int2 od[3];
+    float gd[3], vq[3];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int t = t0 + NTHREADS * u;
-        q[u] = t < ny ? ymap[t] : -1;
-        wq[u] = t < ny ? yw[t] : 0.f;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) y[u] = q[u] >= 0 ? wq[u] * Gp[q[u]] : 0.f;
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (t0 + NTHREADS * u < ny) Y[t0 + NTHREADS * u] = y[u];
+    for (int u = 0; u < 3; ++u) {
+      const int s = tx + NTHREADS * u;
+      od[u] = s < n_soap ? odir[s] : make_int2(0, 0);
+      gd[u] = s < n_soap ? G32[(size_t)i * n_soap + s] : 0.f;
     }
+    for (int t = tx; t < ny; t += NTHREADS) Y[t] = 0.f;
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int s = tx + NTHREADS * u;
+      vq[u] = 0.f;
+      if (s < n_soap) {
+        const int x = od[u].x & 255, y = (od[u].x >> 8) & 255, l = od[u].x >> 16;
+        const int cb = l * (l + 1), cw = 2 * (l + 1);
+        const float* ca = C + x * CS + cb;
+        const float* cc = C + y * CS + cb;
+        float acc = ca[0] * cc[0] + ca[1] * cc[1];
+        for (int c = 2; c < cw; ++c) acc += ca[c] * (2.f * cc[c]);
+        const float v = __int_as_float(od[u].y) * acc;
+        vq[u] = v;
+        sums[0] += v * v;
+        sums[1] += v * gd[u];
+      }
+    }
+    block_sums_f<2>(sums, red);
+    float sdp = sqrtf(sums[0]);
+    if (sdp < 1e-5f) sdp = 1.f;
+    const float inv = 1.f / sdp;
+    const float qg = sums[1] * inv;
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int s = tx + NTHREADS * u;
+      if (s < n_soap) {
+        const float q = vq[u] * inv;
+        q_out[(size_t)i * n_soap + s] = q;
+        const int x = od[u].x & 255, y = (od[u].x >> 8) & 255, l = od[u].x >> 16;
+        const float yv = (x == y ? 2.f : 1.f) * __int_as_float(od[u].y) * ((gd[u] - q * qg) * inv);
+        Y[(l * n_max + x) * n_max + y] = yv;
+        Y[(l * n_max + y) * n_max + x] = yv;
+      }
+    }
+    if (tx == 0) sdp_out[i] = sdp;
     __syncthreads();
   } else {
-    // general compression: dE/dp per uncompressed component, then Y_l from it
-    for (int c0 = tx; c0 < P.n_soap_unc; c0 += 4 * NTHREADS) {
-      float gu[4];
+    for (int s0 = tx; s0 < n_soap; s0 += 4 * NTHREADS) {
+      float vq[4], gq[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int c = c0 + NTHREADS * u;
-        gu[u] = 0.f;
-        if (c < P.n_soap_unc) {
-          float v;
-          if (P.compressed) {
-            v = 0.f;
-            for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v += P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
+        const int s = s0 + NTHREADS * u;
+        vq[u] = gq[u] = 0.f;
+        if (s < n_soap) {
+          float v = 0.f;
+          if (odir) {
+            const int2 o = odir[s];
+            const int x = o.x & 255, y = (o.x >> 8) & 255, l = o.x >> 16;
+            v = __int_as_float(o.y) * Sl[(l * n_max + x) * n_max + y];
           } else {
-            v = Gp[c];
+            const int e0 = P.compressed ? out_ptr[s] : s, e1 = P.compressed ? out_ptr[s + 1] : s + 1;
+            for (int e = e0; e < e1; ++e) {
+              const int4 cm = comp[P.compressed ? out_c[e] : e];
+              const float base = cm.x != cm.y ? 1.41421356f : 1.f;
+              v += (P.compressed ? out_el[e] : 1.f) * base * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
+            }
           }
-          gu[u] = v;
+          const float g = G32[(size_t)i * n_soap + s];
+          vq[u] = v;
+          gq[u] = g;
+          sums[0] += v * v;
+          sums[1] += v * g;
         }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int c = c0 + NTHREADS * u;
-        if (c < P.n_soap_unc) Gu[c] = gu[u];
+        const int s = s0 + NTHREADS * u;
+        if (s < n_soap) {
+          qv[s] = vq[u];
+          Gp[s] = gq[u];
+        }
       }
     }
+    block_sums_f<2>(sums, red);  // its barriers also end the reads of S_l
+    float sdp = sqrtf(sums[0]);
+    if (sdp < 1e-5f) sdp = 1.f;
+    const float inv = 1.f / sdp;
+    const float qg = sums[1] * inv;
+    if (odir) {
+      for (int t = tx; t < ny; t += NTHREADS) Y[t] = 0.f;
+      __syncthreads();
+    }
+    for (int s = tx; s < n_soap; s += NTHREADS) {
+      const float q = qv[s] * inv;
+      q_out[(size_t)i * n_soap + s] = q;
+      const float gp = (Gp[s] - q * qg) * inv;
+      if (odir) {
+        const int2 o = odir[s];
+        const int x = o.x & 255, y = (o.x >> 8) & 255, l = o.x >> 16;
+        const float yv = (x == y ? 2.f : 1.f) * __int_as_float(o.y) * gp;
+        Y[(l * n_max + x) * n_max + y] = yv;
+        Y[(l * n_max + y) * n_max + x] = yv;
+      } else {
+        Gp[s] = gp;
+      }
+    }
+    if (tx == 0) sdp_out[i] = sdp;
     __syncthreads();
-    // Y_l(n, n') = Gu(c) x 2 on the diagonal, sqrt 2 off it, over S_l; one (l, n) row per thread,
-    // eight columns at a time (their loads ahead of their stores); c(a, b, l) for a <= b is the
-    // uncompressed component (a n_max - a(a-1)/2 + b - a) L1 + l
-    for (int row = tx; row < L1 * n_max; row += NTHREADS) {
-      const int l = row / n_max, n = row - l * n_max;
-      float* yr = Y + (size_t)row * n_max;
-      for (int np0 = 0; np0 < n_max; np0 += 8) {
-        float y[8];
+    if (!odir) {
+      // general compression: dE/dp per uncompressed component, then Y_l from it
+      for (int c0 = tx; c0 < P.n_soap_unc; c0 += 4 * NTHREADS) {
+        float gu[4];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int np = np0 + j;
-          const int lo_ = n < np ? n : np, hi_ = n < np ? np : n;
-          const int c = (((lo_ * (2 * n_max - lo_ - 1)) >> 1) + hi_) * L1 + l;  // (a n_max - a(a+1)/2 + b) L1 + l
-          y[j] = np < n_max ? (lo_ == hi_ ? 2.f : 1.41421356f) * Gu[c] : 0.f;
+        for (int u = 0; u < 4; ++u) {
+          const int c = c0 + NTHREADS * u;
+          gu[u] = 0.f;
+          if (c < P.n_soap_unc) {
+            float v;
+            if (P.compressed) {
+              v = 0.f;
+              for (int e = unc_ptr[c]; e < unc_ptr[c + 1]; ++e) v += P_el[unc_e[e]] * Gp[P_i[unc_e[e]] - 1];
+            } else {
+              v = Gp[c];
+            }
+            gu[u] = v;
+          }
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-          if (np0 + j < n_max) yr[np0 + j] = y[j];
+        for (int u = 0; u < 4; ++u) {
+          const int c = c0 + NTHREADS * u;
+          if (c < P.n_soap_unc) Gu[c] = gu[u];
+        }
       }
+      __syncthreads();
+      // Y_l(n, n') = Gu(c) x 2 on the diagonal, sqrt 2 off it, over S_l; one (l, n) row per thread,
+      // eight columns at a time (their loads ahead of their stores); c(a, b, l) for a <= b is the
+      // uncompressed component (a n_max - a(a-1)/2 + b - a) L1 + l
+      for (int row = tx; row < L1 * n_max; row += NTHREADS) {
+        const int l = row / n_max, n = row - l * n_max;
+        float* yr = Y + (size_t)row * n_max;
+        for (int np0 = 0; np0 < n_max; np0 += 8) {
+          float y[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int np = np0 + j;
+            const int lo_ = n < np ? n : np, hi_ = n < np ? np : n;
+            const int c = (((lo_ * (2 * n_max - lo_ - 1)) >> 1) + hi_) * L1 + l;  // (a n_max - a(a+1)/2 + b) L1 + l
+            y[j] = np < n_max ? (lo_ == hi_ ? 2.f : 1.41421356f) * Gu[c] : 0.f;
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (np0 + j < n_max) yr[np0 + j] = y[j];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
+  // the ticket for the next site, taken here so that its latency and the site's lookup below
+  // pass while this site finishes
+  if (tx == 0) tk = atomicAdd(tickets, 1ull);
   // H_l = 4 pi w(m) Y_l C_l over the columns of block l. A job is one column tile of one
   // block and every row tile; each wavefront holds its jobs' C operands in registers across a
   // barrier, after which H is written over C: real parts of k in columns k, imaginary
@@ -840,6 +884,10 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     }
   }
   const float* H = C;
+  if (tx == 0) {
+    const unsigned long long t = tk - ticket_base + gridDim.x;
+    nxt = t < (unsigned long long)P.n_sites ? (order ? order[t] : (int)t) : -1;
+  }
   // --------------------------------------------------------------- backward
   float self[3] = {0.f, 0.f, 0.f};
   for (int tb = p0; tb < p1; tb += PT) {
@@ -993,8 +1041,24 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     pair_out[3 * (size_t)p0] = self[0];
     pair_out[3 * (size_t)p0 + 1] = self[1];
     pair_out[3 * (size_t)p0 + 2] = self[2];
+    site_next = nxt;
+    // lgkmcnt(0): the write must land before the barrier at the top of the queue loop, and the
+    // compiler's own wait for that barrier is missing on this back edge
+    __builtin_amdgcn_s_waitcnt(0xc07f);
   }
   }  // site queue
+}
+
+// the central-atom term of each site: the sum of its species' vectors (species 1-based)
+__global__ void site_central_k(int n_sites, int n_max, int max_mult, const int32_t* __restrict__ mult,
+                               const int32_t* __restrict__ species, const float* __restrict__ central,
+                               float* __restrict__ cenv) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n_sites * n_max) return;
+  const int i = (int)(t / n_max), n = (int)(t - (size_t)i * n_max);
+  float v = 0.f;
+  for (int m = 0; m < mult[i]; ++m) v += central[(size_t)(species[(size_t)i * max_mult + m] - 1) * n_max + n];
+  cenv[t] = v;
 }
 
 // G(n_sites, n_soap) FP64 (site fastest) -> site-major FP32
@@ -1031,24 +1095,21 @@ size_t site_lds(int n_species, int n_max, int l_max, int n_soap, int n_soap_unc)
 
 template <int L, int NTT>
 void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order, const int* site_ptr,
-                 const float* rtab, const float* atab, const float* preflm, const float* central,
-                 const int32_t* species, int max_mult, const int32_t* mult, const uint32_t* spmask, const double* rjs,
-                 const double* th, const double* ph, const float* G32, const int4* comp, const int* out_ptr,
-                 const int* out_c, const float* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
-                 const float* P_el, const int4* ocomp, const float* oel, const int* ymap, const float* yw, float* q,
-                 float* sdp, float* pair, unsigned long long* tickets,
-                 unsigned long long& ticket_base, int n_cu) {
+                 const float* rtab, const float* atab, const float* preflm, const float* cenv,
+                 const uint32_t* spmask, const double* rjs, const double* th, const double* ph, const float* G32,
+                 const int4* comp, const int* out_ptr, const int* out_c, const float* out_el, const int* unc_ptr,
+                 const int* unc_e, const int32_t* P_i, const float* P_el, const int2* odir, float* q, float* sdp,
+                 float* pair, unsigned long long* tickets, unsigned long long& ticket_base, int n_cu) {
   const void* fn = reinterpret_cast<const void*>(&adjoint_site_k<L, NTT>);
   if (sh > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTHREADS, sh) != hipSuccess || per_cu < 1) per_cu = 1;
   const int grid = std::max(1, std::min(P.n_sites, per_cu * n_cu));
-  adjoint_site_k<L, NTT><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, central, species,
-                                                     max_mult, mult, spmask, rjs, th, ph, G32, comp, out_ptr, out_c,
-                                                     out_el, unc_ptr, unc_e, P_i, P_el, ocomp, oel, ymap, yw, q, sdp,
-                                                     pair, tickets, ticket_base);
-  // every workgroup takes one ticket past the last site
-  ticket_base += (unsigned long long)P.n_sites + (unsigned long long)grid;
+  adjoint_site_k<L, NTT><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask, rjs, th,
+                                                     ph, G32, comp, out_ptr, out_c, out_el, unc_ptr, unc_e, P_i,
+                                                     P_el, odir, q, sdp, pair, tickets, ticket_base);
+  // one ticket per site processed (the grid's first sites are taken by workgroup index)
+  ticket_base += (unsigned long long)P.n_sites;
 }
 
 // host copy of the per-call constant tables that live next to the FP64 ones
@@ -1286,43 +1347,36 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     msg = "the FP32 path supports n_max <= 48";
     return ST_ERR_UNSUPPORTED;
   }
-  // compression element tables in FP32
-  // FP32 copies: compression elements by output and in P order, then the direct-map elements
-  size_t n_oe = 0, n_pe = 0;
-  if (in.out_ptr) { n_oe = in.out_el_h->size(); n_pe = in.P_el_h->size(); }
-  const size_t n_o1 = in.ocomp ? in.oel_h->size() : 0, n_u1 = in.ocomp ? in.uel_h->size() : 0;
+  // compression element tables in FP32: by output, then in P order
+  size_t n_oe = 0;
+  if (in.out_ptr) n_oe = in.out_el_h->size();
   if (s.el_key != *in.tables_key) {
     std::vector<float> all;
     if (in.out_ptr) {
       all.insert(all.end(), in.out_el_h->begin(), in.out_el_h->end());
       all.insert(all.end(), in.P_el_h->begin(), in.P_el_h->end());
     }
-    if (in.ocomp) {
-      all.insert(all.end(), in.oel_h->begin(), in.oel_h->end());
-      all.insert(all.end(), in.uel_h->begin(), in.uel_h->end());
-    }
     if ((rc = ensure(s.el32, (all.size() + 1) * 4, msg))) return rc;
     if (!all.empty()) (void)hipMemcpyAsync(s.el32.p, all.data(), all.size() * 4, hipMemcpyHostToDevice, st);
-    // with direct maps, Y_l(n, n') = w * dE/dq(q) for one output q (or none) per (l, n, n'):
-    // q and w = (2 on the diagonal, sqrt 2 off it) x P_el, in Y's own layout
-    std::vector<int32_t> ymap;
-    std::vector<float> yw;
+    // direct maps: output q of component (a, b, l), a <= b, packed as a | b << 8 | l << 16, with
+    // the weight of S_l(a, b) in q (its element, times sqrt 2 off the diagonal) as float bits;
+    // components follow the trivial order, c = (a n_max - a (a - 1) / 2 + b - a) (l_max + 1) + l
     if (in.ocomp) {
       const int nm = in.n_max, L1 = in.l_max + 1;
-      ymap.resize((size_t)L1 * nm * nm);
-      yw.resize(ymap.size());
-      for (int l = 0; l < L1; ++l)
-        for (int n = 0; n < nm; ++n)
-          for (int np = 0; np < nm; ++np) {
-            const int a = std::min(n, np), b = std::max(n, np);
+      std::vector<int2> od((size_t)in.n_soap, make_int2(0, 0));
+      for (int a = 0; a < nm; ++a)
+        for (int b = a; b < nm; ++b)
+          for (int l = 0; l < L1; ++l) {
             const int c = (a * nm - a * (a - 1) / 2 + (b - a)) * L1 + l;
-            const size_t t = ((size_t)l * nm + n) * nm + np;
-            ymap[t] = (*in.umap_h)[c];
-            yw[t] = (float)((a == b ? 2.0 : 1.4142135623730951) * (*in.uel_h)[c]);
+            const int q = (*in.umap_h)[c];
+            if (q < 0) continue;
+            const float w = (float)((a == b ? 1.0 : 1.4142135623730951) * (*in.uel_h)[c]);
+            int wb;
+            std::memcpy(&wb, &w, 4);
+            od[q] = make_int2(a | b << 8 | l << 16, wb);
           }
-      if ((rc = ensure(s.ytab, ymap.size() * 8, msg))) return rc;
-      (void)hipMemcpyAsync(s.ytab.p, ymap.data(), ymap.size() * 4, hipMemcpyHostToDevice, st);
-      (void)hipMemcpyAsync(s.ytab.as<int32_t>() + ymap.size(), yw.data(), yw.size() * 4, hipMemcpyHostToDevice, st);
+      if ((rc = ensure(s.odir, od.size() * sizeof(int2), msg))) return rc;
+      (void)hipMemcpyAsync(s.odir.p, od.data(), od.size() * sizeof(int2), hipMemcpyHostToDevice, st);
     }
     (void)hipStreamSynchronize(st);
     s.el_key = *in.tables_key;
@@ -1338,12 +1392,18 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   const float* cen = pf + s.n_pf;
   const float* oel = in.out_ptr ? s.el32.as<float>() : nullptr;
   const float* pel = in.out_ptr ? s.el32.as<float>() + n_oe : nullptr;
-  const float* oel1 = in.ocomp ? s.el32.as<float>() + n_oe + n_pe : nullptr;
-  (void)n_o1;
-  (void)n_u1;
-  const size_t ny = (size_t)(in.l_max + 1) * in.n_max * in.n_max;
-  const int* ymap = in.ocomp ? s.ytab.as<int>() : nullptr;
-  const float* yw = in.ocomp ? reinterpret_cast<const float*>(s.ytab.as<int>() + ny) : nullptr;
+  const int2* odir = in.ocomp ? s.odir.as<int2>() : nullptr;
+  // the central-atom term of each site (poly3gauss), once per neighbour list and table set
+  if (P.gauss && (!s.cen_ok || s.cen_key != s.table_key)) {
+    const size_t nc = (size_t)in.n_sites * in.n_max;
+    if ((rc = ensure(s.cenv, nc * 4, msg))) return rc;
+    if (nc > 0)
+      site_central_k<<<(unsigned)((nc + 255) / 256), 256, 0, st>>>(in.n_sites, in.n_max, in.max_mult, in.mult,
+                                                                 in.species, cen, s.cenv.as<float>());
+    s.cen_ok = true;
+    s.cen_key = s.table_key;
+  }
+  const float* cenv = P.gauss ? s.cenv.as<float>() : nullptr;
   // sites by falling neighbour count (longest first from the queue), once per neighbour list
   if (!s.order_ok) {
     std::vector<int> ord(in.n_sites);
@@ -1376,10 +1436,9 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     switch (in.l_max * 4 + ntt) {
 #define LS1(l, t)                                                                                                  \
   case l * 4 + t:                                                                                                  \
-    launch_site<l, t>(st, P, sh, s.order.as<int>(), in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cen,  \
-                      in.species, in.max_mult, in.mult, in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(),  \
-                      in.comp, in.out_ptr, in.out_c, oel, in.unc_ptr, in.unc_e, in.P_i, pel, in.ocomp, oel1,       \
-                      ymap, yw, s.soap.as<float>(), s.sdp.as<float>(),                                             \
+    launch_site<l, t>(st, P, sh, s.order.as<int>(), in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cenv, \
+                      in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(), in.comp, in.out_ptr, in.out_c, oel, \
+                      in.unc_ptr, in.unc_e, in.P_i, pel, odir, s.soap.as<float>(), s.sdp.as<float>(),              \
                       s.pair.as<float>(), s.tickets.as<unsigned long long>(), s.ticket_base, in.n_cu);            \
     break;
 #define LS(l) LS1(l, 1) LS1(l, 2) LS1(l, 3)
