@@ -10,6 +10,7 @@ next to this file so that they travel with the source tree.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -28,6 +29,9 @@ CXX_SOURCES = ["basis.cpp"]
 # packed-FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
+# The FP32 site kernel takes its next site's ticket with one lane and uses the result only at the
+# next phase; the atomic optimizer's wave-wide form of that atomic would wait for it at once.
+FILE_FLAGS = {"adjoint_f32.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]}
 
 
 def _run(cmd, cwd=None):
@@ -41,14 +45,16 @@ def _run(cmd, cwd=None):
 def build_hip(verbose=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(objdir, src + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
-        objs.append(obj)
+        jobs.append(([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src, []), "-c", path, "-o", obj], obj))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(jobs) or 1) as pool:
+        list(pool.map(lambda j: _run(j[0]), jobs))
+    objs = [obj for _, obj in jobs]
     for src in CXX_SOURCES:
         obj = os.path.join(objdir, src + ".o")
         _run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"), "-c", os.path.join(CSRC, src), "-o", obj])

@@ -39,6 +39,8 @@ constexpr int MAXSEG = 3;
 constexpr int ADJ_MAX_SPECIES = 8;
 constexpr int PT = 64;            // pairs per tile
 constexpr int NTHREADS = 256;
+// set-up items are (row, pair) = (wavefront + 4 u, lane): a tile's pairs are one wavefront's lanes
+static_assert(PT == 64 && NTHREADS == 256, "tile pairs map to the lanes of a wavefront");
 
 struct SpeciesTab {
   int a, off, do_central, nseg;
@@ -149,7 +151,7 @@ __device__ __forceinline__ int radial_cell(const SpeciesTab& s, double r, float&
 // P_{m-1}^{m-1}, P_{m+1}^m = (2m+1) x P_m^m, then the three-term recursion in
 // l. Calls f(l, P_l^m) for each l.
 template <int LT, class F>
-__device__ __forceinline__ void plm_column(int m, float x, float sq, F&& f) {
+__device__ __forceinline__ void plm_column(int m, float x, float sq, const float* ra, const float* rb, F&& f) {
   float pmm = 1.f;
   for (int j = 1; j <= m; ++j) pmm = -(float)(2 * j - 1) * sq * pmm;
   f(m, pmm);
@@ -157,7 +159,8 @@ __device__ __forceinline__ void plm_column(int m, float x, float sq, F&& f) {
   float p1 = x * (float)(2 * m + 1) * pmm, p0 = pmm;
   f(m + 1, p1);
   for (int l = m + 2; l <= LT; ++l) {
-    const float pn = ((float)(2 * l - 1) * x * p1 - (float)(l - 1 + m) * p0) / (float)(l - m);
+    const int k = l * (l + 1) / 2 + m;
+    const float pn = ra[k] * x * p1 - rb[k] * p0;
     p0 = p1;
     p1 = pn;
     f(l, pn);
@@ -305,6 +308,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   __shared__ float red[16];
   __shared__ float preflm[KM];
   __shared__ int klm[KM];
+  __shared__ float rec_a[KM1], rec_b[KM1];         // P_l^m = rec_a x P_{l-1}^m - rec_b P_{l-2}^m
   __shared__ uint32_t pr_msk[PT];                   // species bits (0: no contribution) | SELF
   __shared__ float pr_r[PT], pr_st[PT], pr_ct[PT], pr_sf[PT], pr_cf[PT];
   __shared__ int pr_acell[PT];                      // angular table cell (floats into atab)
@@ -467,6 +471,14 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     while ((l + 1) * (l + 2) / 2 <= t) ++l;
     klm[t] = l | ((t - l * (l + 1) / 2) << 8);
   }
+  // the three-term recursion's coefficients (2l - 1)/(l - m) and (l - 1 + m)/(l - m), l >= m + 2
+  for (int t = threadIdx.x; t < KM1; t += NTHREADS) {
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= t) ++l;
+    const int m = t - l * (l + 1) / 2;
+    rec_a[t] = l >= m + 2 ? (float)(2 * l - 1) / (float)(l - m) : 0.f;
+    rec_b[t] = l >= m + 2 ? (float)(l - 1 + m) / (float)(l - m) : 0.f;
+  }
   // uniform-index reads of the kernel arguments only: a per-lane index into them would
   // move them to scratch memory
   if (threadIdx.x == 0) {
@@ -525,7 +537,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     // (src/soap_turbo_angular.f90:108-160): P_m^m, P_{m+1}^m = (2m+1) x P_m^m, three-term in l.
 #pragma unroll
     for (int u = 0; u < GIT; ++u) {
-      const int t = tid + NTHREADS * u, m = t / PT, pl = t - m * PT;
+      const int pl = tid & (PT - 1), m = __builtin_amdgcn_readfirstlane(tid >> 6) + 4 * u;  // m uniform
       if (m < L1) {
         const bool live = pr_msk[pl] != 0;
         float gv[L1];
@@ -544,7 +556,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             float pc;
             if (l == m) pc = pmm;
             else if (l == m + 1) pc = x * (float)(2 * m + 1) * pmm;
-            else pc = ((float)(2 * l - 1) * x * p1 - (float)(l - 1 + m) * p0) / (float)(l - m);
+            else pc = rec_a[l * (l + 1) / 2 + m] * x * p1 - rec_b[l * (l + 1) / 2 + m] * p0;
             p0 = p1;
             p1 = pc;
             const int k = l * (l + 1) / 2 + m;
@@ -921,10 +933,10 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
 #pragma unroll
     for (int u = 0; u < PIT; ++u) {
-      const int t = tid + NTHREADS * u, m = t / PT, pl = t - m * PT;
+      const int pl = tid & (PT - 1), m = __builtin_amdgcn_readfirstlane(tid >> 6) + 4 * u;  // m uniform
       if (m <= L + 1) {
         if (pr_msk[pl] != 0 && tb + pl != p0) {
-          plm_column<L + 1>(m, pr_ct[pl], pr_st[pl],
+          plm_column<L + 1>(m, pr_ct[pl], pr_st[pl], rec_a, rec_b,
                             [&](int l, float plm) { Fp[(l * (l + 1) / 2 + m) * PTS + pl] = plm; });
           if (m <= L) Em[m * PTS + pl] = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
         } else {
