@@ -187,9 +187,11 @@ def main():
         name = "adj_site_f32"
         tot, nl = kt32[name]
         avg_ms = tot / max(nl, 1)
-        # algorithmic FP32 flops per site: per neighbour and (n, k), cnk 4 (real x complex FMA) and the two
-        # contractions u = R H, v = R' H 8; per site and (n, n', k), spectrum 4 and H = Y c 4
-        flops_site = n_nb * 12 * n_max * k_max + 8 * n_max * n_max * k_max
+        # algorithmic FP32 flops per site of the adjoint formulation: per neighbour and (n, k), cnk 4
+        # (real x complex FMA) and the vector-Jacobian product 16 (the four complex contractions of
+        # f_rad (two), f_azi and f_pol with H, 4 each); per site and (n, n', k), spectrum 4 and H = Y c 4.
+        # (The kernel factors the product as u = R H, v = R' H, 8, then a per-k contraction.)
+        flops_site = n_nb * 20 * n_max * k_max + 8 * n_max * n_max * k_max
         achieved = flops_site * w.n_sites / (avg_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,

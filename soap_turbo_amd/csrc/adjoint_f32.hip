@@ -49,8 +49,8 @@ struct SpeciesTab {
   double seg_lo[MAXSEG], seg_inv_h[MAXSEG];
   float seg_hi[MAXSEG], seg_ih[MAXSEG];
   int seg_base[MAXSEG], seg_n[MAXSEG];   // first cell and node count of each segment
-  int rtab_off;   // floats into rtab: cell-major, 4*a floats per cell (cubic coefficients c0..c3)
-  int atab_off;   // floats into atab: cell-major, 4*(L+1) floats per cell
+  int rtab_off;   // floats into rtab: per cell, per n, the cubic's c0..c3 (one float4; 4*a floats per cell)
+  int atab_off;   // floats into atab: per cell, per l, c0..c3 (one float4; 4*(L+1) floats per cell)
 };
 
 struct AdjParams {
@@ -165,34 +165,6 @@ __device__ __forceinline__ void plm_column(int m, float x, float sq, const float
     p1 = pn;
     f(l, pn);
   }
-}
-
-// R_n(r) of one pair for n in species block s (table sp), times scale (and the
-// central weight for the site's own pair), zero unless the pair is of that
-// species and inside its cutoff; optionally dR_n/dr.
-__device__ __forceinline__ float radial_one(const SpeciesTab& sp, int s, const float* __restrict__ rtab, double rd,
-                                            uint32_t msk, bool self, float scale, int n, float* dR) {
-  if (dR) *dR = 0.f;
-  if (!(msk & (1u << s)) || !(rd < sp.rcut) || (self && !sp.do_central)) return 0.f;
-  float t, ih;
-  const int cell = radial_cell(sp, rd, t, ih);
-  const float* c = rtab + sp.rtab_off + (size_t)cell * 4 * sp.a;
-  const int q = n - sp.off;
-  const float c0 = c[q], c1 = c[sp.a + q], c2 = c[2 * sp.a + q], c3 = c[3 * sp.a + q];
-  if (dR) *dR = (c1 + t * (2.f * c2 + 3.f * t * c3)) * ih;
-  const float sc = (self ? sp.cw : 1.f) * scale;
-  return sc * (c0 + t * (c1 + t * (c2 + t * c3)));
-}
-
-// Cubic-table cell of the angular radial factors g_l(r) of a species whose
-// table starts atab_off floats into atab.
-__device__ __forceinline__ const float* ang_cell(const AdjParams& P, const float* __restrict__ atab, int L1, double rd,
-                                                 int atab_off, float& ta) {
-  const double xa = rd * P.ang_inv_h;
-  int ia = (int)xa;
-  if (ia > P.ang_n - 2) ia = P.ang_n - 2;
-  ta = (float)(xa - (double)ia);
-  return atab + atab_off + (size_t)ia * 4 * L1;
 }
 
 // e^{-i m phi} by m complex multiplications from (cos phi, sin phi).
@@ -348,16 +320,15 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   float* fout = reinterpret_cast<float*>(Em + L1 * PTS);  //    3 * PT
 
   // The tile's pairs: r, species bits of contributing pairs, sin/cos of theta and phi,
-  // and per species the radial table cell (FP64 cell coordinate) or -1.
+  // and per species the radial table cell (FP64 cell coordinate) or -1. Wavefront 0 takes r,
+  // the species and the cells, wavefronts 1 and 2 the sines and cosines of theta and phi.
   auto stage = [&](int tb, bool with_self) {
+    const int pl = tx & (PT - 1), p = tb + pl;
     if (tx < PT) {
-      const int pl = tx, p = tb + pl;
       uint32_t msk = 0;
-      double rd = 1.0, th = 0.0, ph = 0.0;
+      double rd = 1.0;
       if (p < p1) {
         rd = rjs[p];
-        th = thetas[p];
-        ph = phis[p];
         msk = spmask[p];
         if (!(rd < P.rcut_max) || (p == p0 && !with_self)) msk = 0;
       }
@@ -384,17 +355,22 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
           if (s == s0) aoff = P.sp[s].atab_off;
         acell = aoff + ia * 4 * L1;
       }
-      float st_, ct_, sf_, cf_;
-      sincosf((float)th, &st_, &ct_);
-      sincosf((float)ph, &sf_, &cf_);
       pr_msk[pl] = msk != 0 && self ? (msk | SELF) : msk;
       pr_r[pl] = (float)rd;
-      pr_st[pl] = st_;
-      pr_ct[pl] = ct_;
-      pr_sf[pl] = sf_;
-      pr_cf[pl] = cf_;
       pr_acell[pl] = acell;
       pr_ta[pl] = ta;
+    } else if (tx < 2 * PT) {
+      const double th = p < p1 ? thetas[p] : 0.0;
+      float st_, ct_;
+      sincosf((float)th, &st_, &ct_);
+      pr_st[pl] = st_;
+      pr_ct[pl] = ct_;
+    } else if (tx < 3 * PT) {
+      const double ph = p < p1 ? phis[p] : 0.0;
+      float sf_, cf_;
+      sincosf((float)ph, &sf_, &cf_);
+      pr_sf[pl] = sf_;
+      pr_cf[pl] = cf_;
     }
   };
   // (n, pair) radial items: R_n times scale (and the central weight on the site's own
@@ -418,8 +394,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
           tt[u] = cell_t[s * PT + pl];
           ih[u] = cell_ih[s * PT + pl];
           const int a = sp_a[s], q = n - sp_off[s];
-          const float* cf = rtab + sp_rt[s] + (size_t)c * 4 * a;
-          c0[u] = cf[q]; c1[u] = cf[a + q]; c2[u] = cf[2 * a + q]; c3[u] = cf[3 * a + q];
+          const float4 cf = reinterpret_cast<const float4*>(rtab + sp_rt[s])[(size_t)c * a + q];
+          c0[u] = cf.x; c1[u] = cf.y; c2[u] = cf.z; c3[u] = cf.w;
           sc[u] = ((pr_msk[pl] & SELF) ? sp_cw[s] : 1.f) * scale;
         }
       }
@@ -450,8 +426,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         float g = 0.f, gd = 0.f;
         if (pr_msk[pl] != 0) {
           const float ta = pr_ta[pl];
-          const float* gc = atab + pr_acell[pl];
-          const float c0 = gc[l], c1 = gc[L1 + l], c2 = gc[2 * L1 + l], c3 = gc[3 * L1 + l];
+          const float4 gc = reinterpret_cast<const float4*>(atab + pr_acell[pl])[l];
+          const float c0 = gc.x, c1 = gc.y, c2 = gc.z, c3 = gc.w;
           g = c0 + ta * (c1 + ta * (c2 + ta * c3));
           gd = (c1 + ta * (2.f * c2 + 3.f * ta * c3)) * P.ang_ih;
         }
@@ -1292,8 +1268,8 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
         float* cc = rt.data() + sp.rtab_off + (size_t)(sp.seg_base[j] + k) * 4 * a;
         const size_t n0 = (size_t)(k0 + k) * a, n1 = n0 + a;
         for (int n = 0; n < a; ++n)
-          cell_cubic(v[n0 + n], d[n0 + n], v[n1 + n], d[n1 + n], hseg[j], cc + n, cc + a + n, cc + 2 * a + n,
-                     cc + 3 * a + n);
+          cell_cubic(v[n0 + n], d[n0 + n], v[n1 + n], d[n1 + n], hseg[j], cc + 4 * n, cc + 4 * n + 1,
+                     cc + 4 * n + 2, cc + 4 * n + 3);
       }
       k0 += sp.seg_n[j];
     }
@@ -1322,7 +1298,8 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
         const double* a0 = h.data() + (size_t)k * 2 * L1;
         const double* a1 = a0 + 2 * L1;
         for (int l = 0; l < L1; ++l)
-          cell_cubic(a0[l], a0[L1 + l], a1[l], a1[L1 + l], hh, cc + l, cc + L1 + l, cc + 2 * L1 + l, cc + 3 * L1 + l);
+          cell_cubic(a0[l], a0[L1 + l], a1[l], a1[L1 + l], hh, cc + 4 * l, cc + 4 * l + 1, cc + 4 * l + 2,
+                     cc + 4 * l + 3);
       }
     }
     (void)hipFree(dout.p);
