@@ -547,6 +547,9 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     // cnk: the tile's pairs in chunks of 16 neighbours, each chunk on MFMA, chunks
     // combined with TwoSum compensation
     const int npad = (np_ + 3) & ~3;
+    floatx4 acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
     for (int c0 = 0; c0 < npad; c0 += 16) {
       const int ns = (min(npad, c0 + 16) - c0) >> 2;
 #pragma unroll
@@ -562,13 +565,13 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             av[s] = (s < ns && n < n_max) ? Rt[n * PTF + pp] : 0.f;
             bv[s] = (s < ns && col < NC) ? At[col * PTF + pp] : 0.f;
           }
-          floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int s = 0; s < 4; ++s) acc = mfma4(av[s], bv[s], acc);
-          two_sum_acc(hi[j], lo[j], acc);
+          for (int s = 0; s < 4; ++s) acc[j] = mfma4(av[s], bv[s], acc[j]);
         }
       }
     }
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) two_sum_acc(hi[j], lo[j], acc[j]);
   }
   // C = hi + lo, plus the poly3gauss central-atom term on k = 0 (real part)
 #pragma unroll
@@ -1070,6 +1073,10 @@ __global__ void widen_k(size_t n, const float* __restrict__ a, double* __restric
   if (i < n) b[i] = (double)a[i];
 }
 
+// Dynamic LDS of the site kernel. Its static arrays take about 3.4 KB more. Measured on MI355X:
+// workgroups of 53,360 B (static + dynamic, l_max 8, n_max 24, 3 species) run three per CU, while
+// 53,936 B ran only two at a time although the occupancy query still reported three, so the
+// LDS budget of a three-per-CU shape is tight.
 size_t site_lds(int n_species, int n_max, int l_max, int n_soap, int n_soap_unc) {
   const int KM = (l_max + 1) * (l_max + 2) / 2, KM1 = (l_max + 2) * (l_max + 3) / 2, L1 = l_max + 1;
   const int CSTR = ((2 * KM + 15) / 16) * 16, CS = CSTR + 4, KMP = ((KM + 15) / 16) * 16, HS = 2 * KMP + 16;
