@@ -544,8 +544,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       }
     }
     __syncthreads();
-    // cnk: the tile's pairs in chunks of 16 neighbours, each chunk on MFMA, chunks
-    // combined with TwoSum compensation
+    // cnk: the tile's pairs on MFMA, 16 neighbours per load batch, in one accumulation chain
+    // per tile; tiles are combined with TwoSum compensation
     const int npad = (np_ + 3) & ~3;
     floatx4 acc[TPW];
 #pragma unroll
