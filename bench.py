@@ -94,6 +94,28 @@ def time_path(ctx, step, steps, warmup, kernels):
     return ms, wall, ktimes
 
 
+def time_neighbours(w, reps=20):
+    """The workload's neighbour list built on the device (st_build_neighbours: positions in, the resident
+    list out, its host synchronisations included), and the host cell-list generator on the same input."""
+    import soap_turbo_amd as st
+    from soap_turbo_amd.workload import build_neighbours, spherical
+    log("timing the device neighbour-list builder")
+    rcut = float(np.max(w.params.per_species()["rcut_hard"]))
+    ctx = st.Context(0)
+    args = (w.pos, w.atom_species + 1, w.params.n_species, w.L, rcut, w.site_atom)
+    n_pairs = ctx.build_neighbours(*args)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.build_neighbours(*args)
+    dev_ms = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    n_neigh, nbr, vec = build_neighbours(w.pos, w.L, rcut, w.site_atom)
+    spherical(vec)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    return {"atoms": len(w.pos), "sites": w.n_sites, "pairs": n_pairs, "device_ms": dev_ms,
+            "host_cell_list_ms": host_ms, "host_threads": 1}
+
+
 def cpu_baseline(w, G, cfg):
     """The reference (oracle/_ref, compiled from its sources) on the same workload, all host cores."""
     from oracle import refio
@@ -228,6 +250,7 @@ def main():
                                "kernels_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kt32.items() if v[1]},
                                "speedup_vs_fp64": (ms64 / ms32),
                                "max_rel_err_forces_vs_fp64": ferr, "max_rel_err_virial_vs_fp64": verr}
+    out["neighbour_list"] = time_neighbours(w)
     if not args.no_cpu:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(w, G, args.config)

@@ -21,6 +21,7 @@
  *                           reference's counterpart of the resident state)
  *   st_adjoint_f32          no reference counterpart: dE/dq -> pair gradients, forces and virial in FP32
  *                           without forming soap_cart_der (the FP32 form of soap_contract, :1703-1721)
+ *   st_build_neighbours     no reference counterpart: the callers' neighbour list, built on the device
  *
  * Threading: one context per host thread; calls on a context are ordered on
  * its HIP stream. Like the reference, a context is not reentrant.
@@ -123,6 +124,23 @@ int st_upload_neighbours(st_ctx* ctx, const st_neighbours* nb);
  * pair), n_atoms atoms; needed only for forces. vec(3, n_pairs): r_j - r_i, needed only for the virial
  * (may be NULL: it is rebuilt from rjs, thetas, phis). */
 int st_upload_atoms(st_ctx* ctx, int32_t n_atoms, const int32_t* pair_atom, const double* vec);
+/* Device neighbour list of a periodic cubic box (no reference counterpart: the reference's callers
+ * build it). Fills the same resident arrays as st_upload_neighbours + st_upload_atoms, in the order
+ * get_soap expects (src/soap_turbo.f90:139-169): per site its own atom first (r = 0), then every other
+ * atom j with |r_j - r_i| < rcut under the minimum image in increasing j; thetas = acos(z/r),
+ * phis = atan2(y, x); mask(k, s) = (species of the neighbour == s), the mask semantics of
+ * assign_species_multiplicity (src/soap_turbo.f90:2188-2214); one species per site. positions(3, n_atoms),
+ * atom_species(n_atoms) 1-based, box side > 2 rcut, site_atoms(n_sites) 0-based. *n_pairs gets the pair
+ * count (may be NULL). */
+int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, const int32_t* atom_species,
+                        int32_t n_species, double box, double rcut, int32_t n_sites, const int32_t* site_atoms,
+                        int32_t* n_pairs);
+/* The resident neighbour list: n_neigh(n_sites), pair_atom(n_pairs), rjs, thetas, phis(n_pairs),
+ * mask(n_pairs, n_species) int32, vec(3, n_pairs); any may be NULL. */
+int st_download_neighbours(st_ctx* ctx, int32_t* n_neigh, int32_t* pair_atom, double* rjs, double* thetas,
+                           double* phis, int32_t* mask, double* vec);
+/* The pairs of each atom the force gather reads: atom_ptr(n_atoms + 1), atom_pairs(n_pairs). */
+int st_atom_pairs(st_ctx* ctx, int32_t* atom_ptr, int32_t* atom_pairs);
 /* dE/dq as G(n_sites, n_soap) */
 int st_upload_dedq(st_ctx* ctx, int32_t n_soap, const double* G);
 /* FP64 get_soap on the resident neighbour list. */

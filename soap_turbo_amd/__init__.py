@@ -120,6 +120,34 @@ class Context:
         self.check(lib().st_upload_neighbours(self.h, ctypes.byref(nb)))
         self.n_sites, self.n_pairs = len(n_neigh), len(rjs)
 
+    def build_neighbours(self, pos, atom_species, n_species, box, rcut, site_atom):
+        """Neighbour list of a periodic cubic box built on the device (pos (n_atoms, 3), atom_species
+        1-based, site_atom 0-based): the resident state of upload_neighbours + upload_atoms."""
+        keep = Keep()
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        n = ctypes.c_int32(0)
+        self.check(lib().st_build_neighbours(self.h, len(pos), keep(pos, np.float64, "C"), keep(atom_species, np.int32),
+                                             int(n_species), float(box), float(rcut), len(site_atom),
+                                             keep(site_atom, np.int32), ctypes.byref(n)))
+        self.n_sites, self.n_pairs, self.n_atoms, self.n_species_nb = len(site_atom), n.value, len(pos), int(n_species)
+        return n.value
+
+    def download_neighbours(self):
+        """(n_neigh, pair_atom, rjs, thetas, phis, mask (n_species, n_pairs) bool, vec (n_pairs, 3))."""
+        P = self.n_pairs
+        n_neigh = np.zeros(self.n_sites, np.int32); pa = np.zeros(P, np.int32)
+        rjs, th, ph = np.zeros(P), np.zeros(P), np.zeros(P)
+        mask = np.zeros((self.n_species_nb, P), np.int32); vec = np.zeros((P, 3))
+        self.check(lib().st_download_neighbours(self.h, ptr(n_neigh), ptr(pa), ptr(rjs), ptr(th), ptr(ph), ptr(mask),
+                                                ptr(vec)))
+        return n_neigh, pa, rjs, th, ph, mask.astype(bool), vec
+
+    def atom_pairs(self):
+        """(atom_ptr, atom_pairs): the pairs whose neighbour is each atom, in increasing pair index."""
+        ap = np.zeros(self.n_atoms + 1, np.int32); pr = np.zeros(self.n_pairs, np.int32)
+        self.check(lib().st_atom_pairs(self.h, ptr(ap), ptr(pr)))
+        return ap, pr
+
     def upload_atoms(self, n_atoms, pair_atom, vec=None):
         keep = Keep()
         v = keep(vec, np.float64, "C") if vec is not None else None

@@ -48,6 +48,10 @@ SIGNATURES = {
     "st_get_compress_indices": (ctypes.c_int, [ctypes.c_char_p, I32, P, I32, P, P, P, P, P, I32]),
     "st_upload_neighbours": (ctypes.c_int, [P, P]),
     "st_upload_atoms": (ctypes.c_int, [P, I32, P, P]),
+    "st_build_neighbours": (ctypes.c_int, [P, I32, P, P, I32, ctypes.c_double, ctypes.c_double, I32, P,
+                                           ctypes.POINTER(I32)]),
+    "st_download_neighbours": (ctypes.c_int, [P, P, P, P, P, P, P, P]),
+    "st_atom_pairs": (ctypes.c_int, [P, P, P]),
     "st_upload_dedq": (ctypes.c_int, [P, I32, P]),
     "st_compute": (ctypes.c_int, [P, P, P, I32, I32]),
     "st_pairs_full_jacobian": (ctypes.c_int, [P]),

@@ -54,6 +54,9 @@ struct st_ctx {
   int n_atoms = 0;
   bool have_atoms = false, have_vec = false;
   DevBuf pair_atom, vec, atom_ptr, atom_pairs;
+  // device neighbour-list builder: positions, atom species, sites and the cell list
+  DevBuf nb_pos, nb_species, nb_sites, nb_cell_of, nb_cell_count, nb_cell_start, nb_cursor, nb_cell_atoms,
+      nb_count, nb_flag;
   // dE/dq
   int G_n_soap = 0;
   bool have_G = false;
@@ -925,6 +928,127 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   rc = st::adjoint_run(ctx->adj, ctx->stream, in, timer, msg);
   if (rc) return fail(ctx, rc, msg);
   HIPCHK(hipGetLastError());
+  return ST_OK;
+}
+
+int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, const int32_t* atom_species,
+                        int32_t n_species, double box, double rcut, int32_t n_sites, const int32_t* site_atoms,
+                        int32_t* n_pairs) {
+  if (!ctx) return ST_ERR_ARG;
+  if (n_atoms < 1 || n_sites < 1 || !positions || !atom_species || !site_atoms)
+    return fail(ctx, ST_ERR_ARG, "positions, species and sites are required");
+  if (n_species < 1 || n_species > ST_MAX_SPECIES) return fail(ctx, ST_ERR_ARG, "bad number of species");
+  if (!(rcut > 0.0) || !(box > 2.0 * rcut))
+    return fail(ctx, ST_ERR_ARG, "the box side must exceed twice the cutoff (minimum image)");
+  for (int a = 0; a < n_atoms; ++a)
+    if (atom_species[a] < 1 || atom_species[a] > n_species) return fail(ctx, ST_ERR_ARG, "atom species out of range");
+  for (int s = 0; s < n_sites; ++s)
+    if (site_atoms[s] < 0 || site_atoms[s] >= n_atoms) return fail(ctx, ST_ERR_ARG, "site atom out of range");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  int rc;
+  const int nc = st::neighbour_cells(box, rcut);
+  const size_t n_cells = (size_t)nc * nc * nc;
+  if ((rc = upload(ctx, ctx->nb_pos, positions, (size_t)n_atoms * 24))) return rc;
+  if ((rc = upload(ctx, ctx->nb_species, atom_species, (size_t)n_atoms * 4))) return rc;
+  if ((rc = upload(ctx, ctx->nb_sites, site_atoms, (size_t)n_sites * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->nb_cell_of, (size_t)n_atoms * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->nb_cell_count, n_cells * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->nb_cell_start, (n_cells + 1) * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->nb_cursor, std::max(n_cells, (size_t)n_atoms) * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->nb_cell_atoms, (size_t)n_atoms * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->nb_count, std::max((size_t)n_sites, (size_t)n_atoms) * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->nb_flag, 4))) return rc;
+  if ((rc = ensure(ctx, ctx->site_ptr, ((size_t)n_sites + 1) * 4))) return rc;
+  st::launch_neighbour_bins(st, n_atoms, ctx->nb_pos.as<double>(), box, nc, ctx->nb_cell_of.as<int>(),
+                            ctx->nb_cell_count.as<int>(), ctx->nb_cell_start.as<int>(), ctx->nb_cursor.as<int>(),
+                            ctx->nb_cell_atoms.as<int>());
+  st::launch_neighbour_count(st, n_sites, ctx->nb_sites.as<int>(), ctx->nb_pos.as<double>(), box, rcut, nc,
+                             ctx->nb_cell_start.as<int>(), ctx->nb_cell_atoms.as<int>(), ctx->nb_count.as<int>(),
+                             ctx->site_ptr.as<int>());
+  std::vector<int> ptr((size_t)n_sites + 1);
+  HIPCHK(hipMemcpyAsync(ptr.data(), ctx->site_ptr.p, ptr.size() * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const size_t P = ptr[n_sites];
+  if (P > (size_t)INT32_MAX / 4) return fail(ctx, ST_ERR_UNSUPPORTED, "too many pairs");
+  if ((rc = ensure(ctx, ctx->pair_atom, P * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->vec, P * 24))) return rc;
+  if ((rc = ensure(ctx, ctx->rjs, P * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->thetas, P * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->phis, P * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->mask, P * n_species * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->species, (size_t)n_sites * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->mult, (size_t)n_sites * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->atom_ptr, ((size_t)n_atoms + 1) * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->atom_pairs, P * 4))) return rc;
+  HIPCHK(hipMemsetAsync(ctx->nb_flag.p, 0, 4, st));
+  st::launch_neighbour_fill(st, n_sites, (int)P, n_species, ctx->nb_sites.as<int>(), ctx->nb_species.as<int>(),
+                            ctx->nb_pos.as<double>(), box, rcut, nc, ctx->nb_cell_start.as<int>(),
+                            ctx->nb_cell_atoms.as<int>(), ctx->site_ptr.as<int>(), ctx->pair_atom.as<int>(),
+                            ctx->vec.as<double>(), ctx->rjs.as<double>(), ctx->thetas.as<double>(),
+                            ctx->phis.as<double>(), ctx->mask.as<int32_t>(), ctx->species.as<int32_t>(),
+                            ctx->mult.as<int32_t>(), ctx->nb_flag.as<int>());
+  st::launch_atom_pairs(st, n_atoms, (int)P, ctx->pair_atom.as<int>(), ctx->nb_count.as<int>(),
+                        ctx->nb_cursor.as<int>(), ctx->atom_ptr.as<int>(), ctx->atom_pairs.as<int>(),
+                        ctx->nb_flag.as<int>());
+  int flag = 0;
+  HIPCHK(hipMemcpyAsync(&flag, ctx->nb_flag.p, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  // the resident state now describes the new list (as after st_upload_neighbours + st_upload_atoms)
+  ctx->have_nb = ctx->have_atoms = false;
+  ctx->have_soap = ctx->have_cart = ctx->ct_ready = ctx->have_pair64 = false;
+  ctx->have_G = false;
+  st::adjoint_invalidate(ctx->adj);
+  if (flag & 1) return fail(ctx, ST_ERR_UNSUPPORTED, "a site has more than 1024 neighbours");
+  if (flag & 2) return fail(ctx, ST_ERR_UNSUPPORTED, "an atom is the neighbour in more than 1024 pairs");
+  ctx->n_sites = n_sites;
+  ctx->n_pairs = (int)P;
+  ctx->n_species_mask = n_species;
+  ctx->packed_species = 0;
+  ctx->max_mult = 1;
+  ctx->site_ptr_h = ptr;
+  ctx->have_nb = true;
+  ctx->n_atoms = n_atoms;
+  ctx->have_atoms = ctx->have_vec = true;
+  if (n_pairs) *n_pairs = (int32_t)P;
+  return ST_OK;
+}
+
+int st_download_neighbours(st_ctx* ctx, int32_t* n_neigh, int32_t* pair_atom, double* rjs, double* thetas,
+                           double* phis, int32_t* mask, double* vec) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "no neighbour list");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const size_t P = ctx->n_pairs;
+  if (n_neigh)
+    for (int s = 0; s < ctx->n_sites; ++s) n_neigh[s] = ctx->site_ptr_h[s + 1] - ctx->site_ptr_h[s];
+  if (pair_atom) {
+    if (!ctx->have_atoms) return fail(ctx, ST_ERR_STATE, "no pair atoms");
+    HIPCHK(hipMemcpyAsync(pair_atom, ctx->pair_atom.p, P * 4, hipMemcpyDeviceToHost, st));
+  }
+  if (rjs) HIPCHK(hipMemcpyAsync(rjs, ctx->rjs.p, P * 8, hipMemcpyDeviceToHost, st));
+  if (thetas) HIPCHK(hipMemcpyAsync(thetas, ctx->thetas.p, P * 8, hipMemcpyDeviceToHost, st));
+  if (phis) HIPCHK(hipMemcpyAsync(phis, ctx->phis.p, P * 8, hipMemcpyDeviceToHost, st));
+  if (mask) HIPCHK(hipMemcpyAsync(mask, ctx->mask.p, P * ctx->n_species_mask * 4, hipMemcpyDeviceToHost, st));
+  if (vec) {
+    if (!ctx->have_vec) return fail(ctx, ST_ERR_STATE, "no pair vectors");
+    HIPCHK(hipMemcpyAsync(vec, ctx->vec.p, P * 24, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return ST_OK;
+}
+
+int st_atom_pairs(st_ctx* ctx, int32_t* atom_ptr, int32_t* atom_pairs) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!ctx->have_atoms) return fail(ctx, ST_ERR_STATE, "no pair atoms");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (atom_ptr)
+    HIPCHK(hipMemcpyAsync(atom_ptr, ctx->atom_ptr.p, ((size_t)ctx->n_atoms + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (atom_pairs)
+    HIPCHK(hipMemcpyAsync(atom_pairs, ctx->atom_pairs.p, (size_t)ctx->n_pairs * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return ST_OK;
 }
 

@@ -40,4 +40,19 @@ void launch_contract(hipStream_t st, int n_sites, int n_max, int l_max, int k_ma
                      const double2* ang_p, const double* soap, const double* G, double sgn, const double* sqrt_dot_p,
                      const double* rjs, const double* thetas, const double* phis, double* pair);
 
+// Device neighbour lists (neighbours.hip)
+int neighbour_cells(double L, double rcut);  // cells a side of the cell list
+void launch_neighbour_bins(hipStream_t st, int n_atoms, const double* pos, double L, int nc, int* cell_of,
+                           int* cell_count, int* cell_start, int* cursor, int* cell_atoms);
+void launch_neighbour_count(hipStream_t st, int n_sites, const int* site_atom, const double* pos, double L,
+                            double rcut, int nc, const int* cell_start, const int* cell_atoms, int* n_neigh,
+                            int* site_ptr);
+void launch_neighbour_fill(hipStream_t st, int n_sites, int n_pairs, int n_species, const int* site_atom,
+                           const int* species, const double* pos, double L, double rcut, int nc, const int* cell_start,
+                           const int* cell_atoms, const int* site_ptr, int* pair_atom, double* vec, double* rjs,
+                           double* thetas, double* phis, int32_t* mask, int32_t* site_species, int32_t* mult,
+                           int* overflow);
+void launch_atom_pairs(hipStream_t st, int n_atoms, int n_pairs, const int* pair_atom, int* count, int* cursor,
+                       int* atom_ptr, int* atom_pairs, int* overflow);
+
 }  // namespace st
