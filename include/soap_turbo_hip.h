@@ -151,6 +151,11 @@ int st_pairs_full_jacobian(st_ctx* ctx);
 int st_contract_resident(st_ctx* ctx, double sgn);
 /* FP32 adjoint: descriptors, pair gradients, forces and virial from the resident data and dE/dq. */
 int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c);
+/* Graph mode for st_adjoint_f32 (off by default; no reference counterpart): the second call with the
+ * same parameters, sizes and resident buffers records the step's kernel launches in a HIP graph and
+ * later calls replay it (new dE/dq or coordinates of the same sizes are read by the replay). Calls
+ * made with kernel timing on run uncaptured. */
+int st_set_graphs(st_ctx* ctx, int32_t on);
 int st_download(st_ctx* ctx, double* soap, double* soap_cart_der);
 /* pair(3, n_pairs) of the last st_pairs_full_jacobian / st_contract_resident (FP64) or st_adjoint_f32. */
 int st_download_pairs(st_ctx* ctx, int32_t which, double* pair);   /* which: 0 FP64, 1 FP32 */

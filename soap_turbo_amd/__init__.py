@@ -120,6 +120,10 @@ class Context:
         self.check(lib().st_upload_neighbours(self.h, ctypes.byref(nb)))
         self.n_sites, self.n_pairs = len(n_neigh), len(rjs)
 
+    def set_graphs(self, on: bool):
+        """Replay the FP32 step from a captured HIP graph while its inputs keep their shapes."""
+        self.check(lib().st_set_graphs(self.h, int(bool(on))))
+
     def build_neighbours(self, pos, atom_species, n_species, box, rcut, site_atom):
         """Neighbour list of a periodic cubic box built on the device (pos (n_atoms, 3), atom_species
         1-based, site_atom 0-based): the resident state of upload_neighbours + upload_atoms."""

@@ -27,7 +27,6 @@ struct AdjointState {
   // per-call work and outputs
   AdjBuf soap, sdp, pair, forces, virial_part, virial, G32, order, tickets;
   bool order_ok = false;                 // order holds the sites of the current neighbour list
-  unsigned long long ticket_base = 0;    // the site queue's counter at the start of the next launch
   int g32_sites = 0, g32_soap = 0;  // shape of the site-major FP32 dE/dq in G32 (0: none)
   AdjBuf el32, odir;
   AdjBuf cenv;                  // per-site central-atom term (poly3gauss)
@@ -93,9 +92,12 @@ inline int gather_blocks(int n_atoms) { return n_atoms > 0 ? (n_atoms + 15) / 16
 void launch_forces_f64(hipStream_t st, int n_atoms, const double* pair, const int* atom_ptr, const int* atom_pairs,
                        const double* vec, const double* rjs, const double* thetas, const double* phis,
                        double* forces, double* vpart, double* virial);
+// (reset: a counter the last kernel of the step sets back to zero, or null)
 void launch_forces_f32(hipStream_t st, int n_atoms, const float* pair, const int* atom_ptr, const int* atom_pairs,
                        const double* vec, const double* rjs, const double* thetas, const double* phis,
-                       double* forces, double* vpart, double* virial);
+                       double* forces, double* vpart, double* virial, unsigned long long* reset);
+// sets *counter to zero (a kernel, so that it can be part of a captured graph)
+void launch_counter_reset(hipStream_t st, unsigned long long* counter);
 void adjoint_invalidate(AdjointState& s);
 void adjoint_release(AdjointState& s);
 

@@ -86,10 +86,12 @@ __global__ __launch_bounds__(256) void gather_k(int n_atoms, const T* __restrict
   }
 }
 
-// one wavefront per virial component, lanes striding over the workgroup partials
+// one wavefront per virial component, lanes striding over the workgroup partials; also sets the
+// step's work counter (if any) back to zero
 __global__ __launch_bounds__(576) void virial_sum_k(int nb, const double* __restrict__ vpart,
-                                                    double* __restrict__ virial) {
+                                                    double* __restrict__ virial, unsigned long long* reset) {
   const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (reset && threadIdx.x == 0) *reset = 0ull;
   double t = 0.0;
   for (int b = lane; b < nb; b += 64) t += vpart[(size_t)b * 9 + q];
   for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
@@ -99,11 +101,13 @@ __global__ __launch_bounds__(576) void virial_sum_k(int nb, const double* __rest
 template <class T>
 void gather(hipStream_t st, int n_atoms, const T* pair, const int* atom_ptr, const int* atom_pairs,
             const double* vec, const double* rjs, const double* thetas, const double* phis, double* forces,
-            double* vpart, double* virial) {
+            double* vpart, double* virial, unsigned long long* reset) {
   const int nb = gather_blocks(n_atoms);
   gather_k<T><<<nb, 256, 0, st>>>(n_atoms, pair, atom_ptr, atom_pairs, vec, rjs, thetas, phis, forces, vpart);
-  virial_sum_k<<<1, 576, 0, st>>>(nb, vpart, virial);
+  virial_sum_k<<<1, 576, 0, st>>>(nb, vpart, virial, reset);
 }
+
+__global__ void counter_reset_k(unsigned long long* c) { *c = 0ull; }
 
 int download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial, std::string& msg) {
   if (forces && hipMemcpyAsync(forces, s.forces.p, (size_t)n_atoms * 24, hipMemcpyDeviceToHost, st) != hipSuccess) {
@@ -126,14 +130,16 @@ int download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces
 void launch_forces_f64(hipStream_t st, int n_atoms, const double* pair, const int* atom_ptr, const int* atom_pairs,
                        const double* vec, const double* rjs, const double* thetas, const double* phis,
                        double* forces, double* vpart, double* virial) {
-  gather<double>(st, n_atoms, pair, atom_ptr, atom_pairs, vec, rjs, thetas, phis, forces, vpart, virial);
+  gather<double>(st, n_atoms, pair, atom_ptr, atom_pairs, vec, rjs, thetas, phis, forces, vpart, virial, nullptr);
 }
 
 void launch_forces_f32(hipStream_t st, int n_atoms, const float* pair, const int* atom_ptr, const int* atom_pairs,
                        const double* vec, const double* rjs, const double* thetas, const double* phis,
-                       double* forces, double* vpart, double* virial) {
-  gather<float>(st, n_atoms, pair, atom_ptr, atom_pairs, vec, rjs, thetas, phis, forces, vpart, virial);
+                       double* forces, double* vpart, double* virial, unsigned long long* reset) {
+  gather<float>(st, n_atoms, pair, atom_ptr, atom_pairs, vec, rjs, thetas, phis, forces, vpart, virial, reset);
 }
+
+void launch_counter_reset(hipStream_t st, unsigned long long* counter) { counter_reset_k<<<1, 1, 0, st>>>(counter); }
 
 int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial,
                             std::string& msg) {

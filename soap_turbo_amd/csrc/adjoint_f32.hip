@@ -264,8 +264,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     const int* __restrict__ out_ptr, const int* __restrict__ out_c, const float* __restrict__ out_el,
     const int* __restrict__ unc_ptr, const int* __restrict__ unc_e, const int32_t* __restrict__ P_i,
     const float* __restrict__ P_el, const int2* __restrict__ odir, float* __restrict__ q_out,
-    float* __restrict__ sdp_out, float* __restrict__ pair_out, unsigned long long* __restrict__ tickets,
-    unsigned long long ticket_base) {
+    float* __restrict__ sdp_out, float* __restrict__ pair_out, unsigned long long* __restrict__ tickets) {
   using Sh = SiteShape<L>;
   constexpr int KM = Sh::KM, KM1 = Sh::KM1, L1 = Sh::L1, CSTR = Sh::CSTR, CT = Sh::CT;
   constexpr int CS = Sh::CS, KMP = Sh::KMP, HS = Sh::HS, KT = KMP / 16;
@@ -467,9 +466,9 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     }
   }
   // Sites in the order given (heaviest first): workgroup b starts on site b, and each workgroup
-  // takes its next site from a queue (tickets counted from ticket_base, after the grid's first
-  // sites) while it works on the current one, until the queue runs dry. The grid is one resident
-  // wave of workgroups.
+  // takes its next site from a queue (a ticket counter that is zero at launch; ticket t is site
+  // gridDim.x + t) while it works on the current one, until the queue runs dry. The grid is one
+  // resident wave of workgroups. The step's last kernel sets the counter back to zero.
   if (threadIdx.x == 0)
     site_next = (int)blockIdx.x < P.n_sites ? (order ? order[blockIdx.x] : (int)blockIdx.x) : -1;
   unsigned long long tk = 0;  // thread 0: the ticket for the next site
@@ -876,7 +875,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   }
   const float* H = C;
   if (tx == 0) {
-    const unsigned long long t = tk - ticket_base + gridDim.x;
+    const unsigned long long t = tk + gridDim.x;
     nxt = t < (unsigned long long)P.n_sites ? (order ? order[t] : (int)t) : -1;
   }
   // --------------------------------------------------------------- backward
@@ -1094,7 +1093,7 @@ void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order
                  const uint32_t* spmask, const double* rjs, const double* th, const double* ph, const float* G32,
                  const int4* comp, const int* out_ptr, const int* out_c, const float* out_el, const int* unc_ptr,
                  const int* unc_e, const int32_t* P_i, const float* P_el, const int2* odir, float* q, float* sdp,
-                 float* pair, unsigned long long* tickets, unsigned long long& ticket_base, int n_cu) {
+                 float* pair, unsigned long long* tickets, int n_cu) {
   const void* fn = reinterpret_cast<const void*>(&adjoint_site_k<L, NTT>);
   if (sh > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   int per_cu = 0;
@@ -1102,9 +1101,7 @@ void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order
   const int grid = std::max(1, std::min(P.n_sites, per_cu * n_cu));
   adjoint_site_k<L, NTT><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask, rjs, th,
                                                      ph, G32, comp, out_ptr, out_c, out_el, unc_ptr, unc_e, P_i,
-                                                     P_el, odir, q, sdp, pair, tickets, ticket_base);
-  // one ticket per site processed (the grid's first sites are taken by workgroup index)
-  ticket_base += (unsigned long long)P.n_sites;
+                                                     P_el, odir, q, sdp, pair, tickets);
 }
 
 // host copy of the per-call constant tables that live next to the FP64 ones
@@ -1424,7 +1421,6 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
       msg = "clearing the site queue failed";
       return ST_ERR_HIP;
     }
-    s.ticket_base = 0;
   }
   bool ok = true;
   timer("adj_site_f32", [&] {
@@ -1435,7 +1431,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     launch_site<l, t>(st, P, sh, s.order.as<int>(), in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cenv, \
                       in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(), in.comp, in.out_ptr, in.out_c, oel, \
                       in.unc_ptr, in.unc_e, in.P_i, pel, odir, s.soap.as<float>(), s.sdp.as<float>(),              \
-                      s.pair.as<float>(), s.tickets.as<unsigned long long>(), s.ticket_base, in.n_cu);            \
+                      s.pair.as<float>(), s.tickets.as<unsigned long long>(), in.n_cu);                          \
     break;
 #define LS(l) LS1(l, 1) LS1(l, 2) LS1(l, 3)
       LS(0) LS(1) LS(2) LS(3) LS(4) LS(5) LS(6) LS(7) LS(8) LS(9) LS(10) LS(11) LS(12)
@@ -1454,15 +1450,20 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   s.n_soap = in.n_soap;
   s.have_forces = false;
   if (in.pair_atom) {
-    if ((rc = ensure(s.forces, (size_t)in.n_atoms * 24, msg))) return rc;
-    if ((rc = ensure(s.virial_part, (size_t)gather_blocks(in.n_atoms) * 72, msg))) return rc;
-    if ((rc = ensure(s.virial, 72, msg))) return rc;
+    if ((rc = ensure(s.forces, (size_t)in.n_atoms * 24, msg)) ||
+        (rc = ensure(s.virial_part, (size_t)gather_blocks(in.n_atoms) * 72, msg)) || (rc = ensure(s.virial, 72, msg))) {
+      launch_counter_reset(st, s.tickets.as<unsigned long long>());  // the site queue starts at zero next time
+      return rc;
+    }
     timer("adj_forces", [&] {
       launch_forces_f32(st, in.n_atoms, s.pair.as<float>(), in.atom_ptr, in.atom_pairs, in.vec, in.rjs, in.thetas,
-                        in.phis, s.forces.as<double>(), s.virial_part.as<double>(), s.virial.as<double>());
+                        in.phis, s.forces.as<double>(), s.virial_part.as<double>(), s.virial.as<double>(),
+                        s.tickets.as<unsigned long long>());
     });
     s.have_forces = true;
     s.n_atoms = in.n_atoms;
+  } else {
+    launch_counter_reset(st, s.tickets.as<unsigned long long>());
   }
   return ST_OK;
 }

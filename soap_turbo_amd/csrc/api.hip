@@ -79,6 +79,13 @@ struct st_ctx {
 
   // FP32 path
   st::AdjointState adj;
+  // captured launch sequence of the FP32 step (st_set_graphs): recorded on the second call with the
+  // same key, replayed while the key holds
+  bool graphs = false;
+  uint64_t nb_gen = 0;  // bumped by every new neighbour list or atom upload
+  std::string g32_key, g32_warm;
+  hipGraph_t g32_graph = nullptr;
+  hipGraphExec_t g32_exec = nullptr;
 
   // events
   hipEvent_t ev[16] = {};
@@ -546,6 +553,8 @@ void st_destroy(st_ctx* c) {
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   for (auto& pd : c->pending) { (void)hipEventDestroy(pd.a); (void)hipEventDestroy(pd.b); }
   for (auto e : c->free_events) (void)hipEventDestroy(e);
+  if (c->g32_exec) (void)hipGraphExecDestroy(c->g32_exec);
+  if (c->g32_graph) (void)hipGraphDestroy(c->g32_graph);
   st::adjoint_release(c->adj);
   hipStream_t s = c->stream;
   delete c;
@@ -622,6 +631,7 @@ int st_upload_neighbours(st_ctx* ctx, const st_neighbours* nb) {
   ctx->have_vec = false;
   ctx->have_G = false;  // dE/dq is per site: upload it again for the new sites
   st::adjoint_invalidate(ctx->adj);
+  ++ctx->nb_gen;
   return ST_OK;
 }
 
@@ -648,6 +658,7 @@ int st_upload_atoms(st_ctx* ctx, int32_t n_atoms, const int32_t* pair_atom, cons
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->n_atoms = n_atoms;
   ctx->have_atoms = true;
+  ++ctx->nb_gen;
   return ST_OK;
 }
 
@@ -856,6 +867,59 @@ int st_download_forces(st_ctx* ctx, int32_t which, double* forces, double* viria
   return rc ? fail(ctx, rc, msg) : ST_OK;
 }
 
+namespace {
+
+// Everything the FP32 step's kernels take by value or by address: a captured graph of the step is
+// valid while this key is unchanged (buffer contents may change; addresses, sizes and parameters not).
+std::string adjoint_graph_key(const st_ctx* ctx, const st_params* p) {
+  std::string k;
+  auto add = [&](const void* x, size_t n) { k.append(static_cast<const char*>(x), n); };
+  add(p, 6 * sizeof(int32_t));
+  add(p->alpha_max, 4 * p->n_species);
+  for (const double* a : {p->rcut_hard, p->rcut_soft, p->nf, p->global_scaling, p->atom_sigma_r,
+                          p->atom_sigma_r_scaling, p->atom_sigma_t, p->atom_sigma_t_scaling, p->amplitude_scaling,
+                          p->central_weight})
+    add(a, 8 * p->n_species);
+  k += ctx->basis_key;
+  k += ctx->tables_key;
+  const uint64_t gen = ctx->nb_gen;
+  const int sizes[] = {ctx->n_sites, ctx->n_pairs, ctx->n_atoms, ctx->n_soap, ctx->G_n_soap, ctx->have_atoms,
+                       ctx->have_vec, ctx->packed_species};
+  add(&gen, sizeof gen);
+  add(sizes, sizeof sizes);
+  const void* bufs[] = {ctx->site_ptr.p, ctx->species.p, ctx->mult.p, ctx->spmask.p, ctx->rjs.p, ctx->thetas.p,
+                        ctx->phis.p, ctx->comp.p, ctx->out_ptr.p, ctx->out_c.p, ctx->unc_ptr.p, ctx->unc_e.p,
+                        ctx->P_i.p, ctx->pair_atom.p, ctx->atom_ptr.p, ctx->atom_pairs.p, ctx->vec.p,
+                        ctx->adj.rtab.p, ctx->adj.atab.p, ctx->adj.tabinfo.p, ctx->adj.soap.p, ctx->adj.sdp.p,
+                        ctx->adj.pair.p, ctx->adj.forces.p, ctx->adj.virial_part.p, ctx->adj.virial.p,
+                        ctx->adj.G32.p, ctx->adj.order.p, ctx->adj.tickets.p, ctx->adj.el32.p, ctx->adj.odir.p,
+                        ctx->adj.cenv.p};
+  add(bufs, sizeof bufs);
+  return k;
+}
+
+void drop_graph(st_ctx* ctx) {
+  if (ctx->g32_exec) (void)hipGraphExecDestroy(ctx->g32_exec);
+  if (ctx->g32_graph) (void)hipGraphDestroy(ctx->g32_graph);
+  ctx->g32_exec = nullptr;
+  ctx->g32_graph = nullptr;
+  ctx->g32_key.clear();
+}
+
+}  // namespace
+
+int st_set_graphs(st_ctx* ctx, int32_t on) {
+  if (!ctx) return ST_ERR_ARG;
+  ctx->graphs = on != 0;
+  if (!ctx->graphs) {
+    drop_graph(ctx);
+    ctx->g32_warm.clear();
+  }
+  return ST_OK;
+}
+
+static int run_adjoint(st_ctx* ctx, const st_params* p);
+
 int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   if (!ctx) return ST_ERR_ARG;
   if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
@@ -866,6 +930,39 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   if ((rc = build_tables(ctx, p, c))) return rc;
   if (!ctx->have_G || ctx->G_n_soap != ctx->n_soap) return fail(ctx, ST_ERR_STATE, "dE/dq missing or of another size");
   if ((rc = pack_mask(ctx, p->n_species))) return rc;
+  if (!ctx->graphs || ctx->timing) return run_adjoint(ctx, p);
+  // graph mode: the first call with a key runs as usual (it builds tables and buffers), the second
+  // records the step's launches, later ones replay them
+  const std::string key = adjoint_graph_key(ctx, p);
+  if (ctx->g32_exec && key == ctx->g32_key) {
+    HIPCHK(hipGraphLaunch(ctx->g32_exec, ctx->stream));
+    return ST_OK;
+  }
+  if (key != ctx->g32_warm) {
+    ctx->g32_warm = key;
+    return run_adjoint(ctx, p);
+  }
+  drop_graph(ctx);
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  rc = run_adjoint(ctx, p);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc || e != hipSuccess || !g) {
+    // nothing ran: record the failure and run the step uncaptured
+    if (g) (void)hipGraphDestroy(g);
+    ctx->g32_warm.clear();
+    (void)hipGetLastError();
+    return run_adjoint(ctx, p);
+  }
+  ctx->g32_graph = g;
+  HIPCHK(hipGraphInstantiate(&ctx->g32_exec, g, nullptr, nullptr, 0));
+  ctx->g32_key = key;
+  HIPCHK(hipGraphLaunch(ctx->g32_exec, ctx->stream));
+  return ST_OK;
+}
+
+static int run_adjoint(st_ctx* ctx, const st_params* p) {
+  int rc;
   std::vector<double> cen;
   build_central(ctx, p, cen);
   st::AdjointInputs in;
@@ -1000,6 +1097,7 @@ int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, c
   ctx->have_soap = ctx->have_cart = ctx->ct_ready = ctx->have_pair64 = false;
   ctx->have_G = false;
   st::adjoint_invalidate(ctx->adj);
+  ++ctx->nb_gen;
   if (flag & 1) return fail(ctx, ST_ERR_UNSUPPORTED, "a site has more than 1024 neighbours");
   if (flag & 2) return fail(ctx, ST_ERR_UNSUPPORTED, "an atom is the neighbour in more than 1024 pairs");
   ctx->n_sites = n_sites;
@@ -1011,6 +1109,7 @@ int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, c
   ctx->have_nb = true;
   ctx->n_atoms = n_atoms;
   ctx->have_atoms = ctx->have_vec = true;
+  ++ctx->nb_gen;
   if (n_pairs) *n_pairs = (int32_t)P;
   return ST_OK;
 }

@@ -164,3 +164,40 @@ def test_adjoint_f32_shapes_match_fp64_oracle(ctx, alpha_max, l_max, kind, rcut)
     print(f"{alpha_max} l {l_max} {kind} rcut {rcut} (max {w.n_neigh.max()} neighbours): "
           f"rel err q {eq:.2e} pair {ep:.2e} forces {ef:.2e} virial {ev:.2e}")
     assert eq <= TOL and ef <= TOL and ev <= TOL and ep <= 10 * TOL
+
+
+def test_adjoint_f32_graph_replays_match_eager_runs(ctx):
+    # graph mode: call 1 runs, call 2 records the step, later calls replay it. Every replay reads the
+    # current dE/dq and neighbour data and must equal an uncaptured run on the same inputs exactly.
+    w = make_workload(2, n_sites=500)
+    keep, prm, cmp_, comp, n_soap, W, S, G = setup(ctx, w)
+    eager = st.Context(0)
+    setup(eager, w)
+
+    def run(c, G_):
+        c.upload_dedq(np.asfortranarray(G_.T))
+        c.adjoint_f32(prm, cmp_)
+        f, v = c.download_forces(1)
+        return c.download_soap_f32(n_soap), c.download_pairs(1), f, v
+
+    ctx.set_graphs(True)
+    try:
+        for call in range(5):
+            G_ = normal(100 + call, G.shape)
+            got, want = run(ctx, G_), run(eager, G_)
+            for a, b in zip(got, want):
+                assert np.array_equal(a, b), f"call {call}"
+        # a new neighbour list of the same sizes (other coordinates): the recorded step is not reused
+        w2 = make_workload(2, n_sites=500)
+        w2.rjs = w2.rjs * 0.97
+        for c in (ctx, eager):
+            c.upload_neighbours(w2.n_neigh, w2.species.T, w2.species_multiplicity, w2.mask.T, w2.rjs, w2.thetas,
+                                w2.phis)
+            c.upload_atoms(len(w2.pos), w2.nbr, w2.vec)
+        for call in range(3):
+            G_ = normal(200 + call, G.shape)
+            got, want = run(ctx, G_), run(eager, G_)
+            for a, b in zip(got, want):
+                assert np.array_equal(a, b), f"call {call} after the new list"
+    finally:
+        ctx.set_graphs(False)
