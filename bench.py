@@ -170,6 +170,7 @@ def main():
     w, keep, prm, cmp_, comp, n_soap, G = build_problem(args.config)
     log(f"sites {w.n_sites} pairs {w.n_pairs} n_soap {n_soap}")
     ctx = st.Context(0)
+    ctx.set_autotune(True)  # launch shapes chosen by timing on the first (warm-up) steps
     ctx.upload_neighbours(w.n_neigh, w.species.T, w.species_multiplicity, w.mask.T, w.rjs, w.thetas, w.phis)
     ctx.upload_atoms(len(w.pos), w.nbr, w.vec)
     ctx.upload_dedq(np.asfortranarray(G.T))
@@ -249,7 +250,7 @@ def main():
         "config": {"workload": f"config {args.config}: {w.n_sites} sites, {w.n_pairs} pairs, n_max {n_max}, "
                                f"l_max {p.l_max}, n_soap {n_soap}, basis {p.basis}",
                    "sites": w.n_sites, "pairs": w.n_pairs, "n_soap": n_soap, "path": metric_path,
-                   "graph": have32 and use_graph},
+                   "graph": have32 and use_graph, "autotune": True},
         "roofline": roofline,
         "fp64_full_jacobian": {"atoms_per_s": w.n_sites / (ms64 / args.steps * 1e-3), "ms_per_step": ms64 / args.steps,
                                "kernels_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kt64.items() if v[1]}},

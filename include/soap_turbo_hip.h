@@ -156,6 +156,11 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c);
  * later calls replay it (new dE/dq or coordinates of the same sizes are read by the replay). Calls
  * made with kernel timing on run uncaptured. */
 int st_set_graphs(st_ctx* ctx, int32_t on);
+/* Launch-shape autotuning (off by default; no reference counterpart): the first call per shape of
+ * st_compute's Jacobian kernel (threads x outputs per thread) and of st_adjoint_f32's site kernel
+ * (workgroups per CU of its persistent grid) times each candidate on the current data and keeps the
+ * fastest for later calls. Results do not depend on the choice beyond summation order. */
+int st_set_autotune(st_ctx* ctx, int32_t on);
 int st_download(st_ctx* ctx, double* soap, double* soap_cart_der);
 /* pair(3, n_pairs) of the last st_pairs_full_jacobian / st_contract_resident (FP64) or st_adjoint_f32. */
 int st_download_pairs(st_ctx* ctx, int32_t which, double* pair);   /* which: 0 FP64, 1 FP32 */

@@ -120,6 +120,10 @@ class Context:
         self.check(lib().st_upload_neighbours(self.h, ctypes.byref(nb)))
         self.n_sites, self.n_pairs = len(n_neigh), len(rjs)
 
+    def set_autotune(self, on: bool):
+        """Time the candidate launch shapes of the Jacobian and site kernels on first use; keep the fastest."""
+        self.check(lib().st_set_autotune(self.h, int(bool(on))))
+
     def set_graphs(self, on: bool):
         """Replay the FP32 step from a captured HIP graph while its inputs keep their shapes."""
         self.check(lib().st_set_graphs(self.h, int(bool(on))))

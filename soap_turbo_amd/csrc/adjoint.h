@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,7 @@ struct AdjointState {
   int n_pf = 0;
   int n_pairs = 0, n_atoms = 0, n_sites = 0, n_soap = 0;
   bool have_out = false, have_forces = false;
+  std::map<std::string, int> tuned;  // site kernel: workgroups per CU chosen per shape (0: occupancy)
 };
 
 struct AdjointInputs {
@@ -73,6 +75,7 @@ struct AdjointInputs {
   int n_atoms;
   size_t lds_max;                        // LDS bytes one workgroup may allocate
   int n_cu;                              // compute units of the device
+  bool autotune = false;                 // choose the site kernel's grid by timing (first call per shape)
 };
 
 using Timer = std::function<void(const char*, const std::function<void()>&)>;

@@ -1093,11 +1093,12 @@ void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order
                  const uint32_t* spmask, const double* rjs, const double* th, const double* ph, const float* G32,
                  const int4* comp, const int* out_ptr, const int* out_c, const float* out_el, const int* unc_ptr,
                  const int* unc_e, const int32_t* P_i, const float* P_el, const int2* odir, float* q, float* sdp,
-                 float* pair, unsigned long long* tickets, int n_cu) {
+                 float* pair, unsigned long long* tickets, int n_cu, int per_cu_cap) {
   const void* fn = reinterpret_cast<const void*>(&adjoint_site_k<L, NTT>);
   if (sh > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTHREADS, sh) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (per_cu_cap > 0 && per_cu > per_cu_cap) per_cu = per_cu_cap;
   const int grid = std::max(1, std::min(P.n_sites, per_cu * n_cu));
   adjoint_site_k<L, NTT><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask, rjs, th,
                                                      ph, G32, comp, out_ptr, out_c, out_el, unc_ptr, unc_e, P_i,
@@ -1422,24 +1423,60 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
       return ST_ERR_HIP;
     }
   }
-  bool ok = true;
-  timer("adj_site_f32", [&] {
-    const int ntt = (in.n_max + 15) / 16;
+  const int ntt = (in.n_max + 15) / 16;
+  // the site kernel with at most per_cu_cap workgroups per CU in its persistent grid (0: occupancy)
+  auto site = [&](int per_cu_cap) {
     switch (in.l_max * 4 + ntt) {
 #define LS1(l, t)                                                                                                  \
   case l * 4 + t:                                                                                                  \
     launch_site<l, t>(st, P, sh, s.order.as<int>(), in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cenv, \
                       in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(), in.comp, in.out_ptr, in.out_c, oel, \
                       in.unc_ptr, in.unc_e, in.P_i, pel, odir, s.soap.as<float>(), s.sdp.as<float>(),              \
-                      s.pair.as<float>(), s.tickets.as<unsigned long long>(), in.n_cu);                          \
-    break;
+                      s.pair.as<float>(), s.tickets.as<unsigned long long>(), in.n_cu, per_cu_cap);              \
+    return true;
 #define LS(l) LS1(l, 1) LS1(l, 2) LS1(l, 3)
       LS(0) LS(1) LS(2) LS(3) LS(4) LS(5) LS(6) LS(7) LS(8) LS(9) LS(10) LS(11) LS(12)
 #undef LS
 #undef LS1
-      default: ok = false;
+      default: return false;
     }
-  });
+  };
+  int cap = 0;
+  if (in.autotune) {
+    // the persistent grid's workgroups per CU: the occupancy limit or two, whichever runs faster
+    const std::string key = std::to_string(in.l_max) + " " + std::to_string(ntt) + " " + std::to_string(sh) + " " +
+                            std::to_string(in.n_soap) + " " + std::to_string(in.n_sites / 256);
+    auto it = s.tuned.find(key);
+    if (it != s.tuned.end()) {
+      cap = it->second;
+    } else {
+      hipEvent_t a = nullptr, b = nullptr;
+      float best = 0.f;
+      bool any = false;
+      if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+        for (int c : {0, 2}) {
+          site(c);  // warm-up
+          launch_counter_reset(st, s.tickets.as<unsigned long long>());
+          (void)hipEventRecord(a, st);
+          site(c);
+          (void)hipEventRecord(b, st);
+          launch_counter_reset(st, s.tickets.as<unsigned long long>());
+          float ms = 0.f;
+          if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) continue;
+          if (!any || ms < best) {
+            best = ms;
+            cap = c;
+            any = true;
+          }
+        }
+      }
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+      s.tuned[key] = cap;
+    }
+  }
+  bool ok = true;
+  timer("adj_site_f32", [&] { ok = site(cap); });
   if (!ok) {
     msg = "l_max out of the compiled range";
     return ST_ERR_UNSUPPORTED;

@@ -83,6 +83,9 @@ struct st_ctx {
   // same key, replayed while the key holds
   bool graphs = false;
   uint64_t nb_gen = 0;  // bumped by every new neighbour list or atom upload
+  // launch-shape autotuning (st_set_autotune): the fastest variant per shape key
+  bool autotune = false;
+  std::map<std::string, int> tuned;
   std::string g32_key, g32_warm;
   hipGraph_t g32_graph = nullptr;
   hipGraphExec_t g32_exec = nullptr;
@@ -133,6 +136,11 @@ int upload(st_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
   if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   return ST_OK;
 }
+
+// The fastest of n launch variants of one kernel, timed on the current data (a warm-up launch, then
+// one timed with events, per variant), cached under key. launch(v) returns false for a variant that
+// cannot run (it is skipped).
+int tuned_variant(st_ctx* ctx, const std::string& key, int n, const std::function<bool(int)>& launch);
 
 hipEvent_t get_event(st_ctx* ctx) {
   if (!ctx->free_events.empty()) {
@@ -760,17 +768,26 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
   if (der && !ctr) {
     if (st::derivatives_lds(n_max, L, k_max, n_soap) > ctx->lds_max)
       return fail(ctx, ST_ERR_UNSUPPORTED, "the derivative working set exceeds the LDS");
-    timed(ctx, "derivatives_f64", [&] {
-      ok = st::launch_derivatives(st, NS, (int)P, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(), ctx->comp.as<int4>(), optr,
-                                  ctx->out_c.as<int>(), ctx->out_el.as<double>(),
-                                  ctx->single ? ctx->ocomp.as<int4>() : nullptr, ctx->oel.as<double>(),
-                                  ctx->cnk.as<double2>(),
-                                  ctx->rad.as<double>(), ctx->rad_der.as<double>(), ctx->ang.as<double2>(),
-                                  ctx->ang_r.as<double2>(), ctx->ang_a.as<double2>(), ctx->ang_p.as<double2>(),
-                                  ctx->soap.as<double>(), ctx->sdp.as<double>(), ctx->rjs.as<double>(),
-                                  ctx->thetas.as<double>(), ctx->phis.as<double>(), ctx->rotp.as<double>(),
-                                  ctx->cart.as<double>());
-    });
+    auto jacobian = [&](int variant) {
+      if (st::derivatives_lds(n_max, L, k_max, n_soap, variant) > ctx->lds_max) return false;
+      return st::launch_derivatives(st, NS, (int)P, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(),
+                                    ctx->comp.as<int4>(), optr, ctx->out_c.as<int>(), ctx->out_el.as<double>(),
+                                    ctx->single ? ctx->ocomp.as<int4>() : nullptr, ctx->oel.as<double>(),
+                                    ctx->cnk.as<double2>(), ctx->rad.as<double>(), ctx->rad_der.as<double>(),
+                                    ctx->ang.as<double2>(), ctx->ang_r.as<double2>(), ctx->ang_a.as<double2>(),
+                                    ctx->ang_p.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>(),
+                                    ctx->rjs.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
+                                    ctx->rotp.as<double>(), ctx->cart.as<double>(), variant);
+    };
+    int variant = 0;
+    if (ctx->autotune && !ctx->timing) {
+      // the kernel's shape (threads x outputs per thread) for this output count, basis and batch size
+      const std::string key = "jacobian " + std::to_string(n_soap) + " " + std::to_string(n_max) + " " +
+                              std::to_string(L) + " " + std::to_string(ctx->single) + " " +
+                              std::to_string(NS / 256);
+      variant = tuned_variant(ctx, key, st::derivatives_variants(n_soap), jacobian);
+    }
+    timed(ctx, "derivatives_f64", [&] { ok = jacobian(variant); });
     if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "n_soap > 3072 is not supported with derivatives");
   }
   timed(ctx, "normalise_f64", [&] { st::launch_normalise(st, NS, n_soap, ctx->sdp.as<double>(), ctx->soap.as<double>()); });
@@ -908,6 +925,44 @@ void drop_graph(st_ctx* ctx) {
 
 }  // namespace
 
+namespace {
+
+int tuned_variant(st_ctx* ctx, const std::string& key, int n, const std::function<bool(int)>& launch) {
+  auto it = ctx->tuned.find(key);
+  if (it != ctx->tuned.end()) return it->second;
+  int best = 0;
+  if (n > 1) {
+    hipEvent_t a = get_event(ctx), b = get_event(ctx);
+    float tbest = 0.f;
+    bool any = false;
+    for (int v = 0; v < n; ++v) {
+      if (!launch(v)) continue;
+      (void)hipEventRecord(a, ctx->stream);
+      launch(v);
+      (void)hipEventRecord(b, ctx->stream);
+      float ms = 0.f;
+      if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) continue;
+      if (!any || ms < tbest) {
+        tbest = ms;
+        best = v;
+        any = true;
+      }
+    }
+    ctx->free_events.push_back(a);
+    ctx->free_events.push_back(b);
+  }
+  ctx->tuned[key] = best;
+  return best;
+}
+
+}  // namespace
+
+int st_set_autotune(st_ctx* ctx, int32_t on) {
+  if (!ctx) return ST_ERR_ARG;
+  ctx->autotune = on != 0;
+  return ST_OK;
+}
+
 int st_set_graphs(st_ctx* ctx, int32_t on) {
   if (!ctx) return ST_ERR_ARG;
   ctx->graphs = on != 0;
@@ -1003,6 +1058,7 @@ static int run_adjoint(st_ctx* ctx, const st_params* p) {
   in.tables_key = &ctx->tables_key;
   in.lds_max = ctx->lds_max;
   in.n_cu = ctx->n_cu;
+  in.autotune = ctx->autotune && !ctx->timing;
   {
     int off = 0;
     for (int s = 0; s < p->n_species; ++s) {

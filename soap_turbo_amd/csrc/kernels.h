@@ -22,14 +22,17 @@ void launch_power(hipStream_t st, int n_sites, int n_max, int k_max, int n_soap,
                   const int* out_c, const double* out_el, const int4* ocomp, const double* oel, const double2* cnk,
                   double* soap, double* sqrt_dot_p);
 void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqrt_dot_p, double* soap);
-size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap);
+// launch shapes (threads x outputs per thread) of the Jacobian kernel for n_soap outputs; variant 0
+// (or -1) is the default
+int derivatives_variants(int n_soap);
+size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap, int variant = -1);
 bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int k_max, int n_soap,
                         const int* site_ptr, const int4* comp, const int* out_ptr, const int* out_c,
                         const double* out_el, const int4* ocomp, const double* oel,
                         const double2* cnk, const double* rad, const double* rad_der, const double2* ang,
                         const double2* ang_r, const double2* ang_a, const double2* ang_p, const double* soap,
                         const double* sqrt_dot_p, const double* rjs, const double* thetas, const double* phis,
-                        double* rotp, double* cart);
+                        double* rotp, double* cart, int variant = -1);
 void launch_contract_cart(hipStream_t st, int n_sites, int n_pairs, int n_soap, const int* site_ptr, const double* G,
                           const double* cart, double* pair);
 size_t contract_lds(int n_max, int k_max, int n_soap, int n_soap_unc);

@@ -1104,23 +1104,47 @@ void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqr
   normalise_k<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n_sites, n_soap, sqrt_dot_p, soap);
 }
 
-// (outputs per thread, threads per block) for n_soap outputs: 256 threads up
-// to 768 outputs, 512 threads up to 3072.
-static void der_config(int n_soap, int& nspt, int& nt) {
-  nt = n_soap <= ST_DER_NT256_MAX ? 256 : 512;
-  nspt = (n_soap + nt - 1) / nt;
-  if (nspt == 5) nspt = 6;
+// (outputs per thread, threads per block) shapes of the Jacobian kernel for n_soap outputs: the
+// default first (256 threads up to 768 outputs, 512 threads up to 3072), then the other thread
+// count when a compiled shape covers n_soap with it.
+static int der_shapes(int n_soap, int (*shape)[2]) {
+  int n = 0;
+  const int first = n_soap <= ST_DER_NT256_MAX ? 256 : 512;
+  for (int nt : {first, 768 - first}) {
+    int nspt = (n_soap + nt - 1) / nt;
+    if (nspt == 5) nspt = 6;
+    const bool compiled = nt == 256 ? nspt <= 3 : (nspt >= 2 && nspt <= 6);
+    if (compiled) {
+      shape[n][0] = nspt;
+      shape[n][1] = nt;
+      ++n;
+    }
+  }
+  return n;
 }
 
-static int der_pb(int n_soap) {
+static void der_config(int n_soap, int variant, int& nspt, int& nt) {
+  int sh[2][2] = {{1, 256}, {1, 256}};
+  const int n = der_shapes(n_soap, sh);
+  const int v = variant >= 0 && variant < n ? variant : 0;
+  nspt = sh[v][0];
+  nt = sh[v][1];
+}
+
+static int der_pb(int n_soap, int variant) {
   int nspt, nt;
-  der_config(n_soap, nspt, nt);
+  der_config(n_soap, variant, nspt, nt);
   return nspt == 1 ? DerShape<1>::PB : nspt == 2 ? DerShape<2>::PB : nspt == 3 ? DerShape<3>::PB
        : nspt == 4 ? DerShape<4>::PB : DerShape<6>::PB;
 }
 
-size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap) {
-  const int pb = der_pb(n_soap);
+int derivatives_variants(int n_soap) {
+  int sh[2][2];
+  return n_soap > 3072 ? 0 : der_shapes(n_soap, sh);
+}
+
+size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap, int variant) {
+  const int pb = der_pb(n_soap, variant);
   return sizeof(double2) * (n_max * k_max + 2 * k_max * (4 * pb + 1)) +
          sizeof(double) * (2 * n_max * (2 * pb + 1) + n_max * (l_max + 1) * (4 * pb + 1));
 }
@@ -1131,13 +1155,13 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int
                         const double2* cnk, const double* rad, const double* rad_der, const double2* ang,
                         const double2* ang_r, const double2* ang_a, const double2* ang_p, const double* soap,
                         const double* sqrt_dot_p, const double* rjs, const double* thetas, const double* phis,
-                        double* rotp, double* cart) {
+                        double* rotp, double* cart, int variant) {
   if (n_sites == 0) return true;
   if (n_soap > 3072) return false;
   if (n_pairs > 0) rotation_k<<<(n_pairs + 255) / 256, 256, 0, st>>>(n_pairs, rjs, thetas, phis, rotp);
   int nspt, nt;
-  der_config(n_soap, nspt, nt);
-  size_t sh = derivatives_lds(n_max, l_max, k_max, n_soap);
+  der_config(n_soap, variant, nspt, nt);
+  size_t sh = derivatives_lds(n_max, l_max, k_max, n_soap, variant);
 #define DL(N, T)                                                                                                  \
   if (nspt == N && nt == T) {                                                                                     \
     if (sh > 64 * 1024) {                                                                                         \
