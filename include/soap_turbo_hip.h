@@ -161,6 +161,11 @@ int st_set_graphs(st_ctx* ctx, int32_t on);
  * (workgroups per CU of its persistent grid) times each candidate on the current data and keeps the
  * fastest for later calls. Results do not depend on the choice beyond summation order. */
 int st_set_autotune(st_ctx* ctx, int32_t on);
+/* The Jacobian kernel's launch variants for n_soap outputs (the batched kernel's shapes, then the
+ * wave-per-pair kernel where it applies), and a fixed choice among them (-1: the default, or the
+ * tuned one); for tests and tuning studies. */
+int st_jacobian_variants(st_ctx* ctx, int32_t n_soap);
+int st_set_jacobian_variant(st_ctx* ctx, int32_t variant);
 int st_download(st_ctx* ctx, double* soap, double* soap_cart_der);
 /* pair(3, n_pairs) of the last st_pairs_full_jacobian / st_contract_resident (FP64) or st_adjoint_f32. */
 int st_download_pairs(st_ctx* ctx, int32_t which, double* pair);   /* which: 0 FP64, 1 FP32 */

@@ -124,6 +124,13 @@ class Context:
         """Time the candidate launch shapes of the Jacobian and site kernels on first use; keep the fastest."""
         self.check(lib().st_set_autotune(self.h, int(bool(on))))
 
+    def jacobian_variants(self, n_soap: int) -> int:
+        return lib().st_jacobian_variants(self.h, int(n_soap))
+
+    def set_jacobian_variant(self, variant: int):
+        """Fix the Jacobian kernel's launch variant (-1: default or tuned)."""
+        self.check(lib().st_set_jacobian_variant(self.h, int(variant)))
+
     def set_graphs(self, on: bool):
         """Replay the FP32 step from a captured HIP graph while its inputs keep their shapes."""
         self.check(lib().st_set_graphs(self.h, int(bool(on))))

@@ -59,6 +59,8 @@ SIGNATURES = {
     "st_adjoint_f32": (ctypes.c_int, [P, P, P]),
     "st_set_graphs": (ctypes.c_int, [P, I32]),
     "st_set_autotune": (ctypes.c_int, [P, I32]),
+    "st_jacobian_variants": (ctypes.c_int, [P, I32]),
+    "st_set_jacobian_variant": (ctypes.c_int, [P, I32]),
     "st_download": (ctypes.c_int, [P, P, P]),
     "st_download_pairs": (ctypes.c_int, [P, I32, P]),
     "st_download_forces": (ctypes.c_int, [P, I32, P, P]),

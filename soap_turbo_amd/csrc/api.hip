@@ -86,6 +86,7 @@ struct st_ctx {
   // launch-shape autotuning (st_set_autotune): the fastest variant per shape key
   bool autotune = false;
   std::map<std::string, int> tuned;
+  int jacobian_variant = -1;  // >= 0: the Jacobian kernel's launch variant, fixed (st_set_jacobian_variant)
   std::string g32_key, g32_warm;
   hipGraph_t g32_graph = nullptr;
   hipGraphExec_t g32_exec = nullptr;
@@ -780,7 +781,11 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
                                     ctx->rotp.as<double>(), ctx->cart.as<double>(), variant);
     };
     int variant = 0;
-    if (ctx->autotune && !ctx->timing) {
+    if (ctx->jacobian_variant >= 0) {
+      if (ctx->jacobian_variant >= st::derivatives_variants(n_soap))
+        return fail(ctx, ST_ERR_ARG, "no such Jacobian launch variant for this n_soap");
+      variant = ctx->jacobian_variant;
+    } else if (ctx->autotune && !ctx->timing) {
       // the kernel's shape (threads x outputs per thread) for this output count, basis and batch size
       const std::string key = "jacobian " + std::to_string(n_soap) + " " + std::to_string(n_max) + " " +
                               std::to_string(L) + " " + std::to_string(ctx->single) + " " +
@@ -956,6 +961,17 @@ int tuned_variant(st_ctx* ctx, const std::string& key, int n, const std::functio
 }
 
 }  // namespace
+
+int st_set_jacobian_variant(st_ctx* ctx, int32_t variant) {
+  if (!ctx) return ST_ERR_ARG;
+  ctx->jacobian_variant = variant < 0 ? -1 : variant;
+  return ST_OK;
+}
+
+int st_jacobian_variants(st_ctx* ctx, int32_t n_soap) {
+  if (!ctx) return -1;
+  return st::derivatives_variants(n_soap);
+}
 
 int st_set_autotune(st_ctx* ctx, int32_t on) {
   if (!ctx) return ST_ERR_ARG;

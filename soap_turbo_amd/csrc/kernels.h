@@ -22,8 +22,9 @@ void launch_power(hipStream_t st, int n_sites, int n_max, int k_max, int n_soap,
                   const int* out_c, const double* out_el, const int4* ocomp, const double* oel, const double2* cnk,
                   double* soap, double* sqrt_dot_p);
 void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqrt_dot_p, double* soap);
-// launch shapes (threads x outputs per thread) of the Jacobian kernel for n_soap outputs; variant 0
-// (or -1) is the default
+// launch variants of the Jacobian for n_soap outputs: the batched kernel's shapes (threads x outputs
+// per thread; variant 0, or -1, is the default), then the wave-per-pair kernel when a compiled
+// shape covers n_soap
 int derivatives_variants(int n_soap);
 size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap, int variant = -1);
 bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int k_max, int n_soap,

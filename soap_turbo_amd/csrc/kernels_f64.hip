@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace st {
@@ -1104,6 +1106,190 @@ void launch_normalise(hipStream_t st, int n_sites, int n_soap, const double* sqr
   normalise_k<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n_sites, n_soap, sqrt_dot_p, soap);
 }
 
+// The full Jacobian with one wavefront per pair (soap_derivatives_from_expansion,
+// src/soap_turbo.f90:1927-2075, and the normalisation and rotation of :837-895): a workgroup of
+// four wavefronts per site holds the site's cnk in LDS; each wavefront takes every fourth pair,
+// stages its angular rows, radial values and rotation in its own LDS slice, forms the P, Q, Qa,
+// Qo(n, l) table of the pair (sum over m of w(m) Re(conj(A) c)), then its lanes compute the
+// pair's n_soap outputs (NSW per lane), reduce q . d across the wavefront with no workgroup
+// barrier, and store the normalised, rotated column. The site's own column (minus the sum of the
+// others) is combined from the four wavefronts' partials at the end.
+template <int NSW, bool DIRECT>
+__global__ __launch_bounds__(256, 2) void jacobian_wave_k(
+    int n_sites, int n_max, int l_max, int k_max, int n_soap, const int* __restrict__ site_ptr,
+    const int4* __restrict__ comp, const int* __restrict__ out_ptr, const int* __restrict__ out_c,
+    const double* __restrict__ out_el, const int4* __restrict__ ocomp, const double* __restrict__ oel,
+    const double2* __restrict__ cnk, const double* __restrict__ rad, const double* __restrict__ rad_der,
+    const double2* __restrict__ ang, const double2* __restrict__ ang_r, const double2* __restrict__ ang_a,
+    const double2* __restrict__ ang_p, const double* __restrict__ soap, const double* __restrict__ sqrt_dot_p,
+    const double* __restrict__ rotp, double* __restrict__ cart) {
+  extern __shared__ double2 smw[];
+  const int i = blockIdx.x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int L1 = l_max + 1, nl = n_max * L1;
+  double2* cn = smw;                                                  // n_max * k_max (site)
+  double2* sA = cn + n_max * k_max + w * 4 * k_max;                   // [kind][k] of this wave's pair
+  double* dbase = reinterpret_cast<double*>(cn + n_max * k_max + 4 * 4 * k_max);
+  double* sR = dbase + w * 2 * n_max;                                 // r[n], r'[n]
+  double* PQ = dbase + 4 * 2 * n_max + w * 4 * nl;                    // [(n L1 + l) 4 + kind]
+  double* srot = dbase + 4 * 2 * n_max + 4 * 4 * nl + w * 8;          // rotation of the pair
+  double* sq = dbase + 4 * 2 * n_max + 4 * 4 * nl + 32;               // per output: q (unnormalised)
+  double* sel = sq + n_soap;                                          //   its compression element
+  int* scm = reinterpret_cast<int*>(sel + n_soap);                    //   its packed component
+  double* selfsum = reinterpret_cast<double*>(smw);                   // 3 n_soap, over all at the end
+  const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
+  for (int t = threadIdx.x; t < n_max * k_max; t += 256) cn[t] = cnk[(size_t)i * n_max * k_max + t];
+  const double sdp = sqrt_dot_p[i];
+  const double inv_p = 1.0 / sdp, inv_p3 = 1.0 / (sdp * sdp * sdp);
+  const bool scaled = out_ptr != nullptr;
+  for (int s = threadIdx.x; s < n_soap; s += 256) {
+    sq[s] = soap[(size_t)i * n_soap + s];
+    if (DIRECT) {
+      scm[s] = pack_component(ocomp[s]);
+      sel[s] = oel[s];
+    } else {
+      scm[s] = scaled ? 0 : pack_component(comp[s]);
+      sel[s] = 0.0;
+    }
+  }
+  double self[NSW][3];
+#pragma unroll
+  for (int j = 0; j < NSW; ++j) self[j][0] = self[j][1] = self[j][2] = 0.0;
+  __syncthreads();
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  // one output's (d_rad, d_azi, d_pol) from the pair's table, as der_component
+  auto component = [&](int cm, double& tr, double& ta, double& to) {
+    const int n = cm & 1023, np = (cm >> 10) & 1023, l = cm >> 20;
+    const double* A = PQ + (np * L1 + l) * 4;
+    const double* C = PQ + (n * L1 + l) * 4;
+    const double pre = (4.0 * 3.141592653589793) * (n == np ? 1.0 : 1.4142135623730951);
+    tr = pre * (sR[n_max + n] * A[0] + sR[n] * A[1] + sR[n_max + np] * C[0] + sR[np] * C[1]);
+    ta = pre * (sR[n] * A[2] + sR[np] * C[2]);
+    to = pre * (sR[n] * A[3] + sR[np] * C[3]);
+  };
+  for (int p = p0 + 1 + w; p < p1; p += 4) {
+    for (int t = lane; t < 4 * k_max; t += 64) {
+      const int kind = t / k_max, k = t - kind * k_max;
+      const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
+      sA[t] = src[(size_t)p * k_max + k];
+    }
+    for (int t = lane; t < 2 * n_max; t += 64) sR[t] = (t < n_max ? rad : rad_der)[(size_t)p * n_max + t % n_max];
+    if (lane < 8) srot[lane] = rotp[(size_t)p * 8 + lane];
+    wave_sync();
+    // P, Q, Qa, Qo(n, l) = sum over m <= l of w(m) Re(conj(A_kind(l, m)) c(n, l, m))
+    for (int e = lane; e < nl; e += 64) {
+      const int n = e / L1, l = e - n * L1, kb = l * (l + 1) / 2;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int m = 0; m <= l; ++m) {
+        const double2 c = cn[n * k_max + kb + m];
+        const double wm = m == 0 ? 1.0 : 2.0;
+#pragma unroll
+        for (int kind = 0; kind < 4; ++kind) {
+          const double2 a = sA[kind * k_max + kb + m];
+          acc[kind] = acc[kind] + wm * (a.x * c.x + a.y * c.y);
+        }
+      }
+#pragma unroll
+      for (int kind = 0; kind < 4; ++kind) PQ[e * 4 + kind] = acc[kind];
+    }
+    wave_sync();
+    // output j's (d_rad, d_azi, d_pol) for the pair, through its compression entries
+    auto output = [&](int j, double& vr, double& va, double& vo) {
+      const int s = lane + 64 * j;
+      vr = va = vo = 0.0;
+      if (s >= n_soap) return;
+      if (DIRECT || !scaled) {
+        double tr, ta, to;
+        component(scm[s], tr, ta, to);
+        if (DIRECT && scaled) {
+          const double el = sel[s];
+          vr = 0.0 + el * tr; va = 0.0 + el * ta; vo = 0.0 + el * to;
+        } else {
+          vr = tr; va = ta; vo = to;
+        }
+      } else {
+        for (int e = out_ptr[s]; e < out_ptr[s + 1]; ++e) {
+          double tr, ta, to;
+          component(pack_component(comp[out_c[e]]), tr, ta, to);
+          const double el = out_el[e];
+          vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
+        }
+      }
+    };
+    // q . d (the components are formed again below rather than held: registers for occupancy)
+    double dots[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NSW; ++j) {
+      double vr, va, vo;
+      output(j, vr, va, vo);
+      const double q = lane + 64 * j < n_soap ? sq[lane + 64 * j] : 0.0;
+      dots[0] += q * vr; dots[1] += q * va; dots[2] += q * vo;
+      if (j & 1) __builtin_amdgcn_sched_barrier(0);  // two outputs' loads in flight at a time
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      dots[0] += __shfl_xor(dots[0], o, 64);
+      dots[1] += __shfl_xor(dots[1], o, 64);
+      dots[2] += __shfl_xor(dots[2], o, 64);
+    }
+    const double rr[8] = {srot[0], srot[1], srot[2], srot[3], srot[4], srot[5], srot[6], srot[7]};
+#pragma unroll
+    for (int j = 0; j < NSW; ++j) {
+      const int s = lane + 64 * j;
+      if (s < n_soap) {
+        double dr, da, dd;
+        output(j, dr, da, dd);
+        const double sv = sq[s];
+        const double vr = dr * inv_p - sv * inv_p3 * dots[0];
+        const double va = da * inv_p - sv * inv_p3 * dots[1];
+        const double vo = dd * inv_p - sv * inv_p3 * dots[2];
+        const double c0 = rr[0] * vr + rr[3] * vo + rr[6] * va;
+        const double c1 = rr[1] * vr + rr[4] * vo + rr[7] * va;
+        const double c2 = rr[2] * vr + rr[5] * vo;
+        double* Cc = cart + ((size_t)p * n_soap + s) * 3;
+        Cc[0] = c0; Cc[1] = c1; Cc[2] = c2;
+        self[j][0] = self[j][0] - c0;
+        self[j][1] = self[j][1] - c1;
+        self[j][2] = self[j][2] - c2;
+      }
+      if (j & 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    wave_sync();  // the slice is restaged for the next pair
+  }
+  // the site's own column: the wavefronts' partial sums, added in wavefront order
+  __syncthreads();
+  for (int q = 0; q < 4; ++q) {
+    if (w == q) {
+#pragma unroll
+      for (int j = 0; j < NSW; ++j) {
+        const int s = lane + 64 * j;
+        if (s < n_soap)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) selfsum[3 * s + c] = q == 0 ? self[j][c] : selfsum[3 * s + c] + self[j][c];
+      }
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < 3 * n_soap; t += 256) cart[(size_t)p0 * n_soap * 3 + t] = selfsum[t];
+}
+
+// LDS of jacobian_wave_k: the site's cnk, four wavefront slices (angular rows, radial values,
+// the P/Q table, the rotation), the outputs' q, elements and components, and at the end the site
+// column's 3 n_soap partial sums over all of it
+static size_t jacobian_wave_lds(int n_max, int l_max, int k_max, int n_soap) {
+  const size_t nl = (size_t)n_max * (l_max + 1);
+  const size_t main = sizeof(double2) * ((size_t)n_max * k_max + 16 * (size_t)k_max) +
+                      sizeof(double) * (8 * (size_t)n_max + 16 * nl + 32) + 20 * (size_t)n_soap;
+  return std::max(main, sizeof(double) * 3 * (size_t)n_soap);
+}
+
+// outputs per lane of the compiled jacobian_wave_k shapes (0: none covers n_soap)
+static int jacobian_wave_nsw(int n_soap) { return n_soap <= 384 ? 6 : n_soap <= 640 ? 10 : n_soap <= 832 ? 13 : 0; }
+
 // (outputs per thread, threads per block) shapes of the Jacobian kernel for n_soap outputs: the
 // default first (256 threads up to 768 outputs, 512 threads up to 3072), then the other thread
 // count when a compiled shape covers n_soap with it.
@@ -1140,11 +1326,20 @@ static int der_pb(int n_soap, int variant) {
 
 int derivatives_variants(int n_soap) {
   int sh[2][2];
-  return n_soap > 3072 ? 0 : der_shapes(n_soap, sh);
+  return n_soap > 3072 ? 0 : der_shapes(n_soap, sh) + (jacobian_wave_nsw(n_soap) > 0);
 }
 
+// variants: the batched kernel's shapes (der_shapes; the first is the default), then the
+// wave-per-pair kernel when a compiled shape covers n_soap
+static bool is_wave_variant(int n_soap, int variant) {
+  int sh[2][2];
+  return variant > 0 && variant == der_shapes(n_soap, sh) && jacobian_wave_nsw(n_soap) > 0;
+}
+static int batched_variant(int, int variant) { return variant < 0 ? 0 : variant; }
+
 size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap, int variant) {
-  const int pb = der_pb(n_soap, variant);
+  if (is_wave_variant(n_soap, variant)) return jacobian_wave_lds(n_max, l_max, k_max, n_soap);
+  const int pb = der_pb(n_soap, batched_variant(n_soap, variant));
   return sizeof(double2) * (n_max * k_max + 2 * k_max * (4 * pb + 1)) +
          sizeof(double) * (2 * n_max * (2 * pb + 1) + n_max * (l_max + 1) * (4 * pb + 1));
 }
@@ -1159,8 +1354,34 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int
   if (n_sites == 0) return true;
   if (n_soap > 3072) return false;
   if (n_pairs > 0) rotation_k<<<(n_pairs + 255) / 256, 256, 0, st>>>(n_pairs, rjs, thetas, phis, rotp);
+  if (is_wave_variant(n_soap, variant)) {
+    const size_t shw = jacobian_wave_lds(n_max, l_max, k_max, n_soap);
+    const int nsw = jacobian_wave_nsw(n_soap);
+#define JW(N)                                                                                                     \
+  if (nsw == N) {                                                                                                 \
+    const void* f1 = reinterpret_cast<const void*>(&jacobian_wave_k<N, true>);                                    \
+    const void* f0 = reinterpret_cast<const void*>(&jacobian_wave_k<N, false>);                                   \
+    if (shw > 64 * 1024) {                                                                                        \
+      (void)hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shw);                       \
+      (void)hipFuncSetAttribute(f0, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shw);                       \
+    }                                                                                                             \
+    if (ocomp)                                                                                                    \
+      jacobian_wave_k<N, true><<<n_sites, 256, shw, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,   \
+                                                          out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der,  \
+                                                          ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rotp, cart); \
+    else                                                                                                          \
+      jacobian_wave_k<N, false><<<n_sites, 256, shw, st>>>(n_sites, n_max, l_max, k_max, n_soap, site_ptr, comp,  \
+                                                           out_ptr, out_c, out_el, ocomp, oel, cnk, rad, rad_der, \
+                                                           ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rotp,      \
+                                                           cart);                                                 \
+    return true;                                                                                                  \
+  }
+    JW(6) JW(10) JW(13)
+#undef JW
+    return false;
+  }
   int nspt, nt;
-  der_config(n_soap, variant, nspt, nt);
+  der_config(n_soap, batched_variant(n_soap, variant), nspt, nt);
   size_t sh = derivatives_lds(n_max, l_max, k_max, n_soap, variant);
 #define DL(N, T)                                                                                                  \
   if (nspt == N && nt == T) {                                                                                     \
