@@ -22,6 +22,9 @@
  *   st_adjoint_f32          no reference counterpart: dE/dq -> pair gradients, forces and virial in FP32
  *                           without forming soap_cart_der (the FP32 form of soap_contract, :1703-1721)
  *   st_build_neighbours     no reference counterpart: the callers' neighbour list, built on the device
+ *   st_set_gap_model, st_gap_energy_forces, st_gap_site_energies
+ *                           no reference counterpart: a GAP-form energy model on the descriptors
+ *                           (what get_soap's callers feed back as dE/dq), evaluated on the device
  *
  * Threading: one context per host thread; calls on a context are ordered on
  * its HIP stream. Like the reference, a context is not reentrant.
@@ -173,6 +176,21 @@ int st_download_pairs(st_ctx* ctx, int32_t which, double* pair);   /* which: 0 F
 int st_download_forces(st_ctx* ctx, int32_t which, double* forces, double* virial);
 /* soap(n_soap, n_sites) of the last st_adjoint_f32, widened to FP64. */
 int st_download_soap_f32(st_ctx* ctx, double* soap);
+
+/* ---- GAP energy model (no reference counterpart: the callers' energy model that supplies get_soap
+ * with dE/dq, here on the device) ----
+ * E = sum_i delta^2 sum_t alpha_t (q_i . q_t)^zeta over the descriptors q_i of the sites and the
+ * model's sparse points q_t, sparse_q(n_soap, n_sparse), alpha(n_sparse); zeta >= 1. Stored in FP32
+ * on the device. */
+int st_set_gap_model(st_ctx* ctx, int32_t n_sparse, int32_t n_soap, const double* sparse_q, const double* alpha,
+                     int32_t zeta, double delta);
+/* One energy-and-forces step on the resident neighbour list: FP32 descriptors, E and dE/dq on the
+ * device, then the adjoint pass (pair gradients, forces and virial as after st_adjoint_f32, read with
+ * st_download_forces(ctx, 1, ...)). energy may be NULL. The resident dE/dq is the model's afterwards:
+ * upload one again before the next st_adjoint_f32. */
+int st_gap_energy_forces(st_ctx* ctx, const st_params* p, const st_compression* c, double* energy);
+/* e_site(n_sites): the site energies of the last st_gap_energy_forces. */
+int st_gap_site_energies(st_ctx* ctx, double* e_site);
 
 int st_synchronize(st_ctx* ctx);
 /* Stream-ordered timing: record event slot (0..15) on the context's stream; elapsed ms between two. */

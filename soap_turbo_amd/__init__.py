@@ -209,6 +209,26 @@ class Context:
         self.check(lib().st_download_soap_f32(self.h, ptr(soap)))
         return soap
 
+    # ---- GAP energy model on the device
+    def set_gap_model(self, sparse_q, alpha, zeta: int, delta: float):
+        """sparse_q (n_soap, n_sparse), alpha (n_sparse): E = sum_i delta^2 sum_t alpha_t (q_i . q_t)^zeta."""
+        T = np.asfortranarray(sparse_q, dtype=np.float64)
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        if a.shape != (T.shape[1],):
+            raise ValueError("alpha needs one weight per sparse point")
+        self.check(lib().st_set_gap_model(self.h, T.shape[1], T.shape[0], ptr(T), ptr(a), int(zeta), float(delta)))
+
+    def gap_energy_forces(self, params: StParams, comp: StCompression) -> float:
+        """Descriptors, the model's energy and dE/dq, then forces and virial (download_forces(1))."""
+        e = ctypes.c_double()
+        self.check(lib().st_gap_energy_forces(self.h, ctypes.byref(params), ctypes.byref(comp), ctypes.byref(e)))
+        return e.value
+
+    def gap_site_energies(self):
+        e = np.zeros(self.n_sites)
+        self.check(lib().st_gap_site_energies(self.h, ptr(e)))
+        return e
+
     def synchronize(self):
         self.check(lib().st_synchronize(self.h))
 

@@ -24,7 +24,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "lib", "llvm", "bin", "flang")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kernels_f64.hip", "adjoint.hip", "adjoint_f32.hip", "neighbours.hip", "api.hip"]
+HIP_SOURCES = ["kernels_f64.hip", "adjoint.hip", "adjoint_f32.hip", "neighbours.hip", "gap.hip", "api.hip"]
 CXX_SOURCES = ["basis.cpp"]
 # packed-FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
@@ -61,7 +61,9 @@ def build_hip(verbose=False):
         objs.append(obj)
     out = os.path.join(HERE, "libsoapturbo_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    # rocBLAS: the GAP energy model's two GEMMs
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-L" + os.path.join(ROCM, "lib"), "-lrocblas",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -31,6 +31,7 @@ struct AdjointState {
   int g32_sites = 0, g32_soap = 0;  // shape of the site-major FP32 dE/dq in G32 (0: none)
   AdjBuf el32, odir;
   AdjBuf cenv;                  // per-site central-atom term (poly3gauss)
+  AdjBuf gapK;                  // GAP kernel values / their weights, n_sites x n_sparse
   bool cen_ok = false;          // cenv holds the sites of the current neighbour list ...
   std::string cen_key;          // ... for the tables of this key
   std::string atoms_key, el_key;
@@ -76,6 +77,7 @@ struct AdjointInputs {
   size_t lds_max;                        // LDS bytes one workgroup may allocate
   int n_cu;                              // compute units of the device
   bool autotune = false;                 // choose the site kernel's grid by timing (first call per shape)
+  bool forward_only = false;             // descriptors only (q, |p|): no backward pass, no forces
 };
 
 using Timer = std::function<void(const char*, const std::function<void()>&)>;
@@ -84,10 +86,18 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
 // dE/dq G(n_sites, n_soap) (FP64, site fastest) -> the site-major FP32 copy the site kernel reads;
 // run when dE/dq is uploaded.
 int adjoint_set_dedq(AdjointState& s, hipStream_t st, int n_sites, int n_soap, const double* G, std::string& msg);
+// the site-major FP32 dE/dq buffer of n_sites x n_soap, to be filled on the device (GAP gradient)
+int adjoint_prepare_dedq(AdjointState& s, int n_sites, int n_soap, std::string& msg);
 int adjoint_download_pairs(AdjointState& s, hipStream_t st, int n_pairs, double* pair, std::string& msg);
 int adjoint_download_forces(AdjointState& s, hipStream_t st, int n_atoms, double* forces, double* virial,
                             std::string& msg);
 int adjoint_download_soap(AdjointState& s, hipStream_t st, int n_sites, int n_soap, double* soap, std::string& msg);
+// GAP-form energy of the FP32 descriptors in s.soap, E = sum_i delta^2 sum_t alpha_t (q_i . q_t)^zeta
+// over sparse points q_t (T: n_sparse x n_soap, FP32, row t contiguous), and its gradient dE/dq
+// written over s.G32 (site-major FP32, as the adjoint pass reads it); site energies in e_site, the
+// total in *energy_dev (FP64, device). Two FP32 GEMMs (rocBLAS) and an elementwise kernel.
+int gap_energy_grad(AdjointState& s, hipStream_t st, void* blas, int n_sites, int n_soap, int n_sparse, const float* T,
+                    const float* alpha, int zeta, double delta, double* e_site, double* energy_dev, std::string& msg);
 // forces(:, a) = -sum of pair(:, k) over pairs with neighbour atom a (atom_ptr/atom_pairs: the
 // pairs of each atom); virial = -sum_k vec_k (x) pair_k, reduced through per-workgroup partials
 // (vpart holds gather_blocks(n_atoms) * 9 doubles).

@@ -160,7 +160,7 @@ void adjoint_invalidate(AdjointState& s) {
 
 void adjoint_release(AdjointState& s) {
   AdjBuf* bufs[] = {&s.rtab, &s.atab, &s.tabinfo, &s.soap, &s.sdp, &s.pair, &s.forces,
-                    &s.virial_part, &s.virial, &s.G32, &s.el32, &s.order, &s.tickets, &s.odir, &s.cenv};
+                    &s.virial_part, &s.virial, &s.G32, &s.el32, &s.order, &s.tickets, &s.odir, &s.cenv, &s.gapK};
   for (AdjBuf* b : bufs)
     if (b->p) {
       (void)hipFree(b->p);

@@ -55,6 +55,7 @@ struct SpeciesTab {
 
 struct AdjParams {
   int n_species, n_max, k_max, l_max, n_soap, n_soap_unc, n_sites, compressed, gauss;
+  int forward_only;  // descriptors only: no backward pass, no pair gradients
   double rcut_max, ang_inv_h;
   float ang_ih;
   int ang_n;
@@ -803,6 +804,15 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       __syncthreads();
     }
   }
+  if (P.forward_only) {
+    // descriptors only: the next site now (the wait publishes it to the loop-top barrier)
+    if (tx == 0) {
+      const unsigned long long t = atomicAdd(tickets, 1ull) + gridDim.x;
+      site_next = t < (unsigned long long)P.n_sites ? (order ? order[t] : (int)t) : -1;
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+    }
+    continue;
+  }
   // the ticket for the next site, taken here so that its latency and the site's lookup below
   // pass while this site finishes
   if (tx == 0) tk = atomicAdd(tickets, 1ull);
@@ -1332,6 +1342,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   int rc;
   timer("adj_tables", [&] { rc = build_tables(s, st, in, P, msg); });
   if (rc) return rc;
+  P.forward_only = in.forward_only ? 1 : 0;
   const size_t sh = site_lds(in.p->n_species, in.n_max, in.l_max, in.n_soap, in.n_soap_unc);
   if (sh > in.lds_max) {
     msg = "the FP32 site working set exceeds the LDS";
@@ -1486,7 +1497,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   s.n_sites = in.n_sites;
   s.n_soap = in.n_soap;
   s.have_forces = false;
-  if (in.pair_atom) {
+  if (in.pair_atom && !in.forward_only) {
     if ((rc = ensure(s.forces, (size_t)in.n_atoms * 24, msg)) ||
         (rc = ensure(s.virial_part, (size_t)gather_blocks(in.n_atoms) * 72, msg)) || (rc = ensure(s.virial, 72, msg))) {
       launch_counter_reset(st, s.tickets.as<unsigned long long>());  // the site queue starts at zero next time
@@ -1516,6 +1527,14 @@ int adjoint_set_dedq(AdjointState& s, hipStream_t st, int n_sites, int n_soap, c
       return ST_ERR_HIP;
     }
   }
+  s.g32_sites = n_sites;
+  s.g32_soap = n_soap;
+  return ST_OK;
+}
+
+int adjoint_prepare_dedq(AdjointState& s, int n_sites, int n_soap, std::string& msg) {
+  int rc;
+  if ((rc = ensure(s.G32, (size_t)n_sites * n_soap * 4 + 4, msg))) return rc;
   s.g32_sites = n_sites;
   s.g32_soap = n_soap;
   return ST_OK;

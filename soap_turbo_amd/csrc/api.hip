@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "../../include/soap_turbo_hip.h"
 #include "adjoint.h"
 #include "kernels.h"
@@ -87,6 +89,11 @@ struct st_ctx {
   bool autotune = false;
   std::map<std::string, int> tuned;
   int jacobian_variant = -1;  // >= 0: the Jacobian kernel's launch variant, fixed (st_set_jacobian_variant)
+  // GAP energy model (st_set_gap_model): sparse points and weights in FP32, site energies
+  rocblas_handle blas = nullptr;
+  int gap_n_sparse = 0, gap_n_soap = 0, gap_zeta = 0;
+  double gap_delta = 0.0;
+  DevBuf gap_T, gap_alpha, gap_esite, gap_energy;
   std::string g32_key, g32_warm;
   hipGraph_t g32_graph = nullptr;
   hipGraphExec_t g32_exec = nullptr;
@@ -564,6 +571,7 @@ void st_destroy(st_ctx* c) {
   for (auto e : c->free_events) (void)hipEventDestroy(e);
   if (c->g32_exec) (void)hipGraphExecDestroy(c->g32_exec);
   if (c->g32_graph) (void)hipGraphDestroy(c->g32_graph);
+  if (c->blas) (void)rocblas_destroy_handle(c->blas);
   st::adjoint_release(c->adj);
   hipStream_t s = c->stream;
   delete c;
@@ -989,7 +997,7 @@ int st_set_graphs(st_ctx* ctx, int32_t on) {
   return ST_OK;
 }
 
-static int run_adjoint(st_ctx* ctx, const st_params* p);
+static int run_adjoint(st_ctx* ctx, const st_params* p, bool forward_only = false);
 
 int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   if (!ctx) return ST_ERR_ARG;
@@ -1032,7 +1040,7 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   return ST_OK;
 }
 
-static int run_adjoint(st_ctx* ctx, const st_params* p) {
+static int run_adjoint(st_ctx* ctx, const st_params* p, bool forward_only) {
   int rc;
   std::vector<double> cen;
   build_central(ctx, p, cen);
@@ -1075,6 +1083,7 @@ static int run_adjoint(st_ctx* ctx, const st_params* p) {
   in.lds_max = ctx->lds_max;
   in.n_cu = ctx->n_cu;
   in.autotune = ctx->autotune && !ctx->timing;
+  in.forward_only = forward_only;
   {
     int off = 0;
     for (int s = 0; s < p->n_species; ++s) {
@@ -1219,6 +1228,68 @@ int st_atom_pairs(st_ctx* ctx, int32_t* atom_ptr, int32_t* atom_pairs) {
     HIPCHK(hipMemcpyAsync(atom_ptr, ctx->atom_ptr.p, ((size_t)ctx->n_atoms + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
   if (atom_pairs)
     HIPCHK(hipMemcpyAsync(atom_pairs, ctx->atom_pairs.p, (size_t)ctx->n_pairs * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return ST_OK;
+}
+
+int st_set_gap_model(st_ctx* ctx, int32_t n_sparse, int32_t n_soap, const double* sparse_q, const double* alpha,
+                     int32_t zeta, double delta) {
+  if (!ctx) return ST_ERR_ARG;
+  if (n_sparse < 1 || n_soap < 1 || !sparse_q || !alpha || zeta < 1)
+    return fail(ctx, ST_ERR_ARG, "bad GAP model (n_sparse, n_soap >= 1, zeta >= 1 and the arrays are required)");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+    return fail(ctx, ST_ERR_HIP, "rocblas_create_handle failed");
+  std::vector<float> T((size_t)n_sparse * n_soap), a(n_sparse);
+  for (size_t k = 0; k < T.size(); ++k) T[k] = (float)sparse_q[k];  // (n_soap, n_sparse), point t contiguous
+  for (int t = 0; t < n_sparse; ++t) a[t] = (float)alpha[t];
+  int rc;
+  if ((rc = upload(ctx, ctx->gap_T, T.data(), T.size() * 4))) return rc;
+  if ((rc = upload(ctx, ctx->gap_alpha, a.data(), a.size() * 4))) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->gap_n_sparse = n_sparse;
+  ctx->gap_n_soap = n_soap;
+  ctx->gap_zeta = zeta;
+  ctx->gap_delta = delta;
+  return ST_OK;
+}
+
+int st_gap_energy_forces(st_ctx* ctx, const st_params* p, const st_compression* c, double* energy) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
+  if (!ctx->gap_n_sparse) return fail(ctx, ST_ERR_STATE, "set the GAP model first");
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  if ((rc = basis_for(ctx, p))) return rc;
+  if ((rc = build_tables(ctx, p, c))) return rc;
+  if (ctx->gap_n_soap != ctx->n_soap) return fail(ctx, ST_ERR_ARG, "the GAP model's descriptor size differs from n_soap");
+  if ((rc = pack_mask(ctx, p->n_species))) return rc;
+  std::string msg;
+  if ((rc = st::adjoint_prepare_dedq(ctx->adj, ctx->n_sites, ctx->n_soap, msg))) return fail(ctx, rc, msg);
+  if ((rc = ensure(ctx, ctx->gap_esite, (size_t)std::max(ctx->n_sites, 1) * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->gap_energy, 8))) return rc;
+  // descriptors, the model's energy and dE/dq on the device, then the adjoint pass for the forces
+  if ((rc = run_adjoint(ctx, p, true))) return rc;
+  timed(ctx, "gap_model", [&] {
+    rc = st::gap_energy_grad(ctx->adj, ctx->stream, ctx->blas, ctx->n_sites, ctx->n_soap, ctx->gap_n_sparse,
+                             ctx->gap_T.as<float>(), ctx->gap_alpha.as<float>(), ctx->gap_zeta, ctx->gap_delta,
+                             ctx->gap_esite.as<double>(), ctx->gap_energy.as<double>(), msg);
+  });
+  if (rc) return fail(ctx, rc, msg);
+  if ((rc = run_adjoint(ctx, p, false))) return rc;
+  ctx->have_G = false;  // dE/dq now holds the model's gradient, not an uploaded one
+  if (energy) {
+    HIPCHK(hipMemcpyAsync(energy, ctx->gap_energy.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  return ST_OK;
+}
+
+int st_gap_site_energies(st_ctx* ctx, double* e_site) {
+  if (!ctx || !e_site) return ST_ERR_ARG;
+  if (!ctx->gap_esite.p) return fail(ctx, ST_ERR_STATE, "no GAP energies computed");
+  HIPCHK(hipMemcpyAsync(e_site, ctx->gap_esite.p, (size_t)ctx->n_sites * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return ST_OK;
 }
