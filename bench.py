@@ -116,6 +116,25 @@ def time_neighbours(w, reps=20):
             "host_cell_list_ms": host_ms, "host_threads": 1}
 
 
+def time_gap(ctx, w, prm, cmp_, n_soap, steps, warmup, n_sparse=1024, zeta=4):
+    """One energy-and-forces step of a GAP model on the device (st_gap_energy_forces): a forward-only
+    site pass, K = Q T^T and dE/dq = W T by rocBLAS, then the adjoint pass. The sparse points are
+    descriptors of this workload's sites (as a fitted model's are), the weights random."""
+    log(f"timing the GAP energy model ({n_sparse} sparse points, zeta {zeta})")
+    from soap_turbo_amd.workload import normal
+    q = ctx.download_soap_f32(n_soap)                           # (n_soap, n_sites)
+    idx = np.arange(n_sparse) * 7919 % w.n_sites
+    T = q[:, idx] + 0.05 * normal(77, (n_soap, n_sparse))
+    T /= np.linalg.norm(T, axis=0, keepdims=True)
+    ctx.set_gap_model(T, normal(78, (n_sparse,)), zeta, 0.5)
+    energy = []
+    ms, _, kt = time_path(ctx, lambda: energy.append(ctx.gap_energy_forces(prm, cmp_)), steps, warmup,
+                          ["adj_site_f32", "gap_model", "adj_forces"])
+    return {"n_sparse": n_sparse, "zeta": zeta, "ms_per_step": ms / steps, "atoms_per_s": w.n_sites / (ms / steps * 1e-3),
+            "kernels_ms_per_step": {k: v[0] / max(steps, 1) for k, v in kt.items() if v[1]},
+            "energy": energy[-1]}
+
+
 def cpu_baseline(w, G, cfg):
     """The reference (oracle/_ref, compiled from its sources) on the same workload, all host cores."""
     from oracle import refio
@@ -262,6 +281,8 @@ def main():
                                "kernels_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kt32.items() if v[1]},
                                "speedup_vs_fp64": (ms64 / ms32),
                                "max_rel_err_forces_vs_fp64": ferr, "max_rel_err_virial_vs_fp64": verr}
+    if have32:
+        out["gap_energy_forces"] = time_gap(ctx, w, prm, cmp_, n_soap, args.steps, args.warmup)
     out["neighbour_list"] = time_neighbours(w)
     if not args.no_cpu:
         log("CPU baseline")

@@ -55,7 +55,6 @@ struct SpeciesTab {
 
 struct AdjParams {
   int n_species, n_max, k_max, l_max, n_soap, n_soap_unc, n_sites, compressed, gauss;
-  int forward_only;  // descriptors only: no backward pass, no pair gradients
   double rcut_max, ang_inv_h;
   float ang_ih;
   int ang_n;
@@ -256,7 +255,9 @@ struct SiteShape {
 // LDS row strides: PTF (= 4 mod 64 words) for the forward R and A rows, so the
 // 16 x 4 MFMA operand reads hit 64 distinct banks; PTS (odd) elsewhere; C rows
 // CSTR + 4 and H rows 2 KMP + 16 for the same reason.
-template <int L, int NTT>
+// FO: descriptors only (q and |p|; no dE/dq read, no backward pass), a separate instantiation so
+// that the full kernel's code is the same as without it.
+template <int L, int NTT, bool FO>
 __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_site_k(
     AdjParams P, const int* __restrict__ order, const int* __restrict__ site_ptr, const float* __restrict__ rtab,
     const float* __restrict__ atab, const float* __restrict__ preflm_g, const float* __restrict__ cenv,
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     for (int u = 0; u < 3; ++u) {
       const int s = tx + NTHREADS * u;
       od[u] = s < n_soap ? odir[s] : make_int2(0, 0);
-      gd[u] = s < n_soap ? G32[(size_t)i * n_soap + s] : 0.f;
+      gd[u] = !FO && s < n_soap ? G32[(size_t)i * n_soap + s] : 0.f;
     }
     for (int t = tx; t < ny; t += NTHREADS) Y[t] = 0.f;
 #pragma unroll
@@ -714,7 +715,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
               v += (P.compressed ? out_el[e] : 1.f) * base * Sl[(cm.z * n_max + cm.x) * n_max + cm.y];
             }
           }
-          const float g = G32[(size_t)i * n_soap + s];
+          const float g = FO ? 0.f : G32[(size_t)i * n_soap + s];
           vq[u] = v;
           gq[u] = g;
           sums[0] += v * v;
@@ -804,7 +805,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       __syncthreads();
     }
   }
-  if (P.forward_only) {
+  if constexpr (FO) {
     // descriptors only: the next site now (the wait publishes it to the loop-top barrier)
     if (tx == 0) {
       const unsigned long long t = atomicAdd(tickets, 1ull) + gridDim.x;
@@ -1098,21 +1099,27 @@ size_t site_lds(int n_species, int n_max, int l_max, int n_soap, int n_soap_unc)
 }
 
 template <int L, int NTT>
-void launch_site(hipStream_t st, const AdjParams& P, size_t sh, const int* order, const int* site_ptr,
+void launch_site(bool fo, hipStream_t st, const AdjParams& P, size_t sh, const int* order, const int* site_ptr,
                  const float* rtab, const float* atab, const float* preflm, const float* cenv,
                  const uint32_t* spmask, const double* rjs, const double* th, const double* ph, const float* G32,
                  const int4* comp, const int* out_ptr, const int* out_c, const float* out_el, const int* unc_ptr,
                  const int* unc_e, const int32_t* P_i, const float* P_el, const int2* odir, float* q, float* sdp,
                  float* pair, unsigned long long* tickets, int n_cu, int per_cu_cap) {
-  const void* fn = reinterpret_cast<const void*>(&adjoint_site_k<L, NTT>);
+  const void* fn = fo ? reinterpret_cast<const void*>(&adjoint_site_k<L, NTT, true>)
+                     : reinterpret_cast<const void*>(&adjoint_site_k<L, NTT, false>);
   if (sh > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTHREADS, sh) != hipSuccess || per_cu < 1) per_cu = 1;
   if (per_cu_cap > 0 && per_cu > per_cu_cap) per_cu = per_cu_cap;
   const int grid = std::max(1, std::min(P.n_sites, per_cu * n_cu));
-  adjoint_site_k<L, NTT><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask, rjs, th,
-                                                     ph, G32, comp, out_ptr, out_c, out_el, unc_ptr, unc_e, P_i,
-                                                     P_el, odir, q, sdp, pair, tickets);
+  if (fo)
+    adjoint_site_k<L, NTT, true><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask, rjs,
+                                                           th, ph, G32, comp, out_ptr, out_c, out_el, unc_ptr, unc_e,
+                                                           P_i, P_el, odir, q, sdp, pair, tickets);
+  else
+    adjoint_site_k<L, NTT, false><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask,
+                                                            rjs, th, ph, G32, comp, out_ptr, out_c, out_el, unc_ptr,
+                                                            unc_e, P_i, P_el, odir, q, sdp, pair, tickets);
 }
 
 // host copy of the per-call constant tables that live next to the FP64 ones
@@ -1342,7 +1349,6 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   int rc;
   timer("adj_tables", [&] { rc = build_tables(s, st, in, P, msg); });
   if (rc) return rc;
-  P.forward_only = in.forward_only ? 1 : 0;
   const size_t sh = site_lds(in.p->n_species, in.n_max, in.l_max, in.n_soap, in.n_soap_unc);
   if (sh > in.lds_max) {
     msg = "the FP32 site working set exceeds the LDS";
@@ -1440,7 +1446,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
     switch (in.l_max * 4 + ntt) {
 #define LS1(l, t)                                                                                                  \
   case l * 4 + t:                                                                                                  \
-    launch_site<l, t>(st, P, sh, s.order.as<int>(), in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cenv, \
+    launch_site<l, t>(in.forward_only, st, P, sh, s.order.as<int>(), in.site_ptr, s.rtab.as<float>(), s.atab.as<float>(), pf, cenv, \
                       in.spmask, in.rjs, in.thetas, in.phis, s.G32.as<float>(), in.comp, in.out_ptr, in.out_c, oel, \
                       in.unc_ptr, in.unc_e, in.P_i, pel, odir, s.soap.as<float>(), s.sdp.as<float>(),              \
                       s.pair.as<float>(), s.tickets.as<unsigned long long>(), in.n_cu, per_cu_cap);              \
@@ -1456,7 +1462,8 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   if (in.autotune) {
     // the persistent grid's workgroups per CU: the occupancy limit or two, whichever runs faster
     const std::string key = std::to_string(in.l_max) + " " + std::to_string(ntt) + " " + std::to_string(sh) + " " +
-                            std::to_string(in.n_soap) + " " + std::to_string(in.n_sites / 256);
+                            std::to_string(in.n_soap) + " " + std::to_string(in.n_sites / 256) +
+                            (in.forward_only ? " fo" : "");
     auto it = s.tuned.find(key);
     if (it != s.tuned.end()) {
       cap = it->second;
