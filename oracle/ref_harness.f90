@@ -8,7 +8,8 @@
 !   * W and S per species (get_orthonormalization_matrix_poly3[gauss],
 !     src/soap_turbo_radial.f90:1023, :1113), assembled block-diagonally as
 !     get_soap does (src/soap_turbo.f90:364-386);
-!   * get_compress_indices "trivial" (src/soap_turbo_compress.f90:44);
+!   * get_compress_indices "trivial" and the nu_R_nu_S modes "0_0" .. "2_2"
+!     (src/soap_turbo_compress.f90:44);
 !   * get_soap without derivatives, with derivatives (soap_cart_der), and with
 !     contract = .true. followed by soap_contract (src/soap_turbo.f90:139, :1723);
 !   * the radial and angular expansions alone (src/soap_turbo.f90:491-588);
@@ -39,6 +40,7 @@ program ref_harness
   integer, allocatable :: P_i(:), P_j(:), i_beg(:), i_end(:)
   integer :: n_max, k_max, n_soap, n_soap_unc, P_nonzero, dim, i, rep
   character(len=16) :: basis, scaling_mode
+  character(len=8) :: cmode
   logical :: compress, do_central
   real(dp) :: t0, t1, t_full, t_contract, rcut_max
   ! expansion dump
@@ -79,16 +81,19 @@ program ref_harness
   if( iscal /= 0 ) scaling_mode = "none"
   legacy_filter_seed = ( ilegacy /= 0 )
   compress = ( icompress /= 0 )
+  ! header code: 1 "trivial", 2 + 3 nu_R + nu_S the mode "nu_R_nu_S"
+  cmode = "trivial"
+  if( icompress >= 2 ) write(cmode, '(I1,A,I1)') (icompress - 2) / 3, "_", mod(icompress - 2, 3)
 
   n_max = sum(alpha_max)
   k_max = 1 + l_max*(l_max+1)/2 + l_max
   n_soap_unc = n_max*(n_max+1)/2*(l_max+1)
   if( compress )then
     allocate( P_i(1), P_j(1), P_el(1) )
-    call get_compress_indices("trivial", alpha_max, l_max, dim, P_nonzero, P_i, P_j, P_el, "get_dim")
+    call get_compress_indices(trim(cmode), alpha_max, l_max, dim, P_nonzero, P_i, P_j, P_el, "get_dim")
     deallocate( P_i, P_j, P_el )
     allocate( P_i(P_nonzero), P_j(P_nonzero), P_el(P_nonzero) )
-    call get_compress_indices("trivial", alpha_max, l_max, dim, P_nonzero, P_i, P_j, P_el, "set_indices")
+    call get_compress_indices(trim(cmode), alpha_max, l_max, dim, P_nonzero, P_i, P_j, P_el, "set_indices")
     n_soap = maxval(P_i)
   else
     P_nonzero = 1

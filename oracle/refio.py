@@ -23,6 +23,15 @@ MODE_EXPANSION = 8  # radial and angular expansions alone
 MODE_TIMING = 16    # wall-clock of the full-Jacobian and contract paths
 
 
+def compress_code(mode):
+    """The harness's header code of a compress_mode: 0 none, 1 "trivial", 2 + 3 nu_R + nu_S for "nu_R_nu_S"."""
+    if mode in (None, "", "none"):
+        return 0
+    if mode in ("trivial", "Trivial"):
+        return 1
+    return 2 + 3 * int(mode[0]) + int(mode[2])
+
+
 def available() -> bool:
     return os.path.exists(HARNESS)
 
@@ -41,7 +50,7 @@ def write_input(path, w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1):
     hdr[6] = p.radial_enhancement
     hdr[7] = 1 if p.basis == "poly3gauss" else 0
     hdr[8] = 0 if p.scaling_mode == "polynomial" else 1
-    hdr[9] = 1 if p.compress_mode == "trivial" else 0
+    hdr[9] = compress_code(p.compress_mode)
     hdr[10] = 1 if legacy_seed else 0
     hdr[11] = modes
     hdr[12] = n_reps
@@ -115,6 +124,6 @@ def run(w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1, threads=None, ti
         r = subprocess.run([HARNESS, fi, fo], env=env, cwd=td, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError(f"ref_harness failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
-        out = read_output(fo, w, modes, w.params.compress_mode == "trivial", G is not None)
+        out = read_output(fo, w, modes, compress_code(w.params.compress_mode) != 0, G is not None)
         out["stdout"] = r.stdout
     return out

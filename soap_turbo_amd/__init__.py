@@ -322,8 +322,13 @@ def get_compress_indices(compress_mode, alpha_max, l_max, what_to_do, P_i=None, 
     rc = lib().st_get_compress_indices(compress_mode.encode(), len(am), ptr(am), int(l_max), ctypes.byref(dim),
                                        ctypes.byref(nz), ptr(P_i) if setting else None,
                                        ptr(P_j) if setting else None, ptr(P_el) if setting else None, int(setting))
-    if rc == _lib.ST_ERR_UNSUPPORTED:
-        raise SoapTurboError(f'compress_mode "{compress_mode}" is not supported by this build')
+    if rc == _lib.ST_ERR_STOP:
+        # the reference's two stops (src/soap_turbo_compress.f90:136-138, :306)
+        m = compress_mode.rstrip()
+        if len(m) == 3 and m[1] == "_" and m[0] in "012" and m[2] in "012":
+            raise SoapTurboStop(f"ERROR: compress_mode={m} requires all species to have the same number of "
+                                "radial basis functions!")
+        raise SoapTurboStop(f"ERROR: I don't understand compress_mode ={compress_mode}")
     if rc != 0:
         raise SoapTurboError(f"get_compress_indices failed ({rc})")
     return dim.value, nz.value

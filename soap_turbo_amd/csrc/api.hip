@@ -645,7 +645,8 @@ int st_upload_neighbours(st_ctx* ctx, const st_neighbours* nb) {
   HIPCHK(hipStreamSynchronize(ctx->stream));  // the host arrays may go away after return
   ctx->have_nb = true;
   ctx->have_soap = ctx->have_cart = ctx->ct_ready = ctx->have_pair64 = false;
-  ctx->have_vec = false;
+  // the pair -> atom map and pair vectors belong to the previous list: upload them again
+  ctx->have_atoms = ctx->have_vec = false;
   ctx->have_G = false;  // dE/dq is per site: upload it again for the new sites
   st::adjoint_invalidate(ctx->adj);
   ++ctx->nb_gen;
@@ -1334,13 +1335,12 @@ int st_soap_contract(st_ctx* ctx, const double* soap, const double* G, double sg
   return st_download_pairs(ctx, 0, pair);
 }
 
-int st_get_compress_indices(const char* mode, int32_t n_species, const int32_t* alpha_max, int32_t l_max,
-                            int32_t* dim, int32_t* P_nonzero, int32_t* P_i, int32_t* P_j, double* P_el,
-                            int32_t set_indices) {
-  // "trivial": keep (n, m) when either is the first radial index of a species
-  // (src/soap_turbo_compress.f90:86-119)
-  if (!mode || !alpha_max || !dim || !P_nonzero || n_species < 1 || l_max < 0) return ST_ERR_ARG;
-  if (std::strcmp(mode, "trivial") != 0 && std::strcmp(mode, "Trivial") != 0) return ST_ERR_UNSUPPORTED;
+namespace {
+
+// "trivial": keep (n, m) when either is the first radial index of a species
+// (src/soap_turbo_compress.f90:86-119)
+void compress_trivial(int n_species, const int32_t* alpha_max, int l_max, int32_t* dim, int32_t* P_nonzero,
+                      int32_t* P_i, int32_t* P_j, double* P_el, bool set) {
   std::vector<int> pivot(n_species);
   pivot[0] = 1;
   for (int i = 0; i + 1 < n_species; ++i) pivot[i + 1] = pivot[i] + alpha_max[i];
@@ -1354,7 +1354,7 @@ int st_get_compress_indices(const char* mode, int32_t n_species, const int32_t* 
         for (int q : pivot) keep = keep || n == q || m == q;
         if (keep) {
           ++counter;
-          if (set_indices) {
+          if (set) {
             P_i[counter - 1] = counter;
             P_j[counter - 1] = k;
             P_el[counter - 1] = 1.0;
@@ -1364,7 +1364,84 @@ int st_get_compress_indices(const char* mode, int32_t n_species, const int32_t* 
       }
   *dim = counter;
   *P_nonzero = counter;
-  return ST_OK;
+}
+
+// The nu_R_nu_S modes (src/soap_turbo_compress.f90:120-298): the combined radial index of
+// (species z, radial n) is z alpha + n (every species has the same alpha). Component (i, j >= i, l)
+// of the power spectrum, enumerated in the uncompressed order, goes to compressed channel
+// (I, J, l) with I = (z_i mod S1) N1 + (n_i mod N1) and J = (z_j mod S2) N2 + (n_j mod N2), where
+// N1 = alpha if nu_R > 0, N2 = alpha if nu_R = 2, S1 = n_species if nu_S > 0, S2 = n_species if
+// nu_S = 2 (else 1). With nu_R and nu_S both even the compressed spectrum is symmetric: the pair
+// (min, max) of I, J in triangular order, weight sqrt 2 for i != j falling on the diagonal. Else the
+// (I, J) grid is kept whole, and an off-diagonal component contributes to both (I, J) and its
+// mirror image (I', J') of (j, i), weight 1/sqrt 2 each (sqrt 2 when they coincide).
+void compress_darby(int nu_R, int nu_S, int n_species, int alpha, int l_max, int32_t* dim, int32_t* P_nonzero,
+                    int32_t* P_i, int32_t* P_j, double* P_el, bool set) {
+  const int L1 = l_max + 1, nab = n_species * alpha;
+  const int N1 = nu_R > 0 ? alpha : 1, N2 = nu_R == 2 ? alpha : 1;
+  const int S1 = nu_S > 0 ? n_species : 1, S2 = nu_S == 2 ? n_species : 1;
+  const bool sym = nu_R % 2 == 0 && nu_S % 2 == 0;
+  const double r2 = std::sqrt(2.0), ir2 = 1.0 / std::sqrt(2.0);
+  int counter = 0, ind = 0;
+  auto put = [&](int comp, double w) {
+    ++counter;
+    if (set) {
+      P_i[counter - 1] = comp + 1;
+      P_j[counter - 1] = ind + 1;
+      P_el[counter - 1] = w;
+    }
+  };
+  for (int i = 0; i < nab; ++i) {
+    const int zi = i / alpha, ni = i % alpha;
+    for (int j = i; j < nab; ++j) {
+      const int zj = j / alpha, nj = j % alpha;
+      const int ic = (zi % S1) * N1 + ni % N1, jc = (zj % S2) * N2 + nj % N2;
+      for (int l = 0; l < L1; ++l, ++ind) {
+        if (sym) {
+          const int a = std::min(ic, jc), b = std::max(ic, jc), K = N1 * S1;
+          put(a * (2 * K + 1 - a) * L1 / 2 + (b - a) * L1 + l, i != j && ic == jc ? r2 : 1.0);
+        } else {
+          const int ic2 = (zj % S1) * N1 + nj % N1, jc2 = (zi % S2) * N2 + ni % N2;
+          const int c1 = (ic * S2 * N2 + jc) * L1 + l, c2 = (ic2 * S2 * N2 + jc2) * L1 + l;
+          if (i == j) {
+            put(c1, 1.0);
+          } else if (c1 == c2) {
+            put(c1, r2);
+          } else {
+            put(c1, ir2);
+            put(c2, ir2);
+          }
+        }
+      }
+    }
+  }
+  *dim = sym ? N1 * S1 * (N1 * S1 + 1) / 2 * L1 : N1 * S1 * N2 * S2 * L1;
+  *P_nonzero = counter;
+}
+
+}  // namespace
+
+int st_get_compress_indices(const char* mode, int32_t n_species, const int32_t* alpha_max, int32_t l_max,
+                            int32_t* dim, int32_t* P_nonzero, int32_t* P_i, int32_t* P_j, double* P_el,
+                            int32_t set_indices) {
+  if (!mode || !alpha_max || !dim || !P_nonzero || n_species < 1 || l_max < 0) return ST_ERR_ARG;
+  if (set_indices && (!P_i || !P_j || !P_el)) return ST_ERR_ARG;
+  // Fortran string equality: trailing blanks do not count
+  std::string m(mode);
+  while (!m.empty() && m.back() == ' ') m.pop_back();
+  if (m == "trivial" || m == "Trivial") {
+    compress_trivial(n_species, alpha_max, l_max, dim, P_nonzero, P_i, P_j, P_el, set_indices != 0);
+    return ST_OK;
+  }
+  if (m.size() == 3 && m[1] == '_' && m[0] >= '0' && m[0] <= '2' && m[2] >= '0' && m[2] <= '2') {
+    // every species needs the same number of radial functions, or the reference stops
+    for (int i = 1; i < n_species; ++i)
+      if (alpha_max[i] != alpha_max[0]) return ST_ERR_STOP;
+    compress_darby(m[0] - '0', m[2] - '0', n_species, alpha_max[0], l_max, dim, P_nonzero, P_i, P_j, P_el,
+                   set_indices != 0);
+    return ST_OK;
+  }
+  return ST_ERR_STOP;  // a mode the reference does not know
 }
 
 int st_synchronize(st_ctx* ctx) {

@@ -1,7 +1,7 @@
 ! get_compress_indices with the reference's interface
-! (src/soap_turbo_compress.f90:44-54), over the library's C-ABI. The
-! "trivial"/"Trivial" mode is implemented; the Darby nu_R/nu_S modes stop with
-! a message.
+! (src/soap_turbo_compress.f90:44-54), over the library's C-ABI: the
+! "trivial"/"Trivial" mode and the nu_R_nu_S modes "0_0" .. "2_2", with the
+! reference's messages where it stops.
 module soap_turbo_compress_module
   use, intrinsic :: iso_fortran_env, only: real64
   use, intrinsic :: iso_c_binding
@@ -33,15 +33,21 @@ contains
       stop
     end if
     mode = trim(compress_mode)
-    if( mode /= "trivial" .and. mode /= "Trivial" )then
-      write(*,*) "ERROR: compress_mode ", mode, " is not supported by this build (trivial only)"
-      stop
-    end if
     allocate( am(size(alpha_max)) )
     am = alpha_max
     ! sizes first, so that set_indices writes only what the caller allocated
     rc = st_get_compress_indices(mode // c_null_char, int(size(am), c_int32_t), c_loc(am), int(l_max, c_int32_t), &
                                  c_loc(cdim), c_loc(cnz), c_null_ptr, c_null_ptr, c_null_ptr, 0_c_int32_t)
+    if( rc == ST_ERR_STOP )then
+      if( len(mode) == 3 .and. verify(mode(1:1), "012") == 0 .and. mode(2:2) == "_" .and. &
+          verify(mode(3:3), "012") == 0 )then
+        write(*,'(A,A,A)') "ERROR: compress_mode=", trim(adjustl(compress_mode)), &
+                           " requires all species to have the same number of radial basis functions!"
+      else
+        write(*,*) "ERROR: I don't understand compress_mode =", compress_mode
+      end if
+      stop
+    end if
     if( rc /= ST_OK )then
       write(*,*) "ERROR: get_compress_indices failed"
       stop

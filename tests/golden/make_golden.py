@@ -7,7 +7,10 @@ derivatives, soap_cart_der and the angular expansion (every pair of the first tw
 plus weighted checksums and per-pair norms of the whole Jacobian), and soap_contract's pair gradients for a
 seeded dE/dq. Run in a container where /root/reference exists:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
+
+(no names: every case). The darby_* cases use the reference's own compression indices for the
+nu_R_nu_S modes of get_compress_indices (src/soap_turbo_compress.f90:120-298).
 """
 from __future__ import annotations
 
@@ -57,7 +60,44 @@ CASES = {
                           5, False),
 }
 
+
+def _darby(mode, alpha_max, l_max, basis="poly3", box=SMALL_BOX):
+    return (box, SoapParams(alpha_max=alpha_max, l_max=l_max, rcut_hard=4.6, rcut_soft=4.0, basis=basis,
+                            atom_sigma_r=[0.45, 0.5], atom_sigma_t=[0.5, 0.45], central_weight=[1.0, 0.6],
+                            compress_mode=mode), 4, True)
+
+
+CASES.update({
+    "darby_0_0": _darby("0_0", [3, 3], 3),
+    "darby_0_1": _darby("0_1", [3, 3], 2, "poly3gauss"),
+    "darby_0_2": _darby("0_2", [3, 3], 3),
+    "darby_1_0": _darby("1_0", [4, 4], 4, "poly3gauss"),
+    "darby_1_1": _darby("1_1", [3, 3], 4),
+    "darby_1_2": _darby("1_2", [3, 3], 3, "poly3gauss"),
+    "darby_2_0": _darby("2_0", [4, 4], 3),
+    "darby_2_1": _darby("2_1", [3, 3], 3),
+    "darby_2_2": _darby("2_2", [4, 4], 5, "poly3gauss"),
+})
+
 CART_SITES = 2
+
+
+def n_soap_of(params):
+    """Descriptor length: maxval(P_i) of the compression, which is dim for every mode."""
+    n_max = sum(params.alpha_max)
+    L1 = params.l_max + 1
+    mode = params.compress_mode
+    if mode == "trivial":
+        return int(compress_trivial(params.alpha_max, params.l_max)[0].max())
+    if mode in (None, "", "none"):
+        return n_max * (n_max + 1) // 2 * L1
+    nu_R, nu_S = int(mode[0]), int(mode[2])
+    a, ns = params.alpha_max[0], len(params.alpha_max)
+    N1, N2 = (a if nu_R > 0 else 1), (a if nu_R == 2 else 1)
+    S1, S2 = (ns if nu_S > 0 else 1), (ns if nu_S == 2 else 1)
+    if nu_R % 2 == 0 and nu_S % 2 == 0:
+        return N1 * S1 * (N1 * S1 + 1) // 2 * L1
+    return N1 * S1 * N2 * S2 * L1
 
 
 def checksums(cart):
@@ -68,9 +108,7 @@ def checksums(cart):
 
 def make_case(name, spec, params, n_sites, legacy):
     w = make_workload((spec, params), n_sites=n_sites)
-    comp = compress_trivial(params.alpha_max, params.l_max) if params.compress_mode == "trivial" else None
-    n_max = sum(params.alpha_max)
-    n_soap = int(comp[0].max()) if comp is not None else n_max * (n_max + 1) // 2 * (params.l_max + 1)
+    n_soap = n_soap_of(params)
     G = normal(1000 + n_sites, (n_soap, w.n_sites))  # C (n_soap, n_sites) == G(n_sites, n_soap)
     o = refio.run(w, G, modes=refio.MODE_DESC | refio.MODE_FULL | refio.MODE_CONTRACT | refio.MODE_EXPANSION,
                   legacy_seed=legacy)
@@ -87,7 +125,8 @@ def make_case(name, spec, params, n_sites, legacy):
         ang_azi=o["ang_azi"][:nc], ang_pol=o["ang_pol"][:nc], cart_head=o["soap_cart_der"][:nc], cart_checksums=checksums(o["soap_cart_der"]),
         cart_pair_norm=np.sqrt((o["soap_cart_der"] ** 2).sum(axis=(1, 2))),
         **{k: v for k, v in ps.items()})
-    if comp is not None:
+    if "P_i" in o:
+        assert int(o["P_i"].max()) == n_soap
         out.update(P_i=o["P_i"], P_j=o["P_j"], P_el=o["P_el"])
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     return w.n_sites, w.n_pairs, n_soap
@@ -96,5 +135,6 @@ def make_case(name, spec, params, n_sites, legacy):
 if __name__ == "__main__":
     if not refio.available():
         raise SystemExit("build the reference harness first: make -C oracle ref")
-    for name, (spec, params, ns, legacy) in CASES.items():
+    for name in sys.argv[1:] or CASES:
+        spec, params, ns, legacy = CASES[name]
         print(name, *make_case(name, spec, params, ns, legacy))

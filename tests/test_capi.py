@@ -59,9 +59,87 @@ def test_compress_indices_brute_force_small():
         np.testing.assert_array_equal(P_i, np.arange(1, nz + 1))
 
 
+@pytest.mark.parametrize("name", [c for c in CASES if c.startswith("darby_")])
+def test_darby_compress_indices_match_reference(name):
+    # P_i, P_j, P_el of the nu_R_nu_S modes as the reference's get_compress_indices wrote them
+    _, comp, d = load(name)
+    mode = str(d["compress_mode"])
+    dim, nz = st.get_compress_indices(mode, d["alpha_max"], int(d["l_max"]), "get_dim")
+    assert nz == len(d["P_i"]) and dim == int(d["P_i"].max())
+    P_i = np.zeros(nz, dtype=np.int32); P_j = np.zeros(nz, dtype=np.int32); P_el = np.zeros(nz)
+    st.get_compress_indices(mode + "  ", d["alpha_max"], int(d["l_max"]), "set_indices", P_i, P_j, P_el)
+    np.testing.assert_array_equal(P_i, d["P_i"])
+    np.testing.assert_array_equal(P_j, d["P_j"])
+    np.testing.assert_array_equal(P_el, d["P_el"])
+
+
+def _indices(mode, am, L):
+    dim, nz = st.get_compress_indices(mode, am, L, "get_dim")
+    P_i = np.zeros(nz, dtype=np.int32); P_j = np.zeros(nz, dtype=np.int32); P_el = np.zeros(nz)
+    st.get_compress_indices(mode, am, L, "set_indices", P_i, P_j, P_el)
+    return dim, P_i, P_j, P_el
+
+
+@pytest.mark.parametrize("am,L", [([3, 3], 2), ([2, 2, 2], 1), ([4], 3)])
+def test_darby_modes_closed_forms(am, L):
+    n_max, L1 = sum(am), L + 1
+    n_unc = n_max * (n_max + 1) // 2 * L1
+    comps = [(n, m, l) for n in range(n_max) for m in range(n, n_max) for l in range(L1)]
+    # 2_2 keeps every component once, in place
+    dim, P_i, P_j, P_el = _indices("2_2", am, L)
+    assert dim == n_unc
+    np.testing.assert_array_equal(P_i, np.arange(1, n_unc + 1))
+    np.testing.assert_array_equal(P_j, np.arange(1, n_unc + 1))
+    np.testing.assert_array_equal(P_el, np.ones(n_unc))
+    # 0_0 sums every component of one l into channel l: weight sqrt 2 off the diagonal
+    dim, P_i, P_j, P_el = _indices("0_0", am, L)
+    assert dim == L1
+    np.testing.assert_array_equal(P_j, np.arange(1, n_unc + 1))
+    np.testing.assert_array_equal(P_i, [l + 1 for n, m, l in comps])
+    np.testing.assert_array_equal(P_el, [1.0 if n == m else np.sqrt(2.0) for n, m, l in comps])
+    # every mode is a sum over the full, ordered power spectrum p(a, b, l) with the radial and species
+    # channels collapsed: a -> (z_a mod S1) N1 + (n_a mod N1) first, b -> (z_b mod S2) N2 + (n_b mod N2)
+    # second; symmetric modes fold (I, J) onto I <= J with 1/sqrt 2 off the diagonal. u is the stored
+    # (uncompressed) spectrum, sqrt 2 p off the diagonal.
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal((n_max, n_max, L1))
+    p = p + p.transpose(1, 0, 2)
+    u = np.array([p[n, m, l] * (1.0 if n == m else np.sqrt(2.0)) for n, m, l in comps])
+    a, ns = am[0], len(am)
+    for mode in ("0_1", "0_2", "1_0", "1_1", "1_2", "2_0", "2_1"):
+        nu_R, nu_S = int(mode[0]), int(mode[2])
+        N1, N2 = (a if nu_R > 0 else 1), (a if nu_R == 2 else 1)
+        S1, S2 = (ns if nu_S > 0 else 1), (ns if nu_S == 2 else 1)
+        sym = nu_R % 2 == 0 and nu_S % 2 == 0
+        dim, P_i, P_j, P_el = _indices(mode, am, L)
+        c = np.zeros(dim)
+        np.add.at(c, P_i - 1, P_el * u[P_j - 1])
+        expect = np.zeros(dim)
+        K = N1 * S1
+        for x in range(n_max):
+            for y in range(n_max):
+                I = (x // a % S1) * N1 + x % a % N1
+                J = (y // a % S2) * N2 + y % a % N2
+                for l in range(L1):
+                    if sym:
+                        lo, hi = min(I, J), max(I, J)
+                        k = (lo * (2 * K + 1 - lo) * L1) // 2 + (hi - lo) * L1 + l
+                        expect[k] += p[x, y, l] * (1.0 if I == J else 1.0 / np.sqrt(2.0))
+                    else:
+                        expect[(I * S2 * N2 + J) * L1 + l] += p[x, y, l]
+        np.testing.assert_allclose(c, expect, rtol=1e-12, atol=1e-12 * np.abs(expect).max(), err_msg=mode)
+
+
+def test_darby_modes_stop_on_unequal_radial_counts():
+    with pytest.raises(st.SoapTurboStop, match="requires all species to have the same number"):
+        st.get_compress_indices("1_1", [3, 4], 2, "get_dim")
+
+
 def test_unknown_compress_mode_and_what_to_do_fail():
-    with pytest.raises(st.SoapTurboError):
+    with pytest.raises(st.SoapTurboStop, match="I don't understand compress_mode"):
         st.get_compress_indices("0_0_darby", [2], 1, "get_dim")
+    with pytest.raises(st.SoapTurboStop):
+        st.get_compress_indices("3_0", [2], 1, "get_dim")
     with pytest.raises(st.SoapTurboStop):
         st.get_compress_indices("trivial", [2], 1, "bogus")
 

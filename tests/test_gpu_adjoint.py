@@ -9,7 +9,7 @@ import pytest
 import soap_turbo_amd as st
 from oracle import oracle as orc
 from soap_turbo_amd.workload import BoxSpec, SoapParams, make_workload, normal
-from tests.golden_cases import rel
+from tests.golden_cases import CASES, load, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -201,3 +201,29 @@ def test_adjoint_f32_graph_replays_match_eager_runs(ctx):
                 assert np.array_equal(a, b), f"call {call} after the new list"
     finally:
         ctx.set_graphs(False)
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if c.startswith("darby_")])
+def test_adjoint_f32_matches_reference_darby_fixtures(ctx, name):
+    # the nu_R_nu_S compressions (outputs summing many weighted components) against the reference's
+    # own descriptors and soap_contract pair gradients, with the reference's W and S
+    w, comp, d = load(name)
+    p = w.params
+    keep = st._lib.Keep()
+    prm = st.make_params(keep, alpha_max=p.alpha_max, l_max=p.l_max, radial_enhancement=p.radial_enhancement,
+                         basis=p.basis, scaling_mode=p.scaling_mode, legacy_seed=bool(d["legacy_seed"]),
+                         **p.per_species())
+    cmp_ = st.make_compression(keep, True, len(comp[0]), *comp)
+    n_soap = int(comp[0].max())
+    ctx.set_basis(d["W"], d["S"])
+    try:
+        ctx.upload_neighbours(w.n_neigh, w.species.T, w.species_multiplicity, w.mask.T, w.rjs, w.thetas, w.phis)
+        ctx.upload_dedq(np.asfortranarray(d["G"].T))
+        ctx.adjoint_f32(prm, cmp_)
+        q = ctx.download_soap_f32(n_soap).T
+        pair = ctx.download_pairs(1).T
+    finally:
+        ctx.clear_basis()
+    print(f"{name}: n_soap {n_soap}, rel err q {rel(q, d['soap_contract_q']):.2e} pair {rel(pair, d['pair']):.2e}")
+    assert rel(q, d["soap_contract_q"]) <= TOL
+    assert rel(pair, d["pair"]) <= 10 * TOL
