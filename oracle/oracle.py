@@ -56,6 +56,8 @@ def lib():
         L.so_contract.argtypes = [P, P, P, P, P, P, ctypes.c_double, P, P]
         L.so_preflm.argtypes = [ctypes.c_int, P]
         L.so_overlap_S.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, P]
+        L.so_radial2.argtypes = [P, P, P, ctypes.c_int, P, P, P]
+        L.so_central_hessian.argtypes = [P, P, P, P, P, ctypes.c_int, P, P, P, P, P]
         _LIB = L
     return _LIB
 
@@ -185,3 +187,35 @@ def contract(w, W, S, G, sgn=1.0, compress=None, legacy_seed=True):
     lib().so_contract(ctypes.byref(prm), ctypes.byref(inp), ctypes.c_void_p(Wf.ctypes.data),
                       ctypes.c_void_p(Sf.ctypes.data), ctypes.byref(cmp_), _ptr(Gc), sgn, _ptr(soap), _ptr(pair))
     return soap, pair
+
+
+def radial_der2(w, W):
+    """The radial expansion and its first and second derivatives in r (unified form)."""
+    prm, inp, _, keep = _structs(w, legacy_seed=False)
+    nm = n_max(w)
+    Wf = np.asfortranarray(W)
+    rad, rad_der, rad_der2 = (np.zeros((w.n_pairs, nm)) for _ in range(3))
+    lib().so_radial2(ctypes.byref(prm), ctypes.byref(inp), ctypes.c_void_p(Wf.ctypes.data), 1, _ptr(rad),
+                     _ptr(rad_der), _ptr(rad_der2))
+    return rad, rad_der, rad_der2
+
+
+def central_hessian(w, W, S, vecs, compress=None):
+    """get_soap_central_hessian (src/soap_turbo.f90:1151-1700), unified radial form. vecs is
+    (n_sites, n_vec, n_soap); returns (soap (n_sites, n_soap), hess (n_pairs, n_vec, 3 a, 3 b),
+    rad_der2 (n_pairs, n_max), the self-test's relative error)."""
+    prm, inp, cmp_, keep = _structs(w, False, compress)
+    vecs = np.ascontiguousarray(vecs, dtype=np.float64)
+    n_sites, n_vec, n_soap = vecs.shape
+    soap = np.zeros((w.n_sites, n_soap))
+    hess = np.zeros((w.n_pairs, n_vec, 3, 3))
+    rad2 = np.zeros((w.n_pairs, n_max(w)))
+    err = ctypes.c_double()
+    Wf, Sf = np.asfortranarray(W), np.asfortranarray(S)
+    rc = lib().so_central_hessian(ctypes.byref(prm), ctypes.byref(inp), ctypes.c_void_p(Wf.ctypes.data),
+                                  ctypes.c_void_p(Sf.ctypes.data), ctypes.byref(cmp_), n_vec, _ptr(vecs), _ptr(soap),
+                                  _ptr(hess), _ptr(rad2), ctypes.byref(err))
+    if rc:
+        raise RuntimeError("the central-atom Hessian needs the unified radial form (legacy_seed False)")
+    return soap, hess, rad2, err.value
+

@@ -13,6 +13,9 @@
 !   * get_soap without derivatives, with derivatives (soap_cart_der), and with
 !     contract = .true. followed by soap_contract (src/soap_turbo.f90:139, :1723);
 !   * the radial and angular expansions alone (src/soap_turbo.f90:491-588);
+!   * the central-atom Hessian: get_soap with soap_hessian_enabled (and the
+!     self-test on), then get_soap_central_hessian for n_vec weight vectors
+!     (src/soap_turbo.f90:1151), plus the radial second derivatives alone;
 !   * wall-clock timings (omp_get_wtime) of the full-Jacobian path (get_soap
 !     with derivatives, then pair = sum_s G(i,s) soap_cart_der(:,s,k)) and of
 !     the contract path.
@@ -28,7 +31,8 @@ program ref_harness
   integer, parameter :: dp = real64
   character(len=1024) :: fin, fout
   integer :: hdr(16), n_sites, n_pairs, n_species, l_max, max_mult, radial_enhancement, ibasis, iscal
-  integer :: icompress, ilegacy, modes, n_reps, with_G
+  integer :: icompress, ilegacy, modes, n_reps, with_G, n_vec
+  real(dp), allocatable :: vecs(:,:,:), hess(:,:,:,:), rad2(:,:)
   integer, allocatable :: alpha_max(:), n_neigh(:), species(:,:), species_multiplicity(:), imask(:)
   real(dp), allocatable :: rcut_hard(:), rcut_soft(:), nf(:), global_scaling(:), atom_sigma_r(:), &
                            atom_sigma_r_scaling(:), atom_sigma_t(:), atom_sigma_t_scaling(:), &
@@ -55,7 +59,7 @@ program ref_harness
   read(11) hdr
   n_sites = hdr(2); n_pairs = hdr(3); n_species = hdr(4); l_max = hdr(5); max_mult = hdr(6)
   radial_enhancement = hdr(7); ibasis = hdr(8); iscal = hdr(9); icompress = hdr(10)
-  ilegacy = hdr(11); modes = hdr(12); n_reps = max(hdr(13), 1); with_G = hdr(15)
+  ilegacy = hdr(11); modes = hdr(12); n_reps = max(hdr(13), 1); with_G = hdr(15); n_vec = hdr(14)
   allocate( alpha_max(n_species), rcut_hard(n_species), rcut_soft(n_species), nf(n_species), &
             global_scaling(n_species), atom_sigma_r(n_species), atom_sigma_r_scaling(n_species), &
             atom_sigma_t(n_species), atom_sigma_t_scaling(n_species), amplitude_scaling(n_species), &
@@ -104,6 +108,10 @@ program ref_harness
   if( with_G /= 0 )then
     allocate( G(n_sites, n_soap) )
     read(11) G
+  end if
+  if( n_vec > 0 )then
+    allocate( vecs(n_soap, n_vec, n_sites) )
+    read(11) vecs
   end if
   close(11)
 
@@ -193,6 +201,43 @@ program ref_harness
            mask, n_species, eimphi_rad_der, .true., prefl_rad_der, ang, ang_r, ang_a, ang_p)
     write(12) rad, rad_der
     write(12) ang, ang_r, ang_a, ang_p
+  end if
+
+  if( iand(modes, 32) /= 0 )then
+    ! central-atom Hessian (needs legacy_filter_seed = .false.)
+    if( .not. allocated(soap_cart_der) ) allocate( soap_cart_der(3, n_soap, n_pairs) )
+    allocate( hess(3, 3, n_vec, n_pairs) )
+    soap_hessian_enabled = .true.
+    soap_hessian_selftest = .true.
+    soap = 0.d0; soap_cart_der = 0.d0
+    call run_get_soap(.true., .false., soap_cart_der)
+    call get_soap_central_hessian(n_sites, n_neigh, n_pairs, n_species, mask, rjs, thetas, phis, l_max, &
+                                  rcut_hard, atom_sigma_t, atom_sigma_t_scaling, compress, P_nonzero, P_i, &
+                                  P_j, P_el, soap, n_vec, vecs, hess)
+    soap_hessian_enabled = .false.
+    write(12) soap
+    write(12) hess
+    write(12) soap_hessian_selftest_err
+    ! the radial expansion's second derivative alone, species by species as get_soap does it
+    allocate( rad2(n_max, n_pairs) )
+    if( .not. allocated(rad) ) allocate( rad(n_max, n_pairs), rad_der(n_max, n_pairs) )
+    do i = 1, n_species
+      do_central = ( central_weight(i) /= 0.d0 )
+      if( ibasis == 1 )then
+        call get_radial_expansion_coefficients_poly3gauss(n_sites, n_neigh, rjs, alpha_max(i), rcut_soft(i), &
+               rcut_hard(i), atom_sigma_r(i), atom_sigma_r_scaling(i), amplitude_scaling(i), nf(i), &
+               W(i_beg(i):i_end(i), i_beg(i):i_end(i)), scaling_mode, mask(:,i), radial_enhancement, &
+               .true., rad(i_beg(i):i_end(i), :), rad_der(i_beg(i):i_end(i), :), rad2(i_beg(i):i_end(i), :))
+      else
+        call get_radial_expansion_coefficients_poly3(n_sites, n_neigh, rjs, alpha_max(i), rcut_soft(i), &
+               rcut_hard(i), atom_sigma_r(i), atom_sigma_r_scaling(i), amplitude_scaling(i), nf(i), &
+               W(i_beg(i):i_end(i), i_beg(i):i_end(i)), scaling_mode, mask(:,i), radial_enhancement, &
+               .true., do_central, central_weight(i), rad(i_beg(i):i_end(i), :), rad_der(i_beg(i):i_end(i), :), &
+               rad2(i_beg(i):i_end(i), :))
+      end if
+      rad2(i_beg(i):i_end(i), :) = rad2(i_beg(i):i_end(i), :) * global_scaling(i)
+    end do
+    write(12) rad2
   end if
 
   if( iand(modes, 16) /= 0 )then

@@ -21,6 +21,7 @@ MODE_FULL = 2       # get_soap with soap_cart_der
 MODE_CONTRACT = 4   # get_soap(contract=.true.) + soap_contract
 MODE_EXPANSION = 8  # radial and angular expansions alone
 MODE_TIMING = 16    # wall-clock of the full-Jacobian and contract paths
+MODE_HESSIAN = 32   # get_soap_central_hessian for weight vectors, its self-test, radial second derivatives
 
 
 def compress_code(mode):
@@ -36,7 +37,7 @@ def available() -> bool:
     return os.path.exists(HARNESS)
 
 
-def write_input(path, w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1):
+def write_input(path, w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1, vecs=None):
     p = w.params
     ps = p.per_species()
     ns = p.n_species
@@ -54,6 +55,7 @@ def write_input(path, w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1):
     hdr[10] = 1 if legacy_seed else 0
     hdr[11] = modes
     hdr[12] = n_reps
+    hdr[13] = 0 if vecs is None else vecs.shape[1]
     hdr[14] = 1 if G is not None else 0
     with open(path, "wb") as f:
         f.write(hdr.tobytes())
@@ -69,9 +71,12 @@ def write_input(path, w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1):
             f.write(np.asarray(a, dtype=np.float64).tobytes())
         if G is not None:
             f.write(np.ascontiguousarray(G, dtype=np.float64).tobytes())   # (n_soap, n_sites) C == G(n_sites, n_soap) F
+        if vecs is not None:
+            # (n_sites, n_vec, n_soap) C == vecs(n_soap, n_vec, n_sites) F
+            f.write(np.ascontiguousarray(vecs, dtype=np.float64).tobytes())
 
 
-def read_output(path, w, modes, compress, with_G):
+def read_output(path, w, modes, compress, with_G, n_vec=0):
     out = {}
     with open(path, "rb") as f:
         buf = f.read()
@@ -105,13 +110,19 @@ def read_output(path, w, modes, compress, with_G):
         out["rad_der"] = take(np.float64, n_max * np_).reshape(np_, n_max)
         for key in ("ang", "ang_rad", "ang_azi", "ang_pol"):
             out[key] = take(np.complex128, k_max * np_).reshape(np_, k_max)
+    if modes & MODE_HESSIAN:
+        out["soap_hess"] = take(np.float64, n_soap * ns_).reshape(ns_, n_soap)
+        # hess(a, b, v, k) F -> (n_pairs, n_vec, a, b)
+        out["hess"] = take(np.float64, 9 * n_vec * np_).reshape(np_, n_vec, 3, 3).transpose(0, 1, 3, 2).copy()
+        out["hess_selftest_err"] = float(take(np.float64, 1)[0])
+        out["rad_der2"] = take(np.float64, n_max * np_).reshape(np_, n_max)
     if modes & MODE_TIMING:
         out["t_full"], out["t_contract"] = take(np.float64, 2)
         out["threads"] = int(take(np.int32, 1)[0])
     return out
 
 
-def run(w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1, threads=None, timeout=3600):
+def run(w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1, threads=None, timeout=3600, vecs=None):
     """Runs the reference on workload ``w``; returns a dict of its outputs (numpy C order)."""
     if not available():
         raise RuntimeError("oracle/_ref/ref_harness is not built (make -C oracle ref)")
@@ -120,10 +131,11 @@ def run(w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1, threads=None, ti
         env["OMP_NUM_THREADS"] = str(threads)
     with tempfile.TemporaryDirectory() as td:
         fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        write_input(fi, w, G, modes, legacy_seed, n_reps)
+        write_input(fi, w, G, modes, legacy_seed, n_reps, vecs)
         r = subprocess.run([HARNESS, fi, fo], env=env, cwd=td, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError(f"ref_harness failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
-        out = read_output(fo, w, modes, compress_code(w.params.compress_mode) != 0, G is not None)
+        out = read_output(fo, w, modes, compress_code(w.params.compress_mode) != 0, G is not None,
+                          0 if vecs is None else vecs.shape[1])
         out["stdout"] = r.stdout
     return out

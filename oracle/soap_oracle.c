@@ -104,19 +104,22 @@ void so_overlap_S(int basis, int alpha_max, double atom_sigma_r, double rcut_har
  * for the per-call constants of radial_expansion_range (:394-547). Output
  * before global_scaling. */
 typedef struct {
-  int alpha_max, n_poly, n_hi, do_poly, with_gauss, do_der, unified, has_filter, scaling_poly, radial_enh;
+  int alpha_max, n_poly, n_hi, do_poly, with_gauss, do_der, do_der2, unified, has_filter, scaling_poly, radial_enh;
   int do_central;
   double rcut_soft, atom_sigma, dr, inv_rcut, sqrt_rcut, inv_sqrt_rcut, inv_Nm2, sigma_sq0, N_gauss, q, drnf;
   double scaling, amp_scaling, central_weight;
-  double ra[MAXA], rb[MAXA], rc[MAXA], R1[MAXA], R2[MAXA]; /* index n + 1 (ra(-1) at 0) */
+  double ra[MAXA], rb[MAXA], rc[MAXA], R1[MAXA], R2[MAXA], R3[MAXA], R4[MAXA]; /* ra.. at n + 1 (ra(-1) at 0) */
   const double* W; int ldw;                                /* species block of W */
 } radial_consts;
 
+/* do_der: 0 values, 1 first derivatives, 2 also second derivatives (unified form only,
+ * src/soap_turbo_radial.f90:308-320) */
 static void radial_setup(radial_consts* c, const so_params* p, int s, const double* W, int ldw, int off, int do_der) {
   memset(c, 0, sizeof *c);
   c->alpha_max = p->alpha_max[s];
   c->with_gauss = p->basis == 1;
-  c->do_der = do_der;
+  c->do_der = do_der >= 1;
+  c->do_der2 = do_der >= 2;
   c->unified = !p->legacy_seed;
   c->scaling_poly = p->scaling_poly;
   c->radial_enh = p->radial_enhancement;
@@ -139,7 +142,7 @@ static void radial_setup(radial_consts* c, const so_params* p, int s, const doub
   }
   c->n_poly = c->with_gauss ? c->alpha_max - 1 : c->alpha_max;
   c->do_poly = c->n_poly >= 1;
-  c->n_hi = c->n_poly + (do_der ? 2 : 0);
+  c->n_hi = c->n_poly + (c->do_der2 ? 4 : c->do_der ? 2 : 0);
   double Nv[MAXA + 4]; /* Nv(n) at n + 3 */
   Nv[0] = 1.0;
   for (int n = -2; n <= c->alpha_max + 4; ++n) Nv[n + 3] = sqrt(1.0 / (double)(2 * n + 5));
@@ -153,56 +156,67 @@ static void radial_setup(radial_consts* c, const so_params* p, int s, const doub
   for (int n = 1; n <= c->n_poly; ++n) {
     c->R1[n] = Nv[n + 1 + 3] / Nv[n + 3];
     c->R2[n] = Nv[n + 2 + 3] / Nv[n + 3];
+    c->R3[n] = Nv[n + 3 + 3] / Nv[n + 3];
+    c->R4[n] = Nv[n + 4 + 3] / Nv[n + 3];
   }
   c->W = W + off + (size_t)off * ldw;
   c->ldw = ldw;
 }
 
-static void radial_pair(const radial_consts* c, double rjs, int is_central, double* outv, double* outd) {
+static void radial_pair(const radial_consts* c, double rjs, int is_central, double* outv, double* outd, double* outh) {
   const double pi = acos(-1.0), sq2 = sqrt(2.0), isq2 = 1.0 / sq2, sqpi2 = sqrt(pi / 2.0),
                sq2pi = sqrt(2.0 / pi), sq8pi = sqrt(8.0 / pi);
   const double sc = c->scaling, rs = c->rcut_soft;
-  double T1[MAXA], T2[MAXA], pol[MAXA], DT[MAXA]; /* T(n) at n + 3; pol(n), DT(n) at n */
+  double T1[MAXA], T2[MAXA], pol[MAXA], DT[MAXA], D2T[MAXA]; /* T(n) at n + 3; pol(n), DT(n) at n */
   memset(T2, 0, sizeof T2);
   memset(DT, 0, sizeof DT);
+  memset(D2T, 0, sizeof D2T);
   double rj = rjs * c->inv_rcut;
   double sig = c->atom_sigma + sc * rj, s2 = sig * sig, isig = 1.0 / sig;
   double is2 = isig * isig, is3 = is2 * isig;
-  double amp, ampd;
-  /* amplitude, :589-635 */
+  double amp, ampd, ampd2;
+  /* amplitude and its first two derivatives, :572-635 */
   if (!c->scaling_poly) {
-    amp = 1.0; ampd = 0.0;
+    amp = 1.0; ampd = 0.0; ampd2 = 0.0;
   } else if (c->amp_scaling == 0.0) {
-    amp = isig; ampd = -sc * is2;
+    amp = isig; ampd = -sc * is2; ampd2 = 2.0 * sc * sc * is3;
   } else if (c->amp_scaling == 1.0) {
     double h = 1.0 + 2.0 * (rj * rj * rj) - 3.0 * (rj * rj);
     double hp = 6.0 * (rj * rj) - 6.0 * rj;
+    double hpp = 12.0 * rj - 6.0;
     double g = isig * h;
     amp = h <= 1e-10 ? 0.0 : g;
     ampd = h <= 1e-10 ? 0.0 : hp * isig - sc * isig * g;
+    ampd2 = h <= 1e-10 ? 0.0 : hpp * isig - 2.0 * sc * hp * is2 + 2.0 * sc * sc * h * is3;
   } else {
     double h = 1.0 + 2.0 * (rj * rj * rj) - 3.0 * (rj * rj);
     if (h <= 1e-10) {
-      amp = 0.0; ampd = 0.0;
+      amp = 0.0; ampd = 0.0; ampd2 = 0.0;
     } else {
-      double hp = 6.0 * (rj * rj) - 6.0 * rj, aa = c->amp_scaling;
+      double hp = 6.0 * (rj * rj) - 6.0 * rj, hpp = 12.0 * rj - 6.0, aa = c->amp_scaling;
       amp = isig * pow(h, aa);
       ampd = aa * pow(h, aa - 1.0) * hp * isig - sc * isig * amp;
+      ampd2 = (aa * (aa - 1.0) * pow(h, aa - 2.0) * hp * hp + aa * pow(h, aa - 1.0) * hpp) * isig -
+              2.0 * sc * aa * pow(h, aa - 1.0) * hp * is2 + 2.0 * sc * sc * pow(h, aa) * is3;
     }
   }
   /* central weight, :637-646 */
   if (c->do_central && c->central_weight != 1.0 && is_central) {
     amp = c->central_weight * amp;
     ampd = c->central_weight * ampd;
+    ampd2 = c->central_weight * ampd2;
   }
-  /* radial enhancement, :648-670 */
+  /* radial enhancement, :648-670: (a m)'' = a'' m + 2 a' m' + a m'' */
   if (c->radial_enh == 1) {
     double g = rj + sq2pi * sig, f = 1.0 + sq2pi * sc;
+    ampd2 = ampd2 * g + 2.0 * ampd * f;
     ampd = amp * f + ampd * g;
     amp = amp * g;
   } else if (c->radial_enh == 2) {
     double g = (rj * rj) + s2 + sq8pi * sig * rj;
     double f = 2.0 * rj + 2.0 * sig * sc + sq8pi * sig + sq8pi * rj * sc;
+    double h = 2.0 + 2.0 * sc * sc + 2.0 * sq8pi * sc;
+    ampd2 = ampd2 * g + 2.0 * ampd * f + amp * h;
     ampd = amp * f + ampd * g;
     amp = amp * g;
   }
@@ -265,6 +279,19 @@ static void radial_pair(const radial_consts* c, double rjs, int is_central, doub
     double g2 = sc * is3;
     for (int n = 1; n <= c->n_poly; ++n)
       DT[n] = g0 * pol[n] + c->R1[n] * g1 * pol[n + 1] + c->R2[n] * g2 * pol[n + 2];
+    /* second derivative, :838-861 */
+    if (c->do_der2) {
+      double is4 = is2 * is2, is5 = is4 * isig, is6 = is4 * is2;
+      double k0 = -is2, k1 = -4.0 * sc * is3, k2 = (1.0 - 3.0 * sc * sc) * is4, k3 = 2.0 * sc * is5, k4 = sc * sc * is6;
+      double B0 = k0 + k1 * f + k2 * f * f + k3 * (f * f * f) + k4 * (f * f * f * f);
+      double B1 = -(k1 + 2.0 * k2 * f + 3.0 * k3 * f * f + 4.0 * k4 * (f * f * f));
+      double B2 = k2 + 3.0 * k3 * f + 6.0 * k4 * f * f;
+      double B3 = -(k3 + 4.0 * k4 * f);
+      double B4 = k4;
+      for (int n = 1; n <= c->n_poly; ++n)
+        D2T[n] = B0 * pol[n] + c->R1[n] * B1 * pol[n + 1] + c->R2[n] * B2 * pol[n + 2] + c->R3[n] * B3 * pol[n + 3] +
+                 c->R4[n] * B4 * pol[n + 4];
+    }
   }
   /* Gaussian at the origin, :864-908 */
   if (c->with_gauss) {
@@ -280,6 +307,13 @@ static void radial_pair(const radial_consts* c, double rjs, int is_central, doub
       double M1 = sc0 * ge - pp * M0;
       double M2 = sc0 * ge * (-rj) - pp * M1 + sc0 * M0;
       DT[c->alpha_max] = M1 * is2 + sc * M2 * is3;
+      if (c->do_der2) {
+        double M3 = sc0 * ge * (rj * rj) - pp * M2 + 2.0 * sc0 * M1;
+        double M4 = sc0 * ge * (-(rj * rj * rj)) - pp * M3 + 3.0 * sc0 * M2;
+        double is4 = is2 * is2, is5 = is4 * isig, is6 = is4 * is2;
+        D2T[c->alpha_max] = -M0 * is2 - 4.0 * sc * M1 * is3 + (1.0 - 3.0 * sc * sc) * M2 * is4 + 2.0 * sc * M3 * is5 +
+                            sc * sc * M4 * is6;
+      }
     }
   }
   /* transform and store, :910-1000 */
@@ -292,6 +326,15 @@ static void radial_pair(const radial_consts* c, double rjs, int is_central, doub
       for (int m = 2; m <= a; ++m) acc = acc + c->W[(n - 1) + (size_t)(m - 1) * c->ldw] * dtot[m];
       outd[n - 1] = acc * c->inv_sqrt_rcut;
     }
+    if (c->do_der2) {
+      double d2tot[MAXA];
+      for (int n = 1; n <= a; ++n) d2tot[n] = amp * D2T[n] + 2.0 * ampd * DT[n] + ampd2 * pol[n];
+      for (int n = 1; n <= a; ++n) {
+        double acc = c->W[(n - 1)] * d2tot[1];
+        for (int m = 2; m <= a; ++m) acc = acc + c->W[(n - 1) + (size_t)(m - 1) * c->ldw] * d2tot[m];
+        outh[n - 1] = acc * c->inv_sqrt_rcut * c->inv_rcut;
+      }
+    }
   }
   for (int n = 1; n <= a; ++n) {
     double acc = c->W[(n - 1)] * pol[1];
@@ -303,9 +346,17 @@ static void radial_pair(const radial_consts* c, double rjs, int is_central, doub
 /* get_radial_expansion_coefficients_poly3[gauss] for every species, then
  * global_scaling (src/soap_turbo.f90:491-544; src/soap_turbo_radial.f90:116-384). */
 void so_radial(const so_params* p, const so_inputs* in, const double* W, int do_der, double* rad, double* rad_der) {
+  so_radial2(p, in, W, do_der, rad, rad_der, NULL);
+}
+
+/* The same with the second derivatives (rad_der2 non-NULL; do_der must be set and the
+ * unified form selected, as for the reference, src/soap_turbo_radial.f90:308-320). */
+void so_radial2(const so_params* p, const so_inputs* in, const double* W, int do_der, double* rad, double* rad_der,
+                double* rad_der2) {
   const int n_max = n_max_of(p);
   memset(rad, 0, sizeof(double) * (size_t)n_max * in->n_pairs);
   if (do_der) memset(rad_der, 0, sizeof(double) * (size_t)n_max * in->n_pairs);
+  if (rad_der2) memset(rad_der2, 0, sizeof(double) * (size_t)n_max * in->n_pairs);
   int* first = (int*)malloc(sizeof(int) * (in->n_pairs + 1));
   {
     int k = 0;
@@ -315,18 +366,19 @@ void so_radial(const so_params* p, const so_inputs* in, const double* W, int do_
   int off = 0;
   for (int s = 0; s < p->n_species; ++s) {
     radial_consts c;
-    radial_setup(&c, p, s, W, n_max, off, do_der);
+    radial_setup(&c, p, s, W, n_max, off, rad_der2 ? 2 : do_der);
     const double gs = p->global_scaling[s];
 #pragma omp parallel for schedule(dynamic, 256)
     for (int k = 0; k < in->n_pairs; ++k) {
       if (first[k] && !c.do_central) continue;
       if (in->rjs[k] >= p->rcut_hard[s]) continue;
       if (!in->mask[k + (size_t)s * in->n_pairs]) continue;
-      double v[MAXA], d[MAXA];
-      radial_pair(&c, in->rjs[k], first[k], v, d);
+      double v[MAXA], d[MAXA], h[MAXA];
+      radial_pair(&c, in->rjs[k], first[k], v, d, h);
       for (int n = 0; n < c.alpha_max; ++n) {
         rad[(size_t)k * n_max + off + n] = v[n] * gs;
         if (do_der) rad_der[(size_t)k * n_max + off + n] = d[n] * gs;
+        if (rad_der2) rad_der2[(size_t)k * n_max + off + n] = h[n] * gs;
       }
     }
     off += p->alpha_max[s];
@@ -873,6 +925,394 @@ int so_contract(const so_params* p, const so_inputs* in, const double* W, const 
     free(Gp); free(Gu); free(Y); free(H);
   }
   free(sdp);
+  state_free(&st);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Central-atom Hessian, get_soap_central_hessian, src/soap_turbo.f90:1151-1700: for every pair
+ * k2 = (site i, neighbour j), hess(k2, v, a, b) = sum_d vecs(i, v, d) d^2 q_d / dr_ia dr_jb, with
+ * q the normalised (compressed) descriptor; the self pair holds minus the sum of the others.
+ * Needs the unified radial form (legacy_seed = 0): the second radial derivatives exist only there.
+ * ------------------------------------------------------------------------------------------ */
+#include <complex.h>
+
+/* the unnormalised Cartesian derivative of the compressed descriptor for one pair (the terms of
+ * soap_derivatives_from_expansion, :1927-2075, compressed, then rotated as :837-876 without the
+ * normalisation): d[3 s + a] */
+static void pair_unnorm_der(const so_state* st, const so_params* p, const so_compress* c, const so_inputs* in,
+                            const double* cn, int k2, double* tr, double* ta, double* to, double* P, double* d) {
+  const int n_max = st->n_max, k_max = st->k_max, L = p->l_max, n_soap = st->n_soap;
+  const double fourpi = 4.0 * acos(-1.0), sq2 = sqrt(2.0);
+  double *Q = P + n_max * (L + 1), *Qa = Q + n_max * (L + 1), *Qo = Qa + n_max * (L + 1);
+  const double* A = st->ang + 2 * (size_t)k2 * k_max;
+  const double* Ar = st->ang_r + 2 * (size_t)k2 * k_max;
+  const double* Aa = st->ang_a + 2 * (size_t)k2 * k_max;
+  const double* Ap = st->ang_p + 2 * (size_t)k2 * k_max;
+  for (int n = 0; n < n_max; ++n)
+    for (int l = 0; l <= L; ++l) {
+      int kb = l * (l + 1) / 2;
+      double ap = 0, aq = 0, aa = 0, ao = 0;
+      for (int m = 0; m <= l; ++m) {
+        int k = kb + m;
+        double w = m == 0 ? 1.0 : 2.0;
+        double cr = cn[2 * (n * k_max + k)], ci = cn[2 * (n * k_max + k) + 1];
+        ap = ap + w * (A[2 * k] * cr + A[2 * k + 1] * ci);
+        aq = aq + w * (Ar[2 * k] * cr + Ar[2 * k + 1] * ci);
+        aa = aa + w * (Aa[2 * k] * cr + Aa[2 * k + 1] * ci);
+        ao = ao + w * (Ap[2 * k] * cr + Ap[2 * k + 1] * ci);
+      }
+      P[n + l * n_max] = ap; Q[n + l * n_max] = aq; Qa[n + l * n_max] = aa; Qo[n + l * n_max] = ao;
+    }
+  int cnt = 0;
+  const double* r = st->rad + (size_t)k2 * n_max;
+  const double* rd = st->rad_der + (size_t)k2 * n_max;
+  for (int n = 0; n < n_max; ++n)
+    for (int np = n; np < n_max; ++np) {
+      double base = n == np ? 1.0 : sq2, pre = fourpi * base;
+      for (int l = 0; l <= L; ++l, ++cnt) {
+        tr[cnt] = ta[cnt] = to[cnt] = 0.0;
+        if (st->skip[cnt]) continue;
+        tr[cnt] = pre * (rd[n] * P[np + l * n_max] + r[n] * Q[np + l * n_max] + rd[np] * P[n + l * n_max] +
+                         r[np] * Q[n + l * n_max]);
+        ta[cnt] = pre * (r[n] * Qa[np + l * n_max] + r[np] * Qa[n + l * n_max]);
+        to[cnt] = pre * (r[n] * Qo[np + l * n_max] + r[np] * Qo[n + l * n_max]);
+      }
+    }
+  double sin_t = sin(in->thetas[k2]), cos_t = cos(in->thetas[k2]);
+  double sin_f = sin(in->phis[k2]), cos_f = cos(in->phis[k2]);
+  double inv_r = 1.0 / in->rjs[k2];
+  double cx = sin_t * cos_f, cy = sin_t * sin_f, cz = cos_t;
+  double px = -cos_t * cos_f * inv_r, py = -cos_t * sin_f * inv_r, pz = sin_t * inv_r;
+  double ax = -sin_f * inv_r, ay = cos_f * inv_r;
+  for (int s = 0; s < 3 * n_soap; ++s) d[s] = 0.0;
+  for (int e = 0; e < (c && c->compress ? c->P_nonzero : n_soap); ++e) {
+    int s = c && c->compress ? c->P_i[e] - 1 : e, u = c && c->compress ? c->P_j[e] - 1 : e;
+    double w = c && c->compress ? c->P_el[e] : 1.0;
+    d[3 * s] += w * (cx * tr[u] + px * to[u] + ax * ta[u]);
+    d[3 * s + 1] += w * (cy * tr[u] + py * to[u] + ay * ta[u]);
+    d[3 * s + 2] += w * (cz * tr[u] + pz * to[u]);
+  }
+}
+
+typedef struct {
+  int k_max, l_max;
+  const int *lofk, *mofk;
+  const double* Ck;
+  double nh[3], invr;
+  double Fl[16], Fl1[16], Fl2[16];
+  double complex *Om, *dOm, *d2Om; /* dOm[3 k + a], d2Om[9 k + 3 a + b] */
+} hess_geom;
+
+/* pair_geometry, src/soap_turbo.f90:1585-1697: F_l and its two radial derivatives, and
+ * Omega_k = C_k Q_l^m(t) w^m with its Cartesian gradient and Hessian */
+static int pair_geometry(hess_geom* g, const so_params* p, const so_inputs* in, int k2, double rcut_max, int want2) {
+  const int L = g->l_max, k_max = g->k_max;
+  double rr = in->rjs[k2];
+  if (rr >= rcut_max || rr < 1e-10) return 0;
+  int isp = -1;
+  for (int s = 0; s < p->n_species; ++s)
+    if (in->mask[k2 + (size_t)s * in->n_pairs]) { isp = s; break; }
+  if (isp < 0) return 0;
+  double invr = 1.0 / rr, invr2 = invr * invr;
+  g->invr = invr;
+  double sscal = p->atom_sigma_t_scaling[isp];
+  double sig = p->atom_sigma_t[isp] + sscal * rr;
+  double amp = rcut_max * rcut_max / (sig * sig);
+  double ampd = -2.0 * sscal / sig * amp;
+  double ampdd = 6.0 * (sscal * sscal) / (sig * sig) * amp;
+  double xarg = rr / sig, zz = xarg * xarg, invz = 1.0 / zz;
+  double il[20], dilz[16], d2ilz[16];
+  ilexp(L + 2, xarg, il);
+  for (int l = 0; l <= L; ++l) {
+    double rl = (double)l;
+    dilz[l] = il[l + 1] + (rl * invz - 1.0) * il[l];
+    d2ilz[l] = il[l + 2] + ((double)(2 * l + 1) * invz - 2.0) * il[l + 1] +
+               (rl * (rl - 1.0) * invz * invz - 2.0 * rl * invz + 1.0) * il[l];
+  }
+  double zp = 2.0 * rr / (sig * sig) - 2.0 * (rr * rr) * sscal / (sig * sig * sig);
+  double zp2 = 2.0 / (sig * sig) - 8.0 * rr * sscal / (sig * sig * sig) +
+               6.0 * (rr * rr) * (sscal * sscal) / (sig * sig * sig * sig);
+  for (int l = 0; l <= L; ++l) {
+    double f0 = il[l], f1 = dilz[l] * zp, f2 = d2ilz[l] * zp * zp + dilz[l] * zp2;
+    g->Fl[l] = amp * f0;
+    g->Fl1[l] = ampd * f0 + amp * f1;
+    g->Fl2[l] = ampdd * f0 + 2.0 * ampd * f1 + amp * f2;
+  }
+  double sin_t = sin(in->thetas[k2]), cos_t = cos(in->thetas[k2]);
+  double sin_f = sin(in->phis[k2]), cos_f = cos(in->phis[k2]);
+  double* nh = g->nh;
+  nh[0] = sin_t * cos_f; nh[1] = sin_t * sin_f; nh[2] = cos_t;
+  double tt = nh[2];
+  double complex wc = nh[0] - I * nh[1];
+  const double complex ec[3] = {1.0, -I, 0.0};
+  double tvec[3], thes[3][3];
+  double complex wvec[3], whes[3][3];
+  for (int a = 0; a < 3; ++a) {
+    tvec[a] = -tt * nh[a] * invr;
+    wvec[a] = -wc * nh[a] * invr + ec[a] * invr;
+  }
+  tvec[2] += invr;
+  if (want2) {
+    for (int b = 0; b < 3; ++b)
+      for (int a = 0; a < 3; ++a) {
+        thes[a][b] = 3.0 * tt * nh[a] * nh[b] * invr2;
+        whes[a][b] = (3.0 * wc * nh[a] * nh[b] - ec[a] * nh[b] - ec[b] * nh[a]) * invr2;
+      }
+    for (int a = 0; a < 3; ++a) {
+      thes[2][a] -= nh[a] * invr2;
+      thes[a][2] -= nh[a] * invr2;
+      thes[a][a] -= tt * invr2;
+      whes[a][a] -= wc * invr2;
+    }
+  }
+  double Qt[200];
+  Qt[0] = 1.0;
+  if (L >= 1) { Qt[1] = tt; Qt[2] = 1.0; }
+  for (int ll = 2; ll <= L; ++ll) {
+    int kb = ll * (ll + 1) / 2; /* 0-based k of (ll, 0) */
+    for (int mm = 0; mm <= ll - 2; ++mm) {
+      int kq = kb + mm;
+      Qt[kq] = ((double)(2 * ll - 1) * tt * Qt[kq - ll] - (double)(ll + mm - 1) * Qt[kq - 2 * ll + 1]) / (double)(ll - mm);
+    }
+    Qt[kb + ll - 1] = (double)(2 * ll - 1) * tt * Qt[kb + ll - 1 - ll];
+    Qt[kb + ll] = (double)(2 * ll - 1) * Qt[kb + ll - ll - 1];
+  }
+  double complex wp[20]; /* wpow(m) at m + 2 */
+  wp[0] = wp[1] = 0.0;
+  wp[2] = 1.0;
+  for (int m = 1; m <= L; ++m) wp[m + 2] = wp[m + 1] * wc;
+  for (int k = 0; k < k_max; ++k) {
+    int ll = g->lofk[k], mm = g->mofk[k];
+    double q0 = Qt[k], q1 = mm + 1 <= ll ? Qt[k + 1] : 0.0, q2 = mm + 2 <= ll ? Qt[k + 2] : 0.0;
+    double C = g->Ck[k], dm = (double)mm;
+    g->Om[k] = C * q0 * wp[mm + 2];
+    for (int a = 0; a < 3; ++a)
+      g->dOm[3 * k + a] = C * (q1 * tvec[a] * wp[mm + 2] + q0 * dm * wp[mm + 1] * wvec[a]);
+    if (want2)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          g->d2Om[9 * k + 3 * a + b] =
+              C * (q2 * tvec[a] * tvec[b] * wp[mm + 2] + q1 * thes[a][b] * wp[mm + 2] +
+                   q1 * dm * wp[mm + 1] * (tvec[a] * wvec[b] + tvec[b] * wvec[a]) +
+                   q0 * (double)(mm * (mm - 1)) * wp[mm] * wvec[a] * wvec[b] + q0 * dm * wp[mm + 1] * whes[a][b]);
+  }
+  return 1;
+}
+
+int so_central_hessian(const so_params* p, const so_inputs* in, const double* W, const double* S,
+                       const so_compress* c, int n_vec, const double* vecs, double* soap, double* hess,
+                       double* rad_der2_out, double* selftest_err) {
+  if (p->legacy_seed) return 1; /* the second radial derivatives need the unified form */
+  so_state st;
+  build_state(&st, p, in, W, S, c, 1);
+  const int n_max = st.n_max, k_max = st.k_max, L = p->l_max, n_soap = st.n_soap, n_unc = st.n_soap_unc;
+  const size_t NP = in->n_pairs;
+  double* rad2 = (double*)malloc(sizeof(double) * n_max * NP);
+  {
+    double* r0 = (double*)malloc(sizeof(double) * n_max * NP);
+    double* r1 = (double*)malloc(sizeof(double) * n_max * NP);
+    so_radial2(p, in, W, 1, r0, r1, rad2);
+    free(r0); free(r1);
+  }
+  if (rad_der2_out) memcpy(rad_der2_out, rad2, sizeof(double) * n_max * NP);
+  double* sdp = (double*)malloc(sizeof(double) * in->n_sites);
+  memset(soap, 0, sizeof(double) * (size_t)in->n_sites * n_soap);
+  power_spectrum(&st, p, c, in->n_sites, soap, sdp); /* unnormalised here */
+  const double pi = acos(-1.0), fourpi = 4.0 * pi, sqrt2 = sqrt(2.0);
+  double rcut_max = 0.0;
+  for (int s = 0; s < p->n_species; ++s) rcut_max = p->rcut_hard[s] > rcut_max ? p->rcut_hard[s] : rcut_max;
+  const int nv = n_vec + 1;
+  int* lofk = (int*)malloc(sizeof(int) * k_max);
+  int* mofk = (int*)malloc(sizeof(int) * k_max);
+  double *wm = (double*)malloc(sizeof(double) * k_max), *Ck = (double*)malloc(sizeof(double) * k_max);
+  double* preflm = (double*)malloc(sizeof(double) * k_max);
+  so_preflm(L, preflm);
+  for (int l = 0, k = 0; l <= L; ++l)
+    for (int m = 0; m <= l; ++m, ++k) {
+      lofk[k] = l; mofk[k] = m;
+      wm[k] = m > 0 ? 2.0 : 1.0;
+      Ck[k] = m % 2 == 0 ? preflm[k] : -preflm[k];
+    }
+  memset(hess, 0, sizeof(double) * 9 * (size_t)n_vec * NP);
+  double chk_err = 0.0, chk_ref = 0.0;
+#pragma omp parallel reduction(max : chk_err, chk_ref)
+  {
+    hess_geom g;
+    g.k_max = k_max; g.l_max = L; g.lofk = lofk; g.mofk = mofk; g.Ck = Ck;
+    g.Om = (double complex*)malloc(sizeof(double complex) * k_max);
+    g.dOm = (double complex*)malloc(sizeof(double complex) * 3 * k_max);
+    g.d2Om = (double complex*)malloc(sizeof(double complex) * 9 * k_max);
+    double complex* Gacc = (double complex*)malloc(sizeof(double complex) * 3 * n_max * k_max); /* [a][n][k] */
+    double complex* Zv = (double complex*)malloc(sizeof(double complex) * (size_t)nv * n_max * k_max);
+    double complex* ZGv = (double complex*)malloc(sizeof(double complex) * (size_t)nv * 3 * n_max * k_max);
+    double* Smat = (double*)malloc(sizeof(double) * n_max * n_max * (L + 1));
+    double* vunc = (double*)malloc(sizeof(double) * n_unc);
+    double* pvec = (double*)malloc(sizeof(double) * n_soap);
+    double* dp1 = (double*)malloc(sizeof(double) * 3 * n_soap);
+    double* dp2 = (double*)malloc(sizeof(double) * 3 * n_soap);
+    double *tr = (double*)malloc(sizeof(double) * n_unc), *ta = (double*)malloc(sizeof(double) * n_unc),
+           *to = (double*)malloc(sizeof(double) * n_unc);
+    double* Pw = (double*)malloc(sizeof(double) * 4 * n_max * (L + 1));
+    double* hv = (double*)malloc(sizeof(double) * 9 * nv);
+    double* u2rec = (double*)malloc(sizeof(double) * 3 * nv);
+    double* hsum = (double*)malloc(sizeof(double) * 9 * (n_vec > 0 ? n_vec : 1));
+#pragma omp for schedule(dynamic, 4)
+    for (int i = 0; i < in->n_sites; ++i) {
+      const double s = sdp[i], invs = 1.0 / s, invs3 = invs / (s * s), invs5 = invs3 / (s * s);
+      const double* cn = st.cnk + 2 * (size_t)i * n_max * k_max;
+      for (int d = 0; d < n_soap; ++d) pvec[d] = soap[(size_t)i * n_soap + d];
+      const int k3 = st.k_first[i];
+      /* pass A: G and the unnormalised central-atom derivative */
+      memset(Gacc, 0, sizeof(double complex) * 3 * n_max * k_max);
+      memset(dp1, 0, sizeof(double) * 3 * n_soap);
+      for (int j = 1; j < in->n_neigh[i]; ++j) {
+        int k2 = k3 + j;
+        if (!pair_geometry(&g, p, in, k2, rcut_max, 0)) continue;
+        pair_unnorm_der(&st, p, c, in, cn, k2, tr, ta, to, Pw, dp2);
+        for (int d = 0; d < 3 * n_soap; ++d) dp1[d] -= dp2[d];
+        for (int k = 0; k < k_max; ++k) {
+          double f0 = g.Fl[lofk[k]], f1 = g.Fl1[lofk[k]];
+          for (int n = 0; n < n_max; ++n) {
+            double r0 = fourpi * st.rad[(size_t)k2 * n_max + n], r1 = fourpi * st.rad_der[(size_t)k2 * n_max + n];
+            double grad0 = r0 * f0, grad1 = r1 * f0 + r0 * f1;
+            for (int a = 0; a < 3; ++a)
+              Gacc[(a * n_max + n) * k_max + k] += grad1 * g.nh[a] * g.Om[k] + grad0 * g.dOm[3 * k + a];
+          }
+        }
+      }
+      /* the weight vectors, uncompressed (P^T, skipped components cleared), their S matrices */
+      for (int v = 0; v < nv; ++v) {
+        const double* vin = v < n_vec ? vecs + ((size_t)i * n_vec + v) * n_soap : pvec;
+        memset(vunc, 0, sizeof(double) * n_unc);
+        if (c && c->compress) {
+          for (int e = 0; e < c->P_nonzero; ++e) vunc[c->P_j[e] - 1] += c->P_el[e] * vin[c->P_i[e] - 1];
+        } else {
+          for (int d = 0; d < n_soap; ++d) vunc[d] = vin[d];
+        }
+        for (int u = 0; u < n_unc; ++u)
+          if (st.skip[u]) vunc[u] = 0.0;
+        memset(Smat, 0, sizeof(double) * n_max * n_max * (L + 1));
+        int cnt = 0;
+        for (int n = 0; n < n_max; ++n)
+          for (int np = n; np < n_max; ++np)
+            for (int l = 0; l <= L; ++l, ++cnt) {
+              if (n == np) {
+                Smat[(l * n_max + n) * n_max + n] = 2.0 * vunc[cnt];
+              } else {
+                Smat[(l * n_max + n) * n_max + np] = sqrt2 * vunc[cnt];
+                Smat[(l * n_max + np) * n_max + n] = sqrt2 * vunc[cnt];
+              }
+            }
+        for (int k = 0; k < k_max; ++k) {
+          int l = lofk[k];
+          for (int n = 0; n < n_max; ++n) {
+            double complex zc = 0.0, b0[3] = {0.0, 0.0, 0.0};
+            for (int np = 0; np < n_max; ++np) {
+              double sv = Smat[(l * n_max + n) * n_max + np];
+              zc += sv * (cn[2 * (np * k_max + k)] + I * cn[2 * (np * k_max + k) + 1]);
+              for (int a = 0; a < 3; ++a) b0[a] += sv * Gacc[(a * n_max + np) * k_max + k];
+            }
+            Zv[((size_t)v * n_max + n) * k_max + k] = zc;
+            for (int a = 0; a < 3; ++a) ZGv[(((size_t)v * 3 + a) * n_max + n) * k_max + k] = b0[a];
+          }
+        }
+      }
+      /* pass B: the per-pair contraction */
+      memset(hsum, 0, sizeof(double) * 9 * (n_vec > 0 ? n_vec : 1));
+      for (int j = 1; j < in->n_neigh[i]; ++j) {
+        int k2 = k3 + j;
+        if (!pair_geometry(&g, p, in, k2, rcut_max, 1)) continue;
+        pair_unnorm_der(&st, p, c, in, cn, k2, tr, ta, to, Pw, dp2);
+        memset(hv, 0, sizeof(double) * 9 * nv);
+        memset(u2rec, 0, sizeof(double) * 3 * nv);
+        const double* r = st.rad + (size_t)k2 * n_max;
+        const double* rd = st.rad_der + (size_t)k2 * n_max;
+        const double* rh = rad2 + (size_t)k2 * n_max;
+        const double invr = g.invr, *nh = g.nh;
+        for (int k = 0; k < k_max; ++k) {
+          int l = lofk[k];
+          double f0 = g.Fl[l], f1 = g.Fl1[l], f2 = g.Fl2[l], wmk = wm[k];
+          double complex om = g.Om[k];
+          const double complex* dom = g.dOm + 3 * k;
+          for (int v = 0; v < nv; ++v) {
+            double complex a0 = 0, a1 = 0, a2 = 0, b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
+            for (int n = 0; n < n_max; ++n) {
+              double complex zc = conj(Zv[((size_t)v * n_max + n) * k_max + k]);
+              a0 += r[n] * zc;
+              a1 += rd[n] * zc;
+              a2 += rh[n] * zc;
+              for (int a = 0; a < 3; ++a) {
+                double complex gc = conj(ZGv[(((size_t)v * 3 + a) * n_max + n) * k_max + k]);
+                b0[a] += r[n] * gc;
+                b1[a] += rd[n] * gc;
+              }
+            }
+            double complex z0 = fourpi * f0 * a0;
+            double complex z1 = fourpi * (f0 * a1 + f1 * a0);
+            double complex z2 = fourpi * (f0 * a2 + 2.0 * f1 * a1 + f2 * a0);
+            double complex gg0[3], gg1[3];
+            for (int a = 0; a < 3; ++a) {
+              gg0[a] = fourpi * f0 * b0[a];
+              gg1[a] = fourpi * (f0 * b1[a] + f1 * b0[a]);
+            }
+            for (int b = 0; b < 3; ++b) u2rec[3 * v + b] += wmk * creal(z1 * nh[b] * om + z0 * dom[b]);
+            for (int b = 0; b < 3; ++b)
+              for (int a = 0; a < 3; ++a) {
+                double complex tE = (z2 - z1 * invr) * nh[a] * nh[b] * om + z1 * (nh[a] * dom[b] + nh[b] * dom[a]) +
+                                    z0 * g.d2Om[9 * k + 3 * a + b];
+                if (a == b) tE += z1 * invr * om;
+                double complex tD = gg1[a] * nh[b] * om + gg0[a] * dom[b];
+                hv[9 * v + 3 * a + b] -= wmk * (creal(tE) + creal(tD));
+              }
+          }
+        }
+        /* the self-test: the first derivative from the same z0, z1 against pair_dp */
+        for (int v = 0; v < nv; ++v)
+          for (int b = 0; b < 3; ++b) {
+            const double* w_ = v < n_vec ? vecs + ((size_t)i * n_vec + v) * n_soap : pvec;
+            double refd = 0.0;
+            for (int d = 0; d < n_soap; ++d) refd += w_[d] * dp2[3 * d + b];
+            double e = fabs(u2rec[3 * v + b] - refd);
+            chk_err = e > chk_err ? e : chk_err;
+            chk_ref = fabs(refd) > chk_ref ? fabs(refd) : chk_ref;
+          }
+        /* normalisation, q = p / s */
+        double t1[3], t2[3], c12[3][3];
+        for (int a = 0; a < 3; ++a) {
+          t1[a] = t2[a] = 0.0;
+          for (int d = 0; d < n_soap; ++d) { t1[a] += pvec[d] * dp1[3 * d + a]; t2[a] += pvec[d] * dp2[3 * d + a]; }
+        }
+        for (int b = 0; b < 3; ++b)
+          for (int a = 0; a < 3; ++a) {
+            c12[a][b] = 0.0;
+            for (int d = 0; d < n_soap; ++d) c12[a][b] += dp1[3 * d + a] * dp2[3 * d + b];
+          }
+        for (int v = 0; v < n_vec; ++v) {
+          const double* w_ = vecs + ((size_t)i * n_vec + v) * n_soap;
+          double wp = 0.0, u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
+          for (int d = 0; d < n_soap; ++d) wp += w_[d] * pvec[d];
+          for (int a = 0; a < 3; ++a)
+            for (int d = 0; d < n_soap; ++d) { u1[a] += w_[d] * dp1[3 * d + a]; u2[a] += w_[d] * dp2[3 * d + a]; }
+          double* h = hess + ((size_t)k2 * n_vec + v) * 9;
+          for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+              h[3 * a + b] = hv[9 * v + 3 * a + b] * invs - (u1[a] * t2[b] + u2[b] * t1[a]) * invs3 -
+                             wp * (c12[a][b] + hv[9 * n_vec + 3 * a + b]) * invs3 + 3.0 * wp * t1[a] * t2[b] * invs5;
+              hsum[9 * v + 3 * a + b] += h[3 * a + b];
+            }
+        }
+      }
+      for (int v = 0; v < n_vec; ++v)
+        for (int e = 0; e < 9; ++e) hess[((size_t)k3 * n_vec + v) * 9 + e] = -hsum[9 * v + e];
+    }
+    free(g.Om); free(g.dOm); free(g.d2Om); free(Gacc); free(Zv); free(ZGv); free(Smat); free(vunc); free(pvec);
+    free(dp1); free(dp2); free(tr); free(ta); free(to); free(Pw); free(hv); free(u2rec); free(hsum);
+  }
+  for (int i = 0; i < in->n_sites; ++i)
+    for (int d = 0; d < n_soap; ++d) soap[(size_t)i * n_soap + d] /= sdp[i];
+  if (selftest_err) *selftest_err = chk_err / (chk_ref > 1e-300 ? chk_ref : 1e-300);
+  free(lofk); free(mofk); free(wm); free(Ck); free(preflm); free(rad2); free(sdp);
   state_free(&st);
   return 0;
 }

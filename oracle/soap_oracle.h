@@ -51,6 +51,15 @@ int so_n_soap(const so_params* p, const so_compress* c);
 void so_preflm(int l_max, double* preflm);
 void so_overlap_S(int basis, int alpha_max, double atom_sigma_r, double rcut_hard, double* S);
 void so_radial(const so_params* p, const so_inputs* in, const double* W, int do_der, double* rad, double* rad_der);
+void so_radial2(const so_params* p, const so_inputs* in, const double* W, int do_der, double* rad, double* rad_der,
+                double* rad_der2);
+/* get_soap_central_hessian (src/soap_turbo.f90:1151-1700) after get_soap with derivatives:
+ * vecs (n_sites, n_vec, n_soap) C, hess (n_pairs, n_vec, 3 a, 3 b) C, soap (n_sites, n_soap)
+ * normalised; rad_der2 (n_pairs, n_max) may be NULL. Returns 1 for legacy_seed (the reference
+ * stops there). */
+int so_central_hessian(const so_params* p, const so_inputs* in, const double* W, const double* S,
+                       const so_compress* c, int n_vec, const double* vecs, double* soap, double* hess,
+                       double* rad_der2, double* selftest_err);
 void so_angular(const so_params* p, const so_inputs* in, int do_der, double* ang, double* ang_r, double* ang_a,
                 double* ang_p);
 /* soap (unnormalised before return? no: normalised, as get_soap returns it);

@@ -9,7 +9,8 @@ seeded dE/dq. Run in a container where /root/reference exists:
 
     python tests/golden/make_golden.py [case ...]
 
-(no names: every case). The darby_* cases use the reference's own compression indices for the
+(no names: every case). The hess_* cases (tests/golden/hessian/) hold get_soap_central_hessian's
+output for two weight vectors (src/soap_turbo.f90:1151-1700). The darby_* cases use the reference's own compression indices for the
 nu_R_nu_S modes of get_compress_indices (src/soap_turbo_compress.f90:120-298).
 """
 from __future__ import annotations
@@ -132,9 +133,48 @@ def make_case(name, spec, params, n_sites, legacy):
     return w.n_sites, w.n_pairs, n_soap
 
 
+# central-atom Hessian cases (tests/golden/hessian/): get_soap_central_hessian for two weight vectors
+def _hess(basis, mode, alpha_max, l_max, **kw):
+    return (SMALL_BOX, SoapParams(alpha_max=alpha_max, l_max=l_max, rcut_hard=4.6, rcut_soft=4.0, basis=basis,
+                                  atom_sigma_r=[0.45, 0.5], atom_sigma_t=[0.5, 0.45], compress_mode=mode, **kw), 4)
+
+
+HESSIAN_CASES = {
+    "hess_poly3gauss": _hess("poly3gauss", "none", [4, 3], 5, atom_sigma_r_scaling=[0.05, 0.0],
+                             atom_sigma_t_scaling=[0.02, 0.05], central_weight=[1.0, 0.5]),
+    "hess_poly3_trivial": _hess("poly3", "trivial", [5, 4], 4, radial_enhancement=2, amplitude_scaling=[1.5, 0.0],
+                                central_weight=[0.7, 1.0]),
+    "hess_darby_1_1": _hess("poly3gauss", "1_1", [3, 3], 6, radial_enhancement=1),
+}
+
+
+def make_hessian_case(name, spec, params, n_sites, n_vec=2):
+    w = make_workload((spec, params), n_sites=n_sites)
+    n_soap = n_soap_of(params)
+    vecs = normal(3000 + n_sites, (w.n_sites, n_vec, n_soap))
+    o = refio.run(w, None, modes=refio.MODE_HESSIAN, legacy_seed=False, vecs=vecs)
+    ps = params.per_species()
+    out = dict(
+        n_neigh=w.n_neigh, species=w.species[:, 0], species_multiplicity=w.species_multiplicity,
+        mask=w.mask.astype(np.int8), rjs=w.rjs, thetas=w.thetas, phis=w.phis, nbr=w.nbr,
+        alpha_max=np.asarray(params.alpha_max, dtype=np.int32), l_max=params.l_max,
+        radial_enhancement=params.radial_enhancement, basis=params.basis, scaling_mode=params.scaling_mode,
+        compress_mode=params.compress_mode, legacy_seed=False, vecs=vecs, W=o["W"], S=o["S"],
+        soap=o["soap_hess"], hess=o["hess"], rad_der2=o["rad_der2"], selftest_err=o["hess_selftest_err"],
+        **{k: v for k, v in ps.items()})
+    if "P_i" in o:
+        out.update(P_i=o["P_i"], P_j=o["P_j"], P_el=o["P_el"])
+    os.makedirs(os.path.join(HERE, "hessian"), exist_ok=True)
+    np.savez_compressed(os.path.join(HERE, "hessian", name + ".npz"), **out)
+    return w.n_sites, w.n_pairs, n_soap
+
+
 if __name__ == "__main__":
     if not refio.available():
         raise SystemExit("build the reference harness first: make -C oracle ref")
-    for name in sys.argv[1:] or CASES:
+    for name in sys.argv[1:] or [*CASES, *HESSIAN_CASES]:
+        if name in HESSIAN_CASES:
+            print(name, *make_hessian_case(name, *HESSIAN_CASES[name]))
+            continue
         spec, params, ns, legacy = CASES[name]
         print(name, *make_case(name, spec, params, ns, legacy))
