@@ -22,6 +22,8 @@
  *   st_adjoint_f32          no reference counterpart: dE/dq -> pair gradients, forces and virial in FP32
  *                           without forming soap_cart_der (the FP32 form of soap_contract, :1703-1721)
  *   st_build_neighbours     no reference counterpart: the callers' neighbour list, built on the device
+ *   st_central_hessian      get_soap_central_hessian, src/soap_turbo.f90:1151-1700 (the caller calls
+ *                           it after its get_soap; here it runs that pass itself)
  *   st_set_gap_model, st_gap_energy_forces, st_gap_site_energies
  *                           no reference counterpart: a GAP-form energy model on the descriptors
  *                           (what get_soap's callers feed back as dE/dq), evaluated on the device
@@ -176,6 +178,15 @@ int st_download_pairs(st_ctx* ctx, int32_t which, double* pair);   /* which: 0 F
 int st_download_forces(st_ctx* ctx, int32_t which, double* forces, double* virial);
 /* soap(n_soap, n_sites) of the last st_adjoint_f32, widened to FP64. */
 int st_download_soap_f32(st_ctx* ctx, double* soap);
+
+/* get_soap_central_hessian (src/soap_turbo.f90:1151-1700) on the resident neighbour list, FP64:
+ * runs the descriptor pass itself (st_compute without derivatives, plus the radial second
+ * derivatives), then hess(3, 3, n_vec, n_pairs) with hess(a, b, v, k) =
+ * sum_d vecs(d, v, i) d^2 q_d / dr_ia dr_jb for pair k = (site i, neighbour j); the self pair's
+ * block is d^2 / dr_ia dr_ib. vecs(n_soap, n_vec, n_sites) are weight vectors in the space of the
+ * (normalised, compressed) descriptor. Needs legacy_filter_seed = 0, as the reference does. */
+int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t n_vec, const double* vecs,
+                       double* hess);
 
 /* ---- GAP energy model (no reference counterpart: the callers' energy model that supplies get_soap
  * with dE/dq, here on the device) ----

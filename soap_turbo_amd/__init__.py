@@ -209,6 +209,16 @@ class Context:
         self.check(lib().st_download_soap_f32(self.h, ptr(soap)))
         return soap
 
+    def central_hessian(self, params: StParams, comp: StCompression, vecs):
+        """get_soap_central_hessian on the resident list: vecs (n_soap, n_vec, n_sites) Fortran order;
+        returns hess (3, 3, n_vec, n_pairs), hess[a, b, v, k] = sum_d vecs[d, v, i] d2 q_d / dr_ia dr_jb."""
+        vecs = np.asfortranarray(vecs, dtype=np.float64)
+        n_vec = vecs.shape[1]
+        hess = np.zeros((3, 3, n_vec, self.n_pairs), order="F")
+        self.check(lib().st_central_hessian(self.h, ctypes.byref(params), ctypes.byref(comp), n_vec, ptr(vecs),
+                                            ptr(hess)))
+        return hess
+
     # ---- GAP energy model on the device
     def set_gap_model(self, sparse_q, alpha, zeta: int, delta: float):
         """sparse_q (n_soap, n_sparse), alpha (n_sparse): E = sum_i delta^2 sum_t alpha_t (q_i . q_t)^zeta."""

@@ -65,6 +65,7 @@ SIGNATURES = {
     "st_download_pairs": (ctypes.c_int, [P, I32, P]),
     "st_download_forces": (ctypes.c_int, [P, I32, P, P]),
     "st_download_soap_f32": (ctypes.c_int, [P, P]),
+    "st_central_hessian": (ctypes.c_int, [P, P, P, I32, P, P]),
     "st_set_gap_model": (ctypes.c_int, [P, I32, I32, P, P, I32, ctypes.c_double]),
     "st_gap_energy_forces": (ctypes.c_int, [P, P, P, ctypes.POINTER(ctypes.c_double)]),
     "st_gap_site_energies": (ctypes.c_int, [P, P]),

@@ -24,7 +24,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "lib", "llvm", "bin", "flang")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kernels_f64.hip", "adjoint.hip", "adjoint_f32.hip", "neighbours.hip", "gap.hip", "api.hip"]
+HIP_SOURCES = ["kernels_f64.hip", "adjoint.hip", "adjoint_f32.hip", "neighbours.hip", "gap.hip", "hessian.hip", "api.hip"]
 CXX_SOURCES = ["basis.cpp"]
 # packed-FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",

@@ -94,6 +94,8 @@ struct st_ctx {
   int gap_n_sparse = 0, gap_n_soap = 0, gap_zeta = 0;
   double gap_delta = 0.0;
   DevBuf gap_T, gap_alpha, gap_esite, gap_energy;
+  // central-atom Hessian (st_central_hessian): second radial derivatives, C_k, weight vectors, scratch
+  DevBuf rad_der2, hess_ck, hess_vecs, hess_scratch, hess_out;
   std::string g32_key, g32_warm;
   hipGraph_t g32_graph = nullptr;
   hipGraphExec_t g32_exec = nullptr;
@@ -280,7 +282,7 @@ int basis_for(st_ctx* ctx, const st_params* p) {
 // Per-species radial constants: radial_expansion_range set-up,
 // src/soap_turbo_radial.f90:436-540, computed with the same FP64 operations.
 st::RadialConsts radial_consts(const st_params* p, int s, const std::vector<double>& W, int n_max, int off,
-                               bool do_der) {
+                               bool do_der, bool do_der2 = false) {
   st::RadialConsts c;
   std::memset(&c, 0, sizeof c);
   const int a = p->alpha_max[s];
@@ -292,7 +294,8 @@ st::RadialConsts radial_consts(const st_params* p, int s, const std::vector<doub
   c.do_central = c.with_gauss ? 0 : (p->central_weight[s] != 0.0);
   c.unified = !p->legacy_filter_seed;
   c.radial_enh = p->radial_enhancement;
-  c.do_der = do_der;
+  c.do_der = do_der || do_der2;
+  c.do_der2 = do_der2;
   c.scaling = p->atom_sigma_r_scaling[s];
   c.amp_scaling = p->amplitude_scaling[s];
   c.amp_mode = p->scaling_mode != ST_SCALING_POLYNOMIAL ? 0
@@ -316,7 +319,7 @@ st::RadialConsts radial_consts(const st_params* p, int s, const std::vector<doub
     c.q = c.drnf * c.drnf;
   }
   c.n_poly = c.with_gauss ? a - 1 : a;
-  c.n_hi = c.n_poly + (do_der ? 2 : 0);
+  c.n_hi = c.n_poly + (do_der2 ? 4 : do_der ? 2 : 0);
   double Nv[ST_REC + 8];  // Nv(n) at n + 3
   Nv[0] = 1.0;
   for (int n = -2; n <= a + 4; ++n) Nv[n + 3] = std::sqrt(1.0 / (double)(2 * n + 5));
@@ -330,6 +333,8 @@ st::RadialConsts radial_consts(const st_params* p, int s, const std::vector<doub
   for (int n = 1; n <= c.n_poly; ++n) {
     c.R1[n] = Nv[n + 1 + 3] / Nv[n + 3];
     c.R2[n] = Nv[n + 2 + 3] / Nv[n + 3];
+    c.R3[n] = Nv[n + 3 + 3] / Nv[n + 3];
+    c.R4[n] = Nv[n + 4 + 3] / Nv[n + 3];
   }
   for (int j = 0; j < a; ++j)
     for (int i = 0; i < a; ++i) c.W[i + j * a] = W[(off + i) + (size_t)(off + j) * n_max];
@@ -1292,6 +1297,78 @@ int st_gap_site_energies(st_ctx* ctx, double* e_site) {
   if (!ctx->gap_esite.p) return fail(ctx, ST_ERR_STATE, "no GAP energies computed");
   HIPCHK(hipMemcpyAsync(e_site, ctx->gap_esite.p, (size_t)ctx->n_sites * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  return ST_OK;
+}
+
+int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t n_vec, const double* vecs,
+                       double* hess) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
+  if (n_vec < 1 || !vecs || !hess) return fail(ctx, ST_ERR_ARG, "n_vec >= 1, vecs and hess are required");
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (p->legacy_filter_seed)
+    return fail(ctx, ST_ERR_STOP,
+                "ERROR: second radial derivatives need soap_turbo_radial's legacy_filter_seed = .false. .");
+  // the descriptors, cnk and norms of the FP64 pass, then the radial expansion with both derivatives
+  if ((rc = st_compute(ctx, p, c, 0, 0))) return rc;
+  const int n_max = ctx->n_max, k_max = ctx->k_max, L = ctx->l_max, n_soap = ctx->n_soap, n_unc = ctx->n_soap_unc;
+  const int NS = ctx->n_sites;
+  const size_t P = ctx->n_pairs;
+  hipStream_t st = ctx->stream;
+  if ((rc = ensure(ctx, ctx->rad_der, P * n_max * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->rad_der2, P * n_max * 8))) return rc;
+  st::launch_fill(st, ctx->rad.as<double>(), P * n_max, 0.0);
+  st::launch_fill(st, ctx->rad_der.as<double>(), P * n_max, 0.0);
+  st::launch_fill(st, ctx->rad_der2.as<double>(), P * n_max, 0.0);
+  int off = 0;
+  for (int s = 0; s < p->n_species; ++s) {
+    st::RadialConsts rcst = radial_consts(p, s, ctx->W, n_max, off, true, true);
+    bool ok = true;
+    timed(ctx, "radial_d2_f64", [&] {
+      ok = st::launch_radial(st, rcst, (int)P, ctx->rjs.as<double>(), ctx->spmask.as<uint32_t>(),
+                             ctx->first.as<uint8_t>(), ctx->rad.as<double>(), ctx->rad_der.as<double>(),
+                             ctx->rad_der2.as<double>());
+    });
+    if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "alpha_max out of the compiled range");
+    off += p->alpha_max[s];
+  }
+  // Omega = C_k Q_l^m(t) w^m carries the (-1)^m between P_l^m and Q_l^m
+  std::vector<double> pf, ck(k_max);
+  preflm_host(L, pf);
+  for (int l = 0, k = 0; l <= L; ++l)
+    for (int m = 0; m <= l; ++m, ++k) ck[k] = m % 2 == 0 ? pf[k] : -pf[k];
+  if ((rc = upload(ctx, ctx->hess_ck, ck.data(), ck.size() * 8))) return rc;
+  if ((rc = upload(ctx, ctx->hess_vecs, vecs, (size_t)NS * n_vec * n_soap * 8))) return rc;
+  const size_t stride = st::hessian_scratch_per_site(n_max, k_max, n_soap, n_unc, n_vec);
+  if ((rc = ensure(ctx, ctx->hess_scratch, (size_t)std::max(NS, 1) * stride * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->hess_out, P * 9 * n_vec * 8))) return rc;
+  if (st::hessian_lds(n_unc, n_vec) > ctx->lds_max)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "the Hessian's working set exceeds the LDS");
+  std::vector<double> sig(p->n_species), sigs(p->n_species);
+  double rcut_max = 0.0;
+  for (int s = 0; s < p->n_species; ++s) {
+    sig[s] = p->atom_sigma_t[s];
+    sigs[s] = p->atom_sigma_t_scaling[s];
+    rcut_max = std::max(rcut_max, p->rcut_hard[s]);
+  }
+  bool ok = true;
+  timed(ctx, "hessian_f64", [&] {
+    ok = st::launch_hessian(st, NS, (int)P, n_max, L, n_soap, n_unc, n_vec, ctx->compressed ? 1 : 0, p->n_species,
+                            sig.data(), sigs.data(), rcut_max, ctx->site_ptr.as<int>(), ctx->rjs.as<double>(),
+                            ctx->thetas.as<double>(), ctx->phis.as<double>(), ctx->spmask.as<uint32_t>(),
+                            ctx->hess_ck.as<double>(), ctx->rad.as<double>(), ctx->rad_der.as<double>(),
+                            ctx->rad_der2.as<double>(), ctx->cnk.as<double2>(), ctx->soap.as<double>(),
+                            ctx->sdp.as<double>(), ctx->hess_vecs.as<double>(), ctx->comp.as<int4>(),
+                            ctx->skip.as<uint8_t>(), ctx->compressed ? ctx->out_ptr.as<int>() : nullptr,
+                            ctx->out_c.as<int>(), ctx->out_el.as<double>(), ctx->unc_ptr.as<int>(),
+                            ctx->unc_e.as<int>(), ctx->P_i.as<int32_t>(), ctx->P_el.as<double>(),
+                            ctx->hess_scratch.as<double>(), ctx->hess_out.as<double>());
+  });
+  if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max or n_species out of the Hessian's compiled range");
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(hess, ctx->hess_out.p, P * 9 * n_vec * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return ST_OK;
 }
 

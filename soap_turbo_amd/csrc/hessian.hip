@@ -1,0 +1,542 @@
+// Central-atom Hessian of the descriptor, contracted with weight vectors, in FP64:
+// get_soap_central_hessian, src/soap_turbo.f90:1151-1700. For pair k2 = (site i, neighbour j),
+//   hess(a, b, v, k2) = sum_d vecs(d, v, i) d^2 q_d / dr_ia dr_jb,
+// and the self pair's block is minus the sum of the others. q = p / |p| is the normalised
+// (compressed) descriptor.
+//
+// Three kernels, one workgroup per site, on the resident FP64 state of the descriptor pass (cnk,
+// the radial expansion with first and second derivatives, the normalised descriptor and its norm):
+//   hess_gacc_k     G(n, k, a) = sum_j dc^j_(n,k) / du_ja over the site's pairs (pass A);
+//   hess_weights_k  the central-atom derivative D1 = dp/dr_i from G and c (an accumulated form of
+//                   minus the sum of the per-pair derivatives), then for each weight vector (the
+//                   caller's, p itself, and the three columns of D1) its uncompressed S matrix and
+//                   Z = S c, and, for the caller's vectors and p, ZG = S G;
+//   hess_pairs_k    per pair, Omega = C_k Q_l^m(t) w^m with its Cartesian gradient and Hessian, the
+//                   contraction against Z and ZG, and the quotient rule of the normalisation (pass B).
+// Dot products of the per-pair derivative with a vector (p.dp2, w.dp2, D1.dp2) come out of the same
+// contraction as first derivatives, so the per-pair derivative itself is never formed.
+#include <hip/hip_runtime.h>
+
+#include "../../include/soap_turbo_hip.h"
+#include "kernels.h"
+#include "st_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace st {
+
+namespace {
+
+constexpr int HT = 256;  // threads per workgroup
+constexpr int MAXL = ST_MAX_LMAX;
+constexpr int MAXK = 1 + MAXL * (MAXL + 1) / 2 + MAXL;
+
+struct HessParams {
+  int n_sites, n_pairs, n_max, k_max, l_max, n_soap, n_unc, n_vec, n_species, compressed;
+  double rcut_max;
+  double sigma_t[ST_MAX_SPECIES], sigma_t_scaling[ST_MAX_SPECIES];
+  // per-site scratch: offsets (in doubles) of each array in one site's slot, and the slot size
+  size_t o_gacc, o_z, o_zg, o_d1u, o_d1c, o_pvec, o_scal, stride;
+};
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+
+// get_ilexp, src/soap_turbo_angular.f90:361-422, to lmax (<= MAXL + 2)
+__device__ void ilexp_dev(int lmax, double xx, double* il) {
+  const double xcut = 1e-7;
+  double x2 = xx * xx, x4 = x2 * x2;
+  double inv_x2 = x2 > 0.0 ? 1.0 / x2 : 0.0;
+  double x2pow = x2 * x2;
+  double flm2, flm1;
+  if (xx > 0.0) {
+    double e = exp(-2.0 * x2);
+    flm2 = fabs((1.0 - e) / 2.0 / x2);
+    flm1 = fabs((x2 - 1.0 + e * (x2 + 1.0)) / 2.0 / x4);
+  } else {
+    flm2 = 1.0;
+    flm1 = 0.0;
+  }
+  double fact = 1.0;
+  il[0] = xx < xcut ? 1.0 - x2 : flm2;
+  if (lmax >= 1) {
+    fact = fact * 3.0;
+    il[1] = x2 / 1000.0 < xcut ? (x2 - x4) / fact : flm1;
+  }
+  for (int l = 2; l <= lmax; ++l) {
+    fact = fact * (2.0 * (double)l + 1.0);
+    double ratio = x2pow / fact, fl;
+    if (ratio * (double)l < xcut)
+      fl = ratio;
+    else
+      fl = fabs(flm2 - (2.0 * (double)l - 1.0) * inv_x2 * flm1);
+    x2pow = x2pow * x2;
+    flm2 = flm1;
+    flm1 = fl;
+    il[l] = fl;
+  }
+}
+
+// One pair's geometry in LDS: pair_geometry, src/soap_turbo.f90:1585-1697.
+struct PairGeom {
+  int ok;
+  double invr, nh[3];
+  double Fl[MAXL + 1], Fl1[MAXL + 1], Fl2[MAXL + 1];
+  double Qt[MAXK + 2];
+  double2 wp[MAXL + 3];  // w^m at m + 2 (w^-2 = w^-1 = 0)
+  double tvec[3], thes[9];
+  double2 wvec[3], whes[9];
+  double2 Om[MAXK], dOm[3 * MAXK], d2Om[9 * MAXK];
+};
+
+// Fills g for pair k2; every thread of the workgroup calls it (barriers inside).
+__device__ void pair_geometry_dev(PairGeom& g, const HessParams& P, int k2, const double* __restrict__ rjs,
+                                  const double* __restrict__ thetas, const double* __restrict__ phis,
+                                  const uint32_t* __restrict__ spmask, const double* __restrict__ Ck, bool want2) {
+  const int tx = threadIdx.x, L = P.l_max, k_max = P.k_max;
+  __syncthreads();  // every thread is done with the previous pair's g (also after a skipped pair)
+  if (tx == 0) {
+    const double rr = rjs[k2];
+    const uint32_t msk = spmask[k2];
+    g.ok = !(rr >= P.rcut_max || rr < 1e-10) && msk != 0;
+    if (g.ok) {
+      const int isp = __ffs(msk) - 1;
+      const double invr = 1.0 / rr;
+      g.invr = invr;
+      const double sscal = P.sigma_t_scaling[isp];
+      const double sig = P.sigma_t[isp] + sscal * rr;
+      const double amp = P.rcut_max * P.rcut_max / (sig * sig);
+      const double ampd = -2.0 * sscal / sig * amp;
+      const double ampdd = 6.0 * (sscal * sscal) / (sig * sig) * amp;
+      const double xarg = rr / sig, zz = xarg * xarg, invz = 1.0 / zz;
+      double il[MAXL + 3];
+      ilexp_dev(L + 2, xarg, il);
+      const double zp = 2.0 * rr / (sig * sig) - 2.0 * (rr * rr) * sscal / (sig * sig * sig);
+      const double zp2 = 2.0 / (sig * sig) - 8.0 * rr * sscal / (sig * sig * sig) +
+                         6.0 * (rr * rr) * (sscal * sscal) / (sig * sig * sig * sig);
+      for (int l = 0; l <= L; ++l) {
+        const double rl = (double)l;
+        const double dilz = il[l + 1] + (rl * invz - 1.0) * il[l];
+        const double d2ilz = il[l + 2] + ((double)(2 * l + 1) * invz - 2.0) * il[l + 1] +
+                             (rl * (rl - 1.0) * invz * invz - 2.0 * rl * invz + 1.0) * il[l];
+        const double f0 = il[l], f1 = dilz * zp, f2 = d2ilz * zp * zp + dilz * zp2;
+        g.Fl[l] = amp * f0;
+        g.Fl1[l] = ampd * f0 + amp * f1;
+        g.Fl2[l] = ampdd * f0 + 2.0 * ampd * f1 + amp * f2;
+      }
+      const double sin_t = sin(thetas[k2]), cos_t = cos(thetas[k2]);
+      const double sin_f = sin(phis[k2]), cos_f = cos(phis[k2]);
+      const double nh0 = sin_t * cos_f, nh1 = sin_t * sin_f, nh2 = cos_t;
+      g.nh[0] = nh0; g.nh[1] = nh1; g.nh[2] = nh2;
+      const double tt = nh2;
+      const double2 wc = make_double2(nh0, -nh1);
+      const double2 ec[3] = {make_double2(1.0, 0.0), make_double2(0.0, -1.0), make_double2(0.0, 0.0)};
+      const double nh[3] = {nh0, nh1, nh2};
+      for (int a = 0; a < 3; ++a) {
+        g.tvec[a] = -tt * nh[a] * invr;
+        g.wvec[a] = cadd(cscale(-nh[a] * invr, wc), cscale(invr, ec[a]));
+      }
+      g.tvec[2] += invr;
+      if (want2) {
+        const double invr2 = invr * invr;
+        for (int b = 0; b < 3; ++b)
+          for (int a = 0; a < 3; ++a) {
+            g.thes[3 * a + b] = 3.0 * tt * nh[a] * nh[b] * invr2;
+            double2 w3 = cscale(3.0 * nh[a] * nh[b], wc);
+            w3.x -= ec[a].x * nh[b] + ec[b].x * nh[a];
+            w3.y -= ec[a].y * nh[b] + ec[b].y * nh[a];
+            g.whes[3 * a + b] = cscale(invr2, w3);
+          }
+        for (int a = 0; a < 3; ++a) {
+          g.thes[3 * 2 + a] -= nh[a] * invr2;
+          g.thes[3 * a + 2] -= nh[a] * invr2;
+          g.thes[3 * a + a] -= tt * invr2;
+          g.whes[3 * a + a] = make_double2(g.whes[3 * a + a].x - wc.x * invr2, g.whes[3 * a + a].y - wc.y * invr2);
+        }
+      }
+      g.wp[0] = g.wp[1] = make_double2(0.0, 0.0);
+      g.wp[2] = make_double2(1.0, 0.0);
+      for (int m = 1; m <= L; ++m) g.wp[m + 2] = cmul(g.wp[m + 1], wc);
+    }
+  }
+  // Q_l^m(t) = d^m P_l / dt^m: column m by thread m, recursion in l (the reference's recursion)
+  if (tx <= L) {
+    const int m = tx;
+    const double tt = cos(thetas[k2]);
+    // diagonal Q_m^m = (2m-1)!!, then Q_(m+1)^m, then the three-term recursion
+    double qmm = 1.0;
+    for (int l = 1; l <= m; ++l) qmm = (double)(2 * l - 1) * qmm;
+    g.Qt[m * (m + 1) / 2 + m] = qmm;
+    if (m + 1 <= L) g.Qt[(m + 1) * (m + 2) / 2 + m] = (double)(2 * m + 1) * tt * qmm;
+    for (int l = m + 2; l <= L; ++l) {
+      const int kq = l * (l + 1) / 2 + m;
+      g.Qt[kq] = ((double)(2 * l - 1) * tt * g.Qt[kq - l] - (double)(l + m - 1) * g.Qt[kq - 2 * l + 1]) / (double)(l - m);
+    }
+  }
+  __syncthreads();
+  if (!g.ok) return;
+  for (int k = tx; k < k_max; k += HT) {
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= k) ++l;
+    const int mm = k - l * (l + 1) / 2;
+    const double q0 = g.Qt[k], q1 = mm + 1 <= l ? g.Qt[k + 1] : 0.0, q2 = mm + 2 <= l ? g.Qt[k + 2] : 0.0;
+    const double C = Ck[k], dm = (double)mm;
+    const double2 w0 = g.wp[mm + 2], w1 = g.wp[mm + 1], w2 = g.wp[mm];
+    g.Om[k] = cscale(C * q0, w0);
+    for (int a = 0; a < 3; ++a)
+      g.dOm[3 * k + a] = cscale(C, cadd(cscale(q1 * g.tvec[a], w0), cscale(q0 * dm, cmul(w1, g.wvec[a]))));
+    if (want2)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          double2 t = cscale(q2 * g.tvec[a] * g.tvec[b], w0);
+          t = cadd(t, cscale(q1 * g.thes[3 * a + b], w0));
+          double2 tw = cadd(cscale(g.tvec[a], g.wvec[b]), cscale(g.tvec[b], g.wvec[a]));
+          t = cadd(t, cscale(q1 * dm, cmul(w1, tw)));
+          t = cadd(t, cscale(q0 * (double)(mm * (mm - 1)), cmul(w2, cmul(g.wvec[a], g.wvec[b]))));
+          t = cadd(t, cscale(q0 * dm, cmul(w1, g.whes[3 * a + b])));
+          g.d2Om[9 * k + 3 * a + b] = cscale(C, t);
+        }
+  }
+  __syncthreads();
+}
+
+// block sum of NV values (every thread passes its partials; thread 0 gets the totals)
+template <int NV>
+__device__ void block_sum(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[w * NV + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = red[q] + red[NV + q] + red[2 * NV + q] + red[3 * NV + q];
+  __syncthreads();
+}
+
+// pass A: G(a, n, k) = sum_j [g'(r) n_a Omega + g(r) dOmega_a] with g = 4 pi rad F_l
+__global__ __launch_bounds__(HT) void hess_gacc_k(HessParams P, const int* __restrict__ site_ptr,
+                                                 const double* __restrict__ rjs, const double* __restrict__ thetas,
+                                                 const double* __restrict__ phis, const uint32_t* __restrict__ spmask,
+                                                 const double* __restrict__ Ck, const double* __restrict__ rad,
+                                                 const double* __restrict__ rad_der, double* __restrict__ scratch) {
+  __shared__ PairGeom g;
+  const int i = blockIdx.x, tx = threadIdx.x;
+  const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max;
+  const double fourpi = 4.0 * 3.141592653589793;
+  double2* G = reinterpret_cast<double2*>(scratch + (size_t)i * P.stride + P.o_gacc);
+  for (int e = tx; e < 3 * E; e += HT) G[e] = make_double2(0.0, 0.0);
+  const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
+  for (int k2 = k3 + 1; k2 < k4; ++k2) {
+    pair_geometry_dev(g, P, k2, rjs, thetas, phis, spmask, Ck, false);
+    if (!g.ok) continue;
+    for (int e = tx; e < E; e += HT) {
+      const int n = e / k_max, k = e - n * k_max;
+      int l = 0;
+      while ((l + 1) * (l + 2) / 2 <= k) ++l;
+      const double r0 = fourpi * rad[(size_t)k2 * n_max + n], r1 = fourpi * rad_der[(size_t)k2 * n_max + n];
+      const double grad0 = r0 * g.Fl[l], grad1 = r1 * g.Fl[l] + r0 * g.Fl1[l];
+      for (int a = 0; a < 3; ++a)
+        G[a * E + e] = cadd(G[a * E + e], cadd(cscale(grad1 * g.nh[a], g.Om[k]), cscale(grad0, g.dOm[3 * k + a])));
+    }
+  }
+}
+
+// D1 = dp/dr_i, the weight vectors and Z = S c, ZG = S G
+__global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2* __restrict__ cnk,
+                                                    const double* __restrict__ soap, const double* __restrict__ sdp,
+                                                    const double* __restrict__ vecs, const int4* __restrict__ comp,
+                                                    const uint8_t* __restrict__ skip, const int* __restrict__ out_ptr,
+                                                    const int* __restrict__ out_c, const double* __restrict__ out_el,
+                                                    const int* __restrict__ unc_ptr, const int* __restrict__ unc_e,
+                                                    const int32_t* __restrict__ P_i, const double* __restrict__ P_el,
+                                                    double* __restrict__ scratch) {
+  extern __shared__ double vunc[];  // n_unc
+  __shared__ double red[4 * 8];
+  const int i = blockIdx.x, tx = threadIdx.x;
+  const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, L1 = P.l_max + 1;
+  const int n_unc = P.n_unc, n_soap = P.n_soap, n_vec = P.n_vec;
+  const double sq2 = 1.4142135623730951;
+  double* base = scratch + (size_t)i * P.stride;
+  const double2* G = reinterpret_cast<const double2*>(base + P.o_gacc);
+  double2* Z = reinterpret_cast<double2*>(base + P.o_z);
+  double2* ZG = reinterpret_cast<double2*>(base + P.o_zg);
+  double* D1u = base + P.o_d1u;
+  double* D1c = base + P.o_d1c;
+  double* pvec = base + P.o_pvec;
+  double* scal = base + P.o_scal;
+  const double2* cn = cnk + (size_t)i * E;
+  // D1(n, n', l, a) = -base sum_m w(m) Re(G_n,a conj(c_n') + c_n conj(G_n',a)), skipped components 0
+  for (int u = tx; u < n_unc; u += HT) {
+    const int4 cm = comp[u];
+    const int n = cm.x, np = cm.y, l = cm.z;
+    double d[3] = {0.0, 0.0, 0.0};
+    if (!skip[u]) {
+      const double bs = n == np ? 1.0 : sq2;
+      for (int m = 0; m <= l; ++m) {
+        const int k = l * (l + 1) / 2 + m;
+        const double w = m == 0 ? 1.0 : 2.0;
+        const double2 c1 = cn[n * k_max + k], c2 = cn[np * k_max + k];
+        for (int a = 0; a < 3; ++a) {
+          const double2 g1 = G[a * E + n * k_max + k], g2 = G[a * E + np * k_max + k];
+          d[a] += w * (g1.x * c2.x + g1.y * c2.y + c1.x * g2.x + c1.y * g2.y);
+        }
+      }
+      for (int a = 0; a < 3; ++a) d[a] = -bs * d[a];
+    }
+    for (int a = 0; a < 3; ++a) D1u[a * n_unc + u] = d[a];
+  }
+  __syncthreads();
+  // compressed D1 and the unnormalised descriptor p
+  const double s = sdp[i];
+  for (int q = tx; q < n_soap; q += HT) {
+    for (int a = 0; a < 3; ++a) {
+      double v = 0.0;
+      if (P.compressed) {
+        for (int e = out_ptr[q]; e < out_ptr[q + 1]; ++e) v += out_el[e] * D1u[a * n_unc + out_c[e]];
+      } else {
+        v = D1u[a * n_unc + q];
+      }
+      D1c[a * n_soap + q] = v;
+    }
+    pvec[q] = soap[(size_t)i * n_soap + q] * s;
+  }
+  __syncthreads();
+  // per-site dot products: t1(a) = p.D1_a; for each caller vector, w.p and w.D1_a
+  {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = tx; q < n_soap; q += HT)
+      for (int a = 0; a < 3; ++a) v[a] += pvec[q] * D1c[a * n_soap + q];
+    block_sum<4>(v, red);
+    if (tx == 0)
+      for (int a = 0; a < 3; ++a) scal[a] = v[a];
+    for (int vv = 0; vv < n_vec; ++vv) {
+      const double* w = vecs + ((size_t)i * n_vec + vv) * n_soap;
+      double x[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int q = tx; q < n_soap; q += HT) {
+        x[0] += w[q] * pvec[q];
+        for (int a = 0; a < 3; ++a) x[a + 1] += w[q] * D1c[a * n_soap + q];
+      }
+      block_sum<4>(x, red);
+      if (tx == 0)
+        for (int a = 0; a < 4; ++a) scal[3 + 4 * vv + a] = x[a];
+    }
+  }
+  // weight vectors: the caller's (n_vec), p, then the columns of D1
+  const int nvz = n_vec + 4;
+  for (int v = 0; v < nvz; ++v) {
+    const double* vin = v < n_vec ? vecs + ((size_t)i * n_vec + v) * n_soap : v == n_vec ? pvec : D1c + (v - n_vec - 1) * n_soap;
+    __syncthreads();
+    // P^T v, skipped components cleared (spread_weights, src/soap_turbo.f90:1510-1525)
+    for (int u = tx; u < n_unc; u += HT) {
+      double x = 0.0;
+      if (!skip[u]) {
+        if (P.compressed) {
+          for (int e = unc_ptr[u]; e < unc_ptr[u + 1]; ++e) x += P_el[unc_e[e]] * vin[P_i[unc_e[e]] - 1];
+        } else {
+          x = vin[u];
+        }
+      }
+      vunc[u] = x;
+    }
+    __syncthreads();
+    // Z(n, k) = sum_n' S(n, n', l) c(n', k), S = 2 v on the diagonal, sqrt 2 v off it
+    const bool with_g = v <= n_vec;
+    for (int e = tx; e < E; e += HT) {
+      const int n = e / k_max, k = e - n * k_max;
+      int l = 0;
+      while ((l + 1) * (l + 2) / 2 <= k) ++l;
+      double2 z = make_double2(0.0, 0.0), zg[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+      for (int np = 0; np < n_max; ++np) {
+        const int a_ = n < np ? n : np, b_ = n < np ? np : n;
+        const int u = ((a_ * (2 * n_max - a_ + 1)) / 2 + (b_ - a_)) * L1 + l;
+        const double sv = (n == np ? 2.0 : sq2) * vunc[u];
+        z = cadd(z, cscale(sv, cn[np * k_max + k]));
+        if (with_g)
+          for (int a = 0; a < 3; ++a) zg[a] = cadd(zg[a], cscale(sv, G[a * E + np * k_max + k]));
+      }
+      Z[(size_t)v * E + e] = z;
+      if (with_g)
+        for (int a = 0; a < 3; ++a) ZG[((size_t)v * 3 + a) * E + e] = zg[a];
+    }
+  }
+}
+
+// pass B: per pair, the contraction and the normalisation
+__global__ __launch_bounds__(HT) void hess_pairs_k(HessParams P, const int* __restrict__ site_ptr,
+                                                  const double* __restrict__ rjs, const double* __restrict__ thetas,
+                                                  const double* __restrict__ phis, const uint32_t* __restrict__ spmask,
+                                                  const double* __restrict__ Ck, const double* __restrict__ rad,
+                                                  const double* __restrict__ rad_der, const double* __restrict__ rad_der2,
+                                                  const double* __restrict__ sdp, const double* __restrict__ scratch,
+                                                  double* __restrict__ hess) {
+  __shared__ PairGeom g;
+  extern __shared__ double res[];  // (n_vec + 4) x 12: u2(3) and hv(9) per weight vector
+  const int i = blockIdx.x, tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
+  const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, n_vec = P.n_vec, nvz = n_vec + 4;
+  const double fourpi = 4.0 * 3.141592653589793;
+  const double* base = scratch + (size_t)i * P.stride;
+  const double2* Z = reinterpret_cast<const double2*>(base + P.o_z);
+  const double2* ZG = reinterpret_cast<const double2*>(base + P.o_zg);
+  const double* scal = base + P.o_scal;
+  const double s = sdp[i], invs = 1.0 / s, invs3 = invs / (s * s), invs5 = invs3 / (s * s);
+  const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
+  // thread t < 9 n_vec owns block entry (v, a, b) = (t / 9, (t % 9) / 3, t % 3) and its running sum
+  double hsum = 0.0;
+  for (int k2 = k3 + 1; k2 < k4; ++k2) {
+    pair_geometry_dev(g, P, k2, rjs, thetas, phis, spmask, Ck, true);
+    if (!g.ok) {
+      if (tx < 9 * n_vec) {
+        const int v = tx / 9, a = (tx % 9) / 3, b = tx % 3;
+        hess[(((size_t)k2 * n_vec + v) * 3 + b) * 3 + a] = 0.0;
+      }
+      continue;
+    }
+    const double* r = rad + (size_t)k2 * n_max;
+    const double* rd = rad_der + (size_t)k2 * n_max;
+    const double* rh = rad_der2 + (size_t)k2 * n_max;
+    const double invr = g.invr;
+    for (int v = wv; v < nvz; v += HT / 64) {
+      const bool second = v <= n_vec;
+      double acc[12];
+#pragma unroll
+      for (int q = 0; q < 12; ++q) acc[q] = 0.0;
+      for (int k = lane; k < k_max; k += 64) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) ++l;
+        const int m = k - l * (l + 1) / 2;
+        const double f0 = g.Fl[l], f1 = g.Fl1[l], f2 = g.Fl2[l], wmk = m > 0 ? 2.0 : 1.0;
+        double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0, b0[3] = {a0, a0, a0}, b1[3] = {a0, a0, a0};
+        for (int n = 0; n < n_max; ++n) {
+          double2 zc = Z[(size_t)v * E + n * k_max + k];
+          zc.y = -zc.y;
+          a0 = cadd(a0, cscale(r[n], zc));
+          a1 = cadd(a1, cscale(rd[n], zc));
+          if (second) {
+            a2 = cadd(a2, cscale(rh[n], zc));
+            for (int a = 0; a < 3; ++a) {
+              double2 gc = ZG[((size_t)v * 3 + a) * E + n * k_max + k];
+              gc.y = -gc.y;
+              b0[a] = cadd(b0[a], cscale(r[n], gc));
+              b1[a] = cadd(b1[a], cscale(rd[n], gc));
+            }
+          }
+        }
+        const double2 z0 = cscale(fourpi * f0, a0);
+        const double2 z1 = cscale(fourpi, cadd(cscale(f0, a1), cscale(f1, a0)));
+        const double2 om = g.Om[k];
+        for (int b = 0; b < 3; ++b) {
+          const double2 t = cadd(cmul(cscale(g.nh[b], z1), om), cmul(z0, g.dOm[3 * k + b]));
+          acc[b] += wmk * t.x;
+        }
+        if (second) {
+          const double2 z2 = cscale(fourpi, cadd(cadd(cscale(f0, a2), cscale(2.0 * f1, a1)), cscale(f2, a0)));
+          double2 gg0[3], gg1[3];
+          for (int a = 0; a < 3; ++a) {
+            gg0[a] = cscale(fourpi * f0, b0[a]);
+            gg1[a] = cscale(fourpi, cadd(cscale(f0, b1[a]), cscale(f1, b0[a])));
+          }
+          const double2 zr = make_double2(z2.x - z1.x * invr, z2.y - z1.y * invr);
+          for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+              double2 tE = cmul(cscale(g.nh[a] * g.nh[b], zr), om);
+              tE = cadd(tE, cmul(z1, cadd(cscale(g.nh[a], g.dOm[3 * k + b]), cscale(g.nh[b], g.dOm[3 * k + a]))));
+              tE = cadd(tE, cmul(z0, g.d2Om[9 * k + 3 * a + b]));
+              if (a == b) tE = cadd(tE, cmul(cscale(invr, z1), om));
+              const double2 tD = cadd(cmul(cscale(g.nh[b], gg1[a]), om), cmul(gg0[a], g.dOm[3 * k + b]));
+              acc[3 + 3 * a + b] -= wmk * (tE.x + tD.x);
+            }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 12; ++q)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+      if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 12; ++q) res[v * 12 + q] = acc[q];
+    }
+    __syncthreads();
+    // normalisation, q = p / s (src/soap_turbo.f90:1453-1485): with d1 = d/dr_ia, d2 = d/dr_jb,
+    //   w.d1d2 q = (w.d1d2 p)/s - [(w.d1 p)(p.d2 p) + (w.d2 p)(p.d1 p)]/s^3
+    //              - (w.p)[(d1 p.d2 p) + (p.d1d2 p)]/s^3 + 3 (w.p)(p.d1 p)(p.d2 p)/s^5
+    if (tx < 9 * n_vec) {
+      const int v = tx / 9, a = (tx % 9) / 3, b = tx % 3;
+      const double* rp = res + n_vec * 12;  // p's row: u2 = p.dp2, hv = p.d1d2 p
+      const double t1a = scal[a], t2b = rp[b];
+      const double wp = scal[3 + 4 * v], u1a = scal[3 + 4 * v + 1 + a];
+      const double u2b = res[v * 12 + b];
+      const double c12 = res[(n_vec + 1 + a) * 12 + b];  // D1_a . dp2_b
+      const double h = res[v * 12 + 3 + 3 * a + b] * invs - (u1a * t2b + u2b * t1a) * invs3 -
+                       wp * (c12 + rp[3 + 3 * a + b]) * invs3 + 3.0 * wp * t1a * t2b * invs5;
+      hess[(((size_t)k2 * n_vec + v) * 3 + b) * 3 + a] = h;
+      hsum += h;
+    }
+    __syncthreads();  // res and g are rewritten by the next pair
+  }
+  if (tx < 9 * n_vec) {
+    const int v = tx / 9, a = (tx % 9) / 3, b = tx % 3;
+    hess[(((size_t)k3 * n_vec + v) * 3 + b) * 3 + a] = -hsum;
+  }
+}
+
+}  // namespace
+
+size_t hessian_scratch_per_site(int n_max, int k_max, int n_soap, int n_unc, int n_vec) {
+  const size_t E = (size_t)n_max * k_max;
+  const size_t n = 2 * E * 3 + 2 * E * (n_vec + 4) + 2 * E * 3 * (n_vec + 1) + 3 * (size_t)n_unc +
+                   3 * (size_t)n_soap + (size_t)n_soap + 3 + 4 * (size_t)n_vec + 8;
+  return (n + 31) / 32 * 32;  // each site's slot starts on a 256-byte boundary
+}
+
+size_t hessian_lds(int n_unc, int n_vec) {
+  const size_t geom = sizeof(PairGeom);
+  const size_t w = (size_t)n_unc * 8 + 32 * 8;
+  const size_t pr = geom + (size_t)(n_vec + 4) * 12 * 8;
+  return w > pr ? w : pr;
+}
+
+bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int n_soap, int n_unc, int n_vec,
+                    int compressed, int n_species, const double* sigma_t, const double* sigma_t_scaling,
+                    double rcut_max, const int* site_ptr, const double* rjs, const double* thetas, const double* phis,
+                    const uint32_t* spmask, const double* Ck, const double* rad, const double* rad_der,
+                    const double* rad_der2, const double2* cnk, const double* soap, const double* sdp,
+                    const double* vecs, const int4* comp, const uint8_t* skip, const int* out_ptr, const int* out_c,
+                    const double* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
+                    const double* P_el, double* scratch, double* hess) {
+  if (l_max > MAXL || n_species > ST_MAX_SPECIES || n_vec < 1) return false;
+  HessParams P;
+  P.n_sites = n_sites; P.n_pairs = n_pairs; P.n_max = n_max; P.k_max = 1 + l_max * (l_max + 1) / 2 + l_max;
+  P.l_max = l_max; P.n_soap = n_soap; P.n_unc = n_unc; P.n_vec = n_vec; P.n_species = n_species;
+  P.compressed = compressed; P.rcut_max = rcut_max;
+  for (int s = 0; s < ST_MAX_SPECIES; ++s) {
+    P.sigma_t[s] = s < n_species ? sigma_t[s] : 0.0;
+    P.sigma_t_scaling[s] = s < n_species ? sigma_t_scaling[s] : 0.0;
+  }
+  const size_t E = (size_t)n_max * P.k_max;
+  P.o_gacc = 0;
+  P.o_z = P.o_gacc + 2 * E * 3;
+  P.o_zg = P.o_z + 2 * E * (n_vec + 4);
+  P.o_d1u = P.o_zg + 2 * E * 3 * (n_vec + 1);
+  P.o_d1c = P.o_d1u + 3 * (size_t)n_unc;
+  P.o_pvec = P.o_d1c + 3 * (size_t)n_soap;
+  P.o_scal = P.o_pvec + n_soap;
+  P.stride = hessian_scratch_per_site(n_max, P.k_max, n_soap, n_unc, n_vec);
+  if (n_sites == 0) return true;
+  hess_gacc_k<<<n_sites, HT, 0, st>>>(P, site_ptr, rjs, thetas, phis, spmask, Ck, rad, rad_der, scratch);
+  const size_t wl = (size_t)n_unc * 8;
+  if (wl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_weights_k),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)wl);
+  hess_weights_k<<<n_sites, HT, wl, st>>>(P, cnk, soap, sdp, vecs, comp, skip, out_ptr, out_c, out_el, unc_ptr,
+                                          unc_e, P_i, P_el, scratch);
+  const size_t pl = (size_t)(n_vec + 4) * 12 * 8;
+  hess_pairs_k<<<n_sites, HT, pl, st>>>(P, site_ptr, rjs, thetas, phis, spmask, Ck, rad, rad_der, rad_der2, sdp,
+                                        scratch, hess);
+  return true;
+}
+
+}  // namespace st

@@ -10,8 +10,9 @@ namespace st {
 void launch_pack_pairs(hipStream_t st, int n_pairs, int n_species, const int32_t* mask, const int* site_ptr,
                        int n_sites, uint32_t* spmask, uint8_t* first);
 void launch_fill(hipStream_t st, double* x, size_t n, double v);
+// rad_der2 non-null: also the second derivative (the constants must be built for it)
 bool launch_radial(hipStream_t st, const RadialConsts& c, int n_pairs, const double* rjs, const uint32_t* spmask,
-                   const uint8_t* first, double* rad, double* rad_der);
+                   const uint8_t* first, double* rad, double* rad_der, double* rad_der2 = nullptr);
 bool launch_angular(hipStream_t st, const AngularConsts& c, int n_pairs, const double* rjs, const double* th,
                     const double* ph, const uint32_t* spmask, const double* preflm, double2* a, double2* ar,
                     double2* aa, double2* ap);
@@ -58,5 +59,17 @@ void launch_neighbour_fill(hipStream_t st, int n_sites, int n_pairs, int n_speci
                            int* overflow);
 void launch_atom_pairs(hipStream_t st, int n_atoms, int n_pairs, const int* pair_atom, int* count, int* cursor,
                        int* atom_ptr, int* atom_pairs, int* overflow);
+
+// central-atom Hessian (hessian.hip): scratch doubles per site, LDS bytes, launch
+size_t hessian_scratch_per_site(int n_max, int k_max, int n_soap, int n_unc, int n_vec);
+size_t hessian_lds(int n_unc, int n_vec);
+bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int n_soap, int n_unc, int n_vec,
+                    int compressed, int n_species, const double* sigma_t, const double* sigma_t_scaling,
+                    double rcut_max, const int* site_ptr, const double* rjs, const double* thetas, const double* phis,
+                    const uint32_t* spmask, const double* Ck, const double* rad, const double* rad_der,
+                    const double* rad_der2, const double2* cnk, const double* soap, const double* sdp,
+                    const double* vecs, const int4* comp, const uint8_t* skip, const int* out_ptr, const int* out_c,
+                    const double* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
+                    const double* P_el, double* scratch, double* hess);
 
 }  // namespace st

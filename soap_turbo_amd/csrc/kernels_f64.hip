@@ -53,10 +53,14 @@ __global__ void fill_k(double* __restrict__ x, size_t n, double v) {
 // ---------------------------------------------------------------- radial
 // One thread per pair: process_block of src/soap_turbo_radial.f90:550-1002
 // for one species block, then global_scaling (src/soap_turbo.f90:510-511).
-template <int A>
+// D2: also the second derivative in r (unified form, :838-861, :895-906, :917-920, :993-999),
+// which carries the recursions two steps further.
+template <int A, bool D2>
 __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, const double* __restrict__ rjs,
                                                 const uint32_t* __restrict__ spmask, const uint8_t* __restrict__ first,
-                                                double* __restrict__ rad, double* __restrict__ rad_der) {
+                                                double* __restrict__ rad, double* __restrict__ rad_der,
+                                                double* __restrict__ rad_der2) {
+  constexpr int EX = D2 ? 4 : 2;  // recursion steps beyond the basis
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_pairs) return;
   const bool is_central = first[p] != 0;
@@ -68,26 +72,35 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
   const double pi = 3.141592653589793, sq2 = 1.4142135623730951;
   const double isq2 = 1.0 / sq2, sqpi2 = sqrt(pi / 2.0), sq2pi = sqrt(2.0 / pi), sq8pi = sqrt(8.0 / pi);
   const double sc = c.scaling, rs = c.rcut_soft;
-  double T1[A + 6], T2[A + 6], pol[A + 3], DT[A + 1];  // T(n) at n+3, pol(n)/DT(n) at n
+  double T1[A + 4 + EX], T2[A + 4 + EX], pol[A + 1 + EX], DT[A + 1], D2T[D2 ? A + 1 : 1];  // T(n) at n+3
 #pragma unroll
-  for (int n = 0; n < A + 6; ++n) T2[n] = 0.0;
+  for (int n = 0; n < A + 4 + EX; ++n) T2[n] = 0.0;
 #pragma unroll
   for (int n = 0; n <= A; ++n) DT[n] = 0.0;
+  if constexpr (D2) {
+#pragma unroll
+    for (int n = 0; n <= A; ++n) D2T[n] = 0.0;
+  }
 
   const double rj = rjs_p * c.inv_rcut;
   const double sig = c.atom_sigma + sc * rj, s2 = sig * sig, isig = 1.0 / sig;
   const double is2 = isig * isig, is3 = is2 * isig;
-  double amp, ampd;
+  double amp, ampd, ampd2 = 0.0;
   if (c.amp_mode == 0) {
     amp = 1.0; ampd = 0.0;
   } else if (c.amp_mode == 1) {
     amp = isig; ampd = -sc * is2;
+    if constexpr (D2) ampd2 = 2.0 * sc * sc * is3;
   } else if (c.amp_mode == 2) {
     double h = 1.0 + 2.0 * (rj * rj * rj) - 3.0 * (rj * rj);
     double hp = 6.0 * (rj * rj) - 6.0 * rj;
     double g = isig * h;
     amp = h <= 1e-10 ? 0.0 : g;
     ampd = h <= 1e-10 ? 0.0 : hp * isig - sc * isig * g;
+    if constexpr (D2) {
+      double hpp = 12.0 * rj - 6.0;
+      ampd2 = h <= 1e-10 ? 0.0 : hpp * isig - 2.0 * sc * hp * is2 + 2.0 * sc * sc * h * is3;
+    }
   } else {
     double h = 1.0 + 2.0 * (rj * rj * rj) - 3.0 * (rj * rj);
     if (h <= 1e-10) {
@@ -96,19 +109,30 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
       double hp = 6.0 * (rj * rj) - 6.0 * rj, aa = c.amp_scaling;
       amp = isig * pow(h, aa);
       ampd = aa * pow(h, aa - 1.0) * hp * isig - sc * isig * amp;
+      if constexpr (D2) {
+        double hpp = 12.0 * rj - 6.0;
+        ampd2 = (aa * (aa - 1.0) * pow(h, aa - 2.0) * hp * hp + aa * pow(h, aa - 1.0) * hpp) * isig -
+                2.0 * sc * aa * pow(h, aa - 1.0) * hp * is2 + 2.0 * sc * sc * pow(h, aa) * is3;
+      }
     }
   }
   if (c.do_central && c.central_weight != 1.0 && is_central) {
     amp = c.central_weight * amp;
     ampd = c.central_weight * ampd;
+    if constexpr (D2) ampd2 = c.central_weight * ampd2;
   }
   if (c.radial_enh == 1) {
     double g = rj + sq2pi * sig, f = 1.0 + sq2pi * sc;
+    if constexpr (D2) ampd2 = ampd2 * g + 2.0 * ampd * f;
     ampd = amp * f + ampd * g;
     amp = amp * g;
   } else if (c.radial_enh == 2) {
     double g = (rj * rj) + s2 + sq8pi * sig * rj;
     double f = 2.0 * rj + 2.0 * sig * sc + sq8pi * sig + sq8pi * rj * sc;
+    if constexpr (D2) {
+      double h = 2.0 + 2.0 * sc * sc + 2.0 * sq8pi * sc;
+      ampd2 = ampd2 * g + 2.0 * ampd * f + amp * h;
+    }
     ampd = amp * f + ampd * g;
     amp = amp * g;
   }
@@ -120,7 +144,7 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
   T1[0] = 0.0;
   T1[1] = sqpi2 * sig * (erf((rs - rj) * isq2 * isig) + erf(rj * isq2 * isig)) * c.inv_Nm2;
 #pragma unroll
-  for (int n = -1; n <= A + 2; ++n) {
+  for (int n = -1; n <= A + EX; ++n) {
     if (n > c.n_hi) break;
     if (n >= 0) C1 = C1 * c.dr;
     T1[n + 3] = s2 * c.ra[n + 1] * T1[n + 1] - c.rb[n + 1] * (rj - 1.0) * T1[n + 2] + (C1 - C2) * c.rc[n + 1];
@@ -146,7 +170,7 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
     T2[0] = 0.0;
     T2[1] = sqpi2 * sgf * (erf((1.0 - rjf) * isq2 * isgf) - erf((rs - rjf) * isq2 * isgf)) * c.inv_Nm2;
 #pragma unroll
-    for (int n = -1; n <= A + 2; ++n) {
+    for (int n = -1; n <= A + EX; ++n) {
       if (n > c.n_hi) break;
       if (n >= 0) C2f = C2f * c.dr;
       T2[n + 3] = sf2 * c.ra[n + 1] * T2[n + 1] - c.rb[n + 1] * (rjf - 1.0) * T2[n + 2] - C2f * c.rc[n + 1];
@@ -168,7 +192,7 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
     }
   }
 #pragma unroll
-  for (int n = 1; n <= A + 2; ++n)
+  for (int n = 1; n <= A + EX; ++n)
     if (n <= c.n_hi) pol[n] = T1[n + 3] + pref * T2[n + 3];
   if (c.do_der && do_poly && c.unified) {
     double f = 1.0 - rj;
@@ -178,6 +202,21 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
 #pragma unroll
     for (int n = 1; n <= A; ++n)
       if (n <= c.n_poly) DT[n] = g0 * pol[n] + c.R1[n] * g1 * pol[n + 1] + c.R2[n] * g2 * pol[n + 2];
+    if constexpr (D2) {
+      double is4 = is2 * is2, is5 = is4 * isig, is6 = is4 * is2;
+      double k0 = -is2, k1 = -4.0 * sc * is3, k2 = (1.0 - 3.0 * sc * sc) * is4, k3 = 2.0 * sc * is5,
+             k4 = sc * sc * is6;
+      double B0 = k0 + k1 * f + k2 * f * f + k3 * (f * f * f) + k4 * (f * f * f * f);
+      double B1 = -(k1 + 2.0 * k2 * f + 3.0 * k3 * f * f + 4.0 * k4 * (f * f * f));
+      double B2 = k2 + 3.0 * k3 * f + 6.0 * k4 * f * f;
+      double B3 = -(k3 + 4.0 * k4 * f);
+      double B4 = k4;
+#pragma unroll
+      for (int n = 1; n <= A; ++n)
+        if (n <= c.n_poly)
+          D2T[n] = B0 * pol[n] + c.R1[n] * B1 * pol[n + 1] + c.R2[n] * B2 * pol[n + 2] + c.R3[n] * B3 * pol[n + 3] +
+                   c.R4[n] * B4 * pol[n + 4];
+    }
   }
   if (c.with_gauss) {
     double ss2 = c.sigma_sq0 + s2, iss = 1.0 / ss2, siss = sqrt(iss);
@@ -192,6 +231,13 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
       double M1 = sc0 * ge - pp * M0;
       double M2 = sc0 * ge * (-rj) - pp * M1 + sc0 * M0;
       DT[A] = M1 * is2 + sc * M2 * is3;
+      if constexpr (D2) {
+        double M3 = sc0 * ge * (rj * rj) - pp * M2 + 2.0 * sc0 * M1;
+        double M4 = sc0 * ge * (-(rj * rj * rj)) - pp * M3 + 3.0 * sc0 * M2;
+        double is4 = is2 * is2, is5 = is4 * isig, is6 = is4 * is2;
+        D2T[A] = -M0 * is2 - 4.0 * sc * M1 * is3 + (1.0 - 3.0 * sc * sc) * M2 * is4 + 2.0 * sc * M3 * is5 +
+                 sc * sc * M4 * is6;
+      }
     }
   }
   const double gs = c.global_scaling;
@@ -207,6 +253,19 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
 #pragma unroll
       for (int m = 2; m <= A; ++m) acc = acc + c.W[(n - 1) + (m - 1) * A] * dtot[m];
       outd[n - 1] = acc * c.inv_sqrt_rcut * gs;
+    }
+    if constexpr (D2) {
+      double d2tot[A + 1];
+#pragma unroll
+      for (int n = 1; n <= A; ++n) d2tot[n] = amp * D2T[n] + 2.0 * ampd * DT[n] + ampd2 * pol[n];
+      double* outh = rad_der2 + (size_t)p * c.n_max + c.off;
+#pragma unroll
+      for (int n = 1; n <= A; ++n) {
+        double acc = c.W[n - 1] * d2tot[1];
+#pragma unroll
+        for (int m = 2; m <= A; ++m) acc = acc + c.W[(n - 1) + (m - 1) * A] * d2tot[m];
+        outh[n - 1] = acc * c.inv_sqrt_rcut * c.inv_rcut * gs;
+      }
     }
   }
 #pragma unroll
@@ -1049,15 +1108,18 @@ void launch_fill(hipStream_t st, double* x, size_t n, double v) {
 
 template <int A>
 static void radial_one(hipStream_t st, const RadialConsts& c, int n_pairs, const double* rjs, const uint32_t* spmask,
-                       const uint8_t* first, double* rad, double* rad_der) {
-  radial_k<A><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, spmask, first, rad, rad_der);
+                       const uint8_t* first, double* rad, double* rad_der, double* rad_der2) {
+  if (rad_der2)
+    radial_k<A, true><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, spmask, first, rad, rad_der, rad_der2);
+  else
+    radial_k<A, false><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, spmask, first, rad, rad_der, nullptr);
 }
 
 bool launch_radial(hipStream_t st, const RadialConsts& c, int n_pairs, const double* rjs, const uint32_t* spmask,
-                   const uint8_t* first, double* rad, double* rad_der) {
+                   const uint8_t* first, double* rad, double* rad_der, double* rad_der2) {
   if (n_pairs == 0) return true;
   switch (c.alpha_max) {
-#define RC(a) case a: radial_one<a>(st, c, n_pairs, rjs, spmask, first, rad, rad_der); return true;
+#define RC(a) case a: radial_one<a>(st, c, n_pairs, rjs, spmask, first, rad, rad_der, rad_der2); return true;
     RC(1) RC(2) RC(3) RC(4) RC(5) RC(6) RC(7) RC(8) RC(9) RC(10) RC(11)
 #undef RC
     default: return false;

@@ -9,7 +9,7 @@
 #define ST_MAX_ALPHA 11     // poly3gauss stops above 11 (src/soap_turbo_radial.f90:210-219)
 #define ST_MAX_LMAX 12      // angular kernels are instantiated for l_max <= 12
 #define ST_MAX_SPECIES 32   // species masks are packed into 32-bit words
-#define ST_REC 16           // recursion slots: alpha_max + 5
+#define ST_REC 20           // recursion slots: alpha_max + 6 (second derivatives), rounded up
 
 namespace st {
 
@@ -17,11 +17,11 @@ namespace st {
 // with the operations of radial_expansion_range (src/soap_turbo_radial.f90:436-540).
 struct RadialConsts {
   int alpha_max, n_poly, n_hi, off, n_max;
-  int with_gauss, do_central, has_filter, unified, amp_mode, radial_enh, do_der;
+  int with_gauss, do_central, has_filter, unified, amp_mode, radial_enh, do_der, do_der2;
   uint32_t species_bit;
   double rcut_hard_in, rcut_soft, atom_sigma, dr, inv_rcut, sqrt_rcut, inv_sqrt_rcut, inv_Nm2, sigma_sq0, N_gauss;
   double q, drnf, scaling, amp_scaling, central_weight, global_scaling;
-  double ra[ST_REC], rb[ST_REC], rc[ST_REC], R1[ST_REC], R2[ST_REC];
+  double ra[ST_REC], rb[ST_REC], rc[ST_REC], R1[ST_REC], R2[ST_REC], R3[ST_REC], R4[ST_REC];
   double W[ST_MAX_ALPHA * ST_MAX_ALPHA];  // species block of W, column-major alpha x alpha
 };
 
