@@ -65,6 +65,12 @@ module soap_turbo_c_api
       type(c_ptr), value :: s
       integer(c_size_t) :: c_strlen
     end function
+    function st_central_hessian(ctx, p, cmp, n_vec, vecs, hess) bind(c, name="st_central_hessian")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, p, cmp, vecs, hess
+      integer(c_int32_t), value :: n_vec
+      integer(c_int) :: st_central_hessian
+    end function
   end interface
 
 contains

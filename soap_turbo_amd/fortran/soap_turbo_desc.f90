@@ -18,14 +18,22 @@ module soap_turbo_desc
   integer, parameter :: dp = real64
 
   public :: get_soap, soap_contract, get_soap_central_hessian, assign_species_multiplicity
-  ! the reference's public Hessian flags (src/soap_turbo.f90:76-88); the
-  ! central-atom Hessian itself is not provided by this build
+  ! the reference's public Hessian flags (src/soap_turbo.f90:76-88)
   logical, public, save :: soap_hessian_enabled = .false.
   logical, public, save :: soap_hessian_selftest = .false.
   real(dp), public, save :: soap_hessian_selftest_err = 0.d0
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: ct_n_sites = 0, ct_n_pairs = 0, ct_n_soap = 0
+  ! what get_soap_central_hessian needs from the get_soap before it (that call's parameters; the
+  ! neighbour list stays resident in the library): the reference's hs_* state, src/soap_turbo.f90:948-970
+  logical, save :: hs_ready = .false.
+  integer, save :: hs_n_sites = 0, hs_n_pairs = 0, hs_l_max = -1, hs_n_soap = 0
+  type(st_params), target, save :: hs_prm
+  type(st_compression), target, save :: hs_cmp
+  integer(c_int32_t), allocatable, target, save :: hs_am(:), hs_pi(:), hs_pj(:)
+  real(c_double), allocatable, target, save :: hs_rch(:), hs_rcs(:), hs_nf(:), hs_gs(:), hs_sr(:), hs_srs(:), &
+                                               hs_st(:), hs_sts(:), hs_as(:), hs_cw(:), hs_pel(:)
 
 contains
 
@@ -88,10 +96,12 @@ contains
     character(len=:), allocatable :: b
 
     if( do_timing ) call system_clock(t0, rate)
-    if( soap_hessian_enabled .and. do_derivatives )then
-      write(*,*) "ERROR: soap_turbo: the central-atom Hessian is not provided by this build"
+    if( soap_hessian_enabled .and. do_derivatives .and. legacy_filter_seed )then
+      ! the reference refuses second radial derivatives with the legacy seed (src/soap_turbo_radial.f90:318)
+      write(*,*) "ERROR: second radial derivatives need soap_turbo_radial's legacy_filter_seed = .false. ."
       stop
     end if
+    hs_ready = .false.
     do_ctr = 0
     if( present(contract) )then
       if( contract .and. do_derivatives ) do_ctr = 1
@@ -148,6 +158,25 @@ contains
                        do_ctr, n_soap, c_loc(soap), c_null_ptr)
     end if
     call check(rc)
+    if( soap_hessian_enabled .and. do_derivatives )then
+      hs_am = c_am; hs_rch = c_rch; hs_rcs = c_rcs; hs_nf = c_nf; hs_gs = c_gs; hs_sr = c_sr; hs_srs = c_srs
+      hs_st = c_st; hs_sts = c_sts; hs_as = c_as; hs_cw = c_cw
+      hs_prm = st_params(prm%n_species, prm%l_max, prm%radial_enhancement, prm%basis, prm%scaling_mode, &
+                         prm%legacy_filter_seed, c_loc(hs_am), c_loc(hs_rch), c_loc(hs_rcs), c_loc(hs_nf), &
+                         c_loc(hs_gs), c_loc(hs_sr), c_loc(hs_srs), c_loc(hs_st), c_loc(hs_sts), c_loc(hs_as), &
+                         c_loc(hs_cw))
+      if( compress_soap )then
+        hs_pi = c_pi; hs_pj = c_pj; hs_pel = c_pel
+        hs_cmp = st_compression(cmp%P_nonzero, c_loc(hs_pi), c_loc(hs_pj), c_loc(hs_pel))
+      else
+        hs_cmp = st_compression(0_c_int32_t, c_null_ptr, c_null_ptr, c_null_ptr)
+      end if
+      hs_n_sites = n_sites
+      hs_n_pairs = n_atom_pairs
+      hs_l_max = l_max
+      hs_n_soap = n_soap
+      hs_ready = .true.
+    end if
     if( do_ctr == 1 )then
       ct_n_sites = n_sites
       ct_n_pairs = n_atom_pairs
@@ -185,10 +214,49 @@ contains
     call check(rc)
   end subroutine soap_contract
 
-  ! The central-atom Hessian (src/soap_turbo.f90:1151-1700) is not part of this build.
-  subroutine get_soap_central_hessian()
-    write(*,*) "ERROR: soap_turbo: get_soap_central_hessian is not provided by this build"
-    stop
+  ! get_soap_central_hessian with the reference's argument list (src/soap_turbo.f90:1151-1172), after
+  ! a get_soap with soap_hessian_enabled and do_derivatives on the same batch. The library recomputes
+  ! what it needs on the resident neighbour list; hess(a, b, v, k) is sum_d vecs(d, v, i)
+  ! d^2 q_d / dr_ia dr_jb. The self-test of the reference compares a per-pair derivative this build
+  ! does not form: soap_hessian_selftest_err is left at zero.
+  subroutine get_soap_central_hessian(n_sites, n_neigh, n_atom_pairs, n_species, mask, rjs, thetas, phis, l_max, &
+                                      rcut_hard, atom_sigma_t, atom_sigma_t_scaling, compress_soap, &
+                                      compress_P_nonzero, compress_P_i, compress_P_j, compress_P_el, soap, n_vec, &
+                                      vecs, hess)
+    implicit none
+    integer, intent(in) :: n_sites, n_neigh(:), n_atom_pairs, n_species, l_max, n_vec
+    integer, intent(in) :: compress_P_nonzero, compress_P_i(:), compress_P_j(:)
+    logical, intent(in) :: mask(:,:), compress_soap
+    real(dp), intent(in) :: rjs(:), thetas(:), phis(:), rcut_hard(:)
+    real(dp), intent(in) :: atom_sigma_t(:), atom_sigma_t_scaling(:), compress_P_el(:)
+    real(dp), intent(in) :: soap(:,:)
+    real(dp), intent(in), target, contiguous :: vecs(:,:,:)
+    real(dp), intent(out), target, contiguous :: hess(:,:,:,:)
+    integer(c_int) :: rc
+    if( .not. hs_ready .or. .not. c_associated(ctx) )then
+      write(*,*) "ERROR: get_soap_central_hessian was called without a matching get_soap."
+      write(*,*) "       Set soap_hessian_enabled = .true. and call get_soap with"
+      write(*,*) "       do_derivatives = .true. first."
+      stop
+    end if
+    if( n_sites /= hs_n_sites .or. n_atom_pairs /= hs_n_pairs .or. l_max /= hs_l_max .or. &
+        size(soap, 1) /= hs_n_soap )then
+      write(*,*) "ERROR: get_soap_central_hessian was called on a different batch than the"
+      write(*,*) "       get_soap that filled the buffers it reads."
+      stop
+    end if
+    if( size(vecs, 1) /= hs_n_soap .or. size(vecs, 2) < n_vec .or. size(vecs, 3) < n_sites .or. &
+        size(hess, 1) /= 3 .or. size(hess, 2) /= 3 .or. size(hess, 3) /= n_vec .or. size(hess, 4) < n_atom_pairs )then
+      write(*,*) "ERROR: get_soap_central_hessian: vecs(n_soap, n_vec, n_sites) and hess(3, 3, n_vec, n_atom_pairs)"
+      stop
+    end if
+    if( size(vecs, 2) /= n_vec .or. size(vecs, 3) /= n_sites )then
+      write(*,*) "ERROR: get_soap_central_hessian: vecs must be exactly (n_soap, n_vec, n_sites)"
+      stop
+    end if
+    rc = st_central_hessian(ctx, c_loc(hs_prm), c_loc(hs_cmp), int(n_vec, c_int32_t), c_loc(vecs), c_loc(hess))
+    call check(rc)
+    soap_hessian_selftest_err = 0.d0
   end subroutine
 
   ! Species bookkeeping of the caller side (src/soap_turbo.f90:2099-2221): the

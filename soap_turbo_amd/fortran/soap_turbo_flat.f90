@@ -84,4 +84,28 @@ contains
     P_nonzero = nz
   end subroutine
 
+  subroutine st_f90_set_hessian(on) bind(c, name="st_f90_set_hessian")
+    integer(c_int32_t), value :: on
+    soap_hessian_enabled = on /= 0
+  end subroutine
+
+  subroutine st_f90_central_hessian(n_sites, n_species, n_pairs, n_neigh, mask, rjs, thetas, phis, l_max, rcut_hard, &
+                                    atom_sigma_t, atom_sigma_t_scaling, compress, P_nonzero, P_i, P_j, P_el, n_soap, &
+                                    soap, n_vec, vecs, hess) bind(c, name="st_f90_central_hessian")
+    integer(c_int32_t), value :: n_sites, n_species, n_pairs, l_max, compress, P_nonzero, n_soap, n_vec
+    integer(c_int32_t), intent(in) :: n_neigh(n_sites), mask(n_pairs, n_species), P_i(*), P_j(*)
+    real(c_double), intent(in) :: rjs(n_pairs), thetas(n_pairs), phis(n_pairs), rcut_hard(n_species), &
+                                  atom_sigma_t(n_species), atom_sigma_t_scaling(n_species), P_el(*)
+    real(c_double), intent(in) :: soap(n_soap, n_sites), vecs(n_soap, n_vec, n_sites)
+    real(c_double), intent(out) :: hess(3, 3, n_vec, n_pairs)
+    logical, allocatable :: lmask(:,:)
+    integer :: nz
+    nz = max(int(P_nonzero), 1)
+    allocate( lmask(n_pairs, n_species) )
+    lmask = mask /= 0
+    call get_soap_central_hessian(int(n_sites), n_neigh, int(n_pairs), int(n_species), lmask, rjs, thetas, phis, &
+                                  int(l_max), rcut_hard, atom_sigma_t, atom_sigma_t_scaling, compress /= 0, &
+                                  int(P_nonzero), P_i(1:nz), P_j(1:nz), P_el(1:nz), soap, int(n_vec), vecs, hess)
+  end subroutine
+
 end module soap_turbo_flat

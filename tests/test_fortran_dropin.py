@@ -89,3 +89,35 @@ def test_fortran_get_soap_matches_reference(name):
     L.st_f90_soap_contract(ctypes.c_int32(w.n_sites), ctypes.c_int32(q.shape[0]), ctypes.c_int32(w.n_pairs), _p(q),
                            _p(G), ctypes.c_double(1.0), _p(pair))
     assert rel(pair.T, d["pair"]) <= tol
+
+
+@pytest.mark.gpu
+def test_fortran_central_hessian_matches_reference():
+    # soap_hessian_enabled, get_soap with derivatives, then get_soap_central_hessian, as a
+    # reference caller does; the drop-in's own W (poly3gauss, alpha_max 4 and 3) limits parity
+    # with the reference's fixture to about 1e-8
+    import os as _os
+    w, comp, d = load(_os.path.join("hessian", "hess_poly3gauss"))
+    L = flib()
+    L.st_f90_set_hessian(ctypes.c_int32(1))
+    try:
+        soap, _ = run_fortran_get_soap(L, w, comp, d, True, False, False)
+    finally:
+        L.st_f90_set_hessian(ctypes.c_int32(0))
+    p = w.params
+    ps = p.per_species()
+    vecs = np.asfortranarray(d["vecs"].transpose(2, 1, 0))
+    n_soap, n_vec = vecs.shape[0], vecs.shape[1]
+    hess = np.zeros((3, 3, n_vec, w.n_pairs), order="F")
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    keep = [i32(w.n_neigh), i32(w.mask), f64(w.rjs), f64(w.thetas), f64(w.phis), f64(ps["rcut_hard"]),
+            f64(ps["atom_sigma_t"]), f64(ps["atom_sigma_t_scaling"])]
+    z = np.zeros(1, np.int32)
+    I = ctypes.c_int32
+    L.st_f90_central_hessian(I(w.n_sites), I(p.n_species), I(w.n_pairs), *[_p(a) for a in keep[:5]], I(p.l_max),
+                             *[_p(a) for a in keep[5:]], I(0), I(0), _p(z), _p(z), _p(np.zeros(1)), I(n_soap),
+                             _p(soap), I(n_vec), _p(vecs), _p(hess))
+    h = hess.transpose(3, 2, 0, 1)
+    print(f"Fortran drop-in Hessian: rel err {rel(h, d['hess']):.2e}")
+    assert rel(h, d["hess"]) <= 1e-8
