@@ -135,6 +135,47 @@ def time_gap(ctx, w, prm, cmp_, n_soap, steps, warmup, n_sparse=1024, zeta=4):
             "energy": energy[-1]}
 
 
+def time_hessian(w, comp, n_soap, steps, with_cpu, n_vec=3):
+    """The central-atom Hessian (st_central_hessian) for n_vec weight vectors per site (3: the
+    components of a dipole model's gradient), FP64, unified radial form. One call runs the FP64
+    descriptor pass, the radial second derivatives and the three Hessian kernels, and copies
+    hess (3, 3, n_vec, n_pairs) back. The oracle's OpenMP restatement on the host cores is timed
+    beside it on the same input."""
+    import soap_turbo_amd as st
+    from soap_turbo_amd.workload import normal
+    log(f"timing the central-atom Hessian ({n_vec} weight vectors)")
+    p = w.params
+    keep = st._lib.Keep()
+    prm = st.make_params(keep, alpha_max=p.alpha_max, l_max=p.l_max, radial_enhancement=p.radial_enhancement,
+                         basis=p.basis, scaling_mode=p.scaling_mode, legacy_seed=False, **p.per_species())
+    cmp_ = st.make_compression(keep, comp is not None, len(comp[0]) if comp else 0, *(comp or (None, None, None)))
+    vecs = normal(91, (n_soap, n_vec, w.n_sites))
+    ctx = st.Context(0)
+    ctx.upload_neighbours(w.n_neigh, w.species.T, w.species_multiplicity, w.mask.T, w.rjs, w.thetas, w.phis)
+    ctx.central_hessian(prm, cmp_, vecs)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h = ctx.central_hessian(prm, cmp_, vecs)
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    ctx.kernel_timing(True)
+    ctx.central_hessian(prm, cmp_, vecs)
+    kt = {k: ctx.kernel_time(k)[0] for k in ("radial_f64", "angular_f64", "cnk_f64", "power_f64", "radial_d2_f64",
+                                              "hessian_f64")}
+    ctx.kernel_timing(False)
+    out = {"n_vec": n_vec, "ms_per_call": wall, "atoms_per_s": w.n_sites / (wall * 1e-3),
+           "kernels_ms": kt, "hess_bytes": int(h.nbytes)}
+    if with_cpu:
+        from oracle import oracle as orc
+        W, S = ctx.get_basis(prm, sum(p.alpha_max))
+        t0 = time.perf_counter()
+        orc.central_hessian(w, W, S, np.ascontiguousarray(vecs.transpose(2, 1, 0)), comp)
+        cpu = time.perf_counter() - t0
+        out["cpu_port"] = {"ms_per_call": cpu * 1e3, "kind": "port",
+                           "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    return out
+
+
 def cpu_baseline(w, G, cfg):
     """The reference (oracle/_ref, compiled from its sources) on the same workload, all host cores."""
     from oracle import refio
@@ -283,6 +324,7 @@ def main():
                                "max_rel_err_forces_vs_fp64": ferr, "max_rel_err_virial_vs_fp64": verr}
     if have32:
         out["gap_energy_forces"] = time_gap(ctx, w, prm, cmp_, n_soap, args.steps, args.warmup)
+    out["central_hessian"] = time_hessian(w, comp, n_soap, max(3, args.steps // 4), not args.no_cpu)
     out["neighbour_list"] = time_neighbours(w)
     if not args.no_cpu:
         log("CPU baseline")

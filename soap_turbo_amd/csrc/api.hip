@@ -95,7 +95,7 @@ struct st_ctx {
   double gap_delta = 0.0;
   DevBuf gap_T, gap_alpha, gap_esite, gap_energy;
   // central-atom Hessian (st_central_hessian): second radial derivatives, C_k, weight vectors, scratch
-  DevBuf rad_der2, hess_ck, hess_vecs, hess_scratch, hess_out;
+  DevBuf rad_der2, hess_ck, hess_vecs, hess_scratch, hess_heads, hess_out;
   std::string g32_key, g32_warm;
   hipGraph_t g32_graph = nullptr;
   hipGraphExec_t g32_exec = nullptr;
@@ -1343,6 +1343,7 @@ int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c,
   const size_t stride = st::hessian_scratch_per_site(n_max, k_max, n_soap, n_unc, n_vec);
   if ((rc = ensure(ctx, ctx->hess_scratch, (size_t)std::max(NS, 1) * stride * 8))) return rc;
   if ((rc = ensure(ctx, ctx->hess_out, P * 9 * n_vec * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->hess_heads, P * st::hessian_heads_per_pair() * 8))) return rc;
   if (st::hessian_lds(n_unc, n_vec) > ctx->lds_max)
     return fail(ctx, ST_ERR_UNSUPPORTED, "the Hessian's working set exceeds the LDS");
   std::vector<double> sig(p->n_species), sigs(p->n_species);
@@ -1357,13 +1358,14 @@ int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c,
     ok = st::launch_hessian(st, NS, (int)P, n_max, L, n_soap, n_unc, n_vec, ctx->compressed ? 1 : 0, p->n_species,
                             sig.data(), sigs.data(), rcut_max, ctx->site_ptr.as<int>(), ctx->rjs.as<double>(),
                             ctx->thetas.as<double>(), ctx->phis.as<double>(), ctx->spmask.as<uint32_t>(),
-                            ctx->hess_ck.as<double>(), ctx->rad.as<double>(), ctx->rad_der.as<double>(),
+                            ctx->first.as<uint8_t>(), ctx->hess_ck.as<double>(), ctx->rad.as<double>(),
+                            ctx->rad_der.as<double>(),
                             ctx->rad_der2.as<double>(), ctx->cnk.as<double2>(), ctx->soap.as<double>(),
                             ctx->sdp.as<double>(), ctx->hess_vecs.as<double>(), ctx->comp.as<int4>(),
                             ctx->skip.as<uint8_t>(), ctx->compressed ? ctx->out_ptr.as<int>() : nullptr,
                             ctx->out_c.as<int>(), ctx->out_el.as<double>(), ctx->unc_ptr.as<int>(),
                             ctx->unc_e.as<int>(), ctx->P_i.as<int32_t>(), ctx->P_el.as<double>(),
-                            ctx->hess_scratch.as<double>(), ctx->hess_out.as<double>());
+                            ctx->hess_scratch.as<double>(), ctx->hess_heads.as<double>(), ctx->hess_out.as<double>());
   });
   if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max or n_species out of the Hessian's compiled range");
   HIPCHK(hipGetLastError());

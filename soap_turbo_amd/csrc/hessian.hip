@@ -17,6 +17,8 @@
 // contraction as first derivatives, so the per-pair derivative itself is never formed.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "../../include/soap_turbo_hip.h"
 #include "kernels.h"
 #include "st_internal.h"
@@ -78,105 +80,120 @@ __device__ void ilexp_dev(int lmax, double xx, double* il) {
   }
 }
 
-// One pair's geometry in LDS: pair_geometry, src/soap_turbo.f90:1585-1697.
-struct PairGeom {
-  int ok;
-  double invr, nh[3];
-  double Fl[MAXL + 1], Fl1[MAXL + 1], Fl2[MAXL + 1];
-  double Qt[MAXK + 2];
-  double2 wp[MAXL + 3];  // w^m at m + 2 (w^-2 = w^-1 = 0)
-  double tvec[3], thes[9];
+// One pair's geometry: pair_geometry, src/soap_turbo.f90:1585-1697. The head (up to Om) is
+// per pair and is written by hess_geom_k once per pair; Omega and its derivatives, per k, are
+// built in LDS by the wavefront that uses them.
+#define ST_PAIR_HEAD                                                  \
+  double ok, invr, nh[3];                                             \
+  double Fl[MAXL + 1], Fl1[MAXL + 1], Fl2[MAXL + 1];                  \
+  double Qt[MAXK + 3];                                                \
+  double tvec[3], thes[9];                                            \
+  double2 wp[MAXL + 3]; /* w^m at m + 2 (w^-2 = w^-1 = 0) */          \
   double2 wvec[3], whes[9];
+struct PairHead {
+  ST_PAIR_HEAD
+};
+struct PairGeom {
+  ST_PAIR_HEAD
   double2 Om[MAXK], dOm[3 * MAXK], d2Om[9 * MAXK];
 };
+#undef ST_PAIR_HEAD
+static_assert(offsetof(PairGeom, Om) == sizeof(PairHead), "the head leads PairGeom");
+static_assert(sizeof(PairHead) % 16 == 0, "heads stay 16-byte aligned");
+constexpr int GEOM_HEAD = (int)(sizeof(PairHead) / sizeof(double));  // doubles per pair
 
-// Fills g for pair k2; every thread of the workgroup calls it (barriers inside).
-__device__ void pair_geometry_dev(PairGeom& g, const HessParams& P, int k2, const double* __restrict__ rjs,
-                                  const double* __restrict__ thetas, const double* __restrict__ phis,
-                                  const uint32_t* __restrict__ spmask, const double* __restrict__ Ck, bool want2) {
-  const int tx = threadIdx.x, L = P.l_max, k_max = P.k_max;
-  __syncthreads();  // every thread is done with the previous pair's g (also after a skipped pair)
-  if (tx == 0) {
-    const double rr = rjs[k2];
-    const uint32_t msk = spmask[k2];
-    g.ok = !(rr >= P.rcut_max || rr < 1e-10) && msk != 0;
-    if (g.ok) {
-      const int isp = __ffs(msk) - 1;
-      const double invr = 1.0 / rr;
-      g.invr = invr;
-      const double sscal = P.sigma_t_scaling[isp];
-      const double sig = P.sigma_t[isp] + sscal * rr;
-      const double amp = P.rcut_max * P.rcut_max / (sig * sig);
-      const double ampd = -2.0 * sscal / sig * amp;
-      const double ampdd = 6.0 * (sscal * sscal) / (sig * sig) * amp;
-      const double xarg = rr / sig, zz = xarg * xarg, invz = 1.0 / zz;
-      double il[MAXL + 3];
-      ilexp_dev(L + 2, xarg, il);
-      const double zp = 2.0 * rr / (sig * sig) - 2.0 * (rr * rr) * sscal / (sig * sig * sig);
-      const double zp2 = 2.0 / (sig * sig) - 8.0 * rr * sscal / (sig * sig * sig) +
-                         6.0 * (rr * rr) * (sscal * sscal) / (sig * sig * sig * sig);
-      for (int l = 0; l <= L; ++l) {
-        const double rl = (double)l;
-        const double dilz = il[l + 1] + (rl * invz - 1.0) * il[l];
-        const double d2ilz = il[l + 2] + ((double)(2 * l + 1) * invz - 2.0) * il[l + 1] +
-                             (rl * (rl - 1.0) * invz * invz - 2.0 * rl * invz + 1.0) * il[l];
-        const double f0 = il[l], f1 = dilz * zp, f2 = d2ilz * zp * zp + dilz * zp2;
-        g.Fl[l] = amp * f0;
-        g.Fl1[l] = ampd * f0 + amp * f1;
-        g.Fl2[l] = ampdd * f0 + 2.0 * ampd * f1 + amp * f2;
-      }
-      const double sin_t = sin(thetas[k2]), cos_t = cos(thetas[k2]);
-      const double sin_f = sin(phis[k2]), cos_f = cos(phis[k2]);
-      const double nh0 = sin_t * cos_f, nh1 = sin_t * sin_f, nh2 = cos_t;
-      g.nh[0] = nh0; g.nh[1] = nh1; g.nh[2] = nh2;
-      const double tt = nh2;
-      const double2 wc = make_double2(nh0, -nh1);
-      const double2 ec[3] = {make_double2(1.0, 0.0), make_double2(0.0, -1.0), make_double2(0.0, 0.0)};
-      const double nh[3] = {nh0, nh1, nh2};
-      for (int a = 0; a < 3; ++a) {
-        g.tvec[a] = -tt * nh[a] * invr;
-        g.wvec[a] = cadd(cscale(-nh[a] * invr, wc), cscale(invr, ec[a]));
-      }
-      g.tvec[2] += invr;
-      if (want2) {
-        const double invr2 = invr * invr;
-        for (int b = 0; b < 3; ++b)
-          for (int a = 0; a < 3; ++a) {
-            g.thes[3 * a + b] = 3.0 * tt * nh[a] * nh[b] * invr2;
-            double2 w3 = cscale(3.0 * nh[a] * nh[b], wc);
-            w3.x -= ec[a].x * nh[b] + ec[b].x * nh[a];
-            w3.y -= ec[a].y * nh[b] + ec[b].y * nh[a];
-            g.whes[3 * a + b] = cscale(invr2, w3);
-          }
-        for (int a = 0; a < 3; ++a) {
-          g.thes[3 * 2 + a] -= nh[a] * invr2;
-          g.thes[3 * a + 2] -= nh[a] * invr2;
-          g.thes[3 * a + a] -= tt * invr2;
-          g.whes[3 * a + a] = make_double2(g.whes[3 * a + a].x - wc.x * invr2, g.whes[3 * a + a].y - wc.y * invr2);
-        }
-      }
-      g.wp[0] = g.wp[1] = make_double2(0.0, 0.0);
-      g.wp[2] = make_double2(1.0, 0.0);
-      for (int m = 1; m <= L; ++m) g.wp[m + 2] = cmul(g.wp[m + 1], wc);
-    }
-  }
-  // Q_l^m(t) = d^m P_l / dt^m: column m by thread m, recursion in l (the reference's recursion)
-  if (tx <= L) {
-    const int m = tx;
-    const double tt = cos(thetas[k2]);
-    // diagonal Q_m^m = (2m-1)!!, then Q_(m+1)^m, then the three-term recursion
-    double qmm = 1.0;
-    for (int l = 1; l <= m; ++l) qmm = (double)(2 * l - 1) * qmm;
-    g.Qt[m * (m + 1) / 2 + m] = qmm;
-    if (m + 1 <= L) g.Qt[(m + 1) * (m + 2) / 2 + m] = (double)(2 * m + 1) * tt * qmm;
-    for (int l = m + 2; l <= L; ++l) {
-      const int kq = l * (l + 1) / 2 + m;
-      g.Qt[kq] = ((double)(2 * l - 1) * tt * g.Qt[kq - l] - (double)(l + m - 1) * g.Qt[kq - 2 * l + 1]) / (double)(l - m);
-    }
-  }
-  __syncthreads();
+// pair_geometry in three steps, each run by one wavefront for its own pair; the workgroup
+// synchronises between them, so every wavefront calls each step (with k2 < 0: no pair this round).
+// Step 1, one lane: F_l and its two radial derivatives, the unit vector, and the Cartesian
+// derivatives of t = z/r and w = (x - iy)/r.
+template <class G>
+__device__ void geom_scalars(G& g, const HessParams& P, int k2, const double* __restrict__ rjs,
+                             const double* __restrict__ thetas, const double* __restrict__ phis,
+                             const uint32_t* __restrict__ spmask, bool want2) {
+  const int L = P.l_max;
+  if (k2 < 0) { g.ok = 0.0; return; }
+  const double rr = rjs[k2];
+  const uint32_t msk = spmask[k2];
+  g.ok = (!(rr >= P.rcut_max || rr < 1e-10) && msk != 0) ? 1.0 : 0.0;
   if (!g.ok) return;
-  for (int k = tx; k < k_max; k += HT) {
+  const int isp = __ffs(msk) - 1;
+  const double invr = 1.0 / rr;
+  g.invr = invr;
+  const double sscal = P.sigma_t_scaling[isp];
+  const double sig = P.sigma_t[isp] + sscal * rr;
+  const double amp = P.rcut_max * P.rcut_max / (sig * sig);
+  const double ampd = -2.0 * sscal / sig * amp;
+  const double ampdd = 6.0 * (sscal * sscal) / (sig * sig) * amp;
+  const double xarg = rr / sig, zz = xarg * xarg, invz = 1.0 / zz;
+  double il[MAXL + 3];
+  ilexp_dev(L + 2, xarg, il);
+  const double zp = 2.0 * rr / (sig * sig) - 2.0 * (rr * rr) * sscal / (sig * sig * sig);
+  const double zp2 = 2.0 / (sig * sig) - 8.0 * rr * sscal / (sig * sig * sig) +
+                     6.0 * (rr * rr) * (sscal * sscal) / (sig * sig * sig * sig);
+  for (int l = 0; l <= L; ++l) {
+    const double rl = (double)l;
+    const double dilz = il[l + 1] + (rl * invz - 1.0) * il[l];
+    const double d2ilz = il[l + 2] + ((double)(2 * l + 1) * invz - 2.0) * il[l + 1] +
+                         (rl * (rl - 1.0) * invz * invz - 2.0 * rl * invz + 1.0) * il[l];
+    const double f0 = il[l], f1 = dilz * zp, f2 = d2ilz * zp * zp + dilz * zp2;
+    g.Fl[l] = amp * f0;
+    g.Fl1[l] = ampd * f0 + amp * f1;
+    g.Fl2[l] = ampdd * f0 + 2.0 * ampd * f1 + amp * f2;
+  }
+  const double sin_t = sin(thetas[k2]), cos_t = cos(thetas[k2]);
+  const double sin_f = sin(phis[k2]), cos_f = cos(phis[k2]);
+  const double nh[3] = {sin_t * cos_f, sin_t * sin_f, cos_t};
+  g.nh[0] = nh[0]; g.nh[1] = nh[1]; g.nh[2] = nh[2];
+  const double tt = nh[2];
+  const double2 wc = make_double2(nh[0], -nh[1]);
+  const double2 ec[3] = {make_double2(1.0, 0.0), make_double2(0.0, -1.0), make_double2(0.0, 0.0)};
+  for (int a = 0; a < 3; ++a) {
+    g.tvec[a] = -tt * nh[a] * invr;
+    g.wvec[a] = cadd(cscale(-nh[a] * invr, wc), cscale(invr, ec[a]));
+  }
+  g.tvec[2] += invr;
+  if (want2) {
+    const double invr2 = invr * invr;
+    for (int b = 0; b < 3; ++b)
+      for (int a = 0; a < 3; ++a) {
+        g.thes[3 * a + b] = 3.0 * tt * nh[a] * nh[b] * invr2;
+        double2 w3 = cscale(3.0 * nh[a] * nh[b], wc);
+        w3.x -= ec[a].x * nh[b] + ec[b].x * nh[a];
+        w3.y -= ec[a].y * nh[b] + ec[b].y * nh[a];
+        g.whes[3 * a + b] = cscale(invr2, w3);
+      }
+    for (int a = 0; a < 3; ++a) {
+      g.thes[3 * 2 + a] -= nh[a] * invr2;
+      g.thes[3 * a + 2] -= nh[a] * invr2;
+      g.thes[3 * a + a] -= tt * invr2;
+      g.whes[3 * a + a] = make_double2(g.whes[3 * a + a].x - wc.x * invr2, g.whes[3 * a + a].y - wc.y * invr2);
+    }
+  }
+  g.wp[0] = g.wp[1] = make_double2(0.0, 0.0);
+  g.wp[2] = make_double2(1.0, 0.0);
+  for (int m = 1; m <= L; ++m) g.wp[m + 2] = cmul(g.wp[m + 1], wc);
+}
+
+// Step 1, lanes m <= l_max: Q_l^m(t) = d^m P_l / dt^m, column m, the reference's recursion in l
+template <class G>
+__device__ void geom_qt(G& g, const HessParams& P, int k2, int m, const double* __restrict__ thetas) {
+  const int L = P.l_max;
+  if (k2 < 0 || m > L) return;
+  const double tt = cos(thetas[k2]);
+  double qmm = 1.0;
+  for (int l = 1; l <= m; ++l) qmm = (double)(2 * l - 1) * qmm;
+  g.Qt[m * (m + 1) / 2 + m] = qmm;
+  if (m + 1 <= L) g.Qt[(m + 1) * (m + 2) / 2 + m] = (double)(2 * m + 1) * tt * qmm;
+  for (int l = m + 2; l <= L; ++l) {
+    const int kq = l * (l + 1) / 2 + m;
+    g.Qt[kq] = ((double)(2 * l - 1) * tt * g.Qt[kq - l] - (double)(l + m - 1) * g.Qt[kq - 2 * l + 1]) / (double)(l - m);
+  }
+}
+
+// Step 2, lanes over k: Omega_k = C_k Q_l^m(t) w^m, its gradient and (want2) its Hessian
+__device__ void geom_omega(PairGeom& g, const HessParams& P, int lane, const double* __restrict__ Ck, bool want2) {
+  if (g.ok == 0.0) return;
+  for (int k = lane; k < P.k_max; k += 64) {
     int l = 0;
     while ((l + 1) * (l + 2) / 2 <= k) ++l;
     const int mm = k - l * (l + 1) / 2;
@@ -198,6 +215,37 @@ __device__ void pair_geometry_dev(PairGeom& g, const HessParams& P, int k2, cons
           g.d2Om[9 * k + 3 * a + b] = cscale(C, t);
         }
   }
+}
+
+// The per-pair head of the geometry, one thread per pair (every pair that is not a site's own)
+__global__ __launch_bounds__(128) void hess_geom_k(HessParams P, const double* __restrict__ rjs,
+                                                  const double* __restrict__ thetas, const double* __restrict__ phis,
+                                                  const uint32_t* __restrict__ spmask, const uint8_t* __restrict__ first,
+                                                  double* __restrict__ heads) {
+  const int k2 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k2 >= P.n_pairs) return;
+  // written in place (each head is 16-byte aligned: sizeof(PairHead) is a multiple of 16)
+  PairHead& g = *reinterpret_cast<PairHead*>(heads + (size_t)k2 * GEOM_HEAD);
+  geom_scalars(g, P, first[k2] ? -1 : k2, rjs, thetas, phis, spmask, true);
+  if (g.ok != 0.0)
+    for (int m = 0; m <= P.l_max; ++m) geom_qt(g, P, k2, m, thetas);
+}
+
+// The geometry of pair k2 (< 0: none) for this wavefront, in g: the head copied from hess_geom_k's
+// output, then Omega by the lanes. Called by every thread of the workgroup, which synchronises
+// around each step.
+__device__ void wave_geometry(PairGeom& g, const HessParams& P, int k2, const double* __restrict__ heads,
+                              const double* __restrict__ Ck, bool want2) {
+  const int lane = threadIdx.x & 63;
+  __syncthreads();  // every wavefront is done with the previous round's geometry
+  double* dst = reinterpret_cast<double*>(&g);
+  if (k2 < 0) {
+    if (lane == 0) g.ok = 0.0;
+  } else {
+    for (int q = lane; q < GEOM_HEAD; q += 64) dst[q] = heads[(size_t)k2 * GEOM_HEAD + q];
+  }
+  __syncthreads();
+  geom_omega(g, P, lane, Ck, want2);
   __syncthreads();
 }
 
@@ -219,30 +267,38 @@ __device__ void block_sum(double (&v)[NV], double* red) {
   __syncthreads();
 }
 
-// pass A: G(a, n, k) = sum_j [g'(r) n_a Omega + g(r) dOmega_a] with g = 4 pi rad F_l
+// pass A: G(a, n, k) = sum_j [g'(r) n_a Omega + g(r) dOmega_a] with g = 4 pi rad F_l; the four
+// wavefronts build the geometry of four pairs per round, then every thread adds them to its
+// elements in pair order
 __global__ __launch_bounds__(HT) void hess_gacc_k(HessParams P, const int* __restrict__ site_ptr,
-                                                 const double* __restrict__ rjs, const double* __restrict__ thetas,
-                                                 const double* __restrict__ phis, const uint32_t* __restrict__ spmask,
+                                                 const double* __restrict__ heads,
                                                  const double* __restrict__ Ck, const double* __restrict__ rad,
                                                  const double* __restrict__ rad_der, double* __restrict__ scratch) {
-  __shared__ PairGeom g;
-  const int i = blockIdx.x, tx = threadIdx.x;
+  __shared__ PairGeom gs[HT / 64];
+  const int i = blockIdx.x, tx = threadIdx.x, wv = tx >> 6;
   const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max;
   const double fourpi = 4.0 * 3.141592653589793;
   double2* G = reinterpret_cast<double2*>(scratch + (size_t)i * P.stride + P.o_gacc);
   for (int e = tx; e < 3 * E; e += HT) G[e] = make_double2(0.0, 0.0);
   const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
-  for (int k2 = k3 + 1; k2 < k4; ++k2) {
-    pair_geometry_dev(g, P, k2, rjs, thetas, phis, spmask, Ck, false);
-    if (!g.ok) continue;
+  for (int r0 = k3 + 1; r0 < k4; r0 += HT / 64) {
+    const int k2 = r0 + wv < k4 ? r0 + wv : -1;
+    wave_geometry(gs[wv], P, k2, heads, Ck, false);
     for (int e = tx; e < E; e += HT) {
       const int n = e / k_max, k = e - n * k_max;
       int l = 0;
       while ((l + 1) * (l + 2) / 2 <= k) ++l;
-      const double r0 = fourpi * rad[(size_t)k2 * n_max + n], r1 = fourpi * rad_der[(size_t)k2 * n_max + n];
-      const double grad0 = r0 * g.Fl[l], grad1 = r1 * g.Fl[l] + r0 * g.Fl1[l];
-      for (int a = 0; a < 3; ++a)
-        G[a * E + e] = cadd(G[a * E + e], cadd(cscale(grad1 * g.nh[a], g.Om[k]), cscale(grad0, g.dOm[3 * k + a])));
+      double2 acc[3] = {G[e], G[E + e], G[2 * E + e]};
+      for (int w = 0; w < HT / 64 && r0 + w < k4; ++w) {
+        const PairGeom& g = gs[w];
+        if (g.ok == 0.0) continue;
+        const int kp = r0 + w;
+        const double r_0 = fourpi * rad[(size_t)kp * n_max + n], r_1 = fourpi * rad_der[(size_t)kp * n_max + n];
+        const double grad0 = r_0 * g.Fl[l], grad1 = r_1 * g.Fl[l] + r_0 * g.Fl1[l];
+        for (int a = 0; a < 3; ++a)
+          acc[a] = cadd(acc[a], cadd(cscale(grad1 * g.nh[a], g.Om[k]), cscale(grad0, g.dOm[3 * k + a])));
+      }
+      G[e] = acc[0]; G[E + e] = acc[1]; G[2 * E + e] = acc[2];
     }
   }
 }
@@ -367,18 +423,23 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
   }
 }
 
-// pass B: per pair, the contraction and the normalisation
+// pass B: per pair, the contraction and the normalisation. Each wavefront takes one pair of a
+// round of four: its lanes run over k (and k + 64) for every weight vector in turn, the sums over
+// n read Z and ZG four n at a time; the wavefront's lanes then apply the quotient rule to its
+// pair's blocks and keep the running sums for the self pair.
+constexpr int HMAXT = 4;  // block entries (v, a, b) per lane: n_vec <= 28
+
 __global__ __launch_bounds__(HT) void hess_pairs_k(HessParams P, const int* __restrict__ site_ptr,
-                                                  const double* __restrict__ rjs, const double* __restrict__ thetas,
-                                                  const double* __restrict__ phis, const uint32_t* __restrict__ spmask,
+                                                  const double* __restrict__ heads,
                                                   const double* __restrict__ Ck, const double* __restrict__ rad,
                                                   const double* __restrict__ rad_der, const double* __restrict__ rad_der2,
                                                   const double* __restrict__ sdp, const double* __restrict__ scratch,
                                                   double* __restrict__ hess) {
-  __shared__ PairGeom g;
-  extern __shared__ double res[];  // (n_vec + 4) x 12: u2(3) and hv(9) per weight vector
+  __shared__ PairGeom gs[HT / 64];
+  extern __shared__ double res[];  // per wavefront: (n_vec + 4) x 12, then the self-pair sums
   const int i = blockIdx.x, tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
   const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, n_vec = P.n_vec, nvz = n_vec + 4;
+  const int nt = 9 * n_vec;
   const double fourpi = 4.0 * 3.141592653589793;
   const double* base = scratch + (size_t)i * P.stride;
   const double2* Z = reinterpret_cast<const double2*>(base + P.o_z);
@@ -386,101 +447,133 @@ __global__ __launch_bounds__(HT) void hess_pairs_k(HessParams P, const int* __re
   const double* scal = base + P.o_scal;
   const double s = sdp[i], invs = 1.0 / s, invs3 = invs / (s * s), invs5 = invs3 / (s * s);
   const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
-  // thread t < 9 n_vec owns block entry (v, a, b) = (t / 9, (t % 9) / 3, t % 3) and its running sum
-  double hsum = 0.0;
-  for (int k2 = k3 + 1; k2 < k4; ++k2) {
-    pair_geometry_dev(g, P, k2, rjs, thetas, phis, spmask, Ck, true);
-    if (!g.ok) {
-      if (tx < 9 * n_vec) {
-        const int v = tx / 9, a = (tx % 9) / 3, b = tx % 3;
-        hess[(((size_t)k2 * n_vec + v) * 3 + b) * 3 + a] = 0.0;
-      }
-      continue;
-    }
-    const double* r = rad + (size_t)k2 * n_max;
-    const double* rd = rad_der + (size_t)k2 * n_max;
-    const double* rh = rad_der2 + (size_t)k2 * n_max;
-    const double invr = g.invr;
-    for (int v = wv; v < nvz; v += HT / 64) {
-      const bool second = v <= n_vec;
-      double acc[12];
+  double* rw = res + (size_t)wv * nvz * 12;
+  PairGeom& g = gs[wv];
+  double hsum[HMAXT] = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = k3 + 1; r0 < k4; r0 += HT / 64) {
+    const int k2 = r0 + wv < k4 ? r0 + wv : -1;
+    wave_geometry(g, P, k2, heads, Ck, true);
+    if (k2 >= 0 && g.ok != 0.0) {
+      const double* r = rad + (size_t)k2 * n_max;
+      const double* rd = rad_der + (size_t)k2 * n_max;
+      const double* rh = rad_der2 + (size_t)k2 * n_max;
+      const double invr = g.invr;
+      for (int v = 0; v < nvz; ++v) {
+        const bool second = v <= n_vec;
+        double acc[12];
 #pragma unroll
-      for (int q = 0; q < 12; ++q) acc[q] = 0.0;
-      for (int k = lane; k < k_max; k += 64) {
-        int l = 0;
-        while ((l + 1) * (l + 2) / 2 <= k) ++l;
-        const int m = k - l * (l + 1) / 2;
-        const double f0 = g.Fl[l], f1 = g.Fl1[l], f2 = g.Fl2[l], wmk = m > 0 ? 2.0 : 1.0;
-        double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0, b0[3] = {a0, a0, a0}, b1[3] = {a0, a0, a0};
-        for (int n = 0; n < n_max; ++n) {
-          double2 zc = Z[(size_t)v * E + n * k_max + k];
-          zc.y = -zc.y;
-          a0 = cadd(a0, cscale(r[n], zc));
-          a1 = cadd(a1, cscale(rd[n], zc));
+        for (int q = 0; q < 12; ++q) acc[q] = 0.0;
+        for (int k = lane; k < k_max; k += 64) {
+          int l = 0;
+          while ((l + 1) * (l + 2) / 2 <= k) ++l;
+          const int m = k - l * (l + 1) / 2;
+          const double f0 = g.Fl[l], f1 = g.Fl1[l], f2 = g.Fl2[l], wmk = m > 0 ? 2.0 : 1.0;
+          double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0, b0[3] = {a0, a0, a0}, b1[3] = {a0, a0, a0};
+          const double2* Zv = Z + (size_t)v * E + k;
+          const double2* ZGv = ZG + (size_t)v * 3 * E + k;
+          for (int n0 = 0; n0 < n_max; n0 += 4) {
+            double2 zc[4], gc[4][3];
+            double rr0[4], rr1[4], rr2[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int n = n0 + j < n_max ? n0 + j : n_max - 1;
+              zc[j] = Zv[n * k_max];
+              rr0[j] = n0 + j < n_max ? r[n] : 0.0;
+              rr1[j] = n0 + j < n_max ? rd[n] : 0.0;
+              rr2[j] = n0 + j < n_max ? rh[n] : 0.0;
+              if (second)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) gc[j][a] = ZGv[(size_t)a * E + n * k_max];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const double2 z = make_double2(zc[j].x, -zc[j].y);
+              a0 = cadd(a0, cscale(rr0[j], z));
+              a1 = cadd(a1, cscale(rr1[j], z));
+              if (second) {
+                a2 = cadd(a2, cscale(rr2[j], z));
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                  const double2 gz = make_double2(gc[j][a].x, -gc[j][a].y);
+                  b0[a] = cadd(b0[a], cscale(rr0[j], gz));
+                  b1[a] = cadd(b1[a], cscale(rr1[j], gz));
+                }
+              }
+            }
+          }
+          const double2 z0 = cscale(fourpi * f0, a0);
+          const double2 z1 = cscale(fourpi, cadd(cscale(f0, a1), cscale(f1, a0)));
+          const double2 om = g.Om[k];
+          for (int b = 0; b < 3; ++b) {
+            const double2 t = cadd(cmul(cscale(g.nh[b], z1), om), cmul(z0, g.dOm[3 * k + b]));
+            acc[b] += wmk * t.x;
+          }
           if (second) {
-            a2 = cadd(a2, cscale(rh[n], zc));
+            const double2 z2 = cscale(fourpi, cadd(cadd(cscale(f0, a2), cscale(2.0 * f1, a1)), cscale(f2, a0)));
+            double2 gg0[3], gg1[3];
             for (int a = 0; a < 3; ++a) {
-              double2 gc = ZG[((size_t)v * 3 + a) * E + n * k_max + k];
-              gc.y = -gc.y;
-              b0[a] = cadd(b0[a], cscale(r[n], gc));
-              b1[a] = cadd(b1[a], cscale(rd[n], gc));
+              gg0[a] = cscale(fourpi * f0, b0[a]);
+              gg1[a] = cscale(fourpi, cadd(cscale(f0, b1[a]), cscale(f1, b0[a])));
             }
+            const double2 zr = make_double2(z2.x - z1.x * invr, z2.y - z1.y * invr);
+            for (int b = 0; b < 3; ++b)
+              for (int a = 0; a < 3; ++a) {
+                double2 tE = cmul(cscale(g.nh[a] * g.nh[b], zr), om);
+                tE = cadd(tE, cmul(z1, cadd(cscale(g.nh[a], g.dOm[3 * k + b]), cscale(g.nh[b], g.dOm[3 * k + a]))));
+                tE = cadd(tE, cmul(z0, g.d2Om[9 * k + 3 * a + b]));
+                if (a == b) tE = cadd(tE, cmul(cscale(invr, z1), om));
+                const double2 tD = cadd(cmul(cscale(g.nh[b], gg1[a]), om), cmul(gg0[a], g.dOm[3 * k + b]));
+                acc[3 + 3 * a + b] -= wmk * (tE.x + tD.x);
+              }
           }
         }
-        const double2 z0 = cscale(fourpi * f0, a0);
-        const double2 z1 = cscale(fourpi, cadd(cscale(f0, a1), cscale(f1, a0)));
-        const double2 om = g.Om[k];
-        for (int b = 0; b < 3; ++b) {
-          const double2 t = cadd(cmul(cscale(g.nh[b], z1), om), cmul(z0, g.dOm[3 * k + b]));
-          acc[b] += wmk * t.x;
-        }
-        if (second) {
-          const double2 z2 = cscale(fourpi, cadd(cadd(cscale(f0, a2), cscale(2.0 * f1, a1)), cscale(f2, a0)));
-          double2 gg0[3], gg1[3];
-          for (int a = 0; a < 3; ++a) {
-            gg0[a] = cscale(fourpi * f0, b0[a]);
-            gg1[a] = cscale(fourpi, cadd(cscale(f0, b1[a]), cscale(f1, b0[a])));
-          }
-          const double2 zr = make_double2(z2.x - z1.x * invr, z2.y - z1.y * invr);
-          for (int b = 0; b < 3; ++b)
-            for (int a = 0; a < 3; ++a) {
-              double2 tE = cmul(cscale(g.nh[a] * g.nh[b], zr), om);
-              tE = cadd(tE, cmul(z1, cadd(cscale(g.nh[a], g.dOm[3 * k + b]), cscale(g.nh[b], g.dOm[3 * k + a]))));
-              tE = cadd(tE, cmul(z0, g.d2Om[9 * k + 3 * a + b]));
-              if (a == b) tE = cadd(tE, cmul(cscale(invr, z1), om));
-              const double2 tD = cadd(cmul(cscale(g.nh[b], gg1[a]), om), cmul(gg0[a], g.dOm[3 * k + b]));
-              acc[3 + 3 * a + b] -= wmk * (tE.x + tD.x);
-            }
-        }
+#pragma unroll
+        for (int q = 0; q < 12; ++q)
+          for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+        if (lane == 0)
+#pragma unroll
+          for (int q = 0; q < 12; ++q) rw[v * 12 + q] = acc[q];
       }
-#pragma unroll
-      for (int q = 0; q < 12; ++q)
-        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
-      if (lane == 0)
-#pragma unroll
-        for (int q = 0; q < 12; ++q) res[v * 12 + q] = acc[q];
     }
-    __syncthreads();
+    __syncthreads();  // this round's sums are in LDS
     // normalisation, q = p / s (src/soap_turbo.f90:1453-1485): with d1 = d/dr_ia, d2 = d/dr_jb,
     //   w.d1d2 q = (w.d1d2 p)/s - [(w.d1 p)(p.d2 p) + (w.d2 p)(p.d1 p)]/s^3
     //              - (w.p)[(d1 p.d2 p) + (p.d1d2 p)]/s^3 + 3 (w.p)(p.d1 p)(p.d2 p)/s^5
-    if (tx < 9 * n_vec) {
-      const int v = tx / 9, a = (tx % 9) / 3, b = tx % 3;
-      const double* rp = res + n_vec * 12;  // p's row: u2 = p.dp2, hv = p.d1d2 p
-      const double t1a = scal[a], t2b = rp[b];
-      const double wp = scal[3 + 4 * v], u1a = scal[3 + 4 * v + 1 + a];
-      const double u2b = res[v * 12 + b];
-      const double c12 = res[(n_vec + 1 + a) * 12 + b];  // D1_a . dp2_b
-      const double h = res[v * 12 + 3 + 3 * a + b] * invs - (u1a * t2b + u2b * t1a) * invs3 -
-                       wp * (c12 + rp[3 + 3 * a + b]) * invs3 + 3.0 * wp * t1a * t2b * invs5;
-      hess[(((size_t)k2 * n_vec + v) * 3 + b) * 3 + a] = h;
-      hsum += h;
+    if (k2 >= 0) {
+#pragma unroll
+      for (int u = 0; u < HMAXT; ++u) {
+        const int t = lane + 64 * u;
+        if (t >= nt) break;
+        const int v = t / 9, a = (t % 9) / 3, b = t % 3;
+        double h = 0.0;
+        if (g.ok != 0.0) {
+          const double* rp = rw + n_vec * 12;  // p's row: u2 = p.dp2, hv = p.d1d2 p
+          const double t1a = scal[a], t2b = rp[b];
+          const double wp = scal[3 + 4 * v], u1a = scal[3 + 4 * v + 1 + a];
+          const double u2b = rw[v * 12 + b];
+          const double c12 = rw[(n_vec + 1 + a) * 12 + b];  // D1_a . dp2_b
+          h = rw[v * 12 + 3 + 3 * a + b] * invs - (u1a * t2b + u2b * t1a) * invs3 -
+              wp * (c12 + rp[3 + 3 * a + b]) * invs3 + 3.0 * wp * t1a * t2b * invs5;
+        }
+        hess[(((size_t)k2 * n_vec + v) * 3 + b) * 3 + a] = h;
+        hsum[u] += h;
+      }
     }
-    __syncthreads();  // res and g are rewritten by the next pair
   }
-  if (tx < 9 * n_vec) {
-    const int v = tx / 9, a = (tx % 9) / 3, b = tx % 3;
-    hess[(((size_t)k3 * n_vec + v) * 3 + b) * 3 + a] = -hsum;
+  // the self pair: minus the sum of the others, the wavefronts' partial sums added in order
+  double* hs = res + (size_t)(HT / 64) * nvz * 12;
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < HMAXT; ++u) {
+    const int t = lane + 64 * u;
+    if (t < nt) hs[wv * nt + t] = hsum[u];
+  }
+  __syncthreads();
+  for (int t = tx; t < nt; t += HT) {
+    double sum = 0.0;
+    for (int w = 0; w < HT / 64; ++w) sum += hs[w * nt + t];
+    const int v = t / 9, a = (t % 9) / 3, b = t % 3;
+    hess[(((size_t)k3 * n_vec + v) * 3 + b) * 3 + a] = -sum;
   }
 }
 
@@ -493,22 +586,24 @@ size_t hessian_scratch_per_site(int n_max, int k_max, int n_soap, int n_unc, int
   return (n + 31) / 32 * 32;  // each site's slot starts on a 256-byte boundary
 }
 
+size_t hessian_heads_per_pair() { return GEOM_HEAD; }
+
 size_t hessian_lds(int n_unc, int n_vec) {
-  const size_t geom = sizeof(PairGeom);
+  const size_t geom = (HT / 64) * sizeof(PairGeom);
   const size_t w = (size_t)n_unc * 8 + 32 * 8;
-  const size_t pr = geom + (size_t)(n_vec + 4) * 12 * 8;
+  const size_t pr = geom + ((size_t)(HT / 64) * (n_vec + 4) * 12 + (size_t)(HT / 64) * 9 * n_vec) * 8;
   return w > pr ? w : pr;
 }
 
 bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int n_soap, int n_unc, int n_vec,
                     int compressed, int n_species, const double* sigma_t, const double* sigma_t_scaling,
                     double rcut_max, const int* site_ptr, const double* rjs, const double* thetas, const double* phis,
-                    const uint32_t* spmask, const double* Ck, const double* rad, const double* rad_der,
-                    const double* rad_der2, const double2* cnk, const double* soap, const double* sdp,
-                    const double* vecs, const int4* comp, const uint8_t* skip, const int* out_ptr, const int* out_c,
-                    const double* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
-                    const double* P_el, double* scratch, double* hess) {
-  if (l_max > MAXL || n_species > ST_MAX_SPECIES || n_vec < 1) return false;
+                    const uint32_t* spmask, const uint8_t* first, const double* Ck, const double* rad,
+                    const double* rad_der, const double* rad_der2, const double2* cnk, const double* soap,
+                    const double* sdp, const double* vecs, const int4* comp, const uint8_t* skip, const int* out_ptr,
+                    const int* out_c, const double* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
+                    const double* P_el, double* scratch, double* heads, double* hess) {
+  if (l_max > MAXL || n_species > ST_MAX_SPECIES || n_vec < 1 || 9 * n_vec > 64 * HMAXT) return false;
   HessParams P;
   P.n_sites = n_sites; P.n_pairs = n_pairs; P.n_max = n_max; P.k_max = 1 + l_max * (l_max + 1) / 2 + l_max;
   P.l_max = l_max; P.n_soap = n_soap; P.n_unc = n_unc; P.n_vec = n_vec; P.n_species = n_species;
@@ -527,15 +622,16 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
   P.o_scal = P.o_pvec + n_soap;
   P.stride = hessian_scratch_per_site(n_max, P.k_max, n_soap, n_unc, n_vec);
   if (n_sites == 0) return true;
-  hess_gacc_k<<<n_sites, HT, 0, st>>>(P, site_ptr, rjs, thetas, phis, spmask, Ck, rad, rad_der, scratch);
+  if (n_pairs > 0)
+    hess_geom_k<<<(n_pairs + 127) / 128, 128, 0, st>>>(P, rjs, thetas, phis, spmask, first, heads);
+  hess_gacc_k<<<n_sites, HT, 0, st>>>(P, site_ptr, heads, Ck, rad, rad_der, scratch);
   const size_t wl = (size_t)n_unc * 8;
   if (wl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_weights_k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)wl);
   hess_weights_k<<<n_sites, HT, wl, st>>>(P, cnk, soap, sdp, vecs, comp, skip, out_ptr, out_c, out_el, unc_ptr,
                                           unc_e, P_i, P_el, scratch);
-  const size_t pl = (size_t)(n_vec + 4) * 12 * 8;
-  hess_pairs_k<<<n_sites, HT, pl, st>>>(P, site_ptr, rjs, thetas, phis, spmask, Ck, rad, rad_der, rad_der2, sdp,
-                                        scratch, hess);
+  const size_t pl = ((size_t)(HT / 64) * (n_vec + 4) * 12 + (size_t)(HT / 64) * 9 * n_vec) * 8;
+  hess_pairs_k<<<n_sites, HT, pl, st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2, sdp, scratch, hess);
   return true;
 }
 
