@@ -95,7 +95,7 @@ struct st_ctx {
   double gap_delta = 0.0;
   DevBuf gap_T, gap_alpha, gap_esite, gap_energy;
   // central-atom Hessian (st_central_hessian): second radial derivatives, C_k, weight vectors, scratch
-  DevBuf rad_der2, hess_ck, hess_vecs, hess_scratch, hess_heads, hess_out;
+  DevBuf rad_der2, hess_ck, hess_vecs, hess_scratch, hess_heads, hess_res, hess_out;
   std::string g32_key, g32_warm;
   hipGraph_t g32_graph = nullptr;
   hipGraphExec_t g32_exec = nullptr;
@@ -1344,7 +1344,8 @@ int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c,
   if ((rc = ensure(ctx, ctx->hess_scratch, (size_t)std::max(NS, 1) * stride * 8))) return rc;
   if ((rc = ensure(ctx, ctx->hess_out, P * 9 * n_vec * 8))) return rc;
   if ((rc = ensure(ctx, ctx->hess_heads, P * st::hessian_heads_per_pair() * 8))) return rc;
-  if (st::hessian_lds(n_unc, n_vec) > ctx->lds_max)
+  if ((rc = ensure(ctx, ctx->hess_res, P * st::hessian_res_per_pair(n_vec) * 8))) return rc;
+  if (st::hessian_lds(n_max, k_max, n_unc) > ctx->lds_max)
     return fail(ctx, ST_ERR_UNSUPPORTED, "the Hessian's working set exceeds the LDS");
   std::vector<double> sig(p->n_species), sigs(p->n_species);
   double rcut_max = 0.0;
@@ -1365,7 +1366,8 @@ int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c,
                             ctx->skip.as<uint8_t>(), ctx->compressed ? ctx->out_ptr.as<int>() : nullptr,
                             ctx->out_c.as<int>(), ctx->out_el.as<double>(), ctx->unc_ptr.as<int>(),
                             ctx->unc_e.as<int>(), ctx->P_i.as<int32_t>(), ctx->P_el.as<double>(),
-                            ctx->hess_scratch.as<double>(), ctx->hess_heads.as<double>(), ctx->hess_out.as<double>());
+                            ctx->hess_scratch.as<double>(), ctx->hess_heads.as<double>(), ctx->hess_res.as<double>(),
+                            ctx->hess_out.as<double>());
   });
   if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max or n_species out of the Hessian's compiled range");
   HIPCHK(hipGetLastError());

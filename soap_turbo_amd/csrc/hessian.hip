@@ -4,19 +4,24 @@
 // and the self pair's block is minus the sum of the others. q = p / |p| is the normalised
 // (compressed) descriptor.
 //
-// Three kernels, one workgroup per site, on the resident FP64 state of the descriptor pass (cnk,
-// the radial expansion with first and second derivatives, the normalised descriptor and its norm):
-//   hess_gacc_k     G(n, k, a) = sum_j dc^j_(n,k) / du_ja over the site's pairs (pass A);
-//   hess_weights_k  the central-atom derivative D1 = dp/dr_i from G and c (an accumulated form of
-//                   minus the sum of the per-pair derivatives), then for each weight vector (the
-//                   caller's, p itself, and the three columns of D1) its uncompressed S matrix and
-//                   Z = S c, and, for the caller's vectors and p, ZG = S G;
-//   hess_pairs_k    per pair, Omega = C_k Q_l^m(t) w^m with its Cartesian gradient and Hessian, the
-//                   contraction against Z and ZG, and the quotient rule of the normalisation (pass B).
+// Kernels, on the resident FP64 state of the descriptor pass (cnk, the radial expansion with first
+// and second derivatives, the normalised descriptor and its norm):
+//   hess_geom_k      one thread per pair: the pair's geometry head (F_l and its radial derivatives,
+//                    the unit vector, the derivatives of t = z/r and w = (x - iy)/r, w^m, Q_l^m(t));
+//   hess_gacc_k      one workgroup per site: G(n, k, a) = sum_j dc^j_(n,k) / du_ja (pass A);
+//   hess_weights_k   one workgroup per site: the central-atom derivative D1 = dp/dr_i from G and c
+//                    (an accumulated form of minus the sum of the per-pair derivatives), then for
+//                    each weight vector (the caller's, p itself, the three columns of D1) its
+//                    uncompressed S matrix, Z = S c and, for the caller's vectors and p, ZG = S G;
+//   hess_contract_k  one workgroup per (site, weight vector), Z and ZG in LDS: per pair, Omega =
+//                    C_k Q_l^m(t) w^m with its Cartesian gradient and Hessian, contracted (pass B);
+//   hess_norm_k      one workgroup per site: the quotient rule of the normalisation, and the self
+//                    pair's blocks.
 // Dot products of the per-pair derivative with a vector (p.dp2, w.dp2, D1.dp2) come out of the same
 // contraction as first derivatives, so the per-pair derivative itself is never formed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 
 #include "../../include/soap_turbo_hip.h"
@@ -44,6 +49,13 @@ struct HessParams {
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+// lane l's value of v, l uniform over the wavefront (v_readlane)
+__device__ inline double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // get_ilexp, src/soap_turbo_angular.f90:361-422, to lmax (<= MAXL + 2)
 __device__ void ilexp_dev(int lmax, double xx, double* il) {
@@ -423,156 +435,166 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
   }
 }
 
-// pass B: per pair, the contraction and the normalisation. Each wavefront takes one pair of a
-// round of four: its lanes run over k (and k + 64) for every weight vector in turn, the sums over
-// n read Z and ZG four n at a time; the wavefront's lanes then apply the quotient rule to its
-// pair's blocks and keep the running sums for the self pair.
-constexpr int HMAXT = 4;  // block entries (v, a, b) per lane: n_vec <= 28
+// Omega_k, its gradient and (second) its Hessian for one k, from a pair's head (in registers)
+__device__ inline void omega_k(const PairHead& h, int k, int l, int mm, double C, bool second, double2& om,
+                               double2 (&dom)[3], double2 (&d2om)[9]) {
+  const double q0 = h.Qt[k], q1 = mm + 1 <= l ? h.Qt[k + 1] : 0.0, q2 = mm + 2 <= l ? h.Qt[k + 2] : 0.0;
+  const double dm = (double)mm;
+  const double2 w0 = h.wp[mm + 2], w1 = h.wp[mm + 1], w2 = h.wp[mm];
+  om = cscale(C * q0, w0);
+  for (int a = 0; a < 3; ++a) dom[a] = cscale(C, cadd(cscale(q1 * h.tvec[a], w0), cscale(q0 * dm, cmul(w1, h.wvec[a]))));
+  if (second)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        double2 t = cscale(q2 * h.tvec[a] * h.tvec[b], w0);
+        t = cadd(t, cscale(q1 * h.thes[3 * a + b], w0));
+        double2 tw = cadd(cscale(h.tvec[a], h.wvec[b]), cscale(h.tvec[b], h.wvec[a]));
+        t = cadd(t, cscale(q1 * dm, cmul(w1, tw)));
+        t = cadd(t, cscale(q0 * (double)(mm * (mm - 1)), cmul(w2, cmul(h.wvec[a], h.wvec[b]))));
+        t = cadd(t, cscale(q0 * dm, cmul(w1, h.whes[3 * a + b])));
+        d2om[3 * a + b] = cscale(C, t);
+      }
+}
 
-__global__ __launch_bounds__(HT) void hess_pairs_k(HessParams P, const int* __restrict__ site_ptr,
-                                                  const double* __restrict__ heads,
-                                                  const double* __restrict__ Ck, const double* __restrict__ rad,
-                                                  const double* __restrict__ rad_der, const double* __restrict__ rad_der2,
-                                                  const double* __restrict__ sdp, const double* __restrict__ scratch,
-                                                  double* __restrict__ hess) {
-  __shared__ PairGeom gs[HT / 64];
-  extern __shared__ double res[];  // per wavefront: (n_vec + 4) x 12, then the self-pair sums
-  const int i = blockIdx.x, tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
+// pass B, contraction: one workgroup per (site, weight vector v), with Z(v) (and, for the caller's
+// vectors and p, ZG(v)) staged in LDS once; its wavefronts walk the site's pairs, lanes over k.
+// Per pair and v it writes 12 sums: v.dp/du_jb (3) and, with second derivatives, v.d2p/dr_ia
+// du_jb before the normalisation (9).
+__global__ __launch_bounds__(HT) void hess_contract_k(HessParams P, const int* __restrict__ site_ptr,
+                                                     const double* __restrict__ heads, const double* __restrict__ Ck,
+                                                     const double* __restrict__ rad, const double* __restrict__ rad_der,
+                                                     const double* __restrict__ rad_der2,
+                                                     const double* __restrict__ scratch, double* __restrict__ res) {
+  extern __shared__ double2 zs[];  // Z(v) [n][k], then ZG(v) [a][n][k]
+  const int i = blockIdx.x, v = blockIdx.y, tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
   const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, n_vec = P.n_vec, nvz = n_vec + 4;
-  const int nt = 9 * n_vec;
+  const bool second = v <= n_vec;
   const double fourpi = 4.0 * 3.141592653589793;
   const double* base = scratch + (size_t)i * P.stride;
-  const double2* Z = reinterpret_cast<const double2*>(base + P.o_z);
-  const double2* ZG = reinterpret_cast<const double2*>(base + P.o_zg);
-  const double* scal = base + P.o_scal;
+  const double2* Z = reinterpret_cast<const double2*>(base + P.o_z) + (size_t)v * E;
+  const double2* ZG = reinterpret_cast<const double2*>(base + P.o_zg) + (size_t)v * 3 * E;
+  for (int e = tx; e < E; e += HT) zs[e] = Z[e];
+  if (second)
+    for (int e = tx; e < 3 * E; e += HT) zs[E + e] = ZG[e];
+  __syncthreads();
+  const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
+  for (int k2 = k3 + 1 + wv; k2 < k4; k2 += HT / 64) {
+    const PairHead& h = *reinterpret_cast<const PairHead*>(heads + (size_t)k2 * GEOM_HEAD);
+    if (h.ok == 0.0) continue;  // uniform over the wavefront
+    const double* r = rad + (size_t)k2 * n_max;
+    const double* rd = rad_der + (size_t)k2 * n_max;
+    const double* rh = rad_der2 + (size_t)k2 * n_max;
+    const double invr = h.invr;
+    const double nh[3] = {h.nh[0], h.nh[1], h.nh[2]};
+    double acc[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) acc[q] = 0.0;
+    for (int k = lane; k < k_max; k += 64) {
+      int l = 0;
+      while ((l + 1) * (l + 2) / 2 <= k) ++l;
+      const int m = k - l * (l + 1) / 2;
+      const double f0 = h.Fl[l], f1 = h.Fl1[l], f2 = h.Fl2[l], wmk = m > 0 ? 2.0 : 1.0;
+      double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0, b0[3] = {a0, a0, a0}, b1[3] = {a0, a0, a0};
+      // the pair's radial values: lane n holds n (64 at a time), read out with a uniform lane index
+      for (int c0 = 0; c0 < n_max; c0 += 64) {
+        const int nl = c0 + lane;
+        const double rv0 = nl < n_max ? r[nl] : 0.0, rv1 = nl < n_max ? rd[nl] : 0.0;
+        const double rv2 = second && nl < n_max ? rh[nl] : 0.0;
+        const int c1 = n_max - c0 < 64 ? n_max - c0 : 64;
+        for (int j = 0; j < c1; ++j) {
+          const int n = c0 + j;
+          const double r0 = readlane_d(rv0, j), r1 = readlane_d(rv1, j);
+          const double2 zc = zs[n * k_max + k];
+          const double2 z = make_double2(zc.x, -zc.y);
+          a0 = cadd(a0, cscale(r0, z));
+          a1 = cadd(a1, cscale(r1, z));
+          if (second) {
+            a2 = cadd(a2, cscale(readlane_d(rv2, j), z));
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const double2 gc = zs[E + a * E + n * k_max + k];
+              const double2 gz = make_double2(gc.x, -gc.y);
+              b0[a] = cadd(b0[a], cscale(r0, gz));
+              b1[a] = cadd(b1[a], cscale(r1, gz));
+            }
+          }
+        }
+      }
+      double2 om, dom[3], d2om[9];
+      omega_k(h, k, l, m, Ck[k], second, om, dom, d2om);
+      const double2 z0 = cscale(fourpi * f0, a0);
+      const double2 z1 = cscale(fourpi, cadd(cscale(f0, a1), cscale(f1, a0)));
+      for (int b = 0; b < 3; ++b) {
+        const double2 t = cadd(cmul(cscale(nh[b], z1), om), cmul(z0, dom[b]));
+        acc[b] += wmk * t.x;
+      }
+      if (second) {
+        const double2 z2 = cscale(fourpi, cadd(cadd(cscale(f0, a2), cscale(2.0 * f1, a1)), cscale(f2, a0)));
+        double2 gg0[3], gg1[3];
+        for (int a = 0; a < 3; ++a) {
+          gg0[a] = cscale(fourpi * f0, b0[a]);
+          gg1[a] = cscale(fourpi, cadd(cscale(f0, b1[a]), cscale(f1, b0[a])));
+        }
+        const double2 zr = make_double2(z2.x - z1.x * invr, z2.y - z1.y * invr);
+        for (int b = 0; b < 3; ++b)
+          for (int a = 0; a < 3; ++a) {
+            double2 tE = cmul(cscale(nh[a] * nh[b], zr), om);
+            tE = cadd(tE, cmul(z1, cadd(cscale(nh[a], dom[b]), cscale(nh[b], dom[a]))));
+            tE = cadd(tE, cmul(z0, d2om[3 * a + b]));
+            if (a == b) tE = cadd(tE, cmul(cscale(invr, z1), om));
+            const double2 tD = cadd(cmul(cscale(nh[b], gg1[a]), om), cmul(gg0[a], dom[b]));
+            acc[3 + 3 * a + b] -= wmk * (tE.x + tD.x);
+          }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+      for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+    if (lane < 12) {
+      double x = acc[0];
+#pragma unroll
+      for (int q = 1; q < 12; ++q)
+        if (lane == q) x = acc[q];
+      res[((size_t)k2 * nvz + v) * 12 + lane] = x;
+    }
+  }
+}
+
+// pass B, normalisation, q = p / s (src/soap_turbo.f90:1453-1485): with d1 = d/dr_ia, d2 = d/dr_jb,
+//   w.d1d2 q = (w.d1d2 p)/s - [(w.d1 p)(p.d2 p) + (w.d2 p)(p.d1 p)]/s^3
+//              - (w.p)[(d1 p.d2 p) + (p.d1d2 p)]/s^3 + 3 (w.p)(p.d1 p)(p.d2 p)/s^5
+// then the self pair's block, minus the sum of the others in pair order. One workgroup per site.
+__global__ __launch_bounds__(HT) void hess_norm_k(HessParams P, const int* __restrict__ site_ptr,
+                                                 const double* __restrict__ heads, const double* __restrict__ sdp,
+                                                 const double* __restrict__ scratch, const double* __restrict__ res,
+                                                 double* __restrict__ hess) {
+  const int i = blockIdx.x, tx = threadIdx.x;
+  const int n_vec = P.n_vec, nvz = n_vec + 4, nt = 9 * n_vec;
+  const double* scal = scratch + (size_t)i * P.stride + P.o_scal;
   const double s = sdp[i], invs = 1.0 / s, invs3 = invs / (s * s), invs5 = invs3 / (s * s);
   const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
-  double* rw = res + (size_t)wv * nvz * 12;
-  PairGeom& g = gs[wv];
-  double hsum[HMAXT] = {0.0, 0.0, 0.0, 0.0};
-  for (int r0 = k3 + 1; r0 < k4; r0 += HT / 64) {
-    const int k2 = r0 + wv < k4 ? r0 + wv : -1;
-    wave_geometry(g, P, k2, heads, Ck, true);
-    if (k2 >= 0 && g.ok != 0.0) {
-      const double* r = rad + (size_t)k2 * n_max;
-      const double* rd = rad_der + (size_t)k2 * n_max;
-      const double* rh = rad_der2 + (size_t)k2 * n_max;
-      const double invr = g.invr;
-      for (int v = 0; v < nvz; ++v) {
-        const bool second = v <= n_vec;
-        double acc[12];
-#pragma unroll
-        for (int q = 0; q < 12; ++q) acc[q] = 0.0;
-        for (int k = lane; k < k_max; k += 64) {
-          int l = 0;
-          while ((l + 1) * (l + 2) / 2 <= k) ++l;
-          const int m = k - l * (l + 1) / 2;
-          const double f0 = g.Fl[l], f1 = g.Fl1[l], f2 = g.Fl2[l], wmk = m > 0 ? 2.0 : 1.0;
-          double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0, b0[3] = {a0, a0, a0}, b1[3] = {a0, a0, a0};
-          const double2* Zv = Z + (size_t)v * E + k;
-          const double2* ZGv = ZG + (size_t)v * 3 * E + k;
-          for (int n0 = 0; n0 < n_max; n0 += 4) {
-            double2 zc[4], gc[4][3];
-            double rr0[4], rr1[4], rr2[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const int n = n0 + j < n_max ? n0 + j : n_max - 1;
-              zc[j] = Zv[n * k_max];
-              rr0[j] = n0 + j < n_max ? r[n] : 0.0;
-              rr1[j] = n0 + j < n_max ? rd[n] : 0.0;
-              rr2[j] = n0 + j < n_max ? rh[n] : 0.0;
-              if (second)
-#pragma unroll
-                for (int a = 0; a < 3; ++a) gc[j][a] = ZGv[(size_t)a * E + n * k_max];
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const double2 z = make_double2(zc[j].x, -zc[j].y);
-              a0 = cadd(a0, cscale(rr0[j], z));
-              a1 = cadd(a1, cscale(rr1[j], z));
-              if (second) {
-                a2 = cadd(a2, cscale(rr2[j], z));
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                  const double2 gz = make_double2(gc[j][a].x, -gc[j][a].y);
-                  b0[a] = cadd(b0[a], cscale(rr0[j], gz));
-                  b1[a] = cadd(b1[a], cscale(rr1[j], gz));
-                }
-              }
-            }
-          }
-          const double2 z0 = cscale(fourpi * f0, a0);
-          const double2 z1 = cscale(fourpi, cadd(cscale(f0, a1), cscale(f1, a0)));
-          const double2 om = g.Om[k];
-          for (int b = 0; b < 3; ++b) {
-            const double2 t = cadd(cmul(cscale(g.nh[b], z1), om), cmul(z0, g.dOm[3 * k + b]));
-            acc[b] += wmk * t.x;
-          }
-          if (second) {
-            const double2 z2 = cscale(fourpi, cadd(cadd(cscale(f0, a2), cscale(2.0 * f1, a1)), cscale(f2, a0)));
-            double2 gg0[3], gg1[3];
-            for (int a = 0; a < 3; ++a) {
-              gg0[a] = cscale(fourpi * f0, b0[a]);
-              gg1[a] = cscale(fourpi, cadd(cscale(f0, b1[a]), cscale(f1, b0[a])));
-            }
-            const double2 zr = make_double2(z2.x - z1.x * invr, z2.y - z1.y * invr);
-            for (int b = 0; b < 3; ++b)
-              for (int a = 0; a < 3; ++a) {
-                double2 tE = cmul(cscale(g.nh[a] * g.nh[b], zr), om);
-                tE = cadd(tE, cmul(z1, cadd(cscale(g.nh[a], g.dOm[3 * k + b]), cscale(g.nh[b], g.dOm[3 * k + a]))));
-                tE = cadd(tE, cmul(z0, g.d2Om[9 * k + 3 * a + b]));
-                if (a == b) tE = cadd(tE, cmul(cscale(invr, z1), om));
-                const double2 tD = cadd(cmul(cscale(g.nh[b], gg1[a]), om), cmul(gg0[a], g.dOm[3 * k + b]));
-                acc[3 + 3 * a + b] -= wmk * (tE.x + tD.x);
-              }
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 12; ++q)
-          for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
-        if (lane == 0)
-#pragma unroll
-          for (int q = 0; q < 12; ++q) rw[v * 12 + q] = acc[q];
-      }
-    }
-    __syncthreads();  // this round's sums are in LDS
-    // normalisation, q = p / s (src/soap_turbo.f90:1453-1485): with d1 = d/dr_ia, d2 = d/dr_jb,
-    //   w.d1d2 q = (w.d1d2 p)/s - [(w.d1 p)(p.d2 p) + (w.d2 p)(p.d1 p)]/s^3
-    //              - (w.p)[(d1 p.d2 p) + (p.d1d2 p)]/s^3 + 3 (w.p)(p.d1 p)(p.d2 p)/s^5
-    if (k2 >= 0) {
-#pragma unroll
-      for (int u = 0; u < HMAXT; ++u) {
-        const int t = lane + 64 * u;
-        if (t >= nt) break;
-        const int v = t / 9, a = (t % 9) / 3, b = t % 3;
-        double h = 0.0;
-        if (g.ok != 0.0) {
-          const double* rp = rw + n_vec * 12;  // p's row: u2 = p.dp2, hv = p.d1d2 p
-          const double t1a = scal[a], t2b = rp[b];
-          const double wp = scal[3 + 4 * v], u1a = scal[3 + 4 * v + 1 + a];
-          const double u2b = rw[v * 12 + b];
-          const double c12 = rw[(n_vec + 1 + a) * 12 + b];  // D1_a . dp2_b
-          h = rw[v * 12 + 3 + 3 * a + b] * invs - (u1a * t2b + u2b * t1a) * invs3 -
-              wp * (c12 + rp[3 + 3 * a + b]) * invs3 + 3.0 * wp * t1a * t2b * invs5;
-        }
-        hess[(((size_t)k2 * n_vec + v) * 3 + b) * 3 + a] = h;
-        hsum[u] += h;
-      }
-    }
-  }
-  // the self pair: minus the sum of the others, the wavefronts' partial sums added in order
-  double* hs = res + (size_t)(HT / 64) * nvz * 12;
-  __syncthreads();
-#pragma unroll
-  for (int u = 0; u < HMAXT; ++u) {
-    const int t = lane + 64 * u;
-    if (t < nt) hs[wv * nt + t] = hsum[u];
-  }
-  __syncthreads();
-  for (int t = tx; t < nt; t += HT) {
-    double sum = 0.0;
-    for (int w = 0; w < HT / 64; ++w) sum += hs[w * nt + t];
+  const int np = k4 - k3 - 1;
+  for (int idx = tx; idx < np * nt; idx += HT) {
+    const int k2 = k3 + 1 + idx / nt, t = idx % nt;
     const int v = t / 9, a = (t % 9) / 3, b = t % 3;
+    double h = 0.0;
+    if (heads[(size_t)k2 * GEOM_HEAD] != 0.0) {
+      const double* rw = res + (size_t)k2 * nvz * 12;
+      const double* rp = rw + n_vec * 12;  // p's row: u2 = p.dp2, hv = p.d1d2 p
+      const double t1a = scal[a], t2b = rp[b];
+      const double wp = scal[3 + 4 * v], u1a = scal[3 + 4 * v + 1 + a];
+      const double u2b = rw[v * 12 + b];
+      const double c12 = rw[(n_vec + 1 + a) * 12 + b];  // D1_a . dp2_b
+      h = rw[v * 12 + 3 + 3 * a + b] * invs - (u1a * t2b + u2b * t1a) * invs3 -
+          wp * (c12 + rp[3 + 3 * a + b]) * invs3 + 3.0 * wp * t1a * t2b * invs5;
+    }
+    hess[(((size_t)k2 * n_vec + v) * 3 + b) * 3 + a] = h;
+  }
+  __syncthreads();  // the blocks of this site are written (global memory, same workgroup)
+  for (int t = tx; t < nt; t += HT) {
+    const int v = t / 9, a = (t % 9) / 3, b = t % 3;
+    double sum = 0.0;
+    for (int k2 = k3 + 1; k2 < k4; ++k2) sum += hess[(((size_t)k2 * n_vec + v) * 3 + b) * 3 + a];
     hess[(((size_t)k3 * n_vec + v) * 3 + b) * 3 + a] = -sum;
   }
 }
@@ -588,12 +610,14 @@ size_t hessian_scratch_per_site(int n_max, int k_max, int n_soap, int n_unc, int
 
 size_t hessian_heads_per_pair() { return GEOM_HEAD; }
 
-size_t hessian_lds(int n_unc, int n_vec) {
-  const size_t geom = (HT / 64) * sizeof(PairGeom);
-  const size_t w = (size_t)n_unc * 8 + 32 * 8;
-  const size_t pr = geom + ((size_t)(HT / 64) * (n_vec + 4) * 12 + (size_t)(HT / 64) * 9 * n_vec) * 8;
-  return w > pr ? w : pr;
+size_t hessian_lds(int n_max, int k_max, int n_unc) {
+  const size_t geom = (HT / 64) * sizeof(PairGeom);        // pass A
+  const size_t w = (size_t)n_unc * 8 + 32 * 8;              // weight vectors
+  const size_t c = (size_t)n_max * k_max * 4 * sizeof(double2);  // contraction: Z and ZG of one vector
+  return std::max(geom, std::max(w, c));
 }
+
+size_t hessian_res_per_pair(int n_vec) { return (size_t)(n_vec + 4) * 12; }
 
 bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int n_soap, int n_unc, int n_vec,
                     int compressed, int n_species, const double* sigma_t, const double* sigma_t_scaling,
@@ -602,8 +626,8 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
                     const double* rad_der, const double* rad_der2, const double2* cnk, const double* soap,
                     const double* sdp, const double* vecs, const int4* comp, const uint8_t* skip, const int* out_ptr,
                     const int* out_c, const double* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
-                    const double* P_el, double* scratch, double* heads, double* hess) {
-  if (l_max > MAXL || n_species > ST_MAX_SPECIES || n_vec < 1 || 9 * n_vec > 64 * HMAXT) return false;
+                    const double* P_el, double* scratch, double* heads, double* res, double* hess) {
+  if (l_max > MAXL || n_species > ST_MAX_SPECIES || n_vec < 1) return false;
   HessParams P;
   P.n_sites = n_sites; P.n_pairs = n_pairs; P.n_max = n_max; P.k_max = 1 + l_max * (l_max + 1) / 2 + l_max;
   P.l_max = l_max; P.n_soap = n_soap; P.n_unc = n_unc; P.n_vec = n_vec; P.n_species = n_species;
@@ -630,8 +654,12 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)wl);
   hess_weights_k<<<n_sites, HT, wl, st>>>(P, cnk, soap, sdp, vecs, comp, skip, out_ptr, out_c, out_el, unc_ptr,
                                           unc_e, P_i, P_el, scratch);
-  const size_t pl = ((size_t)(HT / 64) * (n_vec + 4) * 12 + (size_t)(HT / 64) * 9 * n_vec) * 8;
-  hess_pairs_k<<<n_sites, HT, pl, st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2, sdp, scratch, hess);
+  const size_t cl = E * 4 * sizeof(double2);
+  if (cl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_contract_k),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl);
+  hess_contract_k<<<dim3(n_sites, n_vec + 4), HT, cl, st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2, scratch,
+                                                           res);
+  hess_norm_k<<<n_sites, HT, 0, st>>>(P, site_ptr, heads, sdp, scratch, res, hess);
   return true;
 }
 

@@ -62,7 +62,8 @@ void launch_atom_pairs(hipStream_t st, int n_atoms, int n_pairs, const int* pair
 
 // central-atom Hessian (hessian.hip): scratch doubles per site, LDS bytes, launch
 size_t hessian_scratch_per_site(int n_max, int k_max, int n_soap, int n_unc, int n_vec);
-size_t hessian_lds(int n_unc, int n_vec);
+size_t hessian_lds(int n_max, int k_max, int n_unc);
+size_t hessian_res_per_pair(int n_vec);  // doubles of per-pair contraction sums
 size_t hessian_heads_per_pair();  // doubles of per-pair geometry scratch
 bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int n_soap, int n_unc, int n_vec,
                     int compressed, int n_species, const double* sigma_t, const double* sigma_t_scaling,
@@ -71,6 +72,6 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
                     const double* rad_der, const double* rad_der2, const double2* cnk, const double* soap,
                     const double* sdp, const double* vecs, const int4* comp, const uint8_t* skip, const int* out_ptr,
                     const int* out_c, const double* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
-                    const double* P_el, double* scratch, double* heads, double* hess);
+                    const double* P_el, double* scratch, double* heads, double* res, double* hess);
 
 }  // namespace st
