@@ -51,7 +51,7 @@ enum {
   ST_ERR_STOP = 5          /* a condition the reference stops on (alpha_max limits, ...) */
 };
 
-enum { ST_BASIS_POLY3 = 0, ST_BASIS_POLY3GAUSS = 1 };
+enum { ST_BASIS_POLY3 = 0, ST_BASIS_POLY3GAUSS = 1, ST_BASIS_POLY3OPERATOR = 2 /* FP64 path only */ };
 enum { ST_SCALING_POLYNOMIAL = 0, ST_SCALING_OTHER = 1 };
 
 /* Descriptor hyper-parameters (get_soap arguments; one entry per species for the arrays). */

@@ -82,7 +82,7 @@ def _structs(w, legacy_seed=True, compress=None):
     p = w.params
     ps = p.per_species()
     keep = _Keep()
-    prm = _Params(p.n_species, p.l_max, p.radial_enhancement, 1 if p.basis == "poly3gauss" else 0,
+    prm = _Params(p.n_species, p.l_max, p.radial_enhancement, {"poly3": 0, "poly3gauss": 1, "poly3operator": 2}[p.basis],
                   1 if p.scaling_mode == "polynomial" else 0, 1 if legacy_seed else 0,
                   keep(p.alpha_max, np.int32), *[keep(ps[k], np.float64) for k in (
                       "rcut_hard", "rcut_soft", "nf", "global_scaling", "atom_sigma_r", "atom_sigma_r_scaling",

@@ -81,6 +81,7 @@ program ref_harness
 
   basis = "poly3"
   if( ibasis == 1 ) basis = "poly3gauss"
+  if( ibasis == 2 ) basis = "poly3operator"
   scaling_mode = "polynomial"
   if( iscal /= 0 ) scaling_mode = "none"
   legacy_filter_seed = ( ilegacy /= 0 )
@@ -135,6 +136,8 @@ program ref_harness
     Wt = 0.d0; St = 0.d0
     if( ibasis == 1 )then
       call get_orthonormalization_matrix_poly3gauss(alpha_max(i), atom_sigma_r(i), rcut_hard(i), St, Wt)
+    else if( ibasis == 2 )then
+      call get_orthonormalization_matrix_poly3_tabulated(alpha_max(i), St, Wt)
     else
       call get_orthonormalization_matrix_poly3(alpha_max(i), St, Wt)
     end if
@@ -183,6 +186,11 @@ program ref_harness
                rcut_hard(i), atom_sigma_r(i), atom_sigma_r_scaling(i), amplitude_scaling(i), nf(i), &
                W(i_beg(i):i_end(i), i_beg(i):i_end(i)), scaling_mode, mask(:,i), radial_enhancement, &
                .true., rad(i_beg(i):i_end(i), :), rad_der(i_beg(i):i_end(i), :))
+      else if( ibasis == 2 )then
+        call get_radial_expansion_coefficients_poly3operator(n_sites, n_neigh, rjs, alpha_max(i), rcut_soft(i), &
+               rcut_hard(i), atom_sigma_r(i), atom_sigma_r_scaling(i), amplitude_scaling(i), &
+               W(i_beg(i):i_end(i), i_beg(i):i_end(i)), scaling_mode, mask(:,i), radial_enhancement, &
+               .true., .true., central_weight(i), rad(i_beg(i):i_end(i), :), rad_der(i_beg(i):i_end(i), :))
       else
         call get_radial_expansion_coefficients_poly3(n_sites, n_neigh, rjs, alpha_max(i), rcut_soft(i), &
                rcut_hard(i), atom_sigma_r(i), atom_sigma_r_scaling(i), amplitude_scaling(i), nf(i), &

@@ -49,7 +49,7 @@ def write_input(path, w, G=None, modes=MODE_FULL, legacy_seed=True, n_reps=1, ve
     hdr[4] = p.l_max
     hdr[5] = 1
     hdr[6] = p.radial_enhancement
-    hdr[7] = 1 if p.basis == "poly3gauss" else 0
+    hdr[7] = {"poly3": 0, "poly3gauss": 1, "poly3operator": 2}[p.basis]
     hdr[8] = 0 if p.scaling_mode == "polynomial" else 1
     hdr[9] = compress_code(p.compress_mode)
     hdr[10] = 1 if legacy_seed else 0

@@ -343,6 +343,189 @@ static void radial_pair(const radial_consts* c, double rjs, int is_central, doub
   }
 }
 
+/* ---- poly3operator: get_radial_expansion_coefficients_poly3operator,
+ * src/soap_turbo_radial.f90:1549-2141, for one pair of one species (scaled units, rcut_hard = 1).
+ * The atom is a piecewise cubic of half-width w = 2 sqrt(2 ln 2) sigma(r_j) about r_j
+ * (g_aux, :2154-2171); inside rcut_soft it is integrated against the polynomial basis directly
+ * (the "soft" region, :1598-1842), between rcut_soft and rcut_hard multiplied by the cubic filter
+ * (the "buffer" region, :1845-2087). Every integral is a polynomial moment between two limits:
+ * the A matrix (get_constant_poly_coeff, :2276-2297) holds the basis side, and the atom (and
+ * filter) side enters through its derivatives at the limits. */
+static void op_g(double r, double r0, double winv, int right, double* g) {
+  /* g_aux_array, :2179-2217: the cubic and its scaled derivatives */
+  double x = (r - r0) * winv, x2 = x * x, x3 = x2 * x, w2 = winv * winv, w3 = w2 * winv;
+  if (!right) {
+    g[0] = 1.0 - 3.0 * x2 - 2.0 * x3; g[1] = -6.0 * (x2 + x) * winv; g[2] = -3.0 * (2.0 * x + 1.0) * w2; g[3] = -2.0 * w3;
+  } else {
+    g[0] = 1.0 - 3.0 * x2 + 2.0 * x3; g[1] = 6.0 * (x2 - x) * winv; g[2] = 3.0 * (2.0 * x - 1.0) * w2; g[3] = 2.0 * w3;
+  }
+}
+
+static void op_gd(double r, double r0, double winv, double ws, int right, double* g) {
+  /* g_aux_der_array, :2227-2267: derivatives with respect to r0, width = width0 + ws r0 */
+  double x = (r - r0) * winv, x2 = x * x, x3 = x2 * x, w2 = winv * winv, w3 = w2 * winv, w4 = w3 * winv;
+  if (!right) {
+    g[0] = 6.0 * (x + (ws + 1.0) * x2 + ws * x3) * winv;
+    g[1] = 6.0 * (1.0 + 2.0 * (ws + 1.0) * x + 3.0 * ws * x2) * w2;
+    g[2] = 6.0 * (ws + 1.0 + 3.0 * ws * x) * w3;
+    g[3] = 6.0 * ws * w4;
+  } else {
+    g[0] = 6.0 * (x + (ws - 1.0) * x2 - ws * x3) * winv;
+    g[1] = 6.0 * (1.0 + 2.0 * (ws - 1.0) * x - 3.0 * ws * x2) * w2;
+    g[2] = 6.0 * (ws - 1.0 - 3.0 * ws * x) * w3;
+    g[3] = -6.0 * ws * w4;
+  }
+}
+
+static void op_powers(double r, int n, double* t) { /* M_radial_poly_array, :2399-2418: (1 - r)^(i-1) */
+  t[0] = 1.0;
+  for (int i = 1; i < n; ++i) t[i] = t[i - 1] * (1.0 - r);
+}
+
+/* -B . M_radial_monomial(r, 6), :2481-2500: minus the derivatives of the degree-6 polynomial with
+ * coefficients B at r, v(q) = -sum_p B(p) (p-1)!/(p-q)! r^(p-q) */
+static void op_bm(const double* B, double r, double* v) {
+  double rt[7];
+  rt[0] = 1.0;
+  for (int p = 1; p < 7; ++p) rt[p] = r * rt[p - 1];
+  for (int q = 0; q < 7; ++q) {
+    double acc = 0.0;
+    for (int pp = q; pp < 7; ++pp) {
+      double f = 1.0;
+      for (int t = pp - q + 1; t <= pp; ++t) f *= (double)t;
+      acc += -B[pp] * (rt[pp - q] * f);
+    }
+    v[q] = acc;
+  }
+}
+
+/* get_constant_poly_filter_coeff_array (and _der), :2307-2390: the degree-6 product of the atom's
+ * cubic about 0 at -r_j and the filter's cubic at 0 */
+static void op_filter_B(double rj, double winv, double ws, double rs, double fw, int right, int der, double* B) {
+  double cf[4], col[4];
+  { /* g_aux(0, r_filter, filter_width, "right"), :2154-2171 */
+    double x = (0.0 - rs) / fw;
+    cf[0] = 1.0 - 3.0 * (x * x) + 2.0 * (x * x * x);
+    cf[1] = 6.0 * (x * x - x) / fw;
+    cf[2] = 3.0 * (2.0 * x - 1.0) / (fw * fw);
+    cf[3] = 2.0 / (fw * fw * fw);
+  }
+  if (der) op_gd(-rj, 0.0, winv, ws, right, col);
+  else op_g(-rj, 0.0, winv, right, col);
+  for (int r = 0; r < 7; ++r) {
+    double acc = 0.0;
+    for (int i = 0; i < 4; ++i) {
+      int t = r - i;
+      acc += (t >= 0 && t < 4 ? col[t] : 0.0) * cf[i];
+    }
+    B[r] = acc;
+  }
+}
+
+void so_op_pair(const so_params* p, int s, double rjs_in, int is_central, double* outv, double* outd) {
+  const int a = p->alpha_max[s];
+  const double pi = acos(-1.0), K = 2.0 * sqrt(2.0 * log(2.0));
+  const double rc_in = p->rcut_hard[s], rs_in = p->rcut_soft[s], sig_in = p->atom_sigma_r[s];
+  const double sc = p->atom_sigma_r_scaling[s], aa = p->amplitude_scaling[s], cw = p->central_weight[s];
+  const double rs = rs_in / rc_in, sig0 = sig_in / rc_in, fw = K * (1.0 - rs);
+  double A[MAXA][7];
+  for (int i = 1; i <= a; ++i) { /* get_constant_poly_coeff(alpha_max, 1, A) */
+    A[i - 1][0] = 1.0 / sqrt(1.0 / (double)(2 * i + 5)) / (double)(i + 3);
+    for (int j = 2; j <= 7; ++j) A[i - 1][j - 1] = A[i - 1][j - 2] * 1.0 / (double)(i + 3 + j - 1);
+  }
+  double ex[MAXA], exd[MAXA];
+  for (int n = 0; n < a; ++n) ex[n] = exd[n] = 0.0;
+  const double win = K * (sig_in + sc * rjs_in);  /* unscaled width, for the region tests */
+  const int soft = rjs_in - win < rs_in;
+  const int buffer = rs_in < rc_in && rjs_in + win > rs_in;
+  const double rj = rjs_in / rc_in, sg = sig0 + sc * rj, s2 = sg * sg, w = K * sg, winv = 1.0 / w, ws = K * sc;
+  double amp, ampd;
+  /* amplitude, polynomial scaling only (:1701-1727) */
+  double h = 1.0 + 2.0 * (rj * rj * rj) - 3.0 * (rj * rj);
+  if (aa == 0.0) {
+    amp = 1.0 / sg; ampd = -sc / s2;
+  } else if (aa == 1.0) {
+    amp = 1.0 / sg * h;
+    ampd = 6.0 / sg * (rj * rj - rj) - sc / sg * amp;
+  } else {
+    amp = 1.0 / sg * pow(h, aa);
+    ampd = 6.0 * aa / sg * (rj * rj - rj) * pow(h, aa - 1.0) - sc / sg * amp;
+  }
+  if (is_central) { amp = cw * amp; ampd = cw * ampd; }
+  if (p->radial_enhancement == 1) {
+    ampd = amp * (1.0 + sqrt(2.0 / pi) * sc) + ampd * (rj + sqrt(2.0 / pi) * sg);
+    amp = amp * (rj + sqrt(2.0 / pi) * sg);
+  } else if (p->radial_enhancement == 2) {
+    ampd = amp * (2.0 * rj + 2.0 * sg * sc + sqrt(8.0 / pi) * sg + sqrt(8.0 / pi) * rj * sc) +
+           ampd * (rj * rj + s2 + sqrt(8.0 / pi) * sg * rj);
+    amp = amp * (rj * rj + s2 + sqrt(8.0 / pi) * sg * rj);
+  }
+  static const double vect[4] = {-1.0, -1.0, -2.0, -6.0};
+  const int NT = (a + 4 > 7 ? a + 4 : 7);
+  if (soft) {
+    double sv[MAXA], sd[MAXA];
+    double L[3] = {rj - w > 0.0 ? rj - w : 0.0, rj < rs ? rj : rs, rs < rj + w ? rs : rj + w};
+    double I0[3][MAXA + 8];
+    for (int t = 0; t < 3; ++t) op_powers(L[t], a + 4, I0[t]);
+    for (int d = 0; d < 2; ++d) {
+      double M[3][2][4]; /* [limit][piece 0 left 1 right][j] */
+      for (int t = 0; t < 3; ++t)
+        for (int pc = 0; pc < 2; ++pc) {
+          double g[4];
+          if (d == 0) op_g(L[t], rj, winv, pc, g);
+          else op_gd(L[t], rj, winv, ws, pc, g);
+          for (int j = 0; j < 4; ++j) M[t][pc][j] = I0[t][j] * g[j] * vect[j];
+        }
+      for (int n = 0; n < a; ++n) {
+        double s12 = 0, s11 = 0, s23 = 0, s22 = 0;
+        for (int j = 0; j < 4; ++j) {
+          s12 += M[1][0][j] * A[n][j]; s11 += M[0][0][j] * A[n][j];
+          s23 += M[2][1][j] * A[n][j]; s22 += M[1][1][j] * A[n][j];
+        }
+        double il = s12 * I0[1][n + 4] - s11 * I0[0][n + 4];
+        double ir = s23 * I0[2][n + 4] - s22 * I0[1][n + 4];
+        if (d == 0) sv[n] = il + ir;
+        else sd[n] = il + ir;
+      }
+    }
+    for (int n = 0; n < a; ++n) {
+      ex[n] = amp * sv[n];
+      exd[n] = amp * sd[n] + ampd * sv[n];
+    }
+  }
+  if (buffer) {
+    double L[3] = {rs > rj - w ? rs : rj - w, rj > rs ? rj : rs, 1.0 < rj + w ? 1.0 : rj + w};
+    double I0[3][MAXA + 8];
+    for (int t = 0; t < 3; ++t) op_powers(L[t], NT, I0[t]);
+    double bd[MAXA], bb[MAXA];
+    for (int d = 0; d < 2; ++d) {
+      double B[2][7], M[3][2][7];
+      for (int pc = 0; pc < 2; ++pc) op_filter_B(rj, winv, ws, rs, fw, pc, d, B[pc]);
+      for (int t = 0; t < 3; ++t)
+        for (int pc = 0; pc < 2; ++pc) {
+          double v[7];
+          op_bm(B[pc], L[t], v);
+          for (int q = 0; q < 7; ++q) M[t][pc][q] = v[q] * I0[t][q];
+        }
+      for (int n = 0; n < a; ++n) {
+        double s12 = 0, s11 = 0, s23 = 0, s22 = 0;
+        for (int q = 0; q < 7; ++q) {
+          s12 += M[1][0][q] * A[n][q]; s11 += M[0][0][q] * A[n][q];
+          s23 += M[2][1][q] * A[n][q]; s22 += M[1][1][q] * A[n][q];
+        }
+        double v = (s12 * I0[1][n + 4] - s11 * I0[0][n + 4]) + (s23 * I0[2][n + 4] - s22 * I0[1][n + 4]);
+        if (d == 0) bb[n] = v;
+        else bd[n] = v;
+      }
+    }
+    for (int n = 0; n < a; ++n) {
+      ex[n] += amp * bb[n];
+      exd[n] += amp * bd[n] + ampd * bb[n];
+    }
+  }
+  for (int n = 0; n < a; ++n) { outv[n] = ex[n]; outd[n] = exd[n]; }
+}
+
 /* get_radial_expansion_coefficients_poly3[gauss] for every species, then
  * global_scaling (src/soap_turbo.f90:491-544; src/soap_turbo_radial.f90:116-384). */
 void so_radial(const so_params* p, const so_inputs* in, const double* W, int do_der, double* rad, double* rad_der) {
@@ -364,6 +547,33 @@ void so_radial2(const so_params* p, const so_inputs* in, const double* W, int do
       for (int j = 0; j < in->n_neigh[i]; ++j) first[k++] = (j == 0);
   }
   int off = 0;
+  if (p->basis == 2) {
+    /* poly3operator: every masked pair of the species (no cutoff test, the self pair weighted by
+     * central_weight), then W, the sqrt(rcut_hard) of the change of variable and global_scaling
+     * (src/soap_turbo_radial.f90:2089-2107, src/soap_turbo.f90:534-541) */
+    for (int s = 0; s < p->n_species; ++s) {
+      const int a = p->alpha_max[s];
+      const double gs = p->global_scaling[s], sq = sqrt(p->rcut_hard[s]);
+#pragma omp parallel for schedule(dynamic, 256)
+      for (int k = 0; k < in->n_pairs; ++k) {
+        if (!in->mask[k + (size_t)s * in->n_pairs]) continue;
+        double v[MAXA], d[MAXA];
+        so_op_pair(p, s, in->rjs[k], first[k], v, d);
+        for (int n = 0; n < a; ++n) {
+          double av = 0.0, ad = 0.0;
+          for (int m = 0; m < a; ++m) {
+            av += W[(off + n) + (size_t)(off + m) * n_max] * v[m];
+            ad += W[(off + n) + (size_t)(off + m) * n_max] * d[m];
+          }
+          rad[(size_t)k * n_max + off + n] = av * sq * gs;
+          if (do_der) rad_der[(size_t)k * n_max + off + n] = ad / sq * gs;
+        }
+      }
+      off += a;
+    }
+    free(first);
+    return;
+  }
   for (int s = 0; s < p->n_species; ++s) {
     radial_consts c;
     radial_setup(&c, p, s, W, n_max, off, rad_der2 ? 2 : do_der);

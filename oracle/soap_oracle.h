@@ -21,7 +21,7 @@ typedef struct {
   int n_species;
   int l_max;
   int radial_enhancement;
-  int basis;          /* 0 poly3, 1 poly3gauss */
+  int basis;          /* 0 poly3, 1 poly3gauss, 2 poly3operator (polynomial scaling only) */
   int scaling_poly;   /* 1: scaling_mode == "polynomial" */
   int legacy_seed;    /* legacy_filter_seed */
   const int32_t* alpha_max;
@@ -50,6 +50,8 @@ int so_k_max(int l_max);
 int so_n_soap(const so_params* p, const so_compress* c);
 void so_preflm(int l_max, double* preflm);
 void so_overlap_S(int basis, int alpha_max, double atom_sigma_r, double rcut_hard, double* S);
+/* poly3operator, one pair of species s, before W (unscaled rjs): values and r-derivatives */
+void so_op_pair(const so_params* p, int s, double rjs_in, int is_central, double* outv, double* outd);
 void so_radial(const so_params* p, const so_inputs* in, const double* W, int do_der, double* rad, double* rad_der);
 void so_radial2(const so_params* p, const so_inputs* in, const double* W, int do_der, double* rad, double* rad_der,
                 double* rad_der2);

@@ -54,9 +54,9 @@ def make_params(keep: Keep, *, alpha_max, l_max, rcut_hard, rcut_soft, nf, globa
                 atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, amplitude_scaling, radial_enhancement,
                 central_weight, basis, scaling_mode, legacy_seed=None):
     basis = basis.strip()
-    codes = {"poly3": 0, "poly3gauss": 1}
+    codes = {"poly3": 0, "poly3gauss": 1, "poly3operator": 2}
     if basis not in codes:
-        raise SoapTurboError(f'basis "{basis}" is not supported (poly3, poly3gauss)')
+        raise SoapTurboError(f'basis "{basis}" is not supported (poly3, poly3gauss, poly3operator)')
     ns = len(alpha_max)
     seed = legacy_filter_seed if legacy_seed is None else legacy_seed
     return StParams(ns, int(l_max), int(radial_enhancement), codes[basis],

@@ -207,13 +207,19 @@ int check_params(st_ctx* ctx, const st_params* p) {
     return fail(ctx, ST_ERR_ARG, "n_species must be in 1..32");
   if (p->l_max < 0) return fail(ctx, ST_ERR_ARG, "l_max must be >= 0");
   if (p->l_max > ST_MAX_LMAX) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max > 12 is not supported by this build");
-  if (p->basis != ST_BASIS_POLY3 && p->basis != ST_BASIS_POLY3GAUSS)
-    return fail(ctx, ST_ERR_UNSUPPORTED, "only the poly3 and poly3gauss bases are supported");
+  if (p->basis != ST_BASIS_POLY3 && p->basis != ST_BASIS_POLY3GAUSS && p->basis != ST_BASIS_POLY3OPERATOR)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "only the poly3, poly3gauss and poly3operator bases are supported");
+  // the reference's poly3operator amplitude is defined for polynomial scaling only
+  // (src/soap_turbo_radial.f90:1701-1727 leaves it unset otherwise)
+  if (p->basis == ST_BASIS_POLY3OPERATOR && p->scaling_mode != ST_SCALING_POLYNOMIAL)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "the poly3operator basis needs scaling_mode polynomial");
   static bool warned_p3 = false, warned_p3g = false;
   for (int s = 0; s < p->n_species; ++s) {
     const int a = p->alpha_max[s];
     if (a < 1) return fail(ctx, ST_ERR_ARG, "alpha_max must be >= 1");
-    if (p->basis == ST_BASIS_POLY3) {
+    if (p->basis == ST_BASIS_POLY3OPERATOR) {
+      if (a > ST_MAX_ALPHA) return fail(ctx, ST_ERR_UNSUPPORTED, "alpha_max > 11 is not supported by this build");
+    } else if (p->basis == ST_BASIS_POLY3) {
       if (a > 10)
         return fail(ctx, ST_ERR_STOP,
                     "ERROR: Due to numerical instabilities in the basis construction for the poly3 basis, it is "
@@ -320,6 +326,23 @@ st::RadialConsts radial_consts(const st_params* p, int s, const std::vector<doub
   }
   c.n_poly = c.with_gauss ? a - 1 : a;
   c.n_hi = c.n_poly + (do_der2 ? 4 : do_der ? 2 : 0);
+  if (p->basis == ST_BASIS_POLY3OPERATOR) {
+    // get_constant_poly_coeff(alpha_max, 1, A) (src/soap_turbo_radial.f90:2276-2297) and the
+    // filter's cubic g_aux(0, rcut_soft, filter_width, "right") (:2154-2171, :2320)
+    c.op = 1;
+    c.op_K = 2.0 * std::sqrt(2.0 * std::log(2.0));
+    c.op_sigma_in = p->atom_sigma_r[s];
+    c.op_rcut_soft_in = p->rcut_soft[s];
+    for (int i = 1; i <= a; ++i) {
+      c.op_A[(i - 1) * 7] = 1.0 / std::sqrt(1.0 / (double)(2 * i + 5)) / (double)(i + 3);
+      for (int j = 2; j <= 7; ++j) c.op_A[(i - 1) * 7 + j - 1] = c.op_A[(i - 1) * 7 + j - 2] * 1.0 / (double)(i + 3 + j - 1);
+    }
+    const double fw = c.op_K * (1.0 - c.rcut_soft), x = (0.0 - c.rcut_soft) / fw;
+    c.op_cf[0] = 1.0 - 3.0 * (x * x) + 2.0 * (x * x * x);
+    c.op_cf[1] = 6.0 * (x * x - x) / fw;
+    c.op_cf[2] = 3.0 * (2.0 * x - 1.0) / (fw * fw);
+    c.op_cf[3] = 2.0 / (fw * fw * fw);
+  }
   double Nv[ST_REC + 8];  // Nv(n) at n + 3
   Nv[0] = 1.0;
   for (int n = -2; n <= a + 4; ++n) Nv[n + 3] = std::sqrt(1.0 / (double)(2 * n + 5));
@@ -1010,6 +1033,8 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
   int rc = check_params(ctx, p);
   if (rc) return rc;
+  if (p->basis == ST_BASIS_POLY3OPERATOR)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "the FP32 adjoint path supports poly3 and poly3gauss (poly3operator: FP64)");
   HIPCHK(hipSetDevice(ctx->device));
   if ((rc = basis_for(ctx, p))) return rc;
   if ((rc = build_tables(ctx, p, c))) return rc;
@@ -1307,6 +1332,10 @@ int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c,
   if (n_vec < 1 || !vecs || !hess) return fail(ctx, ST_ERR_ARG, "n_vec >= 1, vecs and hess are required");
   int rc = check_params(ctx, p);
   if (rc) return rc;
+  if (p->basis == ST_BASIS_POLY3OPERATOR)
+    return fail(ctx, ST_ERR_STOP,
+                "ERROR: soap_turbo: the central-atom Hessian needs second radial derivatives, which the "
+                "poly3operator basis does not provide. Use poly3 or poly3gauss.");
   if (p->legacy_filter_seed)
     return fail(ctx, ST_ERR_STOP,
                 "ERROR: second radial derivatives need soap_turbo_radial's legacy_filter_seed = .false. .");

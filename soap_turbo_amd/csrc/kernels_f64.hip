@@ -277,6 +277,222 @@ __global__ __launch_bounds__(128) void radial_k(RadialConsts c, int n_pairs, con
   }
 }
 
+// ---------------------------------------------------------------- radial, poly3operator
+// One thread per pair: get_radial_expansion_coefficients_poly3operator,
+// src/soap_turbo_radial.f90:1549-2141, for one species block. The atom is a piecewise cubic of
+// half-width w = 2 sqrt(2 ln 2) sigma(r_j) about r_j (g_aux, :2154-2217); inside rcut_soft it is
+// integrated against the polynomial basis directly, between rcut_soft and rcut_hard times the
+// cubic filter. Each integral is a polynomial moment between two limits: A (per species, on the
+// host, get_constant_poly_coeff :2276-2297) holds the basis side, the atom and filter enter through
+// their derivatives at the limits. No cutoff test (as the reference); the self pair is weighted by
+// central_weight. Then W, sqrt(rcut_hard) (1/sqrt for the derivative) and global_scaling.
+__device__ inline void op_g(double r, double r0, double winv, bool right, double* g) {
+  const double x = (r - r0) * winv, x2 = x * x, x3 = x2 * x, w2 = winv * winv, w3 = w2 * winv;
+  if (!right) {
+    g[0] = 1.0 - 3.0 * x2 - 2.0 * x3; g[1] = -6.0 * (x2 + x) * winv; g[2] = -3.0 * (2.0 * x + 1.0) * w2; g[3] = -2.0 * w3;
+  } else {
+    g[0] = 1.0 - 3.0 * x2 + 2.0 * x3; g[1] = 6.0 * (x2 - x) * winv; g[2] = 3.0 * (2.0 * x - 1.0) * w2; g[3] = 2.0 * w3;
+  }
+}
+
+__device__ inline void op_gd(double r, double r0, double winv, double ws, bool right, double* g) {
+  const double x = (r - r0) * winv, x2 = x * x, x3 = x2 * x, w2 = winv * winv, w3 = w2 * winv, w4 = w3 * winv;
+  if (!right) {
+    g[0] = 6.0 * (x + (ws + 1.0) * x2 + ws * x3) * winv;
+    g[1] = 6.0 * (1.0 + 2.0 * (ws + 1.0) * x + 3.0 * ws * x2) * w2;
+    g[2] = 6.0 * (ws + 1.0 + 3.0 * ws * x) * w3;
+    g[3] = 6.0 * ws * w4;
+  } else {
+    g[0] = 6.0 * (x + (ws - 1.0) * x2 - ws * x3) * winv;
+    g[1] = 6.0 * (1.0 + 2.0 * (ws - 1.0) * x - 3.0 * ws * x2) * w2;
+    g[2] = 6.0 * (ws - 1.0 - 3.0 * ws * x) * w3;
+    g[3] = -6.0 * ws * w4;
+  }
+}
+
+// the degree-6 product of the atom's cubic about 0 at -r_j and the filter's cubic (cf)
+__device__ inline void op_filter_B(double rj, double winv, double ws, const double* cf, bool right, bool der,
+                                   double* B) {
+  double col[4];
+  if (der) op_gd(-rj, 0.0, winv, ws, right, col);
+  else op_g(-rj, 0.0, winv, right, col);
+#pragma unroll
+  for (int r = 0; r < 7; ++r) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc += (r - i >= 0 && r - i < 4 ? col[r - i] : 0.0) * cf[i];
+    B[r] = acc;
+  }
+}
+
+// v(q) = -sum_p B(p) (p-1)!/(p-q)! r^(p-q): minus the derivatives of sum_p B(p) r^(p-1) at r
+__device__ inline void op_bm(const double* B, double r, double* v) {
+  double rt[7];
+  rt[0] = 1.0;
+#pragma unroll
+  for (int p = 1; p < 7; ++p) rt[p] = r * rt[p - 1];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    double acc = 0.0;
+#pragma unroll
+    for (int pp = q; pp < 7; ++pp) {
+      double f = 1.0;
+#pragma unroll
+      for (int t = pp - q + 1; t <= pp; ++t) f *= (double)t;
+      acc += -B[pp] * (rt[pp - q] * f);
+    }
+    v[q] = acc;
+  }
+}
+
+template <int A>
+__global__ __launch_bounds__(128) void radial_op_k(RadialConsts c, int n_pairs, const double* __restrict__ rjs,
+                                                   const uint32_t* __restrict__ spmask,
+                                                   const uint8_t* __restrict__ first, double* __restrict__ rad,
+                                                   double* __restrict__ rad_der) {
+  constexpr int NT = A + 4 > 7 ? A + 4 : 7;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  if (!(spmask[p] & c.species_bit)) return;
+  const bool is_central = first[p] != 0;
+  const double pi = 3.141592653589793;
+  const double rjs_in = rjs[p], K = c.op_K, sc = c.scaling, rs = c.rcut_soft;
+  const double win = K * (c.op_sigma_in + sc * rjs_in);
+  const bool soft = rjs_in - win < c.op_rcut_soft_in;
+  const bool buffer = c.op_rcut_soft_in < c.rcut_hard_in && rjs_in + win > c.op_rcut_soft_in;
+  const double rj = rjs_in * c.inv_rcut;
+  const double sg = c.atom_sigma + sc * rj, s2 = sg * sg, w = K * sg, winv = 1.0 / w, ws = K * sc;
+  double amp, ampd;
+  const double h = 1.0 + 2.0 * (rj * rj * rj) - 3.0 * (rj * rj);
+  if (c.amp_scaling == 0.0) {
+    amp = 1.0 / sg; ampd = -sc / s2;
+  } else if (c.amp_scaling == 1.0) {
+    amp = 1.0 / sg * h;
+    ampd = 6.0 / sg * (rj * rj - rj) - sc / sg * amp;
+  } else {
+    amp = 1.0 / sg * pow(h, c.amp_scaling);
+    ampd = 6.0 * c.amp_scaling / sg * (rj * rj - rj) * pow(h, c.amp_scaling - 1.0) - sc / sg * amp;
+  }
+  if (is_central) { amp = c.central_weight * amp; ampd = c.central_weight * ampd; }
+  if (c.radial_enh == 1) {
+    ampd = amp * (1.0 + sqrt(2.0 / pi) * sc) + ampd * (rj + sqrt(2.0 / pi) * sg);
+    amp = amp * (rj + sqrt(2.0 / pi) * sg);
+  } else if (c.radial_enh == 2) {
+    ampd = amp * (2.0 * rj + 2.0 * sg * sc + sqrt(8.0 / pi) * sg + sqrt(8.0 / pi) * rj * sc) +
+           ampd * (rj * rj + s2 + sqrt(8.0 / pi) * sg * rj);
+    amp = amp * (rj * rj + s2 + sqrt(8.0 / pi) * sg * rj);
+  }
+  const double vect[4] = {-1.0, -1.0, -2.0, -6.0};
+  double ex[A], exd[A];
+#pragma unroll
+  for (int n = 0; n < A; ++n) ex[n] = exd[n] = 0.0;
+  if (soft) {
+    const double L[3] = {rj - w > 0.0 ? rj - w : 0.0, rj < rs ? rj : rs, rs < rj + w ? rs : rj + w};
+    double I0[3][A + 4];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      I0[t][0] = 1.0;
+#pragma unroll
+      for (int i = 1; i < A + 4; ++i) I0[t][i] = I0[t][i - 1] * (1.0 - L[t]);
+    }
+    double sv[A], sd[A];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      double M[3][2][4];
+#pragma unroll
+      for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int pc = 0; pc < 2; ++pc) {
+          double g[4];
+          if (d == 0) op_g(L[t], rj, winv, pc == 1, g);
+          else op_gd(L[t], rj, winv, ws, pc == 1, g);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) M[t][pc][j] = I0[t][j] * g[j] * vect[j];
+        }
+#pragma unroll
+      for (int n = 0; n < A; ++n) {
+        double s12 = 0.0, s11 = 0.0, s23 = 0.0, s22 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double an = c.op_A[n * 7 + j];
+          s12 += M[1][0][j] * an; s11 += M[0][0][j] * an; s23 += M[2][1][j] * an; s22 += M[1][1][j] * an;
+        }
+        const double v = (s12 * I0[1][n + 4] - s11 * I0[0][n + 4]) + (s23 * I0[2][n + 4] - s22 * I0[1][n + 4]);
+        if (d == 0) sv[n] = v;
+        else sd[n] = v;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < A; ++n) {
+      ex[n] = amp * sv[n];
+      exd[n] = amp * sd[n] + ampd * sv[n];
+    }
+  }
+  if (buffer) {
+    const double L[3] = {rs > rj - w ? rs : rj - w, rj > rs ? rj : rs, 1.0 < rj + w ? 1.0 : rj + w};
+    double I0[3][NT];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      I0[t][0] = 1.0;
+#pragma unroll
+      for (int i = 1; i < NT; ++i) I0[t][i] = I0[t][i - 1] * (1.0 - L[t]);
+    }
+    double bb[A], bd[A];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      double M[3][2][7];
+#pragma unroll
+      for (int pc = 0; pc < 2; ++pc) {
+        double B[7];
+        op_filter_B(rj, winv, ws, c.op_cf, pc == 1, d == 1, B);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+          double v[7];
+          op_bm(B, L[t], v);
+#pragma unroll
+          for (int q = 0; q < 7; ++q) M[t][pc][q] = v[q] * I0[t][q];
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < A; ++n) {
+        double s12 = 0.0, s11 = 0.0, s23 = 0.0, s22 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+          const double an = c.op_A[n * 7 + q];
+          s12 += M[1][0][q] * an; s11 += M[0][0][q] * an; s23 += M[2][1][q] * an; s22 += M[1][1][q] * an;
+        }
+        const double v = (s12 * I0[1][n + 4] - s11 * I0[0][n + 4]) + (s23 * I0[2][n + 4] - s22 * I0[1][n + 4]);
+        if (d == 0) bb[n] = v;
+        else bd[n] = v;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < A; ++n) {
+      ex[n] += amp * bb[n];
+      exd[n] += amp * bd[n] + ampd * bb[n];
+    }
+  }
+  const double gs = c.global_scaling;
+  double* out = rad + (size_t)p * c.n_max + c.off;
+#pragma unroll
+  for (int n = 0; n < A; ++n) {
+    double av = 0.0;
+#pragma unroll
+    for (int m = 0; m < A; ++m) av += c.W[n + m * A] * ex[m];
+    out[n] = av * c.sqrt_rcut * gs;
+  }
+  if (c.do_der) {
+    double* outd = rad_der + (size_t)p * c.n_max + c.off;
+#pragma unroll
+    for (int n = 0; n < A; ++n) {
+      double ad = 0.0;
+#pragma unroll
+      for (int m = 0; m < A; ++m) ad += c.W[n + m * A] * exd[m];
+      outd[n] = ad / c.sqrt_rcut * gs;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- angular
 // One thread per pair: get_angular_expansion_coefficients,
 // src/soap_turbo_angular.f90:548-605, with get_eimphi_conjg (:267-329),
@@ -1109,7 +1325,9 @@ void launch_fill(hipStream_t st, double* x, size_t n, double v) {
 template <int A>
 static void radial_one(hipStream_t st, const RadialConsts& c, int n_pairs, const double* rjs, const uint32_t* spmask,
                        const uint8_t* first, double* rad, double* rad_der, double* rad_der2) {
-  if (rad_der2)
+  if (c.op)
+    radial_op_k<A><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, spmask, first, rad, rad_der);
+  else if (rad_der2)
     radial_k<A, true><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, spmask, first, rad, rad_der, rad_der2);
   else
     radial_k<A, false><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, spmask, first, rad, rad_der, nullptr);

@@ -23,6 +23,12 @@ struct RadialConsts {
   double q, drnf, scaling, amp_scaling, central_weight, global_scaling;
   double ra[ST_REC], rb[ST_REC], rc[ST_REC], R1[ST_REC], R2[ST_REC], R3[ST_REC], R4[ST_REC];
   double W[ST_MAX_ALPHA * ST_MAX_ALPHA];  // species block of W, column-major alpha x alpha
+  // poly3operator (op = 1): 2 sqrt(2 ln 2), unscaled sigma and rcut_soft for the region tests, the
+  // basis-side moments A (alpha x 7, row-major) and the filter's cubic at 0 (g_aux(0, rs, fw))
+  int op;
+  double op_K, op_sigma_in, op_rcut_soft_in;
+  double op_A[ST_MAX_ALPHA * 7];
+  double op_cf[4];
 };
 
 // Per-species constants of the angular expansion (src/soap_turbo_angular.f90:548-605).

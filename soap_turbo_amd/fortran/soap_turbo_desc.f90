@@ -111,8 +111,10 @@ contains
       ibasis = 0
     else if( b == "poly3gauss" )then
       ibasis = 1
+    else if( b == "poly3operator" )then
+      ibasis = 2
     else
-      write(*,*) "ERROR: soap_turbo: basis ", b, " is not supported by this build (poly3, poly3gauss)"
+      write(*,*) "ERROR: soap_turbo: basis ", b, " is not supported by this build (poly3, poly3gauss, poly3operator)"
       stop
     end if
     iscal = 0

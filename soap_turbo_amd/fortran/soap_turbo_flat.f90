@@ -36,6 +36,7 @@ contains
     lmask = mask /= 0
     basis = "poly3"
     if( basis_code == 1 ) basis = "poly3gauss"
+    if( basis_code == 2 ) basis = "poly3operator"
     scal = "polynomial"
     if( scaling_code /= 0 ) scal = "none"
     legacy_filter_seed = legacy_seed /= 0

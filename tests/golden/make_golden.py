@@ -80,6 +80,20 @@ CASES.update({
     "darby_2_2": _darby("2_2", [4, 4], 5, "poly3gauss"),
 })
 
+# poly3operator (src/soap_turbo_radial.f90:1549-2141), with the reference's tabulated W. alpha_max >= 4:
+# for smaller alpha_max the reference's soft region writes past an array sized alpha_max
+# (g_aux_left_array(1:nn, 1:4, ...) into (1:nn, 1:alpha_max, ...)) and the run aborts.
+CASES.update({
+    "op_trivial": (SMALL_BOX, SoapParams(alpha_max=[6, 6], l_max=5, rcut_hard=4.6, rcut_soft=3.6, basis="poly3operator",
+                                         atom_sigma_r=[0.45, 0.5], atom_sigma_t=[0.5, 0.45],
+                                         compress_mode="trivial"), 5, True),
+    "op_scalings": (SMALL_BOX, SoapParams(alpha_max=[4, 5], l_max=4, rcut_hard=[4.6, 4.2], rcut_soft=[3.6, 3.9],
+                                          basis="poly3operator", atom_sigma_r=[0.45, 0.5], atom_sigma_t=[0.5, 0.45],
+                                          atom_sigma_r_scaling=[0.05, 0.1], amplitude_scaling=[1.0, 2.0],
+                                          central_weight=[0.5, 1.0], radial_enhancement=1, global_scaling=[1.0, 0.8]),
+                    5, True),
+})
+
 CART_SITES = 2
 
 

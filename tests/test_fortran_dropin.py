@@ -62,7 +62,7 @@ def run_fortran_get_soap(L, w, comp, d, do_der, contract, legacy):
     I = ctypes.c_int32
     args = [I(w.n_sites), I(p.n_species), I(w.n_pairs), I(1)] + [_p(a) for a in keep[:8]] + [I(p.l_max)] + \
         [_p(a) for a in keep[8:17]] + [I(p.radial_enhancement), _p(keep[17]),
-                                      I(1 if p.basis == "poly3gauss" else 0),
+                                      I({"poly3": 0, "poly3gauss": 1, "poly3operator": 2}[p.basis]),
                                       I(0 if p.scaling_mode == "polynomial" else 1), I(int(do_der)),
                                       I(int(comp is not None)), I(len(cp[0]) if comp is not None else 0)] + \
         [_p(a) for a in cp] + [I(n_soap), _p(soap), _p(cart), I(int(contract)), I(int(legacy))]
@@ -71,7 +71,7 @@ def run_fortran_get_soap(L, w, comp, d, do_der, contract, legacy):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["cfg1_legacy", "cfg2_unified", "var_poly3_scalings"])
+@pytest.mark.parametrize("name", ["cfg1_legacy", "cfg2_unified", "var_poly3_scalings", "op_trivial"])
 def test_fortran_get_soap_matches_reference(name):
     # the drop-in builds its own W (FP64 eigen-route), so parity with the reference's W is
     # limited by the conditioning of S: 1e-4 at poly3 alpha_max = 8, 1e-6 otherwise

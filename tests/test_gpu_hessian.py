@@ -86,3 +86,10 @@ def test_hessian_refuses_the_legacy_filter_seed(ctx):
     w, comp, d = load(os.path.join("hessian", HESS_CASES[0]))
     with pytest.raises(st.SoapTurboStop):
         _run(ctx, w, comp, d["vecs"], legacy=True)
+
+
+def test_hessian_refuses_poly3operator(ctx):
+    # the reference stops: the poly3operator basis has no second radial derivatives
+    w, comp, d = load("op_scalings")
+    with pytest.raises(st.SoapTurboStop, match="poly3operator"):
+        _run(ctx, w, comp, np.zeros((w.n_sites, 1, 10)))
