@@ -35,7 +35,7 @@ namespace st {
 namespace {
 
 constexpr int NTAB = 2048;        // grid cells over rcut for the tables
-constexpr int MAXSEG = 3;
+constexpr int MAXSEG = 8;  // radial table segments per species (poly3operator: up to 7)
 constexpr int ADJ_MAX_SPECIES = 8;
 constexpr int PT = 64;            // pairs per tile
 constexpr int NTHREADS = 256;
@@ -60,6 +60,7 @@ struct AdjParams {
   int ang_n;
   SpeciesTab sp[ADJ_MAX_SPECIES];
 };
+static_assert(sizeof(AdjParams) <= 4096, "kernel arguments are limited to 4 KB");
 
 int ensure(AdjBuf& b, size_t bytes, std::string& msg) {
   if (bytes == 0) bytes = 8;
@@ -1193,23 +1194,43 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
     SpeciesTab& sp = P.sp[q];
     const double rc = p->rcut_hard[q];
     const double rs_n = p->rcut_soft[q] / rc, s0 = p->atom_sigma_r[q] / rc, c = p->atom_sigma_r_scaling[q];
+    // poly3operator has no cutoff test: its tables run to the largest cutoff, split where the
+    // integration limits of the atom's cubic (half-width K sigma(r)) switch: r - w = 0, r - w =
+    // rcut_soft, r + w = rcut_soft, r = rcut_soft, r + w = rcut_hard, r = rcut_hard
+    const bool op = p->basis == ST_BASIS_POLY3OPERATOR;
+    const double range = op ? rcut_max : rc;
     std::vector<double> br;
-    if (1.0 - rs_n > 0.0) {
-      double b = rc * (rs_n - 4.0 * s0) / (1.0 + 4.0 * c);
-      if (b > 0.0 && b < rc) br.push_back(b);
-    }
-    if (P.gauss && c < 0.25) {
-      double b = rc * 8.0 * s0 / (1.0 - 4.0 * c);
-      if (b > 0.0 && b < rc) br.push_back(b);
+    if (op) {
+      const double K = 2.0 * std::sqrt(2.0 * std::log(2.0));
+      std::vector<double> cand = {(rs_n - K * s0) / (1.0 + K * c), rs_n, (1.0 - K * s0) / (1.0 + K * c), 1.0};
+      if (1.0 - K * c > 0.0) {
+        cand.push_back(K * s0 / (1.0 - K * c));
+        cand.push_back((rs_n + K * s0) / (1.0 - K * c));
+      }
+      for (double b : cand)
+        if (b * rc > 0.0 && b * rc < range) br.push_back(b * rc);
+    } else {
+      if (1.0 - rs_n > 0.0) {
+        double b = rc * (rs_n - 4.0 * s0) / (1.0 + 4.0 * c);
+        if (b > 0.0 && b < rc) br.push_back(b);
+      }
+      if (P.gauss && c < 0.25) {
+        double b = rc * 8.0 * s0 / (1.0 - 4.0 * c);
+        if (b > 0.0 && b < rc) br.push_back(b);
+      }
     }
     std::sort(br.begin(), br.end());
+    if (br.size() + 1 > (size_t)MAXSEG) {
+      msg = "too many radial table segments";
+      return ST_ERR_UNSUPPORTED;
+    }
     std::vector<double>& e = edges[q];
     e.push_back(0.0);
     for (double b : br)
       if (b - e.back() > 1e-6 * rc) e.push_back(b);
-    e.push_back(rc);
+    e.push_back(range);
     sp.a = p->alpha_max[q];
-    sp.rcut = rc;
+    sp.rcut = range;
     sp.cw = (float)p->central_weight[q];
     sp.do_central = P.gauss ? 0 : (p->central_weight[q] != 0.0);
     sp.nseg = (int)e.size() - 1;

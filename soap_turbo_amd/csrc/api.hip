@@ -1033,8 +1033,6 @@ int st_adjoint_f32(st_ctx* ctx, const st_params* p, const st_compression* c) {
   if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
   int rc = check_params(ctx, p);
   if (rc) return rc;
-  if (p->basis == ST_BASIS_POLY3OPERATOR)
-    return fail(ctx, ST_ERR_UNSUPPORTED, "the FP32 adjoint path supports poly3 and poly3gauss (poly3operator: FP64)");
   HIPCHK(hipSetDevice(ctx->device));
   if ((rc = basis_for(ctx, p))) return rc;
   if ((rc = build_tables(ctx, p, c))) return rc;

@@ -229,15 +229,26 @@ def test_adjoint_f32_matches_reference_darby_fixtures(ctx, name):
     assert rel(pair, d["pair"]) <= 10 * TOL
 
 
-def test_adjoint_f32_refuses_poly3operator(ctx):
-    # the FP32 path covers poly3 and poly3gauss; poly3operator runs on the FP64 path
-    w, comp, d = load("op_trivial")
+@pytest.mark.parametrize("name", ["op_trivial", "op_scalings"])
+def test_adjoint_f32_matches_reference_poly3operator_fixtures(ctx, name):
+    # the poly3operator basis: its tables run to the largest cutoff (the reference has no cutoff
+    # test for it), split where the atom's integration limits switch
+    w, comp, d = load(name)
     p = w.params
     keep = st._lib.Keep()
     prm = st.make_params(keep, alpha_max=p.alpha_max, l_max=p.l_max, radial_enhancement=p.radial_enhancement,
                          basis=p.basis, scaling_mode=p.scaling_mode, **p.per_species())
-    cmp_ = st.make_compression(keep, True, len(comp[0]), *comp)
-    ctx.upload_neighbours(w.n_neigh, w.species.T, w.species_multiplicity, w.mask.T, w.rjs, w.thetas, w.phis)
-    ctx.upload_dedq(np.zeros((w.n_sites, int(comp[0].max()))))
-    with pytest.raises(st.SoapTurboError, match="poly3operator"):
+    cmp_ = st.make_compression(keep, comp is not None, len(comp[0]) if comp else 0, *(comp or (None, None, None)))
+    n_soap = int(comp[0].max()) if comp else d["soap"].shape[1]
+    ctx.set_basis(d["W"], d["S"])
+    try:
+        ctx.upload_neighbours(w.n_neigh, w.species.T, w.species_multiplicity, w.mask.T, w.rjs, w.thetas, w.phis)
+        ctx.upload_dedq(np.asfortranarray(d["G"].T))
         ctx.adjoint_f32(prm, cmp_)
+        q = ctx.download_soap_f32(n_soap).T
+        pair = ctx.download_pairs(1).T
+    finally:
+        ctx.clear_basis()
+    print(f"{name}: rel err q {rel(q, d['soap_contract_q']):.2e} pair {rel(pair, d['pair']):.2e}")
+    assert rel(q, d["soap_contract_q"]) <= TOL
+    assert rel(pair, d["pair"]) <= 10 * TOL
