@@ -282,6 +282,9 @@ __device__ void block_sum(double (&v)[NV], double* red) {
 // pass A: G(a, n, k) = sum_j [g'(r) n_a Omega + g(r) dOmega_a] with g = 4 pi rad F_l; the four
 // wavefronts build the geometry of four pairs per round, then every thread adds them to its
 // elements in pair order
+// EPT > 0: each thread keeps its (at most EPT) elements' sums in registers across the rounds;
+// EPT = 0: they are read and written back in the scratch slot every round
+template <int EPT>
 __global__ __launch_bounds__(HT) void hess_gacc_k(HessParams P, const int* __restrict__ site_ptr,
                                                  const double* __restrict__ heads,
                                                  const double* __restrict__ Ck, const double* __restrict__ rad,
@@ -291,16 +294,32 @@ __global__ __launch_bounds__(HT) void hess_gacc_k(HessParams P, const int* __res
   const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max;
   const double fourpi = 4.0 * 3.141592653589793;
   double2* G = reinterpret_cast<double2*>(scratch + (size_t)i * P.stride + P.o_gacc);
-  for (int e = tx; e < 3 * E; e += HT) G[e] = make_double2(0.0, 0.0);
+  double2 reg[EPT > 0 ? EPT : 1][3];
+  if constexpr (EPT > 0) {
+#pragma unroll
+    for (int u = 0; u < EPT; ++u)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) reg[u][a] = make_double2(0.0, 0.0);
+  } else {
+    for (int e = tx; e < 3 * E; e += HT) G[e] = make_double2(0.0, 0.0);
+  }
   const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
   for (int r0 = k3 + 1; r0 < k4; r0 += HT / 64) {
     const int k2 = r0 + wv < k4 ? r0 + wv : -1;
     wave_geometry(gs[wv], P, k2, heads, Ck, false);
-    for (int e = tx; e < E; e += HT) {
+#pragma unroll
+    for (int u = 0; u < (EPT > 0 ? EPT : 1 << 20); ++u) {
+      const int e = tx + HT * u;
+      if (e >= E) break;
       const int n = e / k_max, k = e - n * k_max;
       int l = 0;
       while ((l + 1) * (l + 2) / 2 <= k) ++l;
-      double2 acc[3] = {G[e], G[E + e], G[2 * E + e]};
+      double2 acc[3];
+      if constexpr (EPT > 0) {
+        acc[0] = reg[u][0]; acc[1] = reg[u][1]; acc[2] = reg[u][2];
+      } else {
+        acc[0] = G[e]; acc[1] = G[E + e]; acc[2] = G[2 * E + e];
+      }
       for (int w = 0; w < HT / 64 && r0 + w < k4; ++w) {
         const PairGeom& g = gs[w];
         if (g.ok == 0.0) continue;
@@ -310,12 +329,24 @@ __global__ __launch_bounds__(HT) void hess_gacc_k(HessParams P, const int* __res
         for (int a = 0; a < 3; ++a)
           acc[a] = cadd(acc[a], cadd(cscale(grad1 * g.nh[a], g.Om[k]), cscale(grad0, g.dOm[3 * k + a])));
       }
-      G[e] = acc[0]; G[E + e] = acc[1]; G[2 * E + e] = acc[2];
+      if constexpr (EPT > 0) {
+        reg[u][0] = acc[0]; reg[u][1] = acc[1]; reg[u][2] = acc[2];
+      } else {
+        G[e] = acc[0]; G[E + e] = acc[1]; G[2 * E + e] = acc[2];
+      }
+    }
+  }
+  if constexpr (EPT > 0) {
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = tx + HT * u;
+      if (e < E) { G[e] = reg[u][0]; G[E + e] = reg[u][1]; G[2 * E + e] = reg[u][2]; }
     }
   }
 }
 
-// D1 = dp/dr_i, the weight vectors and Z = S c, ZG = S G
+// D1 = dp/dr_i, the weight vectors and Z = S c, ZG = S G. (Staging c and G in LDS measured
+// slower at config 2: the 91 KB leave one workgroup per CU.)
 __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2* __restrict__ cnk,
                                                     const double* __restrict__ soap, const double* __restrict__ sdp,
                                                     const double* __restrict__ vecs, const int4* __restrict__ comp,
@@ -648,7 +679,12 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
   if (n_sites == 0) return true;
   if (n_pairs > 0)
     hess_geom_k<<<(n_pairs + 127) / 128, 128, 0, st>>>(P, rjs, thetas, phis, spmask, first, heads);
-  hess_gacc_k<<<n_sites, HT, 0, st>>>(P, site_ptr, heads, Ck, rad, rad_der, scratch);
+  switch ((int)((E + HT - 1) / HT)) {
+#define GA(n) case n: hess_gacc_k<n><<<n_sites, HT, 0, st>>>(P, site_ptr, heads, Ck, rad, rad_der, scratch); break;
+    GA(1) GA(2) GA(3) GA(4) GA(5) GA(6) GA(7) GA(8) GA(9) GA(10) GA(11) GA(12)
+#undef GA
+    default: hess_gacc_k<0><<<n_sites, HT, 0, st>>>(P, site_ptr, heads, Ck, rad, rad_der, scratch);
+  }
   const size_t wl = (size_t)n_unc * 8;
   if (wl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_weights_k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)wl);
