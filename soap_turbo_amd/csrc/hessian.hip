@@ -9,10 +9,12 @@
 //   hess_geom_k      one thread per pair: the pair's geometry head (F_l and its radial derivatives,
 //                    the unit vector, the derivatives of t = z/r and w = (x - iy)/r, w^m, Q_l^m(t));
 //   hess_gacc_k      one workgroup per site: G(n, k, a) = sum_j dc^j_(n,k) / du_ja (pass A);
-//   hess_weights_k   one workgroup per site: the central-atom derivative D1 = dp/dr_i from G and c
-//                    (an accumulated form of minus the sum of the per-pair derivatives), then for
-//                    each weight vector (the caller's, p itself, the three columns of D1) its
-//                    uncompressed S matrix, Z = S c and, for the caller's vectors and p, ZG = S G;
+//   hess_d1_k        one workgroup per site: the central-atom derivative D1 = dp/dr_i from G and c
+//                    (an accumulated form of minus the sum of the per-pair derivatives), and the
+//                    per-site dot products of the normalisation;
+//   hess_weights_k   one workgroup per (site, weight vector): the caller's vectors, p itself and
+//                    the three columns of D1, each spread to its uncompressed S matrix, Z = S c and,
+//                    for the caller's vectors and p, ZG = S G;
 //   hess_contract_k  one workgroup per (site, weight vector), Z and ZG in LDS: per pair, Omega =
 //                    C_k Q_l^m(t) w^m with its Cartesian gradient and Hessian, contracted (pass B);
 //   hess_norm_k      one workgroup per site: the quotient rule of the normalisation, and the self
@@ -345,9 +347,8 @@ __global__ __launch_bounds__(HT) void hess_gacc_k(HessParams P, const int* __res
   }
 }
 
-// D1 = dp/dr_i, the weight vectors and Z = S c, ZG = S G. (Staging c and G in LDS measured
-// slower at config 2: the 91 KB leave one workgroup per CU.)
-__global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2* __restrict__ cnk,
+// D1 = dp/dr_i, the unnormalised p and the per-site dot products, one workgroup per site
+__global__ __launch_bounds__(HT) void hess_d1_k(HessParams P, const double2* __restrict__ cnk,
                                                     const double* __restrict__ soap, const double* __restrict__ sdp,
                                                     const double* __restrict__ vecs, const int4* __restrict__ comp,
                                                     const uint8_t* __restrict__ skip, const int* __restrict__ out_ptr,
@@ -355,16 +356,13 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
                                                     const int* __restrict__ unc_ptr, const int* __restrict__ unc_e,
                                                     const int32_t* __restrict__ P_i, const double* __restrict__ P_el,
                                                     double* __restrict__ scratch) {
-  extern __shared__ double vunc[];  // n_unc
   __shared__ double red[4 * 8];
   const int i = blockIdx.x, tx = threadIdx.x;
-  const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, L1 = P.l_max + 1;
+  const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max;
   const int n_unc = P.n_unc, n_soap = P.n_soap, n_vec = P.n_vec;
   const double sq2 = 1.4142135623730951;
   double* base = scratch + (size_t)i * P.stride;
   const double2* G = reinterpret_cast<const double2*>(base + P.o_gacc);
-  double2* Z = reinterpret_cast<double2*>(base + P.o_z);
-  double2* ZG = reinterpret_cast<double2*>(base + P.o_zg);
   double* D1u = base + P.o_d1u;
   double* D1c = base + P.o_d1c;
   double* pvec = base + P.o_pvec;
@@ -426,9 +424,28 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
         for (int a = 0; a < 4; ++a) scal[3 + 4 * vv + a] = x[a];
     }
   }
-  // weight vectors: the caller's (n_vec), p, then the columns of D1
-  const int nvz = n_vec + 4;
-  for (int v = 0; v < nvz; ++v) {
+}
+
+// one weight vector per workgroup: its spread P^T v, Z = S c and (caller's vectors and p) ZG = S G
+__global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2* __restrict__ cnk,
+                                                    const double* __restrict__ vecs, const uint8_t* __restrict__ skip,
+                                                    const int* __restrict__ unc_ptr, const int* __restrict__ unc_e,
+                                                    const int32_t* __restrict__ P_i, const double* __restrict__ P_el,
+                                                    double* __restrict__ scratch) {
+  extern __shared__ double vunc[];  // n_unc
+  const int i = blockIdx.x, v = blockIdx.y, tx = threadIdx.x;
+  const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, L1 = P.l_max + 1;
+  const int n_unc = P.n_unc, n_soap = P.n_soap, n_vec = P.n_vec;
+  const double sq2 = 1.4142135623730951;
+  double* base = scratch + (size_t)i * P.stride;
+  const double2* G = reinterpret_cast<const double2*>(base + P.o_gacc);
+  double2* Z = reinterpret_cast<double2*>(base + P.o_z);
+  double2* ZG = reinterpret_cast<double2*>(base + P.o_zg);
+  const double* D1c = base + P.o_d1c;
+  const double* pvec = base + P.o_pvec;
+  const double2* cn = cnk + (size_t)i * E;
+  // weight vector v: the caller's (v < n_vec), p (v = n_vec), then the columns of D1
+  {
     const double* vin = v < n_vec ? vecs + ((size_t)i * n_vec + v) * n_soap : v == n_vec ? pvec : D1c + (v - n_vec - 1) * n_soap;
     __syncthreads();
     // P^T v, skipped components cleared (spread_weights, src/soap_turbo.f90:1510-1525)
@@ -685,11 +702,12 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
 #undef GA
     default: hess_gacc_k<0><<<n_sites, HT, 0, st>>>(P, site_ptr, heads, Ck, rad, rad_der, scratch);
   }
+  hess_d1_k<<<n_sites, HT, 0, st>>>(P, cnk, soap, sdp, vecs, comp, skip, out_ptr, out_c, out_el, unc_ptr, unc_e,
+                                     P_i, P_el, scratch);
   const size_t wl = (size_t)n_unc * 8;
   if (wl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_weights_k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)wl);
-  hess_weights_k<<<n_sites, HT, wl, st>>>(P, cnk, soap, sdp, vecs, comp, skip, out_ptr, out_c, out_el, unc_ptr,
-                                          unc_e, P_i, P_el, scratch);
+  hess_weights_k<<<dim3(n_sites, n_vec + 4), HT, wl, st>>>(P, cnk, vecs, skip, unc_ptr, unc_e, P_i, P_el, scratch);
   const size_t cl = E * 4 * sizeof(double2);
   if (cl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_contract_k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl);
