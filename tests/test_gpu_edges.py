@@ -83,3 +83,22 @@ def test_one_site(ctx):
     _, ho, _, _ = orc.central_hessian(w, W, S, vecs, None)
     assert rel(h, ho) <= 1e-10
     assert np.abs(h.sum(axis=0)).max() <= 1e-10 * np.abs(h).max()
+
+
+def test_poly3operator_small_alpha(ctx):
+    # alpha_max < 4: no reference output exists (the reference writes past an array there);
+    # the device paths are held to the oracle
+    p = SoapParams(alpha_max=[2, 3], l_max=3, rcut_hard=4.0, rcut_soft=3.4, basis="poly3operator",
+                   atom_sigma_r=0.45, atom_sigma_t=0.45)
+    w = make_workload((BoxSpec(90, 0.05, 1.2, [0.5, 0.5], [1, 8], seed=11), p), n_sites=20)
+    keep, prm, cmp_, W, S = _setup(ctx, w, legacy=False)
+    n_soap = 5 * 6 // 2 * 4
+    G = normal(3, (n_soap, w.n_sites))
+    qo, pair_o = orc.contract(w, W, S, G, 1.0, None, False)
+    ctx.compute(prm, cmp_, False, False)
+    q64, _ = ctx.download(n_soap, cart=False)
+    assert rel(q64.T, qo) <= 1e-10
+    ctx.upload_dedq(np.asfortranarray(G.T))
+    ctx.adjoint_f32(prm, cmp_)
+    assert rel(ctx.download_soap_f32(n_soap).T, qo) <= 1e-5
+    assert rel(ctx.download_pairs(1).T, pair_o) <= 1e-4
