@@ -504,6 +504,69 @@ __device__ inline void omega_k(const PairHead& h, int k, int l, int mm, double C
       }
 }
 
+// Sums over n of one pair's radial values against the staged Z(v) (and ZG(v)) at one k:
+//   a_t = sum_n R_t(n) conj Z(n, k), t = 0, 1 (and 2 with SECOND), R_t = rad, rad', rad'';
+//   b_t(a) = sum_n R_t(n) conj ZG(a, n, k), t = 0, 1 (SECOND only).
+// zp points at Z(0, k) in LDS. The sums run unconjugated with fused multiply-adds and are
+// conjugated at the end (a negation, exact). Lane n holds R_t(n) (64 n at a time), read out with
+// v_readlane, so every lane of the wavefront must have loaded its values: the caller runs this
+// with the whole wavefront, each lane's k clamped into range. U LDS reads per operand are issued
+// before their sums, so their latencies overlap.
+struct RadialSums {
+  double2 a0, a1, a2, b0[3], b1[3];
+};
+
+template <bool SECOND, int U>
+__device__ inline void radial_steps(const double2* __restrict__ zp, int E, int k_max, int j, double rv0, double rv1,
+                                    double rv2, RadialSums& s) {
+  double2 zc[U], gc[SECOND ? U : 1][3];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    zc[u] = zp[(j + u) * k_max];
+    if constexpr (SECOND)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) gc[u][a] = zp[(a + 1) * E + (j + u) * k_max];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const double r0 = readlane_d(rv0, j + u), r1 = readlane_d(rv1, j + u);
+    s.a0.x = fma(r0, zc[u].x, s.a0.x); s.a0.y = fma(r0, zc[u].y, s.a0.y);
+    s.a1.x = fma(r1, zc[u].x, s.a1.x); s.a1.y = fma(r1, zc[u].y, s.a1.y);
+    if constexpr (SECOND) {
+      const double r2 = readlane_d(rv2, j + u);
+      s.a2.x = fma(r2, zc[u].x, s.a2.x); s.a2.y = fma(r2, zc[u].y, s.a2.y);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        s.b0[a].x = fma(r0, gc[u][a].x, s.b0[a].x); s.b0[a].y = fma(r0, gc[u][a].y, s.b0[a].y);
+        s.b1[a].x = fma(r1, gc[u][a].x, s.b1[a].x); s.b1[a].y = fma(r1, gc[u][a].y, s.b1[a].y);
+      }
+    }
+  }
+}
+
+template <bool SECOND>
+__device__ inline void radial_sums(const double2* __restrict__ zp, int E, int k_max, int n_max, int lane,
+                                   const double* __restrict__ r, const double* __restrict__ rd,
+                                   const double* __restrict__ rh, RadialSums& s) {
+  const double2 zero = make_double2(0.0, 0.0);
+  s.a0 = s.a1 = s.a2 = zero;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) s.b0[a] = s.b1[a] = zero;
+  for (int c0 = 0; c0 < n_max; c0 += 64) {
+    const int nl = c0 + lane;
+    const double rv0 = nl < n_max ? r[nl] : 0.0, rv1 = nl < n_max ? rd[nl] : 0.0;
+    const double rv2 = SECOND && nl < n_max ? rh[nl] : 0.0;
+    const int c1 = n_max - c0 < 64 ? n_max - c0 : 64;
+    const double2* zc = zp + (size_t)c0 * k_max;
+    int j = 0;
+    for (; j + 4 <= c1; j += 4) radial_steps<SECOND, 4>(zc, E, k_max, j, rv0, rv1, rv2, s);
+    for (; j < c1; ++j) radial_steps<SECOND, 1>(zc, E, k_max, j, rv0, rv1, rv2, s);
+  }
+  s.a0.y = -s.a0.y; s.a1.y = -s.a1.y; s.a2.y = -s.a2.y;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { s.b0[a].y = -s.b0[a].y; s.b1[a].y = -s.b1[a].y; }
+}
+
 // pass B, contraction: one workgroup per (site, weight vector v), with Z(v) (and, for the caller's
 // vectors and p, ZG(v)) staged in LDS once; its wavefronts walk the site's pairs, lanes over k.
 // Per pair and v it writes 12 sums: v.dp/du_jb (3) and, with second derivatives, v.d2p/dr_ia
@@ -537,37 +600,23 @@ __global__ __launch_bounds__(HT) void hess_contract_k(HessParams P, const int* _
     double acc[12];
 #pragma unroll
     for (int q = 0; q < 12; ++q) acc[q] = 0.0;
-    for (int k = lane; k < k_max; k += 64) {
+    // lanes over k, 64 at a time; the whole wavefront runs the radial sums (k clamped)
+    for (int kb = 0; kb < k_max; kb += 64) {
+      const int k = kb + lane;
+      RadialSums s;
+      const double2* zp = zs + (k < k_max ? k : k_max - 1);
+      if (second)
+        radial_sums<true>(zp, E, k_max, n_max, lane, r, rd, rh, s);
+      else
+        radial_sums<false>(zp, E, k_max, n_max, lane, r, rd, rh, s);
+      if (k >= k_max) continue;
       int l = 0;
       while ((l + 1) * (l + 2) / 2 <= k) ++l;
       const int m = k - l * (l + 1) / 2;
       const double f0 = h.Fl[l], f1 = h.Fl1[l], f2 = h.Fl2[l], wmk = m > 0 ? 2.0 : 1.0;
-      double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0, b0[3] = {a0, a0, a0}, b1[3] = {a0, a0, a0};
-      // the pair's radial values: lane n holds n (64 at a time), read out with a uniform lane index
-      for (int c0 = 0; c0 < n_max; c0 += 64) {
-        const int nl = c0 + lane;
-        const double rv0 = nl < n_max ? r[nl] : 0.0, rv1 = nl < n_max ? rd[nl] : 0.0;
-        const double rv2 = second && nl < n_max ? rh[nl] : 0.0;
-        const int c1 = n_max - c0 < 64 ? n_max - c0 : 64;
-        for (int j = 0; j < c1; ++j) {
-          const int n = c0 + j;
-          const double r0 = readlane_d(rv0, j), r1 = readlane_d(rv1, j);
-          const double2 zc = zs[n * k_max + k];
-          const double2 z = make_double2(zc.x, -zc.y);
-          a0 = cadd(a0, cscale(r0, z));
-          a1 = cadd(a1, cscale(r1, z));
-          if (second) {
-            a2 = cadd(a2, cscale(readlane_d(rv2, j), z));
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const double2 gc = zs[E + a * E + n * k_max + k];
-              const double2 gz = make_double2(gc.x, -gc.y);
-              b0[a] = cadd(b0[a], cscale(r0, gz));
-              b1[a] = cadd(b1[a], cscale(r1, gz));
-            }
-          }
-        }
-      }
+      const double2 a0 = s.a0, a1 = s.a1, a2 = s.a2;
+      const double2(&b0)[3] = s.b0;
+      const double2(&b1)[3] = s.b1;
       double2 om, dom[3], d2om[9];
       omega_k(h, k, l, m, Ck[k], second, om, dom, d2om);
       const double2 z0 = cscale(fourpi * f0, a0);

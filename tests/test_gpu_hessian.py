@@ -13,7 +13,7 @@ import pytest
 
 import soap_turbo_amd as st
 from oracle import oracle as orc
-from soap_turbo_amd.workload import make_workload, normal
+from soap_turbo_amd.workload import BoxSpec, SoapParams, make_workload, normal
 from tests.golden_cases import GOLDEN, load, rel
 
 pytestmark = pytest.mark.gpu
@@ -56,9 +56,7 @@ def test_hessian_matches_reference_fixtures(ctx, name):
     assert rel(h, d["hess"]) <= TOL
 
 
-@pytest.mark.parametrize("cfg,n_sites", [(1, None), (4, 150)])
-def test_hessian_matches_oracle_on_a_configuration(ctx, cfg, n_sites):
-    w = make_workload(cfg, n_sites=n_sites)
+def _check_against_oracle(ctx, w, seed, label):
     p = w.params
     keep = st._lib.Keep()
     prm = _params(keep, w)
@@ -70,16 +68,31 @@ def test_hessian_matches_oracle_on_a_configuration(ctx, cfg, n_sites):
         comp = (np.zeros(nz, np.int32), np.zeros(nz, np.int32), np.zeros(nz))
         st.get_compress_indices("trivial", p.alpha_max, p.l_max, "set_indices", *comp)
     n_soap = int(comp[0].max()) if comp else n_max * (n_max + 1) // 2 * (p.l_max + 1)
-    vecs = normal(31 + cfg, (w.n_sites, 2, n_soap))
+    vecs = normal(seed, (w.n_sites, 2, n_soap))
     h = _run(ctx, w, comp, vecs)
     _, ho, _, err = orc.central_hessian(w, W, S, vecs, comp)
-    print(f"config {cfg}: {w.n_sites} sites, {w.n_pairs} pairs, rel err {rel(h, ho):.2e}, oracle self-test {err:.1e}")
+    print(f"{label}: {w.n_sites} sites, {w.n_pairs} pairs, rel err {rel(h, ho):.2e}, oracle self-test {err:.1e}")
     assert rel(h, ho) <= TOL
     # every site's row of blocks sums to zero
     k3 = 0
     for n in w.n_neigh[:50]:
         assert np.abs(h[k3:k3 + n].sum(axis=0)).max() <= 1e-10 * np.abs(h[k3:k3 + n]).max()
         k3 += n
+
+
+@pytest.mark.parametrize("cfg,n_sites", [(1, None), (4, 150)])
+def test_hessian_matches_oracle_on_a_configuration(ctx, cfg, n_sites):
+    _check_against_oracle(ctx, make_workload(cfg, n_sites=n_sites), 31 + cfg, f"config {cfg}")
+
+
+# Shapes where the radial index runs past the lanes that hold an angular index: n_max 24 against
+# k_max 10 (l_max 3), and against the 14 k of the second pass of 64 at l_max 11 (k_max 78).
+@pytest.mark.parametrize("l_max", [3, 11])
+def test_hessian_with_more_radial_than_angular_channels(ctx, l_max):
+    box = BoxSpec(432, 0.09, 1.0, [0.5, 0.3, 0.2], [1, 6, 8], seed=12, central=1)
+    prm = SoapParams(alpha_max=[8, 8, 8], l_max=l_max, rcut_hard=4.5, rcut_soft=4.0, basis="poly3gauss",
+                     compress_mode="trivial")
+    _check_against_oracle(ctx, make_workload((box, prm), n_sites=10), 40 + l_max, f"l_max {l_max}, n_max 24")
 
 
 def test_hessian_refuses_the_legacy_filter_seed(ctx):
