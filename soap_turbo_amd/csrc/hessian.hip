@@ -51,6 +51,11 @@ struct HessParams {
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+// Re(a b), and a b, with fused multiply-adds
+__device__ inline double re_mul(double2 a, double2 b) { return fma(a.x, b.x, -(a.y * b.y)); }
+__device__ inline double2 cmul_f(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
 // lane l's value of v, l uniform over the wavefront (v_readlane)
 __device__ inline double readlane_d(double v, int l) {
   const long long b = __double_as_longlong(v);
@@ -483,27 +488,6 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
   }
 }
 
-// Omega_k, its gradient and (second) its Hessian for one k, from a pair's head (in registers)
-__device__ inline void omega_k(const PairHead& h, int k, int l, int mm, double C, bool second, double2& om,
-                               double2 (&dom)[3], double2 (&d2om)[9]) {
-  const double q0 = h.Qt[k], q1 = mm + 1 <= l ? h.Qt[k + 1] : 0.0, q2 = mm + 2 <= l ? h.Qt[k + 2] : 0.0;
-  const double dm = (double)mm;
-  const double2 w0 = h.wp[mm + 2], w1 = h.wp[mm + 1], w2 = h.wp[mm];
-  om = cscale(C * q0, w0);
-  for (int a = 0; a < 3; ++a) dom[a] = cscale(C, cadd(cscale(q1 * h.tvec[a], w0), cscale(q0 * dm, cmul(w1, h.wvec[a]))));
-  if (second)
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) {
-        double2 t = cscale(q2 * h.tvec[a] * h.tvec[b], w0);
-        t = cadd(t, cscale(q1 * h.thes[3 * a + b], w0));
-        double2 tw = cadd(cscale(h.tvec[a], h.wvec[b]), cscale(h.tvec[b], h.wvec[a]));
-        t = cadd(t, cscale(q1 * dm, cmul(w1, tw)));
-        t = cadd(t, cscale(q0 * (double)(mm * (mm - 1)), cmul(w2, cmul(h.wvec[a], h.wvec[b]))));
-        t = cadd(t, cscale(q0 * dm, cmul(w1, h.whes[3 * a + b])));
-        d2om[3 * a + b] = cscale(C, t);
-      }
-}
-
 // Sums over n of one pair's radial values against the staged Z(v) (and ZG(v)) at one k:
 //   a_t = sum_n R_t(n) conj Z(n, k), t = 0, 1 (and 2 with SECOND), R_t = rad, rad', rad'';
 //   b_t(a) = sum_n R_t(n) conj ZG(a, n, k), t = 0, 1 (SECOND only).
@@ -567,6 +551,34 @@ __device__ inline void radial_sums(const double2* __restrict__ zp, int E, int k_
   for (int a = 0; a < 3; ++a) { s.b0[a].y = -s.b0[a].y; s.b1[a].y = -s.b1[a].y; }
 }
 
+// One step of a transposing wavefront sum: lanes with bit d set keep the upper H of their 2H
+// values and send the lower H to lane ^ d, which keeps the lower H and sends the upper.
+template <int H>
+__device__ inline void halve_sum(double (&u)[16], int lane, int d) {
+  const bool hi = (lane & d) != 0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double keep = hi ? u[i + H] : u[i], send = hi ? u[i] : u[i + H];
+    u[i] = keep + __shfl_xor(send, d, 64);
+  }
+}
+
+// Sums of acc[0..11] over the wavefront in 17 exchanges (72 for twelve separate butterflies): lane
+// L ends with the sum of acc[(L >> 2) & 15] (zero for 12..15).
+__device__ inline double wave_sum12(const double (&acc)[12], int lane) {
+  double u[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) u[q] = q < 12 ? acc[q] : 0.0;
+  halve_sum<8>(u, lane, 32);
+  halve_sum<4>(u, lane, 16);
+  halve_sum<2>(u, lane, 8);
+  halve_sum<1>(u, lane, 4);
+  double x = u[0];
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 1, 64);
+  return x;
+}
+
 // pass B, contraction: one workgroup per (site, weight vector v), with Z(v) (and, for the caller's
 // vectors and p, ZG(v)) staged in LDS once; its wavefronts walk the site's pairs, lanes over k.
 // Per pair and v it writes 12 sums: v.dp/du_jb (3) and, with second derivatives, v.d2p/dr_ia
@@ -602,58 +614,81 @@ __global__ __launch_bounds__(HT) void hess_contract_k(HessParams P, const int* _
     for (int q = 0; q < 12; ++q) acc[q] = 0.0;
     // lanes over k, 64 at a time; the whole wavefront runs the radial sums (k clamped)
     for (int kb = 0; kb < k_max; kb += 64) {
-      const int k = kb + lane;
-      RadialSums s;
-      const double2* zp = zs + (k < k_max ? k : k_max - 1);
-      if (second)
-        radial_sums<true>(zp, E, k_max, n_max, lane, r, rd, rh, s);
-      else
-        radial_sums<false>(zp, E, k_max, n_max, lane, r, rd, rh, s);
-      if (k >= k_max) continue;
+      const int k = kb + lane, kc = k < k_max ? k : k_max - 1;
       int l = 0;
-      while ((l + 1) * (l + 2) / 2 <= k) ++l;
-      const int m = k - l * (l + 1) / 2;
+      while ((l + 1) * (l + 2) / 2 <= kc) ++l;
+      const int m = kc - l * (l + 1) / 2;
+      // Omega_k = C Q_l^m(t) w^m enters only through real parts of products with it, so each term
+      // Re(y d^n Omega) is formed from y C w^m, y C w^(m-1) and y C w^(m-2) and the head's
+      // derivatives of t and w (the chain rule of pair_geometry, src/soap_turbo.f90:1648-1697).
+      // The head's per-k values are loaded before the radial sums, so their latency overlaps them.
       const double f0 = h.Fl[l], f1 = h.Fl1[l], f2 = h.Fl2[l], wmk = m > 0 ? 2.0 : 1.0;
-      const double2 a0 = s.a0, a1 = s.a1, a2 = s.a2;
-      const double2(&b0)[3] = s.b0;
-      const double2(&b1)[3] = s.b1;
-      double2 om, dom[3], d2om[9];
-      omega_k(h, k, l, m, Ck[k], second, om, dom, d2om);
-      const double2 z0 = cscale(fourpi * f0, a0);
-      const double2 z1 = cscale(fourpi, cadd(cscale(f0, a1), cscale(f1, a0)));
-      for (int b = 0; b < 3; ++b) {
-        const double2 t = cadd(cmul(cscale(nh[b], z1), om), cmul(z0, dom[b]));
-        acc[b] += wmk * t.x;
-      }
+      const double q0 = h.Qt[kc], q1 = m + 1 <= l ? h.Qt[kc + 1] : 0.0, q2 = m + 2 <= l ? h.Qt[kc + 2] : 0.0;
+      const double2 w0 = h.wp[m + 2], w1 = h.wp[m + 1], w2 = h.wp[m];
+      RadialSums s;
+      if (second)
+        radial_sums<true>(zs + kc, E, k_max, n_max, lane, r, rd, rh, s);
+      else
+        radial_sums<false>(zs + kc, E, k_max, n_max, lane, r, rd, rh, s);
+      if (k >= k_max) continue;
+      const double q0m = q0 * (double)m, q1m = q1 * (double)m, q0mm = q0 * (double)(m * (m - 1));
+      double tv[3];
+      double2 wv[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) { tv[a] = h.tvec[a]; wv[a] = h.wvec[a]; }
+      const double c4 = fourpi * Ck[kc];
+      // y0 = C z0, y1 = C z1 with z0 = 4 pi f0 a0, z1 = 4 pi (f0 a1 + f1 a0)
+      const double2 y0 = cscale(c4 * f0, s.a0);
+      const double2 y1 = cscale(c4, make_double2(fma(f0, s.a1.x, f1 * s.a0.x), fma(f0, s.a1.y, f1 * s.a0.y)));
+      const double P0 = re_mul(y0, w0), P1 = re_mul(y1, w0);  // Re(z0 Omega) / q0, Re(z1 Omega) / q0
+      const double2 U1 = cmul_f(y0, w1), V1 = cmul_f(y1, w1);
+      double R1[3], S1[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) { R1[a] = re_mul(U1, wv[a]); S1[a] = re_mul(V1, wv[a]); }
+      const double zo1 = q0 * P1;  // Re(z1 Omega)
+      // first derivatives: Re(n_b z1 Omega + z0 dOmega_b), Re(z0 dOmega_b) = q1 t_b P0 + q0 m R1_b
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc[b] = fma(wmk, fma(nh[b], zo1, fma(q1 * tv[b], P0, q0m * R1[b])), acc[b]);
       if (second) {
-        const double2 z2 = cscale(fourpi, cadd(cadd(cscale(f0, a2), cscale(2.0 * f1, a1)), cscale(f2, a0)));
-        double2 gg0[3], gg1[3];
+        // yr = C (z2 - z1 / r), z2 = 4 pi (f0 a2 + 2 f1 a1 + f2 a0)
+        const double2 z2 = make_double2(fma(f0, s.a2.x, fma(2.0 * f1, s.a1.x, f2 * s.a0.x)),
+                                        fma(f0, s.a2.y, fma(2.0 * f1, s.a1.y, f2 * s.a0.y)));
+        const double2 yr = make_double2(fma(c4, z2.x, -invr * y1.x), fma(c4, z2.y, -invr * y1.y));
+        const double zor = q0 * re_mul(yr, w0);  // Re(zr Omega)
+        const double2 U2 = cmul_f(y0, w2);
+        double E1[3], H0[3], H1[3];
+        double2 G1[3];
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
-          gg0[a] = cscale(fourpi * f0, b0[a]);
-          gg1[a] = cscale(fourpi, cadd(cscale(f0, b1[a]), cscale(f1, b0[a])));
+          E1[a] = fma(q1 * tv[a], P1, q0m * S1[a]);  // Re(z1 dOmega_a)
+          // C gg0_a and C gg1_a with gg0 = 4 pi f0 b0, gg1 = 4 pi (f0 b1 + f1 b0)
+          const double2 g0 = cscale(c4 * f0, s.b0[a]);
+          const double2 g1 = cscale(c4, make_double2(fma(f0, s.b1[a].x, f1 * s.b0[a].x),
+                                                     fma(f0, s.b1[a].y, f1 * s.b0[a].y)));
+          H1[a] = q0 * re_mul(g1, w0);  // Re(gg1_a Omega)
+          H0[a] = re_mul(g0, w0);
+          G1[a] = cmul_f(g0, w1);
         }
-        const double2 zr = make_double2(z2.x - z1.x * invr, z2.y - z1.y * invr);
+#pragma unroll
         for (int b = 0; b < 3; ++b)
+#pragma unroll
           for (int a = 0; a < 3; ++a) {
-            double2 tE = cmul(cscale(nh[a] * nh[b], zr), om);
-            tE = cadd(tE, cmul(z1, cadd(cscale(nh[a], dom[b]), cscale(nh[b], dom[a]))));
-            tE = cadd(tE, cmul(z0, d2om[3 * a + b]));
-            if (a == b) tE = cadd(tE, cmul(cscale(invr, z1), om));
-            const double2 tD = cadd(cmul(cscale(nh[b], gg1[a]), om), cmul(gg0[a], dom[b]));
-            acc[3 + 3 * a + b] -= wmk * (tE.x + tD.x);
+            // Re(z0 d2Omega_ab)
+            double d2 = fma(fma(q2 * tv[a], tv[b], q1 * h.thes[3 * a + b]), P0,
+                            q1m * fma(tv[a], R1[b], tv[b] * R1[a]));
+            d2 = fma(q0mm, re_mul(U2, cmul_f(wv[a], wv[b])), d2);
+            d2 = fma(q0m, re_mul(U1, h.whes[3 * a + b]), d2);
+            // Re(n_a n_b zr Omega + z1 (n_a dOmega_b + n_b dOmega_a) + z0 d2Omega_ab + [a = b] z1 Omega / r)
+            double tE = fma(nh[a] * nh[b], zor, fma(nh[a], E1[b], fma(nh[b], E1[a], d2)));
+            if (a == b) tE = fma(invr, zo1, tE);
+            // Re(n_b gg1_a Omega + gg0_a dOmega_b)
+            const double tD = fma(nh[b], H1[a], fma(q1 * tv[b], H0[a], q0m * re_mul(G1[a], wv[b])));
+            acc[3 + 3 * a + b] = fma(-wmk, tE + tD, acc[3 + 3 * a + b]);
           }
       }
     }
-#pragma unroll
-    for (int q = 0; q < 12; ++q)
-      for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
-    if (lane < 12) {
-      double x = acc[0];
-#pragma unroll
-      for (int q = 1; q < 12; ++q)
-        if (lane == q) x = acc[q];
-      res[((size_t)k2 * nvz + v) * 12 + lane] = x;
-    }
+    const double x = wave_sum12(acc, lane);
+    if ((lane & 3) == 0 && (lane >> 2) < 12) res[((size_t)k2 * nvz + v) * 12 + (lane >> 2)] = x;
   }
 }
 
