@@ -1396,7 +1396,7 @@ int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c,
                             ctx->hess_scratch.as<double>(), ctx->hess_heads.as<double>(), ctx->hess_res.as<double>(),
                             ctx->hess_out.as<double>());
   });
-  if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max or n_species out of the Hessian's compiled range");
+  if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max or n_species out of the Hessian's compiled range, or n_sites * (n_vec + 4) above 2^31 - 1");
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(hess, ctx->hess_out.p, P * 9 * n_vec * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -64,6 +64,18 @@ __device__ inline double readlane_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// (site, weight vector) of a workgroup of a 1-D grid of n_sites * nv. Workgroups are dealt to the
+// eight XCDs (each with its own L2) in turn; this bijection gives each XCD a contiguous run of the
+// site-major list, so the vectors of one site run together on one XCD and share its L2 copy of the
+// site's data (c and G, the pairs' heads and radial values).
+__device__ inline void site_and_vector(int nv, int& i, int& v) {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b & 7, j = b >> 3, q = nb >> 3, rmd = nb & 7;
+  const int w = x * q + (x < rmd ? x : rmd) + j;
+  i = w / nv;
+  v = w - i * nv;
+}
+
 // get_ilexp, src/soap_turbo_angular.f90:361-422, to lmax (<= MAXL + 2)
 __device__ void ilexp_dev(int lmax, double xx, double* il) {
   const double xcut = 1e-7;
@@ -431,6 +443,59 @@ __global__ __launch_bounds__(HT) void hess_d1_k(HessParams P, const double2* __r
   }
 }
 
+// Z(n, k) = sum_n' S(n, n', l) c(n', k) and (WITH_G) ZG(a, n, k) = sum_n' S(n, n', l) G(a, n', k),
+// S(n, n', l) = 2 vu(n, n', l) on the diagonal and sqrt 2 vu off it, vu the spread weight vector in
+// LDS. Each thread takes NB consecutive n at one k, so a column of c (and G) is read from memory
+// once per NB outputs; consecutive threads take consecutive k.
+template <bool WITH_G>
+__device__ inline void spread_rows(int n_max, int k_max, int L1, int E, const double2* __restrict__ cn,
+                                   const double2* __restrict__ G, const double* __restrict__ vu,
+                                   double2* __restrict__ Z, double2* __restrict__ ZG, int tx) {
+  constexpr int NB = 4;
+  const double sq2 = 1.4142135623730951;
+  const int nblk = (n_max + NB - 1) / NB;
+  for (int t = tx; t < nblk * k_max; t += HT) {
+    const int blk = t / k_max, k = t - blk * k_max, n0 = blk * NB;
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= k) ++l;
+    double2 z[NB], zg[WITH_G ? NB : 1][3];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      z[u] = make_double2(0.0, 0.0);
+      if constexpr (WITH_G)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) zg[u][a] = make_double2(0.0, 0.0);
+    }
+    for (int np = 0; np < n_max; ++np) {
+      const double2 cv = cn[np * k_max + k];
+      double2 gv[3];
+      if constexpr (WITH_G)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) gv[a] = G[a * E + np * k_max + k];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int n = n0 + u;
+        if (n >= n_max) break;
+        const int a_ = n < np ? n : np, b_ = n < np ? np : n;
+        const double sv = (n == np ? 2.0 : sq2) * vu[((a_ * (2 * n_max - a_ + 1)) / 2 + (b_ - a_)) * L1 + l];
+        z[u] = cadd(z[u], cscale(sv, cv));
+        if constexpr (WITH_G)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) zg[u][a] = cadd(zg[u][a], cscale(sv, gv[a]));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int n = n0 + u;
+      if (n >= n_max) break;
+      Z[n * k_max + k] = z[u];
+      if constexpr (WITH_G)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ZG[a * E + n * k_max + k] = zg[u][a];
+    }
+  }
+}
+
 // one weight vector per workgroup: its spread P^T v, Z = S c and (caller's vectors and p) ZG = S G
 __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2* __restrict__ cnk,
                                                     const double* __restrict__ vecs, const uint8_t* __restrict__ skip,
@@ -438,10 +503,11 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
                                                     const int32_t* __restrict__ P_i, const double* __restrict__ P_el,
                                                     double* __restrict__ scratch) {
   extern __shared__ double vunc[];  // n_unc
-  const int i = blockIdx.x, v = blockIdx.y, tx = threadIdx.x;
+  const int tx = threadIdx.x;
+  int i, v;
+  site_and_vector(P.n_vec + 4, i, v);
   const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, L1 = P.l_max + 1;
   const int n_unc = P.n_unc, n_soap = P.n_soap, n_vec = P.n_vec;
-  const double sq2 = 1.4142135623730951;
   double* base = scratch + (size_t)i * P.stride;
   const double2* G = reinterpret_cast<const double2*>(base + P.o_gacc);
   double2* Z = reinterpret_cast<double2*>(base + P.o_z);
@@ -466,25 +532,25 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
       vunc[u] = x;
     }
     __syncthreads();
-    // Z(n, k) = sum_n' S(n, n', l) c(n', k), S = 2 v on the diagonal, sqrt 2 v off it
-    const bool with_g = v <= n_vec;
-    for (int e = tx; e < E; e += HT) {
-      const int n = e / k_max, k = e - n * k_max;
-      int l = 0;
-      while ((l + 1) * (l + 2) / 2 <= k) ++l;
-      double2 z = make_double2(0.0, 0.0), zg[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-      for (int np = 0; np < n_max; ++np) {
-        const int a_ = n < np ? n : np, b_ = n < np ? np : n;
-        const int u = ((a_ * (2 * n_max - a_ + 1)) / 2 + (b_ - a_)) * L1 + l;
-        const double sv = (n == np ? 2.0 : sq2) * vunc[u];
-        z = cadd(z, cscale(sv, cn[np * k_max + k]));
-        if (with_g)
-          for (int a = 0; a < 3; ++a) zg[a] = cadd(zg[a], cscale(sv, G[a * E + np * k_max + k]));
-      }
-      Z[(size_t)v * E + e] = z;
-      if (with_g)
-        for (int a = 0; a < 3; ++a) ZG[((size_t)v * 3 + a) * E + e] = zg[a];
-    }
+    // Z = S c and, for the caller's vectors and p, ZG = S G
+    if (v <= n_vec)
+      spread_rows<true>(n_max, k_max, L1, E, cn, G, vunc, Z + (size_t)v * E, ZG + (size_t)v * 3 * E, tx);
+    else
+      spread_rows<false>(n_max, k_max, L1, E, cn, G, vunc, Z + (size_t)v * E, ZG, tx);
+  }
+}
+
+// dst[0, n) = src[0, n) by the workgroup, eight loads in flight per thread
+__device__ inline void stage_lds(double2* __restrict__ dst, const double2* __restrict__ src, int n, int tx) {
+  constexpr int SU = 8;
+  for (int e0 = tx; e0 < n; e0 += HT * SU) {
+    double2 t[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u)
+      if (e0 + u * HT < n) t[u] = src[e0 + u * HT];
+#pragma unroll
+    for (int u = 0; u < SU; ++u)
+      if (e0 + u * HT < n) dst[e0 + u * HT] = t[u];
   }
 }
 
@@ -589,16 +655,17 @@ __global__ __launch_bounds__(HT) void hess_contract_k(HessParams P, const int* _
                                                      const double* __restrict__ rad_der2,
                                                      const double* __restrict__ scratch, double* __restrict__ res) {
   extern __shared__ double2 zs[];  // Z(v) [n][k], then ZG(v) [a][n][k]
-  const int i = blockIdx.x, v = blockIdx.y, tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
+  const int tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
+  int i, v;
+  site_and_vector(P.n_vec + 4, i, v);
   const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, n_vec = P.n_vec, nvz = n_vec + 4;
   const bool second = v <= n_vec;
   const double fourpi = 4.0 * 3.141592653589793;
   const double* base = scratch + (size_t)i * P.stride;
   const double2* Z = reinterpret_cast<const double2*>(base + P.o_z) + (size_t)v * E;
   const double2* ZG = reinterpret_cast<const double2*>(base + P.o_zg) + (size_t)v * 3 * E;
-  for (int e = tx; e < E; e += HT) zs[e] = Z[e];
-  if (second)
-    for (int e = tx; e < 3 * E; e += HT) zs[E + e] = ZG[e];
+  stage_lds(zs, Z, E, tx);
+  if (second) stage_lds(zs + E, ZG, 3 * E, tx);
   __syncthreads();
   const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
   for (int k2 = k3 + 1 + wv; k2 < k4; k2 += HT / 64) {
@@ -759,7 +826,8 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
                     const double* sdp, const double* vecs, const int4* comp, const uint8_t* skip, const int* out_ptr,
                     const int* out_c, const double* out_el, const int* unc_ptr, const int* unc_e, const int32_t* P_i,
                     const double* P_el, double* scratch, double* heads, double* res, double* hess) {
-  if (l_max > MAXL || n_species > ST_MAX_SPECIES || n_vec < 1) return false;
+  if (l_max > MAXL || n_species > ST_MAX_SPECIES || n_vec < 1 || (long long)n_sites * (n_vec + 4) > 0x7fffffffll)
+    return false;
   HessParams P;
   P.n_sites = n_sites; P.n_pairs = n_pairs; P.n_max = n_max; P.k_max = 1 + l_max * (l_max + 1) / 2 + l_max;
   P.l_max = l_max; P.n_soap = n_soap; P.n_unc = n_unc; P.n_vec = n_vec; P.n_species = n_species;
@@ -791,11 +859,11 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
   const size_t wl = (size_t)n_unc * 8;
   if (wl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_weights_k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)wl);
-  hess_weights_k<<<dim3(n_sites, n_vec + 4), HT, wl, st>>>(P, cnk, vecs, skip, unc_ptr, unc_e, P_i, P_el, scratch);
+  hess_weights_k<<<n_sites * (n_vec + 4), HT, wl, st>>>(P, cnk, vecs, skip, unc_ptr, unc_e, P_i, P_el, scratch);
   const size_t cl = E * 4 * sizeof(double2);
   if (cl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_contract_k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl);
-  hess_contract_k<<<dim3(n_sites, n_vec + 4), HT, cl, st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2, scratch,
+  hess_contract_k<<<n_sites * (n_vec + 4), HT, cl, st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2, scratch,
                                                            res);
   hess_norm_k<<<n_sites, HT, 0, st>>>(P, site_ptr, heads, sdp, scratch, res, hess);
   return true;
