@@ -112,8 +112,8 @@ __device__ void ilexp_dev(int lmax, double xx, double* il) {
 }
 
 // One pair's geometry: pair_geometry, src/soap_turbo.f90:1585-1697. The head (up to Om) is
-// per pair and is written by hess_geom_k once per pair; Omega and its derivatives, per k, are
-// built in LDS by the wavefront that uses them.
+// per pair and is written by hess_geom_k once per pair; pass A builds Omega and its gradient, per
+// k, in LDS after it (the contraction forms its terms from the head directly).
 #define ST_PAIR_HEAD                                                  \
   double ok, invr, nh[3];                                             \
   double Fl[MAXL + 1], Fl1[MAXL + 1], Fl2[MAXL + 1];                  \
@@ -126,7 +126,7 @@ struct PairHead {
 };
 struct PairGeom {
   ST_PAIR_HEAD
-  double2 Om[MAXK], dOm[3 * MAXK], d2Om[9 * MAXK];
+  double2 Om[MAXK], dOm[3 * MAXK];
 };
 #undef ST_PAIR_HEAD
 static_assert(offsetof(PairGeom, Om) == sizeof(PairHead), "the head leads PairGeom");
@@ -221,30 +221,19 @@ __device__ void geom_qt(G& g, const HessParams& P, int k2, int m, const double* 
   }
 }
 
-// Step 2, lanes over k: Omega_k = C_k Q_l^m(t) w^m, its gradient and (want2) its Hessian
-__device__ void geom_omega(PairGeom& g, const HessParams& P, int lane, const double* __restrict__ Ck, bool want2) {
+// Step 2, lanes over k: Omega_k = C_k Q_l^m(t) w^m and its gradient
+__device__ void geom_omega(PairGeom& g, const HessParams& P, int lane, const double* __restrict__ Ck) {
   if (g.ok == 0.0) return;
   for (int k = lane; k < P.k_max; k += 64) {
     int l = 0;
     while ((l + 1) * (l + 2) / 2 <= k) ++l;
     const int mm = k - l * (l + 1) / 2;
-    const double q0 = g.Qt[k], q1 = mm + 1 <= l ? g.Qt[k + 1] : 0.0, q2 = mm + 2 <= l ? g.Qt[k + 2] : 0.0;
+    const double q0 = g.Qt[k], q1 = mm + 1 <= l ? g.Qt[k + 1] : 0.0;
     const double C = Ck[k], dm = (double)mm;
-    const double2 w0 = g.wp[mm + 2], w1 = g.wp[mm + 1], w2 = g.wp[mm];
+    const double2 w0 = g.wp[mm + 2], w1 = g.wp[mm + 1];
     g.Om[k] = cscale(C * q0, w0);
     for (int a = 0; a < 3; ++a)
       g.dOm[3 * k + a] = cscale(C, cadd(cscale(q1 * g.tvec[a], w0), cscale(q0 * dm, cmul(w1, g.wvec[a]))));
-    if (want2)
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-          double2 t = cscale(q2 * g.tvec[a] * g.tvec[b], w0);
-          t = cadd(t, cscale(q1 * g.thes[3 * a + b], w0));
-          double2 tw = cadd(cscale(g.tvec[a], g.wvec[b]), cscale(g.tvec[b], g.wvec[a]));
-          t = cadd(t, cscale(q1 * dm, cmul(w1, tw)));
-          t = cadd(t, cscale(q0 * (double)(mm * (mm - 1)), cmul(w2, cmul(g.wvec[a], g.wvec[b]))));
-          t = cadd(t, cscale(q0 * dm, cmul(w1, g.whes[3 * a + b])));
-          g.d2Om[9 * k + 3 * a + b] = cscale(C, t);
-        }
   }
 }
 
@@ -266,7 +255,7 @@ __global__ __launch_bounds__(128) void hess_geom_k(HessParams P, const double* _
 // output, then Omega by the lanes. Called by every thread of the workgroup, which synchronises
 // around each step.
 __device__ void wave_geometry(PairGeom& g, const HessParams& P, int k2, const double* __restrict__ heads,
-                              const double* __restrict__ Ck, bool want2) {
+                              const double* __restrict__ Ck) {
   const int lane = threadIdx.x & 63;
   __syncthreads();  // every wavefront is done with the previous round's geometry
   double* dst = reinterpret_cast<double*>(&g);
@@ -276,7 +265,7 @@ __device__ void wave_geometry(PairGeom& g, const HessParams& P, int k2, const do
     for (int q = lane; q < GEOM_HEAD; q += 64) dst[q] = heads[(size_t)k2 * GEOM_HEAD + q];
   }
   __syncthreads();
-  geom_omega(g, P, lane, Ck, want2);
+  geom_omega(g, P, lane, Ck);
   __syncthreads();
 }
 
@@ -325,7 +314,7 @@ __global__ __launch_bounds__(HT) void hess_gacc_k(HessParams P, const int* __res
   const int k3 = site_ptr[i], k4 = site_ptr[i + 1];
   for (int r0 = k3 + 1; r0 < k4; r0 += HT / 64) {
     const int k2 = r0 + wv < k4 ? r0 + wv : -1;
-    wave_geometry(gs[wv], P, k2, heads, Ck, false);
+    wave_geometry(gs[wv], P, k2, heads, Ck);
 #pragma unroll
     for (int u = 0; u < (EPT > 0 ? EPT : 1 << 20); ++u) {
       const int e = tx + HT * u;
