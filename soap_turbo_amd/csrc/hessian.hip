@@ -533,10 +533,9 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
 __device__ inline void stage_lds(double2* __restrict__ dst, const double2* __restrict__ src, int n, int tx) {
   constexpr int SU = 8;
   for (int e0 = tx; e0 < n; e0 += HT * SU) {
-    double2 t[SU];
+    double2 t[SU];  // every element loaded (index clamped into range), so t stays in registers
 #pragma unroll
-    for (int u = 0; u < SU; ++u)
-      if (e0 + u * HT < n) t[u] = src[e0 + u * HT];
+    for (int u = 0; u < SU; ++u) t[u] = src[e0 + u * HT < n ? e0 + u * HT : n - 1];
 #pragma unroll
     for (int u = 0; u < SU; ++u)
       if (e0 + u * HT < n) dst[e0 + u * HT] = t[u];

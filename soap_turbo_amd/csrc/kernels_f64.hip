@@ -20,6 +20,24 @@
 
 namespace st {
 
+// One of two or four arrays by a lane-dependent index, chosen on the bits of the addresses: a
+// select among the pointers themselves is compiled to a table in private memory, read per lane.
+template <class T>
+__device__ inline const T* pick2(int which, const T* p0, const T* p1) {
+  const unsigned long long m = which != 0 ? ~0ull : 0ull;
+  return reinterpret_cast<const T*>((reinterpret_cast<unsigned long long>(p0) & ~m) |
+                                    (reinterpret_cast<unsigned long long>(p1) & m));
+}
+template <class T>
+__device__ inline const T* pick4(int which, const T* p0, const T* p1, const T* p2, const T* p3) {
+  const unsigned long long m0 = which == 0 ? ~0ull : 0ull, m1 = which == 1 ? ~0ull : 0ull;
+  const unsigned long long m2 = which == 2 ? ~0ull : 0ull, m3 = which == 3 ? ~0ull : 0ull;
+  return reinterpret_cast<const T*>((reinterpret_cast<unsigned long long>(p0) & m0) |
+                                    (reinterpret_cast<unsigned long long>(p1) & m1) |
+                                    (reinterpret_cast<unsigned long long>(p2) & m2) |
+                                    (reinterpret_cast<unsigned long long>(p3) & m3));
+}
+
 __device__ __forceinline__ int site_of(const int* site_ptr, int n_sites, int p) {
   int lo = 0, hi = n_sites;  // largest i with site_ptr[i] <= p
   while (hi - lo > 1) {
@@ -949,7 +967,7 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
       fa[it] = make_double2(0.0, 0.0);
       if (t < nA) {
         const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
-        const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
+        const double2* src = pick4(kind, ang, ang_r, ang_a, ang_p);
         if (b < nb) fa[it] = src[(size_t)(pb + b) * k_max + k];
       }
     }
@@ -959,7 +977,7 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
       fr[it] = 0.0;
       if (t < nR) {
         const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
-        if (b < nb) fr[it] = (d == 0 ? rad : rad_der)[(size_t)(pb + b) * n_max + n];
+        if (b < nb) fr[it] = pick2(d, rad, rad_der)[(size_t)(pb + b) * n_max + n];
       }
     }
     fo = 0.0;
@@ -979,7 +997,7 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     }
     for (int t = threadIdx.x + FA * NT; t < nA; t += NT) {
       const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
-      const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
+      const double2* src = pick4(kind, ang, ang_r, ang_a, ang_p);
       sA[k * AS + 4 * b + kind] = b < nb ? src[(size_t)(pb + b) * k_max + k] : make_double2(0.0, 0.0);
     }
 #pragma unroll
@@ -992,7 +1010,7 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     }
     for (int t = threadIdx.x + FR * NT; t < nR; t += NT) {
       const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
-      sR[n * RS + 2 * b + d] = b < nb ? (d == 0 ? rad : rad_der)[(size_t)(pb + b) * n_max + n] : 0.0;
+      sR[n * RS + 2 * b + d] = b < nb ? pick2(d, rad, rad_der)[(size_t)(pb + b) * n_max + n] : 0.0;
     }
     if (threadIdx.x < PB * 8) rot[buf][threadIdx.x / 8][threadIdx.x % 8] = fo;
   };
@@ -1454,10 +1472,11 @@ __global__ __launch_bounds__(256, 2) void jacobian_wave_k(
   for (int p = p0 + 1 + w; p < p1; p += 4) {
     for (int t = lane; t < 4 * k_max; t += 64) {
       const int kind = t / k_max, k = t - kind * k_max;
-      const double2* src = kind == 0 ? ang : kind == 1 ? ang_r : kind == 2 ? ang_a : ang_p;
+      const double2* src = pick4(kind, ang, ang_r, ang_a, ang_p);
       sA[t] = src[(size_t)p * k_max + k];
     }
-    for (int t = lane; t < 2 * n_max; t += 64) sR[t] = (t < n_max ? rad : rad_der)[(size_t)p * n_max + t % n_max];
+    for (int t = lane; t < 2 * n_max; t += 64)
+      sR[t] = pick2(t < n_max ? 0 : 1, rad, rad_der)[(size_t)p * n_max + t % n_max];
     if (lane < 8) srot[lane] = rotp[(size_t)p * 8 + lane];
     wave_sync();
     // P, Q, Qa, Qo(n, l) = sum over m <= l of w(m) Re(conj(A_kind(l, m)) c(n, l, m))
