@@ -76,6 +76,19 @@ __device__ inline void site_and_vector(int nv, int& i, int& v) {
   v = w - i * nv;
 }
 
+// dst[0, n) = src[0, n) by the workgroup, eight loads in flight per thread
+__device__ inline void stage_lds(double2* __restrict__ dst, const double2* __restrict__ src, int n, int tx) {
+  constexpr int SU = 8;
+  for (int e0 = tx; e0 < n; e0 += HT * SU) {
+    double2 t[SU];  // every element loaded (index clamped into range), so t stays in registers
+#pragma unroll
+    for (int u = 0; u < SU; ++u) t[u] = src[e0 + u * HT < n ? e0 + u * HT : n - 1];
+#pragma unroll
+    for (int u = 0; u < SU; ++u)
+      if (e0 + u * HT < n) dst[e0 + u * HT] = t[u];
+  }
+}
+
 // get_ilexp, src/soap_turbo_angular.f90:361-422, to lmax (<= MAXL + 2)
 __device__ void ilexp_dev(int lmax, double xx, double* il) {
   const double xcut = 1e-7;
@@ -363,17 +376,22 @@ __global__ __launch_bounds__(HT) void hess_d1_k(HessParams P, const double2* __r
                                                     const int32_t* __restrict__ P_i, const double* __restrict__ P_el,
                                                     double* __restrict__ scratch) {
   __shared__ double red[4 * 8];
+  extern __shared__ double2 cg[];  // the site's c [n][k], then G [a][n][k]
   const int i = blockIdx.x, tx = threadIdx.x;
   const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max;
   const int n_unc = P.n_unc, n_soap = P.n_soap, n_vec = P.n_vec;
   const double sq2 = 1.4142135623730951;
   double* base = scratch + (size_t)i * P.stride;
-  const double2* G = reinterpret_cast<const double2*>(base + P.o_gacc);
   double* D1u = base + P.o_d1u;
   double* D1c = base + P.o_d1c;
   double* pvec = base + P.o_pvec;
   double* scal = base + P.o_scal;
-  const double2* cn = cnk + (size_t)i * E;
+  // every element of c and G is read for each of the n_max components it pairs with: staged once
+  stage_lds(cg, cnk + (size_t)i * E, E, tx);
+  stage_lds(cg + E, reinterpret_cast<const double2*>(base + P.o_gacc), 3 * E, tx);
+  __syncthreads();
+  const double2* cn = cg;
+  const double2* G = cg + E;
   // D1(n, n', l, a) = -base sum_m w(m) Re(G_n,a conj(c_n') + c_n conj(G_n',a)), skipped components 0
   for (int u = tx; u < n_unc; u += HT) {
     const int4 cm = comp[u];
@@ -526,19 +544,6 @@ __global__ __launch_bounds__(HT) void hess_weights_k(HessParams P, const double2
       spread_rows<true>(n_max, k_max, L1, E, cn, G, vunc, Z + (size_t)v * E, ZG + (size_t)v * 3 * E, tx);
     else
       spread_rows<false>(n_max, k_max, L1, E, cn, G, vunc, Z + (size_t)v * E, ZG, tx);
-  }
-}
-
-// dst[0, n) = src[0, n) by the workgroup, eight loads in flight per thread
-__device__ inline void stage_lds(double2* __restrict__ dst, const double2* __restrict__ src, int n, int tx) {
-  constexpr int SU = 8;
-  for (int e0 = tx; e0 < n; e0 += HT * SU) {
-    double2 t[SU];  // every element loaded (index clamped into range), so t stays in registers
-#pragma unroll
-    for (int u = 0; u < SU; ++u) t[u] = src[e0 + u * HT < n ? e0 + u * HT : n - 1];
-#pragma unroll
-    for (int u = 0; u < SU; ++u)
-      if (e0 + u * HT < n) dst[e0 + u * HT] = t[u];
   }
 }
 
@@ -800,7 +805,8 @@ size_t hessian_heads_per_pair() { return GEOM_HEAD; }
 size_t hessian_lds(int n_max, int k_max, int n_unc) {
   const size_t geom = (HT / 64) * sizeof(PairGeom);        // pass A
   const size_t w = (size_t)n_unc * 8 + 32 * 8;              // weight vectors
-  const size_t c = (size_t)n_max * k_max * 4 * sizeof(double2);  // contraction: Z and ZG of one vector
+  // contraction: Z and ZG of one vector; central-atom derivative: c and G, and its reduction array
+  const size_t c = (size_t)n_max * k_max * 4 * sizeof(double2) + 32 * 8;
   return std::max(geom, std::max(w, c));
 }
 
@@ -842,15 +848,19 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
 #undef GA
     default: hess_gacc_k<0><<<n_sites, HT, 0, st>>>(P, site_ptr, heads, Ck, rad, rad_der, scratch);
   }
-  hess_d1_k<<<n_sites, HT, 0, st>>>(P, cnk, soap, sdp, vecs, comp, skip, out_ptr, out_c, out_el, unc_ptr, unc_e,
-                                     P_i, P_el, scratch);
+  const size_t cl = E * 4 * sizeof(double2);  // c and G (hess_d1_k); Z and ZG of one vector (hess_contract_k)
+  if (cl > 64 * 1024) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_d1_k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)cl);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_contract_k),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl);
+  }
+  hess_d1_k<<<n_sites, HT, cl, st>>>(P, cnk, soap, sdp, vecs, comp, skip, out_ptr, out_c, out_el, unc_ptr, unc_e, P_i,
+                                      P_el, scratch);
   const size_t wl = (size_t)n_unc * 8;
   if (wl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_weights_k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)wl);
   hess_weights_k<<<n_sites * (n_vec + 4), HT, wl, st>>>(P, cnk, vecs, skip, unc_ptr, unc_e, P_i, P_el, scratch);
-  const size_t cl = E * 4 * sizeof(double2);
-  if (cl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_contract_k),
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl);
   hess_contract_k<<<n_sites * (n_vec + 4), HT, cl, st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2, scratch,
                                                            res);
   hess_norm_k<<<n_sites, HT, 0, st>>>(P, site_ptr, heads, sdp, scratch, res, hess);
