@@ -16,7 +16,8 @@
 //                    the three columns of D1, each spread to its uncompressed S matrix, Z = S c and,
 //                    for the caller's vectors and p, ZG = S G;
 //   hess_contract_k  one workgroup per (site, weight vector), Z and ZG in LDS: per pair, Omega =
-//                    C_k Q_l^m(t) w^m with its Cartesian gradient and Hessian, contracted (pass B);
+//                    C_k Q_l^m(t) w^m with its Cartesian gradient and Hessian, contracted (pass B;
+//                    one launch for the columns of D1, one for the caller's vectors and p);
 //   hess_norm_k      one workgroup per site: the quotient rule of the normalisation, and the self
 //                    pair's blocks.
 // Dot products of the per-pair derivative with a vector (p.dp2, w.dp2, D1.dp2) come out of the same
@@ -642,6 +643,10 @@ __device__ inline double wave_sum12(const double (&acc)[12], int lane) {
 // vectors and p, ZG(v)) staged in LDS once; its wavefronts walk the site's pairs, lanes over k.
 // Per pair and v it writes 12 sums: v.dp/du_jb (3) and, with second derivatives, v.d2p/dr_ia
 // du_jb before the normalisation (9).
+// SECOND: the caller's vectors and p (second derivatives, Z and ZG staged); otherwise the three
+// columns of D1 (first derivatives only, Z staged), a launch of its own with a quarter of the LDS
+// and fewer registers
+template <bool SECOND>
 __global__ __launch_bounds__(HT) void hess_contract_k(HessParams P, const int* __restrict__ site_ptr,
                                                      const double* __restrict__ heads, const double* __restrict__ Ck,
                                                      const double* __restrict__ rad, const double* __restrict__ rad_der,
@@ -650,9 +655,10 @@ __global__ __launch_bounds__(HT) void hess_contract_k(HessParams P, const int* _
   extern __shared__ double2 zs[];  // Z(v) [n][k], then ZG(v) [a][n][k]
   const int tx = threadIdx.x, lane = tx & 63, wv = tx >> 6;
   int i, v;
-  site_and_vector(P.n_vec + 4, i, v);
+  site_and_vector(SECOND ? P.n_vec + 1 : 3, i, v);
+  if (!SECOND) v += P.n_vec + 1;
   const int n_max = P.n_max, k_max = P.k_max, E = n_max * k_max, n_vec = P.n_vec, nvz = n_vec + 4;
-  const bool second = v <= n_vec;
+  constexpr bool second = SECOND;
   const double fourpi = 4.0 * 3.141592653589793;
   const double* base = scratch + (size_t)i * P.stride;
   const double2* Z = reinterpret_cast<const double2*>(base + P.o_z) + (size_t)v * E;
@@ -852,7 +858,7 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
   if (cl > 64 * 1024) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_d1_k), hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)cl);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_contract_k),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_contract_k<true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)cl);
   }
   hess_d1_k<<<n_sites, HT, cl, st>>>(P, cnk, soap, sdp, vecs, comp, skip, out_ptr, out_c, out_el, unc_ptr, unc_e, P_i,
@@ -861,7 +867,9 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
   if (wl > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&hess_weights_k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)wl);
   hess_weights_k<<<n_sites * (n_vec + 4), HT, wl, st>>>(P, cnk, vecs, skip, unc_ptr, unc_e, P_i, P_el, scratch);
-  hess_contract_k<<<n_sites * (n_vec + 4), HT, cl, st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2, scratch,
+  hess_contract_k<false><<<n_sites * 3, HT, E * sizeof(double2), st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2,
+                                                                       scratch, res);
+  hess_contract_k<true><<<n_sites * (n_vec + 1), HT, cl, st>>>(P, site_ptr, heads, Ck, rad, rad_der, rad_der2, scratch,
                                                            res);
   hess_norm_k<<<n_sites, HT, 0, st>>>(P, site_ptr, heads, sdp, scratch, res, hess);
   return true;
