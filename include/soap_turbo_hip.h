@@ -152,6 +152,16 @@ int st_upload_dedq(st_ctx* ctx, int32_t n_soap, const double* G);
 int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t do_derivatives, int32_t contract);
 /* FP64 pair gradients from the resident soap_cart_der and dE/dq (what a full-Jacobian caller does). */
 int st_pairs_full_jacobian(st_ctx* ctx);
+/* The FP64 full-Jacobian path in batches of sites: for each batch of consecutive sites, get_soap with
+ * soap_cart_der on that batch, then its contraction with dE/dq to pair gradients; forces and virial at
+ * the end (st_download_pairs / st_download_forces with which = 0). This is what the reference's callers
+ * do when soap_cart_der of all sites exceeds memory: they call get_soap once per batch of sites, its
+ * work buffers sized for one batch (src/soap_turbo.f90:43-57). max_pairs bounds the pairs of a batch
+ * (0: as many as the free device memory holds); batches are as even as that bound allows.
+ * *n_batches (may be NULL) gets the batch count. The normalised soap of every site is resident
+ * afterwards (st_download); soap_cart_der is downloadable only when one batch held every pair. */
+int st_full_jacobian_forces(st_ctx* ctx, const st_params* p, const st_compression* c, int64_t max_pairs,
+                            int32_t* n_batches);
 /* FP64 soap_contract on the resident state and dE/dq (normalised soap from st_compute). */
 int st_contract_resident(st_ctx* ctx, double sgn);
 /* FP32 adjoint: descriptors, pair gradients, forces and virial from the resident data and dE/dq. */

@@ -181,6 +181,13 @@ class Context:
     def pairs_full_jacobian(self):
         self.check(lib().st_pairs_full_jacobian(self.h))
 
+    def full_jacobian_forces(self, params, compression, max_pairs=0):
+        """The FP64 full-Jacobian path in batches of sites (st_full_jacobian_forces); the batch count."""
+        nb = ctypes.c_int32(0)
+        self.check(lib().st_full_jacobian_forces(self.h, ctypes.byref(params), ctypes.byref(compression),
+                                                 int(max_pairs), ctypes.byref(nb)))
+        return nb.value
+
     def contract_resident(self, sgn=1.0):
         self.check(lib().st_contract_resident(self.h, sgn))
 

@@ -55,6 +55,7 @@ SIGNATURES = {
     "st_upload_dedq": (ctypes.c_int, [P, I32, P]),
     "st_compute": (ctypes.c_int, [P, P, P, I32, I32]),
     "st_pairs_full_jacobian": (ctypes.c_int, [P]),
+    "st_full_jacobian_forces": (ctypes.c_int, [P, P, P, ctypes.c_int64, ctypes.POINTER(I32)]),
     "st_contract_resident": (ctypes.c_int, [P, ctypes.c_double]),
     "st_adjoint_f32": (ctypes.c_int, [P, P, P]),
     "st_set_graphs": (ctypes.c_int, [P, I32]),

@@ -70,7 +70,8 @@ def build_hip(verbose=False):
 
 def build_workload():
     out = os.path.join(HERE, "_workload.so")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", os.path.join(CSRC, "workload.cpp"), "-o", out + ".tmp"])
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", os.path.join(CSRC, "workload.cpp"), "-o",
+          out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

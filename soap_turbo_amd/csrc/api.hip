@@ -66,6 +66,7 @@ struct st_ctx {
 
   // FP64 work and outputs
   DevBuf rad, rad_der, ang, ang_r, ang_a, ang_p, cnk, soap, sdp, cart, pair64, preflm, central, rotp;
+  DevBuf batch_ptr;  // per-batch site offsets of st_full_jacobian_forces
   DevBuf forces64, vpart64, virial64;
   bool have_forces64 = false;
   DevBuf comp, out_ptr, out_c, out_el, skip, unc_ptr, unc_e, P_i, P_el;
@@ -724,36 +725,66 @@ int st_upload_dedq(st_ctx* ctx, int32_t n_soap, const double* G) {
   return ST_OK;
 }
 
-int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t do_derivatives, int32_t contract) {
-  if (!ctx) return ST_ERR_ARG;
-  if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
-  int rc = check_params(ctx, p);
-  if (rc) return rc;
-  HIPCHK(hipSetDevice(ctx->device));
-  if ((rc = basis_for(ctx, p))) return rc;
-  if ((rc = build_tables(ctx, p, c))) return rc;
-  const bool der = do_derivatives != 0, ctr = der && contract != 0;
-  const int n_max = ctx->n_max, k_max = ctx->k_max, L = ctx->l_max, n_soap = ctx->n_soap;
-  const int NS = ctx->n_sites;
-  const size_t P = ctx->n_pairs;
-  if ((rc = pack_mask(ctx, p->n_species))) return rc;
-  std::vector<double> cen;
-  build_central(ctx, p, cen);
-  if ((rc = upload(ctx, ctx->central, cen.data(), cen.size() * 8))) return rc;
-  if ((rc = ensure(ctx, ctx->rad, P * n_max * 8))) return rc;
-  if ((rc = ensure(ctx, ctx->rad_der, P * n_max * 8))) return rc;
-  if ((rc = ensure(ctx, ctx->ang, P * k_max * 16))) return rc;
+}  // extern "C"
+
+namespace {
+
+// One range of sites [s0, s1) of the resident list, whose pairs are [p0, p1). site_ptr holds the
+// range's own offsets (site_ptr[0] = 0), so every per-pair array is addressed from p0 and every
+// per-site array from s0; the whole list is the range (0, n_sites) with the resident offsets.
+struct SiteRange {
+  int s0, s1;
+  size_t p0, p1;
+  const int* site_ptr;
+};
+
+// Per-pair and per-site device bytes of the FP64 work arrays (radial values and derivatives, the four
+// angular arrays, the rotation coefficients, soap_cart_der; cnk per site).
+size_t fp64_pair_bytes(int n_max, int k_max, int n_soap) {
+  return (size_t)16 * n_max + (size_t)64 * k_max + 64 + (size_t)24 * n_soap;
+}
+size_t fp64_site_bytes(int n_max, int k_max) { return (size_t)16 * n_max * k_max; }
+
+// Grows the FP64 work arrays to hold ranges of up to np pairs and ns sites.
+int fp64_work(st_ctx* ctx, bool der, bool ctr, size_t np, size_t ns) {
+  const int n_max = ctx->n_max, k_max = ctx->k_max, n_soap = ctx->n_soap;
+  int rc;
+  if ((rc = ensure(ctx, ctx->rad, np * n_max * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->rad_der, np * n_max * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->ang, np * k_max * 16))) return rc;
   if (der) {
-    if ((rc = ensure(ctx, ctx->ang_r, P * k_max * 16))) return rc;
-    if ((rc = ensure(ctx, ctx->ang_a, P * k_max * 16))) return rc;
-    if ((rc = ensure(ctx, ctx->ang_p, P * k_max * 16))) return rc;
+    if ((rc = ensure(ctx, ctx->ang_r, np * k_max * 16))) return rc;
+    if ((rc = ensure(ctx, ctx->ang_a, np * k_max * 16))) return rc;
+    if ((rc = ensure(ctx, ctx->ang_p, np * k_max * 16))) return rc;
   }
-  if ((rc = ensure(ctx, ctx->cnk, (size_t)NS * n_max * k_max * 16))) return rc;
-  if ((rc = ensure(ctx, ctx->soap, (size_t)NS * n_soap * 8))) return rc;
-  if ((rc = ensure(ctx, ctx->sdp, (size_t)NS * 8))) return rc;
-  if (der && !ctr && (rc = ensure(ctx, ctx->cart, P * n_soap * 24))) return rc;
-  if (der && !ctr && (rc = ensure(ctx, ctx->rotp, P * 64))) return rc;
+  if ((rc = ensure(ctx, ctx->cnk, ns * n_max * k_max * 16))) return rc;
+  if (der && !ctr) {
+    if ((rc = ensure(ctx, ctx->cart, np * n_soap * 24))) {
+      ctx->err += " (soap_cart_der of " + std::to_string(np) + " pairs; st_full_jacobian_forces runs the "
+                  "Jacobian in batches of sites that fit)";
+      return rc;
+    }
+    if ((rc = ensure(ctx, ctx->rotp, np * 64))) return rc;
+  }
+  return ST_OK;
+}
+
+// get_soap's FP64 pass over one range of sites (src/soap_turbo.f90:489-909): radial and angular
+// expansions of the range's pairs, cnk, the power spectrum, with der the Jacobian into ctx->cart
+// (range-local rows), then the normalisation. Basis, tables, masks and the central term must be set.
+int fp64_range(st_ctx* ctx, const st_params* p, bool der, bool ctr, const SiteRange& r) {
+  const int n_max = ctx->n_max, k_max = ctx->k_max, L = ctx->l_max, n_soap = ctx->n_soap;
+  const int NS = r.s1 - r.s0;
+  const size_t P = r.p1 - r.p0;
+  if (NS == 0) return ST_OK;
   hipStream_t st = ctx->stream;
+  const double* rjs = ctx->rjs.as<double>() + r.p0;
+  const double* th = ctx->thetas.as<double>() + r.p0;
+  const double* ph = ctx->phis.as<double>() + r.p0;
+  const uint32_t* spmask = ctx->spmask.as<uint32_t>() + r.p0;
+  const uint8_t* first = ctx->first.as<uint8_t>() + r.p0;
+  double* soap = ctx->soap.as<double>() + (size_t)r.s0 * n_soap;
+  double* sdp = ctx->sdp.as<double>() + r.s0;
 
   timed(ctx, "fill_rad", [&] {
     st::launch_fill(st, ctx->rad.as<double>(), P * n_max, 0.0);
@@ -764,8 +795,7 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
     st::RadialConsts rcst = radial_consts(p, s, ctx->W, n_max, off, der);
     bool ok = true;
     timed(ctx, "radial_f64", [&] {
-      ok = st::launch_radial(st, rcst, (int)P, ctx->rjs.as<double>(), ctx->spmask.as<uint32_t>(),
-                             ctx->first.as<uint8_t>(), ctx->rad.as<double>(), ctx->rad_der.as<double>());
+      ok = st::launch_radial(st, rcst, (int)P, rjs, spmask, first, ctx->rad.as<double>(), ctx->rad_der.as<double>());
     });
     if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "alpha_max out of the compiled range");
     off += p->alpha_max[s];
@@ -784,38 +814,33 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
   ac.rcut2 = ac.rcut_max * ac.rcut_max;
   bool ok = true;
   timed(ctx, "angular_f64", [&] {
-    ok = st::launch_angular(st, ac, (int)P, ctx->rjs.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
-                            ctx->spmask.as<uint32_t>(), ctx->preflm.as<double>(), ctx->ang.as<double2>(),
+    ok = st::launch_angular(st, ac, (int)P, rjs, th, ph, spmask, ctx->preflm.as<double>(), ctx->ang.as<double2>(),
                             der ? ctx->ang_r.as<double2>() : nullptr, der ? ctx->ang_a.as<double2>() : nullptr,
                             der ? ctx->ang_p.as<double2>() : nullptr);
   });
   if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "l_max out of the compiled range");
   timed(ctx, "cnk_f64", [&] {
-    st::launch_cnk(st, NS, n_max, k_max, ctx->site_ptr.as<int>(), ctx->rad.as<double>(), ctx->ang.as<double2>(),
-                   ctx->species.as<int32_t>(), ctx->max_mult, ctx->mult.as<int32_t>(),
+    st::launch_cnk(st, NS, n_max, k_max, r.site_ptr, ctx->rad.as<double>(), ctx->ang.as<double2>(),
+                   ctx->species.as<int32_t>() + (size_t)r.s0 * ctx->max_mult, ctx->max_mult,
+                   ctx->mult.as<int32_t>() + r.s0,
                    p->basis == ST_BASIS_POLY3GAUSS ? ctx->central.as<double>() : nullptr, ctx->cnk.as<double2>());
   });
   const int* optr = ctx->compressed ? ctx->out_ptr.as<int>() : nullptr;
-  if (sizeof(double2) * n_max * k_max > 64 * 1024)
-    return fail(ctx, ST_ERR_UNSUPPORTED, "n_max * k_max too large for the spectrum kernel");
   timed(ctx, "power_f64", [&] {
     st::launch_power(st, NS, n_max, k_max, n_soap, ctx->comp.as<int4>(), optr, ctx->out_c.as<int>(),
                      ctx->out_el.as<double>(), ctx->single ? ctx->ocomp.as<int4>() : nullptr, ctx->oel.as<double>(),
-                     ctx->cnk.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>());
+                     ctx->cnk.as<double2>(), soap, sdp);
   });
   if (der && !ctr) {
-    if (st::derivatives_lds(n_max, L, k_max, n_soap) > ctx->lds_max)
-      return fail(ctx, ST_ERR_UNSUPPORTED, "the derivative working set exceeds the LDS");
     auto jacobian = [&](int variant) {
       if (st::derivatives_lds(n_max, L, k_max, n_soap, variant) > ctx->lds_max) return false;
-      return st::launch_derivatives(st, NS, (int)P, n_max, L, k_max, n_soap, ctx->site_ptr.as<int>(),
-                                    ctx->comp.as<int4>(), optr, ctx->out_c.as<int>(), ctx->out_el.as<double>(),
+      return st::launch_derivatives(st, NS, (int)P, n_max, L, k_max, n_soap, r.site_ptr, ctx->comp.as<int4>(), optr,
+                                    ctx->out_c.as<int>(), ctx->out_el.as<double>(),
                                     ctx->single ? ctx->ocomp.as<int4>() : nullptr, ctx->oel.as<double>(),
                                     ctx->cnk.as<double2>(), ctx->rad.as<double>(), ctx->rad_der.as<double>(),
                                     ctx->ang.as<double2>(), ctx->ang_r.as<double2>(), ctx->ang_a.as<double2>(),
-                                    ctx->ang_p.as<double2>(), ctx->soap.as<double>(), ctx->sdp.as<double>(),
-                                    ctx->rjs.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
-                                    ctx->rotp.as<double>(), ctx->cart.as<double>(), variant);
+                                    ctx->ang_p.as<double2>(), soap, sdp, rjs, th, ph, ctx->rotp.as<double>(),
+                                    ctx->cart.as<double>(), variant);
     };
     int variant = 0;
     if (ctx->jacobian_variant >= 0) {
@@ -832,13 +857,133 @@ int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t
     timed(ctx, "derivatives_f64", [&] { ok = jacobian(variant); });
     if (!ok) return fail(ctx, ST_ERR_UNSUPPORTED, "n_soap > 3072 is not supported with derivatives");
   }
-  timed(ctx, "normalise_f64", [&] { st::launch_normalise(st, NS, n_soap, ctx->sdp.as<double>(), ctx->soap.as<double>()); });
+  timed(ctx, "normalise_f64", [&] { st::launch_normalise(st, NS, n_soap, sdp, soap); });
+  return ST_OK;
+}
+
+// Set-up shared by the FP64 entry points: parameters, basis, index tables, packed masks, central term,
+// the per-site outputs, and the LDS checks of the spectrum and Jacobian kernels.
+int fp64_setup(st_ctx* ctx, const st_params* p, const st_compression* c, bool der, bool ctr) {
+  if (!ctx->have_nb) return fail(ctx, ST_ERR_STATE, "upload the neighbour list first");
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  if ((rc = basis_for(ctx, p))) return rc;
+  if ((rc = build_tables(ctx, p, c))) return rc;
+  if ((rc = pack_mask(ctx, p->n_species))) return rc;
+  std::vector<double> cen;
+  build_central(ctx, p, cen);
+  if ((rc = upload(ctx, ctx->central, cen.data(), cen.size() * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->soap, (size_t)ctx->n_sites * ctx->n_soap * 8))) return rc;
+  if ((rc = ensure(ctx, ctx->sdp, (size_t)ctx->n_sites * 8))) return rc;
+  if (sizeof(double2) * ctx->n_max * ctx->k_max > 64 * 1024)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "n_max * k_max too large for the spectrum kernel");
+  if (der && !ctr && st::derivatives_lds(ctx->n_max, ctx->l_max, ctx->k_max, ctx->n_soap) > ctx->lds_max)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "the derivative working set exceeds the LDS");
+  return ST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t do_derivatives, int32_t contract) {
+  if (!ctx) return ST_ERR_ARG;
+  const bool der = do_derivatives != 0, ctr = der && contract != 0;
+  int rc = fp64_setup(ctx, p, c, der, ctr);
+  if (rc) return rc;
+  if ((rc = fp64_work(ctx, der, ctr, ctx->n_pairs, ctx->n_sites))) return rc;
+  const SiteRange all{0, ctx->n_sites, 0, (size_t)ctx->n_pairs, ctx->site_ptr.as<int>()};
+  if ((rc = fp64_range(ctx, p, der, ctr, all))) return rc;
   HIPCHK(hipGetLastError());
   ctx->have_soap = true;
   ctx->have_cart = der && !ctr;
   ctx->ct_ready = ctr;
   ctx->have_pair64 = false;
   return ST_OK;
+}
+
+int st_full_jacobian_forces(st_ctx* ctx, const st_params* p, const st_compression* c, int64_t max_pairs,
+                            int32_t* n_batches) {
+  if (!ctx) return ST_ERR_ARG;
+  if (max_pairs < 0) return fail(ctx, ST_ERR_ARG, "max_pairs must be >= 0");
+  int rc = fp64_setup(ctx, p, c, true, false);
+  if (rc) return rc;
+  if (!ctx->have_G || ctx->G_n_soap != ctx->n_soap) return fail(ctx, ST_ERR_STATE, "dE/dq missing or of another size");
+  const int NS = ctx->n_sites;
+  const std::vector<int>& ptr = ctx->site_ptr_h;
+  const size_t pb = fp64_pair_bytes(ctx->n_max, ctx->k_max, ctx->n_soap);
+  const size_t sb = fp64_site_bytes(ctx->n_max, ctx->k_max);
+  int maxn = 0;
+  for (int i = 0; i < NS; ++i) maxn = std::max(maxn, ptr[i + 1] - ptr[i]);
+  size_t avail = 0;
+  if (max_pairs == 0) {
+    // automatic: what the device has free, plus what the work arrays already hold, less a margin
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    size_t held = 0;
+    for (DevBuf* b : {&ctx->rad, &ctx->rad_der, &ctx->ang, &ctx->ang_r, &ctx->ang_a, &ctx->ang_p, &ctx->cnk,
+                      &ctx->cart, &ctx->rotp, &ctx->pair64})
+      held += b->bytes;
+    const size_t margin = ((size_t)2 << 30) + (ctx->pair64.bytes >= (size_t)ctx->n_pairs * 24 ? 0 : (size_t)ctx->n_pairs * 24);
+    avail = fr + held > margin ? fr + held - margin : 0;
+  }
+  size_t cap = max_pairs > 0 ? (size_t)max_pairs : avail / pb;
+  // the batches: consecutive sites, as even in pairs as the cap allows; with an automatic cap, shrunk
+  // until the largest batch's pairs and sites fit
+  const size_t P = ctx->n_pairs;
+  std::vector<int> cut;
+  size_t big_p = 0, big_s = 0;
+  for (;;) {
+    if (cap < (size_t)maxn)
+      return fail(ctx, ST_ERR_ARG, "a batch must hold the largest site's pairs (" + std::to_string(maxn) +
+                                       (max_pairs > 0 ? ")" : "), more than the free device memory holds"));
+    const size_t nb_min = std::max<size_t>(1, (P + cap - 1) / cap);
+    const size_t target = (P + nb_min - 1) / nb_min;
+    cut.assign(1, 0);
+    for (int i = 0; i < NS; ++i) {
+      if ((size_t)ptr[i + 1] - ptr[cut.back()] > cap) cut.push_back(i);                    // site i opens a batch
+      if ((size_t)ptr[i + 1] - ptr[cut.back()] >= target && i + 1 < NS) cut.push_back(i + 1);  // the batch is full
+    }
+    if (cut.back() != NS) cut.push_back(NS);
+    big_p = big_s = 0;
+    for (size_t b = 0; b + 1 < cut.size(); ++b) {
+      big_p = std::max(big_p, (size_t)ptr[cut[b + 1]] - ptr[cut[b]]);
+      big_s = std::max(big_s, (size_t)(cut[b + 1] - cut[b]));
+    }
+    if (max_pairs > 0 || big_p * pb + big_s * sb <= avail) break;
+    cap = cap - cap / 8;
+  }
+  const int nbat = (int)cut.size() - 1;
+  if ((rc = fp64_work(ctx, true, false, big_p, big_s))) return rc;
+  if ((rc = ensure(ctx, ctx->pair64, P * 24))) return rc;
+  // the batches' own site offsets, one run of (sites + 1) entries per batch
+  std::vector<int> bptr;
+  std::vector<size_t> bofs;
+  if (nbat > 1) {
+    for (int b = 0; b < nbat; ++b) {
+      bofs.push_back(bptr.size());
+      for (int i = cut[b]; i <= cut[b + 1]; ++i) bptr.push_back(ptr[i] - ptr[cut[b]]);
+    }
+    if ((rc = upload(ctx, ctx->batch_ptr, bptr.data(), bptr.size() * 4))) return rc;
+  }
+  for (int b = 0; b < nbat; ++b) {
+    const SiteRange r{cut[b], cut[b + 1], (size_t)ptr[cut[b]], (size_t)ptr[cut[b + 1]],
+                      nbat > 1 ? ctx->batch_ptr.as<int>() + bofs[b] : ctx->site_ptr.as<int>()};
+    if ((rc = fp64_range(ctx, p, true, false, r))) return rc;
+    timed(ctx, "pairs_full_jacobian_f64", [&] {
+      st::launch_contract_cart(ctx->stream, r.s1 - r.s0, (int)(r.p1 - r.p0), ctx->n_soap, r.site_ptr,
+                               ctx->G.as<double>() + r.s0, NS, ctx->cart.as<double>(),
+                               ctx->pair64.as<double>() + 3 * r.p0);
+    });
+  }
+  HIPCHK(hipGetLastError());
+  if (n_batches) *n_batches = nbat;
+  ctx->have_soap = true;
+  ctx->have_cart = nbat == 1;  // soap_cart_der is whole only when one batch held every pair
+  ctx->ct_ready = false;
+  ctx->have_pair64 = true;
+  return forces64(ctx);
 }
 
 int st_pairs_full_jacobian(st_ctx* ctx) {
@@ -849,7 +994,7 @@ int st_pairs_full_jacobian(st_ctx* ctx) {
   if (rc) return rc;
   timed(ctx, "pairs_full_jacobian_f64", [&] {
     st::launch_contract_cart(ctx->stream, ctx->n_sites, ctx->n_pairs, ctx->n_soap, ctx->site_ptr.as<int>(),
-                             ctx->G.as<double>(), ctx->cart.as<double>(), ctx->pair64.as<double>());
+                             ctx->G.as<double>(), ctx->n_sites, ctx->cart.as<double>(), ctx->pair64.as<double>());
   });
   HIPCHK(hipGetLastError());
   ctx->have_pair64 = true;

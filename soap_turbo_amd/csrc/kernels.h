@@ -36,7 +36,7 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int
                         const double* sqrt_dot_p, const double* rjs, const double* thetas, const double* phis,
                         double* rotp, double* cart, int variant = -1);
 void launch_contract_cart(hipStream_t st, int n_sites, int n_pairs, int n_soap, const int* site_ptr, const double* G,
-                          const double* cart, double* pair);
+                          int ldg, const double* cart, double* pair);
 size_t contract_lds(int n_max, int k_max, int n_soap, int n_soap_unc);
 void launch_contract(hipStream_t st, int n_sites, int n_max, int l_max, int k_max, int n_soap, int n_soap_unc,
                      const int* site_ptr, const uint8_t* skip, const int* unc_ptr, const int* unc_e,

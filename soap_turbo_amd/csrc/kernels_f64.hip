@@ -1180,14 +1180,14 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
 // pair(:,k) = sum_s G(i,s) soap_cart_der(:,s,k): what a caller of the
 // full-Jacobian get_soap does to turn dE/dq into pair gradients. One
 // workgroup per site holds G(i,:) in LDS; one wavefront per pair.
-__global__ __launch_bounds__(256) void contract_cart_k(int n_sites, int n_pairs, int n_soap,
+__global__ __launch_bounds__(256) void contract_cart_k(int n_sites, int n_pairs, int n_soap, int ldg,
                                                        const int* __restrict__ site_ptr,
                                                        const double* __restrict__ G, const double* __restrict__ cart,
                                                        double* __restrict__ pair) {
   extern __shared__ double gs[];
   const int i = blockIdx.x;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int s = threadIdx.x; s < n_soap; s += blockDim.x) gs[s] = G[(size_t)s * n_sites + i];
+  for (int s = threadIdx.x; s < n_soap; s += blockDim.x) gs[s] = G[(size_t)s * ldg + i];
   __syncthreads();
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const int len = 3 * n_soap;  // the pair's row of soap_cart_der, read as one contiguous stream
@@ -1706,9 +1706,10 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int
 }
 
 void launch_contract_cart(hipStream_t st, int n_sites, int n_pairs, int n_soap, const int* site_ptr, const double* G,
-                          const double* cart, double* pair) {
+                          int ldg, const double* cart, double* pair) {
   if (n_pairs == 0 || n_sites == 0) return;
-  contract_cart_k<<<n_sites, 256, sizeof(double) * n_soap, st>>>(n_sites, n_pairs, n_soap, site_ptr, G, cart, pair);
+  contract_cart_k<<<n_sites, 256, sizeof(double) * n_soap, st>>>(n_sites, n_pairs, n_soap, ldg, site_ptr, G, cart,
+                                                                  pair);
 }
 
 size_t contract_lds(int n_max, int k_max, int n_soap, int n_soap_unc) {

@@ -150,12 +150,11 @@ int64_t stw_neighbours(int64_t n_atoms, const double* pos, double L, double rcut
   for (int64_t a = n_atoms - 1; a >= 0; --a) grid.insert((int)a, pos + 3 * a);
   const double rc2 = rcut * rcut;
   const bool stencil = grid.nc >= 3;
-  int64_t k = 0;
-  std::vector<int> found;
-  for (int64_t s = 0; s < n_sites; ++s) {
+  // Visits the neighbours of site s in increasing atom index (its own atom excluded): emit(b, rx, ry, rz).
+  auto visit = [&](int64_t s, std::vector<int>& uniq, auto&& emit) {
     const int64_t i = site_atom[s];
     const double* pi = pos + 3 * i;
-    found.clear();
+    uniq.clear();
     if (stencil) {
       int cx = grid.coord(pi[0]), cy = grid.coord(pi[1]), cz = grid.coord(pi[2]);
       for (int dz = -1; dz <= 1; ++dz)
@@ -164,46 +163,64 @@ int64_t stw_neighbours(int64_t n_atoms, const double* pos, double L, double rcut
             int ex = ((cx + dx) % grid.nc + grid.nc) % grid.nc;
             int ey = ((cy + dy) % grid.nc + grid.nc) % grid.nc;
             int ez = ((cz + dz) % grid.nc + grid.nc) % grid.nc;
-            for (int b = grid.head[grid.cell(ex, ey, ez)]; b >= 0; b = grid.next[b]) found.push_back(b);
+            for (int b = grid.head[grid.cell(ex, ey, ez)]; b >= 0; b = grid.next[b]) uniq.push_back(b);
           }
     } else {
-      for (int b = 0; b < (int)n_atoms; ++b) found.push_back(b);
+      for (int b = 0; b < (int)n_atoms; ++b) uniq.push_back(b);
     }
     // increasing j, no duplicates (small grids can revisit a cell)
-    std::vector<int> uniq;
-    uniq.reserve(found.size());
-    for (int b : found) uniq.push_back(b);
     std::sort(uniq.begin(), uniq.end());
     uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-    int32_t cnt = 1;
-    if (nbr) {
-      nbr[k] = (int32_t)i;
-      vec[3 * k] = vec[3 * k + 1] = vec[3 * k + 2] = 0.0;
-    }
-    ++k;
     for (int b : uniq) {
       if (b == i) continue;
       double rx = wrap(pos[3 * b] - pi[0], L), ry = wrap(pos[3 * b + 1] - pi[1], L),
              rz = wrap(pos[3 * b + 2] - pi[2], L);
       double r2 = rx * rx + ry * ry + rz * rz;
       if (r2 >= rc2) continue;
-      if (nbr) {
+      emit(b, rx, ry, rz);
+    }
+  };
+  // counts (sites in parallel; each site's list depends on nothing else)
+#pragma omp parallel
+  {
+    std::vector<int> uniq;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t s = 0; s < n_sites; ++s) {
+      int32_t cnt = 1;
+      visit(s, uniq, [&](int, double, double, double) { ++cnt; });
+      n_neigh[s] = cnt;
+    }
+  }
+  int64_t total = 0;
+  std::vector<int64_t> start((size_t)n_sites + 1, 0);
+  for (int64_t s = 0; s < n_sites; ++s) start[s + 1] = start[s] + n_neigh[s];
+  total = start[n_sites];
+  if (!nbr) return total;
+#pragma omp parallel
+  {
+    std::vector<int> uniq;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t s = 0; s < n_sites; ++s) {
+      int64_t k = start[s];
+      nbr[k] = (int32_t)site_atom[s];
+      vec[3 * k] = vec[3 * k + 1] = vec[3 * k + 2] = 0.0;
+      ++k;
+      visit(s, uniq, [&](int b, double rx, double ry, double rz) {
         nbr[k] = b;
         vec[3 * k] = rx;
         vec[3 * k + 1] = ry;
         vec[3 * k + 2] = rz;
-      }
-      ++k;
-      ++cnt;
+        ++k;
+      });
     }
-    n_neigh[s] = cnt;
   }
-  return k;
+  return total;
 }
 
 // Spherical coordinates of the pair vectors: r, theta = acos(z/r),
 // phi = atan2(y, x); the site's own pair (r = 0) gets zeros.
 void stw_spherical(int64_t n_pairs, const double* vec, double* rjs, double* thetas, double* phis) {
+#pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < n_pairs; ++k) {
     double x = vec[3 * k], y = vec[3 * k + 1], z = vec[3 * k + 2];
     double r = std::sqrt(x * x + y * y + z * z);

@@ -189,10 +189,13 @@ def spherical(vec):
     return r, t, f
 
 
-def make_workload(cfg, n_sites: int | None = None) -> Workload:
+def make_workload(cfg, n_sites: int | None = None, sites=None) -> Workload:
     """Builds configuration ``cfg`` (1..5, or a (BoxSpec, SoapParams) pair).
 
-    ``n_sites`` keeps only the first sites (and their pairs), for small parity cases.
+    ``n_sites`` keeps only the first sites (and their pairs); ``sites`` keeps the
+    sites at these positions of the full site list instead (e.g. from
+    ``strided_sites`` or ``slab_sites``). Either gives small parity cases cut
+    from the full box, with every neighbour of a kept site still present.
     """
     spec, params = CONFIGS[cfg] if isinstance(cfg, int) else cfg
     L, pos, asp = place_atoms(spec)
@@ -200,6 +203,8 @@ def make_workload(cfg, n_sites: int | None = None) -> Workload:
         site_atom = np.arange(spec.n_atoms, dtype=np.int64)
     else:
         site_atom = np.nonzero(asp == spec.central)[0].astype(np.int64)
+    if sites is not None:
+        site_atom = site_atom[np.asarray(sites, dtype=np.int64)]
     if n_sites is not None:
         site_atom = site_atom[:n_sites]
     rcut_max = float(np.max(params.per_species()["rcut_hard"]))
@@ -213,6 +218,25 @@ def make_workload(cfg, n_sites: int | None = None) -> Workload:
     species = (asp[site_atom] + 1).astype(np.int32).reshape(-1, 1)
     mult = np.ones(len(site_atom), dtype=np.int32)
     return Workload(params, L, pos, asp, site_atom, n_neigh, nbr, vec, rjs, thetas, phis, mask, species, mult)
+
+
+def strided_sites(n_all: int, n: int, offset: int = 0):
+    """n positions spread evenly over a site list of n_all entries (every (n_all // n)-th, from offset)."""
+    step = max(1, n_all // n)
+    return (offset + step * np.arange(n)) % n_all
+
+
+def slab_sites(cfg, n_dense: int, n_gas: int):
+    """Positions in the site list of a slab box (``BoxSpec.slab``; every atom a site): n_dense spread over
+    the dense half and n_gas over the gas half. Random sequential addition places the dense atoms
+    first, so the dense half is the first ``round(V/2 * dense_density)`` atoms."""
+    spec, _ = CONFIGS[cfg] if isinstance(cfg, int) else cfg
+    if spec.slab is None or spec.central is not None:
+        raise ValueError("slab_sites needs a slab box whose atoms are all sites")
+    dense, gas = spec.slab
+    V = spec.n_atoms / (0.5 * (dense + gas))
+    nd = int(round(0.5 * V * dense))
+    return np.concatenate([strided_sites(nd, n_dense), nd + strided_sites(spec.n_atoms - nd, n_gas)])
 
 
 def normal(seed: int, shape):
