@@ -152,6 +152,22 @@ int st_upload_dedq(st_ctx* ctx, int32_t n_soap, const double* G);
 int st_compute(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t do_derivatives, int32_t contract);
 /* FP64 pair gradients from the resident soap_cart_der and dE/dq (what a full-Jacobian caller does). */
 int st_pairs_full_jacobian(st_ctx* ctx);
+/* Atoms st_pairs_full_jacobian_noalloc accepts: its scratch is a fixed 9 doubles per 16 atoms. */
+#define ST_NOALLOC_MAX_ATOMS (1 << 21)
+/* No reference counterpart: st_pairs_full_jacobian for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc). Every array is the caller's,
+ * in device memory: site_ptr(n_sites + 1) with site_ptr(1) = 0, G(n_sites, n_soap), soap_cart_der(3,
+ * n_soap, n_pairs), atom_ptr(n_atoms + 1) and atom_pairs(n_pairs) (the pairs of each atom, as
+ * st_atom_pairs gives them), vec(3, n_pairs) in; pair(3, n_pairs), forces(3, n_atoms), virial(3, 3) out.
+ * The call allocates nothing: the force gather's partial sums live in a static device array of
+ * ST_NOALLOC_MAX_ATOMS / 16 * 9 doubles (9 MiB, held by each device the library runs on until the
+ * process ends); larger n_atoms are refused (ST_ERR_ARG). Launches are ordered on the context's
+ * stream and the call returns before they finish. All calls on one device share that array, so
+ * calls that overlap on one device (other contexts' streams, other threads) race. */
+int st_pairs_full_jacobian_noalloc(st_ctx* ctx, int32_t n_sites, int32_t n_pairs, int32_t n_soap, int32_t n_atoms,
+                                   const int32_t* site_ptr, const double* G, const double* soap_cart_der,
+                                   const int32_t* atom_ptr, const int32_t* atom_pairs, const double* vec, double* pair,
+                                   double* forces, double* virial);
 /* The FP64 full-Jacobian path in batches of sites: for each batch of consecutive sites, get_soap with
  * soap_cart_der on that batch, then its contraction with dE/dq to pair gradients; forces and virial at
  * the end (st_download_pairs / st_download_forces with which = 0). This is what the reference's callers

@@ -56,6 +56,7 @@ SIGNATURES = {
     "st_compute": (ctypes.c_int, [P, P, P, I32, I32]),
     "st_pairs_full_jacobian": (ctypes.c_int, [P]),
     "st_full_jacobian_forces": (ctypes.c_int, [P, P, P, ctypes.c_int64, ctypes.POINTER(I32)]),
+    "st_pairs_full_jacobian_noalloc": (ctypes.c_int, [P, I32, I32, I32, I32, P, P, P, P, P, P, P, P, P]),
     "st_contract_resident": (ctypes.c_int, [P, ctypes.c_double]),
     "st_adjoint_f32": (ctypes.c_int, [P, P, P]),
     "st_set_graphs": (ctypes.c_int, [P, I32]),

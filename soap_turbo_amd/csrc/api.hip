@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -19,7 +20,29 @@
 #include "kernels.h"
 #include "st_internal.h"
 
+// Scratch of st_pairs_full_jacobian_noalloc: the force gather's per-workgroup virial partials, 9 doubles
+// per 16 atoms, for up to ST_NOALLOC_MAX_ATOMS atoms (9.0 MiB). Each device holds its own copy for the
+// life of the process; host code uses the address hipGetSymbolAddress gives on that device (looked up
+// at st_create), never the array's name.
+static __device__ __align__(256) double scratch[(size_t)((ST_NOALLOC_MAX_ATOMS + 15) / 16) * 9];
+
 namespace {
+
+std::mutex scratch_mu;
+std::vector<double*> scratch_dev;  // scratch's address on each device (null until looked up)
+
+// scratch's address on `device` (the current device), looked up once per device.
+double* scratch_on(int device) {
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if ((int)scratch_dev.size() <= device) scratch_dev.resize(device + 1, nullptr);
+  if (!scratch_dev[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    scratch_dev[device] = static_cast<double*>(p);
+  }
+  return scratch_dev[device];
+}
+
 
 struct DevBuf {
   void* p = nullptr;
@@ -38,6 +61,7 @@ struct st_ctx {
   hipStream_t stream = nullptr;
   size_t lds_max = 64 * 1024;  // LDS bytes one workgroup may allocate on this device
   int n_cu = 1;                // compute units
+  double* scratch = nullptr;   // this device's copy of the no-alloc scratch array
   std::string err;
 
   // basis: injected or built (cache key = parameters that change W and S)
@@ -582,7 +606,8 @@ int st_create(st_ctx** out, int device) {
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
     c->n_cu = ncu;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  c->scratch = scratch_on(device);
+  if (!c->scratch || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return ST_ERR_HIP;
   }
@@ -999,6 +1024,30 @@ int st_pairs_full_jacobian(st_ctx* ctx) {
   HIPCHK(hipGetLastError());
   ctx->have_pair64 = true;
   return forces64(ctx);
+}
+
+int st_pairs_full_jacobian_noalloc(st_ctx* ctx, int32_t n_sites, int32_t n_pairs, int32_t n_soap, int32_t n_atoms,
+                                   const int32_t* site_ptr, const double* G, const double* soap_cart_der,
+                                   const int32_t* atom_ptr, const int32_t* atom_pairs, const double* vec, double* pair,
+                                   double* forces, double* virial) {
+  if (!ctx) return ST_ERR_ARG;
+  if (n_sites < 1 || n_pairs < n_sites || n_soap < 1 || n_atoms < 1)
+    return fail(ctx, ST_ERR_ARG, "st_pairs_full_jacobian_noalloc: bad sizes");
+  if (n_atoms > ST_NOALLOC_MAX_ATOMS)
+    return fail(ctx, ST_ERR_ARG, "st_pairs_full_jacobian_noalloc: n_atoms = " + std::to_string(n_atoms) +
+                                     " exceeds ST_NOALLOC_MAX_ATOMS = " + std::to_string(ST_NOALLOC_MAX_ATOMS) +
+                                     "; use st_pairs_full_jacobian");
+  if (!site_ptr || !G || !soap_cart_der || !atom_ptr || !atom_pairs || !vec || !pair || !forces || !virial)
+    return fail(ctx, ST_ERR_ARG, "st_pairs_full_jacobian_noalloc: every array is required");
+  if ((size_t)n_soap * 8 > ctx->lds_max)
+    return fail(ctx, ST_ERR_UNSUPPORTED, "st_pairs_full_jacobian_noalloc: G(i, :) exceeds the LDS");
+  if (!ctx->scratch) return fail(ctx, ST_ERR_STATE, "st_pairs_full_jacobian_noalloc: no scratch on this device");
+  HIPCHK(hipSetDevice(ctx->device));
+  st::launch_contract_cart(ctx->stream, n_sites, n_pairs, n_soap, site_ptr, G, n_sites, soap_cart_der, pair);
+  st::launch_forces_f64(ctx->stream, n_atoms, pair, atom_ptr, atom_pairs, vec, nullptr, nullptr, nullptr, forces,
+                        ctx->scratch, virial);
+  HIPCHK(hipGetLastError());
+  return ST_OK;
 }
 
 int st_contract_resident(st_ctx* ctx, double sgn) {
