@@ -294,18 +294,23 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   __shared__ int site_next;
   const int n_max = P.n_max, n_species = P.n_species;
   int i = 0, p0 = 0, p1 = 0;  // the current site and its pairs
-  // one tile of neighbours: the backward pass reuses the forward pass's set-up (kept in registers,
-  // affordable up to n_max = 32)
-  bool one_tile = false;
+  // the backward pass takes the tiles last first and reuses the forward pass's set-up of the last
+  // tile (kept in registers, affordable up to n_max = 32; the tile's staged pair data is still in LDS)
+  constexpr bool KEEP = NTT <= 2;
+  int last_tb = 0;
   const float fourpi = 12.566370614359172f;
   int tx = threadIdx.x;
   int wv = tx >> 6, lane = tx & 63;
   int li = lane & 15, lk = lane >> 4;
-  int* cell = reinterpret_cast<int*>(smem);        // n_species * PT radial cells (-1: none)
-  float* cell_t = smem + n_species * PT;           // n_species * PT
-  float* cell_ih = cell_t + n_species * PT;        // n_species * PT
-  float* C = cell_ih + n_species * PT;             // n_max * max(CS, HS) (C, then H)
+  float* C = smem;                                 // n_max * max(CS, HS) (C, then H)
   float* region = C + n_max * (CS > HS ? CS : HS);
+  // the radial table cells of the staged tile (3 n_species PT), live from the staging to the end of
+  // the radial items: past what the items write (Rt and Gt forward; Rb, Rdb, Gl, Gd backward), over
+  // At forward and Em backward, which are written only after the barrier that ends the items
+  const int cell_off = max(n_max * PTF + L1 * PTS, 2 * n_max * PTS + ((KM1 * PTS + 1) & ~1) + 2 * L1 * PTS);
+  int* cell = reinterpret_cast<int*>(region + cell_off);   // n_species * PT radial cells (-1: none)
+  float* cell_t = region + cell_off + n_species * PT;      // n_species * PT
+  float* cell_ih = cell_t + n_species * PT;                // n_species * PT
   float* Rt = region;                              // forward: n_max * PTF (4 pi R, pair fastest)
   float* Gt = Rt + n_max * PTF;                    //          L1 * PTS (g_l)
   float* At = Gt + L1 * PTS;                       //          NC * PTF (A columns, pair fastest)
@@ -485,12 +490,13 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   asm volatile("" : "+v"(tx), "+v"(wv), "+v"(lane), "+v"(li), "+v"(lk));
   p0 = site_ptr[i];
   p1 = site_ptr[i + 1];
-  one_tile = NTT <= 2 && p1 - p0 <= PT;
+  last_tb = p0 + ((p1 - p0 - 1) / PT) * PT;
   // --------------------------------------------------------------- forward
   // With a direct output map (every output one uncompressed component, every component at
-  // most one output) and at most 3 outputs per thread, each thread's outputs are computed from
+  // most one output) and at most DOUT outputs per thread, each thread's outputs are computed from
   // C where they are used, without S_l.
-  const bool spec_direct = odir != nullptr && P.n_soap <= 3 * NTHREADS;
+  constexpr int DOUT = 4;
+  const bool spec_direct = odir != nullptr && P.n_soap <= DOUT * NTHREADS;
   floatx4 hi[TPW], lo[TPW];
 #pragma unroll
   for (int j = 0; j < TPW; ++j) {
@@ -507,8 +513,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     // the thread index made opaque per tile: item addresses are formed here, not hoisted
     int tid = tx;
     asm volatile("" : "+v"(tid));
-    radial_items(tid, fourpi, Rt, nullptr, PTF, one_tile);
-    gl_items(tid, Gt, nullptr, one_tile);
+    radial_items(tid, fourpi, Rt, nullptr, PTF, KEEP && tb == last_tb);
+    gl_items(tid, Gt, nullptr, KEEP && tb == last_tb);
     __syncthreads();
     // (m, pair) items: A_k = preflm P_l^m g_l e^{-i m phi} for l = m .. L, as (re, im) rows.
     // m is uniform over a wavefront; l is unrolled, P_l^m by the recursions of get_plm_array
@@ -596,7 +602,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   __syncthreads();
   // --------------------------------------------------------------- spectrum
   // S_l(n, n') = sum over the l block of w(m) C(n, c) C(n', c), w = 1 (m = 0), 2 (m > 0).
-  // With a direct output map and at most 3 outputs per thread, each output's entry is summed
+  // With a direct output map and at most DOUT outputs per thread, each output's entry is summed
   // where it is used, below. Otherwise S_l is formed on MFMA: tiles with a <= b (the outputs
   // read n <= n'), a wavefront's jobs unrolled so that their loads are in flight together.
   if (!spec_direct) {
@@ -650,17 +656,17 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   float* Y = Sl;
   float sums[2] = {0.f, 0.f};
   if (spec_direct) {
-    int2 od[3];
-    float gd[3], vq[3];
+    int2 od[DOUT];
+    float gd[DOUT], vq[DOUT];
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
+    for (int u = 0; u < DOUT; ++u) {
       const int s = tx + NTHREADS * u;
       od[u] = s < n_soap ? odir[s] : make_int2(0, 0);
       gd[u] = !FO && s < n_soap ? G32[(size_t)i * n_soap + s] : 0.f;
     }
     for (int t = tx; t < ny; t += NTHREADS) Y[t] = 0.f;
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
+    for (int u = 0; u < DOUT; ++u) {
       const int s = tx + NTHREADS * u;
       vq[u] = 0.f;
       if (s < n_soap) {
@@ -682,7 +688,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     const float inv = 1.f / sdp;
     const float qg = sums[1] * inv;
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
+    for (int u = 0; u < DOUT; ++u) {
       const int s = tx + NTHREADS * u;
       if (s < n_soap) {
         const float q = vq[u] * inv;
@@ -892,12 +898,12 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   }
   // --------------------------------------------------------------- backward
   float self[3] = {0.f, 0.f, 0.f};
-  for (int tb = p0; tb < p1; tb += PT) {
+  for (int tb = last_tb; tb >= p0; tb -= PT) {
     __syncthreads();
     int tid = tx;
     asm volatile("" : "+v"(tid));
-    if (one_tile) {
-      // the forward pass staged this tile and kept R, R', g, g' of its items: no table loads
+    if (KEEP && tb == last_tb) {
+      // the forward pass staged this tile last and kept R, R', g, g' of its items: no table loads
 #pragma unroll
       for (int u = 0; u < RIT; ++u) {
         const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
@@ -919,6 +925,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       __syncthreads();
       radial_items(tid, 1.f, Rb, Rdb, PTS, false);
       gl_items(tid, Gl, Gd, false);
+      __syncthreads();  // the radial cells are read; Em is written over them next
     }
     // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
 #pragma unroll
@@ -1091,12 +1098,15 @@ __global__ void widen_k(size_t n, const float* __restrict__ a, double* __restric
 size_t site_lds(int n_species, int n_max, int l_max, int n_soap, int n_soap_unc) {
   const int KM = (l_max + 1) * (l_max + 2) / 2, KM1 = (l_max + 2) * (l_max + 3) / 2, L1 = l_max + 1;
   const int CSTR = ((2 * KM + 15) / 16) * 16, CS = CSTR + 4, KMP = ((KM + 15) / 16) * 16, HS = 2 * KMP + 16;
-  const size_t cells = (size_t)3 * n_species * PT;
   const size_t c = (size_t)n_max * std::max(CS, HS);
   const size_t fwd = (size_t)n_max * PTF + (size_t)L1 * PTS + (size_t)2 * KM * PTF;
   const size_t spec = (size_t)L1 * n_max * n_max + 2 * n_soap + n_soap_unc;
   const size_t bwd = (size_t)2 * n_max * PTS + ((KM1 * PTS + 1) & ~1) + 4 * L1 * PTS + 3 * PT;
-  return sizeof(float) * (cells + c + std::max(std::max(fwd, spec), bwd));
+  // the radial cells sit in the region past what the set-up items write (the kernel's cell_off)
+  const size_t cell_off = std::max((size_t)n_max * PTF + (size_t)L1 * PTS,
+                                   (size_t)2 * n_max * PTS + ((KM1 * PTS + 1) & ~1) + (size_t)2 * L1 * PTS);
+  const size_t cells_end = cell_off + (size_t)3 * n_species * PT;
+  return sizeof(float) * (c + std::max(std::max(std::max(fwd, spec), bwd), cells_end));
 }
 
 template <int L, int NTT>
@@ -1370,7 +1380,8 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
   int rc;
   timer("adj_tables", [&] { rc = build_tables(s, st, in, P, msg); });
   if (rc) return rc;
-  const size_t sh = site_lds(in.p->n_species, in.n_max, in.l_max, in.n_soap, in.n_soap_unc);
+  // (the per-component dE/dp array, n_soap_unc, is needed only without a direct output map)
+  const size_t sh = site_lds(in.p->n_species, in.n_max, in.l_max, in.n_soap, in.ocomp ? 0 : in.n_soap_unc);
   if (sh > in.lds_max) {
     msg = "the FP32 site working set exceeds the LDS";
     return ST_ERR_UNSUPPORTED;
