@@ -55,6 +55,9 @@ struct SpeciesTab {
 
 struct AdjParams {
   int n_species, n_max, k_max, l_max, n_soap, n_soap_unc, n_sites, compressed, gauss;
+  // radial set-up items: each species block split into nch chunks of at most qc functions;
+  // keep: one item per thread and qc <= the kept-register count (the backward pass reuses them)
+  int nch, qc, keep;
   double rcut_max, ang_inv_h;
   float ang_ih;
   int ang_n;
@@ -274,8 +277,10 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   constexpr int NC = 2 * KM;                        // live columns of C
   constexpr int TPW = (NTT * CT + 3) / 4;           // cnk tiles per wavefront
   constexpr int KN = 4 * NTT;                       // K steps over n (padded to 16 NTT)
-  constexpr int RIT = 4 * NTT;                      // (n, pair) radial items per thread
-  constexpr int GIT = (L1 + 3) / 4;                 // (l, pair) items per thread, l <= L
+  constexpr int QB = 4;                             // radial functions per batch of table loads
+  constexpr int QK = 3 * QB;                        // radial functions per item (alpha_max <= 11)
+  constexpr int QKEEP = 8;                          // ... kept in registers for the backward pass
+  constexpr int GIT = (L1 + 3) / 4;                 // l per (pair, l-chunk) item, l <= L
   constexpr int PIT = (L + 5) / 4;                  // (m, pair) items per thread, m <= L + 1
   constexpr uint32_t SELF = 1u << 31;               // pair flag: the site's own pair
   extern __shared__ float smem[];
@@ -287,16 +292,14 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   __shared__ float pr_r[PT], pr_st[PT], pr_ct[PT], pr_sf[PT], pr_cf[PT];
   __shared__ int pr_acell[PT];                      // angular table cell (floats into atab)
   __shared__ float pr_ta[PT];
-  __shared__ int sp_off[ADJ_MAX_SPECIES], sp_a[ADJ_MAX_SPECIES], sp_rt[ADJ_MAX_SPECIES];
-  __shared__ float sp_cw[ADJ_MAX_SPECIES];
-  __shared__ unsigned char nsp[64];                 // species block of each n
+  __shared__ double pr_rd[PT];                      // r in FP64 (the radial table cell coordinate)
   __shared__ float cen_s[64];                       // the site's central-atom term per n (poly3gauss)
   __shared__ int site_next;
   const int n_max = P.n_max, n_species = P.n_species;
   int i = 0, p0 = 0, p1 = 0;  // the current site and its pairs
   // the backward pass takes the tiles last first and reuses the forward pass's set-up of the last
   // tile (kept in registers, affordable up to n_max = 32; the tile's staged pair data is still in LDS)
-  constexpr bool KEEP = NTT <= 2;
+  const bool KEEP = NTT <= 2 && P.keep != 0;
   int last_tb = 0;
   const float fourpi = 12.566370614359172f;
   int tx = threadIdx.x;
@@ -304,13 +307,6 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   int li = lane & 15, lk = lane >> 4;
   float* C = smem;                                 // n_max * max(CS, HS) (C, then H)
   float* region = C + n_max * (CS > HS ? CS : HS);
-  // the radial table cells of the staged tile (3 n_species PT), live from the staging to the end of
-  // the radial items: past what the items write (Rt and Gt forward; Rb, Rdb, Gl, Gd backward), over
-  // At forward and Em backward, which are written only after the barrier that ends the items
-  const int cell_off = max(n_max * PTF + L1 * PTS, 2 * n_max * PTS + ((KM1 * PTS + 1) & ~1) + 2 * L1 * PTS);
-  int* cell = reinterpret_cast<int*>(region + cell_off);   // n_species * PT radial cells (-1: none)
-  float* cell_t = region + cell_off + n_species * PT;      // n_species * PT
-  float* cell_ih = cell_t + n_species * PT;                // n_species * PT
   float* Rt = region;                              // forward: n_max * PTF (4 pi R, pair fastest)
   float* Gt = Rt + n_max * PTF;                    //          L1 * PTS (g_l)
   float* At = Gt + L1 * PTS;                       //          NC * PTF (A columns, pair fastest)
@@ -326,29 +322,39 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   float2* Em = reinterpret_cast<float2*>(Gd + L1 * PTS);  //    L1 * PTS (e^{-i m phi})
   float* fout = reinterpret_cast<float*>(Em + L1 * PTS);  //    3 * PT
 
-  // The tile's pairs: r, species bits of contributing pairs, sin/cos of theta and phi,
-  // and per species the radial table cell (FP64 cell coordinate) or -1. Wavefront 0 takes r,
-  // the species and the cells, wavefronts 1 and 2 the sines and cosines of theta and phi.
+  // The next tile's pair data, loaded a phase or more before it is staged (the loads' latency passes
+  // while the current tile is processed): wavefront 0 r and the species bits, wavefront 1 theta (and
+  // the site's central-atom term with a site's first tile), wavefront 2 phi.
+  double pf_d = 0.0;
+  uint32_t pf_m = 0u;
+  float pf_c = 0.f;
+  auto fetch = [&](int site, int tb, int pend, bool first) {
+    const int p = tb + (tx & (PT - 1));
+    if (tx < PT) {
+      pf_d = p < pend ? rjs[p] : 1.0;
+      pf_m = p < pend ? spmask[p] : 0u;
+    } else if (tx < 2 * PT) {
+      pf_d = p < pend ? thetas[p] : 0.0;
+      if (cenv && first && tx - PT < n_max) pf_c = cenv[(size_t)site * n_max + tx - PT];
+    } else if (tx < 3 * PT) {
+      pf_d = p < pend ? phis[p] : 0.0;
+    }
+  };
+  // The tile's pairs (fetched before): r (FP32, and FP64 for the radial table cell coordinate),
+  // species bits of contributing pairs, the angular table cell, sin/cos of theta and phi.
+  // Wavefront 0 takes r, the species and the angular cell, wavefronts 1 and 2 the sines and cosines
+  // of theta and phi; with the site's own pair (forward pass), wavefront 1 also stores the
+  // central-atom term.
   auto stage = [&](int tb, bool with_self) {
     const int pl = tx & (PT - 1), p = tb + pl;
     if (tx < PT) {
       uint32_t msk = 0;
-      double rd = 1.0;
+      const double rd = pf_d;
       if (p < p1) {
-        rd = rjs[p];
-        msk = spmask[p];
+        msk = pf_m;
         if (!(rd < P.rcut_max) || (p == p0 && !with_self)) msk = 0;
       }
       const bool self = p == p0;
-      for (int s = 0; s < n_species; ++s) {
-        const SpeciesTab& sp = P.sp[s];
-        int c = -1;
-        float t = 0.f, ih = 0.f;
-        if ((msk & (1u << s)) && rd < sp.rcut && !(self && !sp.do_central)) c = radial_cell(sp, rd, t, ih);
-        cell[s * PT + pl] = c;
-        cell_t[s * PT + pl] = t;
-        cell_ih[s * PT + pl] = ih;
-      }
       int acell = 0;
       float ta = 0.f;
       if (msk != 0) {
@@ -364,80 +370,107 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       }
       pr_msk[pl] = msk != 0 && self ? (msk | SELF) : msk;
       pr_r[pl] = (float)rd;
+      pr_rd[pl] = rd;
       pr_acell[pl] = acell;
       pr_ta[pl] = ta;
     } else if (tx < 2 * PT) {
-      const double th = p < p1 ? thetas[p] : 0.0;
+      const double th = pf_d;
       float st_, ct_;
       sincosf((float)th, &st_, &ct_);
       pr_st[pl] = st_;
       pr_ct[pl] = ct_;
+      if (cenv && with_self && tb == p0 && tx - PT < n_max) cen_s[tx - PT] = pf_c;
     } else if (tx < 3 * PT) {
-      const double ph = p < p1 ? phis[p] : 0.0;
+      const double ph = pf_d;
       float sf_, cf_;
       sincosf((float)ph, &sf_, &cf_);
       pr_sf[pl] = sf_;
       pr_cf[pl] = cf_;
     }
   };
-  // (n, pair) radial items: R_n times scale (and the central weight on the site's own
-  // pair), and dR_n/dr if Rd is given. With kv/kd, R_n and dR_n/dr of every pair but the
-  // site's own are also kept in registers, for the backward pass of a one-tile site.
-  float kv[RIT], kd[RIT], kg[GIT], kgd[GIT];
+  // Radial items (species block s, chunk ch, pair): the pair's table cell for species s (FP64
+  // coordinate), then R_n times scale (and the central weight on the site's own pair), and dR_n/dr if
+  // Rd is given, for the chunk's functions n, whose cubics lie side by side in the cell's table row.
+  // (s, ch) is uniform over a wavefront. With keep, R_n and dR_n/dr of every pair but the site's
+  // own are also kept in registers (kv, kd: the thread's one item), for the backward pass.
+  float kv[QKEEP], kd[QKEEP], kg[GIT], kgd[GIT];
   auto radial_items = [&](int tid, float scale, float* R, float* Rd, int stride, bool keep) {
-    // pass 1: every table load of the thread's items; pass 2: evaluate and store (an LDS store
-    // between them would hold back the next item's loads, as the compiler cannot tell
-    // the LDS arrays apart)
-    float c0[RIT], c1[RIT], c2[RIT], c3[RIT], tt[RIT], sc[RIT], ih[RIT];
+    const int n_items = n_species * P.nch * PT;
+    for (int j = tid; j < n_items; j += NTHREADS) {
+      const int pl = j & (PT - 1);
+      const int sc_ = __builtin_amdgcn_readfirstlane(j >> 6);
+      const int s = sc_ / P.nch, ch = sc_ - s * P.nch;
+      const SpeciesTab& sp = P.sp[s];
+      const int q0 = ch * P.qc, qn = min(sp.a - q0, P.qc);
+      const uint32_t msk = pr_msk[pl];
+      const double rd = pr_rd[pl];
+      const bool self = (msk & SELF) != 0;
+      int c = -1;
+      float t = 0.f, ih = 0.f;
+      if ((msk & (1u << s)) && rd < sp.rcut && !(self && !sp.do_central)) c = radial_cell(sp, rd, t, ih);
+      const float sc = (self ? sp.cw : 1.f) * scale;
+      const float4* row = reinterpret_cast<const float4*>(rtab + sp.rtab_off) + (size_t)(c < 0 ? 0 : c) * sp.a + q0;
+      float* Rrow = R + (sp.off + q0) * stride + pl;
+      float* Rdrow = Rd ? Rd + (sp.off + q0) * stride + pl : nullptr;
 #pragma unroll
-    for (int u = 0; u < RIT; ++u) {
-      const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
-      c0[u] = c1[u] = c2[u] = c3[u] = tt[u] = ih[u] = 0.f;
-      sc[u] = 0.f;
-      if (n < n_max) {
-        const int s = nsp[n];
-        const int c = cell[s * PT + pl];
-        if (c >= 0) {
-          tt[u] = cell_t[s * PT + pl];
-          ih[u] = cell_ih[s * PT + pl];
-          const int a = sp_a[s], q = n - sp_off[s];
-          const float4 cf = reinterpret_cast<const float4*>(rtab + sp_rt[s])[(size_t)c * a + q];
-          c0[u] = cf.x; c1[u] = cf.y; c2[u] = cf.z; c3[u] = cf.w;
-          sc[u] = ((pr_msk[pl] & SELF) ? sp_cw[s] : 1.f) * scale;
-        }
-      }
-    }
+      for (int b = 0; b < QK; b += QB) {
+        if (b < qn) {
+          // the batch's table loads, all issued before any is used
+          float4 cf[QB];
 #pragma unroll
-    for (int u = 0; u < RIT; ++u) {
-      const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
-      if (n < n_max) {
-        const float x = tt[u];
-        const float v = c0[u] + x * (c1[u] + x * (c2[u] + x * c3[u]));
-        const float d = (c1[u] + x * (2.f * c2[u] + 3.f * x * c3[u])) * ih[u];
-        R[n * stride + pl] = sc[u] * v;
-        if (Rd) Rd[n * stride + pl] = d;
-        if (keep) {
-          const bool self = (pr_msk[pl] & SELF) != 0;
-          kv[u] = self ? 0.f : v;
-          kd[u] = self ? 0.f : d;
+          for (int q = 0; q < QB; ++q)
+            cf[q] = (c >= 0 && b + q < qn) ? row[b + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int q = 0; q < QB; ++q) {
+            if (b + q < qn) {
+              const float v = cf[q].x + t * (cf[q].y + t * (cf[q].z + t * cf[q].w));
+              const float d = (cf[q].y + t * (2.f * cf[q].z + 3.f * t * cf[q].w)) * ih;
+              Rrow[(b + q) * stride] = sc * v;
+              if (Rd) Rdrow[(b + q) * stride] = d;
+              if (keep && b + q < QKEEP) {
+                kv[b + q] = self ? 0.f : v;
+                kd[b + q] = self ? 0.f : d;
+              }
+            }
+          }
         }
+        __builtin_amdgcn_sched_barrier(0);  // one batch's loads live at a time
       }
     }
   };
-  // (l, pair) items: g_l(r), and dg_l/dr if Gdr is given (both kept in kg/kgd with keep)
+  // The kept radial values back into LDS (the same items as radial_items).
+  auto radial_kept = [&](int tid, float* R, float* Rd, int stride) {
+    const int pl = tid & (PT - 1);
+    const int sc_ = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (sc_ < n_species * P.nch) {
+      const int s = sc_ / P.nch, ch = sc_ - s * P.nch;
+      const int q0 = ch * P.qc, qn = min(P.sp[s].a - q0, P.qc);
+      const int n0 = P.sp[s].off + q0;
+#pragma unroll
+      for (int q = 0; q < QKEEP; ++q)
+        if (q < qn) {
+          R[(n0 + q) * stride + pl] = kv[q];
+          Rd[(n0 + q) * stride + pl] = kd[q];
+        }
+    }
+  };
+  // (pair, l-chunk) items: g_l(r), and dg_l/dr if Gdr is given (both kept in kg/kgd with keep), for
+  // l = GIT wv .. GIT wv + GIT - 1, whose cubics lie side by side in the pair's table row
   auto gl_items = [&](int tid, float* Gr, float* Gdr, bool keep) {
+    const int pl = tid & (PT - 1), l0 = GIT * __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool live = pr_msk[pl] != 0;
+    const float ta = pr_ta[pl];
+    const float4* row = reinterpret_cast<const float4*>(atab + pr_acell[pl]);
+    float4 gc[GIT];
+#pragma unroll
+    for (int u = 0; u < GIT; ++u)
+      gc[u] = (live && l0 + u < L1) ? row[l0 + u] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int u = 0; u < GIT; ++u) {
-      const int t = tid + NTHREADS * u, l = t / PT, pl = t - l * PT;
+      const int l = l0 + u;
       if (l < L1) {
-        float g = 0.f, gd = 0.f;
-        if (pr_msk[pl] != 0) {
-          const float ta = pr_ta[pl];
-          const float4 gc = reinterpret_cast<const float4*>(atab + pr_acell[pl])[l];
-          const float c0 = gc.x, c1 = gc.y, c2 = gc.z, c3 = gc.w;
-          g = c0 + ta * (c1 + ta * (c2 + ta * c3));
-          gd = (c1 + ta * (2.f * c2 + 3.f * ta * c3)) * P.ang_ih;
-        }
+        const float g = gc[u].x + ta * (gc[u].y + ta * (gc[u].z + ta * gc[u].w));
+        const float gd = (gc[u].y + ta * (2.f * gc[u].z + 3.f * ta * gc[u].w)) * P.ang_ih;
         Gr[l * PTS + pl] = g;
         if (Gdr) Gdr[l * PTS + pl] = gd;
         if (keep) {
@@ -462,17 +495,6 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     rec_a[t] = l >= m + 2 ? (float)(2 * l - 1) / (float)(l - m) : 0.f;
     rec_b[t] = l >= m + 2 ? (float)(l - 1 + m) / (float)(l - m) : 0.f;
   }
-  // uniform-index reads of the kernel arguments only: a per-lane index into them would
-  // move them to scratch memory
-  if (threadIdx.x == 0) {
-    for (int s = 0; s < n_species; ++s) {
-      sp_off[s] = P.sp[s].off;
-      sp_a[s] = P.sp[s].a;
-      sp_rt[s] = P.sp[s].rtab_off;
-      sp_cw[s] = P.sp[s].cw;
-      for (int q = 0; q < P.sp[s].a; ++q) nsp[P.sp[s].off + q] = (unsigned char)s;
-    }
-  }
   // Sites in the order given (heaviest first): workgroup b starts on site b, and each workgroup
   // takes its next site from a queue (a ticket counter that is zero at launch; ticket t is site
   // gridDim.x + t) while it works on the current one, until the queue runs dry. The grid is one
@@ -481,6 +503,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     site_next = (int)blockIdx.x < P.n_sites ? (order ? order[blockIdx.x] : (int)blockIdx.x) : -1;
   unsigned long long tk = 0;  // thread 0: the ticket for the next site
   int nxt = -1;               // thread 0: the next site (-1: none)
+  bool pf_site = false;       // the fetch registers hold the first tile of the site taken next
   for (;;) {
   __syncthreads();
   i = site_next;
@@ -491,6 +514,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   p0 = site_ptr[i];
   p1 = site_ptr[i + 1];
   last_tb = p0 + ((p1 - p0 - 1) / PT) * PT;
+  if (!pf_site) fetch(i, p0, p1, true);
+  pf_site = false;
   // --------------------------------------------------------------- forward
   // With a direct output map (every output one uncompressed component, every component at
   // most one output) and at most DOUT outputs per thread, each thread's outputs are computed from
@@ -507,8 +532,11 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     const int np_ = min(PT, p1 - tb);
     __syncthreads();
     stage(tb, true);
-    // the site's central-atom term, loaded by wavefront 1 while wavefront 0 stages the pairs
-    if (cenv && tb == p0 && tx >= 64 && tx < 64 + n_max) cen_s[tx - 64] = cenv[(size_t)i * n_max + tx - 64];
+    {
+      // the next tile to stage: this pass's next, else the backward pass's first restaged one
+      const int nt = tb + PT < p1 ? tb + PT : (KEEP ? last_tb - PT : last_tb);
+      if (nt >= p0) fetch(i, nt, p1, false);
+    }
     __syncthreads();
     // the thread index made opaque per tile: item addresses are formed here, not hoisted
     int tid = tx;
@@ -674,8 +702,17 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         const int cb = l * (l + 1), cw = 2 * (l + 1);
         const float* ca = C + x * CS + cb;
         const float* cc = C + y * CS + cb;
-        float acc = ca[0] * cc[0] + ca[1] * cc[1];
-        for (int c = 2; c < cw; ++c) acc += ca[c] * (2.f * cc[c]);
+        // all 2 (L + 1) columns loaded at once (in the row for any l: cb + 2 L + 1 < 2 KM), the ones
+        // past block l masked out of the sum
+        float va[2 * L1], vb[2 * L1];
+#pragma unroll
+        for (int c = 0; c < 2 * L1; ++c) {
+          va[c] = ca[c];
+          vb[c] = cc[c];
+        }
+        float acc = va[0] * vb[0] + va[1] * vb[1];
+#pragma unroll
+        for (int c = 2; c < 2 * L1; ++c) acc += c < cw ? va[c] * (2.f * vb[c]) : 0.f;
         const float v = __int_as_float(od[u].y) * acc;
         vq[u] = v;
         sums[0] += v * v;
@@ -893,9 +930,21 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   }
   const float* H = C;
   if (tx == 0) {
+    // the next site, published now (no thread reads site_next before the loop-top barrier) so
+    // that the backward pass can fetch its first tile
     const unsigned long long t = tk + gridDim.x;
     nxt = t < (unsigned long long)P.n_sites ? (order ? order[t] : (int)t) : -1;
+    site_next = nxt;
+    __builtin_amdgcn_s_waitcnt(0xc07f);
   }
+  // the next site's first tile into the fetch registers (after a barrier that publishes site_next)
+  auto fetch_next_site = [&]() {
+    const int ns = site_next;
+    if (ns >= 0) {
+      fetch(ns, site_ptr[ns], site_ptr[ns + 1], true);
+      pf_site = true;
+    }
+  };
   // --------------------------------------------------------------- backward
   float self[3] = {0.f, 0.f, 0.f};
   for (int tb = last_tb; tb >= p0; tb -= PT) {
@@ -904,28 +953,25 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     asm volatile("" : "+v"(tid));
     if (KEEP && tb == last_tb) {
       // the forward pass staged this tile last and kept R, R', g, g' of its items: no table loads
+      // (the fetch registers hold the tile restaged next, if any)
+      if (last_tb == p0) fetch_next_site();
+      radial_kept(tid, Rb, Rdb, PTS);
+      {
+        const int pl = tid & (PT - 1), l0 = GIT * (tid >> 6);
 #pragma unroll
-      for (int u = 0; u < RIT; ++u) {
-        const int t = tid + NTHREADS * u, n = t / PT, pl = t - n * PT;
-        if (n < n_max) {
-          Rb[n * PTS + pl] = kv[u];
-          Rdb[n * PTS + pl] = kd[u];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < GIT; ++u) {
-        const int t = tid + NTHREADS * u, l = t / PT, pl = t - l * PT;
-        if (l < L1) {
-          Gl[l * PTS + pl] = kg[u];
-          Gd[l * PTS + pl] = kgd[u];
-        }
+        for (int u = 0; u < GIT; ++u)
+          if (l0 + u < L1) {
+            Gl[(l0 + u) * PTS + pl] = kg[u];
+            Gd[(l0 + u) * PTS + pl] = kgd[u];
+          }
       }
     } else {
       stage(tb, false);
+      if (tb - PT >= p0) fetch(i, tb - PT, p1, false);
+      else fetch_next_site();
       __syncthreads();
       radial_items(tid, 1.f, Rb, Rdb, PTS, false);
       gl_items(tid, Gl, Gd, false);
-      __syncthreads();  // the radial cells are read; Em is written over them next
     }
     // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
 #pragma unroll
@@ -1050,10 +1096,6 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     pair_out[3 * (size_t)p0] = self[0];
     pair_out[3 * (size_t)p0 + 1] = self[1];
     pair_out[3 * (size_t)p0 + 2] = self[2];
-    site_next = nxt;
-    // lgkmcnt(0): the write must land before the barrier at the top of the queue loop, and the
-    // compiler's own wait for that barrier is missing on this back edge
-    __builtin_amdgcn_s_waitcnt(0xc07f);
   }
   }  // site queue
 }
@@ -1102,11 +1144,8 @@ size_t site_lds(int n_species, int n_max, int l_max, int n_soap, int n_soap_unc)
   const size_t fwd = (size_t)n_max * PTF + (size_t)L1 * PTS + (size_t)2 * KM * PTF;
   const size_t spec = (size_t)L1 * n_max * n_max + 2 * n_soap + n_soap_unc;
   const size_t bwd = (size_t)2 * n_max * PTS + ((KM1 * PTS + 1) & ~1) + 4 * L1 * PTS + 3 * PT;
-  // the radial cells sit in the region past what the set-up items write (the kernel's cell_off)
-  const size_t cell_off = std::max((size_t)n_max * PTF + (size_t)L1 * PTS,
-                                   (size_t)2 * n_max * PTS + ((KM1 * PTS + 1) & ~1) + (size_t)2 * L1 * PTS);
-  const size_t cells_end = cell_off + (size_t)3 * n_species * PT;
-  return sizeof(float) * (c + std::max(std::max(std::max(fwd, spec), bwd), cells_end));
+  (void)n_species;
+  return sizeof(float) * (c + std::max(std::max(fwd, spec), bwd));
 }
 
 template <int L, int NTT>
@@ -1183,6 +1222,11 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
   P.n_sites = in.n_sites;
   P.compressed = in.out_ptr != nullptr;
   P.gauss = p->basis == ST_BASIS_POLY3GAUSS;
+  // radial set-up items: (species, chunk, pair) filling the 256 threads where the species are few
+  P.nch = std::max(1, 256 / (ns * 64));
+  P.qc = 0;
+  for (int q = 0; q < ns; ++q) P.qc = std::max(P.qc, (p->alpha_max[q] + P.nch - 1) / P.nch);
+  P.keep = ns * P.nch * 64 <= 256 && P.qc <= 8;  // the kernel keeps 8 radial values per thread
   double rcut_max = 0.0;
   for (int q = 0; q < ns; ++q) rcut_max = std::max(rcut_max, p->rcut_hard[q]);
   P.rcut_max = rcut_max;
