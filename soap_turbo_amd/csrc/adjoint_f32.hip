@@ -150,27 +150,6 @@ __device__ __forceinline__ int radial_cell(const SpeciesTab& s, double r, float&
   return s.seg_base[j] + i;
 }
 
-// Column m of P_l^m(x) (Condon-Shortley phase) for l = m .. LT, by the
-// recursions of get_plm_array (src/soap_turbo_angular.f90:108-160): P_m^m from
-// P_{m-1}^{m-1}, P_{m+1}^m = (2m+1) x P_m^m, then the three-term recursion in
-// l. Calls f(l, P_l^m) for each l.
-template <int LT, class F>
-__device__ __forceinline__ void plm_column(int m, float x, float sq, const float* ra, const float* rb, F&& f) {
-  float pmm = 1.f;
-  for (int j = 1; j <= m; ++j) pmm = -(float)(2 * j - 1) * sq * pmm;
-  f(m, pmm);
-  if (m + 1 > LT) return;
-  float p1 = x * (float)(2 * m + 1) * pmm, p0 = pmm;
-  f(m + 1, p1);
-  for (int l = m + 2; l <= LT; ++l) {
-    const int k = l * (l + 1) / 2 + m;
-    const float pn = ra[k] * x * p1 - rb[k] * p0;
-    p0 = p1;
-    p1 = pn;
-    f(l, pn);
-  }
-}
-
 // e^{-i m phi} by m complex multiplications from (cos phi, sin phi).
 __device__ __forceinline__ float2 exp_mphi(int m, float cf, float sf) {
   float er = 1.f, ei = 0.f;
@@ -908,8 +887,9 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
           for (int s = 0; s < KN; ++s) acc[a] = mfma4(av[s], bv[jj][s], acc[a]);
         }
         if (c < cw) {
-          const float w = (c >= 2 ? 2.f : 1.f) * fourpi;
+          // the column's weight, with the k's prefactor of A folded in
           const int k = l * (l + 1) / 2 + (c >> 1), hc = (c & 1) * KMP + k;
+          const float w = (c >= 2 ? 2.f : 1.f) * fourpi * preflm[k];
 #pragma unroll
           for (int a = 0; a < NTT; ++a)
 #pragma unroll
@@ -978,14 +958,28 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     for (int u = 0; u < PIT; ++u) {
       const int pl = tid & (PT - 1), m = __builtin_amdgcn_readfirstlane(tid >> 6) + 4 * u;  // m uniform
       if (m <= L + 1) {
-        if (pr_msk[pl] != 0 && tb + pl != p0) {
-          plm_column<L + 1>(m, pr_ct[pl], pr_st[pl], rec_a, rec_b,
-                            [&](int l, float plm) { Fp[(l * (l + 1) / 2 + m) * PTS + pl] = plm; });
-          if (m <= L) Em[m * PTS + pl] = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
-        } else {
-          for (int l = m; l <= L + 1; ++l) Fp[(l * (l + 1) / 2 + m) * PTS + pl] = 0.f;
-          if (m <= L) Em[m * PTS + pl] = make_float2(0.f, 0.f);
+        // l unrolled: the recursion's coefficients are read ahead of the chain that uses them
+        const bool live = pr_msk[pl] != 0 && tb + pl != p0;
+        const float x = pr_ct[pl], sq = pr_st[pl];
+        float pmm = 1.f;
+#pragma unroll
+        for (int j = 1; j <= L + 1; ++j)
+          if (j <= m) pmm = -(float)(2 * j - 1) * sq * pmm;
+        float q0 = 0.f, q1 = 0.f;
+#pragma unroll
+        for (int l = 0; l <= L + 1; ++l) {
+          if (l >= m) {
+            const int k = l * (l + 1) / 2 + m;
+            float pc;
+            if (l == m) pc = pmm;
+            else if (l == m + 1) pc = x * (float)(2 * m + 1) * pmm;
+            else pc = rec_a[k] * x * q1 - rec_b[k] * q0;
+            q0 = q1;
+            q1 = pc;
+            Fp[k * PTS + pl] = live ? pc : 0.f;
+          }
         }
+        if (m <= L) Em[m * PTS + pl] = live ? exp_mphi(m, pr_cf[pl], pr_sf[pl]) : make_float2(0.f, 0.f);
       }
     }
     __syncthreads();
@@ -1036,7 +1030,6 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         const int lm = act ? klm[k] : 0;
         const int l = lm & 255, m = lm >> 8;
         const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
-        const float pf = act ? preflm[k] : 0.f;
         if (act) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
@@ -1047,7 +1040,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
             const float ev = e.x * vr + e.y * vi;     // Re(E conj v)
             const float ex = e.x * ui - e.y * ur;     // Re(i E conj u)
-            const float pl = pf * Fp[(kb + m) * PTS + lp];
+            const float pl = Fp[(kb + m) * PTS + lp];
             const float pp1 = m < l ? Fp[(kb + m + 1) * PTS + lp] : 0.f;
             float dms, dsin = 0.f;
             if (m == 0) {
@@ -1057,10 +1050,10 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
               dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * Fp[(kb1 + m - 1) * PTS + lp] +
                      0.5f * Fp[(kb1 + m + 1) * PTS + lp];
             }
-            // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E
+            // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E (the prefactor is in H)
             fr[v] += pl * (gl * ev + gdl * eu);
-            fo[v] += pf * dms * gl * eu;
-            fa[v] += pf * dsin * gl * ex;
+            fo[v] += dms * gl * eu;
+            fa[v] += dsin * gl * ex;
           }
         }
       }
