@@ -213,6 +213,14 @@ int st_download_soap_f32(st_ctx* ctx, double* soap);
  * (normalised, compressed) descriptor. Needs legacy_filter_seed = 0, as the reference does. */
 int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t n_vec, const double* vecs,
                        double* hess);
+/* soap_hessian_selftest of get_soap_central_hessian (src/soap_turbo.f90:1419-1451, :1498-1499): after
+ * st_central_hessian with these vecs on the current list, the first derivatives that the Hessian's
+ * contraction also produces (v.dq/dr_jb for each caller's vector v and non-self pair, from its
+ * Omega/Q_l^m route) against the same quantity from the independent FP64 Jacobian (soap_cart_der,
+ * computed here); *err = max |difference| / max |Jacobian value|. Off the normal path: it recomputes
+ * soap_cart_der of the whole list and compares on the host. */
+int st_hessian_selftest(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t n_vec, const double* vecs,
+                        double* err);
 
 /* ---- GAP energy model (no reference counterpart: the callers' energy model that supplies get_soap
  * with dE/dq, here on the device) ----

@@ -226,6 +226,15 @@ class Context:
                                             ptr(hess)))
         return hess
 
+    def hessian_selftest(self, params: StParams, comp: StCompression, vecs) -> float:
+        """soap_hessian_selftest after central_hessian with the same vecs: the Hessian route's first
+        derivatives against the FP64 Jacobian's, max |difference| / max |value|."""
+        vecs = np.asfortranarray(vecs, dtype=np.float64)
+        err = ctypes.c_double(0.0)
+        self.check(lib().st_hessian_selftest(self.h, ctypes.byref(params), ctypes.byref(comp), vecs.shape[1],
+                                             ptr(vecs), ctypes.byref(err)))
+        return err.value
+
     # ---- GAP energy model on the device
     def set_gap_model(self, sparse_q, alpha, zeta: int, delta: float):
         """sparse_q (n_soap, n_sparse), alpha (n_sparse): E = sum_i delta^2 sum_t alpha_t (q_i . q_t)^zeta."""

@@ -55,6 +55,7 @@ SIGNATURES = {
     "st_upload_dedq": (ctypes.c_int, [P, I32, P]),
     "st_compute": (ctypes.c_int, [P, P, P, I32, I32]),
     "st_pairs_full_jacobian": (ctypes.c_int, [P]),
+    "st_hessian_selftest": (ctypes.c_int, [P, P, P, I32, P, P]),
     "st_full_jacobian_forces": (ctypes.c_int, [P, P, P, ctypes.c_int64, ctypes.POINTER(I32)]),
     "st_pairs_full_jacobian_noalloc": (ctypes.c_int, [P, I32, I32, I32, I32, P, P, P, P, P, P, P, P, P]),
     "st_contract_resident": (ctypes.c_int, [P, ctypes.c_double]),

@@ -121,6 +121,8 @@ struct st_ctx {
   DevBuf gap_T, gap_alpha, gap_esite, gap_energy;
   // central-atom Hessian (st_central_hessian): second radial derivatives, C_k, weight vectors, scratch
   DevBuf rad_der2, hess_ck, hess_vecs, hess_scratch, hess_heads, hess_res, hess_out;
+  int hess_n_vec = 0;      // n_vec of the last st_central_hessian on the current list (0: none)
+  uint64_t hess_gen = 0;   // ... and the neighbour list it ran on
   std::string g32_key, g32_warm;
   hipGraph_t g32_graph = nullptr;
   hipGraphExec_t g32_exec = nullptr;
@@ -1594,6 +1596,65 @@ int st_central_hessian(st_ctx* ctx, const st_params* p, const st_compression* c,
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(hess, ctx->hess_out.p, P * 9 * n_vec * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  ctx->hess_n_vec = n_vec;
+  ctx->hess_gen = ctx->nb_gen;
+  return ST_OK;
+}
+
+int st_hessian_selftest(st_ctx* ctx, const st_params* p, const st_compression* c, int32_t n_vec, const double* vecs,
+                        double* err) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!vecs || !err) return fail(ctx, ST_ERR_ARG, "vecs and err are required");
+  if (ctx->hess_n_vec != n_vec || ctx->hess_gen != ctx->nb_gen || !ctx->have_nb)
+    return fail(ctx, ST_ERR_STATE, "the self-test needs st_central_hessian with these vectors on this list first");
+  HIPCHK(hipSetDevice(ctx->device));
+  const int NS = ctx->n_sites, nvz = n_vec + 4;
+  const size_t P = ctx->n_pairs;
+  // the Hessian route's first derivatives: per pair, v.dp/dr_jb for the caller's vectors and for p
+  // (res), per site w.p and |p|; and whether a pair took part (its geometry head)
+  std::vector<double> res(P * nvz * 12), wpr((size_t)NS * (3 + 4 * n_vec)), sdp(NS), head(P);
+  const size_t stride = st::hessian_scratch_per_site(ctx->n_max, ctx->k_max, ctx->n_soap, ctx->n_soap_unc, n_vec);
+  const size_t o_scal = st::hessian_scal_offset(ctx->n_max, ctx->k_max, ctx->n_soap, ctx->n_soap_unc, n_vec);
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipMemcpyAsync(res.data(), ctx->hess_res.p, res.size() * 8, hipMemcpyDeviceToHost, st));
+  if (NS > 0) {
+    HIPCHK(hipMemcpy2DAsync(wpr.data(), (3 + 4 * n_vec) * 8, ctx->hess_scratch.as<double>() + o_scal, stride * 8,
+                            (3 + 4 * n_vec) * 8, NS, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sdp.data(), ctx->sdp.p, (size_t)NS * 8, hipMemcpyDeviceToHost, st));
+  }
+  if (P > 0)
+    HIPCHK(hipMemcpy2DAsync(head.data(), 8, ctx->hess_heads.p, st::hessian_geom_head() * 8, 8, P,
+                            hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // the independent first derivatives: the FP64 Jacobian of the same descriptor (soap_cart_der)
+  int rc = st_compute(ctx, p, c, 1, 0);
+  if (rc) return rc;
+  const int n_soap = ctx->n_soap;
+  std::vector<double> cart(P * n_soap * 3);
+  HIPCHK(hipMemcpyAsync(cart.data(), ctx->cart.p, cart.size() * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // v.dq/dr_jb = (v.dp)/s - (v.p)(p.dp)/s^3 from the Hessian route, against sum_d v_d soap_cart_der(b, d, k)
+  double num = 0.0, den = 0.0;
+  for (int i = 0; i < NS; ++i) {
+    const double s = sdp[i], s3 = s * s * s;
+    for (int k2 = ctx->site_ptr_h[i] + 1; k2 < ctx->site_ptr_h[i + 1]; ++k2) {
+      if (head[k2] == 0.0) continue;
+      const double* rw = res.data() + (size_t)k2 * nvz * 12;
+      for (int v = 0; v < n_vec; ++v) {
+        const double wp = wpr[(size_t)i * (3 + 4 * n_vec) + 3 + 4 * v];
+        const double* w = vecs + ((size_t)i * n_vec + v) * n_soap;
+        const double* ck = cart.data() + (size_t)k2 * n_soap * 3;
+        for (int b = 0; b < 3; ++b) {
+          double ref = 0.0;
+          for (int d = 0; d < n_soap; ++d) ref += w[d] * ck[3 * d + b];
+          const double rec = rw[v * 12 + b] / s - wp * rw[n_vec * 12 + b] / s3;
+          num = std::max(num, std::abs(rec - ref));
+          den = std::max(den, std::abs(ref));
+        }
+      }
+    }
+  }
+  *err = num / std::max(den, 1e-300);
   return ST_OK;
 }
 

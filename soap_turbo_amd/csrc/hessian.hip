@@ -807,6 +807,7 @@ size_t hessian_scratch_per_site(int n_max, int k_max, int n_soap, int n_unc, int
 }
 
 size_t hessian_heads_per_pair() { return GEOM_HEAD; }
+size_t hessian_geom_head() { return GEOM_HEAD; }
 
 size_t hessian_lds(int n_max, int k_max, int n_unc) {
   const size_t geom = (HT / 64) * sizeof(PairGeom);        // pass A
@@ -817,6 +818,12 @@ size_t hessian_lds(int n_max, int k_max, int n_unc) {
 }
 
 size_t hessian_res_per_pair(int n_vec) { return (size_t)(n_vec + 4) * 12; }
+
+size_t hessian_scal_offset(int n_max, int k_max, int n_soap, int n_unc, int n_vec) {
+  const size_t E = (size_t)n_max * k_max;
+  return 2 * E * 3 + 2 * E * (n_vec + 4) + 2 * E * 3 * (n_vec + 1) + 3 * (size_t)n_unc + 3 * (size_t)n_soap +
+         (size_t)n_soap;
+}
 
 bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int n_soap, int n_unc, int n_vec,
                     int compressed, int n_species, const double* sigma_t, const double* sigma_t_scaling,
@@ -843,7 +850,7 @@ bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_m
   P.o_d1u = P.o_zg + 2 * E * 3 * (n_vec + 1);
   P.o_d1c = P.o_d1u + 3 * (size_t)n_unc;
   P.o_pvec = P.o_d1c + 3 * (size_t)n_soap;
-  P.o_scal = P.o_pvec + n_soap;
+  P.o_scal = P.o_pvec + n_soap;  // == hessian_scal_offset(...)
   P.stride = hessian_scratch_per_site(n_max, P.k_max, n_soap, n_unc, n_vec);
   if (n_sites == 0) return true;
   if (n_pairs > 0)

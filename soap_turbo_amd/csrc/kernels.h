@@ -64,6 +64,10 @@ void launch_atom_pairs(hipStream_t st, int n_atoms, int n_pairs, const int* pair
 size_t hessian_scratch_per_site(int n_max, int k_max, int n_soap, int n_unc, int n_vec);
 size_t hessian_lds(int n_max, int k_max, int n_unc);
 size_t hessian_res_per_pair(int n_vec);  // doubles of per-pair contraction sums
+// doubles into a site's scratch slot of its per-site scalars: |p|-rule dot products, then per weight
+// vector v (w.p, w.d1 p (3)) at 3 + 4 v
+size_t hessian_scal_offset(int n_max, int k_max, int n_soap, int n_unc, int n_vec);
+size_t hessian_geom_head();  // doubles of a pair's geometry head (its first is 0 for an inactive pair)
 size_t hessian_heads_per_pair();  // doubles of per-pair geometry scratch
 bool launch_hessian(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int n_soap, int n_unc, int n_vec,
                     int compressed, int n_species, const double* sigma_t, const double* sigma_t_scaling,

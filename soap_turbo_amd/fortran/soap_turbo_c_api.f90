@@ -71,6 +71,45 @@ module soap_turbo_c_api
       integer(c_int32_t), value :: n_vec
       integer(c_int) :: st_central_hessian
     end function
+    function st_upload_neighbours(ctx, nb) bind(c, name="st_upload_neighbours")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, nb
+      integer(c_int) :: st_upload_neighbours
+    end function
+    function st_upload_atoms(ctx, n_atoms, pair_atom, vec) bind(c, name="st_upload_atoms")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, pair_atom, vec
+      integer(c_int32_t), value :: n_atoms
+      integer(c_int) :: st_upload_atoms
+    end function
+    function st_upload_dedq(ctx, n_soap, G) bind(c, name="st_upload_dedq")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, G
+      integer(c_int32_t), value :: n_soap
+      integer(c_int) :: st_upload_dedq
+    end function
+    function st_adjoint_f32(ctx, p, cmp) bind(c, name="st_adjoint_f32")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, p, cmp
+      integer(c_int) :: st_adjoint_f32
+    end function
+    function st_download_soap_f32(ctx, soap) bind(c, name="st_download_soap_f32")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, soap
+      integer(c_int) :: st_download_soap_f32
+    end function
+    function st_download_forces(ctx, which, forces, virial) bind(c, name="st_download_forces")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, forces, virial
+      integer(c_int32_t), value :: which
+      integer(c_int) :: st_download_forces
+    end function
+    function st_hessian_selftest(ctx, p, cmp, n_vec, vecs, err) bind(c, name="st_hessian_selftest")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, p, cmp, vecs, err
+      integer(c_int32_t), value :: n_vec
+      integer(c_int) :: st_hessian_selftest
+    end function
   end interface
 
 contains

@@ -121,3 +121,70 @@ def test_fortran_central_hessian_matches_reference():
     h = hess.transpose(3, 2, 0, 1)
     print(f"Fortran drop-in Hessian: rel err {rel(h, d['hess']):.2e}")
     assert rel(h, d["hess"]) <= 1e-8
+    # soap_hessian_selftest: the Hessian route's first derivatives against the Jacobian's
+    L.st_f90_hessian_selftest_err.restype = ctypes.c_double
+    L.st_f90_set_hessian_selftest(ctypes.c_int32(1))
+    try:
+        L.st_f90_central_hessian(I(w.n_sites), I(p.n_species), I(w.n_pairs), *[_p(a) for a in keep[:5]],
+                                 I(p.l_max), *[_p(a) for a in keep[5:]], I(0), I(0), _p(z), _p(z), _p(np.zeros(1)),
+                                 I(n_soap), _p(soap), I(n_vec), _p(vecs), _p(hess))
+    finally:
+        L.st_f90_set_hessian_selftest(ctypes.c_int32(0))
+    err = L.st_f90_hessian_selftest_err()
+    print(f"soap_hessian_selftest_err {err:.2e} (the reference's on its fixture: {float(d['selftest_err']):.2e})")
+    assert 0.0 < err <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,n_sites,with_xyz,sgn", [(2, 300, True, 1.0), (1, None, False, -1.0),
+                                                      (4, None, True, 1.0)])
+def test_fortran_adjoint_forces_match_fp64_oracle(cfg, n_sites, with_xyz, sgn):
+    # get_soap_adjoint_forces: get_soap's inputs, G(n_sites, n_soap), the pairs' neighbour atoms ->
+    # descriptors, forces and virial from the FP32 adjoint, against the FP64 oracle with the same W
+    import soap_turbo_amd as st
+    from oracle import oracle as orc
+    from soap_turbo_amd.workload import make_workload, normal, slab_sites
+    w = make_workload(cfg, n_sites=n_sites, sites=slab_sites(4, 150, 150) if cfg == 4 else None)
+    p = w.params
+    ps = p.per_species()
+    comp = None
+    if p.compress_mode == "trivial":
+        dim, nz = st.get_compress_indices("trivial", p.alpha_max, p.l_max, "get_dim")
+        comp = (np.zeros(nz, np.int32), np.zeros(nz, np.int32), np.zeros(nz))
+        st.get_compress_indices("trivial", p.alpha_max, p.l_max, "set_indices", *comp)
+    n_max = sum(p.alpha_max)
+    n_soap = int(comp[0].max()) if comp else n_max * (n_max + 1) // 2 * (p.l_max + 1)
+    keep = st._lib.Keep()
+    prm = st.make_params(keep, alpha_max=p.alpha_max, l_max=p.l_max, radial_enhancement=p.radial_enhancement,
+                         basis=p.basis, scaling_mode=p.scaling_mode, legacy_seed=True, **ps)
+    W, S = st.Context(0).get_basis(prm, n_max)
+    G = normal(9, (n_soap, w.n_sites))      # C (n_soap, n_sites) == G(n_sites, n_soap) F
+    n_atoms = len(w.pos)
+    soap = np.zeros((n_soap, w.n_sites), order="F")
+    forces = np.zeros((3, n_atoms), order="F")
+    virial = np.zeros((3, 3), order="F")
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    arr = [i32(w.n_neigh), i32(w.species[:, 0]), i32(w.species_multiplicity), i32(w.mask), f64(w.rjs),
+           f64(w.thetas), f64(w.phis), i32(p.alpha_max)] + [f64(ps[k]) for k in (
+               "rcut_hard", "rcut_soft", "nf", "global_scaling", "atom_sigma_r", "atom_sigma_r_scaling",
+               "atom_sigma_t", "atom_sigma_t_scaling", "amplitude_scaling")] + [f64(ps["central_weight"])]
+    cp = [i32(comp[0]), i32(comp[1]), f64(comp[2])] if comp else [np.zeros(1, np.int32)] * 2 + [np.zeros(1)]
+    nbr = i32(w.nbr + 1)
+    xyz = f64(w.vec)                          # (n_pairs, 3) C == xyz(3, n_pairs) F
+    I = ctypes.c_int32
+    L = flib()
+    L.st_f90_get_soap_adjoint_forces(
+        I(w.n_sites), I(p.n_species), I(w.n_pairs), I(1), *[_p(a) for a in arr[:8]], I(p.l_max),
+        *[_p(a) for a in arr[8:17]], I(p.radial_enhancement), _p(arr[17]),
+        I({"poly3": 0, "poly3gauss": 1, "poly3operator": 2}[p.basis]), I(0 if p.scaling_mode == "polynomial" else 1),
+        I(int(comp is not None)), I(len(cp[0]) if comp else 0), *[_p(a) for a in cp], I(n_soap), _p(soap), _p(G),
+        ctypes.c_double(sgn), _p(nbr), I(n_atoms), _p(forces), _p(virial), I(int(with_xyz)), _p(xyz), I(1))
+    qo, pair_o = orc.contract(w, W, S, G, 1.0, comp, True)
+    fo = np.zeros_like(w.pos)
+    np.add.at(fo, w.nbr, -pair_o)
+    vo = -np.einsum("ka,kb->ab", w.vec, pair_o)
+    eq, ef, ev = rel(soap.T, qo), rel(forces.T, sgn * fo), rel(virial, sgn * vo)
+    print(f"config {cfg} ({w.n_sites} sites, xyz {with_xyz}, sgn {sgn}): rel err q {eq:.2e} forces {ef:.2e} "
+          f"virial {ev:.2e}")
+    assert eq <= 1e-5 and ef <= 1e-5 and ev <= 1e-5

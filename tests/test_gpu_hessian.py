@@ -106,3 +106,30 @@ def test_hessian_refuses_poly3operator(ctx):
     w, comp, d = load("op_scalings")
     with pytest.raises(st.SoapTurboStop, match="poly3operator"):
         _run(ctx, w, comp, np.zeros((w.n_sites, 1, 10)))
+
+
+@pytest.mark.parametrize("cfg,n_sites", [(1, 60), (2, 80)])
+def test_hessian_selftest_rebuilds_first_derivatives(ctx, cfg, n_sites):
+    # soap_hessian_selftest (src/soap_turbo.f90:1419-1451): the first derivatives the Hessian's
+    # contraction also produces, against the independent FP64 Jacobian; with vectors other than the
+    # ones the Hessian ran with, the two routes must disagree (the check compares something)
+    w = make_workload(cfg, n_sites=n_sites)
+    p = w.params
+    keep = st._lib.Keep()
+    prm = _params(keep, w)
+    comp = None
+    if p.compress_mode == "trivial":
+        dim, nz = st.get_compress_indices("trivial", p.alpha_max, p.l_max, "get_dim")
+        comp = (np.zeros(nz, np.int32), np.zeros(nz, np.int32), np.zeros(nz))
+        st.get_compress_indices("trivial", p.alpha_max, p.l_max, "set_indices", *comp)
+    cmp_ = st.make_compression(keep, comp is not None, len(comp[0]) if comp else 0, *(comp or (None, None, None)))
+    n_max = sum(p.alpha_max)
+    n_soap = int(comp[0].max()) if comp else n_max * (n_max + 1) // 2 * (p.l_max + 1)
+    vecs = np.asfortranarray(normal(61, (n_soap, 2, w.n_sites)))
+    ctx.upload_neighbours(w.n_neigh, w.species.T, w.species_multiplicity, w.mask.T, w.rjs, w.thetas, w.phis)
+    ctx.central_hessian(prm, cmp_, vecs)
+    err = ctx.hessian_selftest(prm, cmp_, vecs)
+    other = ctx.hessian_selftest(prm, cmp_, np.asfortranarray(normal(62, vecs.shape)))
+    print(f"config {cfg}: self-test err {err:.2e}, with other vectors {other:.2e}")
+    assert err <= 1e-10
+    assert other >= 1e-2
