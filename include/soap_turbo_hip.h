@@ -109,6 +109,17 @@ int st_n_soap(const st_params* p, const st_compression* c);
 int st_get_basis(st_ctx* ctx, const st_params* p, double* W, double* S);
 /* Use these W and S instead of building them, while n_max matches; n_max = 0 clears. */
 int st_set_basis(st_ctx* ctx, int32_t n_max, const double* W, const double* S);
+/* How the context builds W = S^{-1/2} (get_orthonormalization_matrix_poly3[gauss],
+ * src/soap_turbo_radial.f90:1023-1219):
+ *   ST_BASIS_ROUTE_EXTENDED (the default): an eigendecomposition in x87 extended precision, accurate to
+ *     about 1e-9 at cond(S) = 1e13 (poly3, alpha_max 8);
+ *   ST_BASIS_ROUTE_LAPACK: the reference's own sequence, dgesvd -> U sqrt(Sigma) V^T (dgemm) ->
+ *     dpotrf/dpotri, on the host's LAPACK (ST_LAPACK names the library, else the MKL runtime or a
+ *     system liblapack is loaded): with the LAPACK the reference was linked with, the reference's W. */
+enum { ST_BASIS_ROUTE_EXTENDED = 0, ST_BASIS_ROUTE_LAPACK = 1 };
+int st_set_basis_route(st_ctx* ctx, int32_t route);
+/* W and S of a route for p, on the host without a device context. */
+int st_basis_matrices(int32_t route, const st_params* p, double* W, double* S);
 
 /* get_soap: soap(n_soap, n_sites) and, with do_derivatives and not contract,
  * soap_cart_der(3, n_soap, n_pairs). soap_cart_der may be NULL otherwise. */

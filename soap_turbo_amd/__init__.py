@@ -99,6 +99,10 @@ class Context:
         W = np.asfortranarray(W, dtype=np.float64); S = np.asfortranarray(S, dtype=np.float64)
         self.check(lib().st_set_basis(self.h, W.shape[0], ptr(W), ptr(S)))
 
+    def set_basis_route(self, lapack: bool):
+        """W by the reference's LAPACK sequence (True) or the extended-precision route (False, default)."""
+        self.check(lib().st_set_basis_route(self.h, 1 if lapack else 0))
+
     def clear_basis(self):
         self.check(lib().st_set_basis(self.h, 0, None, None))
 
@@ -276,6 +280,13 @@ class Context:
 
 
 _DEFAULT = None
+
+
+def basis_matrices(params: StParams, n_max: int, lapack: bool = False):
+    """W and S (n_max x n_max) on the host, no device: the reference's LAPACK route or the extended one."""
+    W = np.zeros((n_max, n_max), order="F"); S = np.zeros((n_max, n_max), order="F")
+    _check(None, lib().st_basis_matrices(1 if lapack else 0, ctypes.byref(params), ptr(W), ptr(S)))
+    return W, S
 
 
 def default_context() -> Context:

@@ -43,6 +43,8 @@ SIGNATURES = {
     "st_n_soap": (ctypes.c_int, [P, P]),
     "st_get_basis": (ctypes.c_int, [P, P, P, P]),
     "st_set_basis": (ctypes.c_int, [P, I32, P, P]),
+    "st_set_basis_route": (ctypes.c_int, [P, I32]),
+    "st_basis_matrices": (ctypes.c_int, [I32, P, P, P]),
     "st_get_soap": (ctypes.c_int, [P, P, P, P, I32, I32, I32, P, P]),
     "st_soap_contract": (ctypes.c_int, [P, P, P, ctypes.c_double, P]),
     "st_get_compress_indices": (ctypes.c_int, [ctypes.c_char_p, I32, P, I32, P, P, P, P, P, I32]),

@@ -64,7 +64,8 @@ struct st_ctx {
   double* scratch = nullptr;   // this device's copy of the no-alloc scratch array
   std::string err;
 
-  // basis: injected or built (cache key = parameters that change W and S)
+  // basis: injected or built (cache key = parameters that change W and S); route of W
+  int basis_route = ST_BASIS_ROUTE_EXTENDED;
   int inj_n_max = 0;
   std::vector<double> injW, injS;
   std::string basis_key;
@@ -286,6 +287,7 @@ int basis_for(st_ctx* ctx, const st_params* p) {
   // the reference rebuilds W when the basis name or alpha_max change, and for
   // poly3gauss also atom_sigma_r and rcut_hard (src/soap_turbo.f90:294-354)
   std::string key(reinterpret_cast<const char*>(&p->basis), sizeof(int32_t));
+  key.append(reinterpret_cast<const char*>(&ctx->basis_route), sizeof(int));
   key.append(reinterpret_cast<const char*>(p->alpha_max), sizeof(int32_t) * p->n_species);
   if (p->basis == ST_BASIS_POLY3GAUSS) {
     key.append(reinterpret_cast<const char*>(p->atom_sigma_r), sizeof(double) * p->n_species);
@@ -299,8 +301,12 @@ int basis_for(st_ctx* ctx, const st_params* p) {
     const int a = p->alpha_max[s];
     std::vector<double> Sb((size_t)a * a), Wb((size_t)a * a);
     st::overlap_matrix(p->basis, a, p->atom_sigma_r[s], p->rcut_hard[s], Sb.data());
-    if (!st::orthonormalization_matrix(a, Sb.data(), Wb.data()))
+    if (ctx->basis_route == ST_BASIS_ROUTE_LAPACK) {
+      std::string msg;
+      if (!st::orthonormalization_matrix_lapack(a, Sb.data(), Wb.data(), msg)) return fail(ctx, ST_ERR_STOP, msg);
+    } else if (!st::orthonormalization_matrix(a, Sb.data(), Wb.data())) {
       return fail(ctx, ST_ERR_STOP, "the overlap matrix of the radial basis is not positive definite");
+    }
     for (int j = 0; j < a; ++j)
       for (int i = 0; i < a; ++i) {
         ctx->W[(off + i) + (size_t)(off + j) * n_max] = Wb[i + j * a];
@@ -654,6 +660,26 @@ int st_get_basis(st_ctx* ctx, const st_params* p, double* W, double* S) {
   if ((rc = basis_for(ctx, p))) return rc;
   if (W) std::memcpy(W, ctx->W.data(), ctx->W.size() * 8);
   if (S) std::memcpy(S, ctx->S.data(), ctx->S.size() * 8);
+  return ST_OK;
+}
+
+int st_basis_matrices(int32_t route, const st_params* p, double* W, double* S) {
+  if (route != ST_BASIS_ROUTE_EXTENDED && route != ST_BASIS_ROUTE_LAPACK) return ST_ERR_ARG;
+  st_ctx tmp;  // host-only: no device is touched
+  tmp.basis_route = route;
+  int rc = check_params(&tmp, p);
+  if (!rc) rc = basis_for(&tmp, p);
+  if (rc) return rc;
+  if (W) std::memcpy(W, tmp.W.data(), tmp.W.size() * 8);
+  if (S) std::memcpy(S, tmp.S.data(), tmp.S.size() * 8);
+  return ST_OK;
+}
+
+int st_set_basis_route(st_ctx* ctx, int32_t route) {
+  if (!ctx) return ST_ERR_ARG;
+  if (route != ST_BASIS_ROUTE_EXTENDED && route != ST_BASIS_ROUTE_LAPACK)
+    return fail(ctx, ST_ERR_ARG, "no such route for W");
+  ctx->basis_route = route;
   return ST_OK;
 }
 

@@ -12,8 +12,13 @@
 // alpha_max = 8, cond(S) = 1.3e13 and a double-precision route loses ~1e-4 of
 // W's accuracy; in extended precision W matches S^{-1/2} to ~1e-9, and only
 // the final rounding to double remains.
+#include <dlfcn.h>
+
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "st_internal.h"
@@ -141,6 +146,89 @@ bool orthonormalization_matrix(int a, const double* S, double* W) {
       for (int k = (i > j ? i : j); k < a; ++k) s += Ui[i + k * a] * Ui[j + k * a];
       W[i + j * a] = (double)s;
     }
+  return true;
+}
+
+namespace {
+
+// LAPACK/BLAS entry points with the Fortran calling convention (LP64 integers, the hidden lengths
+// of the character arguments last).
+using dgesvd_fn = void (*)(const char*, const char*, const int*, const int*, double*, const int*, double*, double*,
+                           const int*, double*, const int*, double*, const int*, int*, size_t, size_t);
+using dgemm_fn = void (*)(const char*, const char*, const int*, const int*, const int*, const double*, const double*,
+                          const int*, const double*, const int*, const double*, double*, const int*, size_t, size_t);
+using dpotr_fn = void (*)(const char*, const int*, double*, const int*, int*, size_t);
+
+struct Lapack {
+  dgesvd_fn gesvd = nullptr;
+  dgemm_fn gemm = nullptr;
+  dpotr_fn potrf = nullptr, potri = nullptr;
+  std::string err;
+};
+
+// The host's LAPACK, loaded at first use: ST_LAPACK names the library, else the image's MKL
+// runtime or a system liblapack.
+const Lapack& lapack() {
+  static Lapack L;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    std::vector<std::string> names;
+    if (const char* e = std::getenv("ST_LAPACK")) names.push_back(e);
+    for (const char* n : {"libmkl_rt.so.1", "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1", "liblapack.so.3",
+                          "libopenblas.so.0"})
+      names.push_back(n);
+    for (const std::string& n : names) {
+      void* h = dlopen(n.c_str(), RTLD_NOW | RTLD_LOCAL);
+      if (!h) continue;
+      L.gesvd = reinterpret_cast<dgesvd_fn>(dlsym(h, "dgesvd_"));
+      L.gemm = reinterpret_cast<dgemm_fn>(dlsym(h, "dgemm_"));
+      L.potrf = reinterpret_cast<dpotr_fn>(dlsym(h, "dpotrf_"));
+      L.potri = reinterpret_cast<dpotr_fn>(dlsym(h, "dpotri_"));
+      if (L.gesvd && L.gemm && L.potrf && L.potri) return;
+      L.gesvd = nullptr;
+      dlclose(h);
+    }
+    L.err = "no LAPACK library found (set ST_LAPACK to one that exports dgesvd_, dgemm_, dpotrf_, dpotri_)";
+  });
+  return L;
+}
+
+}  // namespace
+
+// W = S^{-1/2} by the reference's own LAPACK sequence (get_orthonormalization_matrix_poly3[gauss],
+// src/soap_turbo_radial.f90:1052-1080, :1177-1205): the SVD of S, S^{1/2} = U sqrt(Sigma) V^T by two
+// GEMMs, then its inverse from the Cholesky factor (dpotrf, dpotri), the upper triangle mirrored.
+// With the LAPACK the reference was linked with, W is the reference's W.
+bool orthonormalization_matrix_lapack(int a, const double* S, double* W, std::string& msg) {
+  const Lapack& L = lapack();
+  if (!L.gesvd) {
+    msg = L.err;
+    return false;
+  }
+  std::vector<double> Sw(S, S + (size_t)a * a), U((size_t)a * a), VT((size_t)a * a), Sc((size_t)a * a), svd(a),
+      work((size_t)6 * a);
+  const int lwork = 6 * a;
+  int info = 0;
+  L.gesvd("A", "A", &a, &a, Sw.data(), &a, svd.data(), U.data(), &a, VT.data(), &a, work.data(), &lwork, &info, 1, 1);
+  if (info != 0) {
+    msg = "dgesvd failed on the overlap matrix (info " + std::to_string(info) + ")";
+    return false;
+  }
+  std::fill(Sw.begin(), Sw.end(), 0.0);
+  for (int i = 0; i < a; ++i) Sw[i + (size_t)i * a] = std::sqrt(svd[i]);
+  const double one = 1.0, zero = 0.0;
+  L.gemm("N", "N", &a, &a, &a, &one, U.data(), &a, Sw.data(), &a, &zero, Sc.data(), &a, 1, 1);
+  L.gemm("N", "N", &a, &a, &a, &one, Sc.data(), &a, VT.data(), &a, &zero, Sw.data(), &a, 1, 1);
+  L.potrf("U", &a, Sw.data(), &a, &info, 1);
+  if (info != 0) {
+    msg = "the overlap matrix of the radial basis is not positive definite (dpotrf info " + std::to_string(info) + ")";
+    return false;
+  }
+  L.potri("U", &a, Sw.data(), &a, &info, 1);
+  for (int i = 0; i < a; ++i) {
+    W[i + (size_t)i * a] = Sw[i + (size_t)i * a];
+    for (int j = i + 1; j < a; ++j) W[i + (size_t)j * a] = W[j + (size_t)i * a] = Sw[i + (size_t)j * a];
+  }
   return true;
 }
 

@@ -40,5 +40,7 @@ struct AngularConsts {
 
 void overlap_matrix(int basis, int a, double atom_sigma_r, double rcut_hard, double* S);
 bool orthonormalization_matrix(int a, const double* S, double* W);
+// the reference's LAPACK sequence on the host's LAPACK (false and msg if unavailable or failing)
+bool orthonormalization_matrix_lapack(int a, const double* S, double* W, std::string& msg);
 
 }  // namespace st

@@ -6,6 +6,7 @@ module soap_turbo_c_api
 
   integer(c_int), parameter :: ST_OK = 0, ST_ERR_ARG = 1, ST_ERR_HIP = 2, ST_ERR_UNSUPPORTED = 3, &
                                ST_ERR_STATE = 4, ST_ERR_STOP = 5
+  integer(c_int32_t), parameter :: ST_BASIS_ROUTE_EXTENDED = 0, ST_BASIS_ROUTE_LAPACK = 1
 
   type, bind(c) :: st_params
     integer(c_int32_t) :: n_species, l_max, radial_enhancement, basis, scaling_mode, legacy_filter_seed
@@ -70,6 +71,12 @@ module soap_turbo_c_api
       type(c_ptr), value :: ctx, p, cmp, vecs, hess
       integer(c_int32_t), value :: n_vec
       integer(c_int) :: st_central_hessian
+    end function
+    function st_set_basis_route(ctx, route) bind(c, name="st_set_basis_route")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: route
+      integer(c_int) :: st_set_basis_route
     end function
     function st_upload_neighbours(ctx, nb) bind(c, name="st_upload_neighbours")
       import :: c_ptr, c_int

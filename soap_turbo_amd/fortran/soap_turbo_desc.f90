@@ -12,7 +12,7 @@ module soap_turbo_desc
   use, intrinsic :: iso_fortran_env, only: real64
   use, intrinsic :: iso_c_binding
   use soap_turbo_c_api
-  use soap_turbo_radial, only: legacy_filter_seed
+  use soap_turbo_radial, only: legacy_filter_seed, extended_precision_w
   implicit none
   private
   integer, parameter :: dp = real64
@@ -193,6 +193,7 @@ contains
     iscal = 0
     if( scaling_mode /= "polynomial" ) iscal = 1
     call ensure_context()
+    call check(st_set_basis_route(ctx, merge(ST_BASIS_ROUTE_EXTENDED, ST_BASIS_ROUTE_LAPACK, extended_precision_w)))
 
     pk_am = alpha_max(1:n_species)
     pk_rch = rcut_hard(1:n_species); pk_rcs = rcut_soft(1:n_species); pk_nf = nf(1:n_species)

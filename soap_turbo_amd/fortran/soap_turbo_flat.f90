@@ -5,7 +5,7 @@
 module soap_turbo_flat
   use, intrinsic :: iso_c_binding
   use soap_turbo_desc
-  use soap_turbo_radial, only: legacy_filter_seed
+  use soap_turbo_radial, only: legacy_filter_seed, extended_precision_w
   use soap_turbo_compress_module
   implicit none
 
@@ -139,6 +139,11 @@ contains
     end if
     dim = d
     P_nonzero = nz
+  end subroutine
+
+  subroutine st_f90_set_extended_w(on) bind(c, name="st_f90_set_extended_w")
+    integer(c_int32_t), value :: on
+    extended_precision_w = on /= 0
   end subroutine
 
   subroutine st_f90_set_hessian(on) bind(c, name="st_f90_set_hessian")

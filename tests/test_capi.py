@@ -179,3 +179,28 @@ def test_own_basis_is_the_exact_inverse_square_root(name):
     from tests.golden_cases import rel
     assert rel(W, Wex) <= 1e-7
     assert rel(W, d["W"]) <= 1e-5
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if not c.startswith("op_")])
+def test_lapack_route_w_is_the_reference_w(name):
+    # W by the reference's own sequence (dgesvd -> U sqrt(Sigma) V^T -> dpotrf/dpotri), host-only, on the
+    # host's LAPACK: bit for bit the W of the reference run on the same host and the same LAPACK (MKL
+    # picks its code path per CPU, so the fixtures' W, written on another host, may differ in the
+    # last bits, amplified by cond(S))
+    from oracle import refio
+    w, comp, d = load(name)
+    p = w.params
+    keep = st._lib.Keep()
+    prm = st.make_params(keep, alpha_max=p.alpha_max, l_max=p.l_max, radial_enhancement=p.radial_enhancement,
+                         basis=p.basis, scaling_mode=p.scaling_mode, **p.per_species())
+    try:
+        W, S = st.basis_matrices(prm, sum(p.alpha_max), lapack=True)
+    except st.SoapTurboError as e:
+        pytest.skip(f"no LAPACK on this host: {e}")
+    assert np.array_equal(S, d["S"])
+    if not refio.available():
+        pytest.skip("the reference harness (oracle/_ref) is not built")
+    Wr = refio.run(w, modes=refio.MODE_DESC, legacy_seed=bool(d["legacy_seed"]))["W"]
+    print(f"{name}: W bitwise equal to the same-host reference's: {np.array_equal(W, Wr)}; "
+          f"to the fixture's: {np.array_equal(W, d['W'])}")
+    assert np.array_equal(W, Wr)

@@ -70,32 +70,63 @@ def run_fortran_get_soap(L, w, comp, d, do_der, contract, legacy):
     return soap, cart
 
 
+def same_host_reference(w, d, modes, legacy, vecs=None):
+    """The reference (oracle/_ref, compiled from its sources) on this host and these inputs, or None.
+
+    W = S^{-1/2} depends on the host's LAPACK code path (MKL dispatches per CPU) and amplifies its last
+    bits by cond(S), so the drop-in's default W (the reference's LAPACK sequence on the same library)
+    is compared with the reference run on the same host."""
+    from oracle import refio
+    if not refio.available():
+        return None
+    return refio.run(w, d.get("G"), modes=modes, legacy_seed=legacy, vecs=vecs)
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["cfg1_legacy", "cfg2_unified", "var_poly3_scalings", "op_trivial"])
-def test_fortran_get_soap_matches_reference(name):
-    # the drop-in builds its own W (FP64 eigen-route), so parity with the reference's W is
-    # limited by the conditioning of S: 1e-4 at poly3 alpha_max = 8, 1e-6 otherwise
+@pytest.mark.parametrize("name,extended", [("cfg1_legacy", False), ("cfg2_unified", False),
+                                           ("var_poly3_scalings", False), ("op_trivial", False),
+                                           ("cfg1_legacy", True), ("cfg2_unified", True)])
+def test_fortran_get_soap_matches_reference(name, extended):
+    # By default the drop-in builds W by the reference's own LAPACK sequence on the host's LAPACK, so
+    # its outputs meet the reference's, run on the same host, at the FP64 bar (1e-10). poly3operator:
+    # the reference reads its tabulated W there, which this build does not (1e-6). With
+    # extended_precision_w, W is the library's extended-precision S^{-1/2}, which differs from the
+    # reference's by the conditioning of S: 1e-4 at poly3 alpha_max = 8, 1e-6 otherwise, against the
+    # fixture the reference wrote.
+    from oracle import refio
     w, comp, d = load(name)
     legacy = bool(d["legacy_seed"])
     L = flib()
-    tol = 1e-4 if name.startswith("cfg1") else 1e-6
-    soap, cart = run_fortran_get_soap(L, w, comp, d, True, False, legacy)
-    assert rel(soap.T, d["soap"]) <= tol
-    assert rel(cart.transpose(2, 1, 0)[:len(d["cart_head"])], d["cart_head"]) <= tol
-    q, _ = run_fortran_get_soap(L, w, comp, d, True, True, legacy)
-    assert rel(q.T, d["soap_contract_q"]) <= tol
-    pair = np.zeros((3, w.n_pairs), order="F")
-    G = np.asfortranarray(d["G"].T)
-    L.st_f90_soap_contract(ctypes.c_int32(w.n_sites), ctypes.c_int32(q.shape[0]), ctypes.c_int32(w.n_pairs), _p(q),
-                           _p(G), ctypes.c_double(1.0), _p(pair))
-    assert rel(pair.T, d["pair"]) <= tol
+    exp = {"soap": d["soap"], "cart": d["cart_head"], "q": d["soap_contract_q"], "pair": d["pair"]}
+    if extended:
+        tol = 1e-4 if name.startswith("cfg1") else 1e-6
+    else:
+        tol = 1e-6 if name.startswith("op_") else 1e-10
+        o = same_host_reference(w, d, refio.MODE_FULL | refio.MODE_CONTRACT, legacy)
+        if o is None:
+            pytest.skip("the reference harness (oracle/_ref) is not built")
+        exp = {"soap": o["soap"], "cart": o["soap_cart_der"], "q": o["soap_contract_q"], "pair": o["pair"]}
+    L.st_f90_set_extended_w(ctypes.c_int32(int(extended)))
+    try:
+        soap, cart = run_fortran_get_soap(L, w, comp, d, True, False, legacy)
+        q, _ = run_fortran_get_soap(L, w, comp, d, True, True, legacy)
+        pair = np.zeros((3, w.n_pairs), order="F")
+        G = np.asfortranarray(d["G"].T)
+        L.st_f90_soap_contract(ctypes.c_int32(w.n_sites), ctypes.c_int32(q.shape[0]), ctypes.c_int32(w.n_pairs),
+                               _p(q), _p(G), ctypes.c_double(1.0), _p(pair))
+    finally:
+        L.st_f90_set_extended_w(ctypes.c_int32(0))
+    errs = [rel(soap.T, exp["soap"]), rel(cart.transpose(2, 1, 0)[:len(exp["cart"])], exp["cart"]),
+            rel(q.T, exp["q"]), rel(pair.T, exp["pair"])]
+    print(f"{name} (extended W {extended}): soap {errs[0]:.2e} soap_cart_der {errs[1]:.2e} "
+          f"contract q {errs[2]:.2e} pair {errs[3]:.2e}")
+    assert max(errs) <= tol
 
 
 @pytest.mark.gpu
 def test_fortran_central_hessian_matches_reference():
     # soap_hessian_enabled, get_soap with derivatives, then get_soap_central_hessian, as a
-    # reference caller does; the drop-in's own W (poly3gauss, alpha_max 4 and 3) limits parity
-    # with the reference's fixture to about 1e-8
+    # reference caller does; W by the reference's LAPACK sequence (the drop-in's default)
     import os as _os
     w, comp, d = load(_os.path.join("hessian", "hess_poly3gauss"))
     L = flib()
@@ -119,8 +150,12 @@ def test_fortran_central_hessian_matches_reference():
                              *[_p(a) for a in keep[5:]], I(0), I(0), _p(z), _p(z), _p(np.zeros(1)), I(n_soap),
                              _p(soap), I(n_vec), _p(vecs), _p(hess))
     h = hess.transpose(3, 2, 0, 1)
-    print(f"Fortran drop-in Hessian: rel err {rel(h, d['hess']):.2e}")
-    assert rel(h, d["hess"]) <= 1e-8
+    from oracle import refio
+    o = same_host_reference(w, d, refio.MODE_HESSIAN, False, vecs=d["vecs"])
+    want = d["hess"] if o is None else o["hess"]
+    print(f"Fortran drop-in Hessian: rel err {rel(h, want):.2e} (against the reference on "
+          f"{'this host' if o is not None else 'the fixture host'})")
+    assert rel(h, want) <= (1e-10 if o is not None else 1e-8)
     # soap_hessian_selftest: the Hessian route's first derivatives against the Jacobian's
     L.st_f90_hessian_selftest_err.restype = ctypes.c_double
     L.st_f90_set_hessian_selftest(ctypes.c_int32(1))
