@@ -9,10 +9,12 @@
  *   st_get_soap             get_soap, src/soap_turbo.f90:139-169 (argument list, output layout and
  *                           caller contract: soap and soap_cart_der zero on entry)
  *   st_soap_contract        soap_contract, src/soap_turbo.f90:1723-1730
- *   st_get_compress_indices get_compress_indices, src/soap_turbo_compress.f90:44-54 ("trivial" mode)
+ *   st_get_compress_indices get_compress_indices, src/soap_turbo_compress.f90:44-298 ("trivial" and the
+ *                           nu_R_nu_S modes "0_0" .. "2_2")
  *   st_set_basis            the save'd W/S cache of get_soap, src/soap_turbo.f90:294-388
  *                           (lets a caller inject the W the reference would have built)
- *   st_get_basis            get_orthonormalization_matrix_poly3[gauss], src/soap_turbo_radial.f90:1023, :1113
+ *   st_get_basis, st_set_basis_route, st_basis_matrices
+ *                           get_orthonormalization_matrix_poly3[gauss], src/soap_turbo_radial.f90:1023, :1113
  *   st_upload_neighbours, st_compute, st_download, st_upload_dedq, st_pairs_full_jacobian,
  *   st_adjoint_f32, st_download_pairs, st_download_forces
  *                           the same get_soap / soap_contract split into upload, device-resident
@@ -46,12 +48,12 @@ enum {
   ST_OK = 0,
   ST_ERR_ARG = 1,          /* invalid argument or size */
   ST_ERR_HIP = 2,          /* HIP runtime failure */
-  ST_ERR_UNSUPPORTED = 3,  /* a case this build does not implement (l_max > 12, poly3operator, ...) */
+  ST_ERR_UNSUPPORTED = 3,  /* a case this build does not implement (l_max > 12, the poly3tab basis, ...) */
   ST_ERR_STATE = 4,        /* call out of order, e.g. soap_contract before get_soap(contract=.true.) */
   ST_ERR_STOP = 5          /* a condition the reference stops on (alpha_max limits, ...) */
 };
 
-enum { ST_BASIS_POLY3 = 0, ST_BASIS_POLY3GAUSS = 1, ST_BASIS_POLY3OPERATOR = 2 /* FP64 path only */ };
+enum { ST_BASIS_POLY3 = 0, ST_BASIS_POLY3GAUSS = 1, ST_BASIS_POLY3OPERATOR = 2 };
 enum { ST_SCALING_POLYNOMIAL = 0, ST_SCALING_OTHER = 1 };
 
 /* Descriptor hyper-parameters (get_soap arguments; one entry per species for the arrays). */
