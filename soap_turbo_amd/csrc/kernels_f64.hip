@@ -833,7 +833,7 @@ __global__ void normalise_k(int n_sites, int n_soap, const double* __restrict__ 
 #define ST_DER_WAVES3 2
 #endif
 #ifndef ST_DER_NT256_MAX
-#define ST_DER_NT256_MAX 768
+#define ST_DER_NT256_MAX 1024
 #endif
 
 template <int NSPT>
@@ -939,6 +939,9 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     const double* __restrict__ rad_der, const double2* __restrict__ ang, const double2* __restrict__ ang_r,
     const double2* __restrict__ ang_a, const double2* __restrict__ ang_p, const double* __restrict__ soap,
     const double* __restrict__ sqrt_dot_p, const double* __restrict__ rotp, double* __restrict__ cart) {
+  // fused multiply-adds here: the Jacobian's short sums change only in their last bits (the file's other
+  // kernels keep the reference's separate roundings)
+#pragma clang fp contract(fast)
   using Sh = DerShape<NSPT>;
   constexpr int PB = Sh::PB, PQS = Sh::PQS, RS = Sh::RS, AS = Sh::AS, NRED = Sh::NRED;
   constexpr int FA = (PB * 4 * 48 + NT - 1) / NT, FR = (PB * 2 * 32 + NT - 1) / NT;
@@ -1096,7 +1099,6 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
       }
     }
     __syncthreads();
-    if (more) commit(pb + PB, buf ^ 1);
     // pass 1: the components of every pair of the batch, and q . d
     double v[PB][NSPT][3];
     double dots[NRED];
@@ -1166,6 +1168,10 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
         }
       }
     }
+    // the next batch's inputs into the other buffer, as late as possible: their loads, issued at the
+    // start of this batch, have had the whole batch to arrive (the buffer was last read by the
+    // batch before, which every thread finished at this batch's first barrier)
+    if (more) commit(pb + PB, buf ^ 1);
   }
 #pragma unroll
   for (int u = 0; u < NSPT; ++u) {
@@ -1590,7 +1596,7 @@ static size_t jacobian_wave_lds(int n_max, int l_max, int k_max, int n_soap) {
 static int jacobian_wave_nsw(int n_soap) { return n_soap <= 384 ? 6 : n_soap <= 640 ? 10 : n_soap <= 832 ? 13 : 0; }
 
 // (outputs per thread, threads per block) shapes of the Jacobian kernel for n_soap outputs: the
-// default first (256 threads up to 768 outputs, 512 threads up to 3072), then the other thread
+// default first (256 threads up to 1024 outputs, 512 threads up to 3072), then the other thread
 // count when a compiled shape covers n_soap with it.
 static int der_shapes(int n_soap, int (*shape)[2]) {
   int n = 0;
@@ -1598,7 +1604,7 @@ static int der_shapes(int n_soap, int (*shape)[2]) {
   for (int nt : {first, 768 - first}) {
     int nspt = (n_soap + nt - 1) / nt;
     if (nspt == 5) nspt = 6;
-    const bool compiled = nt == 256 ? nspt <= 3 : (nspt >= 2 && nspt <= 6);
+    const bool compiled = nt == 256 ? nspt <= 4 : (nspt >= 2 && nspt <= 6);
     if (compiled) {
       shape[n][0] = nspt;
       shape[n][1] = nt;
@@ -1700,7 +1706,7 @@ bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int
                                                          ang, ang_r, ang_a, ang_p, soap, sqrt_dot_p, rotp, cart); \
     return true;                                                                                                  \
   }
-  DL(1, 256) DL(2, 256) DL(3, 256) DL(2, 512) DL(3, 512) DL(4, 512) DL(6, 512)
+  DL(1, 256) DL(2, 256) DL(3, 256) DL(4, 256) DL(2, 512) DL(3, 512) DL(4, 512) DL(6, 512)
 #undef DL
   return false;
 }
