@@ -1172,6 +1172,9 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     // start of this batch, have had the whole batch to arrive (the buffer was last read by the
     // batch before, which every thread finished at this batch's first barrier)
     if (more) commit(pb + PB, buf ^ 1);
+    // the loop-top barrier publishes these writes only once they have completed; the compiler
+    // leaves out this wait on the loop's back edge
+    __builtin_amdgcn_s_waitcnt(0xc07f);
   }
 #pragma unroll
   for (int u = 0; u < NSPT; ++u) {
