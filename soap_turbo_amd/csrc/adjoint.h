@@ -25,6 +25,7 @@ struct AdjointState {
   std::string table_key;
   AdjBuf rtab, atab, tabinfo;
   int n_seg = 0;
+  int rt_off[8] = {}, at_off[8] = {};  // per species: where its tables start (equal tables shared)
   // per-call work and outputs
   AdjBuf soap, sdp, pair, forces, virial_part, virial, G32, order, tickets;
   bool order_ok = false;                 // order holds the sites of the current neighbour list

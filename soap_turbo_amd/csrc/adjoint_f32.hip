@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -37,6 +38,7 @@ namespace {
 constexpr int NTAB = 2048;        // grid cells over rcut for the tables
 constexpr int MAXSEG = 8;  // radial table segments per species (poly3operator: up to 7)
 constexpr int ADJ_MAX_SPECIES = 8;
+static_assert(sizeof(AdjointState::rt_off) / sizeof(int) >= ADJ_MAX_SPECIES, "table offsets per species");
 constexpr int PT = 64;            // pairs per tile
 constexpr int NTHREADS = 256;
 // set-up items are (row, pair) = (wavefront + 4 u, lane): a tile's pairs are one wavefront's lanes
@@ -211,6 +213,40 @@ __device__ __forceinline__ float row_total(float x) {
 constexpr int PTS = PT + 1;  // LDS stride of pair-fastest rows (odd: adjacent rows on other banks)
 constexpr int PTF = PT + 4;  // ... of the forward MFMA operand rows (4 mod 64 words)
 
+// Columns m = 0 .. nm - 1 of P_l^m (l = m .. nm - 1, so column m holds nm - m entries) dealt to the
+// four wavefronts longest first, each to the least loaded one. Word u: byte w is the m of
+// wavefront w's item u (0xff: none). Items: the most columns any wavefront takes.
+constexpr uint32_t m_deal(int nm, int u) {
+  int load[4] = {0, 0, 0, 0}, cnt[4] = {0, 0, 0, 0};
+  uint32_t word = 0xffffffffu;
+  for (int m = 0; m < nm; ++m) {
+    int b = 0;
+    for (int w = 1; w < 4; ++w)
+      if (load[w] < load[b]) b = w;
+    if (cnt[b] == u) word = (word & ~(0xffu << (8 * b))) | ((uint32_t)m << (8 * b));
+    ++cnt[b];
+    load[b] += nm - m;
+  }
+  return word;
+}
+constexpr int m_deal_items(int nm) {
+  int u = 0;
+  while (m_deal(nm, u) != 0xffffffffu) ++u;
+  return u;
+}
+static_assert(m_deal_items(9) == 3 && m_deal(9, 0) == 0x03020100u && m_deal(9, 1) == 0x04050607u &&
+                  m_deal(9, 2) == 0xffffff08u,
+              "l_max 8: m {0, 7, 8}, {1, 6}, {2, 5}, {3, 4} (12, 11, 11, 11 entries)");
+
+// The m of wavefront w's item u (0xff: none), from the dealt words of nm columns.
+template <int NM>
+__device__ __forceinline__ int dealt_m(int u, int w) {
+  constexpr uint32_t W0 = m_deal(NM, 0), W1 = m_deal(NM, 1), W2 = m_deal(NM, 2), W3 = m_deal(NM, 3);
+  static_assert(m_deal_items(NM) <= 4, "at most four columns per wavefront");
+  const uint32_t word = u == 0 ? W0 : u == 1 ? W1 : u == 2 ? W2 : W3;
+  return (int)((word >> (8 * w)) & 0xffu);
+}
+
 template <int L>
 struct SiteShape {
   static constexpr int KM = (L + 1) * (L + 2) / 2;    // (l, m) with m <= l <= L
@@ -260,7 +296,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   constexpr int QK = 3 * QB;                        // radial functions per item (alpha_max <= 11)
   constexpr int QKEEP = 8;                          // ... kept in registers for the backward pass
   constexpr int GIT = (L1 + 3) / 4;                 // l per (pair, l-chunk) item, l <= L
-  constexpr int PIT = (L + 5) / 4;                  // (m, pair) items per thread, m <= L + 1
+  constexpr int FIT = m_deal_items(L + 1);          // forward (m, pair) items per thread, m <= L
+  constexpr int PIT = m_deal_items(L + 2);          // backward (m, pair) items per thread, m <= L + 1
   constexpr uint32_t SELF = 1u << 31;               // pair flag: the site's own pair
   extern __shared__ float smem[];
   __shared__ float red[16];
@@ -524,11 +561,12 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     gl_items(tid, Gt, nullptr, KEEP && tb == last_tb);
     __syncthreads();
     // (m, pair) items: A_k = preflm P_l^m g_l e^{-i m phi} for l = m .. L, as (re, im) rows.
-    // m is uniform over a wavefront; l is unrolled, P_l^m by the recursions of get_plm_array
-    // (src/soap_turbo_angular.f90:108-160): P_m^m, P_{m+1}^m = (2m+1) x P_m^m, three-term in l.
+    // m is uniform over a wavefront (columns dealt by length, m_deal); l is unrolled, P_l^m by the
+    // recursions of get_plm_array (src/soap_turbo_angular.f90:108-160): P_m^m, P_{m+1}^m =
+    // (2m+1) x P_m^m, three-term in l.
 #pragma unroll
-    for (int u = 0; u < GIT; ++u) {
-      const int pl = tid & (PT - 1), m = __builtin_amdgcn_readfirstlane(tid >> 6) + 4 * u;  // m uniform
+    for (int u = 0; u < FIT; ++u) {
+      const int pl = tid & (PT - 1), m = dealt_m<L1>(u, __builtin_amdgcn_readfirstlane(tid >> 6));  // m uniform
       if (m < L1) {
         const bool live = pr_msk[pl] != 0;
         float gv[L1];
@@ -873,19 +911,25 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         while (rem >= (2 * (l + 1) + 15) / 16) { rem -= (2 * (l + 1) + 15) / 16; ++l; }
         const int cw = 2 * (l + 1), c = 16 * rem + li;
         floatx4 acc[NTT];
+        // the row tiles' chains over n' < 4 K (K steps of 4; the steps past n_max add nothing)
+        auto chains = [&](auto kn) {
+          constexpr int K = decltype(kn)::value;
 #pragma unroll
-        for (int a = 0; a < NTT; ++a) {
-          const int n = 16 * a + li;
-          float av[KN];
+          for (int a = 0; a < NTT; ++a) {
+            const int n = 16 * a + li;
+            float av[K];
 #pragma unroll
-          for (int s = 0; s < KN; ++s) {
-            const int np = 4 * s + lk;
-            av[s] = (n < n_max && np < n_max) ? Y[(l * n_max + n) * n_max + np] : 0.f;
+            for (int s = 0; s < K; ++s) {
+              const int np = 4 * s + lk;
+              av[s] = (n < n_max && np < n_max) ? Y[(l * n_max + n) * n_max + np] : 0.f;
+            }
+            acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < K; ++s) acc[a] = mfma4(av[s], bv[jj][s], acc[a]);
           }
-          acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int s = 0; s < KN; ++s) acc[a] = mfma4(av[s], bv[jj][s], acc[a]);
-        }
+        };
+        if (n_max <= 4 * (KN - 2)) chains(std::integral_constant<int, KN - 2>{});
+        else chains(std::integral_constant<int, KN>{});
         if (c < cw) {
           // the column's weight, with the k's prefactor of A folded in
           const int k = l * (l + 1) / 2 + (c >> 1), hc = (c & 1) * KMP + k;
@@ -956,7 +1000,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
 #pragma unroll
     for (int u = 0; u < PIT; ++u) {
-      const int pl = tid & (PT - 1), m = __builtin_amdgcn_readfirstlane(tid >> 6) + 4 * u;  // m uniform
+      const int pl = tid & (PT - 1), m = dealt_m<L + 2>(u, __builtin_amdgcn_readfirstlane(tid >> 6));  // m uniform
       if (m <= L + 1) {
         // l unrolled: the recursion's coefficients are read ahead of the chain that uses them
         const bool live = pr_msk[pl] != 0 && tb + pl != p0;
@@ -984,89 +1028,98 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     }
     __syncthreads();
     // wavefront wv: pairs 16 wv .. 16 wv + 15 of the tile. Per tile of 16 k: U = R H and
-    // V = R' H for the real and the imaginary columns; lane li then owns k = 16 kt + li
-    float fr[4] = {0.f, 0.f, 0.f, 0.f}, fa[4] = {0.f, 0.f, 0.f, 0.f}, fo[4] = {0.f, 0.f, 0.f, 0.f};
-    // lane coordinates made opaque here, so that the LDS addresses derived from them are
-    // formed in the loop rather than hoisted out of it into long-lived registers
-    int lrow = 16 * wv + 4 * lk, lcol = li;
-    asm volatile("" : "+v"(lrow), "+v"(lcol));
-    const int pa = 16 * wv + lcol;  // A-operand row (pair) of this lane
-    float ra[KN], rda[KN];
+    // V = R' H for the real and the imaginary columns; lane li then owns k = 16 kt + li. A
+    // wavefront whose pairs all lie past the tile's last one has nothing to do.
+    if (16 * __builtin_amdgcn_readfirstlane(wv) < p1 - tb) {
+      float fr[4] = {0.f, 0.f, 0.f, 0.f}, fa[4] = {0.f, 0.f, 0.f, 0.f}, fo[4] = {0.f, 0.f, 0.f, 0.f};
+      // lane coordinates made opaque here, so that the LDS addresses derived from them are
+      // formed in the loop rather than hoisted out of it into long-lived registers
+      int lrow = 16 * wv + 4 * lk, lcol = li;
+      asm volatile("" : "+v"(lrow), "+v"(lcol));
+      const int pa = 16 * wv + lcol;  // A-operand row (pair) of this lane
+      // U and V over n < 4 K (K steps of 4; the steps past n_max add nothing), then the contraction
+      auto uv = [&](auto kn) {
+        constexpr int K = decltype(kn)::value;
+        float ra[K], rda[K];
 #pragma unroll
-    for (int s = 0; s < KN; ++s) {
-      const int n = 4 * s + lk;
-      ra[s] = n < n_max ? Rb[n * PTS + pa] : 0.f;
-      rda[s] = n < n_max ? Rdb[n * PTS + pa] : 0.f;
-    }
-    constexpr int KG = KT < 3 ? KT : 3;
-#pragma unroll
-    for (int kt0 = 0; kt0 < KT; kt0 += KG) {
-      floatx4 Ur[KG], Ui[KG], Vr[KG], Vi[KG];
-#pragma unroll
-      for (int g = 0; g < KG; ++g) {
-        Ur[g] = Ui[g] = Vr[g] = Vi[g] = floatx4{0.f, 0.f, 0.f, 0.f};
-        if (kt0 + g < KT) {
-          const int k = 16 * (kt0 + g) + lcol;
-          float hr[KN], hm[KN];
-#pragma unroll
-          for (int s = 0; s < KN; ++s) {
-            const int n = 4 * s + lk;
-            hr[s] = n < n_max ? H[n * HS + k] : 0.f;
-            hm[s] = n < n_max ? H[n * HS + KMP + k] : 0.f;
-          }
-#pragma unroll
-          for (int s = 0; s < KN; ++s) {
-            Ur[g] = mfma4(ra[s], hr[s], Ur[g]);
-            Ui[g] = mfma4(ra[s], hm[s], Ui[g]);
-            Vr[g] = mfma4(rda[s], hr[s], Vr[g]);
-            Vi[g] = mfma4(rda[s], hm[s], Vi[g]);
-          }
+        for (int s = 0; s < K; ++s) {
+          const int n = 4 * s + lk;
+          ra[s] = n < n_max ? Rb[n * PTS + pa] : 0.f;
+          rda[s] = n < n_max ? Rdb[n * PTS + pa] : 0.f;
         }
-      }
+        constexpr int KG = KT < 3 ? KT : 3;
 #pragma unroll
-      for (int g = 0; g < KG; ++g) {
-        const int k = 16 * (kt0 + g) + lcol;
-        const bool act = kt0 + g < KT && k < KM;
-        const int lm = act ? klm[k] : 0;
-        const int l = lm & 255, m = lm >> 8;
-        const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
-        if (act) {
+        for (int kt0 = 0; kt0 < KT; kt0 += KG) {
+          floatx4 Ur[KG], Ui[KG], Vr[KG], Vi[KG];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const float ur = Ur[g][v], ui = Ui[g][v], vr = Vr[g][v], vi = Vi[g][v];
-            const int lp = lrow + v;
-            const float gl = Gl[l * PTS + lp], gdl = Gd[l * PTS + lp];
-            const float2 e = Em[m * PTS + lp];
-            const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
-            const float ev = e.x * vr + e.y * vi;     // Re(E conj v)
-            const float ex = e.x * ui - e.y * ur;     // Re(i E conj u)
-            const float pl = Fp[(kb + m) * PTS + lp];
-            const float pp1 = m < l ? Fp[(kb + m + 1) * PTS + lp] : 0.f;
-            float dms, dsin = 0.f;
-            if (m == 0) {
-              dms = -pp1;
-            } else {
-              dms = 0.5f * (float)((l + m) * (l - m + 1)) * Fp[(kb + m - 1) * PTS + lp] - 0.5f * pp1;
-              dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * Fp[(kb1 + m - 1) * PTS + lp] +
-                     0.5f * Fp[(kb1 + m + 1) * PTS + lp];
+          for (int g = 0; g < KG; ++g) {
+            Ur[g] = Ui[g] = Vr[g] = Vi[g] = floatx4{0.f, 0.f, 0.f, 0.f};
+            if (kt0 + g < KT) {
+              const int k = 16 * (kt0 + g) + lcol;
+              float hr[K], hm[K];
+#pragma unroll
+              for (int s = 0; s < K; ++s) {
+                const int n = 4 * s + lk;
+                hr[s] = n < n_max ? H[n * HS + k] : 0.f;
+                hm[s] = n < n_max ? H[n * HS + KMP + k] : 0.f;
+              }
+#pragma unroll
+              for (int s = 0; s < K; ++s) {
+                Ur[g] = mfma4(ra[s], hr[s], Ur[g]);
+                Ui[g] = mfma4(ra[s], hm[s], Ui[g]);
+                Vr[g] = mfma4(rda[s], hr[s], Vr[g]);
+                Vi[g] = mfma4(rda[s], hm[s], Vi[g]);
+              }
             }
-            // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E (the prefactor is in H)
-            fr[v] += pl * (gl * ev + gdl * eu);
-            fo[v] += dms * gl * eu;
-            fa[v] += dsin * gl * ex;
+          }
+#pragma unroll
+          for (int g = 0; g < KG; ++g) {
+            const int k = 16 * (kt0 + g) + lcol;
+            const bool act = kt0 + g < KT && k < KM;
+            const int lm = act ? klm[k] : 0;
+            const int l = lm & 255, m = lm >> 8;
+            const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
+            if (act) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const float ur = Ur[g][v], ui = Ui[g][v], vr = Vr[g][v], vi = Vi[g][v];
+                const int lp = lrow + v;
+                const float gl = Gl[l * PTS + lp], gdl = Gd[l * PTS + lp];
+                const float2 e = Em[m * PTS + lp];
+                const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
+                const float ev = e.x * vr + e.y * vi;     // Re(E conj v)
+                const float ex = e.x * ui - e.y * ur;     // Re(i E conj u)
+                const float pl = Fp[(kb + m) * PTS + lp];
+                const float pp1 = m < l ? Fp[(kb + m + 1) * PTS + lp] : 0.f;
+                float dms, dsin = 0.f;
+                if (m == 0) {
+                  dms = -pp1;
+                } else {
+                  dms = 0.5f * (float)((l + m) * (l - m + 1)) * Fp[(kb + m - 1) * PTS + lp] - 0.5f * pp1;
+                  dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * Fp[(kb1 + m - 1) * PTS + lp] +
+                         0.5f * Fp[(kb1 + m + 1) * PTS + lp];
+                }
+                // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E (the prefactor is in H)
+                fr[v] += pl * (gl * ev + gdl * eu);
+                fo[v] += dms * gl * eu;
+                fa[v] += dsin * gl * ex;
+              }
+            }
           }
         }
-      }
-    }
-    // sums over the 16 k of a row: prefix sums along the DPP row, total in lane 15
+      };
+      if (n_max <= 4 * (KN - 2)) uv(std::integral_constant<int, KN - 2>{});
+      else uv(std::integral_constant<int, KN>{});
+      // sums over the 16 k of a row: prefix sums along the DPP row, total in lane 15
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const float tr = row_total(fr[v]), ta = row_total(fa[v]), to = row_total(fo[v]);
-      if (li == 15) {
-        const int lp = 16 * wv + 4 * lk + v;
-        fout[lp] = tr;
-        fout[PT + lp] = ta;
-        fout[2 * PT + lp] = to;
+      for (int v = 0; v < 4; ++v) {
+        const float tr = row_total(fr[v]), ta = row_total(fa[v]), to = row_total(fo[v]);
+        if (li == 15) {
+          const int lp = 16 * wv + 4 * lk + v;
+          fout[lp] = tr;
+          fout[PT + lp] = ta;
+          fout[2 * PT + lp] = to;
+        }
       }
     }
     __syncthreads();
@@ -1302,7 +1355,13 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
   P.ang_inv_h = NTAB / rcut_max;
   P.ang_ih = (float)(NTAB / rcut_max);
   for (int q = 0; q < ns; ++q) P.sp[q].atab_off = q * (P.ang_n - 1) * 4 * (L + 1);
-  if (key == s.table_key) return ST_OK;
+  if (key == s.table_key) {
+    for (int q = 0; q < ns; ++q) {
+      P.sp[q].rtab_off = s.rt_off[q];
+      P.sp[q].atab_off = s.at_off[q];
+    }
+    return ST_OK;
+  }
 
   int rc_;
   std::vector<float> rt(rfloats);
@@ -1364,6 +1423,33 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
       k0 += sp.seg_n[j];
     }
   }
+  // Species with equal tables share one copy (equal parameters give equal functions), which keeps
+  // the tables the site kernel reads per pair small enough to stay in L2.
+  {
+    std::vector<float> kept;
+    int orig[ADJ_MAX_SPECIES];
+    for (int q = 0; q < ns; ++q) orig[q] = P.sp[q].rtab_off;
+    for (int q = 0; q < ns; ++q) {
+      SpeciesTab& sp = P.sp[q];
+      const size_t len = (size_t)(sp.seg_base[sp.nseg - 1] + sp.seg_n[sp.nseg - 1] - 1) * 4 * sp.a;
+      int same = -1;
+      for (int r = 0; r < q && same < 0; ++r) {
+        const SpeciesTab& o = P.sp[r];
+        bool eq = o.a == sp.a && o.nseg == sp.nseg;
+        for (int j = 0; eq && j < sp.nseg; ++j)
+          eq = o.seg_lo[j] == sp.seg_lo[j] && o.seg_inv_h[j] == sp.seg_inv_h[j] && o.seg_n[j] == sp.seg_n[j] &&
+               o.seg_base[j] == sp.seg_base[j];
+        if (eq && std::memcmp(rt.data() + orig[r], rt.data() + orig[q], len * 4) == 0) same = r;
+      }
+      if (same >= 0) {
+        sp.rtab_off = P.sp[same].rtab_off;
+      } else {
+        sp.rtab_off = (int)kept.size();
+        kept.insert(kept.end(), rt.begin() + orig[q], rt.begin() + orig[q] + len);
+      }
+    }
+    rt.swap(kept);
+  }
   if ((rc_ = ensure(s.rtab, rt.size() * 4, msg))) return rc_;
   (void)hipMemcpyAsync(s.rtab.p, rt.data(), rt.size() * 4, hipMemcpyHostToDevice, st);
   // angular radial factors, uniform grid on [0, rcut_max]
@@ -1394,6 +1480,23 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
     }
     (void)hipFree(dout.p);
   }
+  {
+    // the same sharing for the angular tables
+    const size_t len = (size_t)ncell * 4 * L1;
+    std::vector<float> kept;
+    for (int q = 0; q < ns; ++q) {
+      int same = -1;
+      for (int r = 0; r < q && same < 0; ++r)
+        if (std::memcmp(at.data() + r * len, at.data() + q * len, len * 4) == 0) same = r;
+      if (same >= 0) {
+        P.sp[q].atab_off = P.sp[same].atab_off;
+      } else {
+        P.sp[q].atab_off = (int)kept.size();
+        kept.insert(kept.end(), at.begin() + q * len, at.begin() + (q + 1) * len);
+      }
+    }
+    at.swap(kept);
+  }
   if ((rc_ = ensure(s.atab, at.size() * 4, msg))) return rc_;
   (void)hipMemcpyAsync(s.atab.p, at.data(), at.size() * 4, hipMemcpyHostToDevice, st);
   // constant tables
@@ -1408,6 +1511,10 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
     return ST_ERR_HIP;
   }
   s.table_key = key;
+  for (int q = 0; q < ns; ++q) {
+    s.rt_off[q] = P.sp[q].rtab_off;
+    s.at_off[q] = P.sp[q].atab_off;
+  }
   s.n_pf = (int)pf.size();
   return ST_OK;
 }
