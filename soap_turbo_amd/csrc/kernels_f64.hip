@@ -517,33 +517,45 @@ __global__ __launch_bounds__(128) void radial_op_k(RadialConsts c, int n_pairs, 
 // get_ilexp (:361-422), get_ilexp_der (:436-463), get_plm_array (:108-160)
 // and get_plm_array_der (:181-248).
 template <int L>
-__global__ __launch_bounds__(128) void angular_k(AngularConsts c, int n_pairs, const double* __restrict__ rjs,
-                                                 const double* __restrict__ thetas, const double* __restrict__ phis,
-                                                 const uint32_t* __restrict__ spmask, const double* __restrict__ preflm,
-                                                 double2* __restrict__ ang, double2* __restrict__ ang_r,
-                                                 double2* __restrict__ ang_a, double2* __restrict__ ang_p) {
+//
+// One wavefront per 64 pairs, a thread per pair. The rows of the block's pairs (KMAX values each,
+// contiguous in the pair-major output) are staged in LDS one output array at a time and written out
+// as one contiguous region, so that every store instruction covers consecutive addresses.
+__global__ __launch_bounds__(64) void angular_k(AngularConsts c, int n_pairs, const double* __restrict__ rjs,
+                                                const double* __restrict__ thetas, const double* __restrict__ phis,
+                                                const uint32_t* __restrict__ spmask, const double* __restrict__ preflm,
+                                                double2* __restrict__ ang, double2* __restrict__ ang_r,
+                                                double2* __restrict__ ang_a, double2* __restrict__ ang_p) {
   constexpr int KMAX = 1 + L * (L + 1) / 2 + L;
   constexpr int KMAX1 = 1 + (L + 1) * (L + 2) / 2 + (L + 1);
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
-  const double rj = rjs[p];
-  const uint32_t msk = spmask[p];
-  double2* A = ang + (size_t)p * KMAX;
-  if (!(rj < c.rcut_max) || msk == 0) {
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) A[k] = make_double2(0.0, 0.0);
-    if (c.do_der) {
-#pragma unroll
-      for (int k = 0; k < KMAX; ++k) {
-        ang_r[(size_t)p * KMAX + k] = make_double2(0.0, 0.0);
-        ang_a[(size_t)p * KMAX + k] = make_double2(0.0, 0.0);
-        ang_p[(size_t)p * KMAX + k] = make_double2(0.0, 0.0);
-      }
+  extern __shared__ double2 stage[];  // 64 * KMAX
+  const int lane = threadIdx.x, base = blockIdx.x * 64;
+  const int p = base + lane, nb = min(64, n_pairs - base);
+  // pairs past the list or outside the cutoff are computed on stand-in geometry and written as zeros
+  double rj = 1.0, th = 0.0, phi = 0.0;
+  uint32_t msk = 1u;
+  bool live = false;
+  if (p < n_pairs) {
+    const double r0 = rjs[p];
+    const uint32_t m0 = spmask[p];
+    if (r0 < c.rcut_max && m0 != 0) {
+      live = true;
+      rj = r0;
+      msk = m0;
+      th = thetas[p];
+      phi = phis[p];
     }
-    return;
   }
+  // the staged rows out to the block's region of `out`
+  auto flush = [&](double2* __restrict__ out) {
+    __syncthreads();
+    double2* o = out + (size_t)base * KMAX;
+    for (int e = lane; e < nb * KMAX; e += 64) o[e] = stage[e];
+    __syncthreads();
+  };
+  double2* row = stage + lane * KMAX;
   const int i_sp = __ffs(msk) - 1;
-  const double phi = phis[p], x = cos(thetas[p]);
+  const double x = cos(th);
   const double sigma = c.sigma_t[i_sp] + c.sigma_t_scaling[i_sp] * rj;
   const double amplitude = c.rcut2 / (sigma * sigma);
   // ilexp(rj / sigma)
@@ -635,37 +647,46 @@ __global__ __launch_bounds__(128) void angular_k(AngularConsts c, int n_pairs, c
       }
     }
     const double sig_fac = 2.0 * scal / sigma;
-    double2* Ar = ang_r + (size_t)p * KMAX;
-    double2* Aa = ang_a + (size_t)p * KMAX;
-    double2* Ap = ang_p + (size_t)p * KMAX;
+    // the four arrays in turn (each entry formed as in one pass over (l, m))
 #pragma unroll
-    for (int l = 0; l <= L; ++l) {
+    for (int arr = 0; arr < 4; ++arr) {
 #pragma unroll
-      for (int m = 0; m <= l; ++m) {
-        const int kk = l * (l + 1) / 2 + m;
-        const int k = kk + 1;
-        // sin(t) dP/dcos(t) and -m P/sin(t)
-        double part1, part2;
-        if (m == 0) part1 = l == 0 ? 0.0 : -0.5 * plm[l * (l + 1) / 2 + 1];
-        else part1 = (0.5 * (double)(l + m) * (double)(l - m + 1)) * plm[k - 2];
-        part2 = m == l ? 0.0 : -0.5 * plm[k];
-        const double dms = part1 + part2;
-        double dsin = 0.0;
-        if (m >= 1) {
-          const int kp = 1 + (l + 1) * (l + 2) / 2 + m + 1, km = 1 + (l + 1) * (l + 2) / 2 + m - 1;
-          dsin = (0.5 * (double)(l - m + 1) * (double)(l - m + 2)) * plm[km - 1] + 0.5 * plm[kp - 1];
+      for (int l = 0; l <= L; ++l) {
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+          const int kk = l * (l + 1) / 2 + m;
+          const int k = kk + 1;
+          const double e_re = il[l] * pm_re[m], e_im = il[l] * pm_im[m];
+          const double u = amplitude * preflm[kk], v = u * plm[kk];
+          double2 out;
+          if (arr == 0) {
+            out = make_double2(v * e_re, v * e_im);
+          } else if (arr == 1) {
+            const double er_re = ild[l] * pm_re[m], er_im = ild[l] * pm_im[m];
+            const double ec_re = v * e_re, ec_im = v * e_im;
+            out = make_double2(v * er_re - sig_fac * ec_re, v * er_im - sig_fac * ec_im);
+          } else if (arr == 2) {
+            double dsin = 0.0;
+            if (m >= 1) {
+              const int kp = 1 + (l + 1) * (l + 2) / 2 + m + 1, km = 1 + (l + 1) * (l + 2) / 2 + m - 1;
+              dsin = (0.5 * (double)(l - m + 1) * (double)(l - m + 2)) * plm[km - 1] + 0.5 * plm[kp - 1];
+            }
+            const double t = u * dsin;
+            out = make_double2(-(t * e_im), t * e_re);
+          } else {
+            // sin(t) dP/dcos(t) and -m P/sin(t)
+            double part1, part2;
+            if (m == 0) part1 = l == 0 ? 0.0 : -0.5 * plm[l * (l + 1) / 2 + 1];
+            else part1 = (0.5 * (double)(l + m) * (double)(l - m + 1)) * plm[k - 2];
+            part2 = m == l ? 0.0 : -0.5 * plm[k];
+            const double dms = part1 + part2;
+            const double t2 = u * dms;
+            out = make_double2(t2 * e_re, t2 * e_im);
+          }
+          row[kk] = live ? out : make_double2(0.0, 0.0);
         }
-        const double e_re = il[l] * pm_re[m], e_im = il[l] * pm_im[m];
-        const double er_re = ild[l] * pm_re[m], er_im = ild[l] * pm_im[m];
-        const double u = amplitude * preflm[kk], v = u * plm[kk];
-        const double ec_re = v * e_re, ec_im = v * e_im;
-        A[kk] = make_double2(ec_re, ec_im);
-        Ar[kk] = make_double2(v * er_re - sig_fac * ec_re, v * er_im - sig_fac * ec_im);
-        const double t = u * dsin;
-        Aa[kk] = make_double2(-(t * e_im), t * e_re);
-        const double t2 = u * dms;
-        Ap[kk] = make_double2(t2 * e_re, t2 * e_im);
       }
+      flush(arr == 0 ? ang : arr == 1 ? ang_r : arr == 2 ? ang_a : ang_p);
     }
   } else {
 #pragma unroll
@@ -675,9 +696,10 @@ __global__ __launch_bounds__(128) void angular_k(AngularConsts c, int n_pairs, c
         const int kk = l * (l + 1) / 2 + m;
         const double e_re = il[l] * pm_re[m], e_im = il[l] * pm_im[m];
         const double t = amplitude * preflm[kk] * plm[kk];
-        A[kk] = make_double2(t * e_re, t * e_im);
+        row[kk] = live ? make_double2(t * e_re, t * e_im) : make_double2(0.0, 0.0);
       }
     }
+    flush(ang);
   }
 }
 
@@ -1375,7 +1397,11 @@ template <int L>
 static void angular_one(hipStream_t st, const AngularConsts& c, int n_pairs, const double* rjs, const double* th,
                         const double* ph, const uint32_t* spmask, const double* preflm, double2* a, double2* ar,
                         double2* aa, double2* ap) {
-  angular_k<L><<<(n_pairs + 127) / 128, 128, 0, st>>>(c, n_pairs, rjs, th, ph, spmask, preflm, a, ar, aa, ap);
+  constexpr int KMAX = 1 + L * (L + 1) / 2 + L;
+  constexpr size_t sh = 64 * KMAX * sizeof(double2);
+  if (sh > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&angular_k<L>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  angular_k<L><<<(n_pairs + 63) / 64, 64, sh, st>>>(c, n_pairs, rjs, th, ph, spmask, preflm, a, ar, aa, ap);
 }
 
 bool launch_angular(hipStream_t st, const AngularConsts& c, int n_pairs, const double* rjs, const double* th,
