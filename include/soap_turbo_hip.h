@@ -23,7 +23,9 @@
  *                           reference's counterpart of the resident state)
  *   st_adjoint_f32          no reference counterpart: dE/dq -> pair gradients, forces and virial in FP32
  *                           without forming soap_cart_der (the FP32 form of soap_contract, :1703-1721)
- *   st_build_neighbours     no reference counterpart: the callers' neighbour list, built on the device
+ *   st_build_neighbours, st_build_neighbours_cell
+ *                           no reference counterpart: the callers' neighbour list, built on the device
+ *                           (cubic box; general lattice vectors)
  *   st_central_hessian      get_soap_central_hessian, src/soap_turbo.f90:1151-1700 (the caller calls
  *                           it after its get_soap; here it runs that pass itself)
  *   st_set_gap_model, st_gap_energy_forces, st_gap_site_energies
@@ -153,6 +155,13 @@ int st_upload_atoms(st_ctx* ctx, int32_t n_atoms, const int32_t* pair_atom, cons
 int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, const int32_t* atom_species,
                         int32_t n_species, double box, double rcut, int32_t n_sites, const int32_t* site_atoms,
                         int32_t* n_pairs);
+/* The same for a general periodic cell (no reference counterpart): cell(3, 3) holds the lattice vectors as
+ * columns (cell(:, v), a right-handed cell); every width of the cell (the distance between opposite
+ * faces) must exceed 2 rcut, so that each neighbour has one image inside the cutoff. The pair vector is
+ * d - H rint(H^-1 d) for d = r_j - r_i, H the lattice vectors as columns. */
+int st_build_neighbours_cell(st_ctx* ctx, int32_t n_atoms, const double* positions, const int32_t* atom_species,
+                             int32_t n_species, const double* cell, double rcut, int32_t n_sites,
+                             const int32_t* site_atoms, int32_t* n_pairs);
 /* The resident neighbour list: n_neigh(n_sites), pair_atom(n_pairs), rjs, thetas, phis(n_pairs),
  * mask(n_pairs, n_species) int32, vec(3, n_pairs); any may be NULL. */
 int st_download_neighbours(st_ctx* ctx, int32_t* n_neigh, int32_t* pair_atom, double* rjs, double* thetas,

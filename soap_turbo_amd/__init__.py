@@ -140,14 +140,21 @@ class Context:
         self.check(lib().st_set_graphs(self.h, int(bool(on))))
 
     def build_neighbours(self, pos, atom_species, n_species, box, rcut, site_atom):
-        """Neighbour list of a periodic cubic box built on the device (pos (n_atoms, 3), atom_species
-        1-based, site_atom 0-based): the resident state of upload_neighbours + upload_atoms."""
+        """Neighbour list of a periodic cell built on the device (pos (n_atoms, 3), atom_species 1-based,
+        site_atom 0-based): the resident state of upload_neighbours + upload_atoms. box is a cubic side, or a
+        (3, 3) array whose rows are the lattice vectors of a general cell."""
         keep = Keep()
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         n = ctypes.c_int32(0)
-        self.check(lib().st_build_neighbours(self.h, len(pos), keep(pos, np.float64, "C"), keep(atom_species, np.int32),
-                                             int(n_species), float(box), float(rcut), len(site_atom),
-                                             keep(site_atom, np.int32), ctypes.byref(n)))
+        if np.ndim(box) == 2:
+            self.check(lib().st_build_neighbours_cell(self.h, len(pos), keep(pos, np.float64, "C"),
+                                                      keep(atom_species, np.int32), int(n_species),
+                                                      keep(box, np.float64, "C"), float(rcut), len(site_atom),
+                                                      keep(site_atom, np.int32), ctypes.byref(n)))
+        else:
+            self.check(lib().st_build_neighbours(self.h, len(pos), keep(pos, np.float64, "C"),
+                                                 keep(atom_species, np.int32), int(n_species), float(box), float(rcut),
+                                                 len(site_atom), keep(site_atom, np.int32), ctypes.byref(n)))
         self.n_sites, self.n_pairs, self.n_atoms, self.n_species_nb = len(site_atom), n.value, len(pos), int(n_species)
         return n.value
 

@@ -52,6 +52,7 @@ SIGNATURES = {
     "st_upload_atoms": (ctypes.c_int, [P, I32, P, P]),
     "st_build_neighbours": (ctypes.c_int, [P, I32, P, P, I32, ctypes.c_double, ctypes.c_double, I32, P,
                                            ctypes.POINTER(I32)]),
+    "st_build_neighbours_cell": (ctypes.c_int, [P, I32, P, P, I32, P, ctypes.c_double, I32, P, ctypes.POINTER(I32)]),
     "st_download_neighbours": (ctypes.c_int, [P, P, P, P, P, P, P, P]),
     "st_atom_pairs": (ctypes.c_int, [P, P, P]),
     "st_upload_dedq": (ctypes.c_int, [P, I32, P]),

@@ -1360,15 +1360,13 @@ static int run_adjoint(st_ctx* ctx, const st_params* p, bool forward_only) {
   return ST_OK;
 }
 
-int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, const int32_t* atom_species,
-                        int32_t n_species, double box, double rcut, int32_t n_sites, const int32_t* site_atoms,
-                        int32_t* n_pairs) {
-  if (!ctx) return ST_ERR_ARG;
+// the device neighbour list of either cell form (st_build_neighbours, st_build_neighbours_cell)
+static int build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, const int32_t* atom_species,
+                            int32_t n_species, const st::Lattice& g, double rcut, int32_t n_sites,
+                            const int32_t* site_atoms, int32_t* n_pairs) {
   if (n_atoms < 1 || n_sites < 1 || !positions || !atom_species || !site_atoms)
     return fail(ctx, ST_ERR_ARG, "positions, species and sites are required");
   if (n_species < 1 || n_species > ST_MAX_SPECIES) return fail(ctx, ST_ERR_ARG, "bad number of species");
-  if (!(rcut > 0.0) || !(box > 2.0 * rcut))
-    return fail(ctx, ST_ERR_ARG, "the box side must exceed twice the cutoff (minimum image)");
   for (int a = 0; a < n_atoms; ++a)
     if (atom_species[a] < 1 || atom_species[a] > n_species) return fail(ctx, ST_ERR_ARG, "atom species out of range");
   for (int s = 0; s < n_sites; ++s)
@@ -1376,8 +1374,7 @@ int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, c
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   int rc;
-  const int nc = st::neighbour_cells(box, rcut);
-  const size_t n_cells = (size_t)nc * nc * nc;
+  const size_t n_cells = (size_t)g.n[0] * g.n[1] * g.n[2];
   if ((rc = upload(ctx, ctx->nb_pos, positions, (size_t)n_atoms * 24))) return rc;
   if ((rc = upload(ctx, ctx->nb_species, atom_species, (size_t)n_atoms * 4))) return rc;
   if ((rc = upload(ctx, ctx->nb_sites, site_atoms, (size_t)n_sites * 4))) return rc;
@@ -1389,10 +1386,10 @@ int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, c
   if ((rc = ensure(ctx, ctx->nb_count, std::max((size_t)n_sites, (size_t)n_atoms) * 4))) return rc;
   if ((rc = ensure(ctx, ctx->nb_flag, 4))) return rc;
   if ((rc = ensure(ctx, ctx->site_ptr, ((size_t)n_sites + 1) * 4))) return rc;
-  st::launch_neighbour_bins(st, n_atoms, ctx->nb_pos.as<double>(), box, nc, ctx->nb_cell_of.as<int>(),
+  st::launch_neighbour_bins(st, n_atoms, ctx->nb_pos.as<double>(), g, ctx->nb_cell_of.as<int>(),
                             ctx->nb_cell_count.as<int>(), ctx->nb_cell_start.as<int>(), ctx->nb_cursor.as<int>(),
                             ctx->nb_cell_atoms.as<int>());
-  st::launch_neighbour_count(st, n_sites, ctx->nb_sites.as<int>(), ctx->nb_pos.as<double>(), box, rcut, nc,
+  st::launch_neighbour_count(st, n_sites, ctx->nb_sites.as<int>(), ctx->nb_pos.as<double>(), g, rcut,
                              ctx->nb_cell_start.as<int>(), ctx->nb_cell_atoms.as<int>(), ctx->nb_count.as<int>(),
                              ctx->site_ptr.as<int>());
   std::vector<int> ptr((size_t)n_sites + 1);
@@ -1412,7 +1409,7 @@ int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, c
   if ((rc = ensure(ctx, ctx->atom_pairs, P * 4))) return rc;
   HIPCHK(hipMemsetAsync(ctx->nb_flag.p, 0, 4, st));
   st::launch_neighbour_fill(st, n_sites, (int)P, n_species, ctx->nb_sites.as<int>(), ctx->nb_species.as<int>(),
-                            ctx->nb_pos.as<double>(), box, rcut, nc, ctx->nb_cell_start.as<int>(),
+                            ctx->nb_pos.as<double>(), g, rcut, ctx->nb_cell_start.as<int>(),
                             ctx->nb_cell_atoms.as<int>(), ctx->site_ptr.as<int>(), ctx->pair_atom.as<int>(),
                             ctx->vec.as<double>(), ctx->rjs.as<double>(), ctx->thetas.as<double>(),
                             ctx->phis.as<double>(), ctx->mask.as<int32_t>(), ctx->species.as<int32_t>(),
@@ -1444,6 +1441,31 @@ int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, c
   ++ctx->nb_gen;
   if (n_pairs) *n_pairs = (int32_t)P;
   return ST_OK;
+}
+
+int st_build_neighbours(st_ctx* ctx, int32_t n_atoms, const double* positions, const int32_t* atom_species,
+                        int32_t n_species, double box, double rcut, int32_t n_sites, const int32_t* site_atoms,
+                        int32_t* n_pairs) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!(rcut > 0.0) || !(box > 2.0 * rcut))
+    return fail(ctx, ST_ERR_ARG, "the box side must exceed twice the cutoff (minimum image)");
+  return build_neighbours(ctx, n_atoms, positions, atom_species, n_species, st::cubic_lattice(box, rcut), rcut,
+                          n_sites, site_atoms, n_pairs);
+}
+
+int st_build_neighbours_cell(st_ctx* ctx, int32_t n_atoms, const double* positions, const int32_t* atom_species,
+                             int32_t n_species, const double* cell, double rcut, int32_t n_sites,
+                             const int32_t* site_atoms, int32_t* n_pairs) {
+  if (!ctx) return ST_ERR_ARG;
+  if (!cell) return fail(ctx, ST_ERR_ARG, "the lattice vectors are required");
+  if (!(rcut > 0.0)) return fail(ctx, ST_ERR_ARG, "the cutoff must be positive");
+  st::Lattice g;
+  double width = 0.0;
+  if (!st::general_lattice(cell, rcut, g, width))
+    return fail(ctx, ST_ERR_ARG, "the lattice vectors must span a right-handed cell of positive volume");
+  if (!(width > 2.0 * rcut))
+    return fail(ctx, ST_ERR_ARG, "every width of the cell must exceed twice the cutoff (minimum image)");
+  return build_neighbours(ctx, n_atoms, positions, atom_species, n_species, g, rcut, n_sites, site_atoms, n_pairs);
 }
 
 int st_download_neighbours(st_ctx* ctx, int32_t* n_neigh, int32_t* pair_atom, double* rjs, double* thetas,

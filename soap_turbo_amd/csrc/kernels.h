@@ -46,17 +46,26 @@ void launch_contract(hipStream_t st, int n_sites, int n_max, int l_max, int k_ma
                      const double* rjs, const double* thetas, const double* phis, double* pair);
 
 // Device neighbour lists (neighbours.hip)
-int neighbour_cells(double L, double rcut);  // cells a side of the cell list
-void launch_neighbour_bins(hipStream_t st, int n_atoms, const double* pos, double L, int nc, int* cell_of,
+// The periodic cell of a neighbour list: a cube of side L, or (general) the lattice vectors h[3c + v]
+// (component c of vector v) with h^-1 in hi[3v + c]; n: cells of the cell list along each vector.
+struct Lattice {
+  double L;
+  double h[9], hi[9];
+  int n[3];
+  int general;
+};
+Lattice cubic_lattice(double L, double rcut);
+// false for a left-handed or degenerate cell; min_width: the smallest distance between opposite faces
+bool general_lattice(const double* cell, double rcut, Lattice& g, double& min_width);
+void launch_neighbour_bins(hipStream_t st, int n_atoms, const double* pos, const Lattice& g, int* cell_of,
                            int* cell_count, int* cell_start, int* cursor, int* cell_atoms);
-void launch_neighbour_count(hipStream_t st, int n_sites, const int* site_atom, const double* pos, double L,
-                            double rcut, int nc, const int* cell_start, const int* cell_atoms, int* n_neigh,
-                            int* site_ptr);
+void launch_neighbour_count(hipStream_t st, int n_sites, const int* site_atom, const double* pos, const Lattice& g,
+                            double rcut, const int* cell_start, const int* cell_atoms, int* n_neigh, int* site_ptr);
 void launch_neighbour_fill(hipStream_t st, int n_sites, int n_pairs, int n_species, const int* site_atom,
-                           const int* species, const double* pos, double L, double rcut, int nc, const int* cell_start,
-                           const int* cell_atoms, const int* site_ptr, int* pair_atom, double* vec, double* rjs,
-                           double* thetas, double* phis, int32_t* mask, int32_t* site_species, int32_t* mult,
-                           int* overflow);
+                           const int* species, const double* pos, const Lattice& g, double rcut,
+                           const int* cell_start, const int* cell_atoms, const int* site_ptr, int* pair_atom,
+                           double* vec, double* rjs, double* thetas, double* phis, int32_t* mask,
+                           int32_t* site_species, int32_t* mult, int* overflow);
 void launch_atom_pairs(hipStream_t st, int n_atoms, int n_pairs, const int* pair_atom, int* count, int* cursor,
                        int* atom_ptr, int* atom_pairs, int* overflow);
 

@@ -5,6 +5,7 @@
 //
 // Positions come from random sequential addition in a cubic periodic box,
 // with slabs in z of independent densities (one slab for a uniform box).
+// Neighbour lists are built for a cubic box or for general lattice vectors.
 // The random stream is SplitMix64, so the same seed gives the same atoms on
 // every host.
 #include <algorithm>
@@ -215,6 +216,117 @@ int64_t stw_neighbours(int64_t n_atoms, const double* pos, double L, double rcut
     }
   }
   return total;
+}
+
+// The same list for a general periodic cell: cell[c + 3v] is component c of lattice vector v (the
+// vectors as columns of H). A pair vector is d - H rint(H^-1 d) for d = r_j - r_i, formed with the
+// operations the device builder uses; every width of the cell (the distance between opposite faces)
+// must exceed 2 rcut (returns -1 otherwise, or for a left-handed or degenerate cell). Atoms are binned
+// on their fractional coordinates into cells at least rcut wide across their faces.
+int64_t stw_neighbours_cell(int64_t n_atoms, const double* pos, const double* cell, double rcut, int64_t n_sites,
+                            const int64_t* site_atom, int32_t* n_neigh, int32_t* nbr, double* vec) {
+  double h[9], hi[9];
+  for (int c = 0; c < 3; ++c)
+    for (int v = 0; v < 3; ++v) h[3 * c + v] = cell[c + 3 * v];
+  const double* a = cell;
+  const double* b = cell + 3;
+  const double* d = cell + 6;
+  // rows of H^-1 = (b x c, c x a, a x b) / det
+  const double bc[3] = {b[1] * d[2] - b[2] * d[1], b[2] * d[0] - b[0] * d[2], b[0] * d[1] - b[1] * d[0]};
+  const double ca[3] = {d[1] * a[2] - d[2] * a[1], d[2] * a[0] - d[0] * a[2], d[0] * a[1] - d[1] * a[0]};
+  const double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double det = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+  if (!(det > 0.0)) return -1;
+  const double* rows[3] = {bc, ca, ab};
+  int n[3];
+  for (int v = 0; v < 3; ++v) {
+    const double* r = rows[v];
+    for (int c = 0; c < 3; ++c) hi[3 * v + c] = r[c] / det;
+    const double w = det / std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (!(w > 2.0 * rcut)) return -1;
+    n[v] = std::min(128, (int)std::floor(w / rcut));
+  }
+  const bool stencil = n[0] >= 3 && n[1] >= 3 && n[2] >= 3;
+  if (!stencil) n[0] = n[1] = n[2] = 1;
+  auto cell_of = [&](const double* p, int* c) {
+    for (int v = 0; v < 3; ++v) {
+      const double f = hi[3 * v] * p[0] + hi[3 * v + 1] * p[1] + hi[3 * v + 2] * p[2];
+      int k = (int)((f - std::floor(f)) * n[v]);
+      c[v] = k < 0 ? 0 : (k >= n[v] ? n[v] - 1 : k);
+    }
+  };
+  const size_t n_cells = (size_t)n[0] * n[1] * n[2];
+  std::vector<int> head(n_cells, -1), next((size_t)n_atoms, -1);
+  for (int64_t at = n_atoms - 1; at >= 0; --at) {
+    int c[3];
+    cell_of(pos + 3 * at, c);
+    const size_t k = ((size_t)c[2] * n[1] + c[1]) * n[0] + c[0];
+    next[at] = head[k];
+    head[k] = (int)at;
+  }
+  const double rc2 = rcut * rcut;
+  auto visit = [&](int64_t s, std::vector<int>& uniq, auto&& emit) {
+    const int64_t i = site_atom[s];
+    const double* pi = pos + 3 * i;
+    uniq.clear();
+    int c[3];
+    cell_of(pi, c);
+    const int span = stencil ? 1 : 0;
+    for (int dz = -span; dz <= span; ++dz)
+      for (int dy = -span; dy <= span; ++dy)
+        for (int dx = -span; dx <= span; ++dx) {
+          const int ex = ((c[0] + dx) % n[0] + n[0]) % n[0];
+          const int ey = ((c[1] + dy) % n[1] + n[1]) % n[1];
+          const int ez = ((c[2] + dz) % n[2] + n[2]) % n[2];
+          for (int q = head[((size_t)ez * n[1] + ey) * n[0] + ex]; q >= 0; q = next[q]) uniq.push_back(q);
+        }
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    for (int q : uniq) {
+      if (q == i) continue;
+      double x = pos[3 * q] - pi[0], y = pos[3 * q + 1] - pi[1], z = pos[3 * q + 2] - pi[2];
+      const double n0 = std::nearbyint(hi[0] * x + hi[1] * y + hi[2] * z);
+      const double n1 = std::nearbyint(hi[3] * x + hi[4] * y + hi[5] * z);
+      const double n2 = std::nearbyint(hi[6] * x + hi[7] * y + hi[8] * z);
+      x = x - (h[0] * n0 + h[1] * n1 + h[2] * n2);
+      y = y - (h[3] * n0 + h[4] * n1 + h[5] * n2);
+      z = z - (h[6] * n0 + h[7] * n1 + h[8] * n2);
+      if (x * x + y * y + z * z >= rc2) continue;
+      emit(q, x, y, z);
+    }
+  };
+#pragma omp parallel
+  {
+    std::vector<int> uniq;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t s = 0; s < n_sites; ++s) {
+      int32_t cnt = 1;
+      visit(s, uniq, [&](int, double, double, double) { ++cnt; });
+      n_neigh[s] = cnt;
+    }
+  }
+  std::vector<int64_t> start((size_t)n_sites + 1, 0);
+  for (int64_t s = 0; s < n_sites; ++s) start[s + 1] = start[s] + n_neigh[s];
+  if (!nbr) return start[n_sites];
+#pragma omp parallel
+  {
+    std::vector<int> uniq;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t s = 0; s < n_sites; ++s) {
+      int64_t k = start[s];
+      nbr[k] = (int32_t)site_atom[s];
+      vec[3 * k] = vec[3 * k + 1] = vec[3 * k + 2] = 0.0;
+      ++k;
+      visit(s, uniq, [&](int q, double x, double y, double z) {
+        nbr[k] = q;
+        vec[3 * k] = x;
+        vec[3 * k + 1] = y;
+        vec[3 * k + 2] = z;
+        ++k;
+      });
+    }
+  }
+  return start[n_sites];
 }
 
 // Spherical coordinates of the pair vectors: r, theta = acos(z/r),

@@ -44,6 +44,8 @@ def _lib():
         lib.stw_neighbours.restype = ctypes.c_int64
         lib.stw_neighbours.argtypes = [ctypes.c_int64, P, ctypes.c_double, ctypes.c_double, ctypes.c_int64,
                                        P, P, P, P]
+        lib.stw_neighbours_cell.restype = ctypes.c_int64
+        lib.stw_neighbours_cell.argtypes = [ctypes.c_int64, P, P, ctypes.c_double, ctypes.c_int64, P, P, P, P]
         lib.stw_spherical.restype = None
         lib.stw_spherical.argtypes = [ctypes.c_int64, P, P, P, P]
         _LIB = lib
@@ -167,7 +169,23 @@ def place_atoms(spec: BoxSpec):
 
 
 def build_neighbours(pos, L, rcut, site_atom):
+    """(n_neigh, nbr, vec) of the sites: L is a cubic box side, or a (3, 3) array whose rows are the
+    lattice vectors of a general cell."""
     lib = _lib()
+    if np.ndim(L) == 2:
+        cell = np.ascontiguousarray(L, dtype=np.float64)  # rows a, b, c: the memory of cell(3, 3) by columns
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        site_atom = np.ascontiguousarray(site_atom, dtype=np.int64)
+        n_neigh = np.zeros(len(site_atom), dtype=np.int32)
+        n_pairs = lib.stw_neighbours_cell(len(pos), _p(pos), _p(cell), rcut, len(site_atom), _p(site_atom),
+                                          _p(n_neigh), None, None)
+        if n_pairs < 0:
+            raise ValueError("the cell must be right-handed with every width above twice the cutoff")
+        nbr = np.zeros(n_pairs, dtype=np.int32)
+        vec = np.zeros((n_pairs, 3))
+        lib.stw_neighbours_cell(len(pos), _p(pos), _p(cell), rcut, len(site_atom), _p(site_atom), _p(n_neigh),
+                                _p(nbr), _p(vec))
+        return n_neigh, nbr, vec
     if L <= 2 * rcut:
         raise ValueError("the minimum-image neighbour list needs a box wider than twice the cutoff")
     pos = np.ascontiguousarray(pos, dtype=np.float64)
@@ -218,6 +236,45 @@ def make_workload(cfg, n_sites: int | None = None, sites=None) -> Workload:
     species = (asp[site_atom] + 1).astype(np.int32).reshape(-1, 1)
     mult = np.ones(len(site_atom), dtype=np.int32)
     return Workload(params, L, pos, asp, site_atom, n_neigh, nbr, vec, rjs, thetas, phis, mask, species, mult)
+
+
+def skewed_cell(n_atoms: int, density: float):
+    """Lattice vectors (rows) of a triclinic cell of volume n_atoms / density: angles near 70, 75 and 80
+    degrees, every face-to-face width within 15 % of the cube's side."""
+    shape = np.array([[1.0, 0.0, 0.0], [0.30, 0.95, 0.0], [-0.20, 0.25, 0.93]])
+    return shape * ((n_atoms / density) / np.linalg.det(shape)) ** (1.0 / 3.0)
+
+
+def triclinic_workload(n_atoms: int, density: float, species_prob, params: SoapParams, seed: int,
+                       dmin: float = 1.0, n_sites: int | None = None) -> Workload:
+    """Atoms in a triclinic periodic cell (skewed_cell), placed by random sequential addition with a
+    minimum distance (minimum image over the lattice), a fifth of them shifted by a lattice translation
+    out of the home cell; every atom a site (or the first n_sites). Workload.L holds the lattice vectors."""
+    cell = skewed_cell(n_atoms, density)
+    hi = np.linalg.inv(cell)
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n_atoms, 3))
+    placed = 0
+    while placed < n_atoms:
+        p = rng.random(3) @ cell
+        d = pos[:placed] - p
+        d = d - np.rint(d @ hi) @ cell
+        if placed == 0 or (np.einsum("ij,ij->i", d, d) >= dmin * dmin).all():
+            pos[placed] = p
+            placed += 1
+    shift = rng.random(n_atoms) < 0.2
+    pos[shift] += rng.integers(-1, 2, (int(shift.sum()), 3)) @ cell
+    asp = rng.choice(len(species_prob), n_atoms, p=species_prob).astype(np.int32)
+    site_atom = np.arange(n_atoms if n_sites is None else n_sites, dtype=np.int64)
+    rcut_max = float(np.max(params.per_species()["rcut_hard"]))
+    n_neigh, nbr, vec = build_neighbours(pos, cell, rcut_max, site_atom)
+    rjs, thetas, phis = spherical(vec)
+    mask = np.zeros((params.n_species, len(nbr)), dtype=bool)
+    for q in range(params.n_species):
+        mask[q] = asp[nbr] == q
+    species = (asp[site_atom] + 1).astype(np.int32).reshape(-1, 1)
+    mult = np.ones(len(site_atom), dtype=np.int32)
+    return Workload(params, cell, pos, asp, site_atom, n_neigh, nbr, vec, rjs, thetas, phis, mask, species, mult)
 
 
 def strided_sites(n_all: int, n: int, offset: int = 0):
