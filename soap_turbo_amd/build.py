@@ -61,8 +61,7 @@ def build_hip(verbose=False):
         objs.append(obj)
     out = os.path.join(HERE, "libsoapturbo_hip.so")
     tmp = out + ".tmp"
-    # rocBLAS: the GAP energy model's two GEMMs
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-L" + os.path.join(ROCM, "lib"), "-lrocblas", "-ldl",
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-L" + os.path.join(ROCM, "lib"), "-ldl",
           "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", tmp])
     os.replace(tmp, out)
     return out

@@ -97,7 +97,7 @@ int adjoint_download_soap(AdjointState& s, hipStream_t st, int n_sites, int n_so
 // over sparse points q_t (T: n_sparse x n_soap, FP32, row t contiguous), and its gradient dE/dq
 // written over s.G32 (site-major FP32, as the adjoint pass reads it); site energies in e_site, the
 // total in *energy_dev (FP64, device). Two FP32 GEMMs (rocBLAS) and an elementwise kernel.
-int gap_energy_grad(AdjointState& s, hipStream_t st, void* blas, int n_sites, int n_soap, int n_sparse, const float* T,
+int gap_energy_grad(AdjointState& s, hipStream_t st, int n_sites, int n_soap, int n_sparse, const float* T,
                     const float* alpha, int zeta, double delta, double* e_site, double* energy_dev, std::string& msg);
 // forces(:, a) = -sum of pair(:, k) over pairs with neighbour atom a (atom_ptr/atom_pairs: the
 // pairs of each atom); virial = -sum_k vec_k (x) pair_k, reduced through per-workgroup partials

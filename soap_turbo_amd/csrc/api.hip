@@ -13,7 +13,6 @@
 #include <string>
 #include <vector>
 
-#include <rocblas/rocblas.h>
 
 #include "../../include/soap_turbo_hip.h"
 #include "adjoint.h"
@@ -116,7 +115,6 @@ struct st_ctx {
   std::map<std::string, int> tuned;
   int jacobian_variant = -1;  // >= 0: the Jacobian kernel's launch variant, fixed (st_set_jacobian_variant)
   // GAP energy model (st_set_gap_model): sparse points and weights in FP32, site energies
-  rocblas_handle blas = nullptr;
   int gap_n_sparse = 0, gap_n_soap = 0, gap_zeta = 0;
   double gap_delta = 0.0;
   DevBuf gap_T, gap_alpha, gap_esite, gap_energy;
@@ -633,7 +631,6 @@ void st_destroy(st_ctx* c) {
   for (auto e : c->free_events) (void)hipEventDestroy(e);
   if (c->g32_exec) (void)hipGraphExecDestroy(c->g32_exec);
   if (c->g32_graph) (void)hipGraphDestroy(c->g32_graph);
-  if (c->blas) (void)rocblas_destroy_handle(c->blas);
   st::adjoint_release(c->adj);
   hipStream_t s = c->stream;
   delete c;
@@ -1511,8 +1508,6 @@ int st_set_gap_model(st_ctx* ctx, int32_t n_sparse, int32_t n_soap, const double
   if (n_sparse < 1 || n_soap < 1 || !sparse_q || !alpha || zeta < 1)
     return fail(ctx, ST_ERR_ARG, "bad GAP model (n_sparse, n_soap >= 1, zeta >= 1 and the arrays are required)");
   HIPCHK(hipSetDevice(ctx->device));
-  if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-    return fail(ctx, ST_ERR_HIP, "rocblas_create_handle failed");
   std::vector<float> T((size_t)n_sparse * n_soap), a(n_sparse);
   for (size_t k = 0; k < T.size(); ++k) T[k] = (float)sparse_q[k];  // (n_soap, n_sparse), point t contiguous
   for (int t = 0; t < n_sparse; ++t) a[t] = (float)alpha[t];
@@ -1545,7 +1540,7 @@ int st_gap_energy_forces(st_ctx* ctx, const st_params* p, const st_compression* 
   // descriptors, the model's energy and dE/dq on the device, then the adjoint pass for the forces
   if ((rc = run_adjoint(ctx, p, true))) return rc;
   timed(ctx, "gap_model", [&] {
-    rc = st::gap_energy_grad(ctx->adj, ctx->stream, ctx->blas, ctx->n_sites, ctx->n_soap, ctx->gap_n_sparse,
+    rc = st::gap_energy_grad(ctx->adj, ctx->stream, ctx->n_sites, ctx->n_soap, ctx->gap_n_sparse,
                              ctx->gap_T.as<float>(), ctx->gap_alpha.as<float>(), ctx->gap_zeta, ctx->gap_delta,
                              ctx->gap_esite.as<double>(), ctx->gap_energy.as<double>(), msg);
   });

@@ -72,3 +72,19 @@ def test_adjoint_after_gap_needs_a_new_dedq(ctx):
     ctx.set_gap_model(np.ones((n_soap + 1, 4)), np.ones(4), 2, 1.0)
     with pytest.raises(st.SoapTurboError):
         ctx.gap_energy_forces(prm, cmp_)
+
+
+def test_gap_step_repeats_bitwise(ctx):
+    # the site energies sum their partials in a fixed order: repeated steps agree exactly
+    w = make_workload(2)
+    keep, prm, cmp_, comp, n_soap, W, S, _ = setup(ctx, w)
+    qo, _, _ = orc.get_soap(w, W, S, comp, do_der=False)
+    T, alpha = model(qo, 300, 5)
+    ctx.set_gap_model(T.T, alpha, 3, 0.5)
+    E1 = ctx.gap_energy_forces(prm, cmp_)
+    e1 = ctx.gap_site_energies()
+    f1, v1 = ctx.download_forces(1)
+    E2 = ctx.gap_energy_forces(prm, cmp_)
+    e2 = ctx.gap_site_energies()
+    f2, v2 = ctx.download_forces(1)
+    assert E1 == E2 and np.array_equal(e1, e2) and np.array_equal(f1, f2) and np.array_equal(v1, v2)
