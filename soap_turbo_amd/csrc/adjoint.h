@@ -103,13 +103,19 @@ int gap_energy_grad(AdjointState& s, hipStream_t st, int n_sites, int n_soap, in
 // pairs of each atom); virial = -sum_k vec_k (x) pair_k, reduced through per-workgroup partials
 // (vpart holds gather_blocks(n_atoms) * 9 doubles).
 inline int gather_blocks(int n_atoms) { return n_atoms > 0 ? (n_atoms + 15) / 16 : 1; }
+// workgroups of the FP32 path's virial sum over the pairs (at most gather_blocks(n_atoms), so vpart fits)
+inline int virial_blocks(int n_atoms, int n_pairs) {
+  const int nb = (n_pairs + 255) / 256;
+  return nb < 1 ? 1 : (nb > 1024 ? 1024 : (nb > gather_blocks(n_atoms) ? gather_blocks(n_atoms) : nb));
+}
 void launch_forces_f64(hipStream_t st, int n_atoms, const double* pair, const int* atom_ptr, const int* atom_pairs,
                        const double* vec, const double* rjs, const double* thetas, const double* phis,
                        double* forces, double* vpart, double* virial);
 // (reset: a counter the last kernel of the step sets back to zero, or null)
-void launch_forces_f32(hipStream_t st, int n_atoms, const float* pair, const int* atom_ptr, const int* atom_pairs,
-                       const double* vec, const double* rjs, const double* thetas, const double* phis,
-                       double* forces, double* vpart, double* virial, unsigned long long* reset);
+void launch_forces_f32(hipStream_t st, int n_atoms, int n_pairs, const float* pair, const int* atom_ptr,
+                       const int* atom_pairs, const double* vec, const double* rjs, const double* thetas,
+                       const double* phis, double* forces, double* vpart, double* virial,
+                       unsigned long long* reset);
 // sets *counter to zero (a kernel, so that it can be part of a captured graph)
 void launch_counter_reset(hipStream_t st, unsigned long long* counter);
 void adjoint_invalidate(AdjointState& s);

@@ -1687,7 +1687,7 @@ int adjoint_run(AdjointState& s, hipStream_t st, const AdjointInputs& in, const 
       return rc;
     }
     timer("adj_forces", [&] {
-      launch_forces_f32(st, in.n_atoms, s.pair.as<float>(), in.atom_ptr, in.atom_pairs, in.vec, in.rjs, in.thetas,
+      launch_forces_f32(st, in.n_atoms, in.n_pairs, s.pair.as<float>(), in.atom_ptr, in.atom_pairs, in.vec, in.rjs, in.thetas,
                         in.phis, s.forces.as<double>(), s.virial_part.as<double>(), s.virial.as<double>(),
                         s.tickets.as<unsigned long long>());
     });
