@@ -16,9 +16,10 @@ already resident in HBM:
 
 Beside the line: the maximum relative errors of both paths against the FP64
 oracle on a sample of the workload's sites; the reference (compiled from its
-Fortran sources) on a sample of sites on the host cores; and, at
-``--config 2`` scale, the 4096-atom three-species workload with the GAP
-model, the central-atom Hessian and the neighbour-list builder.
+Fortran sources) on a sample of sites on the host cores; and the 4096-atom
+three-species workload (``--config 2``) with the GAP model, the central-atom
+Hessian and the neighbour-list builder (a ``config2`` block, or the line's own
+extra keys when ``--config 2`` is the workload).
 
 Prints one JSON line (rank 0, N = 1). ``--gpus N`` with N > 1 is refused: this
 build runs on one GPU.
@@ -394,6 +395,12 @@ def main():
     w, keep, prm, cmp_, comp, n_soap, G = build_problem(args.config)
     log(f"sites {w.n_sites} pairs {w.n_pairs} n_soap {n_soap}")
     ctx, fp32, fp64, roofline = time_paths(w, prm, cmp_, n_soap, G, args.steps, args.warmup, args.config)
+    extras = {}
+    if args.config == 2 and not args.no_extras:
+        # the rows after the path on this workload itself: GAP model, Hessian, neighbour list
+        extras["gap_energy_forces"] = time_gap(ctx, w, prm, cmp_, n_soap, args.steps, args.warmup)
+        extras["central_hessian"] = time_hessian(w, comp, n_soap, max(3, args.steps // 4), not args.no_cpu)
+        extras["neighbour_list"] = time_neighbours(w)
     del ctx
     p = w.params
     n_max = sum(p.alpha_max)
@@ -410,6 +417,7 @@ def main():
         "fp32_adjoint": fp32,
         "fp64_full_jacobian": fp64,
     }
+    out.update(extras)
     if not args.no_extras:
         out["max_rel_err_vs_fp64_oracle"] = accuracy(args.config)
     threads = min(os.cpu_count() or 1, 16)
