@@ -1181,8 +1181,12 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
             const double c0 = rr[0] * vr + rr[3] * vo + rr[6] * va;
             const double c1 = rr[1] * vr + rr[4] * vo + rr[7] * va;
             const double c2 = rr[2] * vr + rr[5] * vo;
+            // soap_cart_der is written once and read by the caller's contraction only: stores that do
+            // not keep its lines in L2 over the batches' angular and radial rows
             double* C = cart + (p * n_soap + s) * 3;
-            C[0] = c0; C[1] = c1; C[2] = c2;
+            __builtin_nontemporal_store(c0, C);
+            __builtin_nontemporal_store(c1, C + 1);
+            __builtin_nontemporal_store(c2, C + 2);
             selfacc[u][0] = selfacc[u][0] - c0;
             selfacc[u][1] = selfacc[u][1] - c1;
             selfacc[u][2] = selfacc[u][2] - c2;

@@ -43,6 +43,20 @@ def test_cell_list_equals_brute_force(cfg, n_sites):
     assert np.array_equal(got[2], want[2])
 
 
+def test_small_box_single_cell_branch_equals_brute_force():
+    # a box under three cutoffs wide: the generator takes every atom as a candidate (one cell)
+    from soap_turbo_amd.workload import BoxSpec
+    spec = BoxSpec(40, 0.02, 1.2, [0.5, 0.5], [1, 8], seed=3)
+    L, pos, sp = place_atoms(spec)
+    assert 2 * 4.5 < L < 3 * 4.5
+    site_atom = np.arange(spec.n_atoms)
+    got = build_neighbours(pos, L, 4.5, site_atom)
+    want = brute_force(pos, L, 4.5, site_atom)
+    assert want[0].max() > 3
+    for g, x in zip(got, want):
+        assert np.array_equal(g, x)
+
+
 def test_slab_box_equals_brute_force_on_both_phases():
     # config 4: sites from the dense slab and the dilute gas (0 to ~10 neighbours)
     from soap_turbo_amd.workload import slab_sites
