@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -22,20 +23,28 @@ namespace st {
 
 // One of two or four arrays by a lane-dependent index, chosen on the bits of the addresses: a
 // select among the pointers themselves is compiled to a table in private memory, read per lane.
+// The result is a global-memory pointer (address space 1): a pointer formed from integers is
+// otherwise generic, and its loads are flat loads, which also count against the LDS wait counter
+// (an LDS wait then waits for them too).
 template <class T>
-__device__ inline const T* pick2(int which, const T* p0, const T* p1) {
-  const unsigned long long m = which != 0 ? ~0ull : 0ull;
-  return reinterpret_cast<const T*>((reinterpret_cast<unsigned long long>(p0) & ~m) |
-                                    (reinterpret_cast<unsigned long long>(p1) & m));
+using gptr = const __attribute__((address_space(1))) T*;
+__device__ inline double ld(gptr<double> p, size_t i) { return p[i]; }
+__device__ inline double2 ld(gptr<double2> p, size_t i) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  const d2v v = reinterpret_cast<const __attribute__((address_space(1))) d2v*>(p)[i];
+  return make_double2(v.x, v.y);
 }
 template <class T>
-__device__ inline const T* pick4(int which, const T* p0, const T* p1, const T* p2, const T* p3) {
+__device__ inline gptr<T> pick2(int which, const T* p0, const T* p1) {
+  const unsigned long long m = which != 0 ? ~0ull : 0ull;
+  return (gptr<T>)((reinterpret_cast<unsigned long long>(p0) & ~m) | (reinterpret_cast<unsigned long long>(p1) & m));
+}
+template <class T>
+__device__ inline gptr<T> pick4(int which, const T* p0, const T* p1, const T* p2, const T* p3) {
   const unsigned long long m0 = which == 0 ? ~0ull : 0ull, m1 = which == 1 ? ~0ull : 0ull;
   const unsigned long long m2 = which == 2 ? ~0ull : 0ull, m3 = which == 3 ? ~0ull : 0ull;
-  return reinterpret_cast<const T*>((reinterpret_cast<unsigned long long>(p0) & m0) |
-                                    (reinterpret_cast<unsigned long long>(p1) & m1) |
-                                    (reinterpret_cast<unsigned long long>(p2) & m2) |
-                                    (reinterpret_cast<unsigned long long>(p3) & m3));
+  return (gptr<T>)((reinterpret_cast<unsigned long long>(p0) & m0) | (reinterpret_cast<unsigned long long>(p1) & m1) |
+                   (reinterpret_cast<unsigned long long>(p2) & m2) | (reinterpret_cast<unsigned long long>(p3) & m3));
 }
 
 __device__ __forceinline__ int site_of(const int* site_ptr, int n_sites, int p) {
@@ -984,25 +993,43 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
   // batch inputs: global -> registers (issue), registers -> LDS buffer (complete)
   double2 fa[FA];
   double fr[FR], fo = 0.0;
+  // each register slot's element of a batch: pair b, array (kind or d) and row entry (k or n), packed
+  // as b << 24 | array << 16 | entry (-1: none). Made opaque where it is used, so that the array
+  // pointers are formed there: hoisted out of the batch loop they were kept in scratch memory, and a
+  // scratch reload waits for every earlier vector memory operation, the batch's stores included
+  int aslot[FA], rslot[FR];
+#pragma unroll
+  for (int it = 0; it < FA; ++it) {
+    const int t = threadIdx.x + it * NT;
+    const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
+    aslot[it] = t < nA ? (b << 24) | (kind << 16) | k : -1;
+  }
+#pragma unroll
+  for (int it = 0; it < FR; ++it) {
+    const int t = threadIdx.x + it * NT;
+    const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
+    rslot[it] = t < nR ? (b << 24) | (d << 16) | n : -1;
+  }
   auto fetch = [&](int pb) {
     const int nb = min(PB, p1 - pb);
 #pragma unroll
     for (int it = 0; it < FA; ++it) {
-      const int t = threadIdx.x + it * NT;
+      int sl = aslot[it];
+      asm volatile("" : "+v"(sl));
       fa[it] = make_double2(0.0, 0.0);
-      if (t < nA) {
-        const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
-        const double2* src = pick4(kind, ang, ang_r, ang_a, ang_p);
-        if (b < nb) fa[it] = src[(size_t)(pb + b) * k_max + k];
+      if (sl >= 0) {
+        const int b = sl >> 24, kind = (sl >> 16) & 255, k = sl & 0xffff;
+        if (b < nb) fa[it] = ld(pick4(kind, ang, ang_r, ang_a, ang_p), (size_t)(pb + b) * k_max + k);
       }
     }
 #pragma unroll
     for (int it = 0; it < FR; ++it) {
-      const int t = threadIdx.x + it * NT;
+      int sl = rslot[it];
+      asm volatile("" : "+v"(sl));
       fr[it] = 0.0;
-      if (t < nR) {
-        const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
-        if (b < nb) fr[it] = pick2(d, rad, rad_der)[(size_t)(pb + b) * n_max + n];
+      if (sl >= 0) {
+        const int b = sl >> 24, d = (sl >> 16) & 255, n = sl & 0xffff;
+        if (b < nb) fr[it] = ld(pick2(d, rad, rad_der), (size_t)(pb + b) * n_max + n);
       }
     }
     fo = 0.0;
@@ -1014,28 +1041,22 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     double* sR = sRbuf + buf * n_max * RS;
 #pragma unroll
     for (int it = 0; it < FA; ++it) {
-      const int t = threadIdx.x + it * NT;
-      if (t < nA) {
-        const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
-        sA[k * AS + 4 * b + kind] = fa[it];
-      }
+      const int sl = aslot[it];
+      if (sl >= 0) sA[(sl & 0xffff) * AS + 4 * (sl >> 24) + ((sl >> 16) & 255)] = fa[it];
     }
     for (int t = threadIdx.x + FA * NT; t < nA; t += NT) {
       const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
-      const double2* src = pick4(kind, ang, ang_r, ang_a, ang_p);
-      sA[k * AS + 4 * b + kind] = b < nb ? src[(size_t)(pb + b) * k_max + k] : make_double2(0.0, 0.0);
+      const auto src = pick4(kind, ang, ang_r, ang_a, ang_p);
+      sA[k * AS + 4 * b + kind] = b < nb ? ld(src, (size_t)(pb + b) * k_max + k) : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int it = 0; it < FR; ++it) {
-      const int t = threadIdx.x + it * NT;
-      if (t < nR) {
-        const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
-        sR[n * RS + 2 * b + d] = fr[it];
-      }
+      const int sl = rslot[it];
+      if (sl >= 0) sR[(sl & 0xffff) * RS + 2 * (sl >> 24) + ((sl >> 16) & 255)] = fr[it];
     }
     for (int t = threadIdx.x + FR * NT; t < nR; t += NT) {
       const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
-      sR[n * RS + 2 * b + d] = b < nb ? pick2(d, rad, rad_der)[(size_t)(pb + b) * n_max + n] : 0.0;
+      sR[n * RS + 2 * b + d] = b < nb ? ld(pick2(d, rad, rad_der), (size_t)(pb + b) * n_max + n) : 0.0;
     }
     if (threadIdx.x < PB * 8) rot[buf][threadIdx.x / 8][threadIdx.x % 8] = fo;
   };
@@ -1087,29 +1108,43 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
         const int l = l_max - q / ntile, tile = q - (q / ntile) * ntile;
         const int kb = l * (l + 1) / 2, nK = 2 * (l + 1), nsteps = (nK + 3) >> 2;
         const int n = tile * 16 + r16;
-        constexpr int MAXS = 7;   // l_max <= 12
-        double av[MAXS], bv[MAXS];
-#pragma unroll
-        for (int st = 0; st < MAXS; ++st) {
-          const int kappa = 4 * st + g, m = kappa >> 1;
-          av[st] = 0.0;
-          bv[st] = 0.0;
-          if (st < nsteps && kappa < nK) {
-            const double wm = m == 0 ? 1.0 : 2.0;
-            if (n < n_max) {
-              const double2 c = cn[n * k_max + kb + m];
-              av[st] = wm * (kappa & 1 ? c.y : c.x);
-            }
-            if (r16 < 4 * PB) {
-              const double2 t = sA[(kb + m) * AS + r16];
-              bv[st] = kappa & 1 ? t.y : t.x;
-            }
-          }
-        }
+        // The job's product with its step count NS fixed at compile time: every operand load (one
+        // double: the real or imaginary part of the entry) is issued before any is used, with the row,
+        // column and m clamped into range and the terms they stand in for zeroed, so the job waits for
+        // LDS once. (Loads under per-step conditions were each waited for on their own.)
+        const int nc = n < n_max ? n : n_max - 1, rc = r16 < 4 * PB ? r16 : 0;
+        const double* cnd = reinterpret_cast<const double*>(cn);
+        const double* sAd = reinterpret_cast<const double*>(sA);
         f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+        auto product = [&](auto ns_) {
+          constexpr int NS = decltype(ns_)::value;
+          double av[NS], bv[NS];
 #pragma unroll
-        for (int st = 0; st < MAXS; ++st)
-          if (st < nsteps) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st], bv[st], acc, 0, 0, 0);
+          for (int st = 0; st < NS; ++st) {
+            const int kappa = 4 * st + g, m = min(kappa >> 1, l);
+            av[st] = cnd[2 * (nc * k_max + kb + m) + (kappa & 1)];
+            bv[st] = sAd[2 * ((kb + m) * AS + rc) + (kappa & 1)];
+          }
+#pragma unroll
+          for (int st = 0; st < NS; ++st) {
+            const int kappa = 4 * st + g, m = kappa >> 1;
+            const bool ok = kappa < nK;
+            const double wm = m == 0 ? 1.0 : 2.0;
+            av[st] = ok && n < n_max ? wm * av[st] : 0.0;
+            bv[st] = ok && r16 < 4 * PB ? bv[st] : 0.0;
+          }
+#pragma unroll
+          for (int st = 0; st < NS; ++st) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st], bv[st], acc, 0, 0, 0);
+        };
+        switch (nsteps) {  // l_max <= 12: at most 7 steps
+          case 1: product(std::integral_constant<int, 1>{}); break;
+          case 2: product(std::integral_constant<int, 2>{}); break;
+          case 3: product(std::integral_constant<int, 3>{}); break;
+          case 4: product(std::integral_constant<int, 4>{}); break;
+          case 5: product(std::integral_constant<int, 5>{}); break;
+          case 6: product(std::integral_constant<int, 6>{}); break;
+          default: product(std::integral_constant<int, 7>{}); break;
+        }
         // D(row, col) sits in lane col + 16 (row % 4), element row / 4
         if (r16 < 4 * PB) {
 #pragma unroll
@@ -1158,6 +1193,14 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
       const double t = wave_multi_sum<NRED>(dots, lane);
       if ((lane & (64 / NRED - 1)) == 0) red[w][wave_multi_index<NRED>(lane)] = t;
     }
+    // the next batch's inputs into the other buffer (last read by the batch before, which every
+    // thread finished at this batch's first barrier). Here, before this batch's stores: vector memory
+    // operations complete in order, so a wait for these loads placed after the stores would also wait
+    // for the stores to be written. The explicit wait (vmcnt(0)) is on every path, so that no load is
+    // pending past it in the compiler's view: it would otherwise wait again at the loop top, for the
+    // stores too, before the fetch registers are reused
+    __builtin_amdgcn_s_waitcnt(0x0f70);
+    if (more) commit(pb + PB, buf ^ 1);
     __syncthreads();
     // pass 2: normalise, rotate, store, accumulate the self column
 #pragma unroll
@@ -1194,13 +1237,6 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
         }
       }
     }
-    // the next batch's inputs into the other buffer, as late as possible: their loads, issued at the
-    // start of this batch, have had the whole batch to arrive (the buffer was last read by the
-    // batch before, which every thread finished at this batch's first barrier)
-    if (more) commit(pb + PB, buf ^ 1);
-    // the loop-top barrier publishes these writes only once they have completed; the compiler
-    // leaves out this wait on the loop's back edge
-    __builtin_amdgcn_s_waitcnt(0xc07f);
   }
 #pragma unroll
   for (int u = 0; u < NSPT; ++u) {
@@ -1511,11 +1547,11 @@ __global__ __launch_bounds__(256, 2) void jacobian_wave_k(
   for (int p = p0 + 1 + w; p < p1; p += 4) {
     for (int t = lane; t < 4 * k_max; t += 64) {
       const int kind = t / k_max, k = t - kind * k_max;
-      const double2* src = pick4(kind, ang, ang_r, ang_a, ang_p);
-      sA[t] = src[(size_t)p * k_max + k];
+      const auto src = pick4(kind, ang, ang_r, ang_a, ang_p);
+      sA[t] = ld(src, (size_t)p * k_max + k);
     }
     for (int t = lane; t < 2 * n_max; t += 64)
-      sR[t] = pick2(t < n_max ? 0 : 1, rad, rad_der)[(size_t)p * n_max + t % n_max];
+      sR[t] = ld(pick2(t < n_max ? 0 : 1, rad, rad_der), (size_t)p * n_max + t % n_max);
     if (lane < 8) srot[lane] = rotp[(size_t)p * 8 + lane];
     wave_sync();
     // P, Q, Qa, Qo(n, l) = sum over m <= l of w(m) Re(conj(A_kind(l, m)) c(n, l, m))
