@@ -274,9 +274,11 @@ struct SiteShape {
 // LDS row strides: PTF (= 4 mod 64 words) for the forward R and A rows, so the
 // 16 x 4 MFMA operand reads hit 64 distinct banks; PTS (odd) elsewhere; C rows
 // CSTR + 4 and H rows 2 KMP + 16 for the same reason.
-// FO: descriptors only (q and |p|; no dE/dq read, no backward pass), a separate instantiation so
-// that the full kernel's code is the same as without it.
-template <int L, int NTT, bool FO>
+// MODE: 0 the full kernel; 1 the full kernel for n_max <= 16 NTT - 8, whose products over n (H,
+// U and V) take two K steps of 4 fewer; 2 (FO) descriptors only (q and |p|; no dE/dq read, no
+// backward pass). Separate instantiations, so that each holds only the code it runs: with both K
+// forms in one kernel, the registers the longer form needs spill in the shorter one's runs too.
+template <int L, int NTT, int MODE>
 __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_site_k(
     AdjParams P, const int* __restrict__ order, const int* __restrict__ site_ptr, const float* __restrict__ rtab,
     const float* __restrict__ atab, const float* __restrict__ preflm_g, const float* __restrict__ cenv,
@@ -286,14 +288,15 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     const int* __restrict__ unc_ptr, const int* __restrict__ unc_e, const int32_t* __restrict__ P_i,
     const float* __restrict__ P_el, const int2* __restrict__ odir, float* __restrict__ q_out,
     float* __restrict__ sdp_out, float* __restrict__ pair_out, unsigned long long* __restrict__ tickets) {
+  constexpr bool FO = MODE == 2;
   using Sh = SiteShape<L>;
   constexpr int KM = Sh::KM, KM1 = Sh::KM1, L1 = Sh::L1, CSTR = Sh::CSTR, CT = Sh::CT;
   constexpr int CS = Sh::CS, KMP = Sh::KMP, HS = Sh::HS, KT = KMP / 16;
   constexpr int NC = 2 * KM;                        // live columns of C
   constexpr int TPW = (NTT * CT + 3) / 4;           // cnk tiles per wavefront
   constexpr int KN = 4 * NTT;                       // K steps over n (padded to 16 NTT)
-  constexpr int QB = 4;                             // radial functions per batch of table loads
-  constexpr int QK = 3 * QB;                        // radial functions per item (alpha_max <= 11)
+  constexpr int QB = 6;                             // radial functions per batch of table loads
+  constexpr int QK = 2 * QB;                        // radial functions per item (alpha_max <= 11)
   constexpr int QKEEP = 8;                          // ... kept in registers for the backward pass
   constexpr int GIT = (L1 + 3) / 4;                 // l per (pair, l-chunk) item, l <= L
   constexpr int FIT = m_deal_items(L + 1);          // forward (m, pair) items per thread, m <= L
@@ -472,15 +475,19 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   };
   // (pair, l-chunk) items: g_l(r), and dg_l/dr if Gdr is given (both kept in kg/kgd with keep), for
   // l = GIT wv .. GIT wv + GIT - 1, whose cubics lie side by side in the pair's table row
-  auto gl_items = [&](int tid, float* Gr, float* Gdr, bool keep) {
+  // (gl_load issues the item's table loads; they are issued before the radial items', so that both
+  // sets of loads are in flight together)
+  auto gl_load = [&](int tid, float4 (&gc)[GIT]) {
     const int pl = tid & (PT - 1), l0 = GIT * __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool live = pr_msk[pl] != 0;
-    const float ta = pr_ta[pl];
     const float4* row = reinterpret_cast<const float4*>(atab + pr_acell[pl]);
-    float4 gc[GIT];
 #pragma unroll
     for (int u = 0; u < GIT; ++u)
       gc[u] = (live && l0 + u < L1) ? row[l0 + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+  };
+  auto gl_items = [&](int tid, const float4 (&gc)[GIT], float* Gr, float* Gdr, bool keep) {
+    const int pl = tid & (PT - 1), l0 = GIT * __builtin_amdgcn_readfirstlane(tid >> 6);
+    const float ta = pr_ta[pl];
 #pragma unroll
     for (int u = 0; u < GIT; ++u) {
       const int l = l0 + u;
@@ -557,8 +564,10 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     // the thread index made opaque per tile: item addresses are formed here, not hoisted
     int tid = tx;
     asm volatile("" : "+v"(tid));
+    float4 gc[GIT];
+    gl_load(tid, gc);
     radial_items(tid, fourpi, Rt, nullptr, PTF, KEEP && tb == last_tb);
-    gl_items(tid, Gt, nullptr, KEEP && tb == last_tb);
+    gl_items(tid, gc, Gt, nullptr, KEEP && tb == last_tb);
     __syncthreads();
     // (m, pair) items: A_k = preflm P_l^m g_l e^{-i m phi} for l = m .. L, as (re, im) rows.
     // m is uniform over a wavefront (columns dealt by length, m_deal); l is unrolled, P_l^m by the
@@ -928,7 +937,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             for (int s = 0; s < K; ++s) acc[a] = mfma4(av[s], bv[jj][s], acc[a]);
           }
         };
-        if (n_max <= 4 * (KN - 2)) chains(std::integral_constant<int, KN - 2>{});
+        if constexpr (MODE == 1) chains(std::integral_constant<int, KN - 2>{});
         else chains(std::integral_constant<int, KN>{});
         if (c < cw) {
           // the column's weight, with the k's prefactor of A folded in
@@ -994,8 +1003,10 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       if (tb - PT >= p0) fetch(i, tb - PT, p1, false);
       else fetch_next_site();
       __syncthreads();
+      float4 gc[GIT];
+      gl_load(tid, gc);
       radial_items(tid, 1.f, Rb, Rdb, PTS, false);
-      gl_items(tid, Gl, Gd, false);
+      gl_items(tid, gc, Gl, Gd, false);
     }
     // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
 #pragma unroll
@@ -1108,7 +1119,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
           }
         }
       };
-      if (n_max <= 4 * (KN - 2)) uv(std::integral_constant<int, KN - 2>{});
+      if constexpr (MODE == 1) uv(std::integral_constant<int, KN - 2>{});
       else uv(std::integral_constant<int, KN>{});
       // sums over the 16 k of a row: prefix sums along the DPP row, total in lane 15
 #pragma unroll
@@ -1201,21 +1212,23 @@ void launch_site(bool fo, hipStream_t st, const AdjParams& P, size_t sh, const i
                  const int4* comp, const int* out_ptr, const int* out_c, const float* out_el, const int* unc_ptr,
                  const int* unc_e, const int32_t* P_i, const float* P_el, const int2* odir, float* q, float* sdp,
                  float* pair, unsigned long long* tickets, int n_cu, int per_cu_cap) {
-  const void* fn = fo ? reinterpret_cast<const void*>(&adjoint_site_k<L, NTT, true>)
-                     : reinterpret_cast<const void*>(&adjoint_site_k<L, NTT, false>);
-  if (sh > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTHREADS, sh) != hipSuccess || per_cu < 1) per_cu = 1;
-  if (per_cu_cap > 0 && per_cu > per_cu_cap) per_cu = per_cu_cap;
-  const int grid = std::max(1, std::min(P.n_sites, per_cu * n_cu));
-  if (fo)
-    adjoint_site_k<L, NTT, true><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask, rjs,
-                                                           th, ph, G32, comp, out_ptr, out_c, out_el, unc_ptr, unc_e,
-                                                           P_i, P_el, odir, q, sdp, pair, tickets);
-  else
-    adjoint_site_k<L, NTT, false><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask,
-                                                            rjs, th, ph, G32, comp, out_ptr, out_c, out_el, unc_ptr,
-                                                            unc_e, P_i, P_el, odir, q, sdp, pair, tickets);
+  // the short K form when every site's n rows fit in 4 NTT - 2 steps of 4
+  const int mode = fo ? 2 : P.n_max <= 4 * (4 * NTT - 2) ? 1 : 0;
+  auto run = [&](auto mode_) {
+    constexpr int M = decltype(mode_)::value;
+    const void* fn = reinterpret_cast<const void*>(&adjoint_site_k<L, NTT, M>);
+    if (sh > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTHREADS, sh) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (per_cu_cap > 0 && per_cu > per_cu_cap) per_cu = per_cu_cap;
+    const int grid = std::max(1, std::min(P.n_sites, per_cu * n_cu));
+    adjoint_site_k<L, NTT, M><<<grid, NTHREADS, sh, st>>>(P, order, site_ptr, rtab, atab, preflm, cenv, spmask, rjs, th,
+                                                         ph, G32, comp, out_ptr, out_c, out_el, unc_ptr, unc_e, P_i,
+                                                         P_el, odir, q, sdp, pair, tickets);
+  };
+  if (mode == 2) run(std::integral_constant<int, 2>{});
+  else if (mode == 1) run(std::integral_constant<int, 1>{});
+  else run(std::integral_constant<int, 0>{});
 }
 
 // host copy of the per-call constant tables that live next to the FP64 ones
