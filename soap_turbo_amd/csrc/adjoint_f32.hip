@@ -247,6 +247,41 @@ __device__ __forceinline__ int dealt_m(int u, int w) {
   return (int)((word >> (8 * w)) & 0xffu);
 }
 
+// For a tile of at most 32 pairs: each wavefront takes two columns at a time, one per half (lanes
+// 0-31 and 32-63), and a round costs the longer one. Columns are paired in order of length, (0, 1),
+// (2, 3), ..., and the pairs dealt longest first to the least loaded wavefront. Word u: byte
+// 2 w + h is the m of half h of wavefront w in round u (0xff: none).
+constexpr unsigned long long m_deal2(int nm, int u) {
+  int load[4] = {0, 0, 0, 0}, cnt[4] = {0, 0, 0, 0};
+  unsigned long long word = ~0ull;
+  for (int m = 0; m < nm; m += 2) {
+    int b = 0;
+    for (int w = 1; w < 4; ++w)
+      if (load[w] < load[b]) b = w;
+    if (cnt[b] == u) {
+      word = (word & ~(0xffull << (16 * b))) | ((unsigned long long)m << (16 * b));
+      if (m + 1 < nm) word = (word & ~(0xffull << (16 * b + 8))) | ((unsigned long long)(m + 1) << (16 * b + 8));
+    }
+    ++cnt[b];
+    load[b] += nm - m;
+  }
+  return word;
+}
+constexpr int m_deal2_items(int nm) {
+  int u = 0;
+  while (m_deal2(nm, u) != ~0ull) ++u;
+  return u;
+}
+static_assert(m_deal2_items(9) == 2 && m_deal2(9, 0) == 0x0706050403020100ull && m_deal2(9, 1) == 0xff08ffffffffffffull,
+              "l_max 8: rounds {0|1}, {2|3}, {4|5}, {6|7} then {8} (9, 7, 5, 3 + 1 entries)");
+template <int NM>
+__device__ __forceinline__ int dealt2_m(int u, int w, int h) {
+  constexpr unsigned long long W0 = m_deal2(NM, 0), W1 = m_deal2(NM, 1), W2 = m_deal2(NM, 2);
+  static_assert(m_deal2_items(NM) <= 3, "at most three rounds per wavefront");
+  const unsigned long long word = u == 0 ? W0 : u == 1 ? W1 : W2;
+  return (int)((word >> (16 * w + 8 * h)) & 0xffu);
+}
+
 template <int L>
 struct SiteShape {
   static constexpr int KM = (L + 1) * (L + 2) / 2;    // (l, m) with m <= l <= L
@@ -301,6 +336,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   constexpr int GIT = (L1 + 3) / 4;                 // l per (pair, l-chunk) item, l <= L
   constexpr int FIT = m_deal_items(L + 1);          // forward (m, pair) items per thread, m <= L
   constexpr int PIT = m_deal_items(L + 2);          // backward (m, pair) items per thread, m <= L + 1
+  constexpr int FIT2 = m_deal2_items(L + 1);        // ... for a tile of at most 32 pairs (two columns
+  constexpr int PIT2 = m_deal2_items(L + 2);        //     per wavefront at a time)
   constexpr uint32_t SELF = 1u << 31;               // pair flag: the site's own pair
   extern __shared__ float smem[];
   __shared__ float red[16];
@@ -573,10 +610,9 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     // m is uniform over a wavefront (columns dealt by length, m_deal); l is unrolled, P_l^m by the
     // recursions of get_plm_array (src/soap_turbo_angular.f90:108-160): P_m^m, P_{m+1}^m =
     // (2m+1) x P_m^m, three-term in l.
-#pragma unroll
-    for (int u = 0; u < FIT; ++u) {
-      const int pl = tid & (PT - 1), m = dealt_m<L1>(u, __builtin_amdgcn_readfirstlane(tid >> 6));  // m uniform
-      if (m < L1) {
+    // A tile of at most 32 pairs takes two columns per wavefront at a time (m per half-wave).
+    auto fwd_item = [&](int pl, int m) {
+      {
         const bool live = pr_msk[pl] != 0;
         float gv[L1];
 #pragma unroll
@@ -603,6 +639,19 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             At[(2 * k + 1) * PTF + pl] = v * e.y;
           }
         }
+      }
+    };
+    if (np_ <= 32) {
+#pragma unroll
+      for (int u = 0; u < FIT2; ++u) {
+        const int m = dealt2_m<L1>(u, __builtin_amdgcn_readfirstlane(tid >> 6), (tid >> 5) & 1);
+        if (m < L1) fwd_item(tid & 31, m);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < FIT; ++u) {
+        const int m = dealt_m<L1>(u, __builtin_amdgcn_readfirstlane(tid >> 6));  // m uniform
+        if (m < L1) fwd_item(tid & (PT - 1), m);
       }
     }
     __syncthreads();
@@ -1009,10 +1058,9 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       gl_items(tid, gc, Gl, Gd, false);
     }
     // (m, pair) items, m <= L + 1: column m of P_l^m (l <= L + 1); e^{-i m phi} for m <= L
-#pragma unroll
-    for (int u = 0; u < PIT; ++u) {
-      const int pl = tid & (PT - 1), m = dealt_m<L + 2>(u, __builtin_amdgcn_readfirstlane(tid >> 6));  // m uniform
-      if (m <= L + 1) {
+    // (a tile of at most 32 pairs: two columns per wavefront at a time, as in the forward pass)
+    auto bwd_item = [&](int pl, int m) {
+      {
         // l unrolled: the recursion's coefficients are read ahead of the chain that uses them
         const bool live = pr_msk[pl] != 0 && tb + pl != p0;
         const float x = pr_ct[pl], sq = pr_st[pl];
@@ -1036,21 +1084,41 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         }
         if (m <= L) Em[m * PTS + pl] = live ? exp_mphi(m, pr_cf[pl], pr_sf[pl]) : make_float2(0.f, 0.f);
       }
+    };
+    if (p1 - tb <= 32) {
+#pragma unroll
+      for (int u = 0; u < PIT2; ++u) {
+        const int m = dealt2_m<L + 2>(u, __builtin_amdgcn_readfirstlane(tid >> 6), (tid >> 5) & 1);
+        if (m <= L + 1) bwd_item(tid & 31, m);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < PIT; ++u) {
+        const int m = dealt_m<L + 2>(u, __builtin_amdgcn_readfirstlane(tid >> 6));  // m uniform
+        if (m <= L + 1) bwd_item(tid & (PT - 1), m);
+      }
     }
     __syncthreads();
     // wavefront wv: pairs 16 wv .. 16 wv + 15 of the tile. Per tile of 16 k: U = R H and
     // V = R' H for the real and the imaginary columns; lane li then owns k = 16 kt + li. A
     // wavefront whose pairs all lie past the tile's last one has nothing to do.
-    if (16 * __builtin_amdgcn_readfirstlane(wv) < p1 - tb) {
+    // A tile of at most 32 pairs (KT > 1): wavefronts w and w + 2 share the rows 16 w .. 16 w + 15,
+    // w taking the k tiles below KS and w + 2 the rest; w + 2's sums go to the upper half of fout.
+    constexpr int KS = KT / 2;
+    const int wq = __builtin_amdgcn_readfirstlane(wv);
+    const bool ksplit = KT > 1 && p1 - tb <= 32;
+    const int rw = ksplit ? (wq & 1) : wq;  // this wavefront's row group
+    if (16 * rw < p1 - tb) {
       float fr[4] = {0.f, 0.f, 0.f, 0.f}, fa[4] = {0.f, 0.f, 0.f, 0.f}, fo[4] = {0.f, 0.f, 0.f, 0.f};
       // lane coordinates made opaque here, so that the LDS addresses derived from them are
       // formed in the loop rather than hoisted out of it into long-lived registers
-      int lrow = 16 * wv + 4 * lk, lcol = li;
+      int lrow = 16 * rw + 4 * lk, lcol = li;
       asm volatile("" : "+v"(lrow), "+v"(lcol));
-      const int pa = 16 * wv + lcol;  // A-operand row (pair) of this lane
-      // U and V over n < 4 K (K steps of 4; the steps past n_max add nothing), then the contraction
-      auto uv = [&](auto kn) {
-        constexpr int K = decltype(kn)::value;
+      const int pa = 16 * rw + lcol;  // A-operand row (pair) of this lane
+      // U and V over n < 4 K (K steps of 4; the steps past n_max add nothing) for the k tiles
+      // KA .. KB - 1, then the contraction
+      auto uv = [&](auto kn, auto ka_, auto kb_) {
+        constexpr int K = decltype(kn)::value, KA = decltype(ka_)::value, KB = decltype(kb_)::value;
         float ra[K], rda[K];
 #pragma unroll
         for (int s = 0; s < K; ++s) {
@@ -1058,14 +1126,14 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
           ra[s] = n < n_max ? Rb[n * PTS + pa] : 0.f;
           rda[s] = n < n_max ? Rdb[n * PTS + pa] : 0.f;
         }
-        constexpr int KG = KT < 3 ? KT : 3;
+        constexpr int KG = KB - KA < 3 ? KB - KA : 3;
 #pragma unroll
-        for (int kt0 = 0; kt0 < KT; kt0 += KG) {
+        for (int kt0 = KA; kt0 < KB; kt0 += KG) {
           floatx4 Ur[KG], Ui[KG], Vr[KG], Vi[KG];
 #pragma unroll
           for (int g = 0; g < KG; ++g) {
             Ur[g] = Ui[g] = Vr[g] = Vi[g] = floatx4{0.f, 0.f, 0.f, 0.f};
-            if (kt0 + g < KT) {
+            if (kt0 + g < KB) {
               const int k = 16 * (kt0 + g) + lcol;
               float hr[K], hm[K];
 #pragma unroll
@@ -1086,7 +1154,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
 #pragma unroll
           for (int g = 0; g < KG; ++g) {
             const int k = 16 * (kt0 + g) + lcol;
-            const bool act = kt0 + g < KT && k < KM;
+            const bool act = kt0 + g < KB && k < KM;
             const int lm = act ? klm[k] : 0;
             const int l = lm & 255, m = lm >> 8;
             const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
@@ -1119,14 +1187,25 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
           }
         }
       };
-      if constexpr (MODE == 1) uv(std::integral_constant<int, KN - 2>{});
-      else uv(std::integral_constant<int, KN>{});
+      auto uv_k = [&](auto kn) {
+        if constexpr (KT > 1) {
+          if (ksplit) {
+            if (wq < 2) uv(kn, std::integral_constant<int, 0>{}, std::integral_constant<int, KS>{});
+            else uv(kn, std::integral_constant<int, KS>{}, std::integral_constant<int, KT>{});
+            return;
+          }
+        }
+        uv(kn, std::integral_constant<int, 0>{}, std::integral_constant<int, KT>{});
+      };
+      if constexpr (MODE == 1) uv_k(std::integral_constant<int, KN - 2>{});
+      else uv_k(std::integral_constant<int, KN>{});
       // sums over the 16 k of a row: prefix sums along the DPP row, total in lane 15
+      const int fo0 = ksplit && wq >= 2 ? 32 : 0;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float tr = row_total(fr[v]), ta = row_total(fa[v]), to = row_total(fo[v]);
         if (li == 15) {
-          const int lp = 16 * wv + 4 * lk + v;
+          const int lp = fo0 + 16 * rw + 4 * lk + v;
           fout[lp] = tr;
           fout[PT + lp] = ta;
           fout[2 * PT + lp] = to;
@@ -1136,7 +1215,12 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     __syncthreads();
     const int p = tb + lane;
     if (wv == 0 && p < p1 && p != p0) {
-      const float f_rad = fout[lane], f_azi = fout[PT + lane], f_pol = fout[2 * PT + lane];
+      float f_rad = fout[lane], f_azi = fout[PT + lane], f_pol = fout[2 * PT + lane];
+      if (ksplit) {  // (lane < 32 here) the upper k tiles' sums
+        f_rad += fout[32 + lane];
+        f_azi += fout[PT + 32 + lane];
+        f_pol += fout[2 * PT + 32 + lane];
+      }
       const float st_ = pr_st[lane], ct_ = pr_ct[lane], sf_ = pr_sf[lane], cf_ = pr_cf[lane];
       const float inv_r = 1.f / pr_r[lane];
       const float x0 = st_ * cf_ * f_rad - ct_ * cf_ * inv_r * f_pol - sf_ * inv_r * f_azi;
