@@ -947,10 +947,16 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       return n;
     }();
     constexpr int JHW = (NJH + 3) / 4;
+    // When the last round has fewer jobs than wavefronts and their row tiles number at most four, the
+    // row tiles are dealt one per wavefront (wavefront w: job w / NTT of the round, row tile w % NTT)
+    constexpr int RL = NJH - 4 * (JHW - 1);
+    constexpr bool RSPLIT = RL < 4 && RL * NTT <= 4;
+    const int wq = __builtin_amdgcn_readfirstlane(wv);
+    auto job_of = [&](int jj) { return RSPLIT && jj == JHW - 1 ? 4 * jj + wq / NTT : wq + 4 * jj; };
     float bv[JHW][KN];
 #pragma unroll
     for (int jj = 0; jj < JHW; ++jj) {
-      const int job = wv + 4 * jj;
+      const int job = job_of(jj);
       int l = 0, rem = job < NJH ? job : 0;
       while (rem >= (2 * (l + 1) + 15) / 16) { rem -= (2 * (l + 1) + 15) / 16; ++l; }
       const int cb = l * (l + 1), cw = 2 * (l + 1), c = 16 * rem + li;
@@ -963,18 +969,21 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     __syncthreads();
 #pragma unroll
     for (int jj = 0; jj < JHW; ++jj) {
-      const int job = wv + 4 * jj;
+      const int job = job_of(jj);
+      // row tiles of this job: all, or the one dealt to this wavefront
+      const bool one = RSPLIT && jj == JHW - 1;
+      const int arow = one ? wq % NTT : 0;
       if (job < NJH) {
         int l = 0, rem = job;
         while (rem >= (2 * (l + 1) + 15) / 16) { rem -= (2 * (l + 1) + 15) / 16; ++l; }
         const int cw = 2 * (l + 1), c = 16 * rem + li;
         floatx4 acc[NTT];
         // the row tiles' chains over n' < 4 K (K steps of 4; the steps past n_max add nothing)
-        auto chains = [&](auto kn) {
-          constexpr int K = decltype(kn)::value;
+        auto chains = [&](auto kn, auto na_) {
+          constexpr int K = decltype(kn)::value, NA = decltype(na_)::value;
 #pragma unroll
-          for (int a = 0; a < NTT; ++a) {
-            const int n = 16 * a + li;
+          for (int a = 0; a < NA; ++a) {
+            const int n = 16 * (a + arow) + li;
             float av[K];
 #pragma unroll
             for (int s = 0; s < K; ++s) {
@@ -986,20 +995,25 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             for (int s = 0; s < K; ++s) acc[a] = mfma4(av[s], bv[jj][s], acc[a]);
           }
         };
-        if constexpr (MODE == 1) chains(std::integral_constant<int, KN - 2>{});
-        else chains(std::integral_constant<int, KN>{});
-        if (c < cw) {
-          // the column's weight, with the k's prefactor of A folded in
-          const int k = l * (l + 1) / 2 + (c >> 1), hc = (c & 1) * KMP + k;
-          const float w = (c >= 2 ? 2.f : 1.f) * fourpi * preflm[k];
+        auto rows = [&](auto na_) {
+          constexpr int NA = decltype(na_)::value;
+          if constexpr (MODE == 1) chains(std::integral_constant<int, KN - 2>{}, na_);
+          else chains(std::integral_constant<int, KN>{}, na_);
+          if (c < cw) {
+            // the column's weight, with the k's prefactor of A folded in
+            const int k = l * (l + 1) / 2 + (c >> 1), hc = (c & 1) * KMP + k;
+            const float w = (c >= 2 ? 2.f : 1.f) * fourpi * preflm[k];
 #pragma unroll
-          for (int a = 0; a < NTT; ++a)
+            for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int r = 16 * a + 4 * lk + v;
-              if (r < n_max) C[r * HS + hc] = w * acc[a][v];
-            }
-        }
+              for (int v = 0; v < 4; ++v) {
+                const int r = 16 * (a + arow) + 4 * lk + v;
+                if (r < n_max) C[r * HS + hc] = w * acc[a][v];
+              }
+          }
+        };
+        if (RSPLIT && one) rows(std::integral_constant<int, 1>{});
+        else rows(std::integral_constant<int, NTT>{});
       }
       __builtin_amdgcn_sched_barrier(0);  // one job's operands live at a time
     }
