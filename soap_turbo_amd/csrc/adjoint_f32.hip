@@ -663,22 +663,35 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     for (int j = 0; j < TPW; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
     for (int c0 = 0; c0 < npad; c0 += 16) {
       const int ns = (min(npad, c0 + 16) - c0) >> 2;
+      // every tile's operands of the batch loaded before any is used (rows and columns clamped into
+      // range, the terms past them zeroed), so that the batch waits for LDS once
+      float av[TPW][4], bv[TPW][4];
 #pragma unroll
       for (int j = 0; j < TPW; ++j) {
         const int t = wv + 4 * j;
-        if (t < NTT * CT) {
-          const int a = t / CT, b = t - a * CT;
-          const int n = 16 * a + li, col = 16 * b + li;
-          float av[4], bv[4];
+        const int a = t < NTT * CT ? t / CT : 0, b = t < NTT * CT ? t - a * CT : 0;
+        const int n = 16 * a + li, col = 16 * b + li;
+        const int nr = n < n_max ? n : 0, cr = col < NC ? col : 0;
 #pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            const int pp = c0 + 4 * s + lk;
-            av[s] = (s < ns && n < n_max) ? Rt[n * PTF + pp] : 0.f;
-            bv[s] = (s < ns && col < NC) ? At[col * PTF + pp] : 0.f;
-          }
-#pragma unroll
-          for (int s = 0; s < 4; ++s) acc[j] = mfma4(av[s], bv[s], acc[j]);
+        for (int s = 0; s < 4; ++s) {
+          const int pp = c0 + 4 * s + lk;
+          av[j][s] = Rt[nr * PTF + pp];
+          bv[j][s] = At[cr * PTF + pp];
         }
+      }
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) {
+        const int t = wv + 4 * j;
+        const int a = t / CT, b = t - a * CT;
+        const bool ok = t < NTT * CT;
+        const int n = 16 * a + li, col = 16 * b + li;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          av[j][s] = ok && s < ns && n < n_max ? av[j][s] : 0.f;
+          bv[j][s] = ok && s < ns && col < NC ? bv[j][s] : 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc[j] = mfma4(av[j][s], bv[j][s], acc[j]);
       }
     }
 #pragma unroll
