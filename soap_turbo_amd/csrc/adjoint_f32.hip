@@ -614,9 +614,18 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     auto fwd_item = [&](int pl, int m) {
       {
         const bool live = pr_msk[pl] != 0;
-        float gv[L1];
+        // g_l, the recursion's coefficients and the prefactors of the column, all read before the
+        // recursion (k clamped into the column for l < m, whose values are not used), so that the
+        // chain does not wait for LDS at every step
+        float gv[L1], ca[L1], cb[L1], pf[L1];
 #pragma unroll
-        for (int l = 0; l < L1; ++l) gv[l] = l >= m ? Gt[l * PTS + pl] : 0.f;
+        for (int l = 0; l < L1; ++l) {
+          const int kl = l * (l + 1) / 2 + (m < l ? m : l);
+          gv[l] = Gt[l * PTS + pl];
+          ca[l] = rec_a[kl];
+          cb[l] = rec_b[kl];
+          pf[l] = preflm[kl];
+        }
         const float x = pr_ct[pl], sq = pr_st[pl];
         const float2 e = exp_mphi(m, pr_cf[pl], pr_sf[pl]);
         float pmm = 1.f;
@@ -630,11 +639,11 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             float pc;
             if (l == m) pc = pmm;
             else if (l == m + 1) pc = x * (float)(2 * m + 1) * pmm;
-            else pc = rec_a[l * (l + 1) / 2 + m] * x * p1 - rec_b[l * (l + 1) / 2 + m] * p0;
+            else pc = ca[l] * x * p1 - cb[l] * p0;
             p0 = p1;
             p1 = pc;
             const int k = l * (l + 1) / 2 + m;
-            const float v = live ? preflm[k] * pc * gv[l] : 0.f;
+            const float v = live ? pf[l] * pc * gv[l] : 0.f;
             At[(2 * k) * PTF + pl] = v * e.x;
             At[(2 * k + 1) * PTF + pl] = v * e.y;
           }
@@ -992,20 +1001,31 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         const int cw = 2 * (l + 1), c = 16 * rem + li;
         floatx4 acc[NTT];
         // the row tiles' chains over n' < 4 K (K steps of 4; the steps past n_max add nothing)
+        // the column's weight, with the k's prefactor of A folded in (read with the operands)
+        const int kc = l * (l + 1) / 2 + (min(c, cw - 1) >> 1);
+        const float pf = preflm[kc];
         auto chains = [&](auto kn, auto na_) {
           constexpr int K = decltype(kn)::value, NA = decltype(na_)::value;
+          // every row tile's Y operands loaded before the first product (rows clamped into range,
+          // the terms past n_max zeroed), so that the job waits for LDS once
+          float av[NA][K];
+#pragma unroll
+          for (int a = 0; a < NA; ++a) {
+            const int n = 16 * (a + arow) + li, nr = n < n_max ? n : 0;
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+              const int np = 4 * s + lk, npr = np < n_max ? np : 0;
+              av[a][s] = Y[(l * n_max + nr) * n_max + npr];
+            }
+          }
 #pragma unroll
           for (int a = 0; a < NA; ++a) {
             const int n = 16 * (a + arow) + li;
-            float av[K];
 #pragma unroll
-            for (int s = 0; s < K; ++s) {
-              const int np = 4 * s + lk;
-              av[s] = (n < n_max && np < n_max) ? Y[(l * n_max + n) * n_max + np] : 0.f;
-            }
+            for (int s = 0; s < K; ++s) av[a][s] = n < n_max && 4 * s + lk < n_max ? av[a][s] : 0.f;
             acc[a] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < K; ++s) acc[a] = mfma4(av[s], bv[jj][s], acc[a]);
+            for (int s = 0; s < K; ++s) acc[a] = mfma4(av[a][s], bv[jj][s], acc[a]);
           }
         };
         auto rows = [&](auto na_) {
@@ -1013,9 +1033,8 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
           if constexpr (MODE == 1) chains(std::integral_constant<int, KN - 2>{}, na_);
           else chains(std::integral_constant<int, KN>{}, na_);
           if (c < cw) {
-            // the column's weight, with the k's prefactor of A folded in
             const int k = l * (l + 1) / 2 + (c >> 1), hc = (c & 1) * KMP + k;
-            const float w = (c >= 2 ? 2.f : 1.f) * fourpi * preflm[k];
+            const float w = (c >= 2 ? 2.f : 1.f) * fourpi * pf;
 #pragma unroll
             for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -1091,6 +1110,14 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         // l unrolled: the recursion's coefficients are read ahead of the chain that uses them
         const bool live = pr_msk[pl] != 0 && tb + pl != p0;
         const float x = pr_ct[pl], sq = pr_st[pl];
+        // the recursion's coefficients of the column, read before it (k clamped for l < m)
+        float ca[L + 2], cb[L + 2];
+#pragma unroll
+        for (int l = 0; l <= L + 1; ++l) {
+          const int kl = l * (l + 1) / 2 + (m < l ? m : l);
+          ca[l] = rec_a[kl];
+          cb[l] = rec_b[kl];
+        }
         float pmm = 1.f;
 #pragma unroll
         for (int j = 1; j <= L + 1; ++j)
@@ -1103,7 +1130,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             float pc;
             if (l == m) pc = pmm;
             else if (l == m + 1) pc = x * (float)(2 * m + 1) * pmm;
-            else pc = rec_a[k] * x * q1 - rec_b[k] * q0;
+            else pc = ca[l] * x * q1 - cb[l] * q0;
             q0 = q1;
             q1 = pc;
             Fp[k * PTS + pl] = live ? pc : 0.f;
@@ -1185,25 +1212,29 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
             const int lm = act ? klm[k] : 0;
             const int l = lm & 255, m = lm >> 8;
             const int kb = l * (l + 1) / 2, kb1 = (l + 1) * (l + 2) / 2;
-            if (act) {
+            // the P_l^m rows a k reads, clamped into range where it lacks them (m = 0, m = l), so that
+            // every load of a row is issued before the branch on act and on m
+            const int rp1 = kb + (m < l ? m + 1 : m), rm1 = kb + (m > 0 ? m - 1 : 0), r1m1 = kb1 + (m > 0 ? m - 1 : 0);
 #pragma unroll
-              for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < 4; ++v) {
+              const int lp = lrow + v;
+              const float gl = Gl[l * PTS + lp], gdl = Gd[l * PTS + lp];
+              const float2 e = Em[m * PTS + lp];
+              const float pl = Fp[(kb + m) * PTS + lp];
+              const float fp1 = Fp[rp1 * PTS + lp], fm1 = Fp[rm1 * PTS + lp];
+              const float f1m1 = Fp[r1m1 * PTS + lp], f1p1 = Fp[(kb1 + m + 1) * PTS + lp];
+              if (act) {
                 const float ur = Ur[g][v], ui = Ui[g][v], vr = Vr[g][v], vi = Vi[g][v];
-                const int lp = lrow + v;
-                const float gl = Gl[l * PTS + lp], gdl = Gd[l * PTS + lp];
-                const float2 e = Em[m * PTS + lp];
                 const float eu = e.x * ur + e.y * ui;     // Re(E conj u)
                 const float ev = e.x * vr + e.y * vi;     // Re(E conj v)
                 const float ex = e.x * ui - e.y * ur;     // Re(i E conj u)
-                const float pl = Fp[(kb + m) * PTS + lp];
-                const float pp1 = m < l ? Fp[(kb + m + 1) * PTS + lp] : 0.f;
+                const float pp1 = m < l ? fp1 : 0.f;
                 float dms, dsin = 0.f;
                 if (m == 0) {
                   dms = -pp1;
                 } else {
-                  dms = 0.5f * (float)((l + m) * (l - m + 1)) * Fp[(kb + m - 1) * PTS + lp] - 0.5f * pp1;
-                  dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * Fp[(kb1 + m - 1) * PTS + lp] +
-                         0.5f * Fp[(kb1 + m + 1) * PTS + lp];
+                  dms = 0.5f * (float)((l + m) * (l - m + 1)) * fm1 - 0.5f * pp1;
+                  dsin = 0.5f * (float)((l - m + 1) * (l - m + 2)) * f1m1 + 0.5f * f1p1;
                 }
                 // A = pl g E, A_r = pl g' E, A_pol = dms g E, A_azi = i dsin g E (the prefactor is in H)
                 fr[v] += pl * (gl * ev + gdl * eu);
