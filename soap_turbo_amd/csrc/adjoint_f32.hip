@@ -140,16 +140,30 @@ __global__ void ang_table_k(int n_nodes, double h, int L, double rcut2, double s
 // (t in [0, 1) across the cell) of the Hermite interpolant of the FP64 values
 // and derivatives at the cell's ends; the coefficients are formed in FP64, so
 // the FP32 derivative (c1 + 2 c2 t + 3 c3 t^2)/h has no cancellation.
+// The segment is the last whose lower edge r reaches (edges increase). s is uniform over the
+// wavefront: its fields are read with uniform indices and the lane's segment selected from them (a
+// lane-indexed read of the kernel argument is a vector memory load, which waits behind every earlier
+// load and store of the wavefront).
 __device__ __forceinline__ int radial_cell(const SpeciesTab& s, double r, float& t, float& ih) {
-  int j = 0;
-  while (j + 1 < s.nseg && r >= s.seg_lo[j + 1]) ++j;
-  const double x = (r - s.seg_lo[j]) * s.seg_inv_h[j];
+  double lo = s.seg_lo[0], inv_h = s.seg_inv_h[0];
+  int nn = s.seg_n[0], base = s.seg_base[0];
+  float ihf = s.seg_ih[0];
+#pragma unroll
+  for (int q = 1; q < MAXSEG; ++q)
+    if (q < s.nseg && r >= s.seg_lo[q]) {
+      lo = s.seg_lo[q];
+      inv_h = s.seg_inv_h[q];
+      nn = s.seg_n[q];
+      base = s.seg_base[q];
+      ihf = s.seg_ih[q];
+    }
+  const double x = (r - lo) * inv_h;
   int i = (int)x;
   if (i < 0) i = 0;
-  if (i > s.seg_n[j] - 2) i = s.seg_n[j] - 2;
+  if (i > nn - 2) i = nn - 2;
   t = (float)(x - (double)i);
-  ih = s.seg_ih[j];
-  return s.seg_base[j] + i;
+  ih = ihf;
+  return base + i;
 }
 
 // e^{-i m phi} by m complex multiplications from (cos phi, sin phi).
@@ -707,18 +721,26 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     for (int j = 0; j < TPW; ++j) two_sum_acc(hi[j], lo[j], acc[j]);
   }
   // C = hi + lo, plus the poly3gauss central-atom term on k = 0 (real part)
+  // (the central terms read for every row first, index clamped: a read under the column's
+  // condition was waited for before the next)
 #pragma unroll
   for (int j = 0; j < TPW; ++j) {
     const int t = wv + 4 * j;
     if (t < NTT * CT) {
       const int a = t / CT, b = t - a * CT;
       const int col = 16 * b + li;
+      float cen[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = 16 * a + 4 * lk + v;
+        cen[v] = cen_s[n < 64 ? n : 63];
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int n = 16 * a + 4 * lk + v;
         if (n < n_max) {
           float val = hi[j][v] + lo[j][v];
-          if (cenv && col == 0) val += cen_s[n];
+          if (cenv && col == 0) val += cen[v];
           C[n * CS + col] = val;
         }
       }
@@ -801,11 +823,18 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         const float* cc = C + y * CS + cb;
         // all 2 (L + 1) columns loaded at once (in the row for any l: cb + 2 L + 1 < 2 KM), the ones
         // past block l masked out of the sum
+        // (as float2: rows and cb are even, so 8-byte aligned; half the reads, issued together)
+        static_assert(CS % 2 == 0, "C rows of whole float2");
         float va[2 * L1], vb[2 * L1];
+        const float2* ca2 = reinterpret_cast<const float2*>(ca);
+        const float2* cc2 = reinterpret_cast<const float2*>(cc);
 #pragma unroll
-        for (int c = 0; c < 2 * L1; ++c) {
-          va[c] = ca[c];
-          vb[c] = cc[c];
+        for (int c = 0; c < L1; ++c) {
+          const float2 x2 = ca2[c], y2 = cc2[c];
+          va[2 * c] = x2.x;
+          va[2 * c + 1] = x2.y;
+          vb[2 * c] = y2.x;
+          vb[2 * c + 1] = y2.y;
         }
         float acc = va[0] * vb[0] + va[1] * vb[1];
 #pragma unroll
