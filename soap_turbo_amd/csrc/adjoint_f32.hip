@@ -365,6 +365,9 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
   __shared__ double pr_rd[PT];                      // r in FP64 (the radial table cell coordinate)
   __shared__ float cen_s[64];                       // the site's central-atom term per n (poly3gauss)
   __shared__ int site_next;
+  // each species' angular table offset, read per lane from here (a lane-indexed read of the kernel
+  // argument is a vector memory load, which waits behind every earlier load and store)
+  __shared__ int sp_atab[ADJ_MAX_SPECIES];
   const int n_max = P.n_max, n_species = P.n_species;
   int i = 0, p0 = 0, p1 = 0;  // the current site and its pairs
   // the backward pass takes the tiles last first and reuses the forward pass's set-up of the last
@@ -433,10 +436,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
         if (ia > P.ang_n - 2) ia = P.ang_n - 2;
         ta = (float)(xa - (double)ia);
         const int s0 = __ffs(msk) - 1;
-        int aoff = 0;
-        for (int s = 0; s < n_species; ++s)
-          if (s == s0) aoff = P.sp[s].atab_off;
-        acell = aoff + ia * 4 * L1;
+        acell = sp_atab[s0] + ia * 4 * L1;
       }
       pr_msk[pl] = msk != 0 && self ? (msk | SELF) : msk;
       pr_r[pl] = (float)rd;
@@ -555,6 +555,7 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
     }
   };
 
+  if (threadIdx.x < ADJ_MAX_SPECIES) sp_atab[threadIdx.x] = threadIdx.x < n_species ? P.sp[threadIdx.x].atab_off : 0;
   for (int t = threadIdx.x; t < KM; t += NTHREADS) {
     preflm[t] = preflm_g[t];
     int l = 0;
