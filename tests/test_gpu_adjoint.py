@@ -136,6 +136,8 @@ def test_adjoint_f32_central_differences(ctx):
     ([3, 2], 12, None, 4.5),
     ([2, 2, 2, 2, 2], 5, None, 4.5),
     ([4], 4, None, 6.5),                # about 100 neighbours per site: two tiles
+    ([7, 7, 7, 7], 6, "scaled", 4.5),   # n_max 28: two row tiles with every K step (the full form)
+    ([11, 11, 11, 11], 4, None, 4.5),   # n_max 44: three row tiles, the full form
 ])
 def test_adjoint_f32_shapes_match_fp64_oracle(ctx, alpha_max, l_max, kind, rcut):
     from tests.golden_cases import compression_indices
