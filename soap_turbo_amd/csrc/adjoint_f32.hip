@@ -60,7 +60,7 @@ struct AdjParams {
   // radial set-up items: each species block split into nch chunks of at most qc functions;
   // keep: one item per thread and qc <= the kept-register count (the backward pass reuses them)
   int nch, qc, keep;
-  double rcut_max, ang_inv_h;
+  double rcut_max;
   float ang_ih;
   int ang_n;
   SpeciesTab sp[ADJ_MAX_SPECIES];
@@ -431,10 +431,12 @@ __global__ __launch_bounds__(NTHREADS, SiteShape<L>::waves(NTT)) void adjoint_si
       int acell = 0;
       float ta = 0.f;
       if (msk != 0) {
-        const double xa = rd * P.ang_inv_h;
+        // FP32 cell coordinate: g_l(r) is smooth, so a pair that rounds into the next cell takes the
+        // same value there (unlike the radial table's segments, whose jumps need the FP64 distance)
+        const float xa = (float)rd * P.ang_ih;
         int ia = (int)xa;
         if (ia > P.ang_n - 2) ia = P.ang_n - 2;
-        ta = (float)(xa - (double)ia);
+        ta = xa - (float)ia;
         const int s0 = __ffs(msk) - 1;
         acell = sp_atab[s0] + ia * 4 * L1;
       }
@@ -1537,7 +1539,6 @@ static int build_tables(AdjointState& s, hipStream_t st, const AdjointInputs& in
   int off = 0;
   for (int q = 0; q < ns; ++q) { P.sp[q].off = off; off += p->alpha_max[q]; }
   P.ang_n = NTAB + 1;
-  P.ang_inv_h = NTAB / rcut_max;
   P.ang_ih = (float)(NTAB / rcut_max);
   for (int q = 0; q < ns; ++q) P.sp[q].atab_off = q * (P.ang_n - 1) * 4 * (L + 1);
   if (key == s.table_key) {
