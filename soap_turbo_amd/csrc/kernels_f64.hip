@@ -918,6 +918,29 @@ __device__ __forceinline__ int wave_multi_index(int lane) {
   return idx;
 }
 
+// One output's prefactor (4 pi, times sqrt 2 off the n = n' diagonal) and its three sums for pair b of a
+// batch, without it: d/dr, d/dtheta, d/dphi parts from the P/Q rows at (n', l), (n, l) and the radial rows
+// at n, n' (src/soap_turbo.f90:2026-2045).
+__device__ __forceinline__ double der_pre(const int cm) {
+  const int n = cm & 1023, np = (cm >> 10) & 1023;
+  return (4.0 * 3.141592653589793) * (n == np ? 1.0 : 1.4142135623730951);
+}
+
+template <int PB>
+__device__ __forceinline__ void der_sums(const int cm, int b, const double* __restrict__ PQ,
+                                         const double* __restrict__ R, int L1, double& tr, double& ta, double& to) {
+#pragma clang fp contract(fast)
+  constexpr int PQS = 4 * PB + 1, RS = 2 * PB + 1;
+  const int n = cm & 1023, np = (cm >> 10) & 1023, l = cm >> 20;
+  const double* A = PQ + (np * L1 + l) * PQS + 4 * b;
+  const double* C = PQ + (n * L1 + l) * PQS + 4 * b;
+  const double* Rn = R + n * RS + 2 * b;
+  const double* Rp = R + np * RS + 2 * b;
+  tr = Rn[1] * A[0] + Rn[0] * A[1] + Rp[1] * C[0] + Rp[0] * C[1];
+  ta = Rn[0] * A[2] + Rp[0] * C[2];
+  to = Rn[0] * A[3] + Rp[0] * C[3];
+}
+
 template <int PB>
 __device__ __forceinline__ void der_component(const int cm, int b, const double* __restrict__ PQ,
                                               const double* __restrict__ R, int L1, double& tr, double& ta,
@@ -958,7 +981,7 @@ __global__ void rotation_k(int n_pairs, const double* __restrict__ rjs, const do
 // The batch inputs (angular rows, radial values, rotation) are double
 // buffered in LDS: the loads of batch k + 1 are issued to registers before the
 // P/Q products of batch k and written to the other buffer during its pass 1,
-// so that their latency is hidden behind a batch of work. The FA/FR registers
+// so that their latency is hidden behind a batch of work. The FK/FN registers
 // per thread cover k_max <= 48 (l_max <= 8) and n_max <= 32; larger shapes
 // load the remainder when the buffer is written.
 template <int NSPT, int NT, bool DIRECT>
@@ -975,14 +998,12 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
 #pragma clang fp contract(fast)
   using Sh = DerShape<NSPT>;
   constexpr int PB = Sh::PB, PQS = Sh::PQS, RS = Sh::RS, AS = Sh::AS, NRED = Sh::NRED;
-  constexpr int FA = (PB * 4 * 48 + NT - 1) / NT, FR = (PB * 2 * 32 + NT - 1) / NT;
   extern __shared__ double2 smem2[];
   constexpr int NW = NT / 64;
   __shared__ double red[NW][NRED];
   __shared__ double rot[2][PB][8];
   const int i = blockIdx.x;
   const int L1 = l_max + 1, nl = n_max * L1;
-  const int nA = PB * 4 * k_max, nR = PB * 2 * n_max;
   double2* cn = smem2;                                   // n_max * k_max
   double2* sAbuf = cn + n_max * k_max;                   // 2 x k_max * AS
   double* sRbuf = (double*)(sAbuf + 2 * k_max * AS);     // 2 x n_max * RS
@@ -990,80 +1011,89 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
   const int p0 = site_ptr[i], p1 = site_ptr[i + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 
-  // batch inputs: global -> registers (issue), registers -> LDS buffer (complete)
-  double2 fa[FA];
-  double fr[FR], fo = 0.0;
-  // each register slot's element of a batch: pair b, array (kind or d) and row entry (k or n), packed
-  // as b << 24 | array << 16 | entry (-1: none). Made opaque where it is used, so that the array
-  // pointers are formed there: hoisted out of the batch loop they were kept in scratch memory, and a
-  // scratch reload waits for every earlier vector memory operation, the batch's stores included
-  int aslot[FA], rslot[FR];
-#pragma unroll
-  for (int it = 0; it < FA; ++it) {
-    const int t = threadIdx.x + it * NT;
-    const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
-    aslot[it] = t < nA ? (b << 24) | (kind << 16) | k : -1;
-  }
-#pragma unroll
-  for (int it = 0; it < FR; ++it) {
-    const int t = threadIdx.x + it * NT;
-    const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
-    rslot[it] = t < nR ? (b << 24) | (d << 16) | n : -1;
-  }
+  // batch inputs: global -> registers (issue), registers -> LDS buffer (complete). A batch's rows
+  // of each input array are one contiguous range (pairs pb .. pb + nb - 1), so thread t loads
+  // element t (+ NT, ...) of each array from a workgroup-uniform base: the address is the base plus
+  // the thread's fixed offset. Its LDS position (row entry k or n, pair b) is fixed as well.
+  constexpr int FK = (PB * 48 + NT - 1) / NT, FN = (PB * 32 + NT - 1) / NT;
+  double2 fa[4][FK];
+  double fr[2][FN], fo = 0.0;
+  const gptr<double2> angs[4] = {(gptr<double2>)ang, (gptr<double2>)ang_r, (gptr<double2>)ang_a,
+                                 (gptr<double2>)ang_p};
+  const gptr<double> rads[2] = {(gptr<double>)rad, (gptr<double>)rad_der};
   auto fetch = [&](int pb) {
-    const int nb = min(PB, p1 - pb);
+    const int na = min(PB, p1 - pb) * k_max, nr = min(PB, p1 - pb) * n_max;
+    // the thread's offset is made opaque here, so that base + offset is formed in the loop (as a
+    // uniform base and a 32-bit offset) and not kept across it per lane
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
 #pragma unroll
-    for (int it = 0; it < FA; ++it) {
-      int sl = aslot[it];
-      asm volatile("" : "+v"(sl));
-      fa[it] = make_double2(0.0, 0.0);
-      if (sl >= 0) {
-        const int b = sl >> 24, kind = (sl >> 16) & 255, k = sl & 0xffff;
-        if (b < nb) fa[it] = ld(pick4(kind, ang, ang_r, ang_a, ang_p), (size_t)(pb + b) * k_max + k);
+    for (int kind = 0; kind < 4; ++kind) {
+      const gptr<double2> src = angs[kind] + (size_t)pb * k_max;
+#pragma unroll
+      for (int it = 0; it < FK; ++it) {
+        const int t = tid + it * NT;
+        fa[kind][it] = t < na ? ld(src, t) : make_double2(0.0, 0.0);
       }
     }
 #pragma unroll
-    for (int it = 0; it < FR; ++it) {
-      int sl = rslot[it];
-      asm volatile("" : "+v"(sl));
-      fr[it] = 0.0;
-      if (sl >= 0) {
-        const int b = sl >> 24, d = (sl >> 16) & 255, n = sl & 0xffff;
-        if (b < nb) fr[it] = ld(pick2(d, rad, rad_der), (size_t)(pb + b) * n_max + n);
+    for (int d = 0; d < 2; ++d) {
+      const gptr<double> src = rads[d] + (size_t)pb * n_max;
+#pragma unroll
+      for (int it = 0; it < FN; ++it) {
+        const int t = tid + it * NT;
+        fr[d][it] = t < nr ? ld(src, t) : 0.0;
       }
     }
     fo = 0.0;
-    if (threadIdx.x < PB * 8 && (int)threadIdx.x / 8 < nb) fo = rotp[(size_t)pb * 8 + threadIdx.x];
+    if (tid < PB * 8 && tid < 8 * min(PB, p1 - pb)) fo = ld((gptr<double>)rotp + (size_t)pb * 8, tid);
   };
   auto commit = [&](int pb, int buf) {
-    const int nb = min(PB, p1 - pb);
     double2* sA = sAbuf + buf * k_max * AS;
     double* sR = sRbuf + buf * n_max * RS;
 #pragma unroll
-    for (int it = 0; it < FA; ++it) {
-      const int sl = aslot[it];
-      if (sl >= 0) sA[(sl & 0xffff) * AS + 4 * (sl >> 24) + ((sl >> 16) & 255)] = fa[it];
+    for (int it = 0; it < FK; ++it) {
+      const int t = threadIdx.x + it * NT;
+      if (t < PB * k_max) {
+        const int b = t / k_max, k = t - b * k_max;
+#pragma unroll
+        for (int kind = 0; kind < 4; ++kind) sA[k * AS + 4 * b + kind] = fa[kind][it];
+      }
     }
-    for (int t = threadIdx.x + FA * NT; t < nA; t += NT) {
-      const int b = t / (4 * k_max), rem = t - b * 4 * k_max, kind = rem / k_max, k = rem - kind * k_max;
-      const auto src = pick4(kind, ang, ang_r, ang_a, ang_p);
-      sA[k * AS + 4 * b + kind] = b < nb ? ld(src, (size_t)(pb + b) * k_max + k) : make_double2(0.0, 0.0);
+    const int na = min(PB, p1 - pb) * k_max, nr = min(PB, p1 - pb) * n_max;
+    for (int t = threadIdx.x + FK * NT; t < PB * k_max; t += NT) {
+      const int b = t / k_max, k = t - b * k_max;
+#pragma unroll
+      for (int kind = 0; kind < 4; ++kind)
+        sA[k * AS + 4 * b + kind] = t < na ? ld(angs[kind] + (size_t)pb * k_max, t) : make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int it = 0; it < FR; ++it) {
-      const int sl = rslot[it];
-      if (sl >= 0) sR[(sl & 0xffff) * RS + 2 * (sl >> 24) + ((sl >> 16) & 255)] = fr[it];
+    for (int it = 0; it < FN; ++it) {
+      const int t = threadIdx.x + it * NT;
+      if (t < PB * n_max) {
+        const int b = t / n_max, n = t - b * n_max;
+        sR[n * RS + 2 * b] = fr[0][it];
+        sR[n * RS + 2 * b + 1] = fr[1][it];
+      }
     }
-    for (int t = threadIdx.x + FR * NT; t < nR; t += NT) {
-      const int b = t / (2 * n_max), rem = t - b * 2 * n_max, d = rem / n_max, n = rem - d * n_max;
-      sR[n * RS + 2 * b + d] = b < nb ? ld(pick2(d, rad, rad_der), (size_t)(pb + b) * n_max + n) : 0.0;
+    for (int t = threadIdx.x + FN * NT; t < PB * n_max; t += NT) {
+      const int b = t / n_max, n = t - b * n_max;
+#pragma unroll
+      for (int d = 0; d < 2; ++d) sR[n * RS + 2 * b + d] = t < nr ? ld(rads[d] + (size_t)pb * n_max, t) : 0.0;
     }
     if (threadIdx.x < PB * 8) rot[buf][threadIdx.x / 8][threadIdx.x % 8] = fo;
   };
 
-  for (int t = threadIdx.x; t < n_max * k_max; t += NT) cn[t] = cnk[(size_t)i * n_max * k_max + t];
+  // the site's c_nk with the m-weight of the P/Q sums (1 for m = 0, else 2: exact) applied once
+  for (int t = threadIdx.x; t < n_max * k_max; t += NT) {
+    const int k = t % k_max;
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= k) ++l;
+    const double2 c = cnk[(size_t)i * n_max * k_max + t];
+    cn[t] = k == l * (l + 1) / 2 ? c : make_double2(2.0 * c.x, 2.0 * c.y);
+  }
   const double sdp = sqrt_dot_p[i];
-  const double inv_p = 1.0 / sdp, inv_p3 = 1.0 / (sdp * sdp * sdp);
+  const double inv_p = 1.0 / sdp, inv_p2 = 1.0 / (sdp * sdp);
   const bool scaled = out_ptr != nullptr;
   double selfacc[NSPT][3];
   double qv[NSPT];
@@ -1077,10 +1107,10 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     qv[u] = live ? soap[(size_t)i * n_soap + s] : 0.0;
     if (DIRECT) {
       cmu[u] = live ? pack_component(ocomp[s]) : 0;
-      elu[u] = live ? oel[s] : 0.0;
+      elu[u] = live ? der_pre(cmu[u]) * (scaled ? oel[s] : 1.0) * inv_p : 0.0;
     } else {
       cmu[u] = (live && !scaled) ? pack_component(comp[s]) : 0;
-      elu[u] = 0.0;
+      elu[u] = live && !scaled ? der_pre(cmu[u]) * inv_p : 0.0;
     }
   }
   if (p0 + 1 < p1) {
@@ -1113,26 +1143,21 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
         // column and m clamped into range and the terms they stand in for zeroed, so the job waits for
         // LDS once. (Loads under per-step conditions were each waited for on their own.)
         const int nc = n < n_max ? n : n_max - 1, rc = r16 < 4 * PB ? r16 : 0;
-        const double* cnd = reinterpret_cast<const double*>(cn);
-        const double* sAd = reinterpret_cast<const double*>(sA);
+        // K entry kappa = 4 st + g is part (kappa & 1) of m = kappa / 2: the operands of step st sit a
+        // fixed distance past step 0's (two entries of cn's row, two rows of sA)
+        const double* ca = reinterpret_cast<const double*>(cn + nc * k_max + kb + (g >> 1)) + (g & 1);
+        const double* cb = reinterpret_cast<const double*>(sA + (kb + (g >> 1)) * AS + rc) + (g & 1);
         f64x4 acc = {0.0, 0.0, 0.0, 0.0};
         auto product = [&](auto ns_) {
           constexpr int NS = decltype(ns_)::value;
           double av[NS], bv[NS];
 #pragma unroll
           for (int st = 0; st < NS; ++st) {
-            const int kappa = 4 * st + g, m = min(kappa >> 1, l);
-            av[st] = cnd[2 * (nc * k_max + kb + m) + (kappa & 1)];
-            bv[st] = sAd[2 * ((kb + m) * AS + rc) + (kappa & 1)];
+            av[st] = ca[4 * st];
+            bv[st] = cb[4 * AS * st];
           }
-#pragma unroll
-          for (int st = 0; st < NS; ++st) {
-            const int kappa = 4 * st + g, m = kappa >> 1;
-            const bool ok = kappa < nK;
-            const double wm = m == 0 ? 1.0 : 2.0;
-            av[st] = ok && n < n_max ? wm * av[st] : 0.0;
-            bv[st] = ok && r16 < 4 * PB ? bv[st] : 0.0;
-          }
+          // only the last step can reach past the 2 (l + 1) entries (m up to l + 1: inside the LDS)
+          if (4 * (NS - 1) + g >= nK) av[NS - 1] = bv[NS - 1] = 0.0;
 #pragma unroll
           for (int st = 0; st < NS; ++st) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st], bv[st], acc, 0, 0, 0);
         };
@@ -1167,22 +1192,21 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
 #pragma unroll
       for (int b = 0; b < PB; ++b) {
         double vr = 0.0, va = 0.0, vo = 0.0;
-        if (s < n_soap) {
-          if (DIRECT || !scaled) {
-            double tr, ta, to;
-            der_component<PB>(cmu[u], b, PQ, sR, L1, tr, ta, to);
-            if (DIRECT && scaled) {
-              vr = 0.0 + elu[u] * tr; va = 0.0 + elu[u] * ta; vo = 0.0 + elu[u] * to;
-            } else {
-              vr = tr; va = ta; vo = to;
-            }
-          } else {
+        if (DIRECT || !scaled) {
+          // on every lane, with no branch, so that the LDS reads of all outputs and pairs can be issued
+          // together: an output past n_soap has component (0, 0, 0) and factor 0
+          double tr, ta, to;
+          der_sums<PB>(cmu[u], b, PQ, sR, L1, tr, ta, to);
+          vr = elu[u] * tr; va = elu[u] * ta; vo = elu[u] * to;
+        } else if (s < n_soap) {
+          {
             for (int e = out_ptr[s]; e < out_ptr[s + 1]; ++e) {
               double tr, ta, to;
               der_component<PB>(pack_component(comp[out_c[e]]), b, PQ, sR, L1, tr, ta, to);
               const double el = out_el[e];
               vr = vr + el * tr; va = va + el * ta; vo = vo + el * to;
             }
+            vr *= inv_p; va *= inv_p; vo *= inv_p;
           }
         }
         v[b][u][0] = vr; v[b][u][1] = va; v[b][u][2] = vo;
@@ -1217,10 +1241,10 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
         for (int u = 0; u < NSPT; ++u) {
           const int s = threadIdx.x + u * NT;
           if (s < n_soap) {
-            const double sv = qv[u];
-            const double vr = v[b][u][0] * inv_p - sv * inv_p3 * dot_r;
-            const double va = v[b][u][1] * inv_p - sv * inv_p3 * dot_a;
-            const double vo = v[b][u][2] * inv_p - sv * inv_p3 * dot_o;
+            const double sv = qv[u] * inv_p2;
+            const double vr = v[b][u][0] - sv * dot_r;
+            const double va = v[b][u][1] - sv * dot_a;
+            const double vo = v[b][u][2] - sv * dot_o;
             const double c0 = rr[0] * vr + rr[3] * vo + rr[6] * va;
             const double c1 = rr[1] * vr + rr[4] * vo + rr[7] * va;
             const double c2 = rr[2] * vr + rr[5] * vo;
@@ -1714,8 +1738,10 @@ static int batched_variant(int, int variant) { return variant < 0 ? 0 : variant;
 size_t derivatives_lds(int n_max, int l_max, int k_max, int n_soap, int variant) {
   if (is_wave_variant(n_soap, variant)) return jacobian_wave_lds(n_max, l_max, k_max, n_soap);
   const int pb = der_pb(n_soap, batched_variant(n_soap, variant));
+  // (one more angular row: the P/Q product's last step may read one row past a buffer, into this pad)
   return sizeof(double2) * (n_max * k_max + 2 * k_max * (4 * pb + 1)) +
-         sizeof(double) * (2 * n_max * (2 * pb + 1) + n_max * (l_max + 1) * (4 * pb + 1));
+         sizeof(double) * (2 * n_max * (2 * pb + 1) + n_max * (l_max + 1) * (4 * pb + 1)) +
+         sizeof(double2) * (4 * pb + 1);
 }
 
 bool launch_derivatives(hipStream_t st, int n_sites, int n_pairs, int n_max, int l_max, int k_max, int n_soap,
