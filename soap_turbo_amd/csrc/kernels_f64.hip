@@ -886,14 +886,48 @@ __device__ __forceinline__ int pack_component(const int4 c) { return c.x | (c.y 
 // V/2 + V/4 + ... shuffles instead of 6 V. Lane j ends with the total of value
 // idx(j) = bit5(j) V/2 + bit4(j) V/4 + ...; lanes with the low bits below that
 // hold the same totals.
+//
+// The exchanges across 32 and 16 lanes use gfx950's lane swaps, which move no data through the LDS:
+// permlane32_swap(a, b) exchanges lanes 32-63 of a with lanes 0-31 of b, permlane16_swap the odd
+// 16-lane rows of a with the even rows of b. With a = x[j], b = x[j + H], a lane with the bit clear
+// then holds its own x[j] in a and its partner's in b, a lane with the bit set its partner's x[j + H]
+// in a and its own in b: a + b is the exchange's sum in both (the same sum as own + partner).
+template <int BIT>
+__device__ __forceinline__ void swap_halves(double& a, double& b) {
+  const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  const unsigned alo = (unsigned)ua, ahi = (unsigned)(ua >> 32), blo = (unsigned)ub, bhi = (unsigned)(ub >> 32);
+  unsigned lo0, lo1, hi0, hi1;
+  if constexpr (BIT == 32) {
+    const auto l = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+    lo0 = l[0]; lo1 = l[1]; hi0 = h[0]; hi1 = h[1];
+  } else {
+    static_assert(BIT == 16, "BIT");
+    const auto l = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+    lo0 = l[0]; lo1 = l[1]; hi0 = h[0]; hi1 = h[1];
+  }
+  a = __longlong_as_double((long long)(((unsigned long long)hi0 << 32) | lo0));
+  b = __longlong_as_double((long long)(((unsigned long long)hi1 << 32) | lo1));
+}
+
 template <int H, int BIT, int V>
 __device__ __forceinline__ void halving_stage(double (&x)[V], int lane) {
-  const bool hi = (lane & BIT) != 0;
+  if constexpr (BIT >= 16) {
 #pragma unroll
-  for (int j = 0; j < H; ++j) {
-    const double keep = hi ? x[j + H] : x[j];
-    const double send = hi ? x[j] : x[j + H];
-    x[j] = keep + __shfl_xor(send, BIT, 64);
+    for (int j = 0; j < H; ++j) {
+      double a = x[j], b = x[j + H];
+      swap_halves<BIT>(a, b);
+      x[j] = a + b;
+    }
+  } else {
+    const bool hi = (lane & BIT) != 0;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const double keep = hi ? x[j + H] : x[j];
+      const double send = hi ? x[j] : x[j + H];
+      x[j] = keep + __shfl_xor(send, BIT, 64);
+    }
   }
   if constexpr (H > 1) halving_stage<H / 2, BIT / 2, V>(x, lane);
 }
