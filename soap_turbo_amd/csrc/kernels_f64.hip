@@ -911,6 +911,20 @@ __device__ __forceinline__ void swap_halves(double& a, double& b) {
   b = __longlong_as_double((long long)(((unsigned long long)hi1 << 32) | lo1));
 }
 
+// Within a 16-lane row the partner is reached by a DPP move, which also leaves the LDS alone: the row
+// mirror (lane i <- 15 - i) and half-row mirror (i <- 7 - i) flip bit 3 and bit 2 with the bits below
+// them, the quad permutations flip bit 1 or bit 0 alone. Each pairs every lane with one lane of the
+// other half, so the sums cover every lane once as with exclusive-or partners.
+template <int BIT>
+__device__ __forceinline__ double dpp_partner(double v) {
+  constexpr int CTRL = BIT == 8 ? 0x140 : BIT == 4 ? 0x141 : BIT == 2 ? 0x4e : 0xb1;
+  static_assert(BIT == 8 || BIT == 4 || BIT == 2 || BIT == 1, "BIT");
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 template <int H, int BIT, int V>
 __device__ __forceinline__ void halving_stage(double (&x)[V], int lane) {
   if constexpr (BIT >= 16) {
@@ -926,7 +940,7 @@ __device__ __forceinline__ void halving_stage(double (&x)[V], int lane) {
     for (int j = 0; j < H; ++j) {
       const double keep = hi ? x[j + H] : x[j];
       const double send = hi ? x[j] : x[j + H];
-      x[j] = keep + __shfl_xor(send, BIT, 64);
+      x[j] = keep + dpp_partner<BIT>(send);
     }
   }
   if constexpr (H > 1) halving_stage<H / 2, BIT / 2, V>(x, lane);
@@ -938,8 +952,10 @@ __device__ __forceinline__ double wave_multi_sum(double (&x)[V], int lane) {
   halving_stage<V / 2, 32, V>(x, lane);
   double t = x[0];
   constexpr int LOG = V == 4 ? 2 : V == 8 ? 3 : 4;
-#pragma unroll
-  for (int bit = 32 >> LOG; bit >= 1; bit >>= 1) t += __shfl_xor(t, bit, 64);
+  if constexpr (LOG <= 2) t += dpp_partner<8>(t);
+  if constexpr (LOG <= 3) t += dpp_partner<4>(t);
+  t += dpp_partner<2>(t);
+  t += dpp_partner<1>(t);
   return t;
 }
 
