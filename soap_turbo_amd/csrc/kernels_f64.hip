@@ -1208,6 +1208,9 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
           }
           // only the last step can reach past the 2 (l + 1) entries (m up to l + 1: inside the LDS)
           if (4 * (NS - 1) + g >= nK) av[NS - 1] = bv[NS - 1] = 0.0;
+          // every load is issued before the first product: left to itself the scheduler alternates
+          // load, wait, product, and the job waits for the LDS once per step
+          __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int st = 0; st < NS; ++st) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st], bv[st], acc, 0, 0, 0);
         };
