@@ -1179,40 +1179,48 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
     // block-diagonal product per l on the FP64 matrix cores: rows n (tiles of 16), columns
     // j = 4 pair + kind (16), K = the 2 (l + 1) real terms (re, im of each m), padded to 4.
     {
-      const int ntile = (n_max + 15) >> 4, njob = ntile * L1;
+      const int ntile = (n_max + 15) >> 4, npair = (ntile + 1) >> 1, njob = npair * L1;
       const int r16 = lane & 15, g = lane >> 4;
-      // jobs in order of falling l, dealt to the waves in a snake, so that their K steps even out
+      // A job is one l and two row tiles (rows 32 tp .. 32 tp + 31): the tiles share the sA operand
+      // and their two product chains are independent, so they interleave. Jobs in order of falling
+      // l, dealt to the waves in a snake, so that their K steps even out.
       for (int q = 0; q < njob; ++q) {
         const int round = q / NW, pos = q - round * NW;
         if ((round & 1 ? NW - 1 - pos : pos) != w) continue;
-        const int l = l_max - q / ntile, tile = q - (q / ntile) * ntile;
+        const int l = l_max - q / npair, tp = q - (q / npair) * npair;
         const int kb = l * (l + 1) / 2, nK = 2 * (l + 1), nsteps = (nK + 3) >> 2;
-        const int n = tile * 16 + r16;
-        // The job's product with its step count NS fixed at compile time: every operand load (one
-        // double: the real or imaginary part of the entry) is issued before any is used, with the row,
-        // column and m clamped into range and the terms they stand in for zeroed, so the job waits for
-        // LDS once. (Loads under per-step conditions were each waited for on their own.)
-        const int nc = n < n_max ? n : n_max - 1, rc = r16 < 4 * PB ? r16 : 0;
+        const int n0 = 32 * tp + r16, n1 = n0 + 16;
+        // The job's products with the step count NS fixed at compile time: every operand load (one
+        // double: the real or imaginary part of the entry) is issued before any is used, so the job
+        // waits for the LDS once. Rows past n_max read row n_max - 1 and columns past the batch's
+        // pairs read column 0 (their results are not kept).
+        const int nc0 = n0 < n_max ? n0 : n_max - 1, nc1 = n1 < n_max ? n1 : n_max - 1;
+        const int rc = r16 < 4 * PB ? r16 : 0;
         // K entry kappa = 4 st + g is part (kappa & 1) of m = kappa / 2: the operands of step st sit a
         // fixed distance past step 0's (two entries of cn's row, two rows of sA)
-        const double* ca = reinterpret_cast<const double*>(cn + nc * k_max + kb + (g >> 1)) + (g & 1);
+        const double* ca0 = reinterpret_cast<const double*>(cn + nc0 * k_max + kb + (g >> 1)) + (g & 1);
+        const double* ca1 = reinterpret_cast<const double*>(cn + nc1 * k_max + kb + (g >> 1)) + (g & 1);
         const double* cb = reinterpret_cast<const double*>(sA + (kb + (g >> 1)) * AS + rc) + (g & 1);
-        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+        f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
         auto product = [&](auto ns_) {
           constexpr int NS = decltype(ns_)::value;
-          double av[NS], bv[NS];
+          double av0[NS], av1[NS], bv[NS];
 #pragma unroll
           for (int st = 0; st < NS; ++st) {
-            av[st] = ca[4 * st];
+            av0[st] = ca0[4 * st];
+            av1[st] = ca1[4 * st];
             bv[st] = cb[4 * AS * st];
           }
           // only the last step can reach past the 2 (l + 1) entries (m up to l + 1: inside the LDS)
-          if (4 * (NS - 1) + g >= nK) av[NS - 1] = bv[NS - 1] = 0.0;
+          if (4 * (NS - 1) + g >= nK) bv[NS - 1] = av0[NS - 1] = av1[NS - 1] = 0.0;
           // every load is issued before the first product: left to itself the scheduler alternates
           // load, wait, product, and the job waits for the LDS once per step
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int st = 0; st < NS; ++st) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st], bv[st], acc, 0, 0, 0);
+          for (int st = 0; st < NS; ++st) {
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0[st], bv[st], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1[st], bv[st], acc1, 0, 0, 0);
+          }
         };
         switch (nsteps) {  // l_max <= 12: at most 7 steps
           case 1: product(std::integral_constant<int, 1>{}); break;
@@ -1227,8 +1235,9 @@ __global__ __launch_bounds__(NT, DerShape<NSPT>::waves(NT)) void derivatives_k(
         if (r16 < 4 * PB) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int nn = tile * 16 + g + 4 * v;
-            if (nn < n_max) PQ[(nn * L1 + l) * PQS + r16] = acc[v];
+            const int nn = 32 * tp + g + 4 * v;
+            if (nn < n_max) PQ[(nn * L1 + l) * PQS + r16] = acc0[v];
+            if (nn + 16 < n_max) PQ[((nn + 16) * L1 + l) * PQS + r16] = acc1[v];
           }
         }
       }
